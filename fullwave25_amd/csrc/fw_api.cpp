@@ -1,0 +1,624 @@
+// C-ABI of the engine: handle management, uploads, the step loop and the directory contract.
+// Declarations and the contract are in include/fullwave_hip.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/fullwave_hip.h"
+#include "fw_internal.h"
+
+namespace fw {
+hipError_t launch_velocity_simple(const DevSim& s, hipStream_t st);
+hipError_t launch_pressure_simple(const DevSim& s, hipStream_t st);
+bool streaming_supported(const DevSim& s);
+}  // namespace fw
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const char* fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+#define FW_HIP(expr)                                                                      \
+    do {                                                                                  \
+        hipError_t e_ = (expr);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            return fail(FW_ERR_DEVICE, "%s failed: %s", #expr, hipGetErrorString(e_));    \
+    } while (0)
+
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev) {
+        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+        (void)hipSetDevice(dev);
+    }
+    ~DeviceGuard() {
+        if (prev >= 0) (void)hipSetDevice(prev);
+    }
+};
+
+const char* kAxesV[3] = {"u", "w", "v"};  // velocity-side map suffix per axis
+const char* kAxesP[3] = {"x", "y", "z"};  // pressure-side map suffix per axis
+
+}  // namespace
+
+struct fw_sim {
+    fw_desc d{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int64_t ncell = 0;
+    int ndim = 2;
+    fw::DevSim ds{};
+    fw::DevBoundary db{};
+    float* state = nullptr;  // p, v[ndim], psi[ndim][2], phi[ndim][2]
+    int64_t nstate = 0;
+    std::map<std::string, void*> dev;  // every other device allocation, by input name
+    std::map<std::string, bool> have;
+    std::vector<int32_t> icc, outc, zero;
+    std::vector<float> dmap;
+    bool ready = false;
+    int64_t step = 0;
+    int64_t rec_cap = 0;
+    bool use_streaming = false;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+};
+
+namespace {
+
+int64_t cells(const fw_desc& d) { return d.nx * d.ny * (d.ndim == 3 ? d.nz : 1); }
+
+std::vector<std::string> required_maps(const fw_desc& d) {
+    std::vector<std::string> names = {"K", "rho", "dcmap", "dmap", "icmat", "icc", "outc"};
+    if (d.ncoordszero > 0) names.push_back("icczero");
+    const int nax = d.isotropic ? 1 : d.ndim;
+    for (int a = 0; a < nax; ++a) {
+        names.push_back(std::string("kappa") + kAxesV[a]);
+        names.push_back(std::string("kappa") + kAxesP[a]);
+        for (int nu = 1; nu <= fw::kNRelax; ++nu) {
+            for (const char* ab : {"apml", "bpml"}) {
+                names.push_back(std::string(ab) + kAxesV[a] + std::to_string(nu));
+                names.push_back(std::string(ab) + kAxesP[a] + std::to_string(nu));
+            }
+        }
+    }
+    return names;
+}
+
+bool is_voxel_map(const fw_desc& d, const std::string& name) {
+    if (name == "K" || name == "rho" || name == "c" || name == "beta" || name == "dcmap") return true;
+    const int nax = d.isotropic ? 1 : d.ndim;
+    for (int a = 0; a < nax; ++a) {
+        if (name == std::string("kappa") + kAxesV[a] || name == std::string("kappa") + kAxesP[a]) return true;
+        for (int nu = 1; nu <= fw::kNRelax; ++nu)
+            for (const char* ab : {"apml", "bpml"})
+                if (name == std::string(ab) + kAxesV[a] + std::to_string(nu) ||
+                    name == std::string(ab) + kAxesP[a] + std::to_string(nu))
+                    return true;
+    }
+    return false;
+}
+
+int alloc(fw_sim* s, const std::string& name, size_t bytes, void** out) {
+    auto it = s->dev.find(name);
+    if (it != s->dev.end()) {
+        *out = it->second;
+        return FW_OK;
+    }
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes ? bytes : 4);
+    if (e != hipSuccess) return fail(FW_ERR_NOMEM, "hipMalloc(%zu) for '%s' failed: %s", bytes, name.c_str(), hipGetErrorString(e));
+    s->dev[name] = p;
+    *out = p;
+    return FW_OK;
+}
+
+int check_coords(const fw_sim* s, const std::vector<int32_t>& c, int64_t n, const char* what,
+                 std::vector<int64_t>& lin) {
+    const fw_desc& d = s->d;
+    const int nd = s->ndim;
+    lin.resize((size_t)n);
+    const int64_t dims[3] = {d.nx, d.ny, nd == 3 ? d.nz : 1};
+    for (int64_t q = 0; q < n; ++q) {
+        int64_t idx = 0;
+        for (int a = 0; a < nd; ++a) {
+            const int64_t x = c[(size_t)(q * nd + a)];
+            if (x < 0 || x >= dims[a])
+                return fail(FW_ERR_INVALID, "%s coordinate %lld (axis %d) = %lld outside [0, %lld)", what,
+                            (long long)q, a, (long long)x, (long long)dims[a]);
+            idx = idx * dims[a] + x;
+        }
+        lin[(size_t)q] = idx;
+    }
+    return FW_OK;
+}
+
+// Resolve coordinate lists into device index arrays and build the coefficient table.
+int finalize(fw_sim* s) {
+    const fw_desc& d = s->d;
+    for (const auto& name : required_maps(d))
+        if (!s->have.count(name)) return fail(FW_ERR_STATE, "input '%s' has not been uploaded", name.c_str());
+
+    std::vector<int64_t> src, zero, out;
+    int rc;
+    if ((rc = check_coords(s, s->icc, d.ncoords, "source", src)) != FW_OK) return rc;
+    if ((rc = check_coords(s, s->outc, d.ncoordsout, "sensor", out)) != FW_OK) return rc;
+    if ((rc = check_coords(s, s->zero, d.ncoordszero, "pressure-release", zero)) != FW_OK) return rc;
+
+    // pressure release wins over injection; a repeated source point keeps its last signal
+    std::unordered_map<int64_t, int32_t> air, last_src;
+    for (int64_t q = 0; q < d.ncoordszero; ++q) air[zero[(size_t)q]] = 1;
+    for (int64_t q = 0; q < d.ncoords; ++q) last_src[src[(size_t)q]] = (int32_t)q;
+    std::vector<int64_t> src_lin((size_t)d.ncoords);
+    for (int64_t q = 0; q < d.ncoords; ++q) {
+        const int64_t l = src[(size_t)q];
+        src_lin[(size_t)q] = (air.count(l) || last_src[l] != q) ? -1 : l;
+    }
+    std::vector<int32_t> out_src((size_t)d.ncoordsout);
+    std::vector<int8_t> out_air((size_t)d.ncoordsout);
+    for (int64_t q = 0; q < d.ncoordsout; ++q) {
+        const int64_t l = out[(size_t)q];
+        out_air[(size_t)q] = air.count(l) ? 1 : 0;
+        auto it = last_src.find(l);
+        out_src[(size_t)q] = it == last_src.end() ? -1 : it->second;
+    }
+
+    // deduplicate zero list (same address written twice with the same value is harmless,
+    // but keep one writer per address)
+    std::sort(zero.begin(), zero.end());
+    zero.erase(std::unique(zero.begin(), zero.end()), zero.end());
+
+    void* p;
+    auto up = [&](const char* name, const void* h, size_t bytes) -> int {
+        int r = alloc(s, name, bytes, &p);
+        if (r != FW_OK) return r;
+        if (bytes) FW_HIP(hipMemcpy(p, h, bytes, hipMemcpyHostToDevice));
+        return FW_OK;
+    };
+    if ((rc = up("_src_lin", src_lin.data(), src_lin.size() * 8)) != FW_OK) return rc;
+    s->db.src_lin = (const int64_t*)s->dev["_src_lin"];
+    if ((rc = up("_zero_lin", zero.data(), zero.size() * 8)) != FW_OK) return rc;
+    s->db.zero_lin = (const int64_t*)s->dev["_zero_lin"];
+    if ((rc = up("_out_lin", out.data(), out.size() * 8)) != FW_OK) return rc;
+    s->db.out_lin = (const int64_t*)s->dev["_out_lin"];
+    if ((rc = up("_out_src", out_src.data(), out_src.size() * 4)) != FW_OK) return rc;
+    s->db.out_src = (const int32_t*)s->dev["_out_src"];
+    if ((rc = up("_out_air", out_air.data(), out_air.size())) != FW_OK) return rc;
+    s->db.out_air = (const int8_t*)s->dev["_out_air"];
+    s->db.ncoords = d.ncoords;
+    s->db.nzero = (int64_t)zero.size();
+    s->db.nout = d.ncoordsout;
+    s->db.ntic = d.ntic;
+
+    // coefficient table [ndmap][ndim][9] from dmap [9][2][ndmap]; spacing folded in
+    const int nd = s->ndim;
+    const double h[3] = {d.dx, d.dy, d.dz};
+    std::vector<float> coef((size_t)d.ndmap * nd * fw::kNCoef);
+    for (int64_t m = 0; m < d.ndmap; ++m) {
+        for (int a = 0; a < nd; ++a) {
+            float* row = &coef[(size_t)((m * nd + a) * fw::kNCoef)];
+            for (int k = 1; k <= 8; ++k)
+                row[k - 1] = (float)((double)s->dmap[(size_t)((k * 2 + 0) * d.ndmap + m)] / h[a]);
+            row[8] = (float)((double)s->dmap[(size_t)((1 * 2 + 1) * d.ndmap + m)] / h[a]);
+        }
+    }
+    if ((rc = up("_coef", coef.data(), coef.size() * 4)) != FW_OK) return rc;
+    s->ds.coef = (const float*)s->dev["_coef"];
+
+    // dcmap range check on the host would need a download; the kernels trust it, so check it
+    // here once against ndmap
+    {
+        std::vector<int32_t> dc((size_t)s->ncell);
+        FW_HIP(hipMemcpy(dc.data(), s->dev["dcmap"], dc.size() * 4, hipMemcpyDeviceToHost));
+        for (int32_t v : dc)
+            if (v < 0 || v >= d.ndmap) return fail(FW_ERR_INVALID, "dcmap value %d outside [0, %lld)", v, (long long)d.ndmap);
+    }
+
+    s->rec_cap = d.modt > 0 ? (d.nt + d.modt - 1) / d.modt : 0;
+    if ((rc = alloc(s, "_rec", (size_t)(s->rec_cap * d.ncoordsout) * 4, &p)) != FW_OK) return rc;
+    s->db.rec = (float*)p;
+    FW_HIP(hipMemset(p, 0, (size_t)(s->rec_cap * d.ncoordsout) * 4));
+
+    // map pointers
+    const int nax = d.isotropic ? 1 : nd;
+    for (int a = 0; a < nd; ++a) {
+        const int src_axis = nax == 1 ? 0 : a;
+        s->ds.ikv[a] = (const float*)s->dev[std::string("kappa") + kAxesV[src_axis]];
+        s->ds.ikp[a] = (const float*)s->dev[std::string("kappa") + kAxesP[src_axis]];
+        for (int nu = 0; nu < fw::kNRelax; ++nu) {
+            const std::string n = std::to_string(nu + 1);
+            s->ds.av[a][nu] = (const float*)s->dev[std::string("apml") + kAxesV[src_axis] + n];
+            s->ds.bv[a][nu] = (const float*)s->dev[std::string("bpml") + kAxesV[src_axis] + n];
+            s->ds.ap[a][nu] = (const float*)s->dev[std::string("apml") + kAxesP[src_axis] + n];
+            s->ds.bp[a][nu] = (const float*)s->dev[std::string("bpml") + kAxesP[src_axis] + n];
+        }
+    }
+    s->ds.K = (const float*)s->dev["K"];
+    s->ds.rho = (const float*)s->dev["rho"];
+    s->ds.dcmap = (const int32_t*)s->dev["dcmap"];
+    s->ds.ndmap = (int32_t)d.ndmap;
+    s->ds.isotropic = d.isotropic;
+    s->use_streaming = fw::streaming_supported(s->ds);
+    s->ready = true;
+    return FW_OK;
+}
+
+int launch_step(fw_sim* s, bool timed, double* ms) {
+    hipError_t e;
+    auto timed_launch = [&](int kind, auto&& fn) -> int {
+        if (timed) FW_HIP(hipEventRecord(s->ev[0], s->stream));
+        e = fn();
+        if (e != hipSuccess) return fail(FW_ERR_DEVICE, "kernel launch failed: %s", hipGetErrorString(e));
+        if (timed) {
+            FW_HIP(hipEventRecord(s->ev[1], s->stream));
+            FW_HIP(hipEventSynchronize(s->ev[1]));
+            float t = 0.f;
+            FW_HIP(hipEventElapsedTime(&t, s->ev[0], s->ev[1]));
+            ms[kind] += t;
+        }
+        return FW_OK;
+    };
+    int rc;
+    rc = timed_launch(fw::kVelocity, [&] {
+        return s->use_streaming ? fw::launch_velocity(s->ds, s->stream) : fw::launch_velocity_simple(s->ds, s->stream);
+    });
+    if (rc != FW_OK) return rc;
+    rc = timed_launch(fw::kPressure, [&] {
+        return s->use_streaming ? fw::launch_pressure(s->ds, s->stream) : fw::launch_pressure_simple(s->ds, s->stream);
+    });
+    if (rc != FW_OK) return rc;
+    rc = timed_launch(fw::kBoundary, [&] { return fw::launch_boundary(s->ds, s->db, s->step, s->d.modt, s->stream); });
+    if (rc != FW_OK) return rc;
+    s->step += 1;
+    return FW_OK;
+}
+
+int run_impl(fw_sim* s, int64_t n_steps, bool timed, double* ms) {
+    if (!s) return fail(FW_ERR_INVALID, "null handle");
+    if (n_steps < 0) return fail(FW_ERR_INVALID, "n_steps < 0");
+    DeviceGuard g(s->device);
+    if (!s->ready) {
+        int rc = finalize(s);
+        if (rc != FW_OK) return rc;
+    }
+    if (s->step + n_steps > s->d.nt)
+        return fail(FW_ERR_INVALID, "running %lld steps from step %lld exceeds nt = %lld", (long long)n_steps,
+                    (long long)s->step, (long long)s->d.nt);
+    for (int64_t it = 0; it < n_steps; ++it) {
+        int rc = launch_step(s, timed, ms);
+        if (rc != FW_OK) return rc;
+    }
+    FW_HIP(hipStreamSynchronize(s->stream));
+    return FW_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fw_abi_version(void) { return FW_ABI_VERSION; }
+
+const char* fw_version_string(void) { return "fullwave25_amd engine 0.1 (gfx950, HIP)"; }
+
+const char* fw_last_error(void) { return g_last_error.c_str(); }
+
+int fw_device_count(int* count) {
+    if (!count) return fail(FW_ERR_INVALID, "null count");
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) {
+        *count = 0;
+        return fail(FW_ERR_DEVICE, "hipGetDeviceCount failed: %s", hipGetErrorString(e));
+    }
+    *count = n;
+    return FW_OK;
+}
+
+int fw_sim_create(const fw_desc* desc, int device, fw_sim** out) {
+    if (!desc || !out) return fail(FW_ERR_INVALID, "null argument");
+    *out = nullptr;
+    const fw_desc& d = *desc;
+    if (d.abi_version != FW_ABI_VERSION) return fail(FW_ERR_INVALID, "abi_version %d, library has %d", d.abi_version, FW_ABI_VERSION);
+    if (d.ndim != 2 && d.ndim != 3) return fail(FW_ERR_INVALID, "ndim must be 2 or 3, got %d", d.ndim);
+    if (d.n_relax != 2) return fail(FW_ERR_UNSUPPORTED, "only n_relax = 2 is supported, got %d", d.n_relax);
+    const int64_t nz = d.ndim == 3 ? d.nz : 1;
+    if (d.nx < 2 * fw::kHalo + 1 || d.ny < 2 * fw::kHalo + 1 || (d.ndim == 3 && nz < 2 * fw::kHalo + 1))
+        return fail(FW_ERR_INVALID, "every axis needs at least %d cells (8 ghost cells per side)", 2 * fw::kHalo + 1);
+    if (d.nt < 0 || d.ncoords < 0 || d.ncoordsout < 0 || d.ncoordszero < 0 || d.ntic < 0)
+        return fail(FW_ERR_INVALID, "negative count in descriptor");
+    if (d.modt < 1) return fail(FW_ERR_INVALID, "modt must be >= 1");
+    if (d.ndmap < 1) return fail(FW_ERR_INVALID, "ndmap must be >= 1");
+    if (!(d.dt > 0) || !(d.dx > 0) || !(d.dy > 0) || (d.ndim == 3 && !(d.dz > 0)))
+        return fail(FW_ERR_INVALID, "dt and grid spacings must be positive");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev)
+        return fail(FW_ERR_DEVICE, "device %d not available (%d devices)", device, ndev);
+
+    fw_sim* s = new fw_sim();
+    s->d = d;
+    s->d.nz = nz;
+    s->device = device;
+    s->ndim = d.ndim;
+    s->ncell = cells(s->d);
+    DeviceGuard g(device);
+    auto cleanup = [&](int rc) {
+        fw_sim_destroy(s);
+        return rc;
+    };
+    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
+        return cleanup(fail(FW_ERR_DEVICE, "hipStreamCreate failed"));
+    if (hipEventCreate(&s->ev[0]) != hipSuccess || hipEventCreate(&s->ev[1]) != hipSuccess)
+        return cleanup(fail(FW_ERR_DEVICE, "hipEventCreate failed"));
+    const int nd = d.ndim;
+    s->nstate = (int64_t)(1 + nd + 4 * nd) * s->ncell;
+    if (hipMalloc(&s->state, (size_t)s->nstate * 4) != hipSuccess)
+        return cleanup(fail(FW_ERR_NOMEM, "hipMalloc of %lld state floats failed", (long long)s->nstate));
+    if (hipMemset(s->state, 0, (size_t)s->nstate * 4) != hipSuccess)
+        return cleanup(fail(FW_ERR_DEVICE, "hipMemset failed"));
+    float* f = s->state;
+    s->ds.p = f;
+    f += s->ncell;
+    for (int a = 0; a < nd; ++a, f += s->ncell) s->ds.v[a] = f;
+    for (int a = 0; a < nd; ++a)
+        for (int nu = 0; nu < 2; ++nu, f += s->ncell) s->ds.psi[a][nu] = f;
+    for (int a = 0; a < nd; ++a)
+        for (int nu = 0; nu < 2; ++nu, f += s->ncell) s->ds.phi[a][nu] = f;
+    s->ds.ndim = nd;
+    s->ds.nx = d.nx;
+    s->ds.ny = d.ny;
+    s->ds.nz = nz;
+    s->ds.sy = nz;
+    s->ds.sx = d.ny * nz;
+    s->ds.dt = (float)d.dt;
+    *out = s;
+    return FW_OK;
+}
+
+int fw_sim_upload(fw_sim* s, const char* name_c, const void* host, size_t bytes) {
+    if (!s || !name_c || (!host && bytes)) return fail(FW_ERR_INVALID, "null argument");
+    const std::string name(name_c);
+    const fw_desc& d = s->d;
+    DeviceGuard g(s->device);
+    if (s->ready && (name == "icc" || name == "outc" || name == "icczero" || name == "dmap"))
+        return fail(FW_ERR_STATE, "'%s' cannot change after the first run; create a new handle", name_c);
+    void* p = nullptr;
+    int rc;
+    if (name == "c" || name == "beta") {
+        // c enters only through dcmap; beta (nonlinearity) is not modelled by this engine
+        if (bytes != (size_t)s->ncell * 4) return fail(FW_ERR_INVALID, "'%s' must be %lld bytes", name_c, (long long)s->ncell * 4);
+        s->have[name] = true;
+        return FW_OK;
+    }
+    if (is_voxel_map(d, name)) {
+        if (bytes != (size_t)s->ncell * 4)
+            return fail(FW_ERR_INVALID, "'%s' must be %lld bytes, got %zu", name_c, (long long)s->ncell * 4, bytes);
+        if ((rc = alloc(s, name, bytes, &p)) != FW_OK) return rc;
+        FW_HIP(hipMemcpy(p, host, bytes, hipMemcpyHostToDevice));
+        if (name.rfind("kappa", 0) == 0) {
+            FW_HIP(fw::launch_reciprocal((float*)p, s->ncell, s->stream));
+            FW_HIP(hipStreamSynchronize(s->stream));
+        }
+        s->have[name] = true;
+        return FW_OK;
+    }
+    if (name == "dmap") {
+        if (bytes != (size_t)(9 * 2 * d.ndmap) * 4) return fail(FW_ERR_INVALID, "'dmap' must be 9*2*ndmap floats");
+        s->dmap.assign((const float*)host, (const float*)host + 9 * 2 * d.ndmap);
+        s->have[name] = true;
+        return FW_OK;
+    }
+    if (name == "icc" || name == "outc" || name == "icczero") {
+        const int64_t n = name == "icc" ? d.ncoords : name == "outc" ? d.ncoordsout : d.ncoordszero;
+        if (bytes != (size_t)(n * s->ndim) * 4)
+            return fail(FW_ERR_INVALID, "'%s' must be [%lld, %d] int32", name_c, (long long)n, s->ndim);
+        auto& dst = name == "icc" ? s->icc : name == "outc" ? s->outc : s->zero;
+        dst.assign((const int32_t*)host, (const int32_t*)host + n * s->ndim);
+        s->have[name] = true;
+        return FW_OK;
+    }
+    if (name == "icmat") {
+        if (bytes != (size_t)(d.ncoords * d.ntic) * 4)
+            return fail(FW_ERR_INVALID, "'icmat' must be [ncoords, ntic] float32");
+        // device layout [ntic][ncoords]: one coalesced row per step
+        std::vector<float> t((size_t)(d.ncoords * d.ntic));
+        const float* src = (const float*)host;
+        for (int64_t q = 0; q < d.ncoords; ++q)
+            for (int64_t n = 0; n < d.ntic; ++n) t[(size_t)(n * d.ncoords + q)] = src[q * d.ntic + n];
+        if ((rc = alloc(s, name, bytes, &p)) != FW_OK) return rc;
+        if (bytes) FW_HIP(hipMemcpy(p, t.data(), bytes, hipMemcpyHostToDevice));
+        s->db.icmat_t = (const float*)p;
+        s->have[name] = true;
+        return FW_OK;
+    }
+    return fail(FW_ERR_INVALID, "unknown input '%s' for a %dD %s simulation", name_c, s->ndim,
+                d.isotropic ? "isotropic" : "anisotropic");
+}
+
+int fw_sim_run(fw_sim* s, int64_t n_steps) { return run_impl(s, n_steps, false, nullptr); }
+
+int fw_sim_run_timed(fw_sim* s, int64_t n_steps, double* ms, int32_t* n_kinds) {
+    if (!ms) return fail(FW_ERR_INVALID, "null ms");
+    if (n_kinds) *n_kinds = fw::kNumKinds;
+    return run_impl(s, n_steps, true, ms);
+}
+
+int fw_sim_reset(fw_sim* s) {
+    if (!s) return fail(FW_ERR_INVALID, "null handle");
+    DeviceGuard g(s->device);
+    FW_HIP(hipMemsetAsync(s->state, 0, (size_t)s->nstate * 4, s->stream));
+    if (s->db.rec && s->rec_cap)
+        FW_HIP(hipMemsetAsync(s->db.rec, 0, (size_t)(s->rec_cap * s->d.ncoordsout) * 4, s->stream));
+    FW_HIP(hipStreamSynchronize(s->stream));
+    s->step = 0;
+    return FW_OK;
+}
+
+int fw_sim_step_count(const fw_sim* s, int64_t* steps) {
+    if (!s || !steps) return fail(FW_ERR_INVALID, "null argument");
+    *steps = s->step;
+    return FW_OK;
+}
+
+int fw_sim_record_count(const fw_sim* s, int64_t* n) {
+    if (!s || !n) return fail(FW_ERR_INVALID, "null argument");
+    *n = s->d.modt > 0 ? (s->step + s->d.modt - 1) / s->d.modt : 0;
+    return FW_OK;
+}
+
+int fw_sim_download_sensors(fw_sim* s, float* out, size_t bytes) {
+    if (!s || (!out && bytes)) return fail(FW_ERR_INVALID, "null argument");
+    int64_t nrec = 0;
+    fw_sim_record_count(s, &nrec);
+    const size_t want = (size_t)(nrec * s->d.ncoordsout) * 4;
+    if (bytes != want) return fail(FW_ERR_INVALID, "sensor buffer must be %zu bytes (%lld records), got %zu", want, (long long)nrec, bytes);
+    if (!want) return FW_OK;
+    DeviceGuard g(s->device);
+    FW_HIP(hipMemcpy(out, s->db.rec, want, hipMemcpyDeviceToHost));
+    return FW_OK;
+}
+
+int fw_sim_download_field(fw_sim* s, const char* name, float* out, size_t bytes) {
+    if (!s || !name || !out) return fail(FW_ERR_INVALID, "null argument");
+    const float* src = nullptr;
+    if (!strcmp(name, "p")) src = s->ds.p;
+    else if (!strcmp(name, "vx")) src = s->ds.v[0];
+    else if (!strcmp(name, "vy")) src = s->ds.v[1];
+    else if (!strcmp(name, "vz") && s->ndim == 3) src = s->ds.v[2];
+    if (!src) return fail(FW_ERR_INVALID, "unknown field '%s'", name);
+    if (bytes != (size_t)s->ncell * 4) return fail(FW_ERR_INVALID, "field buffer must be %lld bytes", (long long)s->ncell * 4);
+    DeviceGuard g(s->device);
+    FW_HIP(hipMemcpy(out, src, bytes, hipMemcpyDeviceToHost));
+    return FW_OK;
+}
+
+int fw_sim_set_field(fw_sim* s, const char* name, const float* values, size_t bytes) {
+    if (!s || !name || !values) return fail(FW_ERR_INVALID, "null argument");
+    float* dst = nullptr;
+    if (!strcmp(name, "p")) dst = s->ds.p;
+    else if (!strcmp(name, "vx")) dst = s->ds.v[0];
+    else if (!strcmp(name, "vy")) dst = s->ds.v[1];
+    else if (!strcmp(name, "vz") && s->ndim == 3) dst = s->ds.v[2];
+    if (!dst) return fail(FW_ERR_INVALID, "unknown field '%s'", name);
+    if (bytes != (size_t)s->ncell * 4) return fail(FW_ERR_INVALID, "field buffer must be %lld bytes", (long long)s->ncell * 4);
+    DeviceGuard g(s->device);
+    FW_HIP(hipMemcpy(dst, values, bytes, hipMemcpyHostToDevice));
+    return FW_OK;
+}
+
+int fw_sim_destroy(fw_sim* s) {
+    if (!s) return FW_OK;
+    DeviceGuard g(s->device);
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    for (auto& kv : s->dev) (void)hipFree(kv.second);
+    if (s->state) (void)hipFree(s->state);
+    for (auto& e : s->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    delete s;
+    return FW_OK;
+}
+
+// ---- directory contract -----------------------------------------------------------------
+
+static bool read_file(const std::string& path, std::vector<char>& buf) {
+    std::ifstream f(path, std::ios::binary | std::ios::ate);
+    if (!f) return false;
+    const std::streamsize n = f.tellg();
+    f.seekg(0);
+    buf.resize((size_t)n);
+    return (bool)f.read(buf.data(), n) || n == 0;
+}
+
+static bool file_exists(const std::string& path) {
+    std::ifstream f(path);
+    return (bool)f;
+}
+
+int fw_run_dir(const char* sim_dir, int device) {
+    if (!sim_dir) return fail(FW_ERR_INVALID, "null directory");
+    const std::string dir = std::string(sim_dir) + "/";
+    std::vector<char> buf;
+    auto scalar_i = [&](const char* stem, int64_t* v) -> int {
+        if (!read_file(dir + stem + ".dat", buf) || buf.size() < 4) return fail(FW_ERR_IO, "cannot read %s%s.dat", dir.c_str(), stem);
+        int32_t x;
+        memcpy(&x, buf.data(), 4);
+        *v = x;
+        return FW_OK;
+    };
+    auto scalar_f = [&](const char* stem, double* v) -> int {
+        if (!read_file(dir + stem + ".dat", buf) || buf.size() < 4) return fail(FW_ERR_IO, "cannot read %s%s.dat", dir.c_str(), stem);
+        float x;
+        memcpy(&x, buf.data(), 4);
+        *v = x;
+        return FW_OK;
+    };
+    fw_desc d{};
+    d.abi_version = FW_ABI_VERSION;
+    d.n_relax = 2;
+    d.ndim = file_exists(dir + "nZ.dat") ? 3 : 2;
+    d.isotropic = file_exists(dir + "kappaw.dat") ? 0 : 1;
+    int rc;
+    if ((rc = scalar_i("nX", &d.nx)) || (rc = scalar_i("nY", &d.ny)) || (rc = scalar_i("nT", &d.nt)) ||
+        (rc = scalar_i("ncoords", &d.ncoords)) || (rc = scalar_i("ncoordsout", &d.ncoordsout)) ||
+        (rc = scalar_i("nTic", &d.ntic)) || (rc = scalar_i("modT", &d.modt)) || (rc = scalar_i("ndmap", &d.ndmap)) ||
+        (rc = scalar_f("dX", &d.dx)) || (rc = scalar_f("dY", &d.dy)) || (rc = scalar_f("dT", &d.dt)) ||
+        (rc = scalar_f("c0", &d.c0)))
+        return rc;
+    if (file_exists(dir + "ncoordszero.dat")) {
+        if ((rc = scalar_i("ncoordszero", &d.ncoordszero))) return rc;
+    }
+    d.nz = 1;
+    if (d.ndim == 3 && ((rc = scalar_i("nZ", &d.nz)) || (rc = scalar_f("dZ", &d.dz)))) return rc;
+    fw_sim* s = nullptr;
+    if ((rc = fw_sim_create(&d, device, &s)) != FW_OK) return rc;
+    std::vector<std::string> names = required_maps(d);
+    for (const auto& name : names) {
+        if (name == "icczero" && d.ncoordszero == 0) continue;
+        if (!read_file(dir + name + ".dat", buf)) {
+            fw_sim_destroy(s);
+            return fail(FW_ERR_IO, "cannot read %s%s.dat", dir.c_str(), name.c_str());
+        }
+        if ((rc = fw_sim_upload(s, name.c_str(), buf.data(), buf.size())) != FW_OK) {
+            std::string msg = std::string("file ") + name + ".dat: " + g_last_error;
+            fw_sim_destroy(s);
+            return fail(rc, "%s", msg.c_str());
+        }
+    }
+    if ((rc = fw_sim_run(s, d.nt)) != FW_OK) {
+        std::string msg = g_last_error;
+        fw_sim_destroy(s);
+        return fail(rc, "%s", msg.c_str());
+    }
+    int64_t nrec = 0;
+    fw_sim_record_count(s, &nrec);
+    std::vector<float> rec((size_t)(nrec * d.ncoordsout));
+    if ((rc = fw_sim_download_sensors(s, rec.data(), rec.size() * 4)) != FW_OK) {
+        std::string msg = g_last_error;
+        fw_sim_destroy(s);
+        return fail(rc, "%s", msg.c_str());
+    }
+    fw_sim_destroy(s);
+    std::ofstream out(dir + "genout.dat", std::ios::binary | std::ios::trunc);
+    if (!out) return fail(FW_ERR_IO, "cannot write %sgenout.dat", dir.c_str());
+    out.write((const char*)rec.data(), (std::streamsize)(rec.size() * 4));
+    if (!out) return fail(FW_ERR_IO, "short write to %sgenout.dat", dir.c_str());
+    return FW_OK;
+}
+
+}  // extern "C"

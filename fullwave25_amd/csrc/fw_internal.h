@@ -1,0 +1,62 @@
+// Internal interface between the C-ABI host code (fw_api.cpp) and the kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fw {
+
+constexpr int kHalo = 8;      // axial stencil half-width == ghost border width
+constexpr int kNCoef = 9;     // w_1..w_8, e
+constexpr int kMaxDim = 3;
+constexpr int kNRelax = 2;
+
+// Device view of one simulation. Fields and maps are float32 over the full grid
+// [nx][ny][nz] (nz == 1 in 2D), C order. Cells with every coordinate in [8, n - 8) are
+// updated; the ghost border is never written and stays zero.
+struct DevSim {
+    int ndim;
+    int64_t nx, ny, nz;
+    int64_t sx, sy;  // strides of x and y (z stride is 1)
+    float dt;
+
+    float* p;
+    float* v[kMaxDim];
+    float* psi[kMaxDim][kNRelax];  // velocity-side memory variables
+    float* phi[kMaxDim][kNRelax];  // pressure-side memory variables
+
+    const float* K;
+    const float* rho;
+    const float* ikv[kMaxDim];  // 1 / kappa, velocity side, per axis
+    const float* ikp[kMaxDim];  // 1 / kappa, pressure side, per axis
+    const float* av[kMaxDim][kNRelax];
+    const float* bv[kMaxDim][kNRelax];
+    const float* ap[kMaxDim][kNRelax];
+    const float* bp[kMaxDim][kNRelax];
+    const int32_t* dcmap;
+    const float* coef;  // [ndmap][ndim][9], w and e already divided by the axis spacing
+    int32_t ndmap;
+    int32_t isotropic;
+};
+
+// Source injection, pressure release and sensor gather of one step.
+struct DevBoundary {
+    int64_t ncoords, nzero, nout, ntic;
+    const int64_t* src_lin;    // -1 = dropped (duplicate or air point)
+    const float* icmat_t;      // [ntic][ncoords]
+    const int64_t* zero_lin;
+    const int64_t* out_lin;
+    const int32_t* out_src;    // source index at the sensor's voxel, or -1
+    const int8_t* out_air;     // 1 if the sensor's voxel is a pressure-release point
+    float* rec;                // [n_records][nout]
+};
+
+enum KernelKind { kVelocity = 0, kPressure = 1, kBoundary = 2, kNumKinds = 3 };
+
+hipError_t launch_velocity(const DevSim& s, hipStream_t st);
+hipError_t launch_pressure(const DevSim& s, hipStream_t st);
+hipError_t launch_boundary(const DevSim& s, const DevBoundary& b, int64_t step, int64_t modt,
+                           hipStream_t st);
+hipError_t launch_reciprocal(float* data, int64_t n, hipStream_t st);
+
+}  // namespace fw

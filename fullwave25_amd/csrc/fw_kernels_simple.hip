@@ -1,0 +1,162 @@
+// Per-cell kernels: one thread per grid cell, stencil operands straight from global memory.
+// They are the plain statement of the update (same arithmetic order as the streaming
+// kernels in fw_kernels_2d.hip / fw_kernels_3d.hip) and are used for grids the streaming
+// kernels do not take, and as the second GPU implementation the tests compare against.
+#include "fw_internal.h"
+
+namespace fw {
+
+namespace {
+
+template <int ND>
+__device__ inline bool cell_of_thread(const DevSim& s, int64_t& i, int64_t& j, int64_t& k) {
+    if (ND == 2) {
+        k = 0;
+        j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        i = (int64_t)blockIdx.y * blockDim.y + threadIdx.y;
+        return i >= kHalo && i < s.nx - kHalo && j >= kHalo && j < s.ny - kHalo;
+    }
+    k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    j = (int64_t)blockIdx.y * blockDim.y + threadIdx.y;
+    i = blockIdx.z;
+    return i >= kHalo && i < s.nx - kHalo && j >= kHalo && j < s.ny - kHalo && k >= kHalo &&
+           k < s.nz - kHalo;
+}
+
+template <int ND>
+__global__ __launch_bounds__(256) void k_velocity_simple(DevSim s) {
+    int64_t i, j, k;
+    if (!cell_of_thread<ND>(s, i, j, k)) return;
+    const int64_t c = i * s.sx + j * s.sy + k;
+    const float* cf = s.coef + (int64_t)s.dcmap[c] * ND * kNCoef;
+    const int64_t stride[3] = {s.sx, s.sy, 1};
+    const float* p = s.p;
+    const float rho_c = s.rho[c];
+#pragma unroll
+    for (int a = 0; a < ND; ++a) {
+        const float* w = cf + a * kNCoef;
+        const int64_t sa = stride[a];
+        float acc = 0.f;
+#pragma unroll
+        for (int m = 1; m <= 8; ++m) acc += w[m - 1] * (p[c + m * sa] - p[c - (m - 1) * sa]);
+        float tr = 0.f;
+#pragma unroll
+        for (int b = 0; b < ND; ++b) {
+            if (b == a) continue;
+            const int64_t sb = stride[b];
+            tr += (p[c + sa + sb] - p[c + sb]) + (p[c + sa - sb] - p[c - sb]);
+        }
+        const float D = acc + w[8] * tr;
+        const float q1 = s.bv[a][0][c] * s.psi[a][0][c] + s.av[a][0][c] * D;
+        const float q2 = s.bv[a][1][c] * s.psi[a][1][c] + s.av[a][1][c] * D;
+        s.psi[a][0][c] = q1;
+        s.psi[a][1][c] = q2;
+        const float rho_bar = 0.5f * (rho_c + s.rho[c + sa]);
+        s.v[a][c] -= s.dt / rho_bar * (D * s.ikv[a][c] + (q1 + q2));
+    }
+}
+
+template <int ND>
+__global__ __launch_bounds__(256) void k_pressure_simple(DevSim s) {
+    int64_t i, j, k;
+    if (!cell_of_thread<ND>(s, i, j, k)) return;
+    const int64_t c = i * s.sx + j * s.sy + k;
+    const float* cf = s.coef + (int64_t)s.dcmap[c] * ND * kNCoef;
+    const int64_t stride[3] = {s.sx, s.sy, 1};
+    float div = 0.f;
+#pragma unroll
+    for (int a = 0; a < ND; ++a) {
+        const float* w = cf + a * kNCoef;
+        const float* va = s.v[a];
+        const int64_t sa = stride[a];
+        float acc = 0.f;
+#pragma unroll
+        for (int m = 1; m <= 8; ++m) acc += w[m - 1] * (va[c + (m - 1) * sa] - va[c - m * sa]);
+        float tr = 0.f;
+#pragma unroll
+        for (int b = 0; b < ND; ++b) {
+            if (b == a) continue;
+            const int64_t sb = stride[b];
+            tr += (va[c + sb] - va[c - sa + sb]) + (va[c - sb] - va[c - sa - sb]);
+        }
+        const float D = acc + w[8] * tr;
+        const float q1 = s.bp[a][0][c] * s.phi[a][0][c] + s.ap[a][0][c] * D;
+        const float q2 = s.bp[a][1][c] * s.phi[a][1][c] + s.ap[a][1][c] * D;
+        s.phi[a][0][c] = q1;
+        s.phi[a][1][c] = q2;
+        div += D * s.ikp[a][c] + (q1 + q2);
+    }
+    s.p[c] -= s.dt * s.K[c] * div;
+}
+
+// Source injection, then pressure release, then sensor gather, for step `n`. Sources that
+// sit on a pressure-release point or repeat a later source are dropped on the host, and
+// sensors on such points read the value written there, so no two threads touch one address.
+__global__ __launch_bounds__(256) void k_boundary(DevSim s, DevBoundary b, int64_t n, int64_t rec_row) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool inject = n < b.ntic;
+    if (inject && t < b.ncoords) {
+        const int64_t lin = b.src_lin[t];
+        if (lin >= 0) s.p[lin] = b.icmat_t[n * b.ncoords + t];
+    }
+    if (t < b.nzero) s.p[b.zero_lin[t]] = 0.f;
+    if (rec_row >= 0 && t < b.nout) {
+        float val;
+        if (b.out_air[t]) {
+            val = 0.f;
+        } else {
+            const int32_t src = b.out_src[t];
+            val = (inject && src >= 0) ? b.icmat_t[n * b.ncoords + src] : s.p[b.out_lin[t]];
+        }
+        b.rec[rec_row * b.nout + t] = val;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_reciprocal(float* data, int64_t n) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) data[t] = 1.0f / data[t];
+}
+
+}  // namespace
+
+static dim3 simple_grid(const DevSim& s) {
+    if (s.ndim == 2) return dim3((unsigned)((s.ny + 63) / 64), (unsigned)((s.nx + 3) / 4), 1);
+    return dim3((unsigned)((s.nz + 63) / 64), (unsigned)((s.ny + 3) / 4), (unsigned)s.nx);
+}
+
+hipError_t launch_velocity_simple(const DevSim& s, hipStream_t st) {
+    if (s.ndim == 2)
+        hipLaunchKernelGGL(k_velocity_simple<2>, simple_grid(s), dim3(64, 4), 0, st, s);
+    else
+        hipLaunchKernelGGL(k_velocity_simple<3>, simple_grid(s), dim3(64, 4), 0, st, s);
+    return hipGetLastError();
+}
+
+hipError_t launch_pressure_simple(const DevSim& s, hipStream_t st) {
+    if (s.ndim == 2)
+        hipLaunchKernelGGL(k_pressure_simple<2>, simple_grid(s), dim3(64, 4), 0, st, s);
+    else
+        hipLaunchKernelGGL(k_pressure_simple<3>, simple_grid(s), dim3(64, 4), 0, st, s);
+    return hipGetLastError();
+}
+
+hipError_t launch_boundary(const DevSim& s, const DevBoundary& b, int64_t step, int64_t modt,
+                           hipStream_t st) {
+    int64_t n = b.ncoords;
+    if (b.nzero > n) n = b.nzero;
+    if (b.nout > n) n = b.nout;
+    if (n == 0) return hipSuccess;
+    const int64_t rec_row = (modt > 0 && step % modt == 0) ? step / modt : -1;
+    if (rec_row < 0 && step >= b.ntic && b.nzero == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_boundary, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, s, b, step,
+                       rec_row);
+    return hipGetLastError();
+}
+
+hipError_t launch_reciprocal(float* data, int64_t n, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_reciprocal, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, data, n);
+    return hipGetLastError();
+}
+
+}  // namespace fw

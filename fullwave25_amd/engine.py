@@ -1,0 +1,209 @@
+"""ctypes binding of ``libfullwave_hip.so`` (declared in include/fullwave_hip.h).
+
+This module loads the in-tree HIP library and fails loudly when it is missing: there is no
+CPU path. :class:`HipSimulation` owns one engine handle; :func:`run_dir` is the directory
+contract used by :class:`~fullwave25_amd.solver.launcher.HipLauncher`.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import threading
+from pathlib import Path
+
+import numpy as np
+
+from .solver.inputs import SimulationInputs
+
+LIB_NAME = "libfullwave_hip.so"
+LIB_PATH = Path(__file__).resolve().parent / LIB_NAME
+
+FW_OK = 0
+STATUS_NAMES = {
+    0: "FW_OK", 1: "FW_ERR_INVALID", 2: "FW_ERR_DEVICE", 3: "FW_ERR_NOMEM",
+    4: "FW_ERR_IO", 5: "FW_ERR_STATE", 6: "FW_ERR_UNSUPPORTED",
+}
+KERNEL_KINDS = ("velocity", "pressure", "boundary")
+
+# every symbol include/fullwave_hip.h declares
+EXPORTED_SYMBOLS = (
+    "fw_abi_version", "fw_version_string", "fw_device_count", "fw_last_error", "fw_run_dir",
+    "fw_sim_create", "fw_sim_upload", "fw_sim_run", "fw_sim_reset", "fw_sim_step_count",
+    "fw_sim_record_count", "fw_sim_download_sensors", "fw_sim_download_field",
+    "fw_sim_set_field", "fw_sim_run_timed", "fw_sim_destroy",
+)
+
+
+class EngineError(RuntimeError):
+    """A non-zero status from the engine, with its message."""
+
+    def __init__(self, status: int, message: str) -> None:
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {message}")
+        self.status = status
+
+
+class FwDesc(ctypes.Structure):
+    _fields_ = [
+        ("abi_version", ctypes.c_int32), ("ndim", ctypes.c_int32), ("isotropic", ctypes.c_int32),
+        ("n_relax", ctypes.c_int32), ("nx", ctypes.c_int64), ("ny", ctypes.c_int64),
+        ("nz", ctypes.c_int64), ("nt", ctypes.c_int64), ("dx", ctypes.c_double),
+        ("dy", ctypes.c_double), ("dz", ctypes.c_double), ("dt", ctypes.c_double),
+        ("c0", ctypes.c_double), ("ncoords", ctypes.c_int64), ("ntic", ctypes.c_int64),
+        ("ncoordsout", ctypes.c_int64), ("modt", ctypes.c_int64), ("ncoordszero", ctypes.c_int64),
+        ("ndmap", ctypes.c_int64),
+    ]
+
+
+_lib = None
+_lib_lock = threading.Lock()
+
+
+def load_library(path: Path | None = None) -> ctypes.CDLL:
+    """Load (once) and return the engine library; raise if it has not been built."""
+    global _lib
+    with _lib_lock:
+        if _lib is not None and path is None:
+            return _lib
+        p = Path(path) if path is not None else LIB_PATH
+        if not p.exists():
+            raise RuntimeError(
+                f"{p} is missing: build the HIP engine first (python -c 'import __graft_entry__ as g; g.build()')",
+            )
+        lib = ctypes.CDLL(str(p))
+        vp, i32, i64, sz = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_size_t
+        lib.fw_abi_version.restype = ctypes.c_int
+        lib.fw_version_string.restype = ctypes.c_char_p
+        lib.fw_last_error.restype = ctypes.c_char_p
+        lib.fw_device_count.argtypes = [ctypes.POINTER(ctypes.c_int)]
+        lib.fw_run_dir.argtypes = [ctypes.c_char_p, ctypes.c_int]
+        lib.fw_sim_create.argtypes = [ctypes.POINTER(FwDesc), ctypes.c_int, ctypes.POINTER(vp)]
+        lib.fw_sim_upload.argtypes = [vp, ctypes.c_char_p, vp, sz]
+        lib.fw_sim_run.argtypes = [vp, i64]
+        lib.fw_sim_reset.argtypes = [vp]
+        lib.fw_sim_step_count.argtypes = [vp, ctypes.POINTER(i64)]
+        lib.fw_sim_record_count.argtypes = [vp, ctypes.POINTER(i64)]
+        lib.fw_sim_download_sensors.argtypes = [vp, vp, sz]
+        lib.fw_sim_download_field.argtypes = [vp, ctypes.c_char_p, vp, sz]
+        lib.fw_sim_set_field.argtypes = [vp, ctypes.c_char_p, vp, sz]
+        lib.fw_sim_run_timed.argtypes = [vp, i64, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(i32)]
+        lib.fw_sim_destroy.argtypes = [vp]
+        if lib.fw_abi_version() != 1:
+            raise RuntimeError(f"{p}: ABI version {lib.fw_abi_version()}, expected 1")
+        if path is None:
+            _lib = lib
+        return lib
+
+
+def _check(lib, status: int) -> None:
+    if status != FW_OK:
+        raise EngineError(status, lib.fw_last_error().decode(errors="replace"))
+
+
+def device_count() -> int:
+    lib = load_library()
+    n = ctypes.c_int(0)
+    _check(lib, lib.fw_device_count(ctypes.byref(n)))
+    return n.value
+
+
+def run_dir(sim_dir: Path, device: int = 0) -> None:
+    """Run the engine on a simulation directory; writes ``genout.dat`` there."""
+    lib = load_library()
+    _check(lib, lib.fw_run_dir(str(Path(sim_dir).absolute()).encode(), int(device)))
+
+
+def make_desc(inp: SimulationInputs) -> FwDesc:
+    return FwDesc(
+        abi_version=1, ndim=inp.ndim, isotropic=1 if inp.isotropic else 0, n_relax=inp.n_relax,
+        nx=inp.nx, ny=inp.ny, nz=inp.nz if inp.ndim == 3 else 1, nt=inp.nt,
+        dx=inp.dx, dy=inp.dy, dz=inp.dz, dt=inp.dt, c0=inp.c0,
+        ncoords=inp.ncoords, ntic=inp.ntic, ncoordsout=inp.ncoordsout, modt=inp.modt,
+        ncoordszero=inp.ncoordszero, ndmap=inp.ndmap,
+    )
+
+
+class HipSimulation:
+    """One engine handle with its inputs resident in HBM."""
+
+    def __init__(self, inp: SimulationInputs, device: int = 0) -> None:
+        self.lib = load_library()
+        self.inp = inp
+        self._h = ctypes.c_void_p()
+        desc = make_desc(inp)
+        _check(self.lib, self.lib.fw_sim_create(ctypes.byref(desc), int(device), ctypes.byref(self._h)))
+        try:
+            self._upload_all()
+        except Exception:
+            self.close()
+            raise
+
+    def upload(self, name: str, arr: np.ndarray) -> None:
+        a = np.ascontiguousarray(arr)
+        _check(self.lib, self.lib.fw_sim_upload(self._h, name.encode(), a.ctypes.data_as(ctypes.c_void_p), a.nbytes))
+
+    def _upload_all(self) -> None:
+        inp = self.inp
+        for name in ["K", "rho", *inp.relaxation_map_stems()]:
+            self.upload(name, inp.maps[name].astype(np.float32, copy=False))
+        self.upload("dcmap", inp.dcmap.astype(np.int32, copy=False))
+        self.upload("dmap", inp.dmap.astype(np.float32, copy=False))
+        self.upload("icc", inp.icc.astype(np.int32, copy=False))
+        self.upload("icmat", inp.icmat.astype(np.float32, copy=False))
+        self.upload("outc", inp.outc.astype(np.int32, copy=False))
+        if inp.ncoordszero:
+            self.upload("icczero", inp.icczero.astype(np.int32, copy=False))
+
+    def run(self, n_steps: int) -> None:
+        _check(self.lib, self.lib.fw_sim_run(self._h, int(n_steps)))
+
+    def run_timed(self, n_steps: int) -> dict:
+        """Run with HIP events around every launch; returns summed ms per kernel kind."""
+        ms = (ctypes.c_double * 8)()
+        nk = ctypes.c_int32(0)
+        _check(self.lib, self.lib.fw_sim_run_timed(self._h, int(n_steps), ms, ctypes.byref(nk)))
+        return {KERNEL_KINDS[i]: ms[i] for i in range(nk.value)}
+
+    def reset(self) -> None:
+        _check(self.lib, self.lib.fw_sim_reset(self._h))
+
+    @property
+    def step(self) -> int:
+        v = ctypes.c_int64(0)
+        _check(self.lib, self.lib.fw_sim_step_count(self._h, ctypes.byref(v)))
+        return v.value
+
+    def sensors(self) -> np.ndarray:
+        """Recorded traces ``[n_records, n_sensors]`` (genout.dat layout)."""
+        n = ctypes.c_int64(0)
+        _check(self.lib, self.lib.fw_sim_record_count(self._h, ctypes.byref(n)))
+        out = np.empty((n.value, self.inp.ncoordsout), dtype=np.float32)
+        _check(self.lib, self.lib.fw_sim_download_sensors(self._h, out.ctypes.data_as(ctypes.c_void_p), out.nbytes))
+        return out
+
+    def field(self, name: str) -> np.ndarray:
+        out = np.empty(self.inp.shape, dtype=np.float32)
+        _check(self.lib, self.lib.fw_sim_download_field(self._h, name.encode(), out.ctypes.data_as(ctypes.c_void_p), out.nbytes))
+        return out
+
+    def set_field(self, name: str, values: np.ndarray) -> None:
+        a = np.ascontiguousarray(values, dtype=np.float32)
+        if a.shape != tuple(self.inp.shape):
+            raise ValueError(f"field shape {a.shape} != grid {self.inp.shape}")
+        _check(self.lib, self.lib.fw_sim_set_field(self._h, name.encode(), a.ctypes.data_as(ctypes.c_void_p), a.nbytes))
+
+    def close(self) -> None:
+        if self._h:
+            self.lib.fw_sim_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __enter__(self) -> HipSimulation:
+        return self
+
+    def __exit__(self, *exc) -> None:
+        self.close()
+
+    def __del__(self) -> None:
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,232 @@
+"""Synthetic workloads built through the public front end (Grid, Medium, Source, Sensor,
+Solver), at the sizes of the benchmark configurations and at small sizes for checks.
+
+Every builder returns a :class:`~fullwave25_amd.solver.inputs.SimulationInputs` (plus, where a
+transmit is involved, what is needed to post-process the traces). Random media use
+``numpy.random.default_rng(seed)``.
+"""
+
+from __future__ import annotations
+
+import tempfile
+from dataclasses import dataclass
+from pathlib import Path
+
+import numpy as np
+
+from .grid import Grid
+from .medium import Medium
+from .sensor import Sensor
+from .solver.solver import Solver
+from .source import Source
+from .utils.pulse import gaussian_modulated_sinusoidal_signal
+
+
+@dataclass
+class Workload:
+    name: str
+    inputs: object  # SimulationInputs
+    element_of_sensor: np.ndarray | None = None  # per-sensor element index (-1 = none)
+    n_elements: int = 0
+
+    @property
+    def cells(self) -> int:
+        return self.inputs.ncells
+
+
+def _grid(nx: int, ny: int, *, f0=3e6, c0=1540.0, ppw=12, cfl=0.4, nt=2000, nz: int | None = None) -> Grid:
+    dx = c0 / f0 / ppw
+    dt = cfl * dx / c0
+    size = (nx * dx, ny * dx) if nz is None else (nx * dx, ny * dx, nz * dx)
+    return Grid(domain_size=size, f0=f0, duration=nt * dt, c0=c0, ppw=ppw, cfl=cfl)
+
+
+def _inputs(grid, medium, source, sensor, *, n_pml: int, n_transition: int | None = None,
+            isotropic: bool = True):
+    with tempfile.TemporaryDirectory() as tmp:
+        solver = Solver(Path(tmp), grid, medium, source, sensor, pml_layer_thickness_px=n_pml,
+                        n_transition_layer=n_transition, use_isotropic_relaxation=isotropic)
+        inputs, _ = solver.build_inputs()
+    return inputs
+
+
+def plane_wave_2d(nx: int = 512, ny: int = 512, nt: int = 2000, *, n_pml: int = 20,
+                  n_transition: int | None = None, sensor_depth: int | None = 400,
+                  alpha0: float = 0.5, gamma: float = 1.1, ncycles: int = 3,
+                  isotropic: bool = True, f0: float = 3e6) -> Workload:
+    """Homogeneous 2D medium (c = 1540 m/s, rho = 1000 kg/m^3, alpha0, gamma), a 3-cycle
+    Gaussian tone burst on the top row as a line source, a line of receivers at
+    ``sensor_depth`` (the 512 x 512 / 2000-step plane-wave configuration at default sizes)."""
+    grid = _grid(nx, ny, nt=nt, f0=f0)
+    shape = (grid.nx, grid.ny)
+    medium = Medium(grid, np.full(shape, 1540.0), np.full(shape, 1000.0), np.full(shape, alpha0),
+                    np.full(shape, gamma), np.zeros(shape), use_isotropic_relaxation=isotropic)
+    mask = np.zeros(shape, dtype=bool)
+    mask[0, :] = True
+    sig = gaussian_modulated_sinusoidal_signal(grid.nt, grid.duration, ncycles, 2, f0, 1e5)
+    source = Source(np.tile(sig, (int(mask.sum()), 1)), mask)
+    smask = np.zeros(shape, dtype=bool)
+    smask[min(sensor_depth if sensor_depth is not None else nx // 2, nx - 1), :] = True
+    sensor = Sensor(smask)
+    return Workload("plane_wave_2d", _inputs(grid, medium, source, sensor, n_pml=n_pml,
+                                             n_transition=n_transition, isotropic=isotropic))
+
+
+def layered_medium_2d(shape, rng, *, n_layers=6, amp_cells=10.0, dx=None, scatter_per_cell=15,
+                      wavelength_cells=12):
+    """Seeded layered medium: ``n_layers`` horizontal layers whose interfaces carry a random
+    sinusoidal perturbation of ``amp_cells``; per-layer c ~ U[1450, 1620], rho ~ U[900, 1100],
+    alpha0 ~ U[0.3, 1.5], gamma ~ U[0.8, 1.6]; sub-resolution scatterers at
+    ``scatter_per_cell`` per resolution cell (wavelength^2) with a +-2 % Gaussian density
+    fluctuation."""
+    nx, ny = shape
+    c = np.empty(shape)
+    rho = np.empty(shape)
+    a0 = np.empty(shape)
+    gm = np.empty(shape)
+    props = np.stack([rng.uniform(1450, 1620, n_layers), rng.uniform(900, 1100, n_layers),
+                      rng.uniform(0.3, 1.5, n_layers), rng.uniform(0.8, 1.6, n_layers)], 1)
+    depth = np.arange(nx)[:, None].astype(np.float64)
+    lat = np.arange(ny)[None, :].astype(np.float64)
+    layer = np.zeros(shape, dtype=np.int64)
+    bounds = np.linspace(0, nx, n_layers + 1)[1:-1]
+    for b in bounds:
+        k = rng.uniform(1, 4) * 2 * np.pi / ny
+        ph = rng.uniform(0, 2 * np.pi)
+        iface = b + amp_cells * np.sin(k * lat + ph)
+        layer += (depth >= iface).astype(np.int64)
+    for i in range(n_layers):
+        sel = layer == i
+        c[sel], rho[sel], a0[sel], gm[sel] = props[i]
+    density = scatter_per_cell / float(wavelength_cells**2)
+    scat = rng.random(shape) < density
+    rho = rho * np.where(scat, 1.0 + 0.02 * rng.standard_normal(shape), 1.0)
+    return c, rho, a0, gm
+
+
+def focused_linear_array_2d(nx: int = 4096, ny: int = 2048, nt: int = 8000, *, n_elements: int = 128,
+                            pitch_cells: int = 12, focus_m: float = 40e-3, seed: int = 2026,
+                            n_pml: int = 36, layers: int = 3, isotropic: bool = True,
+                            f0: float = 3e6, ncycles: int = 3) -> Workload:
+    """Heterogeneous 2D medium with a focused transmit from a linear array
+    (the 4096 x 2048 / 8000-step configuration at default sizes).
+
+    The array sits on the top ``layers`` rows (a thick source with one-cell transit delays per
+    row, as the reference's examples build it), element width ``pitch_cells - 1`` (one-cell
+    kerf), focused at ``focus_m`` on the centre line. Receivers are the element faces one row
+    below the source layers; traces are averaged per element afterwards.
+    """
+    grid = _grid(nx, ny, nt=nt, f0=f0)
+    shape = (grid.nx, grid.ny)
+    rng = np.random.default_rng(seed)
+    c, rho, a0, gm = layered_medium_2d(shape, rng, wavelength_cells=int(round(grid.wavelength / grid.dx)))
+    medium = Medium(grid, c, rho, a0, gm, np.zeros(shape), use_isotropic_relaxation=isotropic)
+    width = pitch_cells - 1
+    aperture = n_elements * pitch_cells
+    y0 = (ny - aperture) // 2
+    elem_of_col = np.full(ny, -1)
+    for e in range(n_elements):
+        elem_of_col[y0 + e * pitch_cells : y0 + e * pitch_cells + width] = e
+    cols = np.nonzero(elem_of_col >= 0)[0]
+    centres = np.array([y0 + e * pitch_cells + (width - 1) / 2 for e in range(n_elements)])
+    yc = (ny - 1) / 2
+    dist = np.sqrt(focus_m**2 + ((centres - yc) * grid.dx) ** 2)
+    delay = (dist.max() - dist) / grid.c0
+    mask = np.zeros(shape, dtype=bool)
+    mask[:layers, cols] = True
+    coords = np.argwhere(mask)
+    p0 = np.zeros((coords.shape[0], grid.nt))
+    cache = {}
+    for r, (ix, iy) in enumerate(coords):
+        e = elem_of_col[iy]
+        key = (e, ix)
+        if key not in cache:
+            cache[key] = gaussian_modulated_sinusoidal_signal(
+                grid.nt, grid.duration, ncycles, 2, f0, 1e5, delay_sec=delay[e],
+                i_layer=int(ix), dt_for_layer_delay=grid.dt, cfl_for_layer_delay=grid.cfl)
+        p0[r] = cache[key]
+    source = Source(p0, mask)
+    smask = np.zeros(shape, dtype=bool)
+    smask[layers, cols] = True
+    sensor = Sensor(smask)
+    inputs = _inputs(grid, medium, source, sensor, n_pml=n_pml, isotropic=isotropic)
+    elem = elem_of_col[np.argwhere(smask)[:, 1]]
+    return Workload("focused_linear_array_2d", inputs, element_of_sensor=elem, n_elements=n_elements)
+
+
+def element_traces(traces: np.ndarray, workload: Workload) -> np.ndarray:
+    """Average ``[n_records, n_sensors]`` traces per array element -> ``[n_elements, n_records]``."""
+    e = workload.element_of_sensor
+    out = np.zeros((workload.n_elements, traces.shape[0]))
+    for k in range(workload.n_elements):
+        out[k] = traces[:, e == k].mean(axis=1)
+    return out
+
+
+def random_medium_2d(nx: int, ny: int, nt: int, *, seed: int = 0, n_pml: int = 6, n_transition: int = 4,
+                     isotropic: bool = True, n_src: int = 5, n_sensors: int = 17, air: int = 0) -> Workload:
+    """Small seeded heterogeneous 2D problem for parity checks: random smooth c/rho/alpha/gamma,
+    a few point sources with distinct pulses, scattered sensors and optional air points."""
+    grid = _grid(nx, ny, nt=nt)
+    shape = (grid.nx, grid.ny)
+    rng = np.random.default_rng(seed)
+    from scipy.ndimage import gaussian_filter
+
+    def field(lo, hi):
+        f = gaussian_filter(rng.random(shape), 2.0)
+        f = (f - f.min()) / (np.ptp(f) + 1e-12)
+        return lo + (hi - lo) * f
+
+    c = field(1450, 1620)
+    rho = field(900, 1100)
+    # a few attenuation levels keep the number of distinct relaxation fits small
+    a0 = np.round(field(0.3, 1.5), 1)
+    gm = np.round(field(0.8, 1.6), 1)
+    air_map = np.zeros(shape, dtype=bool)
+    if air:
+        idx = rng.choice(nx * ny, size=air, replace=False)
+        air_map.reshape(-1)[idx] = True
+    medium = Medium(grid, c, rho, a0, gm, np.zeros(shape), air_map=air_map, use_isotropic_relaxation=isotropic)
+    mask = np.zeros(shape, dtype=bool)
+    mask.reshape(-1)[rng.choice(nx * ny, size=n_src, replace=False)] = True
+    p0 = np.stack([gaussian_modulated_sinusoidal_signal(grid.nt, grid.duration, 2, 2, 3e6, 1e5,
+                                                         delay_sec=1e-7 * i) for i in range(n_src)])
+    source = Source(p0, mask)
+    smask = np.zeros(shape, dtype=bool)
+    smask.reshape(-1)[rng.choice(nx * ny, size=n_sensors, replace=False)] = True
+    sensor = Sensor(smask, sampling_modulus_time=1)
+    return Workload("random_medium_2d", _inputs(grid, medium, source, sensor, n_pml=n_pml,
+                                                n_transition=n_transition, isotropic=isotropic))
+
+
+def random_medium_3d(nx: int, ny: int, nz: int, nt: int, *, seed: int = 0, n_pml: int = 4,
+                     n_transition: int = 3, isotropic: bool = True, n_src: int = 4,
+                     n_sensors: int = 13, air: int = 0) -> Workload:
+    """Small seeded heterogeneous 3D problem for parity checks."""
+    grid = _grid(nx, ny, nt=nt, nz=nz)
+    shape = (grid.nx, grid.ny, grid.nz)
+    rng = np.random.default_rng(seed)
+    from scipy.ndimage import gaussian_filter
+
+    def field(lo, hi):
+        f = gaussian_filter(rng.random(shape), 1.5)
+        f = (f - f.min()) / (np.ptp(f) + 1e-12)
+        return lo + (hi - lo) * f
+
+    air_map = np.zeros(shape, dtype=bool)
+    if air:
+        air_map.reshape(-1)[rng.choice(air_map.size, size=air, replace=False)] = True
+    medium = Medium(grid, field(1450, 1620), field(900, 1100), np.round(field(0.3, 1.5), 1),
+                    np.round(field(0.7, 1.5), 1),
+                    np.zeros(shape), air_map=air_map, use_isotropic_relaxation=isotropic)
+    n = int(np.prod(shape))
+    mask = np.zeros(shape, dtype=bool)
+    mask.reshape(-1)[rng.choice(n, size=n_src, replace=False)] = True
+    p0 = np.stack([gaussian_modulated_sinusoidal_signal(grid.nt, grid.duration, 2, 2, 3e6, 1e5,
+                                                         delay_sec=1e-7 * i) for i in range(n_src)])
+    source = Source(p0, mask)
+    smask = np.zeros(shape, dtype=bool)
+    smask.reshape(-1)[rng.choice(n, size=n_sensors, replace=False)] = True
+    sensor = Sensor(smask)
+    return Workload("random_medium_3d", _inputs(grid, medium, source, sensor, n_pml=n_pml,
+                                                n_transition=n_transition, isotropic=isotropic))

@@ -1,0 +1,109 @@
+/*
+ * fullwave_hip.h -- C ABI of the MI355X Fullwave 2.5 time-stepping engine.
+ *
+ * The reference runs its engine as a separate executable: Launcher.run() chdirs into a
+ * directory of raw .dat inputs, runs the binary and reads genout.dat back
+ * (fullwave/solver/launcher.py:160-254, called from fullwave/solver/solver.py:756-759).
+ * This library replaces that process boundary in two ways:
+ *
+ *   - fw_run_dir():  same contract as the executable -- reads the input files the reference's
+ *                    InputFileWriter writes (fullwave/solver/input_file_writer.py:563-881) and
+ *                    writes genout.dat (read by fullwave/solver/utils.py:12-45);
+ *   - fw_sim_*():    the same inputs handed over in memory, one handle per simulation.
+ *
+ * Conventions: plain C types only; every function returns an fw_status (0 = FW_OK) and never
+ * throws; on failure fw_last_error() returns a message for the calling thread. Host buffers
+ * belong to the caller and are only read during the call. Arrays are C order, the last axis
+ * fastest; coordinate lists are int32 [n, ndim] as in icc.dat / outc.dat / icczero.dat.
+ * Calls on one handle must not overlap; different handles may run from different threads.
+ */
+#ifndef FULLWAVE_HIP_H
+#define FULLWAVE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FW_ABI_VERSION 1
+
+typedef enum fw_status {
+    FW_OK = 0,
+    FW_ERR_INVALID = 1,     /* bad argument, missing input, inconsistent sizes */
+    FW_ERR_DEVICE = 2,      /* HIP runtime error */
+    FW_ERR_NOMEM = 3,       /* device allocation failed */
+    FW_ERR_IO = 4,          /* file missing or short (fw_run_dir) */
+    FW_ERR_STATE = 5,       /* call not valid in the handle's current state */
+    FW_ERR_UNSUPPORTED = 6  /* valid request this build does not implement */
+} fw_status;
+
+/* Simulation description: the scalar .dat files of the contract
+ * (fullwave/solver/input_file_writer.py:764-813). */
+typedef struct fw_desc {
+    int32_t abi_version; /* FW_ABI_VERSION */
+    int32_t ndim;        /* 2 or 3 */
+    int32_t isotropic;   /* 1: kappau/kappax, apml/bpml{u,x}{nu}; 0: per-axis maps */
+    int32_t n_relax;     /* relaxation mechanisms; 2 */
+    int64_t nx, ny, nz;  /* grid (nX, nY, nZ); nz = 1 in 2D */
+    int64_t nt;          /* time steps of a full run (nT) */
+    double dx, dy, dz, dt, c0;
+    int64_t ncoords;     /* source points (ncoords) */
+    int64_t ntic;        /* samples per source signal (nTic) */
+    int64_t ncoordsout;  /* sensors (ncoordsout) */
+    int64_t modt;        /* record every modt steps (modT) */
+    int64_t ncoordszero; /* pressure-release points (ncoordszero) */
+    int64_t ndmap;       /* coefficient sets in dmap (ndmap) */
+} fw_desc;
+
+typedef struct fw_sim fw_sim;
+
+/* ---- library ------------------------------------------------------------------------ */
+int fw_abi_version(void);
+const char* fw_version_string(void);
+/* Replaces the reference's CUDA device probing (fullwave/solver/cuda_utils.py:317-395,
+ * fullwave/solver/launcher.py:107-139). */
+int fw_device_count(int* count);
+/* Message of the last failed call on this thread ("" if none). */
+const char* fw_last_error(void);
+
+/* ---- directory contract: drop-in for running the engine executable ------------------- */
+/* Reads every input of sim_dir, runs nT steps on `device` and writes sim_dir/genout.dat
+ * (float32 [n_out, ncoordsout], n_out = ceil(nT / modT)). Replaces
+ * fullwave/solver/launcher.py:196-215. */
+int fw_run_dir(const char* sim_dir, int device);
+
+/* ---- in-memory handle ------------------------------------------------------------------ */
+int fw_sim_create(const fw_desc* desc, int device, fw_sim** out);
+/* Upload one input by its file stem: "K", "rho", "c", "beta", "dcmap" (int32 per voxel),
+ * "dmap" (float32 [9, 2, ndmap]), "icc", "outc", "icczero" (int32 [n, ndim]),
+ * "icmat" (float32 [ncoords, ntic]), and the relaxation maps "kappau", "kappax",
+ * "apmlu1", "bpmlu1", ... (isotropic) or additionally "kappaw", "kappay", "kappav", "kappaz",
+ * "apmlw1", ... (anisotropic). Per-voxel maps are float32 [nx, ny(, nz)]. `bytes` must match. */
+int fw_sim_upload(fw_sim* sim, const char* name, const void* host, size_t bytes);
+/* Advance n_steps time steps (synchronous). The first call checks that every input is there. */
+int fw_sim_run(fw_sim* sim, int64_t n_steps);
+/* Zero the wave state, memory variables, recorded traces and the step counter. */
+int fw_sim_reset(fw_sim* sim);
+int fw_sim_step_count(const fw_sim* sim, int64_t* steps);
+/* Number of records written so far (ceil(steps / modt)). */
+int fw_sim_record_count(const fw_sim* sim, int64_t* n_records);
+/* Recorded traces, float32 [n_records, ncoordsout] time-major (genout.dat layout). */
+int fw_sim_download_sensors(fw_sim* sim, float* out, size_t bytes);
+/* Current wave field by name: "p", "vx", "vy", "vz"; float32 [nx, ny(, nz)]. */
+int fw_sim_download_field(fw_sim* sim, const char* name, float* out, size_t bytes);
+/* Overwrite a wave field by name ("p", "vx", "vy", "vz") from float32 [nx, ny(, nz)]; ghost
+ * cells are left as given. Used to start from an initial condition or a checkpoint. */
+int fw_sim_set_field(fw_sim* sim, const char* name, const float* values, size_t bytes);
+/* Run n_steps with a HIP event pair around every launch on the handle's stream and add the
+ * summed milliseconds per kernel kind to ms[0..n_kinds) (kinds: 0 velocity, 1 pressure,
+ * 2 source/air/sensor); also returns the number of kinds in *n_kinds. */
+int fw_sim_run_timed(fw_sim* sim, int64_t n_steps, double* ms, int32_t* n_kinds);
+int fw_sim_destroy(fw_sim* sim);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FULLWAVE_HIP_H */

@@ -1,0 +1,210 @@
+/*
+ * TEST INFRASTRUCTURE ONLY: body of the oracle, included by fw_oracle.c once per precision
+ * with REAL and SFX defined. The update is described at the top of fw_oracle.c.
+ */
+
+typedef struct {
+    fwo_problem P;
+    int64_t n; /* cells */
+    int64_t step;
+    int64_t s[3];
+    REAL h[3];
+    REAL dt;
+    REAL* p;
+    REAL* v[3];
+    REAL* psi[3][2];
+    REAL* phi[3][2];
+    REAL* K;
+    REAL* rho;
+    REAL* kv[3];
+    REAL* kp[3];
+    REAL* av[3][2];
+    REAL* bv[3][2];
+    REAL* ap[3][2];
+    REAL* bp[3][2];
+    REAL* coef;
+} SFX(fwo_state);
+
+static REAL* SFX(fwo_conv)(const double* src, int64_t n) {
+    REAL* dst = (REAL*)malloc(sizeof(REAL) * (size_t)n);
+    for (int64_t i = 0; i < n; ++i) dst[i] = (REAL)src[i];
+    return dst;
+}
+
+void* SFX(fwo_create)(const fwo_problem* P) {
+    SFX(fwo_state)* st = (SFX(fwo_state)*)calloc(1, sizeof(SFX(fwo_state)));
+    if (!st) return 0;
+    st->P = *P;
+    int64_t nz = P->ndim == 3 ? P->nz : 1;
+    st->n = P->nx * P->ny * nz;
+    st->s[0] = P->ny * nz;
+    st->s[1] = nz;
+    st->s[2] = 1;
+    st->h[0] = (REAL)P->dx;
+    st->h[1] = (REAL)P->dy;
+    st->h[2] = (REAL)P->dz;
+    st->dt = (REAL)P->dt;
+    st->p = (REAL*)calloc((size_t)st->n, sizeof(REAL));
+    for (int a = 0; a < P->ndim; ++a) {
+        st->v[a] = (REAL*)calloc((size_t)st->n, sizeof(REAL));
+        st->kv[a] = SFX(fwo_conv)(P->kappa_v[a], st->n);
+        st->kp[a] = SFX(fwo_conv)(P->kappa_p[a], st->n);
+        for (int nu = 0; nu < 2; ++nu) {
+            st->psi[a][nu] = (REAL*)calloc((size_t)st->n, sizeof(REAL));
+            st->phi[a][nu] = (REAL*)calloc((size_t)st->n, sizeof(REAL));
+            st->av[a][nu] = SFX(fwo_conv)(P->a_v[a][nu], st->n);
+            st->bv[a][nu] = SFX(fwo_conv)(P->b_v[a][nu], st->n);
+            st->ap[a][nu] = SFX(fwo_conv)(P->a_p[a][nu], st->n);
+            st->bp[a][nu] = SFX(fwo_conv)(P->b_p[a][nu], st->n);
+        }
+    }
+    st->K = SFX(fwo_conv)(P->K, st->n);
+    st->rho = SFX(fwo_conv)(P->rho, st->n);
+    st->coef = SFX(fwo_conv)(P->coef, P->ndmap * 9);
+    return st;
+}
+
+void SFX(fwo_destroy)(void* handle) {
+    SFX(fwo_state)* st = (SFX(fwo_state)*)handle;
+    if (!st) return;
+    free(st->p);
+    for (int a = 0; a < st->P.ndim; ++a) {
+        free(st->v[a]);
+        free(st->kv[a]);
+        free(st->kp[a]);
+        for (int nu = 0; nu < 2; ++nu) {
+            free(st->psi[a][nu]);
+            free(st->phi[a][nu]);
+            free(st->av[a][nu]);
+            free(st->bv[a][nu]);
+            free(st->ap[a][nu]);
+            free(st->bp[a][nu]);
+        }
+    }
+    free(st->K);
+    free(st->rho);
+    free(st->coef);
+    free(st);
+}
+
+static inline int SFX(fwo_active)(const SFX(fwo_state)* st, int64_t i, int64_t j, int64_t k) {
+    const fwo_problem* P = &st->P;
+    if (i < 8 || i >= P->nx - 8 || j < 8 || j >= P->ny - 8) return 0;
+    if (P->ndim == 3 && (k < 8 || k >= P->nz - 8)) return 0;
+    return 1;
+}
+
+/* velocity half step: v^{n+1/2} from p^n */
+static void SFX(fwo_velocity)(SFX(fwo_state)* st) {
+    const int nd = st->P.ndim;
+    const int64_t nx = st->P.nx, ny = st->P.ny, nz = nd == 3 ? st->P.nz : 1;
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < nx; ++i) {
+        for (int64_t j = 0; j < ny; ++j) {
+            for (int64_t k = 0; k < nz; ++k) {
+                if (!SFX(fwo_active)(st, i, j, k)) continue;
+                const int64_t c = (i * ny + j) * nz + k;
+                const REAL* w = st->coef + 9 * (int64_t)st->P.dcmap[c];
+                for (int a = 0; a < nd; ++a) {
+                    const int64_t sa = st->s[a];
+                    REAL acc = 0;
+                    for (int m = 1; m <= 8; ++m)
+                        acc += w[m - 1] * (st->p[c + m * sa] - st->p[c - (m - 1) * sa]);
+                    REAL tr = 0;
+                    for (int b = 0; b < nd; ++b) {
+                        if (b == a) continue;
+                        const int64_t sb = st->s[b];
+                        tr += (st->p[c + sa + sb] - st->p[c + sb]) + (st->p[c + sa - sb] - st->p[c - sb]);
+                    }
+                    const REAL D = (acc + w[8] * tr) / st->h[a];
+                    REAL mem = 0;
+                    for (int nu = 0; nu < 2; ++nu) {
+                        REAL q = st->bv[a][nu][c] * st->psi[a][nu][c] + st->av[a][nu][c] * D;
+                        st->psi[a][nu][c] = q;
+                        mem += q;
+                    }
+                    const REAL rho_bar = (REAL)0.5 * (st->rho[c] + st->rho[c + sa]);
+                    st->v[a][c] -= st->dt / rho_bar * (D / st->kv[a][c] + mem);
+                }
+            }
+        }
+    }
+}
+
+/* pressure step: p^{n+1} from v^{n+1/2} */
+static void SFX(fwo_pressure)(SFX(fwo_state)* st) {
+    const int nd = st->P.ndim;
+    const int64_t nx = st->P.nx, ny = st->P.ny, nz = nd == 3 ? st->P.nz : 1;
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < nx; ++i) {
+        for (int64_t j = 0; j < ny; ++j) {
+            for (int64_t k = 0; k < nz; ++k) {
+                if (!SFX(fwo_active)(st, i, j, k)) continue;
+                const int64_t c = (i * ny + j) * nz + k;
+                const REAL* w = st->coef + 9 * (int64_t)st->P.dcmap[c];
+                REAL div = 0;
+                for (int a = 0; a < nd; ++a) {
+                    const REAL* va = st->v[a];
+                    const int64_t sa = st->s[a];
+                    REAL acc = 0;
+                    for (int m = 1; m <= 8; ++m)
+                        acc += w[m - 1] * (va[c + (m - 1) * sa] - va[c - m * sa]);
+                    REAL tr = 0;
+                    for (int b = 0; b < nd; ++b) {
+                        if (b == a) continue;
+                        const int64_t sb = st->s[b];
+                        tr += (va[c + sb] - va[c - sa + sb]) + (va[c - sb] - va[c - sa - sb]);
+                    }
+                    const REAL D = (acc + w[8] * tr) / st->h[a];
+                    REAL mem = 0;
+                    for (int nu = 0; nu < 2; ++nu) {
+                        REAL q = st->bp[a][nu][c] * st->phi[a][nu][c] + st->ap[a][nu][c] * D;
+                        st->phi[a][nu][c] = q;
+                        mem += q;
+                    }
+                    div += D / st->kp[a][c] + mem;
+                }
+                st->p[c] -= st->dt * st->K[c] * div;
+            }
+        }
+    }
+}
+
+/* Advance nsteps; rec (may be NULL) is [ceil(total / modt)][nout] in double, indexed by the
+ * global step count. Returns 0. */
+int SFX(fwo_run)(void* handle, int64_t nsteps, double* rec) {
+    SFX(fwo_state)* st = (SFX(fwo_state)*)handle;
+    const fwo_problem* P = &st->P;
+    for (int64_t it = 0; it < nsteps; ++it) {
+        const int64_t n = st->step;
+        SFX(fwo_velocity)(st);
+        SFX(fwo_pressure)(st);
+        if (n < P->ntic)
+            for (int64_t s = 0; s < P->ncoords; ++s) st->p[P->icc[s]] = (REAL)P->icmat[s * P->ntic + n];
+        for (int64_t q = 0; q < P->nzero; ++q) st->p[P->zero[q]] = 0;
+        if (rec && P->modt > 0 && n % P->modt == 0) {
+            double* row = rec + (n / P->modt) * P->nout;
+            for (int64_t j = 0; j < P->nout; ++j) row[j] = (double)st->p[P->outc[j]];
+        }
+        st->step = n + 1;
+    }
+    return 0;
+}
+
+/* Copy a wave field out as double: 0 = p, 1..3 = v_x, v_y, v_z. */
+int SFX(fwo_get_field)(void* handle, int which, double* out) {
+    SFX(fwo_state)* st = (SFX(fwo_state)*)handle;
+    const REAL* f = which == 0 ? st->p : (which - 1 < st->P.ndim ? st->v[which - 1] : 0);
+    if (!f) return 1;
+    for (int64_t i = 0; i < st->n; ++i) out[i] = (double)f[i];
+    return 0;
+}
+
+/* Overwrite a wave field from double (for tests that start from an initial condition). */
+int SFX(fwo_set_field)(void* handle, int which, const double* in) {
+    SFX(fwo_state)* st = (SFX(fwo_state)*)handle;
+    REAL* f = which == 0 ? st->p : (which - 1 < st->P.ndim ? st->v[which - 1] : 0);
+    if (!f) return 1;
+    for (int64_t i = 0; i < st->n; ++i) f[i] = (REAL)in[i];
+    return 0;
+}

@@ -1,0 +1,64 @@
+"""The C-ABI library: it loads, exports every declared symbol, and validates without a GPU."""
+
+import ctypes
+import re
+from pathlib import Path
+
+import pytest
+
+from fullwave25_amd import engine
+
+HEADER = Path(__file__).resolve().parents[1] / "include" / "fullwave_hip.h"
+
+
+def declared_functions() -> set[str]:
+    text = HEADER.read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return set(re.findall(r"\b(fw_[a-z_]+)\s*\(", text))
+
+
+def test_library_is_built():
+    assert engine.LIB_PATH.exists(), "run __graft_entry__.build() first"
+
+
+def test_exports_every_declared_symbol():
+    lib = ctypes.CDLL(str(engine.LIB_PATH))
+    declared = declared_functions()
+    assert declared, "no declarations parsed"
+    missing = [name for name in sorted(declared) if not hasattr(lib, name)]
+    assert not missing, missing
+    assert declared == set(engine.EXPORTED_SYMBOLS)
+
+
+def test_abi_version_and_string():
+    lib = engine.load_library()
+    assert lib.fw_abi_version() == 1
+    assert b"gfx950" in lib.fw_version_string()
+
+
+def test_invalid_descriptor_rejected_before_device():
+    lib = engine.load_library()
+    desc = engine.FwDesc(abi_version=1, ndim=4, isotropic=1, n_relax=2, nx=32, ny=32, nz=1, nt=1,
+                         dx=1e-4, dy=1e-4, dz=0, dt=1e-8, c0=1540, modt=1, ndmap=1)
+    h = ctypes.c_void_p()
+    rc = lib.fw_sim_create(ctypes.byref(desc), 0, ctypes.byref(h))
+    assert rc == 1  # FW_ERR_INVALID
+    assert b"ndim" in lib.fw_last_error()
+    desc.ndim, desc.n_relax = 2, 3
+    assert lib.fw_sim_create(ctypes.byref(desc), 0, ctypes.byref(h)) == 6  # FW_ERR_UNSUPPORTED
+    desc.n_relax, desc.nx = 2, 10
+    assert lib.fw_sim_create(ctypes.byref(desc), 0, ctypes.byref(h)) == 1
+    assert b"ghost" in lib.fw_last_error()
+
+
+def test_null_arguments_rejected():
+    lib = engine.load_library()
+    assert lib.fw_sim_run(None, 1) == 1
+    assert lib.fw_device_count(None) == 1
+    assert lib.fw_run_dir(None, 0) == 1
+    assert lib.fw_sim_destroy(None) == 0
+
+
+def test_missing_library_fails_loudly(tmp_path):
+    with pytest.raises(RuntimeError, match="missing"):
+        engine.load_library(tmp_path / "libfullwave_hip.so")

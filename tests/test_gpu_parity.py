@@ -1,0 +1,150 @@
+"""HIP engine vs the FP64 oracle, through the C ABI (needs an MI355X).
+
+Tolerance: relative L2 error of the sensor traces <= 1e-4 (FP32 engine vs FP64 oracle on the
+same float32 inputs).
+"""
+
+import dataclasses
+
+import numpy as np
+import pytest
+
+import oracle
+from fullwave25_amd import engine, workloads
+from tests.helpers import raw_inputs, rel_l2
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-4
+
+
+def _gpu(inp, n=None):
+    with engine.HipSimulation(inp) as sim:
+        sim.run(inp.nt if n is None else n)
+        return sim.sensors().astype(np.float64), sim.field("p").astype(np.float64)
+
+
+def _ref(inp, n=None):
+    o = oracle.OracleSimulation(inp, 64)
+    o.run(inp.nt if n is None else n)
+    return o.sensors(), o.field("p")
+
+
+CASES = {
+    "2d_iso": lambda: workloads.random_medium_2d(60, 52, 300, seed=11, air=6).inputs,
+    "2d_aniso": lambda: workloads.random_medium_2d(44, 70, 200, seed=12, isotropic=False).inputs,
+    "2d_ragged": lambda: workloads.random_medium_2d(37, 131, 150, seed=13, n_src=9, n_sensors=40).inputs,
+    "3d_iso": lambda: workloads.random_medium_3d(20, 18, 22, 120, seed=14, air=4).inputs,
+    "3d_aniso": lambda: workloads.random_medium_3d(16, 21, 19, 90, seed=15, isotropic=False).inputs,
+}
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_engine_matches_fp64_oracle(case):
+    inp = CASES[case]()
+    print(f"case {case}: shape {inp.shape}, nt {inp.nt}", flush=True)
+    got, p_got = _gpu(inp)
+    want, p_want = _ref(inp)
+    assert np.abs(want).max() > 0
+    assert got.shape == want.shape
+    err = rel_l2(got, want)
+    print(f"case {case}: rel-L2 traces {err:.3e}, field {rel_l2(p_got, p_want):.3e}", flush=True)
+    assert err <= TOL
+    assert rel_l2(p_got, p_want) <= TOL
+
+
+def test_plane_wave_512_full_length():
+    """The 512 x 512, 2000-step homogeneous plane-wave configuration at full size."""
+    inp = workloads.plane_wave_2d().inputs
+    print(f"plane wave: shape {inp.shape}, nt {inp.nt}", flush=True)
+    got, _ = _gpu(inp)
+    want, _ = _ref(inp)
+    err = rel_l2(got, want)
+    print(f"plane wave 512: rel-L2 {err:.3e}", flush=True)
+    assert err <= TOL
+
+
+def test_source_air_sensor_semantics():
+    shape = (24, 24)
+    icc = [[12, 12], [12, 12], [14, 10], [15, 15]]
+    sig = np.stack([np.full(20, 1.0), np.full(20, 2.0), np.full(20, 3.0), np.full(20, 4.0)])
+    inp = raw_inputs(shape, nt=20, icc=icc, icmat=sig, outc=[[12, 12], [14, 10], [15, 15], [10, 10]],
+                     icczero=[[15, 15]])
+    got, _ = _gpu(inp)
+    want, _ = _ref(inp)
+    np.testing.assert_allclose(got[:, :3], want[:, :3], rtol=0, atol=0)
+    assert rel_l2(got, want) <= TOL
+
+
+def test_deterministic():
+    inp = workloads.random_medium_2d(40, 40, 100, seed=21).inputs
+    a, _ = _gpu(inp)
+    b, _ = _gpu(inp)
+    np.testing.assert_array_equal(a, b)
+
+
+def test_run_in_pieces_and_reset():
+    inp = workloads.random_medium_2d(40, 36, 120, seed=22).inputs
+    full, _ = _gpu(inp)
+    with engine.HipSimulation(inp) as sim:
+        sim.run(50)
+        sim.run(70)
+        pieces = sim.sensors().astype(np.float64)
+        sim.reset()
+        assert sim.step == 0
+        sim.run(120)
+        again = sim.sensors().astype(np.float64)
+    np.testing.assert_array_equal(pieces, full)
+    np.testing.assert_array_equal(again, full)
+
+
+def test_modt_and_record_count():
+    inp = dataclasses.replace(workloads.random_medium_2d(30, 30, 95, seed=23).inputs, modt=4)
+    got, _ = _gpu(inp)
+    assert got.shape[0] == -(-95 // 4)
+    want, _ = _ref(inp)
+    assert rel_l2(got, want) <= TOL
+
+
+def test_reciprocity_on_gpu():
+    """Initial-pressure reciprocity p_{A->B} / K_B = p_{B->A} / K_A (uniform c, varying rho,
+    no absorbing layer): holds when the velocity and pressure stencils are adjoint."""
+    shape = (40, 44)
+    rng = np.random.default_rng(5)
+    rho = 1000 + 200 * rng.random(shape)
+    inp = raw_inputs(shape, rho=rho, nt=60)
+    K = inp.maps["K"].astype(np.float64)
+    A, B = (15, 17), (24, 29)
+    out = []
+    for src, dst in ((A, B), (B, A)):
+        p0 = np.zeros(shape, np.float32)
+        p0[src] = 1.0
+        with engine.HipSimulation(inp) as sim:
+            sim.set_field("p", p0)
+            sim.run(60)
+            out.append(sim.field("p").astype(np.float64)[dst] / K[dst])
+    assert abs(out[0]) > 0
+    assert abs(out[0] - out[1]) <= 1e-4 * abs(out[0])
+
+
+def test_run_dir_contract(tmp_path):
+    inp = workloads.random_medium_2d(32, 30, 80, seed=24, air=3).inputs
+    inp.write_dir(tmp_path / "txrx_0")
+    engine.run_dir(tmp_path / "txrx_0")
+    genout = np.fromfile(tmp_path / "txrx_0" / "genout.dat", dtype=np.float32).reshape(-1, inp.ncoordsout)
+    want, _ = _ref(inp)
+    assert rel_l2(genout, want) <= TOL
+
+
+def test_invalid_coordinates_raise():
+    inp = workloads.random_medium_2d(30, 30, 20, seed=25).inputs
+    bad = dataclasses.replace(inp, outc=np.array([[0, 10_000]], np.int32))
+    with engine.HipSimulation(bad) as sim, pytest.raises(engine.EngineError, match="outside"):
+        sim.run(1)
+
+
+def test_run_past_nt_raises():
+    inp = workloads.random_medium_2d(30, 30, 10, seed=26).inputs
+    with engine.HipSimulation(inp) as sim:
+        sim.run(10)
+        with pytest.raises(engine.EngineError, match="exceeds"):
+            sim.run(1)
