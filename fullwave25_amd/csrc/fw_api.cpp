@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <map>
@@ -254,7 +255,10 @@ int finalize(fw_sim* s) {
     s->ds.dcmap = (const int32_t*)s->dev["dcmap"];
     s->ds.ndmap = (int32_t)d.ndmap;
     s->ds.isotropic = d.isotropic;
-    s->use_streaming = fw::streaming_supported(s->ds);
+    // FULLWAVE_HIP_KERNELS=percell selects the per-cell kernels (used to cross-check the two
+    // GPU implementations)
+    const char* kv = getenv("FULLWAVE_HIP_KERNELS");
+    s->use_streaming = fw::streaming_supported(s->ds) && !(kv && !strcmp(kv, "percell"));
     s->ready = true;
     return FW_OK;
 }

@@ -148,3 +148,44 @@ def test_run_past_nt_raises():
         sim.run(10)
         with pytest.raises(engine.EngineError, match="exceeds"):
             sim.run(1)
+
+
+def test_streaming_kernels_match_per_cell_kernels(monkeypatch):
+    """The two GPU implementations (x-marching and per-cell) on the same inputs."""
+    for inp in (workloads.random_medium_2d(70, 300, 120, seed=31, air=5).inputs,
+                workloads.random_medium_2d(150, 90, 100, seed=32, isotropic=False).inputs,
+                workloads.plane_wave_2d(60, 280, 150, n_pml=10, sensor_depth=40).inputs):
+        a, pa = _gpu(inp)
+        monkeypatch.setenv("FULLWAVE_HIP_KERNELS", "percell")
+        b, pb = _gpu(inp)
+        monkeypatch.delenv("FULLWAVE_HIP_KERNELS")
+        assert rel_l2(a, b) <= 1e-6
+        assert rel_l2(pa, pb) <= 1e-6
+
+
+def test_solver_end_to_end(tmp_path):
+    """Front end -> Solver.run -> engine, in memory and through the directory contract."""
+    import fullwave25_amd as fw
+    from fullwave25_amd.utils.pulse import gaussian_modulated_sinusoidal_signal
+
+    grid = workloads._grid(60, 50, nt=160)
+    shape = (grid.nx, grid.ny)
+    medium = fw.Medium(grid, np.full(shape, 1540.0), np.full(shape, 1000.0), np.full(shape, 0.5),
+                       np.full(shape, 1.0), np.zeros(shape))
+    mask = np.zeros(shape, bool)
+    mask[0, :] = True
+    sig = gaussian_modulated_sinusoidal_signal(grid.nt, grid.duration, 2, 2, 3e6, 1e5)
+    source = fw.Source(np.tile(sig, (shape[1], 1)), mask)
+    smask = np.zeros(shape, bool)
+    smask[40, ::5] = True
+    sensor = fw.Sensor(smask, sampling_modulus_time=2)
+    solver = fw.Solver(tmp_path, grid, medium, source, sensor, pml_layer_thickness_px=10, n_transition_layer=8)
+    out = solver.run()
+    assert out.shape == (sensor.n_sensors, -(-grid.nt // 2))
+    inputs, _ = solver.build_inputs()
+    want, _ = _ref(inputs)
+    assert rel_l2(out.T, want) <= TOL
+    path = solver.run("txrx_files", load_results=False)
+    assert path.name == "genout.dat" and (path.parent / "fw2_execution.log").exists()
+    files = np.fromfile(path, np.float32).reshape(-1, sensor.n_sensors)
+    np.testing.assert_array_equal(files, out.T)
