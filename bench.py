@@ -84,7 +84,15 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=0,
                     help="run only this many steps and exit (for rocprofv3)")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="only run the plain copy kernel over this many bytes (counter calibration)")
     args = ap.parse_args()
+    if args.copy_probe:
+        from fullwave25_amd import engine
+
+        gbs = engine.copy_bandwidth(args.copy_probe, repeats=5)
+        print(json.dumps({"copy_bytes": args.copy_probe, "copy_gbs": round(gbs, 1)}), flush=True)
+        return 0
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs one simulation per GPU and this build measures one GPU; "
                           "run with --gpus 1"}))
@@ -114,6 +122,7 @@ def main() -> int:
     sim.reset()
     kms = sim.run_timed(args.steps)
     sim.close()
+    copy_gbs = engine.copy_bandwidth(1 << 30, repeats=10)
 
     cells = inp.ncells
     updated = int(np.prod([s - 16 for s in inp.shape]))
@@ -154,6 +163,7 @@ def main() -> int:
             "bytes_per_cell": bpc[dominant],
             "avg_launch_us": round(avg_s * 1e6, 2),
             "kernel_ms_per_step": {k: round(v / args.steps, 5) for k, v in kms.items()},
+            "copy_kernel_gbs": round(copy_gbs, 1),
         },
     }
     if not args.no_cpu_baseline:

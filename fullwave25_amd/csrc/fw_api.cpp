@@ -334,6 +334,39 @@ int fw_device_count(int* count) {
     return FW_OK;
 }
 
+int fw_device_copy_bandwidth(int device, size_t bytes, int repeats, double* gbs) {
+    if (!gbs || repeats < 1 || bytes < 16) return fail(FW_ERR_INVALID, "need gbs, repeats >= 1, bytes >= 16");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev)
+        return fail(FW_ERR_DEVICE, "device %d not available", device);
+    DeviceGuard g(device);
+    const int64_t n16 = (int64_t)(bytes / 16);
+    void *a = nullptr, *b = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = FW_OK;
+    if (hipMalloc(&a, n16 * 16) != hipSuccess || hipMalloc(&b, n16 * 16) != hipSuccess) {
+        rc = fail(FW_ERR_NOMEM, "hipMalloc of 2 x %lld bytes failed", (long long)(n16 * 16));
+    } else if (hipMemset(a, 0, n16 * 16) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+               hipEventCreate(&e1) != hipSuccess) {
+        rc = fail(FW_ERR_DEVICE, "setup failed");
+    } else {
+        hipError_t e = fw::launch_copy(a, b, n16, nullptr);  // warm-up
+        (void)hipEventRecord(e0, nullptr);
+        for (int r = 0; r < repeats && e == hipSuccess; ++r) e = fw::launch_copy(a, b, n16, nullptr);
+        (void)hipEventRecord(e1, nullptr);
+        float ms = 0.f;
+        if (e != hipSuccess || hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+            rc = fail(FW_ERR_DEVICE, "copy kernel failed");
+        else
+            *gbs = 2.0 * (double)(n16 * 16) * repeats / (ms * 1e-3) / 1e9;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    return rc;
+}
+
 int fw_sim_create(const fw_desc* desc, int device, fw_sim** out) {
     if (!desc || !out) return fail(FW_ERR_INVALID, "null argument");
     *out = nullptr;

@@ -58,5 +58,6 @@ hipError_t launch_pressure(const DevSim& s, hipStream_t st);
 hipError_t launch_boundary(const DevSim& s, const DevBoundary& b, int64_t step, int64_t modt,
                            hipStream_t st);
 hipError_t launch_reciprocal(float* data, int64_t n, hipStream_t st);
+hipError_t launch_copy(const void* src, void* dst, int64_t n16, hipStream_t st);
 
 }  // namespace fw

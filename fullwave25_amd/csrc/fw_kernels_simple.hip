@@ -1,6 +1,6 @@
 // Per-cell kernels: one thread per grid cell, stencil operands straight from global memory.
 // They are the plain statement of the update (same arithmetic order as the streaming
-// kernels in fw_kernels_2d.hip / fw_kernels_3d.hip) and are used for grids the streaming
+// kernels in fw_kernels_stream.hip) and are used for grids the streaming
 // kernels do not take, and as the second GPU implementation the tests compare against.
 #include "fw_internal.h"
 
@@ -112,6 +112,11 @@ __global__ __launch_bounds__(256) void k_boundary(DevSim s, DevBoundary b, int64
     }
 }
 
+__global__ __launch_bounds__(256) void k_copy(const float4* __restrict__ src, float4* __restrict__ dst, int64_t n) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += stride) dst[t] = src[t];
+}
+
 __global__ __launch_bounds__(256) void k_reciprocal(float* data, int64_t n) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t < n) data[t] = 1.0f / data[t];
@@ -150,6 +155,14 @@ hipError_t launch_boundary(const DevSim& s, const DevBoundary& b, int64_t step, 
     if (rec_row < 0 && step >= b.ntic && b.nzero == 0) return hipSuccess;
     hipLaunchKernelGGL(k_boundary, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, s, b, step,
                        rec_row);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy(const void* src, void* dst, int64_t n16, hipStream_t st) {
+    if (n16 == 0) return hipSuccess;
+    int64_t blocks = (n16 + 255) / 256;
+    if (blocks > 256 * 64) blocks = 256 * 64;
+    hipLaunchKernelGGL(k_copy, dim3((unsigned)blocks), dim3(256), 0, st, (const float4*)src, (float4*)dst, n16);
     return hipGetLastError();
 }
 

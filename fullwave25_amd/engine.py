@@ -28,6 +28,7 @@ KERNEL_KINDS = ("velocity", "pressure", "boundary")
 # every symbol include/fullwave_hip.h declares
 EXPORTED_SYMBOLS = (
     "fw_abi_version", "fw_version_string", "fw_device_count", "fw_last_error", "fw_run_dir",
+    "fw_device_copy_bandwidth",
     "fw_sim_create", "fw_sim_upload", "fw_sim_run", "fw_sim_reset", "fw_sim_step_count",
     "fw_sim_record_count", "fw_sim_download_sensors", "fw_sim_download_field",
     "fw_sim_set_field", "fw_sim_run_timed", "fw_sim_destroy",
@@ -76,6 +77,7 @@ def load_library(path: Path | None = None) -> ctypes.CDLL:
         lib.fw_last_error.restype = ctypes.c_char_p
         lib.fw_device_count.argtypes = [ctypes.POINTER(ctypes.c_int)]
         lib.fw_run_dir.argtypes = [ctypes.c_char_p, ctypes.c_int]
+        lib.fw_device_copy_bandwidth.argtypes = [ctypes.c_int, sz, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
         lib.fw_sim_create.argtypes = [ctypes.POINTER(FwDesc), ctypes.c_int, ctypes.POINTER(vp)]
         lib.fw_sim_upload.argtypes = [vp, ctypes.c_char_p, vp, sz]
         lib.fw_sim_run.argtypes = [vp, i64]
@@ -104,6 +106,14 @@ def device_count() -> int:
     n = ctypes.c_int(0)
     _check(lib, lib.fw_device_count(ctypes.byref(n)))
     return n.value
+
+
+def copy_bandwidth(nbytes: int = 1 << 30, repeats: int = 10, device: int = 0) -> float:
+    """GB/s (read + write) of a plain device-to-device copy kernel."""
+    lib = load_library()
+    g = ctypes.c_double(0)
+    _check(lib, lib.fw_device_copy_bandwidth(int(device), int(nbytes), int(repeats), ctypes.byref(g)))
+    return g.value
 
 
 def run_dir(sim_dir: Path, device: int = 0) -> None:

@@ -67,6 +67,11 @@ const char* fw_version_string(void);
 int fw_device_count(int* count);
 /* Message of the last failed call on this thread ("" if none). */
 const char* fw_last_error(void);
+/* Device bandwidth self-check; it has no counterpart in the reference. Copies `bytes` bytes
+ * (rounded down to 16) between two device buffers with a plain vectorized copy kernel
+ * `repeats` times and returns the achieved read + write rate in GB/s. Used to calibrate
+ * hardware-counter byte counts against a kernel of known traffic. */
+int fw_device_copy_bandwidth(int device, size_t bytes, int repeats, double* gbs);
 
 /* ---- directory contract: drop-in for running the engine executable ------------------- */
 /* Reads every input of sim_dir, runs nT steps on `device` and writes sim_dir/genout.dat
