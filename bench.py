@@ -84,6 +84,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=0,
                     help="run only this many steps and exit (for rocprofv3)")
+    ap.add_argument("--sweep", type=int, default=0,
+                    help="time every kernel family/shape over this many steps and exit")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run the plain copy kernel over this many bytes (counter calibration)")
     args = ap.parse_args()
@@ -104,7 +106,25 @@ def main() -> int:
     wl = build_workload(args)
     inp = wl.inputs
     print(f"# workload {wl.name} {inp.shape} nt={inp.nt} built in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+    if args.sweep:
+        shapes = ["percell"] + [f"stream:{w}:{c}" for w in (64, 128, 256) for c in (8, 16, 32, 64)]
+        if inp.ndim == 3:
+            shapes = ["percell"] + [f"stream:64:{c}" for c in (8, 16, 32, 64)]
+        bpc = algorithmic_bytes_per_cell(inp.ndim, inp.isotropic)
+        updated = int(np.prod([s - 16 for s in inp.shape]))
+        for shape in shapes:
+            os.environ["FULLWAVE_HIP_KERNELS"] = shape
+            with engine.HipSimulation(inp, device=0) as sim:
+                sim.run_timed(min(20, args.sweep))
+                sim.reset()
+                kms = sim.run_timed(args.sweep)
+            gbs = {k: bpc[k] * updated / (kms[k] / 1e3 / args.sweep) / 1e9 for k in ("velocity", "pressure")}
+            print(json.dumps({"shape": shape, "us_per_step": {k: round(v / args.sweep * 1e3, 2) for k, v in kms.items()},
+                              "GBs": {k: round(v, 1) for k, v in gbs.items()}}), flush=True)
+        os.environ.pop("FULLWAVE_HIP_KERNELS", None)
+        return 0
     sim = engine.HipSimulation(inp, device=0)
+    kcfg = sim.kernel_config()
     if args.profile_steps:
         sim.run(args.profile_steps)
         sim.close()
@@ -164,6 +184,7 @@ def main() -> int:
             "avg_launch_us": round(avg_s * 1e6, 2),
             "kernel_ms_per_step": {k: round(v / args.steps, 5) for k, v in kms.items()},
             "copy_kernel_gbs": round(copy_gbs, 1),
+            "kernel_config": kcfg,
         },
     }
     if not args.no_cpu_baseline:

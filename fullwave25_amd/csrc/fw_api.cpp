@@ -17,11 +17,6 @@
 #include "../../include/fullwave_hip.h"
 #include "fw_internal.h"
 
-namespace fw {
-hipError_t launch_velocity_simple(const DevSim& s, hipStream_t st);
-hipError_t launch_pressure_simple(const DevSim& s, hipStream_t st);
-bool streaming_supported(const DevSim& s);
-}  // namespace fw
 
 namespace {
 
@@ -77,7 +72,8 @@ struct fw_sim {
     bool ready = false;
     int64_t step = 0;
     int64_t rec_cap = 0;
-    bool use_streaming = false;
+    fw::LaunchCfg cfg;
+    bool fields_set = false;  // a field was set before the first run: do not tune on it
     hipEvent_t ev[2] = {nullptr, nullptr};
 };
 
@@ -146,6 +142,90 @@ int check_coords(const fw_sim* s, const std::vector<int32_t>& c, int64_t n, cons
             idx = idx * dims[a] + x;
         }
         lin[(size_t)q] = idx;
+    }
+    return FW_OK;
+}
+
+// Time one configuration: the two stencil kernels, `reps` times after one warm-up.
+int time_config(fw_sim* s, const fw::LaunchCfg& cfg, int reps, float* ms) {
+    FW_HIP(fw::launch_velocity(s->ds, cfg, s->stream));
+    FW_HIP(fw::launch_pressure(s->ds, cfg, s->stream));
+    FW_HIP(hipEventRecord(s->ev[0], s->stream));
+    for (int r = 0; r < reps; ++r) {
+        FW_HIP(fw::launch_velocity(s->ds, cfg, s->stream));
+        FW_HIP(fw::launch_pressure(s->ds, cfg, s->stream));
+    }
+    FW_HIP(hipEventRecord(s->ev[1], s->stream));
+    FW_HIP(hipEventSynchronize(s->ev[1]));
+    FW_HIP(hipEventElapsedTime(ms, s->ev[0], s->ev[1]));
+    return FW_OK;
+}
+
+// Pick the kernel family and shape for this grid. FULLWAVE_HIP_KERNELS overrides:
+// "percell", or "stream[:width[:chunk]]". Otherwise every candidate is timed on the handle's
+// own arrays while the wave state is still all zero (the update then leaves it zero, so the
+// timing runs change nothing) and the fastest is kept.
+int choose_kernels(fw_sim* s) {
+    const char* env = getenv("FULLWAVE_HIP_KERNELS");
+    const bool stream_ok = fw::streaming_supported(s->ds);
+    fw::LaunchCfg def;
+    def.variant = stream_ok ? 1 : 0;
+    def.width = s->ndim == 3 ? 64 : 128;
+    def.chunk = 32;
+    if (env && *env) {
+        fw::LaunchCfg c = def;
+        if (!strncmp(env, "percell", 7)) {
+            c.variant = 0;
+        } else if (!strncmp(env, "stream", 6)) {
+            if (!stream_ok) return fail(FW_ERR_UNSUPPORTED, "FULLWAVE_HIP_KERNELS=stream: this grid needs the per-cell kernels");
+            c.variant = 1;
+            int w = 0, ch = 0;
+            const int n = sscanf(env, "stream:%d:%d", &w, &ch);
+            if (n >= 1) c.width = w;
+            if (n >= 2) c.chunk = ch;
+            if (s->ndim == 2 && c.width != 64 && c.width != 128 && c.width != 256)
+                return fail(FW_ERR_INVALID, "FULLWAVE_HIP_KERNELS: width must be 64, 128 or 256");
+            if (c.chunk < 2 || c.chunk > 4096) return fail(FW_ERR_INVALID, "FULLWAVE_HIP_KERNELS: chunk out of range");
+        } else {
+            return fail(FW_ERR_INVALID, "FULLWAVE_HIP_KERNELS must be 'percell' or 'stream[:width[:chunk]]'");
+        }
+        s->cfg = c;
+        return FW_OK;
+    }
+    s->cfg = def;
+    if (s->fields_set || !stream_ok) return FW_OK;
+    std::vector<fw::LaunchCfg> cands;
+    fw::LaunchCfg pc;
+    pc.variant = 0;
+    cands.push_back(pc);
+    const int widths2d[] = {64, 128, 256};
+    const int chunks[] = {16, 32, 64};
+    for (int ch : chunks) {
+        if (s->ndim == 2) {
+            for (int w : widths2d) {
+                fw::LaunchCfg c;
+                c.variant = 1;
+                c.width = w;
+                c.chunk = ch;
+                cands.push_back(c);
+            }
+        } else {
+            fw::LaunchCfg c;
+            c.variant = 1;
+            c.width = 64;
+            c.chunk = ch;
+            cands.push_back(c);
+        }
+    }
+    float best = 1e30f;
+    for (const auto& c : cands) {
+        float ms = 0.f;
+        int rc = time_config(s, c, 3, &ms);
+        if (rc != FW_OK) return rc;
+        if (ms < best) {
+            best = ms;
+            s->cfg = c;
+        }
     }
     return FW_OK;
 }
@@ -255,10 +335,7 @@ int finalize(fw_sim* s) {
     s->ds.dcmap = (const int32_t*)s->dev["dcmap"];
     s->ds.ndmap = (int32_t)d.ndmap;
     s->ds.isotropic = d.isotropic;
-    // FULLWAVE_HIP_KERNELS=percell selects the per-cell kernels (used to cross-check the two
-    // GPU implementations)
-    const char* kv = getenv("FULLWAVE_HIP_KERNELS");
-    s->use_streaming = fw::streaming_supported(s->ds) && !(kv && !strcmp(kv, "percell"));
+    if ((rc = choose_kernels(s)) != FW_OK) return rc;
     s->ready = true;
     return FW_OK;
 }
@@ -279,13 +356,9 @@ int launch_step(fw_sim* s, bool timed, double* ms) {
         return FW_OK;
     };
     int rc;
-    rc = timed_launch(fw::kVelocity, [&] {
-        return s->use_streaming ? fw::launch_velocity(s->ds, s->stream) : fw::launch_velocity_simple(s->ds, s->stream);
-    });
+    rc = timed_launch(fw::kVelocity, [&] { return fw::launch_velocity(s->ds, s->cfg, s->stream); });
     if (rc != FW_OK) return rc;
-    rc = timed_launch(fw::kPressure, [&] {
-        return s->use_streaming ? fw::launch_pressure(s->ds, s->stream) : fw::launch_pressure_simple(s->ds, s->stream);
-    });
+    rc = timed_launch(fw::kPressure, [&] { return fw::launch_pressure(s->ds, s->cfg, s->stream); });
     if (rc != FW_OK) return rc;
     rc = timed_launch(fw::kBoundary, [&] { return fw::launch_boundary(s->ds, s->db, s->step, s->d.modt, s->stream); });
     if (rc != FW_OK) return rc;
@@ -555,6 +628,20 @@ int fw_sim_set_field(fw_sim* s, const char* name, const float* values, size_t by
     if (bytes != (size_t)s->ncell * 4) return fail(FW_ERR_INVALID, "field buffer must be %lld bytes", (long long)s->ncell * 4);
     DeviceGuard g(s->device);
     FW_HIP(hipMemcpy(dst, values, bytes, hipMemcpyHostToDevice));
+    if (!s->ready) s->fields_set = true;
+    return FW_OK;
+}
+
+int fw_sim_kernel_config(fw_sim* s, int32_t* variant, int32_t* width, int32_t* chunk) {
+    if (!s || !variant || !width || !chunk) return fail(FW_ERR_INVALID, "null argument");
+    if (!s->ready) {
+        DeviceGuard g(s->device);
+        int rc = finalize(s);
+        if (rc != FW_OK) return rc;
+    }
+    *variant = s->cfg.variant;
+    *width = s->cfg.width;
+    *chunk = s->cfg.chunk;
     return FW_OK;
 }
 

@@ -53,8 +53,19 @@ struct DevBoundary {
 
 enum KernelKind { kVelocity = 0, kPressure = 1, kBoundary = 2, kNumKinds = 3 };
 
-hipError_t launch_velocity(const DevSim& s, hipStream_t st);
-hipError_t launch_pressure(const DevSim& s, hipStream_t st);
+// Kernel family and shape for the two stencil kernels of a step.
+//   variant 0: per-cell kernels (one thread per cell)
+//   variant 1: x-marching kernels; `width` lanes per workgroup along the contiguous axis
+//              (2D: 64, 128 or 256; 3D: fixed 64 x 8 tile) and `chunk` planes per workgroup
+struct LaunchCfg {
+    int variant = 0;
+    int width = 128;
+    int chunk = 32;
+};
+
+bool streaming_supported(const DevSim& s);
+hipError_t launch_velocity(const DevSim& s, const LaunchCfg& cfg, hipStream_t st);
+hipError_t launch_pressure(const DevSim& s, const LaunchCfg& cfg, hipStream_t st);
 hipError_t launch_boundary(const DevSim& s, const DevBoundary& b, int64_t step, int64_t modt,
                            hipStream_t st);
 hipError_t launch_reciprocal(float* data, int64_t n, hipStream_t st);

@@ -1,12 +1,14 @@
-// Streaming (x-marching) kernels for gfx950.
+// Streaming (x-marching) kernels for gfx950: the 2D kernels and the dispatch for both kernel
+// families (3D marching kernels are in fw_kernels_stream3d.hip).
 //
 // 2D grid [nx][ny], y contiguous. A workgroup owns TY consecutive columns (one per lane, 64-wide
 // wavefronts, TY / 64 waves) and a chunk of rows; it marches down the chunk one row per
 // iteration. The 16-point axial (x) stencil reads a per-lane register queue of the last 16 rows
-// of the input field (one new coalesced row load per iteration). The lateral (y) stencil and the
-// transverse corrections read rows staged in a 4-slot LDS ring (current row with an 8-cell halo
-// on both sides); one barrier per iteration. Every map and memory variable is read and written
-// once per cell, coalesced.
+// of the input field. The lateral (y) stencil and the transverse corrections read rows staged
+// in a 4-slot LDS ring (current row with an 8-cell halo on both sides); one barrier per row.
+// Every map and memory variable is read and written once per cell, coalesced, and the operands
+// of row r + 1 are loaded while row r is computed (two register sets, loop unrolled by two), so
+// each workgroup keeps a row of loads in flight across the barrier.
 //
 // Arithmetic order is the per-cell kernels' (fw_kernels_simple.hip), so both paths compute the
 // same sums.
@@ -16,269 +18,356 @@ namespace fw {
 
 hipError_t launch_velocity_simple(const DevSim& s, hipStream_t st);
 hipError_t launch_pressure_simple(const DevSim& s, hipStream_t st);
+hipError_t launch_velocity_3d(const DevSim& s, const LaunchCfg& cfg, hipStream_t st);
+hipError_t launch_pressure_3d(const DevSim& s, const LaunchCfg& cfg, hipStream_t st);
 
 namespace {
 
-constexpr int kRing = 4;  // LDS ring slots (rows i-1, i, i+1 in use, one being refilled)
-constexpr int kChunk2D = 64;  // rows per workgroup
+constexpr int kRing = 4;  // LDS ring slots (rows r-1, r, r+1 in use, one being refilled)
 
-template <int TY>
-struct Ring2D {
-    float row[kRing][TY + 2 * kHalo];
-};
-
-// coefficient set for an axis: from the per-voxel table, or the single set when ndmap == 1
 template <bool UNIFORM>
-__device__ inline const float* coef_row(const float* lds_coef, const int32_t* dcmap, int64_t c, int axis,
-                                        int ndim) {
+__device__ inline const float* coef_row(const float* lds_coef, int dc, int axis) {
     if (UNIFORM) return lds_coef + axis * kNCoef;
-    return lds_coef + ((int64_t)dcmap[c] * ndim + axis) * kNCoef;
-}
-
-__device__ inline void load_coef_table(float* lds_coef, const float* coef, int n) {
-    for (int t = threadIdx.x; t < n; t += blockDim.x) lds_coef[t] = coef[t];
+    return lds_coef + (dc * 2 + axis) * kNCoef;
 }
 
 // ---------------------------------------------------------------------------------------------
 // velocity, 2D
+
+// operands of one row (r) of the velocity update for one lane
+template <bool ISO>
+struct VelRow {
+    static constexpr int NA = ISO ? 1 : 2;
+    float pq;         // p(r + 8, j): next element of the x queue
+    float halo;       // lanes 0..15: p(r + 1, halo column) for LDS row r + 1
+    float rho_next;   // rho(r + 1, j)
+    float rho_right;  // rho(r, j + 1)
+    int dc;
+    float ik[NA], b1[NA], b2[NA], a1[NA], a2[NA];
+    float ps1[2], ps2[2], v[2];
+};
+
 template <int TY, bool UNIFORM, bool ISO>
-__global__ __launch_bounds__(TY) void k_velocity_2d(DevSim s) {
+__global__ __launch_bounds__(TY) void k_velocity_2d(DevSim s, int chunk) {
     extern __shared__ float lds_coef[];
-    __shared__ Ring2D<TY> ring;
+    __shared__ float ring[kRing][TY + 2 * kHalo];
     const int tid = threadIdx.x;
     const int64_t ny = s.ny, nx = s.nx;
     const int64_t j0 = (int64_t)blockIdx.x * TY;
     const int64_t j = j0 + tid;
-    const int64_t i0 = kHalo + (int64_t)blockIdx.y * kChunk2D;
-    const int64_t i1 = min(i0 + kChunk2D, nx - kHalo);
+    const int64_t i0 = kHalo + (int64_t)blockIdx.y * chunk;
+    const int64_t i1 = min(i0 + chunk, nx - kHalo);
     const bool col_ok = j < ny;
     const bool active = j >= kHalo && j < ny - kHalo;
+    const bool halo_lane = tid < 2 * kHalo;
+    const int64_t hj = tid < kHalo ? j0 - kHalo + tid : j0 + TY + (tid - kHalo);
+    const bool halo_ok = halo_lane && hj >= 0 && hj < ny;
+    const int halo_slot = tid < kHalo ? tid : TY + tid;
     const float* __restrict__ p = s.p;
 
-    load_coef_table(lds_coef, s.coef, (UNIFORM ? 1 : s.ndmap) * 2 * kNCoef);
+    const int ncoef = (UNIFORM ? 1 : s.ndmap) * 2 * kNCoef;
+    for (int t = tid; t < ncoef; t += TY) lds_coef[t] = s.coef[t];
 
-    // halo columns of a row: lanes 0..7 take j0-8..j0-1, lanes 8..15 take j0+TY..j0+TY+7
-    auto stage_row = [&](int64_t r, float own) {
-        float* dst = ring.row[r & (kRing - 1)];
-        dst[kHalo + tid] = own;
-        if (tid < 2 * kHalo) {
-            const int64_t hj = tid < kHalo ? j0 - kHalo + tid : j0 + TY + (tid - kHalo);
-            const int slot = tid < kHalo ? tid : TY + tid;
-            dst[slot] = (hj >= 0 && hj < ny) ? p[r * ny + hj] : 0.f;
+    auto load = [&](VelRow<ISO>& R, int64_t r) {
+        const int64_t c = r * ny + j;
+        R.pq = col_ok ? p[c + 8 * ny] : 0.f;
+        R.rho_next = col_ok ? s.rho[c + ny] : 0.f;
+        R.halo = halo_ok ? p[(r + 1) * ny + hj] : 0.f;
+        if (active) {
+            R.rho_right = s.rho[c + 1];
+            R.dc = UNIFORM ? 0 : s.dcmap[c];
+#pragma unroll
+            for (int a = 0; a < VelRow<ISO>::NA; ++a) {
+                R.ik[a] = s.ikv[a][c];
+                R.b1[a] = s.bv[a][0][c];
+                R.b2[a] = s.bv[a][1][c];
+                R.a1[a] = s.av[a][0][c];
+                R.a2[a] = s.av[a][1][c];
+            }
+#pragma unroll
+            for (int a = 0; a < 2; ++a) {
+                R.ps1[a] = s.psi[a][0][c];
+                R.ps2[a] = s.psi[a][1][c];
+                R.v[a] = s.v[a][c];
+            }
         }
     };
 
-    float q[16];  // q[m] = p(i - 7 + m, j)
+    float q[16];  // q[m] = p(r - 7 + m, j)
 #pragma unroll
     for (int m = 0; m < 16; ++m) q[m] = col_ok ? p[(i0 - 7 + m) * ny + j] : 0.f;
-    stage_row(i0 - 1, q[6]);
-    stage_row(i0, q[7]);
+    for (int64_t r = i0 - 1; r <= i0; ++r) {
+        float* dst = ring[r & (kRing - 1)];
+        dst[kHalo + tid] = r < i0 ? q[6] : q[7];
+        if (halo_lane) dst[halo_slot] = halo_ok ? p[r * ny + hj] : 0.f;
+    }
     float rho_i = col_ok ? s.rho[i0 * ny + j] : 0.f;
 
-    for (int64_t i = i0; i < i1; ++i) {
-        if (i > i0) {
+    auto step = [&](int64_t r, const VelRow<ISO>& R) {
+        if (r > i0) {
 #pragma unroll
             for (int m = 0; m < 15; ++m) q[m] = q[m + 1];
-            q[15] = col_ok ? p[(i + 8) * ny + j] : 0.f;
+            q[15] = R.pq;
         }
-        stage_row(i + 1, q[8]);
+        {
+            float* dst = ring[(r + 1) & (kRing - 1)];
+            dst[kHalo + tid] = q[8];
+            if (halo_lane) dst[halo_slot] = R.halo;
+        }
         __syncthreads();
-        const int64_t c = i * ny + j;
-        const float rho_next = col_ok ? s.rho[c + ny] : 0.f;
         if (active) {
-            const float* rm = ring.row[(i - 1) & (kRing - 1)] + kHalo + tid;
-            const float* r0 = ring.row[i & (kRing - 1)] + kHalo + tid;
-            const float* rp = ring.row[(i + 1) & (kRing - 1)] + kHalo + tid;
-            // x: axial from the register queue, transverse from rows i, i+1 at j-1, j+1
-            {
-                const float* w = coef_row<UNIFORM>(lds_coef, s.dcmap, c, 0, 2);
+            const int64_t c = r * ny + j;
+            const float* rm = ring[(r - 1) & (kRing - 1)] + kHalo + tid;
+            const float* r0 = ring[r & (kRing - 1)] + kHalo + tid;
+            const float* rp = ring[(r + 1) & (kRing - 1)] + kHalo + tid;
+            {  // x: axial from the register queue, transverse from rows r, r+1 at j-1, j+1
+                const float* w = coef_row<UNIFORM>(lds_coef, R.dc, 0);
                 float acc = 0.f;
 #pragma unroll
                 for (int m = 1; m <= 8; ++m) acc += w[m - 1] * (q[7 + m] - q[8 - m]);
                 float tr = 0.f;
                 tr += (rp[1] - r0[1]) + (rp[-1] - r0[-1]);
                 const float D = acc + w[8] * tr;
-                const float b1 = s.bv[0][0][c], b2 = s.bv[0][1][c];
-                const float a1 = s.av[0][0][c], a2 = s.av[0][1][c];
-                const float qq1 = b1 * s.psi[0][0][c] + a1 * D;
-                const float qq2 = b2 * s.psi[0][1][c] + a2 * D;
+                const float qq1 = R.b1[0] * R.ps1[0] + R.a1[0] * D;
+                const float qq2 = R.b2[0] * R.ps2[0] + R.a2[0] * D;
                 s.psi[0][0][c] = qq1;
                 s.psi[0][1][c] = qq2;
-                const float rho_bar = 0.5f * (rho_i + rho_next);
-                s.v[0][c] -= s.dt / rho_bar * (D * s.ikv[0][c] + (qq1 + qq2));
+                const float rho_bar = 0.5f * (rho_i + R.rho_next);
+                s.v[0][c] = R.v[0] - s.dt / rho_bar * (D * R.ik[0] + (qq1 + qq2));
             }
-            // y: axial from row i in LDS, transverse from rows i-1, i+1 at j, j+1
-            {
-                const float* w = coef_row<UNIFORM>(lds_coef, s.dcmap, c, 1, 2);
+            {  // y: axial from row r in LDS, transverse from rows r-1, r+1 at j, j+1
+                constexpr int ax = ISO ? 0 : 1;
+                const float* w = coef_row<UNIFORM>(lds_coef, R.dc, 1);
                 float acc = 0.f;
 #pragma unroll
                 for (int m = 1; m <= 8; ++m) acc += w[m - 1] * (r0[m] - r0[-(m - 1)]);
                 float tr = 0.f;
                 tr += (rp[1] - rp[0]) + (rm[1] - rm[0]);
                 const float D = acc + w[8] * tr;
-                const int ax = ISO ? 0 : 1;
-                const float b1 = s.bv[ax][0][c], b2 = s.bv[ax][1][c];
-                const float a1 = s.av[ax][0][c], a2 = s.av[ax][1][c];
-                const float qq1 = b1 * s.psi[1][0][c] + a1 * D;
-                const float qq2 = b2 * s.psi[1][1][c] + a2 * D;
+                const float qq1 = R.b1[ax] * R.ps1[1] + R.a1[ax] * D;
+                const float qq2 = R.b2[ax] * R.ps2[1] + R.a2[ax] * D;
                 s.psi[1][0][c] = qq1;
                 s.psi[1][1][c] = qq2;
-                const float rho_bar = 0.5f * (rho_i + s.rho[c + 1]);
-                const float ik = ISO ? s.ikv[0][c] : s.ikv[1][c];
-                s.v[1][c] -= s.dt / rho_bar * (D * ik + (qq1 + qq2));
+                const float rho_bar = 0.5f * (rho_i + R.rho_right);
+                s.v[1][c] = R.v[1] - s.dt / rho_bar * (D * R.ik[ax] + (qq1 + qq2));
             }
         }
-        rho_i = rho_next;
+        rho_i = R.rho_next;
+    };
+
+    VelRow<ISO> A, B;
+    load(A, i0);
+    for (int64_t r = i0; r < i1; r += 2) {
+        if (r + 1 < i1) load(B, r + 1);
+        step(r, A);
+        if (r + 1 >= i1) break;
+        if (r + 2 < i1) load(A, r + 2);
+        step(r + 1, B);
     }
 }
 
 // ---------------------------------------------------------------------------------------------
 // pressure, 2D
+
+template <bool ISO>
+struct PreRow {
+    static constexpr int NA = ISO ? 1 : 2;
+    float vxq;      // vx(r + 7, j): next element of the x queue
+    float vy_own;   // vy(r + 1, j) for LDS row r + 1
+    float vy_halo;  // lanes 0..15: vy(r + 1, halo column)
+    float vx_halo;  // lanes 0..1: vx(r, j0 - 1) / vx(r, j0 + TY)
+    float K, p;
+    int dc;
+    float ik[NA], b1[NA], b2[NA], a1[NA], a2[NA];
+    float ph1[2], ph2[2];
+};
+
 template <int TY, bool UNIFORM, bool ISO>
-__global__ __launch_bounds__(TY) void k_pressure_2d(DevSim s) {
+__global__ __launch_bounds__(TY) void k_pressure_2d(DevSim s, int chunk) {
     extern __shared__ float lds_coef[];
-    __shared__ Ring2D<TY> ring_y;  // vy rows i-1, i, i+1 (row i with the 8-cell halo)
-    __shared__ float ring_x[kRing][TY + 2];  // vx rows i-1, i with one halo column per side
+    __shared__ float ring_y[kRing][TY + 2 * kHalo];  // vy rows r-1, r, r+1 (8-cell halo)
+    __shared__ float ring_x[kRing][TY + 2];          // vx rows r-1, r (1-cell halo)
     const int tid = threadIdx.x;
     const int64_t ny = s.ny, nx = s.nx;
     const int64_t j0 = (int64_t)blockIdx.x * TY;
     const int64_t j = j0 + tid;
-    const int64_t i0 = kHalo + (int64_t)blockIdx.y * kChunk2D;
-    const int64_t i1 = min(i0 + kChunk2D, nx - kHalo);
+    const int64_t i0 = kHalo + (int64_t)blockIdx.y * chunk;
+    const int64_t i1 = min(i0 + chunk, nx - kHalo);
     const bool col_ok = j < ny;
     const bool active = j >= kHalo && j < ny - kHalo;
+    const bool halo_lane = tid < 2 * kHalo;
+    const int64_t hj = tid < kHalo ? j0 - kHalo + tid : j0 + TY + (tid - kHalo);
+    const bool halo_ok = halo_lane && hj >= 0 && hj < ny;
+    const int halo_slot = tid < kHalo ? tid : TY + tid;
+    const bool xh_lane = tid < 2;
+    const int64_t xhj = tid == 0 ? j0 - 1 : j0 + TY;
+    const bool xh_ok = xh_lane && xhj >= 0 && xhj < ny;
+    const int xh_slot = tid == 0 ? 0 : TY + 1;
     const float* __restrict__ vx = s.v[0];
     const float* __restrict__ vy = s.v[1];
 
-    load_coef_table(lds_coef, s.coef, (UNIFORM ? 1 : s.ndmap) * 2 * kNCoef);
+    const int ncoef = (UNIFORM ? 1 : s.ndmap) * 2 * kNCoef;
+    for (int t = tid; t < ncoef; t += TY) lds_coef[t] = s.coef[t];
 
-    auto stage_vy = [&](int64_t r, float own) {
-        float* dst = ring_y.row[r & (kRing - 1)];
-        dst[kHalo + tid] = own;
-        if (tid < 2 * kHalo) {
-            const int64_t hj = tid < kHalo ? j0 - kHalo + tid : j0 + TY + (tid - kHalo);
-            const int slot = tid < kHalo ? tid : TY + tid;
-            dst[slot] = (hj >= 0 && hj < ny) ? vy[r * ny + hj] : 0.f;
+    auto load = [&](PreRow<ISO>& R, int64_t r) {
+        const int64_t c = r * ny + j;
+        R.vxq = col_ok ? vx[c + 7 * ny] : 0.f;
+        R.vy_own = col_ok ? vy[c + ny] : 0.f;
+        R.vy_halo = halo_ok ? vy[(r + 1) * ny + hj] : 0.f;
+        R.vx_halo = xh_ok ? vx[r * ny + xhj] : 0.f;
+        if (active) {
+            R.K = s.K[c];
+            R.p = s.p[c];
+            R.dc = UNIFORM ? 0 : s.dcmap[c];
+#pragma unroll
+            for (int a = 0; a < PreRow<ISO>::NA; ++a) {
+                R.ik[a] = s.ikp[a][c];
+                R.b1[a] = s.bp[a][0][c];
+                R.b2[a] = s.bp[a][1][c];
+                R.a1[a] = s.ap[a][0][c];
+                R.a2[a] = s.ap[a][1][c];
+            }
+#pragma unroll
+            for (int a = 0; a < 2; ++a) {
+                R.ph1[a] = s.phi[a][0][c];
+                R.ph2[a] = s.phi[a][1][c];
+            }
         }
     };
-    auto stage_vx = [&](int64_t r, float own) {
-        float* dst = ring_x[r & (kRing - 1)];
-        dst[1 + tid] = own;
-        if (tid < 2) {
-            const int64_t hj = tid == 0 ? j0 - 1 : j0 + TY;
-            dst[tid == 0 ? 0 : TY + 1] = (hj >= 0 && hj < ny) ? vx[r * ny + hj] : 0.f;
-        }
-    };
 
-    float q[16];  // q[m] = vx(i - 8 + m, j)
+    float q[16];  // q[m] = vx(r - 8 + m, j)
 #pragma unroll
     for (int m = 0; m < 16; ++m) q[m] = col_ok ? vx[(i0 - 8 + m) * ny + j] : 0.f;
-    float vy_next = col_ok ? vy[i0 * ny + j] : 0.f;
-    stage_vy(i0 - 1, col_ok ? vy[(i0 - 1) * ny + j] : 0.f);
-    stage_vy(i0, vy_next);
-    stage_vx(i0 - 1, q[7]);
+    for (int64_t r = i0 - 1; r <= i0; ++r) {
+        float* dst = ring_y[r & (kRing - 1)];
+        dst[kHalo + tid] = col_ok ? vy[r * ny + j] : 0.f;
+        if (halo_lane) dst[halo_slot] = halo_ok ? vy[r * ny + hj] : 0.f;
+    }
+    {
+        float* dst = ring_x[(i0 - 1) & (kRing - 1)];
+        dst[1 + tid] = q[7];
+        if (xh_lane) dst[xh_slot] = xh_ok ? vx[(i0 - 1) * ny + xhj] : 0.f;
+    }
 
-    for (int64_t i = i0; i < i1; ++i) {
-        if (i > i0) {
+    auto step = [&](int64_t r, const PreRow<ISO>& R) {
+        if (r > i0) {
 #pragma unroll
             for (int m = 0; m < 15; ++m) q[m] = q[m + 1];
-            q[15] = col_ok ? vx[(i + 7) * ny + j] : 0.f;
+            q[15] = R.vxq;
         }
-        stage_vx(i, q[8]);
-        stage_vy(i + 1, col_ok ? vy[(i + 1) * ny + j] : 0.f);
+        {
+            float* dx_ = ring_x[r & (kRing - 1)];
+            dx_[1 + tid] = q[8];
+            if (xh_lane) dx_[xh_slot] = R.vx_halo;
+            float* dy_ = ring_y[(r + 1) & (kRing - 1)];
+            dy_[kHalo + tid] = R.vy_own;
+            if (halo_lane) dy_[halo_slot] = R.vy_halo;
+        }
         __syncthreads();
-        const int64_t c = i * ny + j;
         if (active) {
-            const float* xm = ring_x[(i - 1) & (kRing - 1)] + 1 + tid;
-            const float* x0 = ring_x[i & (kRing - 1)] + 1 + tid;
-            const float* ym = ring_y.row[(i - 1) & (kRing - 1)] + kHalo + tid;
-            const float* y0 = ring_y.row[i & (kRing - 1)] + kHalo + tid;
-            const float* yp = ring_y.row[(i + 1) & (kRing - 1)] + kHalo + tid;
+            const int64_t c = r * ny + j;
+            const float* xm = ring_x[(r - 1) & (kRing - 1)] + 1 + tid;
+            const float* x0 = ring_x[r & (kRing - 1)] + 1 + tid;
+            const float* ym = ring_y[(r - 1) & (kRing - 1)] + kHalo + tid;
+            const float* y0 = ring_y[r & (kRing - 1)] + kHalo + tid;
+            const float* yp = ring_y[(r + 1) & (kRing - 1)] + kHalo + tid;
             float div = 0.f;
             {
-                const float* w = coef_row<UNIFORM>(lds_coef, s.dcmap, c, 0, 2);
+                const float* w = coef_row<UNIFORM>(lds_coef, R.dc, 0);
                 float acc = 0.f;
 #pragma unroll
                 for (int m = 1; m <= 8; ++m) acc += w[m - 1] * (q[7 + m] - q[8 - m]);
                 float tr = 0.f;
                 tr += (x0[1] - xm[1]) + (x0[-1] - xm[-1]);
                 const float D = acc + w[8] * tr;
-                const float qq1 = s.bp[0][0][c] * s.phi[0][0][c] + s.ap[0][0][c] * D;
-                const float qq2 = s.bp[0][1][c] * s.phi[0][1][c] + s.ap[0][1][c] * D;
+                const float qq1 = R.b1[0] * R.ph1[0] + R.a1[0] * D;
+                const float qq2 = R.b2[0] * R.ph2[0] + R.a2[0] * D;
                 s.phi[0][0][c] = qq1;
                 s.phi[0][1][c] = qq2;
-                div += D * s.ikp[0][c] + (qq1 + qq2);
+                div += D * R.ik[0] + (qq1 + qq2);
             }
             {
-                const float* w = coef_row<UNIFORM>(lds_coef, s.dcmap, c, 1, 2);
+                constexpr int ax = ISO ? 0 : 1;
+                const float* w = coef_row<UNIFORM>(lds_coef, R.dc, 1);
                 float acc = 0.f;
 #pragma unroll
                 for (int m = 1; m <= 8; ++m) acc += w[m - 1] * (y0[m - 1] - y0[-m]);
                 float tr = 0.f;
                 tr += (yp[0] - yp[-1]) + (ym[0] - ym[-1]);
                 const float D = acc + w[8] * tr;
-                const int ax = ISO ? 0 : 1;
-                const float qq1 = s.bp[ax][0][c] * s.phi[1][0][c] + s.ap[ax][0][c] * D;
-                const float qq2 = s.bp[ax][1][c] * s.phi[1][1][c] + s.ap[ax][1][c] * D;
+                const float qq1 = R.b1[ax] * R.ph1[1] + R.a1[ax] * D;
+                const float qq2 = R.b2[ax] * R.ph2[1] + R.a2[ax] * D;
                 s.phi[1][0][c] = qq1;
                 s.phi[1][1][c] = qq2;
-                const float ik = ISO ? s.ikp[0][c] : s.ikp[1][c];
-                div += D * ik + (qq1 + qq2);
+                div += D * R.ik[ax] + (qq1 + qq2);
             }
-            s.p[c] -= s.dt * s.K[c] * div;
+            s.p[c] = R.p - s.dt * R.K * div;
         }
+    };
+
+    PreRow<ISO> A, B;
+    load(A, i0);
+    for (int64_t r = i0; r < i1; r += 2) {
+        if (r + 1 < i1) load(B, r + 1);
+        step(r, A);
+        if (r + 1 >= i1) break;
+        if (r + 2 < i1) load(A, r + 2);
+        step(r + 1, B);
     }
 }
 
-int coef_lds_bytes(const DevSim& s, bool uniform) { return (uniform ? 1 : s.ndmap) * s.ndim * kNCoef * 4; }
+template <int TY>
+hipError_t launch_2d(const DevSim& s, int chunk, bool velocity, hipStream_t st) {
+    const bool u = s.ndmap == 1;
+    const int lds = (u ? 1 : s.ndmap) * 2 * kNCoef * 4;
+    const int64_t rows = s.nx - 2 * kHalo;
+    const dim3 g((unsigned)((s.ny + TY - 1) / TY), (unsigned)((rows + chunk - 1) / chunk), 1);
+    const dim3 b(TY);
+    if (velocity) {
+        if (u && s.isotropic) hipLaunchKernelGGL((k_velocity_2d<TY, true, true>), g, b, lds, st, s, chunk);
+        else if (u) hipLaunchKernelGGL((k_velocity_2d<TY, true, false>), g, b, lds, st, s, chunk);
+        else if (s.isotropic) hipLaunchKernelGGL((k_velocity_2d<TY, false, true>), g, b, lds, st, s, chunk);
+        else hipLaunchKernelGGL((k_velocity_2d<TY, false, false>), g, b, lds, st, s, chunk);
+    } else {
+        if (u && s.isotropic) hipLaunchKernelGGL((k_pressure_2d<TY, true, true>), g, b, lds, st, s, chunk);
+        else if (u) hipLaunchKernelGGL((k_pressure_2d<TY, true, false>), g, b, lds, st, s, chunk);
+        else if (s.isotropic) hipLaunchKernelGGL((k_pressure_2d<TY, false, true>), g, b, lds, st, s, chunk);
+        else hipLaunchKernelGGL((k_pressure_2d<TY, false, false>), g, b, lds, st, s, chunk);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_2d_any(const DevSim& s, const LaunchCfg& cfg, bool velocity, hipStream_t st) {
+    const int chunk = cfg.chunk < 2 ? 2 : cfg.chunk;
+    switch (cfg.width) {
+        case 64: return launch_2d<64>(s, chunk, velocity, st);
+        case 256: return launch_2d<256>(s, chunk, velocity, st);
+        default: return launch_2d<128>(s, chunk, velocity, st);
+    }
+}
 
 }  // namespace
 
-// the per-voxel coefficient table goes to LDS; beyond this many sets the per-cell kernels run
-constexpr int kMaxLdsCoefSets = 1024;
+// the per-voxel coefficient table goes to LDS; beyond this many sets only the per-cell
+// kernels run (the 3D pressure kernel's plane tiles already take 48 KB of LDS)
+constexpr int kMaxLdsCoefSets2D = 1024;
+constexpr int kMaxLdsCoefSets3D = 128;
 
 bool streaming_supported(const DevSim& s) {
-    if (s.ndim != 2) return false;
-    if (s.ndmap > kMaxLdsCoefSets) return false;
-    return s.nx >= 2 * kHalo + 1 && s.ny >= 2 * kHalo + 1;
+    if (s.ndim == 3) return s.ndmap <= kMaxLdsCoefSets3D;
+    return s.ndmap <= kMaxLdsCoefSets2D;
 }
 
-template <bool U, bool I>
-static void launch_v2(const DevSim& s, hipStream_t st, dim3 grid, int lds) {
-    hipLaunchKernelGGL((k_velocity_2d<256, U, I>), grid, dim3(256), lds, st, s);
-}
-template <bool U, bool I>
-static void launch_p2(const DevSim& s, hipStream_t st, dim3 grid, int lds) {
-    hipLaunchKernelGGL((k_pressure_2d<256, U, I>), grid, dim3(256), lds, st, s);
+hipError_t launch_velocity(const DevSim& s, const LaunchCfg& cfg, hipStream_t st) {
+    if (cfg.variant == 0 || !streaming_supported(s)) return launch_velocity_simple(s, st);
+    if (s.ndim == 3) return launch_velocity_3d(s, cfg, st);
+    return launch_2d_any(s, cfg, true, st);
 }
 
-static dim3 grid_2d(const DevSim& s) {
-    const int64_t rows = s.nx - 2 * kHalo;
-    return dim3((unsigned)((s.ny + 255) / 256), (unsigned)((rows + kChunk2D - 1) / kChunk2D), 1);
-}
-
-hipError_t launch_velocity(const DevSim& s, hipStream_t st) {
-    if (!streaming_supported(s)) return launch_velocity_simple(s, st);
-    const bool u = s.ndmap == 1;
-    const int lds = coef_lds_bytes(s, u);
-    const dim3 g = grid_2d(s);
-    if (u && s.isotropic) launch_v2<true, true>(s, st, g, lds);
-    else if (u) launch_v2<true, false>(s, st, g, lds);
-    else if (s.isotropic) launch_v2<false, true>(s, st, g, lds);
-    else launch_v2<false, false>(s, st, g, lds);
-    return hipGetLastError();
-}
-
-hipError_t launch_pressure(const DevSim& s, hipStream_t st) {
-    if (!streaming_supported(s)) return launch_pressure_simple(s, st);
-    const bool u = s.ndmap == 1;
-    const int lds = coef_lds_bytes(s, u);
-    const dim3 g = grid_2d(s);
-    if (u && s.isotropic) launch_p2<true, true>(s, st, g, lds);
-    else if (u) launch_p2<true, false>(s, st, g, lds);
-    else if (s.isotropic) launch_p2<false, true>(s, st, g, lds);
-    else launch_p2<false, false>(s, st, g, lds);
-    return hipGetLastError();
+hipError_t launch_pressure(const DevSim& s, const LaunchCfg& cfg, hipStream_t st) {
+    if (cfg.variant == 0 || !streaming_supported(s)) return launch_pressure_simple(s, st);
+    if (s.ndim == 3) return launch_pressure_3d(s, cfg, st);
+    return launch_2d_any(s, cfg, false, st);
 }
 
 }  // namespace fw

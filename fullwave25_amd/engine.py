@@ -31,7 +31,7 @@ EXPORTED_SYMBOLS = (
     "fw_device_copy_bandwidth",
     "fw_sim_create", "fw_sim_upload", "fw_sim_run", "fw_sim_reset", "fw_sim_step_count",
     "fw_sim_record_count", "fw_sim_download_sensors", "fw_sim_download_field",
-    "fw_sim_set_field", "fw_sim_run_timed", "fw_sim_destroy",
+    "fw_sim_set_field", "fw_sim_kernel_config", "fw_sim_run_timed", "fw_sim_destroy",
 )
 
 
@@ -87,6 +87,7 @@ def load_library(path: Path | None = None) -> ctypes.CDLL:
         lib.fw_sim_download_sensors.argtypes = [vp, vp, sz]
         lib.fw_sim_download_field.argtypes = [vp, ctypes.c_char_p, vp, sz]
         lib.fw_sim_set_field.argtypes = [vp, ctypes.c_char_p, vp, sz]
+        lib.fw_sim_kernel_config.argtypes = [vp, ctypes.POINTER(i32), ctypes.POINTER(i32), ctypes.POINTER(i32)]
         lib.fw_sim_run_timed.argtypes = [vp, i64, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(i32)]
         lib.fw_sim_destroy.argtypes = [vp]
         if lib.fw_abi_version() != 1:
@@ -172,6 +173,12 @@ class HipSimulation:
         nk = ctypes.c_int32(0)
         _check(self.lib, self.lib.fw_sim_run_timed(self._h, int(n_steps), ms, ctypes.byref(nk)))
         return {KERNEL_KINDS[i]: ms[i] for i in range(nk.value)}
+
+    def kernel_config(self) -> dict:
+        """Kernel family/shape this handle runs (chosen on first use)."""
+        v, w, c = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        _check(self.lib, self.lib.fw_sim_kernel_config(self._h, ctypes.byref(v), ctypes.byref(w), ctypes.byref(c)))
+        return {"variant": "percell" if v.value == 0 else "stream", "width": w.value, "chunk": c.value}
 
     def reset(self) -> None:
         _check(self.lib, self.lib.fw_sim_reset(self._h))

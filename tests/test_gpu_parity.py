@@ -150,17 +150,49 @@ def test_run_past_nt_raises():
             sim.run(1)
 
 
-def test_streaming_kernels_match_per_cell_kernels(monkeypatch):
-    """The two GPU implementations (x-marching and per-cell) on the same inputs."""
+STREAM_SHAPES_2D = ["stream:64:2", "stream:128:7", "stream:256:64", "stream:128:33"]
+
+
+@pytest.mark.parametrize("shape", STREAM_SHAPES_2D)
+def test_streaming_kernels_match_per_cell_kernels(monkeypatch, shape):
+    """The two GPU implementations (x-marching at several tile shapes, and per-cell) on the
+    same inputs: 2D isotropic with air, 2D per-axis maps, 2D uniform sound speed."""
     for inp in (workloads.random_medium_2d(70, 300, 120, seed=31, air=5).inputs,
                 workloads.random_medium_2d(150, 90, 100, seed=32, isotropic=False).inputs,
                 workloads.plane_wave_2d(60, 280, 150, n_pml=10, sensor_depth=40).inputs):
+        print(f"{shape}: {inp.shape}", flush=True)
+        monkeypatch.setenv("FULLWAVE_HIP_KERNELS", shape)
         a, pa = _gpu(inp)
         monkeypatch.setenv("FULLWAVE_HIP_KERNELS", "percell")
         b, pb = _gpu(inp)
         monkeypatch.delenv("FULLWAVE_HIP_KERNELS")
         assert rel_l2(a, b) <= 1e-6
         assert rel_l2(pa, pb) <= 1e-6
+
+
+@pytest.mark.parametrize("shape", ["stream:64:2", "stream:64:16", "stream:64:9"])
+def test_streaming_3d_matches_per_cell(monkeypatch, shape):
+    for inp in (workloads.random_medium_3d(40, 30, 150, 60, seed=33, air=4).inputs,
+                workloads.random_medium_3d(26, 70, 36, 50, seed=34, isotropic=False).inputs):
+        print(f"{shape}: {inp.shape}", flush=True)
+        monkeypatch.setenv("FULLWAVE_HIP_KERNELS", shape)
+        a, pa = _gpu(inp)
+        monkeypatch.setenv("FULLWAVE_HIP_KERNELS", "percell")
+        b, pb = _gpu(inp)
+        monkeypatch.delenv("FULLWAVE_HIP_KERNELS")
+        assert rel_l2(a, b) <= 1e-6
+        assert rel_l2(pa, pb) <= 1e-6
+
+
+def test_autotuned_config_is_reported():
+    inp = workloads.random_medium_2d(80, 200, 20, seed=35).inputs
+    with engine.HipSimulation(inp) as sim:
+        cfg = sim.kernel_config()
+        assert cfg["variant"] in ("percell", "stream")
+        sim.run(20)
+        got = sim.sensors().astype(np.float64)
+    want, _ = _ref(inp)
+    assert rel_l2(got, want) <= TOL
 
 
 def test_solver_end_to_end(tmp_path):
