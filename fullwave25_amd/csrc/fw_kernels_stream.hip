@@ -348,13 +348,12 @@ hipError_t launch_2d_any(const DevSim& s, const LaunchCfg& cfg, bool velocity, h
 
 }  // namespace
 
-// the per-voxel coefficient table goes to LDS; beyond this many sets only the per-cell
-// kernels run (the 3D pressure kernel's plane tiles already take 48 KB of LDS)
+// the per-voxel coefficient table goes to LDS; beyond this many sets the 2D grids run the
+// per-cell kernels
 constexpr int kMaxLdsCoefSets2D = 1024;
-constexpr int kMaxLdsCoefSets3D = 128;
 
 bool streaming_supported(const DevSim& s) {
-    if (s.ndim == 3) return s.ndmap <= kMaxLdsCoefSets3D;
+    if (s.ndim == 3) return true;  // large 3D tables are read from global memory
     return s.ndmap <= kMaxLdsCoefSets2D;
 }
 

@@ -40,10 +40,21 @@ struct Tile {
     }
 };
 
+// coefficient tables of up to kLdsSets3D sets are copied to LDS; larger ones are read from
+// global memory (a few KB, cache resident)
+constexpr int kLdsSets3D = 128;
+
 template <bool UNIFORM>
 __device__ inline const float* coef3(const float* lds_coef, const int32_t* dcmap, int64_t c, int axis) {
     if (UNIFORM) return lds_coef + axis * kNCoef;
     return lds_coef + ((int64_t)dcmap[c] * 3 + axis) * kNCoef;
+}
+
+__device__ inline const float* coef_base_3d(const DevSim& s, float* lds_coef, bool uniform) {
+    const int nsets = uniform ? 1 : s.ndmap;
+    if (nsets > kLdsSets3D) return s.coef;
+    for (int t = threadIdx.x; t < nsets * 3 * kNCoef; t += NT3) lds_coef[t] = s.coef[t];
+    return lds_coef;
 }
 
 template <bool UNIFORM, bool ISO>
@@ -59,8 +70,7 @@ __global__ __launch_bounds__(NT3) void k_velocity_3d(DevSim s, int chunk) {
     const bool active = j >= kHalo && j < s.ny - kHalo && k >= kHalo && k < s.nz - kHalo;
     const int64_t sx = s.sx, sy = s.sy;
     const float* __restrict__ p = s.p;
-    const int ncoef = (UNIFORM ? 1 : s.ndmap) * 3 * kNCoef;
-    for (int t = threadIdx.x; t < ncoef; t += NT3) lds_coef[t] = s.coef[t];
+    const float* coefs = coef_base_3d(s, lds_coef, UNIFORM);
 
     const int64_t col = j * sy + k;
     float q[16];  // q[m] = p(i - 7 + m, j, k)
@@ -87,7 +97,7 @@ __global__ __launch_bounds__(NT3) void k_velocity_3d(DevSim s, int chunk) {
             auto A = [&](const float* pl, int dy, int dz) { return pl[Tile<kHalo, kHalo>::at(ty, tz, dy, dz)]; };
             // x
             {
-                const float* w = coef3<UNIFORM>(lds_coef, s.dcmap, c, 0);
+                const float* w = coef3<UNIFORM>(coefs, s.dcmap, c, 0);
                 float acc = 0.f;
 #pragma unroll
                 for (int m = 1; m <= 8; ++m) acc += w[m - 1] * (q[7 + m] - q[8 - m]);
@@ -104,7 +114,7 @@ __global__ __launch_bounds__(NT3) void k_velocity_3d(DevSim s, int chunk) {
             }
             // y
             {
-                const float* w = coef3<UNIFORM>(lds_coef, s.dcmap, c, 1);
+                const float* w = coef3<UNIFORM>(coefs, s.dcmap, c, 1);
                 float acc = 0.f;
 #pragma unroll
                 for (int m = 1; m <= 8; ++m) acc += w[m - 1] * (A(p0, m, 0) - A(p0, -(m - 1), 0));
@@ -122,7 +132,7 @@ __global__ __launch_bounds__(NT3) void k_velocity_3d(DevSim s, int chunk) {
             }
             // z
             {
-                const float* w = coef3<UNIFORM>(lds_coef, s.dcmap, c, 2);
+                const float* w = coef3<UNIFORM>(coefs, s.dcmap, c, 2);
                 float acc = 0.f;
 #pragma unroll
                 for (int m = 1; m <= 8; ++m) acc += w[m - 1] * (A(p0, 0, m) - A(p0, 0, -(m - 1)));
@@ -160,8 +170,7 @@ __global__ __launch_bounds__(NT3) void k_pressure_3d(DevSim s, int chunk) {
     const float* __restrict__ vx = s.v[0];
     const float* __restrict__ vy = s.v[1];
     const float* __restrict__ vz = s.v[2];
-    const int ncoef = (UNIFORM ? 1 : s.ndmap) * 3 * kNCoef;
-    for (int t = threadIdx.x; t < ncoef; t += NT3) lds_coef[t] = s.coef[t];
+    const float* coefs = coef_base_3d(s, lds_coef, UNIFORM);
 
     const int64_t col = j * sy + k;
     float q[16];  // q[m] = vx(i - 8 + m, j, k)
@@ -198,7 +207,7 @@ __global__ __launch_bounds__(NT3) void k_pressure_3d(DevSim s, int chunk) {
             auto Z = [&](const float* pl, int dy, int dz) { return pl[Tile<1, kHalo>::at(ty, tz, dy, dz)]; };
             float div = 0.f;
             {
-                const float* w = coef3<UNIFORM>(lds_coef, s.dcmap, c, 0);
+                const float* w = coef3<UNIFORM>(coefs, s.dcmap, c, 0);
                 float acc = 0.f;
 #pragma unroll
                 for (int m = 1; m <= 8; ++m) acc += w[m - 1] * (q[7 + m] - q[8 - m]);
@@ -213,7 +222,7 @@ __global__ __launch_bounds__(NT3) void k_pressure_3d(DevSim s, int chunk) {
                 div += D * s.ikp[0][c] + (qq1 + qq2);
             }
             {
-                const float* w = coef3<UNIFORM>(lds_coef, s.dcmap, c, 1);
+                const float* w = coef3<UNIFORM>(coefs, s.dcmap, c, 1);
                 float acc = 0.f;
 #pragma unroll
                 for (int m = 1; m <= 8; ++m) acc += w[m - 1] * (Y(y0, m - 1, 0) - Y(y0, -m, 0));
@@ -229,7 +238,7 @@ __global__ __launch_bounds__(NT3) void k_pressure_3d(DevSim s, int chunk) {
                 div += D * s.ikp[ax][c] + (qq1 + qq2);
             }
             {
-                const float* w = coef3<UNIFORM>(lds_coef, s.dcmap, c, 2);
+                const float* w = coef3<UNIFORM>(coefs, s.dcmap, c, 2);
                 float acc = 0.f;
 #pragma unroll
                 for (int m = 1; m <= 8; ++m) acc += w[m - 1] * (Z(z0, 0, m - 1) - Z(z0, 0, -m));
@@ -256,9 +265,14 @@ dim3 grid_3d(const DevSim& s, int chunk) {
 
 }  // namespace
 
+static int coef_lds_bytes_3d(const DevSim& s) {
+    const int nsets = s.ndmap == 1 ? 1 : s.ndmap;
+    return nsets > kLdsSets3D ? 0 : nsets * 3 * kNCoef * 4;
+}
+
 hipError_t launch_velocity_3d(const DevSim& s, const LaunchCfg& cfg, hipStream_t st) {
     const bool u = s.ndmap == 1;
-    const int lds = (u ? 1 : s.ndmap) * 3 * kNCoef * 4;
+    const int lds = coef_lds_bytes_3d(s);
     const int chunk = cfg.chunk < 1 ? 1 : cfg.chunk;
     const dim3 g = grid_3d(s, chunk);
     if (u && s.isotropic) hipLaunchKernelGGL((k_velocity_3d<true, true>), g, dim3(NT3), lds, st, s, chunk);
@@ -270,7 +284,7 @@ hipError_t launch_velocity_3d(const DevSim& s, const LaunchCfg& cfg, hipStream_t
 
 hipError_t launch_pressure_3d(const DevSim& s, const LaunchCfg& cfg, hipStream_t st) {
     const bool u = s.ndmap == 1;
-    const int lds = (u ? 1 : s.ndmap) * 3 * kNCoef * 4;
+    const int lds = coef_lds_bytes_3d(s);
     const int chunk = cfg.chunk < 1 ? 1 : cfg.chunk;
     const dim3 g = grid_3d(s, chunk);
     if (u && s.isotropic) hipLaunchKernelGGL((k_pressure_3d<true, true>), g, dim3(NT3), lds, st, s, chunk);

@@ -170,10 +170,24 @@ def test_streaming_kernels_match_per_cell_kernels(monkeypatch, shape):
         assert rel_l2(pa, pb) <= 1e-6
 
 
+def _three_speed_3d(shape, nt, seed):
+    """3D inputs whose sound speed takes three values (61 coefficient sets, LDS-resident)."""
+    rng = np.random.default_rng(seed)
+    c = rng.choice([1500.0, 1540.0, 1560.0], size=shape)
+    rho = 950 + 100 * rng.random(shape)
+    relax = {"v": {"kappa": 1.01, "d": (2e5, 4e5), "alpha": (3e6, 2e7)},
+             "p": {"kappa": 0.99, "d": (1e5, 3e5), "alpha": (5e6, 4e7)}}
+    src = [[shape[0] // 2, shape[1] // 2, shape[2] // 2], [12, 14, 20]]
+    sig = np.sin(np.arange(nt) * 0.3)[None, :] * np.array([[1.0], [0.5]])
+    return raw_inputs(shape, c=c, rho=rho, nt=nt, relax=relax, icc=src, icmat=sig,
+                      outc=[[20, 20, 20], [15, 30, 40], [30, 22, 11]])
+
+
 @pytest.mark.parametrize("shape", ["stream:64:2", "stream:64:16", "stream:64:9"])
 def test_streaming_3d_matches_per_cell(monkeypatch, shape):
     for inp in (workloads.random_medium_3d(40, 30, 150, 60, seed=33, air=4).inputs,
-                workloads.random_medium_3d(26, 70, 36, 50, seed=34, isotropic=False).inputs):
+                workloads.random_medium_3d(26, 70, 36, 50, seed=34, isotropic=False).inputs,
+                _three_speed_3d((44, 50, 90), 60, seed=36)):
         print(f"{shape}: {inp.shape}", flush=True)
         monkeypatch.setenv("FULLWAVE_HIP_KERNELS", shape)
         a, pa = _gpu(inp)
