@@ -107,7 +107,8 @@ def main() -> int:
     inp = wl.inputs
     print(f"# workload {wl.name} {inp.shape} nt={inp.nt} built in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
     if args.sweep:
-        shapes = ["percell"] + [f"stream:{w}:{c}" for w in (64, 128, 256) for c in (8, 16, 32, 64)]
+        shapes = ["percell"] + [f"stream:{w}:{c}" for w in (128, 256) for c in (8, 16, 32)]
+        shapes += [f"tile:{c}" for c in (8, 16, 32)]
         if inp.ndim == 3:
             shapes = ["percell"] + [f"stream:64:{c}" for c in (8, 16, 32, 64)]
         bpc = algorithmic_bytes_per_cell(inp.ndim, inp.isotropic)

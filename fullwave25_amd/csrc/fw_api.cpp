@@ -176,6 +176,14 @@ int choose_kernels(fw_sim* s) {
         fw::LaunchCfg c = def;
         if (!strncmp(env, "percell", 7)) {
             c.variant = 0;
+        } else if (!strncmp(env, "tile", 4)) {
+            if (!fw::tile2d_supported(s->ds)) return fail(FW_ERR_UNSUPPORTED, "FULLWAVE_HIP_KERNELS=tile needs a 2D grid with ny %% 4 == 0");
+            c.variant = 2;
+            c.width = 256;
+            int ch = 16;
+            if (sscanf(env, "tile:%d", &ch) == 1) c.chunk = ch;
+            else c.chunk = 16;
+            if (c.chunk != 8 && c.chunk != 16 && c.chunk != 32) return fail(FW_ERR_INVALID, "FULLWAVE_HIP_KERNELS: tile rows must be 8, 16 or 32");
         } else if (!strncmp(env, "stream", 6)) {
             if (!stream_ok) return fail(FW_ERR_UNSUPPORTED, "FULLWAVE_HIP_KERNELS=stream: this grid needs the per-cell kernels");
             c.variant = 1;
@@ -187,7 +195,7 @@ int choose_kernels(fw_sim* s) {
                 return fail(FW_ERR_INVALID, "FULLWAVE_HIP_KERNELS: width must be 64, 128 or 256");
             if (c.chunk < 2 || c.chunk > 4096) return fail(FW_ERR_INVALID, "FULLWAVE_HIP_KERNELS: chunk out of range");
         } else {
-            return fail(FW_ERR_INVALID, "FULLWAVE_HIP_KERNELS must be 'percell' or 'stream[:width[:chunk]]'");
+            return fail(FW_ERR_INVALID, "FULLWAVE_HIP_KERNELS must be 'percell', 'stream[:width[:chunk]]' or 'tile[:rows]'");
         }
         s->cfg = c;
         return FW_OK;
@@ -213,6 +221,15 @@ int choose_kernels(fw_sim* s) {
             fw::LaunchCfg c;
             c.variant = 1;
             c.width = 64;
+            c.chunk = ch;
+            cands.push_back(c);
+        }
+    }
+    if (fw::tile2d_supported(s->ds)) {
+        for (int ch : {8, 16, 32}) {
+            fw::LaunchCfg c;
+            c.variant = 2;
+            c.width = 256;
             c.chunk = ch;
             cands.push_back(c);
         }

@@ -57,6 +57,8 @@ enum KernelKind { kVelocity = 0, kPressure = 1, kBoundary = 2, kNumKinds = 3 };
 //   variant 0: per-cell kernels (one thread per cell)
 //   variant 1: x-marching kernels; `width` lanes per workgroup along the contiguous axis
 //              (2D: 64, 128 or 256; 3D: fixed 64 x 8 tile) and `chunk` planes per workgroup
+//   variant 2: 2D tiled kernels with 4-wide vector access; tiles of `chunk` rows (8, 16, 32)
+//              by 256 columns (needs ny % 4 == 0)
 struct LaunchCfg {
     int variant = 0;
     int width = 128;
@@ -64,6 +66,7 @@ struct LaunchCfg {
 };
 
 bool streaming_supported(const DevSim& s);
+bool tile2d_supported(const DevSim& s);
 hipError_t launch_velocity(const DevSim& s, const LaunchCfg& cfg, hipStream_t st);
 hipError_t launch_pressure(const DevSim& s, const LaunchCfg& cfg, hipStream_t st);
 hipError_t launch_boundary(const DevSim& s, const DevBoundary& b, int64_t step, int64_t modt,

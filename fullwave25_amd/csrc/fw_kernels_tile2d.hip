@@ -1,0 +1,288 @@
+// Tiled 2D kernels with 4-wide vector access (gfx950).
+//
+// A workgroup of 256 lanes (4 wavefronts) owns a tile of TX rows x 256 columns. It first stages
+// the input field(s) of the tile plus the stencil halo into LDS with 16-byte loads, then every
+// lane computes 4 consecutive cells (one float4 column quad) on TX / 4 rows. All per-cell maps
+// and memory variables move as float4 (one 16-byte load or store per quad), which is what lets
+// the HBM stream approach the copy-kernel rate; the stencil operands come from LDS. Rows of a
+// tile are independent once the tile is staged, so each lane issues the loads of all its rows
+// without waiting at barriers.
+//
+// Requires ny % 4 == 0 (16-byte aligned rows). Arithmetic order is the per-cell kernels'.
+#include "fw_internal.h"
+
+namespace fw {
+
+namespace {
+
+constexpr int kTW = 256;        // columns per tile
+constexpr int kLanes = 256;     // threads per workgroup
+constexpr int kQuads = kTW / 4; // column quads per tile row (64: one wavefront)
+
+struct alignas(16) F4 {
+    float x, y, z, w;
+    __device__ float& operator[](int i) { return (&x)[i]; }
+    __device__ float operator[](int i) const { return (&x)[i]; }
+};
+struct alignas(16) I4 {
+    int x, y, z, w;
+    __device__ int operator[](int i) const { return (&x)[i]; }
+};
+
+__device__ inline F4 ld4(const float* p) { return *reinterpret_cast<const F4*>(p); }
+__device__ inline I4 ldi4(const int32_t* p) { return *reinterpret_cast<const I4*>(p); }
+__device__ inline void st4(float* p, const F4& v) { *reinterpret_cast<F4*>(p) = v; }
+
+// Stage rows [r0, r0 + nrows) x columns [j0 - hl, j0 + kTW + hr) of f into lds (row pitch
+// `pitch` floats, column j0 - hl at offset 0). hl and hr are multiples of 4; ny % 4 == 0, so
+// every quad is either entirely inside [0, ny) or entirely outside (then zero). Rows outside
+// [0, nx) are zero.
+__device__ inline void stage_tile(float* lds, const float* __restrict__ f, int64_t r0, int nrows, int64_t j0,
+                                  int hl, int hr, int pitch, int64_t nx, int64_t ny) {
+    const int qpr = (kTW + hl + hr) / 4;  // quads per row
+    const int total = nrows * qpr;
+    for (int t = threadIdx.x; t < total; t += kLanes) {
+        const int row = t / qpr, qd = t - row * qpr;
+        const int64_t r = r0 + row;
+        const int64_t jj = j0 - hl + 4 * qd;
+        F4 v{0.f, 0.f, 0.f, 0.f};
+        if (r >= 0 && r < nx && jj >= 0 && jj < ny) v = ld4(f + r * ny + jj);
+        *reinterpret_cast<F4*>(lds + row * pitch + 4 * qd) = v;
+    }
+}
+
+template <bool UNIFORM>
+__device__ inline const float* cset(const float* lds_coef, int dc, int axis) {
+    return UNIFORM ? lds_coef + axis * kNCoef : lds_coef + (dc * 2 + axis) * kNCoef;
+}
+
+// velocity: p tile rows i0-7 .. i0+TX+7 (TX + 15 rows), columns j0-8 .. j0+TW+8
+template <int TX, bool UNIFORM, bool ISO>
+__global__ __launch_bounds__(kLanes) void k_velocity_tile(DevSim s) {
+    constexpr int PITCH = kTW + 16;
+    constexpr int NROWS = TX + 15;
+    __shared__ __attribute__((aligned(16))) float P[NROWS * PITCH];
+    extern __shared__ float lds_coef[];
+    const int64_t nx = s.nx, ny = s.ny;
+    const int64_t j0 = (int64_t)blockIdx.x * kTW;
+    const int64_t i0 = kHalo + (int64_t)blockIdx.y * TX;
+    const int ncoef = (UNIFORM ? 1 : s.ndmap) * 2 * kNCoef;
+    for (int t = threadIdx.x; t < ncoef; t += kLanes) lds_coef[t] = s.coef[t];
+    stage_tile(P, s.p, i0 - 7, NROWS, j0, 8, 8, PITCH, nx, ny);
+    __syncthreads();
+
+    const int qd = threadIdx.x % kQuads;
+    const int ty = threadIdx.x / kQuads;
+    const int64_t jq = j0 + 4 * qd;  // first column of this lane's quad
+    if (jq >= ny) return;
+    const float dt = s.dt;
+#pragma unroll 1
+    for (int m = 0; m < TX / 4; ++m) {
+        const int lr = ty + 4 * m;  // row within the tile
+        const int64_t i = i0 + lr;
+        if (i >= nx - kHalo) break;
+        const int64_t c = i * ny + jq;
+        const F4 rho0 = ld4(s.rho + c);
+        const F4 rho1 = ld4(s.rho + c + ny);
+        const float rho_r = (jq + 4 < ny) ? s.rho[c + 4] : 0.f;
+        const I4 dc = UNIFORM ? I4{0, 0, 0, 0} : ldi4(s.dcmap + c);
+        F4 ik[2], b1[2], b2[2], a1[2], a2[2];
+        constexpr int NA = ISO ? 1 : 2;
+#pragma unroll
+        for (int a = 0; a < NA; ++a) {
+            ik[a] = ld4(s.ikv[a] + c);
+            b1[a] = ld4(s.bv[a][0] + c);
+            b2[a] = ld4(s.bv[a][1] + c);
+            a1[a] = ld4(s.av[a][0] + c);
+            a2[a] = ld4(s.av[a][1] + c);
+        }
+        F4 ps1[2], ps2[2], v[2];
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+            ps1[a] = ld4(s.psi[a][0] + c);
+            ps2[a] = ld4(s.psi[a][1] + c);
+            v[a] = ld4(s.v[a] + c);
+        }
+        // LDS row of tile row lr is lr + 7; column j is at (j - j0 + 8)
+        const float* row0 = P + (lr + 7) * PITCH + 4 * qd + 8;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int64_t j = jq + e;
+            if (j < kHalo || j >= ny - kHalo) continue;
+            const float* pc = row0 + e;  // p(i, j)
+            {
+                const float* w = cset<UNIFORM>(lds_coef, dc[e], 0);
+                float acc = 0.f;
+#pragma unroll
+                for (int k = 1; k <= 8; ++k) acc += w[k - 1] * (pc[k * PITCH] - pc[-(k - 1) * PITCH]);
+                float tr = 0.f;
+                tr += (pc[PITCH + 1] - pc[1]) + (pc[PITCH - 1] - pc[-1]);
+                const float D = acc + w[8] * tr;
+                const float q1 = b1[0][e] * ps1[0][e] + a1[0][e] * D;
+                const float q2 = b2[0][e] * ps2[0][e] + a2[0][e] * D;
+                ps1[0][e] = q1;
+                ps2[0][e] = q2;
+                const float rho_bar = 0.5f * (rho0[e] + rho1[e]);
+                v[0][e] -= dt / rho_bar * (D * ik[0][e] + (q1 + q2));
+            }
+            {
+                constexpr int ax = ISO ? 0 : 1;
+                const float* w = cset<UNIFORM>(lds_coef, dc[e], 1);
+                float acc = 0.f;
+#pragma unroll
+                for (int k = 1; k <= 8; ++k) acc += w[k - 1] * (pc[k] - pc[-(k - 1)]);
+                float tr = 0.f;
+                tr += (pc[PITCH + 1] - pc[PITCH]) + (pc[-PITCH + 1] - pc[-PITCH]);
+                const float D = acc + w[8] * tr;
+                const float q1 = b1[ax][e] * ps1[1][e] + a1[ax][e] * D;
+                const float q2 = b2[ax][e] * ps2[1][e] + a2[ax][e] * D;
+                ps1[1][e] = q1;
+                ps2[1][e] = q2;
+                const float rho_right = e < 3 ? rho0[e + 1] : rho_r;
+                const float rho_bar = 0.5f * (rho0[e] + rho_right);
+                v[1][e] -= dt / rho_bar * (D * ik[ax][e] + (q1 + q2));
+            }
+        }
+        // cells outside the active columns keep their loaded values, so whole quads store back
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+            st4(s.psi[a][0] + c, ps1[a]);
+            st4(s.psi[a][1] + c, ps2[a]);
+            st4(s.v[a] + c, v[a]);
+        }
+    }
+}
+
+// pressure: vx tile rows i0-8 .. i0+TX+6 (TX + 15 rows), columns j0-4 .. j0+TW+4;
+//           vy tile rows i0-1 .. i0+TX (TX + 2 rows), columns j0-8 .. j0+TW+8
+template <int TX, bool UNIFORM, bool ISO>
+__global__ __launch_bounds__(kLanes) void k_pressure_tile(DevSim s) {
+    constexpr int XP = kTW + 8;
+    constexpr int XR = TX + 15;
+    constexpr int YP = kTW + 16;
+    constexpr int YR = TX + 2;
+    __shared__ __attribute__((aligned(16))) float VX[XR * XP];
+    __shared__ __attribute__((aligned(16))) float VY[YR * YP];
+    extern __shared__ float lds_coef[];
+    const int64_t nx = s.nx, ny = s.ny;
+    const int64_t j0 = (int64_t)blockIdx.x * kTW;
+    const int64_t i0 = kHalo + (int64_t)blockIdx.y * TX;
+    const int ncoef = (UNIFORM ? 1 : s.ndmap) * 2 * kNCoef;
+    for (int t = threadIdx.x; t < ncoef; t += kLanes) lds_coef[t] = s.coef[t];
+    stage_tile(VX, s.v[0], i0 - 8, XR, j0, 4, 4, XP, nx, ny);
+    stage_tile(VY, s.v[1], i0 - 1, YR, j0, 8, 8, YP, nx, ny);
+    __syncthreads();
+
+    const int qd = threadIdx.x % kQuads;
+    const int ty = threadIdx.x / kQuads;
+    const int64_t jq = j0 + 4 * qd;
+    if (jq >= ny) return;
+    const float dt = s.dt;
+#pragma unroll 1
+    for (int m = 0; m < TX / 4; ++m) {
+        const int lr = ty + 4 * m;
+        const int64_t i = i0 + lr;
+        if (i >= nx - kHalo) break;
+        const int64_t c = i * ny + jq;
+        F4 pv = ld4(s.p + c);
+        const F4 K = ld4(s.K + c);
+        const I4 dc = UNIFORM ? I4{0, 0, 0, 0} : ldi4(s.dcmap + c);
+        constexpr int NA = ISO ? 1 : 2;
+        F4 ik[2], b1[2], b2[2], a1[2], a2[2];
+#pragma unroll
+        for (int a = 0; a < NA; ++a) {
+            ik[a] = ld4(s.ikp[a] + c);
+            b1[a] = ld4(s.bp[a][0] + c);
+            b2[a] = ld4(s.bp[a][1] + c);
+            a1[a] = ld4(s.ap[a][0] + c);
+            a2[a] = ld4(s.ap[a][1] + c);
+        }
+        F4 ph1[2], ph2[2];
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+            ph1[a] = ld4(s.phi[a][0] + c);
+            ph2[a] = ld4(s.phi[a][1] + c);
+        }
+        const float* xrow = VX + (lr + 8) * XP + 4 * qd + 4;  // vx(i, jq)
+        const float* yrow = VY + (lr + 1) * YP + 4 * qd + 8;  // vy(i, jq)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int64_t j = jq + e;
+            if (j < kHalo || j >= ny - kHalo) continue;
+            float div = 0.f;
+            {
+                const float* vc = xrow + e;
+                const float* w = cset<UNIFORM>(lds_coef, dc[e], 0);
+                float acc = 0.f;
+#pragma unroll
+                for (int k = 1; k <= 8; ++k) acc += w[k - 1] * (vc[(k - 1) * XP] - vc[-k * XP]);
+                float tr = 0.f;
+                tr += (vc[1] - vc[-XP + 1]) + (vc[-1] - vc[-XP - 1]);
+                const float D = acc + w[8] * tr;
+                const float q1 = b1[0][e] * ph1[0][e] + a1[0][e] * D;
+                const float q2 = b2[0][e] * ph2[0][e] + a2[0][e] * D;
+                ph1[0][e] = q1;
+                ph2[0][e] = q2;
+                div += D * ik[0][e] + (q1 + q2);
+            }
+            {
+                constexpr int ax = ISO ? 0 : 1;
+                const float* vc = yrow + e;
+                const float* w = cset<UNIFORM>(lds_coef, dc[e], 1);
+                float acc = 0.f;
+#pragma unroll
+                for (int k = 1; k <= 8; ++k) acc += w[k - 1] * (vc[k - 1] - vc[-k]);
+                float tr = 0.f;
+                tr += (vc[YP] - vc[YP - 1]) + (vc[-YP] - vc[-YP - 1]);
+                const float D = acc + w[8] * tr;
+                const float q1 = b1[ax][e] * ph1[1][e] + a1[ax][e] * D;
+                const float q2 = b2[ax][e] * ph2[1][e] + a2[ax][e] * D;
+                ph1[1][e] = q1;
+                ph2[1][e] = q2;
+                div += D * ik[ax][e] + (q1 + q2);
+            }
+            pv[e] -= dt * K[e] * div;
+        }
+        st4(s.p + c, pv);
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+            st4(s.phi[a][0] + c, ph1[a]);
+            st4(s.phi[a][1] + c, ph2[a]);
+        }
+    }
+}
+
+template <int TX>
+hipError_t launch_tile(const DevSim& s, bool velocity, hipStream_t st) {
+    const bool u = s.ndmap == 1;
+    const int lds = (u ? 1 : s.ndmap) * 2 * kNCoef * 4;
+    const int64_t rows = s.nx - 2 * kHalo;
+    const dim3 g((unsigned)((s.ny + kTW - 1) / kTW), (unsigned)((rows + TX - 1) / TX), 1);
+    const dim3 b(kLanes);
+    if (velocity) {
+        if (u && s.isotropic) hipLaunchKernelGGL((k_velocity_tile<TX, true, true>), g, b, lds, st, s);
+        else if (u) hipLaunchKernelGGL((k_velocity_tile<TX, true, false>), g, b, lds, st, s);
+        else if (s.isotropic) hipLaunchKernelGGL((k_velocity_tile<TX, false, true>), g, b, lds, st, s);
+        else hipLaunchKernelGGL((k_velocity_tile<TX, false, false>), g, b, lds, st, s);
+    } else {
+        if (u && s.isotropic) hipLaunchKernelGGL((k_pressure_tile<TX, true, true>), g, b, lds, st, s);
+        else if (u) hipLaunchKernelGGL((k_pressure_tile<TX, true, false>), g, b, lds, st, s);
+        else if (s.isotropic) hipLaunchKernelGGL((k_pressure_tile<TX, false, true>), g, b, lds, st, s);
+        else hipLaunchKernelGGL((k_pressure_tile<TX, false, false>), g, b, lds, st, s);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace
+
+bool tile2d_supported(const DevSim& s) { return s.ndim == 2 && s.ny % 4 == 0 && s.ndmap <= 1024; }
+
+hipError_t launch_tile2d(const DevSim& s, const LaunchCfg& cfg, bool velocity, hipStream_t st) {
+    switch (cfg.chunk) {
+        case 8: return launch_tile<8>(s, velocity, st);
+        case 32: return launch_tile<32>(s, velocity, st);
+        default: return launch_tile<16>(s, velocity, st);
+    }
+}
+
+}  // namespace fw

@@ -178,7 +178,8 @@ class HipSimulation:
         """Kernel family/shape this handle runs (chosen on first use)."""
         v, w, c = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
         _check(self.lib, self.lib.fw_sim_kernel_config(self._h, ctypes.byref(v), ctypes.byref(w), ctypes.byref(c)))
-        return {"variant": "percell" if v.value == 0 else "stream", "width": w.value, "chunk": c.value}
+        name = {0: "percell", 1: "stream", 2: "tile"}.get(v.value, str(v.value))
+        return {"variant": name, "width": w.value, "chunk": c.value}
 
     def reset(self) -> None:
         _check(self.lib, self.lib.fw_sim_reset(self._h))

@@ -103,9 +103,11 @@ int fw_sim_download_field(fw_sim* sim, const char* name, float* out, size_t byte
 int fw_sim_set_field(fw_sim* sim, const char* name, const float* values, size_t bytes);
 /* Kernel family and shape the handle runs (it has no counterpart in the reference, whose
  * binary fixes its kernels): variant 0 = per-cell kernels, 1 = x-marching kernels with
- * `width` lanes per workgroup along the contiguous axis and `chunk` planes per workgroup.
+ * `width` lanes per workgroup along the contiguous axis and `chunk` planes per workgroup,
+ * 2 = 2D tiled kernels with 4-wide vector access on tiles of `chunk` rows x 256 columns.
  * Chosen on the first run by timing the candidates on the handle's own arrays (or forced with
- * the FULLWAVE_HIP_KERNELS environment variable: "percell" or "stream[:width[:chunk]]").
+ * the FULLWAVE_HIP_KERNELS environment variable: "percell", "stream[:width[:chunk]]" or
+ * "tile[:rows]").
  * Calling this before the first run makes that choice. */
 int fw_sim_kernel_config(fw_sim* sim, int32_t* variant, int32_t* width, int32_t* chunk);
 /* Run n_steps with a HIP event pair around every launch on the handle's stream and add the
