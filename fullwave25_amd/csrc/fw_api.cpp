@@ -72,7 +72,7 @@ struct fw_sim {
     bool ready = false;
     int64_t step = 0;
     int64_t rec_cap = 0;
-    fw::LaunchCfg cfg;
+    fw::LaunchCfg cfg_v, cfg_p;  // velocity and pressure kernels, chosen independently
     bool fields_set = false;  // a field was set before the first run: do not tune on it
     hipEvent_t ev[2] = {nullptr, nullptr};
 };
@@ -146,71 +146,80 @@ int check_coords(const fw_sim* s, const std::vector<int32_t>& c, int64_t n, cons
     return FW_OK;
 }
 
-// Time one configuration: the two stencil kernels, `reps` times after one warm-up.
-int time_config(fw_sim* s, const fw::LaunchCfg& cfg, int reps, float* ms) {
-    FW_HIP(fw::launch_velocity(s->ds, cfg, s->stream));
-    FW_HIP(fw::launch_pressure(s->ds, cfg, s->stream));
+// Time one configuration of one stencil kernel, `reps` launches after one warm-up.
+int time_config(fw_sim* s, const fw::LaunchCfg& cfg, bool velocity, int reps, float* ms) {
+    auto launch = [&] {
+        return velocity ? fw::launch_velocity(s->ds, cfg, s->stream) : fw::launch_pressure(s->ds, cfg, s->stream);
+    };
+    FW_HIP(launch());
     FW_HIP(hipEventRecord(s->ev[0], s->stream));
-    for (int r = 0; r < reps; ++r) {
-        FW_HIP(fw::launch_velocity(s->ds, cfg, s->stream));
-        FW_HIP(fw::launch_pressure(s->ds, cfg, s->stream));
-    }
+    for (int r = 0; r < reps; ++r) FW_HIP(launch());
     FW_HIP(hipEventRecord(s->ev[1], s->stream));
     FW_HIP(hipEventSynchronize(s->ev[1]));
     FW_HIP(hipEventElapsedTime(ms, s->ev[0], s->ev[1]));
     return FW_OK;
 }
 
-// Pick the kernel family and shape for this grid. FULLWAVE_HIP_KERNELS overrides:
-// "percell", or "stream[:width[:chunk]]". Otherwise every candidate is timed on the handle's
-// own arrays while the wave state is still all zero (the update then leaves it zero, so the
-// timing runs change nothing) and the fastest is kept.
+int parse_kernel_env(fw_sim* s, const char* env, fw::LaunchCfg* out) {
+    const bool stream_ok = fw::streaming_supported(s->ds);
+    fw::LaunchCfg c;
+    c.width = s->ndim == 3 ? 64 : 128;
+    c.chunk = 16;
+    if (!strncmp(env, "percell", 7)) {
+        c.variant = 0;
+    } else if (!strncmp(env, "tile", 4)) {
+        if (!fw::tile2d_supported(s->ds))
+            return fail(FW_ERR_UNSUPPORTED, "FULLWAVE_HIP_KERNELS=tile needs a 2D grid with ny %% 4 == 0");
+        c.variant = 2;
+        c.width = 256;
+        int ch = 16;
+        c.chunk = sscanf(env, "tile:%d", &ch) == 1 ? ch : 16;
+        if (c.chunk != 8 && c.chunk != 16 && c.chunk != 32)
+            return fail(FW_ERR_INVALID, "FULLWAVE_HIP_KERNELS: tile rows must be 8, 16 or 32");
+    } else if (!strncmp(env, "stream", 6)) {
+        if (!stream_ok) return fail(FW_ERR_UNSUPPORTED, "FULLWAVE_HIP_KERNELS=stream: this grid needs the per-cell kernels");
+        c.variant = 1;
+        int w = 0, ch = 0;
+        const int n = sscanf(env, "stream:%d:%d", &w, &ch);
+        if (n >= 1) c.width = w;
+        if (n >= 2) c.chunk = ch;
+        if (s->ndim == 2 && c.width != 64 && c.width != 128 && c.width != 256)
+            return fail(FW_ERR_INVALID, "FULLWAVE_HIP_KERNELS: width must be 64, 128 or 256");
+        if (c.chunk < 2 || c.chunk > 4096) return fail(FW_ERR_INVALID, "FULLWAVE_HIP_KERNELS: chunk out of range");
+    } else {
+        return fail(FW_ERR_INVALID, "FULLWAVE_HIP_KERNELS must be 'percell', 'stream[:width[:chunk]]' or 'tile[:rows]'");
+    }
+    *out = c;
+    return FW_OK;
+}
+
+// Pick the kernel family and shape of each stencil kernel for this grid.
+// FULLWAVE_HIP_KERNELS overrides both: "percell", "stream[:width[:chunk]]" or "tile[:rows]".
+// Otherwise every candidate is timed on the handle's own arrays while the wave state is still
+// all zero (the update then leaves it zero, so the timing runs change nothing), and the
+// fastest velocity kernel and the fastest pressure kernel are kept.
 int choose_kernels(fw_sim* s) {
     const char* env = getenv("FULLWAVE_HIP_KERNELS");
+    if (env && *env) {
+        int rc = parse_kernel_env(s, env, &s->cfg_v);
+        if (rc != FW_OK) return rc;
+        s->cfg_p = s->cfg_v;
+        return FW_OK;
+    }
     const bool stream_ok = fw::streaming_supported(s->ds);
     fw::LaunchCfg def;
     def.variant = stream_ok ? 1 : 0;
-    def.width = s->ndim == 3 ? 64 : 128;
-    def.chunk = 32;
-    if (env && *env) {
-        fw::LaunchCfg c = def;
-        if (!strncmp(env, "percell", 7)) {
-            c.variant = 0;
-        } else if (!strncmp(env, "tile", 4)) {
-            if (!fw::tile2d_supported(s->ds)) return fail(FW_ERR_UNSUPPORTED, "FULLWAVE_HIP_KERNELS=tile needs a 2D grid with ny %% 4 == 0");
-            c.variant = 2;
-            c.width = 256;
-            int ch = 16;
-            if (sscanf(env, "tile:%d", &ch) == 1) c.chunk = ch;
-            else c.chunk = 16;
-            if (c.chunk != 8 && c.chunk != 16 && c.chunk != 32) return fail(FW_ERR_INVALID, "FULLWAVE_HIP_KERNELS: tile rows must be 8, 16 or 32");
-        } else if (!strncmp(env, "stream", 6)) {
-            if (!stream_ok) return fail(FW_ERR_UNSUPPORTED, "FULLWAVE_HIP_KERNELS=stream: this grid needs the per-cell kernels");
-            c.variant = 1;
-            int w = 0, ch = 0;
-            const int n = sscanf(env, "stream:%d:%d", &w, &ch);
-            if (n >= 1) c.width = w;
-            if (n >= 2) c.chunk = ch;
-            if (s->ndim == 2 && c.width != 64 && c.width != 128 && c.width != 256)
-                return fail(FW_ERR_INVALID, "FULLWAVE_HIP_KERNELS: width must be 64, 128 or 256");
-            if (c.chunk < 2 || c.chunk > 4096) return fail(FW_ERR_INVALID, "FULLWAVE_HIP_KERNELS: chunk out of range");
-        } else {
-            return fail(FW_ERR_INVALID, "FULLWAVE_HIP_KERNELS must be 'percell', 'stream[:width[:chunk]]' or 'tile[:rows]'");
-        }
-        s->cfg = c;
-        return FW_OK;
-    }
-    s->cfg = def;
+    def.width = s->ndim == 3 ? 64 : 256;
+    def.chunk = 8;
+    s->cfg_v = s->cfg_p = def;
     if (s->fields_set || !stream_ok) return FW_OK;
     std::vector<fw::LaunchCfg> cands;
     fw::LaunchCfg pc;
     pc.variant = 0;
     cands.push_back(pc);
-    const int widths2d[] = {64, 128, 256};
-    const int chunks[] = {16, 32, 64};
-    for (int ch : chunks) {
+    for (int ch : {8, 16, 32}) {
         if (s->ndim == 2) {
-            for (int w : widths2d) {
+            for (int w : {128, 256}) {
                 fw::LaunchCfg c;
                 c.variant = 1;
                 c.width = w;
@@ -234,14 +243,16 @@ int choose_kernels(fw_sim* s) {
             cands.push_back(c);
         }
     }
-    float best = 1e30f;
-    for (const auto& c : cands) {
-        float ms = 0.f;
-        int rc = time_config(s, c, 3, &ms);
-        if (rc != FW_OK) return rc;
-        if (ms < best) {
-            best = ms;
-            s->cfg = c;
+    for (int kind = 0; kind < 2; ++kind) {
+        float best = 1e30f;
+        for (const auto& c : cands) {
+            float ms = 0.f;
+            int rc = time_config(s, c, kind == 0, 3, &ms);
+            if (rc != FW_OK) return rc;
+            if (ms < best) {
+                best = ms;
+                (kind == 0 ? s->cfg_v : s->cfg_p) = c;
+            }
         }
     }
     return FW_OK;
@@ -373,9 +384,9 @@ int launch_step(fw_sim* s, bool timed, double* ms) {
         return FW_OK;
     };
     int rc;
-    rc = timed_launch(fw::kVelocity, [&] { return fw::launch_velocity(s->ds, s->cfg, s->stream); });
+    rc = timed_launch(fw::kVelocity, [&] { return fw::launch_velocity(s->ds, s->cfg_v, s->stream); });
     if (rc != FW_OK) return rc;
-    rc = timed_launch(fw::kPressure, [&] { return fw::launch_pressure(s->ds, s->cfg, s->stream); });
+    rc = timed_launch(fw::kPressure, [&] { return fw::launch_pressure(s->ds, s->cfg_p, s->stream); });
     if (rc != FW_OK) return rc;
     rc = timed_launch(fw::kBoundary, [&] { return fw::launch_boundary(s->ds, s->db, s->step, s->d.modt, s->stream); });
     if (rc != FW_OK) return rc;
@@ -649,16 +660,18 @@ int fw_sim_set_field(fw_sim* s, const char* name, const float* values, size_t by
     return FW_OK;
 }
 
-int fw_sim_kernel_config(fw_sim* s, int32_t* variant, int32_t* width, int32_t* chunk) {
+int fw_sim_kernel_config(fw_sim* s, int32_t kind, int32_t* variant, int32_t* width, int32_t* chunk) {
     if (!s || !variant || !width || !chunk) return fail(FW_ERR_INVALID, "null argument");
+    if (kind != 0 && kind != 1) return fail(FW_ERR_INVALID, "kind must be 0 (velocity) or 1 (pressure)");
     if (!s->ready) {
         DeviceGuard g(s->device);
         int rc = finalize(s);
         if (rc != FW_OK) return rc;
     }
-    *variant = s->cfg.variant;
-    *width = s->cfg.width;
-    *chunk = s->cfg.chunk;
+    const fw::LaunchCfg& c = kind == 0 ? s->cfg_v : s->cfg_p;
+    *variant = c.variant;
+    *width = c.width;
+    *chunk = c.chunk;
     return FW_OK;
 }
 

@@ -87,7 +87,7 @@ def load_library(path: Path | None = None) -> ctypes.CDLL:
         lib.fw_sim_download_sensors.argtypes = [vp, vp, sz]
         lib.fw_sim_download_field.argtypes = [vp, ctypes.c_char_p, vp, sz]
         lib.fw_sim_set_field.argtypes = [vp, ctypes.c_char_p, vp, sz]
-        lib.fw_sim_kernel_config.argtypes = [vp, ctypes.POINTER(i32), ctypes.POINTER(i32), ctypes.POINTER(i32)]
+        lib.fw_sim_kernel_config.argtypes = [vp, i32, ctypes.POINTER(i32), ctypes.POINTER(i32), ctypes.POINTER(i32)]
         lib.fw_sim_run_timed.argtypes = [vp, i64, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(i32)]
         lib.fw_sim_destroy.argtypes = [vp]
         if lib.fw_abi_version() != 1:
@@ -175,11 +175,14 @@ class HipSimulation:
         return {KERNEL_KINDS[i]: ms[i] for i in range(nk.value)}
 
     def kernel_config(self) -> dict:
-        """Kernel family/shape this handle runs (chosen on first use)."""
-        v, w, c = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
-        _check(self.lib, self.lib.fw_sim_kernel_config(self._h, ctypes.byref(v), ctypes.byref(w), ctypes.byref(c)))
-        name = {0: "percell", 1: "stream", 2: "tile"}.get(v.value, str(v.value))
-        return {"variant": name, "width": w.value, "chunk": c.value}
+        """Kernel family/shape of the velocity and pressure kernels (chosen on first use)."""
+        out = {}
+        for kind, label in ((0, "velocity"), (1, "pressure")):
+            v, w, c = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+            _check(self.lib, self.lib.fw_sim_kernel_config(self._h, kind, ctypes.byref(v), ctypes.byref(w), ctypes.byref(c)))
+            name = {0: "percell", 1: "stream", 2: "tile"}.get(v.value, str(v.value))
+            out[label] = {"variant": name, "width": w.value, "chunk": c.value}
+        return out
 
     def reset(self) -> None:
         _check(self.lib, self.lib.fw_sim_reset(self._h))

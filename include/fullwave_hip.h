@@ -101,15 +101,17 @@ int fw_sim_download_field(fw_sim* sim, const char* name, float* out, size_t byte
 /* Overwrite a wave field by name ("p", "vx", "vy", "vz") from float32 [nx, ny(, nz)]; ghost
  * cells are left as given. Used to start from an initial condition or a checkpoint. */
 int fw_sim_set_field(fw_sim* sim, const char* name, const float* values, size_t bytes);
-/* Kernel family and shape the handle runs (it has no counterpart in the reference, whose
- * binary fixes its kernels): variant 0 = per-cell kernels, 1 = x-marching kernels with
+/* Kernel family and shape the handle runs for `kind` (0 = velocity kernel, 1 = pressure
+ * kernel). It has no counterpart in the reference, whose binary fixes its kernels.
+ * variant 0 = per-cell kernels, 1 = x-marching kernels with
  * `width` lanes per workgroup along the contiguous axis and `chunk` planes per workgroup,
  * 2 = 2D tiled kernels with 4-wide vector access on tiles of `chunk` rows x 256 columns.
- * Chosen on the first run by timing the candidates on the handle's own arrays (or forced with
+ * Chosen on the first run by timing the candidates on the handle's own arrays, separately for
+ * the two kernels (or forced for both with
  * the FULLWAVE_HIP_KERNELS environment variable: "percell", "stream[:width[:chunk]]" or
  * "tile[:rows]").
  * Calling this before the first run makes that choice. */
-int fw_sim_kernel_config(fw_sim* sim, int32_t* variant, int32_t* width, int32_t* chunk);
+int fw_sim_kernel_config(fw_sim* sim, int32_t kind, int32_t* variant, int32_t* width, int32_t* chunk);
 /* Run n_steps with a HIP event pair around every launch on the handle's stream and add the
  * summed milliseconds per kernel kind to ms[0..n_kinds) (kinds: 0 velocity, 1 pressure,
  * 2 source/air/sensor); also returns the number of kinds in *n_kinds. */
