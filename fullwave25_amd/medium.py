@@ -319,13 +319,15 @@ class Medium:
                 f"Unknown attenuation_builder: {self.attenuation_builder}. "
                 'Only "lookup" is supported currently.',
             )
-        from .relaxation import generate_relaxation_params
+        from .relaxation import fit_band_for_grid, generate_relaxation_params
 
         params = generate_relaxation_params(
             alpha_coeff=self.alpha_coeff,
             alpha_power=self.alpha_power,
             n_relaxation_mechanisms=self.n_relaxation_mechanisms,
             path_database=self.path_relaxation_parameters_database,
+            dt=self.grid.dt,
+            band=fit_band_for_grid(self.grid.dx, self.grid.c0),
         )
         return MediumRelaxationMaps(
             grid=self.grid,

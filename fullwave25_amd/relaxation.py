@@ -91,9 +91,25 @@ def _unpack(theta):
     return p
 
 
+def stretch_inverse_discrete(w, kappa, d, alpha, dt) -> NDArray[np.complex128]:
+    """The engine's S(w): the recursive convolution psi_n = b psi_(n-1) + a D_n with
+    a, b of (d, kappa, alpha, dt), i.e. 1/kappa + sum_nu a_nu / (1 - b_nu exp(i w dt))."""
+    w = np.asarray(w, dtype=np.float64)
+    out = np.full(w.shape, 1.0 / kappa, dtype=np.complex128)
+    for dn, an in zip(d, alpha):
+        b = np.exp(-(dn / kappa + an) * dt)
+        a = dn / (kappa * (dn + kappa * an) + 1e-10) * (b - 1)
+        out += a / (1 - b * np.exp(1j * w * dt))
+    return out
+
+
 def fit_relaxation_parameters(alpha0: float, gamma: float, *, band=FIT_BAND_HZ, n_freq: int = N_FIT_FREQ,
+                              dt: float | None = None,
                               warm_start: NDArray[np.float64] | None = None, warm_alpha0: float | None = None):
     """Fit one (alpha0, gamma) pair; returns ``(params[10], max relative error over band)``.
+
+    With ``dt`` the model is the engine's discrete-time recursive convolution at that time
+    step (what the simulation actually attenuates by); without it, the continuous-time model.
 
     ``warm_start`` (a fitted row, for ``warm_alpha0``) seeds a single local solve with its
     ``d`` rates scaled by ``alpha0 / warm_alpha0`` (loss is nearly linear in ``d``); the
@@ -108,10 +124,15 @@ def fit_relaxation_parameters(alpha0: float, gamma: float, *, band=FIT_BAND_HZ, 
     tgt = target_attenuation(f, alpha0, gamma)
     w = 2 * np.pi * f
 
+    def sides(p):
+        if dt is None:
+            return (stretch_inverse(w, p[0], p[2::4], p[3::4]), stretch_inverse(w, p[1], p[4::4], p[5::4]))
+        return (stretch_inverse_discrete(w, p[0], p[2::4], p[3::4], dt),
+                stretch_inverse_discrete(w, p[1], p[4::4], p[5::4], dt))
+
     def resid(theta):
         p = _unpack(theta)
-        s1 = stretch_inverse(w, p[0], p[2::4], p[3::4])
-        s2 = stretch_inverse(w, p[1], p[4::4], p[5::4])
+        s1, s2 = sides(p)
         k = (w / C_REF) / np.sqrt(s1 * s2)
         att = np.imag(k)
         r_att = np.log(np.maximum(att, 1e-30) / tgt)
@@ -121,11 +142,15 @@ def fit_relaxation_parameters(alpha0: float, gamma: float, *, band=FIT_BAND_HZ, 
         imb = 10.0 * np.log(s1 / s2)
         return np.concatenate([r_att, [10.0 * vel.mean()], imb.real, imb.imag])
 
-    lo = np.array([-0.5, -0.5] + [np.log(1e2)] * 8)
+    # kappa >= 1 on both sides: the high-frequency wave speed c / sqrt(kappa_1 kappa_2) must not
+    # exceed c, or the time step would leave the stable range
+    lo = np.array([0.0, 0.0] + [np.log(1e2)] * 8)
     hi = np.array([0.5, 0.5] + [np.log(1e10)] * 8)
 
     def max_err(p):
-        return float(np.max(np.abs(model_attenuation(f, p) / tgt - 1)))
+        s1, s2 = sides(p)
+        att = np.imag((w / C_REF) / np.sqrt(s1 * s2))
+        return float(np.max(np.abs(att / tgt - 1)))
 
     if warm_start is not None and warm_alpha0 and warm_alpha0 > 0:
         ws = np.array(warm_start, dtype=np.float64)
@@ -167,7 +192,10 @@ class RelaxationTable:
     reference's variable names loads with :meth:`load`.
     """
 
-    def __init__(self, alpha_list=ALPHA_0_LIST, power_list=POWER_LIST) -> None:
+    def __init__(self, alpha_list=ALPHA_0_LIST, power_list=POWER_LIST, *, dt: float | None = None,
+                 band=FIT_BAND_HZ) -> None:
+        self.dt = dt
+        self.band = tuple(band)
         self.alpha_list = np.asarray(alpha_list, dtype=np.float64).reshape(-1)
         self.power_list = np.asarray(power_list, dtype=np.float64).reshape(-1)
         na, npw = self.alpha_list.size, self.power_list.size
@@ -209,6 +237,7 @@ class RelaxationTable:
                         k = filled[np.argmin(np.abs(filled[:, 1] - p) * 1000 + np.abs(filled[:, 0] - a))]
                         warm, warm_a0 = self.database[k[0], k[1]], self.alpha_list[k[0]]
                 params, err = fit_relaxation_parameters(self.alpha_list[a], self.power_list[p],
+                                                        band=self.band, dt=self.dt,
                                                         warm_start=warm, warm_alpha0=warm_a0)
                 self.database[a, p] = params
                 self.fit_error[a, p] = err
@@ -240,6 +269,8 @@ class RelaxationTable:
     def save(self, path: Path) -> None:
         path = Path(path)
         data = {
+            "dt": np.array(np.nan if self.dt is None else self.dt),
+            "band_hz": np.array(self.band),
             "database": self.database, "alpha_0_list": self.alpha_list[None, :],
             "power_list": self.power_list[None, :], "invalid_matrix": self.invalid_matrix,
         }
@@ -259,7 +290,9 @@ class RelaxationTable:
             data = loadmat(path)
         else:
             data = dict(np.load(path, allow_pickle=False))
-        t = cls(data["alpha_0_list"], data["power_list"])
+        dt = float(np.asarray(data["dt"]).reshape(-1)[0]) if "dt" in data else float("nan")
+        band = tuple(np.asarray(data["band_hz"]).reshape(-1)) if "band_hz" in data else FIT_BAND_HZ
+        t = cls(data["alpha_0_list"], data["power_list"], dt=None if np.isnan(dt) else dt, band=band)
         t.database = np.asarray(data["database"], dtype=np.float64)
         t.invalid_matrix = np.asarray(data["invalid_matrix"]).astype(bool)
         if t.database.ndim != 3 or t.database.shape[2] != 10:
@@ -269,14 +302,21 @@ class RelaxationTable:
         return t
 
 
-_default_table: RelaxationTable | None = None
+_tables: dict = {}
 
 
-def default_table() -> RelaxationTable:
-    global _default_table
-    if _default_table is None:
-        _default_table = RelaxationTable()
-    return _default_table
+def default_table(dt: float | None = None, band=FIT_BAND_HZ) -> RelaxationTable:
+    """The process-wide table for one time step and band (filled lazily)."""
+    key = (None if dt is None else float(np.float64(dt)), tuple(float(b) for b in band))
+    if key not in _tables:
+        _tables[key] = RelaxationTable(dt=dt, band=band)
+    return _tables[key]
+
+
+def fit_band_for_grid(dx: float, c0: float = C_REF) -> tuple[float, float]:
+    """1 MHz up to 20 MHz or to 4 points per wavelength on the grid, whichever is lower."""
+    f_hi = min(FIT_BAND_HZ[1], c0 / (4.0 * dx))
+    return (min(FIT_BAND_HZ[0], f_hi / 10.0), f_hi)
 
 
 def generate_relaxation_params(
@@ -284,12 +324,18 @@ def generate_relaxation_params(
     alpha_power,
     n_relaxation_mechanisms: int = 2,
     path_database: Path | None = None,
+    *,
+    dt: float | None = None,
+    band=FIT_BAND_HZ,
 ) -> dict:
     """Per-voxel relaxation dictionary for ``alpha_coeff`` / ``alpha_power`` maps
-    (same signature and clipping as fullwave/utils/relaxation_parameters.py:78-112)."""
+    (same signature and clipping as fullwave/utils/relaxation_parameters.py:78-112).
+
+    ``dt`` fits the engine's discrete-time model for that time step (``Medium.build`` passes
+    its grid's); ``path_database`` uses a saved table instead of fitting."""
     if n_relaxation_mechanisms != 2:
         raise ValueError("Only n_relaxation_mechanisms=2 are supported currently.")
-    table = RelaxationTable.load(path_database) if path_database is not None else default_table()
+    table = RelaxationTable.load(path_database) if path_database is not None else default_table(dt, band)
     a = np.asarray(alpha_coeff, dtype=np.float64)
     g = np.asarray(alpha_power, dtype=np.float64)
     if np.any(a < table.alpha_min) or np.any(g < table.power_min):

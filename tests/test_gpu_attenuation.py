@@ -1,0 +1,43 @@
+"""Attenuation accuracy against the reference's published figure (needs an MI355X).
+
+For each of the 15 (alpha0, gamma) panels of the reference's ``figs/attenuation_modeling.svg``
+(digitized into tests/golden/attenuation_modeling_reference.json), the engine propagates a plane
+wave through the medium that ``Medium.build`` makes for (alpha0, gamma), and the attenuation is
+measured by spectral ratio between two depths (tests/attenuation.py). At every frequency of the
+figure the relative error against alpha0 f^gamma must stay within the reference's own error at
+that panel and frequency, plus 2 % (1 % digitization, 1 % measurement).
+"""
+
+import json
+import os
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from fullwave25_amd import engine
+from tests import attenuation as A
+
+pytestmark = pytest.mark.gpu
+SLACK = 0.02
+
+
+@pytest.mark.parametrize("panel", A.reference_panels(), ids=lambda p: f"a{p['alpha0_db_mhz_cm']}_g{p['gamma']}")
+def test_attenuation_within_reference_error(panel):
+    a0, g = panel["alpha0_db_mhz_cm"], panel["gamma"]
+    inp, meta = A.build_inputs(a0, g)
+    print(f"panel {panel['panel']}: alpha0 {a0} gamma {g}, grid {inp.shape}, {inp.nt} steps", flush=True)
+    with engine.HipSimulation(inp) as sim:
+        sim.run(inp.nt)
+        traces = sim.sensors().astype(np.float64)
+    f = np.array(panel["freq_mhz"])
+    err = A.attenuation_from_traces(traces, meta, f) / A.target(a0, g, f) - 1
+    ref = np.array(panel["rel_err"])
+    print("  ours %:", np.round(100 * err, 2).tolist(), flush=True)
+    print("  ref  %:", np.round(100 * ref, 2).tolist(), flush=True)
+    out = Path(os.environ.get("GRAFT_REPO_ROOT", ".")) / "gpurun_out"
+    if out.is_dir():
+        with open(out / "attenuation.jsonl", "a") as fh:
+            fh.write(json.dumps({"panel": panel["panel"], "alpha0": a0, "gamma": g, "freq_mhz": f.tolist(),
+                                 "ours": err.tolist(), "reference": ref.tolist()}) + "\n")
+    assert np.all(np.abs(err) <= np.abs(ref) + SLACK), (err, ref)
