@@ -84,6 +84,10 @@ int64_t cells(const fw_desc& d) { return d.nx * d.ny * (d.ndim == 3 ? d.nz : 1);
 std::vector<std::string> required_maps(const fw_desc& d) {
     std::vector<std::string> names = {"K", "rho", "dcmap", "dmap", "icmat", "icc", "outc"};
     if (d.ncoordszero > 0) names.push_back("icczero");
+    if (d.attenuation_model == 1) {
+        names.push_back("a_exp");
+        return names;
+    }
     const int nax = d.isotropic ? 1 : d.ndim;
     for (int a = 0; a < nax; ++a) {
         names.push_back(std::string("kappa") + kAxesV[a]);
@@ -100,6 +104,7 @@ std::vector<std::string> required_maps(const fw_desc& d) {
 
 bool is_voxel_map(const fw_desc& d, const std::string& name) {
     if (name == "K" || name == "rho" || name == "c" || name == "beta" || name == "dcmap") return true;
+    if (d.attenuation_model == 1) return name == "a_exp";
     const int nax = d.isotropic ? 1 : d.ndim;
     for (int a = 0; a < nax; ++a) {
         if (name == std::string("kappa") + kAxesV[a] || name == std::string("kappa") + kAxesP[a]) return true;
@@ -346,7 +351,8 @@ int finalize(fw_sim* s) {
 
     // map pointers
     const int nax = d.isotropic ? 1 : nd;
-    for (int a = 0; a < nd; ++a) {
+    s->ds.aexp = d.attenuation_model == 1 ? (const float*)s->dev["a_exp"] : nullptr;
+    for (int a = 0; a < nd && d.attenuation_model == 0; ++a) {
         const int src_axis = nax == 1 ? 0 : a;
         s->ds.ikv[a] = (const float*)s->dev[std::string("kappa") + kAxesV[src_axis]];
         s->ds.ikp[a] = (const float*)s->dev[std::string("kappa") + kAxesP[src_axis]];
@@ -482,6 +488,8 @@ int fw_sim_create(const fw_desc* desc, int device, fw_sim** out) {
         return fail(FW_ERR_INVALID, "negative count in descriptor");
     if (d.modt < 1) return fail(FW_ERR_INVALID, "modt must be >= 1");
     if (d.ndmap < 1) return fail(FW_ERR_INVALID, "ndmap must be >= 1");
+    if (d.attenuation_model != 0 && d.attenuation_model != 1)
+        return fail(FW_ERR_INVALID, "attenuation_model must be 0 (relaxation) or 1 (exponential)");
     if (!(d.dt > 0) || !(d.dx > 0) || !(d.dy > 0) || (d.ndim == 3 && !(d.dz > 0)))
         return fail(FW_ERR_INVALID, "dt and grid spacings must be positive");
     int ndev = 0;
@@ -504,7 +512,8 @@ int fw_sim_create(const fw_desc* desc, int device, fw_sim** out) {
     if (hipEventCreate(&s->ev[0]) != hipSuccess || hipEventCreate(&s->ev[1]) != hipSuccess)
         return cleanup(fail(FW_ERR_DEVICE, "hipEventCreate failed"));
     const int nd = d.ndim;
-    s->nstate = (int64_t)(1 + nd + 4 * nd) * s->ncell;
+    const bool relax = d.attenuation_model == 0;  // the exponential model has no memory variables
+    s->nstate = (int64_t)(1 + nd + (relax ? 4 * nd : 0)) * s->ncell;
     if (hipMalloc(&s->state, (size_t)s->nstate * 4) != hipSuccess)
         return cleanup(fail(FW_ERR_NOMEM, "hipMalloc of %lld state floats failed", (long long)s->nstate));
     if (hipMemset(s->state, 0, (size_t)s->nstate * 4) != hipSuccess)
@@ -513,9 +522,9 @@ int fw_sim_create(const fw_desc* desc, int device, fw_sim** out) {
     s->ds.p = f;
     f += s->ncell;
     for (int a = 0; a < nd; ++a, f += s->ncell) s->ds.v[a] = f;
-    for (int a = 0; a < nd; ++a)
+    for (int a = 0; a < nd && relax; ++a)
         for (int nu = 0; nu < 2; ++nu, f += s->ncell) s->ds.psi[a][nu] = f;
-    for (int a = 0; a < nd; ++a)
+    for (int a = 0; a < nd && relax; ++a)
         for (int nu = 0; nu < 2; ++nu, f += s->ncell) s->ds.phi[a][nu] = f;
     s->ds.ndim = nd;
     s->ds.nx = d.nx;
@@ -727,6 +736,7 @@ int fw_run_dir(const char* sim_dir, int device) {
     d.n_relax = 2;
     d.ndim = file_exists(dir + "nZ.dat") ? 3 : 2;
     d.isotropic = file_exists(dir + "kappaw.dat") ? 0 : 1;
+    d.attenuation_model = file_exists(dir + "a_exp.dat") ? 1 : 0;
     int rc;
     if ((rc = scalar_i("nX", &d.nx)) || (rc = scalar_i("nY", &d.ny)) || (rc = scalar_i("nT", &d.nt)) ||
         (rc = scalar_i("ncoords", &d.ncoords)) || (rc = scalar_i("ncoordsout", &d.ncoordsout)) ||

@@ -37,6 +37,9 @@ struct DevSim {
     const float* coef;  // [ndmap][ndim][9], w and e already divided by the axis spacing
     int32_t ndmap;
     int32_t isotropic;
+    // exponential-attenuation model: no memory variables; v and p are scaled by aexp after
+    // their update (aexp == nullptr for the relaxation model)
+    const float* aexp;
 };
 
 // Source injection, pressure release and sensor gather of one step.

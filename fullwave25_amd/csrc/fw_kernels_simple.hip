@@ -23,7 +23,7 @@ __device__ inline bool cell_of_thread(const DevSim& s, int64_t& i, int64_t& j, i
            k < s.nz - kHalo;
 }
 
-template <int ND>
+template <int ND, bool EXPO>
 __global__ __launch_bounds__(256) void k_velocity_simple(DevSim s) {
     int64_t i, j, k;
     if (!cell_of_thread<ND>(s, i, j, k)) return;
@@ -32,6 +32,7 @@ __global__ __launch_bounds__(256) void k_velocity_simple(DevSim s) {
     const int64_t stride[3] = {s.sx, s.sy, 1};
     const float* p = s.p;
     const float rho_c = s.rho[c];
+    const float decay = EXPO ? s.aexp[c] : 1.f;
 #pragma unroll
     for (int a = 0; a < ND; ++a) {
         const float* w = cf + a * kNCoef;
@@ -47,16 +48,20 @@ __global__ __launch_bounds__(256) void k_velocity_simple(DevSim s) {
             tr += (p[c + sa + sb] - p[c + sb]) + (p[c + sa - sb] - p[c - sb]);
         }
         const float D = acc + w[8] * tr;
+        const float rho_bar = 0.5f * (rho_c + s.rho[c + sa]);
+        if (EXPO) {
+            s.v[a][c] = decay * (s.v[a][c] - s.dt / rho_bar * D);
+            continue;
+        }
         const float q1 = s.bv[a][0][c] * s.psi[a][0][c] + s.av[a][0][c] * D;
         const float q2 = s.bv[a][1][c] * s.psi[a][1][c] + s.av[a][1][c] * D;
         s.psi[a][0][c] = q1;
         s.psi[a][1][c] = q2;
-        const float rho_bar = 0.5f * (rho_c + s.rho[c + sa]);
         s.v[a][c] -= s.dt / rho_bar * (D * s.ikv[a][c] + (q1 + q2));
     }
 }
 
-template <int ND>
+template <int ND, bool EXPO>
 __global__ __launch_bounds__(256) void k_pressure_simple(DevSim s) {
     int64_t i, j, k;
     if (!cell_of_thread<ND>(s, i, j, k)) return;
@@ -80,13 +85,20 @@ __global__ __launch_bounds__(256) void k_pressure_simple(DevSim s) {
             tr += (va[c + sb] - va[c - sa + sb]) + (va[c - sb] - va[c - sa - sb]);
         }
         const float D = acc + w[8] * tr;
+        if (EXPO) {
+            div += D;
+            continue;
+        }
         const float q1 = s.bp[a][0][c] * s.phi[a][0][c] + s.ap[a][0][c] * D;
         const float q2 = s.bp[a][1][c] * s.phi[a][1][c] + s.ap[a][1][c] * D;
         s.phi[a][0][c] = q1;
         s.phi[a][1][c] = q2;
         div += D * s.ikp[a][c] + (q1 + q2);
     }
-    s.p[c] -= s.dt * s.K[c] * div;
+    if (EXPO)
+        s.p[c] = s.aexp[c] * (s.p[c] - s.dt * s.K[c] * div);
+    else
+        s.p[c] -= s.dt * s.K[c] * div;
 }
 
 // Source injection, then pressure release, then sensor gather, for step `n`. Sources that
@@ -130,18 +142,20 @@ static dim3 simple_grid(const DevSim& s) {
 }
 
 hipError_t launch_velocity_simple(const DevSim& s, hipStream_t st) {
-    if (s.ndim == 2)
-        hipLaunchKernelGGL(k_velocity_simple<2>, simple_grid(s), dim3(64, 4), 0, st, s);
-    else
-        hipLaunchKernelGGL(k_velocity_simple<3>, simple_grid(s), dim3(64, 4), 0, st, s);
+    const dim3 g = simple_grid(s), b(64, 4);
+    if (s.ndim == 2 && s.aexp) hipLaunchKernelGGL((k_velocity_simple<2, true>), g, b, 0, st, s);
+    else if (s.ndim == 2) hipLaunchKernelGGL((k_velocity_simple<2, false>), g, b, 0, st, s);
+    else if (s.aexp) hipLaunchKernelGGL((k_velocity_simple<3, true>), g, b, 0, st, s);
+    else hipLaunchKernelGGL((k_velocity_simple<3, false>), g, b, 0, st, s);
     return hipGetLastError();
 }
 
 hipError_t launch_pressure_simple(const DevSim& s, hipStream_t st) {
-    if (s.ndim == 2)
-        hipLaunchKernelGGL(k_pressure_simple<2>, simple_grid(s), dim3(64, 4), 0, st, s);
-    else
-        hipLaunchKernelGGL(k_pressure_simple<3>, simple_grid(s), dim3(64, 4), 0, st, s);
+    const dim3 g = simple_grid(s), b(64, 4);
+    if (s.ndim == 2 && s.aexp) hipLaunchKernelGGL((k_pressure_simple<2, true>), g, b, 0, st, s);
+    else if (s.ndim == 2) hipLaunchKernelGGL((k_pressure_simple<2, false>), g, b, 0, st, s);
+    else if (s.aexp) hipLaunchKernelGGL((k_pressure_simple<3, true>), g, b, 0, st, s);
+    else hipLaunchKernelGGL((k_pressure_simple<3, false>), g, b, 0, st, s);
     return hipGetLastError();
 }
 

@@ -353,6 +353,7 @@ hipError_t launch_2d_any(const DevSim& s, const LaunchCfg& cfg, bool velocity, h
 constexpr int kMaxLdsCoefSets2D = 1024;
 
 bool streaming_supported(const DevSim& s) {
+    if (s.aexp) return false;  // the exponential model runs the per-cell kernels
     if (s.ndim == 3) return true;  // large 3D tables are read from global memory
     return s.ndmap <= kMaxLdsCoefSets2D;
 }

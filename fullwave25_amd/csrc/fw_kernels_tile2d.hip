@@ -275,7 +275,7 @@ hipError_t launch_tile(const DevSim& s, bool velocity, hipStream_t st) {
 
 }  // namespace
 
-bool tile2d_supported(const DevSim& s) { return s.ndim == 2 && s.ny % 4 == 0 && s.ndmap <= 1024; }
+bool tile2d_supported(const DevSim& s) { return !s.aexp && s.ndim == 2 && s.ny % 4 == 0 && s.ndmap <= 1024; }
 
 hipError_t launch_tile2d(const DevSim& s, const LaunchCfg& cfg, bool velocity, hipStream_t st) {
     switch (cfg.chunk) {

@@ -19,6 +19,7 @@ LIB_NAME = "libfullwave_hip.so"
 LIB_PATH = Path(__file__).resolve().parent / LIB_NAME
 
 FW_OK = 0
+ABI_VERSION = 2
 STATUS_NAMES = {
     0: "FW_OK", 1: "FW_ERR_INVALID", 2: "FW_ERR_DEVICE", 3: "FW_ERR_NOMEM",
     4: "FW_ERR_IO", 5: "FW_ERR_STATE", 6: "FW_ERR_UNSUPPORTED",
@@ -51,7 +52,7 @@ class FwDesc(ctypes.Structure):
         ("dy", ctypes.c_double), ("dz", ctypes.c_double), ("dt", ctypes.c_double),
         ("c0", ctypes.c_double), ("ncoords", ctypes.c_int64), ("ntic", ctypes.c_int64),
         ("ncoordsout", ctypes.c_int64), ("modt", ctypes.c_int64), ("ncoordszero", ctypes.c_int64),
-        ("ndmap", ctypes.c_int64),
+        ("ndmap", ctypes.c_int64), ("attenuation_model", ctypes.c_int32), ("reserved", ctypes.c_int32),
     ]
 
 
@@ -90,8 +91,8 @@ def load_library(path: Path | None = None) -> ctypes.CDLL:
         lib.fw_sim_kernel_config.argtypes = [vp, i32, ctypes.POINTER(i32), ctypes.POINTER(i32), ctypes.POINTER(i32)]
         lib.fw_sim_run_timed.argtypes = [vp, i64, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(i32)]
         lib.fw_sim_destroy.argtypes = [vp]
-        if lib.fw_abi_version() != 1:
-            raise RuntimeError(f"{p}: ABI version {lib.fw_abi_version()}, expected 1")
+        if lib.fw_abi_version() != ABI_VERSION:
+            raise RuntimeError(f"{p}: ABI version {lib.fw_abi_version()}, expected {ABI_VERSION}")
         if path is None:
             _lib = lib
         return lib
@@ -125,11 +126,12 @@ def run_dir(sim_dir: Path, device: int = 0) -> None:
 
 def make_desc(inp: SimulationInputs) -> FwDesc:
     return FwDesc(
-        abi_version=1, ndim=inp.ndim, isotropic=1 if inp.isotropic else 0, n_relax=inp.n_relax,
+        abi_version=ABI_VERSION, ndim=inp.ndim, isotropic=1 if inp.isotropic else 0, n_relax=inp.n_relax,
         nx=inp.nx, ny=inp.ny, nz=inp.nz if inp.ndim == 3 else 1, nt=inp.nt,
         dx=inp.dx, dy=inp.dy, dz=inp.dz, dt=inp.dt, c0=inp.c0,
         ncoords=inp.ncoords, ntic=inp.ntic, ncoordsout=inp.ncoordsout, modt=inp.modt,
         ncoordszero=inp.ncoordszero, ndmap=inp.ndmap,
+        attenuation_model=1 if inp.exponential else 0,
     )
 
 
@@ -154,7 +156,7 @@ class HipSimulation:
 
     def _upload_all(self) -> None:
         inp = self.inp
-        for name in ["K", "rho", *inp.relaxation_map_stems()]:
+        for name in ["K", "rho", *inp.attenuation_map_stems()]:
             self.upload(name, inp.maps[name].astype(np.float32, copy=False))
         self.upload("dcmap", inp.dcmap.astype(np.int32, copy=False))
         self.upload("dmap", inp.dmap.astype(np.float32, copy=False))

@@ -61,6 +61,7 @@ class SimulationInputs:
     icczero: NDArray[np.int32] | None = None  # [nzero, ndim]
     modt: int = 1
     n_relax: int = 2
+    exponential: bool = False  # exponential-attenuation model: one "a_exp" map, no relaxation maps
 
     @property
     def shape(self) -> tuple:
@@ -97,9 +98,13 @@ class SimulationInputs:
     def relaxation_map_stems(self) -> list[str]:
         return [engine_file_stem(n) for n in fw2_map_names(is_3d=self.ndim == 3, isotropic=self.isotropic, n_relax=self.n_relax)]
 
+    def attenuation_map_stems(self) -> list[str]:
+        """The attenuation maps the engine reads for this model."""
+        return ["a_exp"] if self.exponential else self.relaxation_map_stems()
+
     def validate(self) -> None:
         shape = self.shape
-        for name in ["K", "rho", *self.relaxation_map_stems()]:
+        for name in ["K", "rho", *self.attenuation_map_stems()]:
             if name not in self.maps:
                 raise ValueError(f"missing map '{name}'")
             if self.maps[name].shape != shape:
@@ -147,7 +152,7 @@ class SimulationInputs:
         files["d"] = self.d.astype(_F32)
         files["dmap"] = self.dmap.astype(_F32)
         files["dcmap"] = self.dcmap.astype(np.int32)
-        for stem in self.relaxation_map_stems():
+        for stem in self.attenuation_map_stems():
             files[stem] = self.maps[stem].astype(_F32)
         for k, arr in files.items():
             arr.tofile(target / f"{k}.dat")
@@ -169,6 +174,7 @@ class SimulationInputs:
 
         ndim = 3 if (sim_dir / "nZ.dat").exists() else 2
         iso = not (sim_dir / "kappaw.dat").exists()
+        expo = (sim_dir / "a_exp.dat").exists()
         nx, ny, nt = int(rd("nX", np.int32)[0]), int(rd("nY", np.int32)[0]), int(rd("nT", np.int32)[0])
         nz = int(rd("nZ", np.int32)[0]) if ndim == 3 else 1
         shape = (nx, ny, nz) if ndim == 3 else (nx, ny)
@@ -181,12 +187,12 @@ class SimulationInputs:
             dx=float(rd("dX", _F32)[0]), dy=float(rd("dY", _F32)[0]),
             dz=float(rd("dZ", _F32)[0]) if ndim == 3 else 0.0,
             dt=float(rd("dT", _F32)[0]), c0=float(rd("c0", _F32)[0]), isotropic=iso,
-            modt=int(rd("modT", np.int32)[0]),
+            modt=int(rd("modT", np.int32)[0]), exponential=expo,
         )
         for name in ("K", "rho", "c", "beta"):
             if (sim_dir / f"{name}.dat").exists():
                 inp.maps[name] = rd(name, _F32).reshape(shape)
-        for stem in inp.relaxation_map_stems():
+        for stem in inp.attenuation_map_stems():
             inp.maps[stem] = rd(stem, _F32).reshape(shape)
         inp.dcmap = rd("dcmap", np.int32).reshape(shape)
         inp.d = rd("d", _F32).reshape(9, 2)
@@ -203,26 +209,31 @@ def build_simulation_inputs(grid, medium, source, sensor, *, isotropic: bool = T
     """Assemble the contract from padded front-end objects.
 
     ``grid``, ``medium`` (a :class:`~fullwave25_amd.medium.MediumRelaxationMaps` with the PML
-    applied), ``source`` and ``sensor`` are the padded objects of a ``PMLBuilder``; this does
+    applied, or a :class:`~fullwave25_amd.medium.MediumExponentialAttenuation`), ``source`` and
+    ``sensor`` are the padded objects of a ``PMLBuilder``; this does
     what ``InputFileWriter.__init__`` and ``_save_variables_into_dat_file`` do
     (fullwave/solver/input_file_writer.py:26-103, 563-627) without touching the disk.
     """
     is_3d = grid.is_3d
     c = np.asarray(medium.sound_speed, dtype=np.float64)
+    exponential = hasattr(medium, "alpha_exp")
     inp = SimulationInputs(
         ndim=3 if is_3d else 2,
         nx=grid.nx, ny=grid.ny, nz=grid.nz if is_3d else 1, nt=grid.nt,
         dx=_f32(grid.dx), dy=_f32(grid.dy), dz=_f32(grid.dz) if is_3d else 0.0,
         dt=_f32(grid.dt), c0=_f32(grid.c0), isotropic=isotropic,
-        modt=int(sensor.sampling_modulus_time), n_relax=n_relax,
+        modt=int(sensor.sampling_modulus_time), n_relax=n_relax, exponential=exponential,
     )
     inp.maps["c"] = c.astype(_F32)
     inp.maps["K"] = np.asarray(medium.bulk_modulus).astype(_F32)
     inp.maps["rho"] = np.asarray(medium.density).astype(_F32)
     inp.maps["beta"] = np.asarray(medium.beta).astype(_F32)
-    fw2 = medium.relaxation_param_dict_for_fw2
-    for name in fw2_map_names(is_3d=is_3d, isotropic=isotropic, n_relax=n_relax):
-        inp.maps[engine_file_stem(name)] = np.asarray(fw2[name]).astype(_F32)
+    if exponential:
+        inp.maps["a_exp"] = np.asarray(medium.alpha_exp).astype(_F32)
+    else:
+        fw2 = medium.relaxation_param_dict_for_fw2
+        for name in fw2_map_names(is_3d=is_3d, isotropic=isotropic, n_relax=n_relax):
+            inp.maps[engine_file_stem(name)] = np.asarray(fw2[name]).astype(_F32)
     inp.d = coefficients.d_matrix(grid.cfl, is_3d=is_3d).astype(_F32)
     inp.dmap = coefficients.d_map(c, grid.dt, grid.dx, is_3d=is_3d).astype(_F32)
     if coefficients.ndmap_value(c) == 1:

@@ -320,14 +320,40 @@ def _shifted_sensor(sensor: Sensor, nb: int) -> Sensor:
 
 
 class PMLBuilderExponentialAttenuation(PMLBuilder):
-    """Padding for the exponential-attenuation model: no transition layer
-    (fullwave/solver/pml_builder.py:1652-1935)."""
+    """Padding and absorbing layer for the exponential-attenuation model
+    (fullwave/solver/pml_builder.py:1652-1935).
+
+    The border is ``n_pml + m_spatial_order`` cells (no transition layer); maps, and here also
+    the air map, are padded by edge replication. ``run`` turns alpha0 into the per-step decay
+    factor ``exp(-dt alpha0/2 f0 c0 / (1e-2 * 10 log10 e))`` (``Medium.build_exponential``) and
+    scales it by ``1 - m``, where ``m`` is the Euclidean distance into the border, counted from
+    1 at the first border cell, normalized to 1 at the grid corner.
+    """
 
     def __init__(self, grid, medium, source, sensor, *, m_spatial_order: int = 8, n_pml_layer: int = 40):
         if not isinstance(medium, Medium):
             raise TypeError("medium must be a Medium")
         super().__init__(grid, medium, source, sensor, m_spatial_order=m_spatial_order,
                          n_pml_layer=n_pml_layer, n_transition_layer=0)
+        # the reference pads the air map of this model like a medium map
+        self.extended_medium.air_map = self._extend_map_for_pml(medium.air_map)
+
+    @staticmethod
+    def _mask_body(shape, n_body: int) -> NDArray[np.float64]:
+        dist2 = 0.0
+        for axis, n in enumerate(shape):
+            x = np.arange(1, n + 1, dtype=np.float64)
+            r = np.where(x <= n_body, n_body - x + 1, np.where(x > n - n_body, x - (n - n_body), 0))
+            view = [1] * len(shape)
+            view[axis] = -1
+            dist2 = dist2 + (r**2).reshape(view)
+        mask = np.sqrt(np.broadcast_to(dist2, shape))
+        if mask.max() > 0:
+            mask = mask / mask.max()
+        return mask
 
     def run(self, *, use_pml: bool = True) -> MediumExponentialAttenuation:
-        raise NotImplementedError("the exponential-attenuation engine variant is not built yet")
+        built = self.extended_medium.build_exponential()
+        if use_pml:
+            built.alpha_exp = built.alpha_exp * (1 - self._mask_body(built.alpha_exp.shape, self.num_boundary_points))
+        return built

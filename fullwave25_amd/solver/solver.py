@@ -62,8 +62,6 @@ class Solver:
     ) -> None:
         if m_spatial_order != 8:
             raise ValueError("the engine's stencil half-width is 8 (m_spatial_order=8)")
-        if use_exponential_attenuation:
-            raise NotImplementedError("the exponential-attenuation engine variant is not built yet")
         if run_on_memory:
             self._tempdir = tempfile.TemporaryDirectory()
             self.work_dir = Path(self._tempdir.name)
@@ -76,11 +74,13 @@ class Solver:
         self.use_exponential_attenuation = use_exponential_attenuation
         self.use_isotropic_relaxation = use_isotropic_relaxation
         self.write_input_files = write_input_files
-        self.n_relax_mechanisms = medium.n_relaxation_mechanisms
+        self.n_relax_mechanisms = getattr(medium, "n_relaxation_mechanisms", 2)
         self._check_input(grid, medium, source, sensor, transducer)
 
         self.medium = medium
-        if use_isotropic_relaxation != medium.use_isotropic_relaxation:
+        if use_exponential_attenuation and not isinstance(medium, Medium):
+            raise TypeError("the exponential-attenuation model needs a Medium (alpha_coeff)")
+        if not use_exponential_attenuation and use_isotropic_relaxation != medium.use_isotropic_relaxation:
             logger.warning(
                 "Solver is set to use %s relaxation, but the provided medium is using %s "
                 "relaxation. Overriding the medium.",
@@ -104,11 +104,17 @@ class Solver:
         self.path_fullwave_simulation_bin = path_fullwave_simulation_bin
         self.cuda_device_id = cuda_device_id
         self.fullwave_launcher = HipLauncher(is_3d=self.is_3d, use_gpu=use_gpu, cuda_device_id=cuda_device_id)
-        self.pml_builder = PMLBuilder(
-            grid=self.grid, medium=self.medium, source=self.source, sensor=self.sensor,
-            m_spatial_order=m_spatial_order, n_pml_layer=pml_layer_thickness_px,
-            n_transition_layer=n_transition_layer, use_isotropic_relaxation=use_isotropic_relaxation,
-        )
+        if use_exponential_attenuation:
+            self.pml_builder = PMLBuilderExponentialAttenuation(
+                grid=self.grid, medium=self.medium, source=self.source, sensor=self.sensor,
+                m_spatial_order=m_spatial_order, n_pml_layer=pml_layer_thickness_px,
+            )
+        else:
+            self.pml_builder = PMLBuilder(
+                grid=self.grid, medium=self.medium, source=self.source, sensor=self.sensor,
+                m_spatial_order=m_spatial_order, n_pml_layer=pml_layer_thickness_px,
+                n_transition_layer=n_transition_layer, use_isotropic_relaxation=use_isotropic_relaxation,
+            )
         if self.medium.beta is not None and np.any(np.asarray(self.medium.beta) != 0):
             logger.warning("beta (nonlinearity) is non-zero; this engine solves the linear equations and ignores it")
 
@@ -228,4 +234,4 @@ class Solver:
     __repr__ = __str__
 
 
-__all__ = ["PMLBuilderExponentialAttenuation", "Solver"]
+__all__ = ["Solver"]

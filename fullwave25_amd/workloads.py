@@ -42,10 +42,11 @@ def _grid(nx: int, ny: int, *, f0=3e6, c0=1540.0, ppw=12, cfl=0.4, nt=2000, nz: 
 
 
 def _inputs(grid, medium, source, sensor, *, n_pml: int, n_transition: int | None = None,
-            isotropic: bool = True):
+            isotropic: bool = True, exponential: bool = False):
     with tempfile.TemporaryDirectory() as tmp:
         solver = Solver(Path(tmp), grid, medium, source, sensor, pml_layer_thickness_px=n_pml,
-                        n_transition_layer=n_transition, use_isotropic_relaxation=isotropic)
+                        n_transition_layer=n_transition, use_isotropic_relaxation=isotropic,
+                        use_exponential_attenuation=exponential)
         inputs, _ = solver.build_inputs()
     return inputs
 
@@ -164,9 +165,11 @@ def element_traces(traces: np.ndarray, workload: Workload) -> np.ndarray:
 
 
 def random_medium_2d(nx: int, ny: int, nt: int, *, seed: int = 0, n_pml: int = 6, n_transition: int = 4,
-                     isotropic: bool = True, n_src: int = 5, n_sensors: int = 17, air: int = 0) -> Workload:
+                     isotropic: bool = True, n_src: int = 5, n_sensors: int = 17, air: int = 0,
+                     exponential: bool = False) -> Workload:
     """Small seeded heterogeneous 2D problem for parity checks: random smooth c/rho/alpha/gamma,
-    a few point sources with distinct pulses, scattered sensors and optional air points."""
+    a few point sources with distinct pulses, scattered sensors and optional air points
+    (``exponential``: the exponential-attenuation model instead of relaxation)."""
     grid = _grid(nx, ny, nt=nt)
     shape = (grid.nx, grid.ny)
     rng = np.random.default_rng(seed)
@@ -196,12 +199,13 @@ def random_medium_2d(nx: int, ny: int, nt: int, *, seed: int = 0, n_pml: int = 6
     smask.reshape(-1)[rng.choice(nx * ny, size=n_sensors, replace=False)] = True
     sensor = Sensor(smask, sampling_modulus_time=1)
     return Workload("random_medium_2d", _inputs(grid, medium, source, sensor, n_pml=n_pml,
-                                                n_transition=n_transition, isotropic=isotropic))
+                                                n_transition=n_transition, isotropic=isotropic,
+                                                exponential=exponential))
 
 
 def random_medium_3d(nx: int, ny: int, nz: int, nt: int, *, seed: int = 0, n_pml: int = 4,
                      n_transition: int = 3, isotropic: bool = True, n_src: int = 4,
-                     n_sensors: int = 13, air: int = 0) -> Workload:
+                     n_sensors: int = 13, air: int = 0, exponential: bool = False) -> Workload:
     """Small seeded heterogeneous 3D problem for parity checks."""
     grid = _grid(nx, ny, nt=nt, nz=nz)
     shape = (grid.nx, grid.ny, grid.nz)
@@ -229,4 +233,5 @@ def random_medium_3d(nx: int, ny: int, nz: int, nt: int, *, seed: int = 0, n_pml
     smask.reshape(-1)[rng.choice(n, size=n_sensors, replace=False)] = True
     sensor = Sensor(smask)
     return Workload("random_medium_3d", _inputs(grid, medium, source, sensor, n_pml=n_pml,
-                                                n_transition=n_transition, isotropic=isotropic))
+                                                n_transition=n_transition, isotropic=isotropic,
+                                                exponential=exponential))

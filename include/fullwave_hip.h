@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define FW_ABI_VERSION 1
+#define FW_ABI_VERSION 2
 
 typedef enum fw_status {
     FW_OK = 0,
@@ -55,6 +55,11 @@ typedef struct fw_desc {
     int64_t modt;        /* record every modt steps (modT) */
     int64_t ncoordszero; /* pressure-release points (ncoordszero) */
     int64_t ndmap;       /* coefficient sets in dmap (ndmap) */
+    /* 0: multi-relaxation power-law attenuation (kappa, apml/bpml maps);
+     * 1: exponential attenuation, one per-step decay factor per voxel ("a_exp", the reference's
+     *    fullwave2_*_exponential_attenuation engines, fullwave/solver/solver.py:193-218) */
+    int32_t attenuation_model;
+    int32_t reserved;
 } fw_desc;
 
 typedef struct fw_sim fw_sim;
@@ -85,7 +90,8 @@ int fw_sim_create(const fw_desc* desc, int device, fw_sim** out);
  * "dmap" (float32 [9, 2, ndmap]), "icc", "outc", "icczero" (int32 [n, ndim]),
  * "icmat" (float32 [ncoords, ntic]), and the relaxation maps "kappau", "kappax",
  * "apmlu1", "bpmlu1", ... (isotropic) or additionally "kappaw", "kappay", "kappav", "kappaz",
- * "apmlw1", ... (anisotropic). Per-voxel maps are float32 [nx, ny(, nz)]. `bytes` must match. */
+ * "apmlw1", ... (anisotropic), or "a_exp" (exponential model). Per-voxel maps are float32
+ * [nx, ny(, nz)]. `bytes` must match. */
 int fw_sim_upload(fw_sim* sim, const char* name, const void* host, size_t bytes);
 /* Advance n_steps time steps (synchronous). The first call checks that every input is there. */
 int fw_sim_run(fw_sim* sim, int64_t n_steps);

@@ -41,6 +41,7 @@ class _Problem(ctypes.Structure):
         ("ncoords", ctypes.c_int64), ("icc", ctypes.POINTER(ctypes.c_int64)), ("icmat", _P), ("ntic", ctypes.c_int64),
         ("nzero", ctypes.c_int64), ("zero", ctypes.POINTER(ctypes.c_int64)),
         ("nout", ctypes.c_int64), ("outc", ctypes.POINTER(ctypes.c_int64)), ("modt", ctypes.c_int64),
+        ("aexp", _P),
     ]
 
 
@@ -115,7 +116,9 @@ class OracleSimulation:
         P.nx, P.ny, P.nz = inp.nx, inp.ny, inp.nz if inp.ndim == 3 else 1
         P.dx, P.dy, P.dz, P.dt = inp.dx, inp.dy, inp.dz, inp.dt
         P.K, P.rho = dbl(inp.maps["K"]), dbl(inp.maps["rho"])
-        for a in range(inp.ndim):
+        if getattr(inp, "exponential", False):
+            P.aexp = dbl(inp.maps["a_exp"])
+        for a in range(inp.ndim if not getattr(inp, "exponential", False) else 0):
             va = "u" if inp.isotropic else _V_AXES[a]
             pa = "x" if inp.isotropic else _P_AXES[a]
             P.kappa_v[a] = dbl(inp.maps[f"kappa{va}"])
@@ -208,6 +211,7 @@ def brute_force_run(inp, n_steps: int) -> tuple[np.ndarray, np.ndarray]:
         return [sum(o[a] for o in offs) for a in range(nd)]
 
     m = {k: np.asarray(v, dtype=np.float64) for k, v in inp.maps.items()}
+    expo = getattr(inp, "exponential", False)
     p = np.zeros(shape)
     v = [np.zeros(shape) for _ in range(nd)]
     psi = [[np.zeros(shape) for _ in range(2)] for _ in range(nd)]
@@ -229,12 +233,15 @@ def brute_force_run(inp, n_steps: int) -> tuple[np.ndarray, np.ndarray]:
                     eb = unit(b, s)
                     tr = tr + (sh(p, add(ea, eb)) - sh(p, eb))
             D = (acc + w[..., 8] * tr) / h[a]
+            rho_bar = 0.5 * (m["rho"][inner] + sh(m["rho"], ea))
+            if expo:
+                v[a][inner] = m["a_exp"][inner] * (v[a][inner] - inp.dt / rho_bar * D)
+                continue
             mem = 0.0
             for nu in range(2):
                 q = m[f"bpml{va}{nu + 1}"][inner] * psi[a][nu][inner] + m[f"apml{va}{nu + 1}"][inner] * D
                 psi[a][nu][inner] = q
                 mem = mem + q
-            rho_bar = 0.5 * (m["rho"][inner] + sh(m["rho"], ea))
             v[a][inner] -= inp.dt / rho_bar * (D / m[f"kappa{va}"][inner] + mem)
         div = 0.0
         for a in range(nd):
@@ -249,13 +256,19 @@ def brute_force_run(inp, n_steps: int) -> tuple[np.ndarray, np.ndarray]:
                     eb = unit(b, s)
                     tr = tr + (sh(v[a], eb) - sh(v[a], add(unit(a, -1), eb)))
             D = (acc + w[..., 8] * tr) / h[a]
+            if expo:
+                div = div + D
+                continue
             mem = 0.0
             for nu in range(2):
                 q = m[f"bpml{pa}{nu + 1}"][inner] * phi[a][nu][inner] + m[f"apml{pa}{nu + 1}"][inner] * D
                 phi[a][nu][inner] = q
                 mem = mem + q
             div = div + D / m[f"kappa{pa}"][inner] + mem
-        p[inner] -= inp.dt * m["K"][inner] * div
+        if expo:
+            p[inner] = m["a_exp"][inner] * (p[inner] - inp.dt * m["K"][inner] * div)
+        else:
+            p[inner] -= inp.dt * m["K"][inner] * div
         pf = p.reshape(-1)
         if n < inp.ntic:
             pf[icc] = np.asarray(inp.icmat, dtype=np.float64)[:, n]

@@ -22,6 +22,13 @@
  *   then p[source s] <- icmat[s][n] (n < nTic), p[air] <- 0, and if n % modT == 0 the
  *   sensors record p.
  *
+ * Exponential-attenuation model (a_exp.dat, fullwave/solver/input_file_writer.py:629-644): no
+ * memory variables, kappa = 1, and both updates are scaled by the voxel's decay factor A:
+ *     v_a <- A (v_a - dt / rho_bar_a D+_a p),   p <- A (p - dt K sum_a D-_a v_a).
+ * Scaling both fields makes the leapfrog's per-step amplification matrix have determinant A^2,
+ * so every mode decays by exactly A per step; A = exp(-dt alpha(f0) c0) with alpha in Np/m
+ * (fullwave/medium.py:1068-1090) is then a frequency-independent attenuation alpha(f0).
+ *
  * w_1..w_8, e come from the per-voxel coefficient set dcmap[c] (dmap table); a staggered point
  * c + 1/2 e_a uses voxel c's set and maps. rho_bar_a = (rho[c] + rho[c + e_a]) / 2. Only cells
  * with every coordinate in [8, n - 8) are updated; the 8-cell ghost border stays zero.
@@ -57,6 +64,8 @@ typedef struct {
     int64_t nout;
     const int64_t* outc;
     int64_t modt;
+    const double* aexp; /* exponential-attenuation model: per-voxel decay factor; NULL for the
+                           relaxation model (then kappa/a/b are used and aexp is not) */
 } fwo_problem;
 
 #define REAL double
@@ -71,4 +80,4 @@ typedef struct {
 #undef REAL
 #undef SFX
 
-int fwo_abi(void) { return 1; }
+int fwo_abi(void) { return 2; }

@@ -23,9 +23,11 @@ typedef struct {
     REAL* ap[3][2];
     REAL* bp[3][2];
     REAL* coef;
+    REAL* aexp; /* NULL unless the exponential-attenuation model */
 } SFX(fwo_state);
 
 static REAL* SFX(fwo_conv)(const double* src, int64_t n) {
+    if (!src) return 0;
     REAL* dst = (REAL*)malloc(sizeof(REAL) * (size_t)n);
     for (int64_t i = 0; i < n; ++i) dst[i] = (REAL)src[i];
     return dst;
@@ -61,6 +63,7 @@ void* SFX(fwo_create)(const fwo_problem* P) {
     st->K = SFX(fwo_conv)(P->K, st->n);
     st->rho = SFX(fwo_conv)(P->rho, st->n);
     st->coef = SFX(fwo_conv)(P->coef, P->ndmap * 9);
+    st->aexp = SFX(fwo_conv)(P->aexp, st->n);
     return st;
 }
 
@@ -84,6 +87,7 @@ void SFX(fwo_destroy)(void* handle) {
     free(st->K);
     free(st->rho);
     free(st->coef);
+    free(st->aexp);
     free(st);
 }
 
@@ -117,13 +121,17 @@ static void SFX(fwo_velocity)(SFX(fwo_state)* st) {
                         tr += (st->p[c + sa + sb] - st->p[c + sb]) + (st->p[c + sa - sb] - st->p[c - sb]);
                     }
                     const REAL D = (acc + w[8] * tr) / st->h[a];
+                    const REAL rho_bar = (REAL)0.5 * (st->rho[c] + st->rho[c + sa]);
+                    if (st->aexp) {
+                        st->v[a][c] = st->aexp[c] * (st->v[a][c] - st->dt / rho_bar * D);
+                        continue;
+                    }
                     REAL mem = 0;
                     for (int nu = 0; nu < 2; ++nu) {
                         REAL q = st->bv[a][nu][c] * st->psi[a][nu][c] + st->av[a][nu][c] * D;
                         st->psi[a][nu][c] = q;
                         mem += q;
                     }
-                    const REAL rho_bar = (REAL)0.5 * (st->rho[c] + st->rho[c + sa]);
                     st->v[a][c] -= st->dt / rho_bar * (D / st->kv[a][c] + mem);
                 }
             }
@@ -156,6 +164,10 @@ static void SFX(fwo_pressure)(SFX(fwo_state)* st) {
                         tr += (va[c + sb] - va[c - sa + sb]) + (va[c - sb] - va[c - sa - sb]);
                     }
                     const REAL D = (acc + w[8] * tr) / st->h[a];
+                    if (st->aexp) {
+                        div += D;
+                        continue;
+                    }
                     REAL mem = 0;
                     for (int nu = 0; nu < 2; ++nu) {
                         REAL q = st->bp[a][nu][c] * st->phi[a][nu][c] + st->ap[a][nu][c] * D;
@@ -164,7 +176,10 @@ static void SFX(fwo_pressure)(SFX(fwo_state)* st) {
                     }
                     div += D / st->kp[a][c] + mem;
                 }
-                st->p[c] -= st->dt * st->K[c] * div;
+                if (st->aexp)
+                    st->p[c] = st->aexp[c] * (st->p[c] - st->dt * st->K[c] * div);
+                else
+                    st->p[c] -= st->dt * st->K[c] * div;
             }
         }
     }

@@ -8,7 +8,8 @@ container only. The reference is imported from /root/reference with a stand-in f
 (OpenCV is absent; the front end uses it only for plotting). Its engine is not involved.
 
 Output: tests/golden/frontend_golden.npz holding, per case, the inputs given to the front end and
-every file the reference's InputFileWriter writes (``<case>/<file stem>``), plus pulse samples.
+every file the reference's InputFileWriter writes (``<case>/<file stem>``) for isotropic and
+per-axis relaxation and for the exponential-attenuation model, plus pulse samples.
 
 Usage: python tests/golden/make_frontend_golden.py
 """
@@ -58,6 +59,13 @@ def cases():
             "src": src, "sen": sen, "p0": p0, "nt": np.array(nt), "modT": np.array(2),
             "n_pml": np.array(npml), "n_transition": np.array(ntr), **{f"relax/{k}": v for k, v in relax.items()},
         }
+    # power-law maps for the exponential-attenuation model (a separate stream, so the draws
+    # above stay as they were)
+    rng_exp = np.random.default_rng(20251203)
+    for cs in out.values():
+        shape = tuple(int(v) for v in cs["shape"])
+        cs["alpha_coeff"] = 0.2 + 1.3 * rng_exp.random(shape)
+        cs["alpha_power"] = 0.8 + 0.8 * rng_exp.random(shape)
     return out
 
 
@@ -73,7 +81,7 @@ def main() -> None:
     sys.path.insert(0, str(REF))
     import fullwave  # noqa: E402  (the reference)
     from fullwave.solver.input_file_writer import InputFileWriter
-    from fullwave.solver.pml_builder import PMLBuilder
+    from fullwave.solver.pml_builder import PMLBuilder, PMLBuilderExponentialAttenuation
 
     data = {}
     for name, cs in cases().items():
@@ -102,6 +110,30 @@ def main() -> None:
                 sim = writer.run("txrx_0")
                 for f in sorted(sim.glob("*.dat")):
                     data[f"{tag}/{f.stem}"] = np.fromfile(f, dtype=np.uint8)
+        # exponential-attenuation model: its own padding (no transition layer), the air map
+        # padded like a medium map, and the a_exp map with the radial absorbing mask
+        tag = f"{name}_exp"
+        with tempfile.TemporaryDirectory() as tmp:
+            tmp = Path(tmp)
+            # Medium checks that a relaxation database path exists; the exponential model never
+            # reads it, so an empty file satisfies the check
+            no_db = tmp / "unused_database.mat"
+            no_db.write_bytes(b"")
+            grid = fullwave.Grid(**grid_args(shape, nt))
+            medium = fullwave.Medium(grid, cs["c"], cs["rho"], cs["alpha_coeff"], cs["alpha_power"], cs["beta"],
+                                     air_map=cs["air"], path_relaxation_parameters_database=no_db)
+            source = fullwave.Source(cs["p0"], cs["src"])
+            sensor = fullwave.Sensor(cs["sen"], sampling_modulus_time=int(cs["modT"]))
+            pml = PMLBuilderExponentialAttenuation(grid, medium, source, sensor, m_spatial_order=8,
+                                                   n_pml_layer=int(cs["n_pml"]))
+            ext = pml.run(use_pml=True)
+            fake_bin = tmp / "engine_stand_in"
+            fake_bin.write_bytes(b"")
+            writer = InputFileWriter(tmp, pml.extended_grid, ext, pml.extended_source, pml.extended_sensor,
+                                     path_fullwave_simulation_bin=fake_bin, use_exponential_attenuation=True)
+            sim = writer.run("txrx_0")
+            for f in sorted(sim.glob("*.dat")):
+                data[f"{tag}/{f.stem}"] = np.fromfile(f, dtype=np.uint8)
     from fullwave.utils.pulse import gaussian_modulated_sinusoidal_signal
 
     for i, (nt, dur, nc, do, f0, p0, dl, il) in enumerate([

@@ -11,11 +11,12 @@ from fullwave25_amd.solver.inputs import SimulationInputs
 
 def raw_inputs(shape, *, c=1540.0, rho=1000.0, dx=1540.0 / 3e6 / 12, cfl=0.4, nt=100,
                relax=None, icc=None, icmat=None, outc=None, icczero=None, modt=1,
-               isotropic=True, rng=None) -> SimulationInputs:
+               isotropic=True, rng=None, a_exp=None) -> SimulationInputs:
     """Engine inputs on ``shape`` with maps given as scalars or arrays.
 
     ``relax=None`` is lossless (kappa = 1, d = 0); otherwise a dict with per-side
-    ``kappa``, ``d`` (2 values) and ``alpha`` (2 values) applied uniformly.
+    ``kappa``, ``d`` (2 values) and ``alpha`` (2 values) applied uniformly. ``a_exp`` (scalar
+    or array) selects the exponential-attenuation model with that per-step decay map instead.
     """
     ndim = len(shape)
     c = np.broadcast_to(np.asarray(c, dtype=np.float64), shape).copy()
@@ -30,6 +31,10 @@ def raw_inputs(shape, *, c=1540.0, rho=1000.0, dx=1540.0 / 3e6 / 12, cfl=0.4, nt
     inp.maps["rho"] = rho.astype(np.float32)
     vax = ["u"] if isotropic else ["u", "w", "v"][:ndim]
     pax = ["x"] if isotropic else ["x", "y", "z"][:ndim]
+    if a_exp is not None:
+        inp.exponential = True
+        inp.maps["a_exp"] = np.broadcast_to(np.asarray(a_exp, np.float32), shape).copy()
+        vax, pax = [], []
     for side, axes in (("v", vax), ("p", pax)):
         r = (relax or {}).get(side, {"kappa": 1.0, "d": (0.0, 0.0), "alpha": (1e6, 2e6)})
         for ax in axes:

@@ -32,13 +32,13 @@ def test_exports_every_declared_symbol():
 
 def test_abi_version_and_string():
     lib = engine.load_library()
-    assert lib.fw_abi_version() == 1
+    assert lib.fw_abi_version() == engine.ABI_VERSION
     assert b"gfx950" in lib.fw_version_string()
 
 
 def test_invalid_descriptor_rejected_before_device():
     lib = engine.load_library()
-    desc = engine.FwDesc(abi_version=1, ndim=4, isotropic=1, n_relax=2, nx=32, ny=32, nz=1, nt=1,
+    desc = engine.FwDesc(abi_version=engine.ABI_VERSION, ndim=4, isotropic=1, n_relax=2, nx=32, ny=32, nz=1, nt=1,
                          dx=1e-4, dy=1e-4, dz=0, dt=1e-8, c0=1540, modt=1, ndmap=1)
     h = ctypes.c_void_p()
     rc = lib.fw_sim_create(ctypes.byref(desc), 0, ctypes.byref(h))

@@ -142,15 +142,21 @@ def test_table_save_load_roundtrip(tmp_path):
 
 
 def test_fit_meets_power_law_in_model():
-    """The FP64 fit reproduces alpha0 f^gamma in the engine's discrete-time model to 4 % over
-    the band for mid-range parameters (its worst case at gamma 1.6 is a few percent)."""
+    """The FP64 fit reproduces alpha0 f^gamma in the engine's discrete-time model inside the
+    band (the fit range without its margins): 2 % for mid-range parameters, 6 % at gamma 1.6
+    with a strong alpha0 (kappa >= 1 limits how steep the model can rise)."""
     from fullwave25_amd import relaxation as R
 
     dt = 0.4 * (1540 / 3e6 / 12) / 1540
     band = R.fit_band_for_grid(1540 / 3e6 / 12)
-    for a0, g in ((0.5, 1.1), (0.3, 0.8), (1.5, 1.0)):
-        p, err = R.fit_relaxation_parameters(a0, g, dt=dt, band=band)
-        assert err < 0.04, (a0, g, err)
+    f = np.geomspace(band[0] * R.FIT_MARGIN, band[1] / R.FIT_MARGIN, 60)
+    w = 2 * np.pi * f
+    for a0, g, tol in ((0.5, 1.1, 0.02), (0.3, 0.8, 0.02), (1.5, 1.0, 0.02), (1.5, 1.6, 0.06)):
+        p, _ = R.fit_relaxation_parameters(a0, g, dt=dt, band=band)
+        s1 = R.stretch_inverse_discrete(w, p[0], p[2::4], p[3::4], dt)
+        s2 = R.stretch_inverse_discrete(w, p[1], p[4::4], p[5::4], dt)
+        err = np.abs(np.imag((w / R.C_REF) / np.sqrt(s1 * s2)) / R.target_attenuation(f, a0, g) - 1).max()
+        assert err < tol, (a0, g, err)
         assert p[0] >= 1.0 and p[1] >= 1.0
 
 

@@ -2,8 +2,10 @@
 tests/golden/make_frontend_golden.py from the reference front end).
 
 Every ``.dat`` file the reference's InputFileWriter writes for a 2D and a 3D case, isotropic and
-per-axis, is compared with what this package's Solver path produces: padding, absorbing-layer
-ramps, relaxation maps, coefficient tables, coordinate lists, scalars and the source matrix.
+per-axis relaxation and the exponential-attenuation model, is compared with what this package's
+Solver path produces: padding, absorbing-layer ramps, relaxation maps, the exponential model's
+decay map with its radial absorbing mask, coefficient tables, coordinate lists, scalars and the
+source matrix.
 Every file must match byte for byte.
 """
 
@@ -14,10 +16,10 @@ import numpy as np
 import pytest
 
 from fullwave25_amd.grid import Grid
-from fullwave25_amd.medium import MediumRelaxationMaps
+from fullwave25_amd.medium import Medium, MediumRelaxationMaps
 from fullwave25_amd.sensor import Sensor
 from fullwave25_amd.solver.inputs import build_simulation_inputs
-from fullwave25_amd.solver.pml_builder import PMLBuilder
+from fullwave25_amd.solver.pml_builder import PMLBuilder, PMLBuilderExponentialAttenuation
 from fullwave25_amd.source import Source
 from fullwave25_amd.utils.pulse import gaussian_modulated_sinusoidal_signal
 
@@ -55,10 +57,25 @@ def _our_files(gold, name, iso):
         return {f.stem: np.fromfile(f, dtype=np.uint8) for f in Path(tmp).glob("*.dat")}
 
 
-@pytest.mark.parametrize("case", ["2d_iso", "2d_aniso", "3d_iso", "3d_aniso"])
+def _our_files_exp(gold, name):
+    g = {k[len(name) + 4:]: v for k, v in gold.items() if k.startswith(f"{name}/in/")}
+    shape = tuple(int(v) for v in g["shape"])
+    grid = _grid(shape, int(g["nt"]))
+    medium = Medium(grid, g["c"], g["rho"], g["alpha_coeff"], g["alpha_power"], g["beta"], air_map=g["air"])
+    pml = PMLBuilderExponentialAttenuation(grid, medium, Source(g["p0"], g["src"]), Sensor(g["sen"], int(g["modT"])),
+                                           m_spatial_order=8, n_pml_layer=int(g["n_pml"]))
+    ext = pml.run(use_pml=True)
+    inp = build_simulation_inputs(pml.extended_grid, ext, pml.extended_source, pml.extended_sensor)
+    assert inp.exponential
+    with tempfile.TemporaryDirectory() as tmp:
+        inp.write_dir(Path(tmp))
+        return {f.stem: np.fromfile(f, dtype=np.uint8) for f in Path(tmp).glob("*.dat")}
+
+
+@pytest.mark.parametrize("case", ["2d_iso", "2d_aniso", "3d_iso", "3d_aniso", "2d_exp", "3d_exp"])
 def test_engine_inputs_match_reference(gold, case):
     name, kind = case.split("_")
-    ours = _our_files(gold, name, kind == "iso")
+    ours = _our_files_exp(gold, name) if kind == "exp" else _our_files(gold, name, kind == "iso")
     theirs = {k[len(case) + 1:]: v for k, v in gold.items() if k.startswith(case + "/")}
     assert theirs, "no golden files"
     missing = sorted(set(theirs) - set(ours))

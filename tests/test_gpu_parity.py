@@ -35,6 +35,8 @@ CASES = {
     "2d_ragged": lambda: workloads.random_medium_2d(37, 131, 150, seed=13, n_src=9, n_sensors=40).inputs,
     "3d_iso": lambda: workloads.random_medium_3d(20, 18, 22, 120, seed=14, air=4).inputs,
     "3d_aniso": lambda: workloads.random_medium_3d(16, 21, 19, 90, seed=15, isotropic=False).inputs,
+    "2d_exp": lambda: workloads.random_medium_2d(50, 46, 260, seed=16, air=5, exponential=True).inputs,
+    "3d_exp": lambda: workloads.random_medium_3d(18, 20, 17, 100, seed=17, air=3, exponential=True).inputs,
 }
 
 
@@ -202,12 +204,13 @@ def test_streaming_3d_matches_per_cell(monkeypatch, shape):
 
 
 def test_autotuned_config_is_reported():
-    inp = workloads.random_medium_2d(80, 200, 20, seed=35).inputs
+    # long enough for the sensors to see the wave itself, not only the stencil's precursor
+    inp = workloads.random_medium_2d(80, 200, 240, seed=35).inputs
     with engine.HipSimulation(inp) as sim:
         cfg = sim.kernel_config()
         assert cfg["velocity"]["variant"] in ("percell", "stream", "tile")
         assert cfg["pressure"]["variant"] in ("percell", "stream", "tile")
-        sim.run(20)
+        sim.run(inp.nt)
         got = sim.sensors().astype(np.float64)
     want, _ = _ref(inp)
     assert rel_l2(got, want) <= TOL

@@ -93,3 +93,22 @@ def test_ghost_border_stays_zero(oracle_lib, small_2d):
     p = _oracle(small_2d, small_2d.nt).field("p")
     assert np.all(p[:8] == 0) and np.all(p[-8:] == 0)
     assert np.all(p[:, :8] == 0) and np.all(p[:, -8:] == 0)
+
+
+@pytest.mark.parametrize("is_3d", [False, True])
+def test_exponential_model_c_oracle_matches_numpy(oracle_lib, is_3d):
+    """The exponential-attenuation model (decay map with the radial absorbing mask, air points,
+    sources): C oracle against the numpy statement."""
+    from fullwave25_amd import workloads
+
+    if is_3d:
+        inp = workloads.random_medium_3d(12, 10, 11, 30, seed=4, air=2, exponential=True).inputs
+    else:
+        inp = workloads.random_medium_2d(30, 26, 60, seed=3, air=3, exponential=True).inputs
+    assert inp.exponential and "a_exp" in inp.maps and "kappax" not in inp.maps
+    o = oracle.OracleSimulation(inp, 64)
+    o.run(inp.nt)
+    rec, pf = oracle.brute_force_run(inp, inp.nt)
+    assert np.abs(rec).max() > 0
+    assert rel_l2(o.sensors(), rec) < 1e-12
+    assert rel_l2(o.field("p"), pf) < 1e-12

@@ -186,6 +186,43 @@ def test_discrete_energy_conserved(oracle_lib):
     assert np.max(np.abs(energies / energies[0] - 1)) < 1e-10
 
 
+def test_exponential_model_energy_decay(oracle_lib):
+    """Exponential-attenuation model, uniform decay factor A: rescaling p_n by A^-n and
+    v_{n+1/2} by A^-(n+1) turns the update into the lossless leapfrog with K replaced by A K, whose
+    discrete energy is conserved; so the lossless energy form of the actual fields,
+    E_n = sum p_n^2 / 2K + sum rho_bar v_{n-1/2} v_{n+1/2} / 2, satisfies E_n = A^(2n) E_0 exactly.
+    A model that scaled only one of the two fields would decay like A^n instead (determinant A
+    per step), 10 % away after 120 steps at A = 0.999."""
+    shape = (34, 38)
+    rng = np.random.default_rng(21)
+    rho = 1000 + 150 * rng.random(shape)
+    inp = raw_inputs(shape, rho=rho, nt=200, a_exp=0.999)
+    K = inp.maps["K"].astype(np.float64)
+    A = float(inp.maps["a_exp"][0, 0])
+    rhof = inp.maps["rho"].astype(np.float64)
+    rbar = [0.5 * (rhof + np.roll(rhof, -1, 0)), 0.5 * (rhof + np.roll(rhof, -1, 1))]
+    idx = np.indices(shape)
+    p0 = np.exp(-((idx[0] - 17.0) ** 2 + (idx[1] - 19.0) ** 2) / 6.0)
+    ghost = np.ones(shape, bool)
+    ghost[8:-8, 8:-8] = False
+    p0[ghost] = 0.0
+    sim = oracle.OracleSimulation(inp, 64)
+    sim.set_field("p", p0)
+    v_prev = [np.zeros(shape), np.zeros(shape)]
+    p_n = p0.copy()
+    energies = []
+    for _ in range(120):
+        sim.run(1)
+        v_now = [sim.field("vx"), sim.field("vy")]
+        energies.append(0.5 * np.sum(p_n**2 / K) + 0.5 * sum(np.sum(rbar[a] * v_prev[a] * v_now[a]) for a in range(2)))
+        v_prev = v_now
+        p_n = sim.field("p")
+    e = np.array(energies) / energies[0]
+    n = np.arange(e.size)
+    assert np.max(np.abs(e / A ** (2 * n) - 1)) < 1e-10
+    assert abs(e[-1] / A ** n[-1] - 1) > 0.05
+
+
 def test_absorbing_layer_reflection(oracle_lib):
     """A pulse from a point source leaves through the absorbing layer: the trace next to the
     source in a small domain matches the trace in a domain large enough that nothing comes
