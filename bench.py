@@ -49,7 +49,56 @@ def build_workload(args):
         return workloads.focused_linear_array_2d(nx=args.nx, ny=args.ny, nt=args.steps)
     if args.config == "plane_wave_2d":
         return workloads.plane_wave_2d(nx=args.nx, ny=args.ny, nt=args.steps)
+    if args.config == "random_3d":
+        return workloads.random_medium_3d(args.nx, args.ny, args.nz, args.steps, seed=5)
     raise SystemExit(f"unknown config {args.config}")
+
+
+def pmc_summary(fetch_dir: Path, write_dir: Path, copy_fetch_dir: Path, copy_write_dir: Path,
+                copy_bytes: int, config_file: Path) -> dict:
+    """Per-kernel HBM traffic from two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE; KiB per
+    dispatch), each scaled by what the same counter reads for the plain copy kernel moving
+    ``copy_bytes`` each way. The kernels are the velocity and pressure kernels of the kernel
+    configuration in ``config_file`` (written by ``--profile-steps``; the pmc runs pin it)."""
+    import csv
+
+    def per_kernel(d: Path) -> dict:
+        rows = {}
+        for r in csv.DictReader(open(d / "run_counter_collection.csv")):
+            rows.setdefault(r["Kernel_Name"], []).append(float(r["Counter_Value"]))
+        return rows
+
+    def copy_kib(d: Path) -> float:
+        vals = [v for k, v in per_kernel(d).items() if "k_copy" in k]
+        return float(np.mean(vals[0]))
+
+    f_scale = copy_bytes / 1024.0 / copy_kib(copy_fetch_dir)
+    w_scale = copy_bytes / 1024.0 / copy_kib(copy_write_dir)
+    fetch, write = per_kernel(fetch_dir), per_kernel(write_dir)
+    cfg = json.loads(Path(config_file).read_text())
+    out = {"kernel_config": cfg["kernel_config"], "fetch_scale": round(f_scale, 4), "write_scale": round(w_scale, 4),
+           "kernels": {}}
+    for kind, tag in (("velocity", "k_velocity"), ("pressure", "k_pressure")):
+        # the pinned kernel is the one dispatched every step; others ran only once or twice
+        name = max((k for k in fetch if tag in k), key=lambda k: len(fetch[k]))
+        f_kib, w_kib = float(np.mean(fetch[name])), float(np.mean(write[name]))
+        out["kernels"][kind] = {
+            "symbol": name, "dispatches": len(fetch[name]), "fetch_kib": round(f_kib, 1), "write_kib": round(w_kib, 1),
+            "bytes_per_launch": int(round((f_kib * f_scale + w_kib * w_scale) * 1024)),
+        }
+    return out
+
+
+def measured_traffic(workload: str, kind: str, kernel_config: dict):
+    """HBM bytes per launch of ``kind`` from the committed counter summary for this workload, if
+    it was measured with the same kernel configuration; else None."""
+    path = ROOT / "profiles" / f"pmc_traffic_{workload}.json"
+    if not path.exists():
+        return None
+    d = json.loads(path.read_text())
+    if d.get("kernel_config", {}).get(kind) != kernel_config.get(kind):
+        return None
+    return d["kernels"][kind]["bytes_per_launch"]
 
 
 def cpu_baseline(inp, steps: int) -> dict:
@@ -80,6 +129,7 @@ def main() -> int:
     ap.add_argument("--config", default="focused_2d")
     ap.add_argument("--nx", type=int, default=4096)
     ap.add_argument("--ny", type=int, default=2048)
+    ap.add_argument("--nz", type=int, default=256)
     ap.add_argument("--cpu-steps", type=int, default=150)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=0,
@@ -88,7 +138,19 @@ def main() -> int:
                     help="time every kernel family/shape over this many steps and exit")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run the plain copy kernel over this many bytes (counter calibration)")
+    ap.add_argument("--kernels", default="",
+                    help="pin the kernel configuration (FULLWAVE_HIP_KERNELS syntax) instead of autotuning")
+    ap.add_argument("--pmc-summary", nargs=5, metavar=("FETCH_DIR", "WRITE_DIR", "COPY_FETCH_DIR", "COPY_WRITE_DIR",
+                                                       "CONFIG_JSON"),
+                    help="summarize rocprofv3 --pmc runs into per-launch HBM traffic (JSON) and exit")
+    ap.add_argument("--copy-bytes", type=int, default=268435456)
     args = ap.parse_args()
+    if args.pmc_summary:
+        dirs = [Path(p) for p in args.pmc_summary]
+        print(json.dumps(pmc_summary(*dirs[:4], args.copy_bytes, dirs[4]), indent=1))
+        return 0
+    if args.kernels:
+        os.environ["FULLWAVE_HIP_KERNELS"] = args.kernels
     if args.copy_probe:
         from fullwave25_amd import engine
 
@@ -110,7 +172,7 @@ def main() -> int:
         shapes = ["percell"] + [f"stream:{w}:{c}" for w in (128, 256) for c in (8, 16, 32)]
         shapes += [f"tile:{c}" for c in (8, 16, 32)]
         if inp.ndim == 3:
-            shapes = ["percell"] + [f"stream:64:{c}" for c in (8, 16, 32, 64)]
+            shapes = ["percell"] + [f"stream:{w}:{c}" for w in (512, 1024) for c in (8, 16, 32)]
         bpc = algorithmic_bytes_per_cell(inp.ndim, inp.isotropic)
         updated = int(np.prod([s - 16 for s in inp.shape]))
         for shape in shapes:
@@ -129,6 +191,7 @@ def main() -> int:
     if args.profile_steps:
         sim.run(args.profile_steps)
         sim.close()
+        print(json.dumps({"kernel_config": kcfg, "kernels": engine.kernel_override_string(kcfg)}), flush=True)
         return 0
     sim.run(min(args.warmup, inp.nt))
     sim.reset()
@@ -180,12 +243,13 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": measured_traffic(wl.name, dominant, kcfg),
             "bytes_per_cell": bpc[dominant],
             "avg_launch_us": round(avg_s * 1e6, 2),
             "kernel_ms_per_step": {k: round(v / args.steps, 5) for k, v in kms.items()},
             "copy_kernel_gbs": round(copy_gbs, 1),
             "kernel_config": kcfg,
+            "kernels": engine.kernel_override_string(kcfg),
         },
     }
     if not args.no_cpu_baseline:

@@ -16,6 +16,7 @@
 
 #include "../../include/fullwave_hip.h"
 #include "fw_internal.h"
+#include "fw_sim.h"
 
 
 namespace {
@@ -55,27 +56,6 @@ const char* kAxesP[3] = {"x", "y", "z"};  // pressure-side map suffix per axis
 
 }  // namespace
 
-struct fw_sim {
-    fw_desc d{};
-    int device = 0;
-    hipStream_t stream = nullptr;
-    int64_t ncell = 0;
-    int ndim = 2;
-    fw::DevSim ds{};
-    fw::DevBoundary db{};
-    float* state = nullptr;  // p, v[ndim], psi[ndim][2], phi[ndim][2]
-    int64_t nstate = 0;
-    std::map<std::string, void*> dev;  // every other device allocation, by input name
-    std::map<std::string, bool> have;
-    std::vector<int32_t> icc, outc, zero;
-    std::vector<float> dmap;
-    bool ready = false;
-    int64_t step = 0;
-    int64_t rec_cap = 0;
-    fw::LaunchCfg cfg_v, cfg_p;  // velocity and pressure kernels, chosen independently
-    bool fields_set = false;  // a field was set before the first run: do not tune on it
-    hipEvent_t ev[2] = {nullptr, nullptr};
-};
 
 namespace {
 
@@ -165,10 +145,32 @@ int time_config(fw_sim* s, const fw::LaunchCfg& cfg, bool velocity, int reps, fl
     return FW_OK;
 }
 
+// Whether `c` can run this handle's grid.
+int check_cfg(fw_sim* s, const fw::LaunchCfg& c) {
+    if (c.variant == 0) return FW_OK;
+    if (c.variant == 2) {
+        if (!fw::tile2d_supported(s->ds))
+            return fail(FW_ERR_UNSUPPORTED, "tile kernels need a 2D relaxation grid with ny %% 4 == 0");
+        if (c.width != 256 || (c.chunk != 8 && c.chunk != 16 && c.chunk != 32))
+            return fail(FW_ERR_INVALID, "tile kernels: width 256 and rows 8, 16 or 32");
+        return FW_OK;
+    }
+    if (c.variant == 1) {
+        if (!fw::streaming_supported(s->ds)) return fail(FW_ERR_UNSUPPORTED, "this grid needs the per-cell kernels");
+        if (s->ndim == 2 && c.width != 64 && c.width != 128 && c.width != 256)
+            return fail(FW_ERR_INVALID, "x-marching kernels: width must be 64, 128 or 256");
+        if (s->ndim == 3 && c.width != 512 && c.width != 1024)
+            return fail(FW_ERR_INVALID, "3D x-marching kernels: workgroup size (width) must be 512 or 1024");
+        if (c.chunk < 2 || c.chunk > 4096) return fail(FW_ERR_INVALID, "x-marching kernels: chunk out of range");
+        return FW_OK;
+    }
+    return fail(FW_ERR_INVALID, "kernel variant must be 0, 1 or 2");
+}
+
 int parse_kernel_env(fw_sim* s, const char* env, fw::LaunchCfg* out) {
     const bool stream_ok = fw::streaming_supported(s->ds);
     fw::LaunchCfg c;
-    c.width = s->ndim == 3 ? 64 : 128;
+    c.width = s->ndim == 3 ? 512 : 128;
     c.chunk = 16;
     if (!strncmp(env, "percell", 7)) {
         c.variant = 0;
@@ -188,9 +190,8 @@ int parse_kernel_env(fw_sim* s, const char* env, fw::LaunchCfg* out) {
         const int n = sscanf(env, "stream:%d:%d", &w, &ch);
         if (n >= 1) c.width = w;
         if (n >= 2) c.chunk = ch;
-        if (s->ndim == 2 && c.width != 64 && c.width != 128 && c.width != 256)
-            return fail(FW_ERR_INVALID, "FULLWAVE_HIP_KERNELS: width must be 64, 128 or 256");
-        if (c.chunk < 2 || c.chunk > 4096) return fail(FW_ERR_INVALID, "FULLWAVE_HIP_KERNELS: chunk out of range");
+        int rc = check_cfg(s, c);
+        if (rc != FW_OK) return rc;
     } else {
         return fail(FW_ERR_INVALID, "FULLWAVE_HIP_KERNELS must be 'percell', 'stream[:width[:chunk]]' or 'tile[:rows]'");
     }
@@ -198,26 +199,51 @@ int parse_kernel_env(fw_sim* s, const char* env, fw::LaunchCfg* out) {
     return FW_OK;
 }
 
+int choose_kernels_unpinned(fw_sim* s);
+
 // Pick the kernel family and shape of each stencil kernel for this grid.
-// FULLWAVE_HIP_KERNELS overrides both: "percell", "stream[:width[:chunk]]" or "tile[:rows]".
+// FULLWAVE_HIP_KERNELS overrides both: "percell", "stream[:width[:chunk]]" or "tile[:rows]",
+// or each on its own as "<velocity>/<pressure>" (e.g. "tile:8/stream:256:8").
 // Otherwise every candidate is timed on the handle's own arrays while the wave state is still
 // all zero (the update then leaves it zero, so the timing runs change nothing), and the
 // fastest velocity kernel and the fastest pressure kernel are kept.
 int choose_kernels(fw_sim* s) {
+    int rc0 = choose_kernels_unpinned(s);
+    if (rc0 != FW_OK) return rc0;
+    for (int kind = 0; kind < 2; ++kind) {
+        if (!s->pinned[kind]) continue;
+        if ((rc0 = check_cfg(s, s->pin[kind])) != FW_OK) return rc0;
+        (kind == 0 ? s->cfg_v : s->cfg_p) = s->pin[kind];
+    }
+    return FW_OK;
+}
+
+int choose_kernels_unpinned(fw_sim* s) {
     const char* env = getenv("FULLWAVE_HIP_KERNELS");
     if (env && *env) {
-        int rc = parse_kernel_env(s, env, &s->cfg_v);
+        const char* slash = strchr(env, '/');
+        if (!slash) {
+            int rc = parse_kernel_env(s, env, &s->cfg_v);
+            if (rc != FW_OK) return rc;
+            s->cfg_p = s->cfg_v;
+            return FW_OK;
+        }
+        char vel[64];
+        const size_t n = (size_t)(slash - env);
+        if (n == 0 || n >= sizeof(vel)) return fail(FW_ERR_INVALID, "FULLWAVE_HIP_KERNELS: bad '<velocity>/<pressure>' pair");
+        memcpy(vel, env, n);
+        vel[n] = 0;
+        int rc = parse_kernel_env(s, vel, &s->cfg_v);
         if (rc != FW_OK) return rc;
-        s->cfg_p = s->cfg_v;
-        return FW_OK;
+        return parse_kernel_env(s, slash + 1, &s->cfg_p);
     }
     const bool stream_ok = fw::streaming_supported(s->ds);
     fw::LaunchCfg def;
     def.variant = stream_ok ? 1 : 0;
-    def.width = s->ndim == 3 ? 64 : 256;
+    def.width = s->ndim == 3 ? 512 : 256;
     def.chunk = 8;
     s->cfg_v = s->cfg_p = def;
-    if (s->fields_set || !stream_ok) return FW_OK;
+    if (s->fields_set || !stream_ok || (s->pinned[0] && s->pinned[1])) return FW_OK;
     std::vector<fw::LaunchCfg> cands;
     fw::LaunchCfg pc;
     pc.variant = 0;
@@ -232,11 +258,13 @@ int choose_kernels(fw_sim* s) {
                 cands.push_back(c);
             }
         } else {
-            fw::LaunchCfg c;
-            c.variant = 1;
-            c.width = 64;
-            c.chunk = ch;
-            cands.push_back(c);
+            for (int w : {512, 1024}) {
+                fw::LaunchCfg c;
+                c.variant = 1;
+                c.width = w;
+                c.chunk = ch;
+                cands.push_back(c);
+            }
         }
     }
     if (fw::tile2d_supported(s->ds)) {
@@ -249,6 +277,7 @@ int choose_kernels(fw_sim* s) {
         }
     }
     for (int kind = 0; kind < 2; ++kind) {
+        if (s->pinned[kind]) continue;
         float best = 1e30f;
         for (const auto& c : cands) {
             float ms = 0.f;
@@ -309,6 +338,7 @@ int finalize(fw_sim* s) {
     s->db.src_lin = (const int64_t*)s->dev["_src_lin"];
     if ((rc = up("_zero_lin", zero.data(), zero.size() * 8)) != FW_OK) return rc;
     s->db.zero_lin = (const int64_t*)s->dev["_zero_lin"];
+    s->out_lin_h = out;
     if ((rc = up("_out_lin", out.data(), out.size() * 8)) != FW_OK) return rc;
     s->db.out_lin = (const int64_t*)s->dev["_out_lin"];
     if ((rc = up("_out_src", out_src.data(), out_src.size() * 4)) != FW_OK) return rc;
@@ -345,9 +375,11 @@ int finalize(fw_sim* s) {
     }
 
     s->rec_cap = d.modt > 0 ? (d.nt + d.modt - 1) / d.modt : 0;
-    if ((rc = alloc(s, "_rec", (size_t)(s->rec_cap * d.ncoordsout) * 4, &p)) != FW_OK) return rc;
+    const size_t rec_bytes = (size_t)(s->nshots * s->rec_cap * d.ncoordsout) * 4;
+    if ((rc = alloc(s, "_rec", rec_bytes, &p)) != FW_OK) return rc;
     s->db.rec = (float*)p;
-    FW_HIP(hipMemset(p, 0, (size_t)(s->rec_cap * d.ncoordsout) * 4));
+    s->db.rec_stride = s->rec_cap * d.ncoordsout;
+    FW_HIP(hipMemset(p, 0, rec_bytes));
 
     // map pointers
     const int nax = d.isotropic ? 1 : nd;
@@ -421,6 +453,24 @@ int run_impl(fw_sim* s, int64_t n_steps, bool timed, double* ms) {
 
 }  // namespace
 
+namespace fw {
+
+int set_error(int code, const char* fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+int prepare(fw_sim* s) { return s->ready ? FW_OK : finalize(s); }
+
+int forward_step(fw_sim* s) { return launch_step(s, false, nullptr); }
+
+}  // namespace fw
+
 extern "C" {
 
 int fw_abi_version(void) { return FW_ABI_VERSION; }
@@ -474,6 +524,16 @@ int fw_device_copy_bandwidth(int device, size_t bytes, int repeats, double* gbs)
     return rc;
 }
 
+int fw_device_memory(int device, size_t* free_bytes, size_t* total_bytes) {
+    if (!free_bytes || !total_bytes) return fail(FW_ERR_INVALID, "null argument");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev)
+        return fail(FW_ERR_DEVICE, "device %d not available", device);
+    DeviceGuard g(device);
+    FW_HIP(hipMemGetInfo(free_bytes, total_bytes));
+    return FW_OK;
+}
+
 int fw_sim_create(const fw_desc* desc, int device, fw_sim** out) {
     if (!desc || !out) return fail(FW_ERR_INVALID, "null argument");
     *out = nullptr;
@@ -490,6 +550,11 @@ int fw_sim_create(const fw_desc* desc, int device, fw_sim** out) {
     if (d.ndmap < 1) return fail(FW_ERR_INVALID, "ndmap must be >= 1");
     if (d.attenuation_model != 0 && d.attenuation_model != 1)
         return fail(FW_ERR_INVALID, "attenuation_model must be 0 (relaxation) or 1 (exponential)");
+    const int64_t nshots = d.n_shots < 1 ? 1 : d.n_shots;
+    // launch grids carry the shots in z: 3D grids spend up to nx z blocks per shot
+    if (nshots * (d.ndim == 3 ? d.nx : 1) > (int64_t)fw::kMaxGridZ)
+        return fail(FW_ERR_UNSUPPORTED, "%lld shots of this grid exceed one launch; split the batch (at most %lld)",
+                    (long long)nshots, (long long)(fw::kMaxGridZ / (d.ndim == 3 ? d.nx : 1)));
     if (!(d.dt > 0) || !(d.dx > 0) || !(d.dy > 0) || (d.ndim == 3 && !(d.dz > 0)))
         return fail(FW_ERR_INVALID, "dt and grid spacings must be positive");
     int ndev = 0;
@@ -502,6 +567,8 @@ int fw_sim_create(const fw_desc* desc, int device, fw_sim** out) {
     s->device = device;
     s->ndim = d.ndim;
     s->ncell = cells(s->d);
+    s->nshots = (int)nshots;
+    s->d.n_shots = (int32_t)nshots;
     DeviceGuard g(device);
     auto cleanup = [&](int rc) {
         fw_sim_destroy(s);
@@ -513,19 +580,22 @@ int fw_sim_create(const fw_desc* desc, int device, fw_sim** out) {
         return cleanup(fail(FW_ERR_DEVICE, "hipEventCreate failed"));
     const int nd = d.ndim;
     const bool relax = d.attenuation_model == 0;  // the exponential model has no memory variables
-    s->nstate = (int64_t)(1 + nd + (relax ? 4 * nd : 0)) * s->ncell;
+    const int64_t field = s->ncell * nshots;  // one wave field of every shot
+    s->nstate = (int64_t)(1 + nd + (relax ? 4 * nd : 0)) * field;
     if (hipMalloc(&s->state, (size_t)s->nstate * 4) != hipSuccess)
         return cleanup(fail(FW_ERR_NOMEM, "hipMalloc of %lld state floats failed", (long long)s->nstate));
     if (hipMemset(s->state, 0, (size_t)s->nstate * 4) != hipSuccess)
         return cleanup(fail(FW_ERR_DEVICE, "hipMemset failed"));
     float* f = s->state;
     s->ds.p = f;
-    f += s->ncell;
-    for (int a = 0; a < nd; ++a, f += s->ncell) s->ds.v[a] = f;
+    f += field;
+    for (int a = 0; a < nd; ++a, f += field) s->ds.v[a] = f;
     for (int a = 0; a < nd && relax; ++a)
-        for (int nu = 0; nu < 2; ++nu, f += s->ncell) s->ds.psi[a][nu] = f;
+        for (int nu = 0; nu < 2; ++nu, f += field) s->ds.psi[a][nu] = f;
     for (int a = 0; a < nd && relax; ++a)
-        for (int nu = 0; nu < 2; ++nu, f += s->ncell) s->ds.phi[a][nu] = f;
+        for (int nu = 0; nu < 2; ++nu, f += field) s->ds.phi[a][nu] = f;
+    s->ds.nshots = (int32_t)nshots;
+    s->ds.shot_stride = s->ncell;
     s->ds.ndim = nd;
     s->ds.nx = d.nx;
     s->ds.ny = d.ny;
@@ -580,13 +650,18 @@ int fw_sim_upload(fw_sim* s, const char* name_c, const void* host, size_t bytes)
         return FW_OK;
     }
     if (name == "icmat") {
-        if (bytes != (size_t)(d.ncoords * d.ntic) * 4)
-            return fail(FW_ERR_INVALID, "'icmat' must be [ncoords, ntic] float32");
-        // device layout [ntic][ncoords]: one coalesced row per step
-        std::vector<float> t((size_t)(d.ncoords * d.ntic));
+        const int64_t per_shot = d.ncoords * d.ntic;
+        if (bytes != (size_t)(s->nshots * per_shot) * 4)
+            return fail(FW_ERR_INVALID, "'icmat' must be [n_shots, ncoords, ntic] float32 (%lld bytes)",
+                        (long long)(s->nshots * per_shot * 4));
+        // device layout [shot][ntic][ncoords]: one coalesced row per step
+        std::vector<float> t((size_t)(s->nshots * per_shot));
         const float* src = (const float*)host;
-        for (int64_t q = 0; q < d.ncoords; ++q)
-            for (int64_t n = 0; n < d.ntic; ++n) t[(size_t)(n * d.ncoords + q)] = src[q * d.ntic + n];
+        for (int64_t b = 0; b < s->nshots; ++b)
+            for (int64_t q = 0; q < d.ncoords; ++q)
+                for (int64_t n = 0; n < d.ntic; ++n)
+                    t[(size_t)(b * per_shot + n * d.ncoords + q)] = src[b * per_shot + q * d.ntic + n];
+        s->db.icmat_stride = per_shot;
         if ((rc = alloc(s, name, bytes, &p)) != FW_OK) return rc;
         if (bytes) FW_HIP(hipMemcpy(p, t.data(), bytes, hipMemcpyHostToDevice));
         s->db.icmat_t = (const float*)p;
@@ -610,7 +685,7 @@ int fw_sim_reset(fw_sim* s) {
     DeviceGuard g(s->device);
     FW_HIP(hipMemsetAsync(s->state, 0, (size_t)s->nstate * 4, s->stream));
     if (s->db.rec && s->rec_cap)
-        FW_HIP(hipMemsetAsync(s->db.rec, 0, (size_t)(s->rec_cap * s->d.ncoordsout) * 4, s->stream));
+        FW_HIP(hipMemsetAsync(s->db.rec, 0, (size_t)(s->nshots * s->rec_cap * s->d.ncoordsout) * 4, s->stream));
     FW_HIP(hipStreamSynchronize(s->stream));
     s->step = 0;
     return FW_OK;
@@ -632,11 +707,15 @@ int fw_sim_download_sensors(fw_sim* s, float* out, size_t bytes) {
     if (!s || (!out && bytes)) return fail(FW_ERR_INVALID, "null argument");
     int64_t nrec = 0;
     fw_sim_record_count(s, &nrec);
-    const size_t want = (size_t)(nrec * s->d.ncoordsout) * 4;
-    if (bytes != want) return fail(FW_ERR_INVALID, "sensor buffer must be %zu bytes (%lld records), got %zu", want, (long long)nrec, bytes);
+    const size_t row = (size_t)(nrec * s->d.ncoordsout) * 4;  // one shot's records
+    const size_t want = row * (size_t)s->nshots;
+    if (bytes != want)
+        return fail(FW_ERR_INVALID, "sensor buffer must be %zu bytes (%d shots x %lld records), got %zu", want,
+                    s->nshots, (long long)nrec, bytes);
     if (!want) return FW_OK;
     DeviceGuard g(s->device);
-    FW_HIP(hipMemcpy(out, s->db.rec, want, hipMemcpyDeviceToHost));
+    FW_HIP(hipMemcpy2D(out, row, s->db.rec, (size_t)s->db.rec_stride * 4, row, (size_t)s->nshots,
+                       hipMemcpyDeviceToHost));
     return FW_OK;
 }
 
@@ -648,7 +727,9 @@ int fw_sim_download_field(fw_sim* s, const char* name, float* out, size_t bytes)
     else if (!strcmp(name, "vy")) src = s->ds.v[1];
     else if (!strcmp(name, "vz") && s->ndim == 3) src = s->ds.v[2];
     if (!src) return fail(FW_ERR_INVALID, "unknown field '%s'", name);
-    if (bytes != (size_t)s->ncell * 4) return fail(FW_ERR_INVALID, "field buffer must be %lld bytes", (long long)s->ncell * 4);
+    if (bytes != (size_t)s->ncell * 4 && bytes != (size_t)(s->ncell * s->nshots) * 4)
+        return fail(FW_ERR_INVALID, "field buffer must be %lld bytes (first shot) or %lld (every shot)",
+                    (long long)s->ncell * 4, (long long)(s->ncell * s->nshots) * 4);
     DeviceGuard g(s->device);
     FW_HIP(hipMemcpy(out, src, bytes, hipMemcpyDeviceToHost));
     return FW_OK;
@@ -662,7 +743,9 @@ int fw_sim_set_field(fw_sim* s, const char* name, const float* values, size_t by
     else if (!strcmp(name, "vy")) dst = s->ds.v[1];
     else if (!strcmp(name, "vz") && s->ndim == 3) dst = s->ds.v[2];
     if (!dst) return fail(FW_ERR_INVALID, "unknown field '%s'", name);
-    if (bytes != (size_t)s->ncell * 4) return fail(FW_ERR_INVALID, "field buffer must be %lld bytes", (long long)s->ncell * 4);
+    if (bytes != (size_t)s->ncell * 4 && bytes != (size_t)(s->ncell * s->nshots) * 4)
+        return fail(FW_ERR_INVALID, "field buffer must be %lld bytes (first shot) or %lld (every shot)",
+                    (long long)s->ncell * 4, (long long)(s->ncell * s->nshots) * 4);
     DeviceGuard g(s->device);
     FW_HIP(hipMemcpy(dst, values, bytes, hipMemcpyHostToDevice));
     if (!s->ready) s->fields_set = true;
@@ -681,6 +764,20 @@ int fw_sim_kernel_config(fw_sim* s, int32_t kind, int32_t* variant, int32_t* wid
     *variant = c.variant;
     *width = c.width;
     *chunk = c.chunk;
+    return FW_OK;
+}
+
+int fw_sim_set_kernel_config(fw_sim* s, int32_t kind, int32_t variant, int32_t width, int32_t chunk) {
+    if (!s) return fail(FW_ERR_INVALID, "null handle");
+    if (kind != 0 && kind != 1) return fail(FW_ERR_INVALID, "kind must be 0 (velocity) or 1 (pressure)");
+    if (s->ready) return fail(FW_ERR_STATE, "the kernels are fixed at the first run; set them before it");
+    if (variant < 0 || variant > 2) return fail(FW_ERR_INVALID, "kernel variant must be 0, 1 or 2");
+    fw::LaunchCfg c;
+    c.variant = variant;
+    c.width = width;
+    c.chunk = chunk;
+    s->pin[kind] = c;
+    s->pinned[kind] = true;
     return FW_OK;
 }
 

@@ -40,7 +40,39 @@ struct DevSim {
     // exponential-attenuation model: no memory variables; v and p are scaled by aexp after
     // their update (aexp == nullptr for the relaxation model)
     const float* aexp;
+
+    // a batch of `nshots` transmits on this medium: every wave field (p, v, psi, phi) holds
+    // nshots consecutive grids, shot_stride floats apart; the maps are shared
+    int32_t nshots;
+    int64_t shot_stride;
 };
+
+// The fields of one shot of the batch.
+__host__ __device__ inline DevSim shot_view(const DevSim& s, int shot) {
+    DevSim t = s;
+    const int64_t o = (int64_t)shot * s.shot_stride;
+    t.p += o;
+    for (int a = 0; a < kMaxDim; ++a) {
+        t.v[a] = s.v[a] ? s.v[a] + o : nullptr;
+        for (int nu = 0; nu < kNRelax; ++nu) {
+            t.psi[a][nu] = s.psi[a][nu] ? s.psi[a][nu] + o : nullptr;
+            t.phi[a][nu] = s.phi[a][nu] ? s.phi[a][nu] + o : nullptr;
+        }
+    }
+    return t;
+}
+
+// Launch grids carry the shots in z: gridDim.z = (blocks per shot in z) * nshots. Returns the
+// shot of this workgroup and, in `zl`, its z block within the shot.
+__device__ inline int shot_of_block(const DevSim& s, unsigned& zl) {
+    const unsigned zper = gridDim.z / (unsigned)s.nshots;
+    const unsigned shot = blockIdx.z / zper;
+    zl = blockIdx.z - shot * zper;
+    return (int)shot;
+}
+
+// Largest z extent of a launch grid.
+constexpr unsigned kMaxGridZ = 65535;
 
 // Source injection, pressure release and sensor gather of one step.
 struct DevBoundary {
@@ -51,7 +83,27 @@ struct DevBoundary {
     const int64_t* out_lin;
     const int32_t* out_src;    // source index at the sensor's voxel, or -1
     const int8_t* out_air;     // 1 if the sensor's voxel is a pressure-release point
-    float* rec;                // [n_records][nout]
+    float* rec;                // [nshots][n_records][nout]
+    int64_t icmat_stride;      // floats between the source matrices of consecutive shots
+    int64_t rec_stride;        // floats between the records of consecutive shots
+};
+
+// Adjoint sweep of one time step (fw_kernels_adjoint.hip): the adjoint state (lp, lv, lpsi,
+// lphi, laid out like the forward state), the forward state around the step (p before it; v,
+// psi, phi after it), per-axis scratch H and G, and the gradient accumulators.
+struct AdjView {
+    float* lp;
+    float* lv[kMaxDim];
+    float* lpsi[kMaxDim][kNRelax];
+    float* lphi[kMaxDim][kNRelax];
+    const float* fp_prev;
+    const float* fv[kMaxDim];
+    const float* fpsi[kMaxDim][kNRelax];
+    const float* fphi[kMaxDim][kNRelax];
+    float* H[kMaxDim];
+    float* G[kMaxDim];
+    float* gK;               // d/dK, summed over steps
+    float* gRb[kMaxDim];     // d/d rho_bar_a, summed over steps
 };
 
 enum KernelKind { kVelocity = 0, kPressure = 1, kBoundary = 2, kNumKinds = 3 };
@@ -76,5 +128,11 @@ hipError_t launch_boundary(const DevSim& s, const DevBoundary& b, int64_t step, 
                            hipStream_t st);
 hipError_t launch_reciprocal(float* data, int64_t n, hipStream_t st);
 hipError_t launch_copy(const void* src, void* dst, int64_t n16, hipStream_t st);
+hipError_t launch_adj_records(float* lp, const float* grow, const int64_t* vox, const int32_t* first,
+                              const int32_t* order, int64_t nvox, hipStream_t st);
+hipError_t launch_adj_mask(float* lp, const int64_t* src_lin, int64_t nsrc, const int64_t* zero_lin, int64_t nzero,
+                           hipStream_t st);
+hipError_t launch_adj_step(const DevSim& s, const AdjView& w, hipStream_t st);
+hipError_t launch_adj_rho(const DevSim& s, const AdjView& w, float* grho, hipStream_t st);
 
 }  // namespace fw

@@ -1,0 +1,301 @@
+// Adjoint (reverse-mode) kernels: one time step of the update transposed.
+//
+// The forward step is, per active cell c and axis a (h_a folded into the coefficients):
+//   velocity:  g_a = D+_a p,  psi' = b psi + a g_a,  v_a' = v_a - beta_a (g_a / kappa + sum psi')
+//   pressure:  h_a = D-_a v_a',  phi' = b phi + a h_a,  p' = p - dt K sum_a (h_a / kappa + sum phi')
+//   then p'[source] = signal, p'[air] = 0, and the sensors record p'
+// (the exponential model: v_a' = A (v_a - beta_a g_a), p' = A (p - dt K sum_a h_a)), with
+// beta_a = dt / rho_bar_a. Going back through step n with the adjoint state lambda of the state
+// after the step:
+//   1. lambda_p[sensor] += dL/d record   (k_adj_records)
+//   2. lambda_p[source] = lambda_p[air] = 0   (k_adj_mask; those values were overwritten)
+//   3. pressure transposed (k_adj_pressure): t = -dt K mu, mu = lambda_p;
+//        lambda_phi_tot = lambda_phi + t, H_a = t / kappa + sum a lambda_phi_tot,
+//        lambda_phi = b lambda_phi_tot, dL/dK += mu * (-dt) * div (div from the stored state)
+//   4. lambda_v_a += (D-_a)^T H_a   (k_adj_dminus_t, gather form)
+//   5. velocity transposed (k_adj_velocity): nu = lambda_v_a;
+//        lambda_psi_tot = lambda_psi - beta nu, G_a = -beta nu / kappa + sum a lambda_psi_tot,
+//        lambda_psi = b lambda_psi_tot, dL/d rho_bar_a += nu * u_a * dt / rho_bar^2 with
+//        u_a = g_a / kappa + sum psi'
+//   6. lambda_p += sum_a (D+_a)^T G_a   (k_adj_dplus_t, gather form)
+// The coefficient sets (dcmap) are constants. The transposed stencils gather: the value at x
+// collects every active cell whose stencil reads x, with that cell's coefficients.
+#include "fw_internal.h"
+
+namespace fw {
+
+namespace {
+
+template <int ND>
+__device__ inline bool adj_cell(const DevSim& s, int64_t& i, int64_t& j, int64_t& k) {
+    if (ND == 2) {
+        k = 0;
+        j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        i = (int64_t)blockIdx.y * blockDim.y + threadIdx.y;
+        return i >= kHalo && i < s.nx - kHalo && j >= kHalo && j < s.ny - kHalo;
+    }
+    k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    j = (int64_t)blockIdx.y * blockDim.y + threadIdx.y;
+    i = blockIdx.z;
+    return i >= kHalo && i < s.nx - kHalo && j >= kHalo && j < s.ny - kHalo && k >= kHalo && k < s.nz - kHalo;
+}
+
+__device__ inline const float* wts(const DevSim& s, int64_t c, int nd, int a) {
+    return s.coef + ((int64_t)s.dcmap[c] * nd + a) * kNCoef;
+}
+
+template <int ND>
+__device__ inline float dplus(const DevSim& s, const float* p, int64_t c, int a, const int64_t* st) {
+    const float* w = wts(s, c, ND, a);
+    const int64_t sa = st[a];
+    float acc = 0.f;
+#pragma unroll
+    for (int m = 1; m <= 8; ++m) acc += w[m - 1] * (p[c + m * sa] - p[c - (m - 1) * sa]);
+    float tr = 0.f;
+#pragma unroll
+    for (int b = 0; b < ND; ++b) {
+        if (b == a) continue;
+        const int64_t sb = st[b];
+        tr += (p[c + sa + sb] - p[c + sb]) + (p[c + sa - sb] - p[c - sb]);
+    }
+    return acc + w[8] * tr;
+}
+
+template <int ND>
+__device__ inline float dminus(const DevSim& s, const float* f, int64_t c, int a, const int64_t* st) {
+    const float* w = wts(s, c, ND, a);
+    const int64_t sa = st[a];
+    float acc = 0.f;
+#pragma unroll
+    for (int m = 1; m <= 8; ++m) acc += w[m - 1] * (f[c + (m - 1) * sa] - f[c - m * sa]);
+    float tr = 0.f;
+#pragma unroll
+    for (int b = 0; b < ND; ++b) {
+        if (b == a) continue;
+        const int64_t sb = st[b];
+        tr += (f[c + sb] - f[c - sa + sb]) + (f[c - sb] - f[c - sa - sb]);
+    }
+    return acc + w[8] * tr;
+}
+
+// records of one step: one thread per distinct sensor voxel sums its sensors' weights
+__global__ __launch_bounds__(256) void k_adj_records(float* lp, const float* grow, const int64_t* vox,
+                                                     const int32_t* first, const int32_t* order, int64_t nvox) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nvox) return;
+    float acc = 0.f;
+    for (int32_t q = first[t]; q < first[t + 1]; ++q) acc += grow[order[q]];
+    lp[vox[t]] += acc;
+}
+
+__global__ __launch_bounds__(256) void k_adj_mask(float* lp, const int64_t* src_lin, int64_t nsrc,
+                                                  const int64_t* zero_lin, int64_t nzero) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < nsrc && src_lin[t] >= 0) lp[src_lin[t]] = 0.f;
+    if (t < nzero) lp[zero_lin[t]] = 0.f;
+}
+
+template <int ND, bool EXPO>
+__global__ __launch_bounds__(256) void k_adj_pressure(DevSim s, AdjView w) {
+    int64_t i, j, k;
+    if (!adj_cell<ND>(s, i, j, k)) return;
+    const int64_t c = i * s.sx + j * s.sy + k;
+    const int64_t st[3] = {s.sx, s.sy, 1};
+    const float mu = w.lp[c];
+    float div = 0.f;
+    if (EXPO) {
+        const float A = s.aexp[c];
+        const float t = -A * s.dt * s.K[c] * mu;
+#pragma unroll
+        for (int a = 0; a < ND; ++a) {
+            div += dminus<ND>(s, w.fv[a], c, a, st);
+            w.H[a][c] = t;
+        }
+        w.lp[c] = A * mu;
+        w.gK[c] += mu * (-A * s.dt) * div;
+        return;
+    }
+    const float t = -s.dt * s.K[c] * mu;
+#pragma unroll
+    for (int a = 0; a < ND; ++a) {
+        const float h = dminus<ND>(s, w.fv[a], c, a, st);
+        const float ik = s.ikp[a][c];
+        float lh = t * ik;
+#pragma unroll
+        for (int nu = 0; nu < kNRelax; ++nu) {
+            const float tot = w.lphi[a][nu][c] + t;
+            lh += s.ap[a][nu][c] * tot;
+            w.lphi[a][nu][c] = s.bp[a][nu][c] * tot;
+        }
+        div += h * ik + (w.fphi[a][0][c] + w.fphi[a][1][c]);
+        w.H[a][c] = lh;
+    }
+    w.gK[c] += mu * (-s.dt) * div;
+}
+
+// lambda_v_a += (D-_a)^T H_a at active cells
+template <int ND>
+__global__ __launch_bounds__(256) void k_adj_dminus_t(DevSim s, AdjView w) {
+    int64_t i, j, k;
+    if (!adj_cell<ND>(s, i, j, k)) return;
+    const int64_t x = i * s.sx + j * s.sy + k;
+    const int64_t st[3] = {s.sx, s.sy, 1};
+#pragma unroll
+    for (int a = 0; a < ND; ++a) {
+        const float* H = w.H[a];
+        const int64_t sa = st[a];
+        float acc = 0.f;
+#pragma unroll
+        for (int m = 1; m <= 8; ++m) {
+            const int64_t c1 = x - (m - 1) * sa, c2 = x + m * sa;
+            acc += wts(s, c1, ND, a)[m - 1] * H[c1] - wts(s, c2, ND, a)[m - 1] * H[c2];
+        }
+#pragma unroll
+        for (int b = 0; b < ND; ++b) {
+            if (b == a) continue;
+            const int64_t sb = st[b];
+#pragma unroll
+            for (int sg = -1; sg <= 1; sg += 2) {
+                const int64_t c1 = x - sg * sb, c2 = x + sa - sg * sb;
+                acc += wts(s, c1, ND, a)[8] * H[c1] - wts(s, c2, ND, a)[8] * H[c2];
+            }
+        }
+        w.lv[a][x] += acc;
+    }
+}
+
+template <int ND, bool EXPO>
+__global__ __launch_bounds__(256) void k_adj_velocity(DevSim s, AdjView w) {
+    int64_t i, j, k;
+    if (!adj_cell<ND>(s, i, j, k)) return;
+    const int64_t c = i * s.sx + j * s.sy + k;
+    const int64_t st[3] = {s.sx, s.sy, 1};
+    const float rho_c = s.rho[c];
+#pragma unroll
+    for (int a = 0; a < ND; ++a) {
+        const int64_t sa = st[a];
+        const float nu = w.lv[a][c];
+        const float rb = 0.5f * (rho_c + s.rho[c + sa]);
+        const float beta = s.dt / rb;
+        const float g = dplus<ND>(s, w.fp_prev, c, a, st);
+        float u, lg;
+        if (EXPO) {
+            const float A = s.aexp[c];
+            u = A * g;
+            lg = -A * beta * nu;
+            w.lv[a][c] = A * nu;
+        } else {
+            const float ik = s.ikv[a][c];
+            u = g * ik + (w.fpsi[a][0][c] + w.fpsi[a][1][c]);
+            lg = -beta * nu * ik;
+#pragma unroll
+            for (int q = 0; q < kNRelax; ++q) {
+                const float tot = w.lpsi[a][q][c] - beta * nu;
+                lg += s.av[a][q][c] * tot;
+                w.lpsi[a][q][c] = s.bv[a][q][c] * tot;
+            }
+        }
+        w.G[a][c] = lg;
+        w.gRb[a][c] += nu * u * s.dt / (rb * rb);
+    }
+}
+
+// lambda_p += sum_a (D+_a)^T G_a at active cells
+template <int ND>
+__global__ __launch_bounds__(256) void k_adj_dplus_t(DevSim s, AdjView w) {
+    int64_t i, j, k;
+    if (!adj_cell<ND>(s, i, j, k)) return;
+    const int64_t x = i * s.sx + j * s.sy + k;
+    const int64_t st[3] = {s.sx, s.sy, 1};
+    float acc = 0.f;
+#pragma unroll
+    for (int a = 0; a < ND; ++a) {
+        const float* G = w.G[a];
+        const int64_t sa = st[a];
+#pragma unroll
+        for (int m = 1; m <= 8; ++m) {
+            const int64_t c1 = x - m * sa, c2 = x + (m - 1) * sa;
+            acc += wts(s, c1, ND, a)[m - 1] * G[c1] - wts(s, c2, ND, a)[m - 1] * G[c2];
+        }
+#pragma unroll
+        for (int b = 0; b < ND; ++b) {
+            if (b == a) continue;
+            const int64_t sb = st[b];
+#pragma unroll
+            for (int sg = -1; sg <= 1; sg += 2) {
+                const int64_t c1 = x - sa - sg * sb, c2 = x - sg * sb;
+                acc += wts(s, c1, ND, a)[8] * G[c1] - wts(s, c2, ND, a)[8] * G[c2];
+            }
+        }
+    }
+    w.lp[x] += acc;
+}
+
+// d/d rho[x] = sum_a (g_rb_a[x] + g_rb_a[x - e_a]) / 2, over the whole grid
+__global__ __launch_bounds__(256) void k_adj_rho(DevSim s, AdjView w, float* grho) {
+    const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t n = s.nx * s.ny * s.nz;
+    if (x >= n) return;
+    const int64_t i = x / s.sx, j = (x / s.sy) % s.ny, k = x % s.nz;
+    const int64_t coord[3] = {i, j, k};
+    const int64_t st[3] = {s.sx, s.sy, 1};
+    float acc = 0.f;
+    for (int a = 0; a < s.ndim; ++a) {
+        acc += w.gRb[a][x];
+        if (coord[a] > 0) acc += w.gRb[a][x - st[a]];
+    }
+    grho[x] = 0.5f * acc;
+}
+
+}  // namespace
+
+static dim3 adj_grid(const DevSim& s) {
+    if (s.ndim == 2) return dim3((unsigned)((s.ny + 63) / 64), (unsigned)((s.nx + 3) / 4), 1);
+    return dim3((unsigned)((s.nz + 63) / 64), (unsigned)((s.ny + 3) / 4), (unsigned)s.nx);
+}
+
+hipError_t launch_adj_records(float* lp, const float* grow, const int64_t* vox, const int32_t* first,
+                              const int32_t* order, int64_t nvox, hipStream_t st) {
+    if (nvox == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_adj_records, dim3((unsigned)((nvox + 255) / 256)), dim3(256), 0, st, lp, grow, vox, first,
+                       order, nvox);
+    return hipGetLastError();
+}
+
+hipError_t launch_adj_mask(float* lp, const int64_t* src_lin, int64_t nsrc, const int64_t* zero_lin, int64_t nzero,
+                           hipStream_t st) {
+    const int64_t n = nsrc > nzero ? nsrc : nzero;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_adj_mask, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, lp, src_lin, nsrc, zero_lin,
+                       nzero);
+    return hipGetLastError();
+}
+
+hipError_t launch_adj_step(const DevSim& s, const AdjView& w, hipStream_t st) {
+    const dim3 g = adj_grid(s), b(64, 4);
+    const bool e = s.aexp != nullptr;
+    if (s.ndim == 2) {
+        if (e) hipLaunchKernelGGL((k_adj_pressure<2, true>), g, b, 0, st, s, w);
+        else hipLaunchKernelGGL((k_adj_pressure<2, false>), g, b, 0, st, s, w);
+        hipLaunchKernelGGL((k_adj_dminus_t<2>), g, b, 0, st, s, w);
+        if (e) hipLaunchKernelGGL((k_adj_velocity<2, true>), g, b, 0, st, s, w);
+        else hipLaunchKernelGGL((k_adj_velocity<2, false>), g, b, 0, st, s, w);
+        hipLaunchKernelGGL((k_adj_dplus_t<2>), g, b, 0, st, s, w);
+    } else {
+        if (e) hipLaunchKernelGGL((k_adj_pressure<3, true>), g, b, 0, st, s, w);
+        else hipLaunchKernelGGL((k_adj_pressure<3, false>), g, b, 0, st, s, w);
+        hipLaunchKernelGGL((k_adj_dminus_t<3>), g, b, 0, st, s, w);
+        if (e) hipLaunchKernelGGL((k_adj_velocity<3, true>), g, b, 0, st, s, w);
+        else hipLaunchKernelGGL((k_adj_velocity<3, false>), g, b, 0, st, s, w);
+        hipLaunchKernelGGL((k_adj_dplus_t<3>), g, b, 0, st, s, w);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_adj_rho(const DevSim& s, const AdjView& w, float* grho, hipStream_t st) {
+    const int64_t n = s.nx * s.ny * s.nz;
+    hipLaunchKernelGGL(k_adj_rho, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, s, w, grho);
+    return hipGetLastError();
+}
+
+}  // namespace fw

@@ -9,7 +9,7 @@ namespace fw {
 namespace {
 
 template <int ND>
-__device__ inline bool cell_of_thread(const DevSim& s, int64_t& i, int64_t& j, int64_t& k) {
+__device__ inline bool cell_of_thread(const DevSim& s, unsigned zl, int64_t& i, int64_t& j, int64_t& k) {
     if (ND == 2) {
         k = 0;
         j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -18,15 +18,17 @@ __device__ inline bool cell_of_thread(const DevSim& s, int64_t& i, int64_t& j, i
     }
     k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     j = (int64_t)blockIdx.y * blockDim.y + threadIdx.y;
-    i = blockIdx.z;
+    i = zl;
     return i >= kHalo && i < s.nx - kHalo && j >= kHalo && j < s.ny - kHalo && k >= kHalo &&
            k < s.nz - kHalo;
 }
 
 template <int ND, bool EXPO>
-__global__ __launch_bounds__(256) void k_velocity_simple(DevSim s) {
+__global__ __launch_bounds__(256) void k_velocity_simple(DevSim batch) {
+    unsigned zl;
+    const DevSim s = shot_view(batch, shot_of_block(batch, zl));
     int64_t i, j, k;
-    if (!cell_of_thread<ND>(s, i, j, k)) return;
+    if (!cell_of_thread<ND>(s, zl, i, j, k)) return;
     const int64_t c = i * s.sx + j * s.sy + k;
     const float* cf = s.coef + (int64_t)s.dcmap[c] * ND * kNCoef;
     const int64_t stride[3] = {s.sx, s.sy, 1};
@@ -62,9 +64,11 @@ __global__ __launch_bounds__(256) void k_velocity_simple(DevSim s) {
 }
 
 template <int ND, bool EXPO>
-__global__ __launch_bounds__(256) void k_pressure_simple(DevSim s) {
+__global__ __launch_bounds__(256) void k_pressure_simple(DevSim batch) {
+    unsigned zl;
+    const DevSim s = shot_view(batch, shot_of_block(batch, zl));
     int64_t i, j, k;
-    if (!cell_of_thread<ND>(s, i, j, k)) return;
+    if (!cell_of_thread<ND>(s, zl, i, j, k)) return;
     const int64_t c = i * s.sx + j * s.sy + k;
     const float* cf = s.coef + (int64_t)s.dcmap[c] * ND * kNCoef;
     const int64_t stride[3] = {s.sx, s.sy, 1};
@@ -104,23 +108,27 @@ __global__ __launch_bounds__(256) void k_pressure_simple(DevSim s) {
 // Source injection, then pressure release, then sensor gather, for step `n`. Sources that
 // sit on a pressure-release point or repeat a later source are dropped on the host, and
 // sensors on such points read the value written there, so no two threads touch one address.
+// The shot is blockIdx.y.
 __global__ __launch_bounds__(256) void k_boundary(DevSim s, DevBoundary b, int64_t n, int64_t rec_row) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t shot = blockIdx.y;
+    float* p = s.p + shot * s.shot_stride;
+    const float* icmat_t = b.icmat_t + shot * b.icmat_stride;
     const bool inject = n < b.ntic;
     if (inject && t < b.ncoords) {
         const int64_t lin = b.src_lin[t];
-        if (lin >= 0) s.p[lin] = b.icmat_t[n * b.ncoords + t];
+        if (lin >= 0) p[lin] = icmat_t[n * b.ncoords + t];
     }
-    if (t < b.nzero) s.p[b.zero_lin[t]] = 0.f;
+    if (t < b.nzero) p[b.zero_lin[t]] = 0.f;
     if (rec_row >= 0 && t < b.nout) {
         float val;
         if (b.out_air[t]) {
             val = 0.f;
         } else {
             const int32_t src = b.out_src[t];
-            val = (inject && src >= 0) ? b.icmat_t[n * b.ncoords + src] : s.p[b.out_lin[t]];
+            val = (inject && src >= 0) ? icmat_t[n * b.ncoords + src] : p[b.out_lin[t]];
         }
-        b.rec[rec_row * b.nout + t] = val;
+        b.rec[shot * b.rec_stride + rec_row * b.nout + t] = val;
     }
 }
 
@@ -137,8 +145,8 @@ __global__ __launch_bounds__(256) void k_reciprocal(float* data, int64_t n) {
 }  // namespace
 
 static dim3 simple_grid(const DevSim& s) {
-    if (s.ndim == 2) return dim3((unsigned)((s.ny + 63) / 64), (unsigned)((s.nx + 3) / 4), 1);
-    return dim3((unsigned)((s.nz + 63) / 64), (unsigned)((s.ny + 3) / 4), (unsigned)s.nx);
+    if (s.ndim == 2) return dim3((unsigned)((s.ny + 63) / 64), (unsigned)((s.nx + 3) / 4), (unsigned)s.nshots);
+    return dim3((unsigned)((s.nz + 63) / 64), (unsigned)((s.ny + 3) / 4), (unsigned)(s.nx * s.nshots));
 }
 
 hipError_t launch_velocity_simple(const DevSim& s, hipStream_t st) {
@@ -167,8 +175,8 @@ hipError_t launch_boundary(const DevSim& s, const DevBoundary& b, int64_t step, 
     if (n == 0) return hipSuccess;
     const int64_t rec_row = (modt > 0 && step % modt == 0) ? step / modt : -1;
     if (rec_row < 0 && step >= b.ntic && b.nzero == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_boundary, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, s, b, step,
-                       rec_row);
+    hipLaunchKernelGGL(k_boundary, dim3((unsigned)((n + 255) / 256), (unsigned)s.nshots), dim3(256), 0, st, s, b,
+                       step, rec_row);
     return hipGetLastError();
 }
 

@@ -48,7 +48,8 @@ struct VelRow {
 };
 
 template <int TY, bool UNIFORM, bool ISO>
-__global__ __launch_bounds__(TY) void k_velocity_2d(DevSim s, int chunk) {
+__global__ __launch_bounds__(TY) void k_velocity_2d(DevSim batch, int chunk) {
+    const DevSim s = shot_view(batch, blockIdx.z);
     extern __shared__ float lds_coef[];
     __shared__ float ring[kRing][TY + 2 * kHalo];
     const int tid = threadIdx.x;
@@ -183,7 +184,8 @@ struct PreRow {
 };
 
 template <int TY, bool UNIFORM, bool ISO>
-__global__ __launch_bounds__(TY) void k_pressure_2d(DevSim s, int chunk) {
+__global__ __launch_bounds__(TY) void k_pressure_2d(DevSim batch, int chunk) {
+    const DevSim s = shot_view(batch, blockIdx.z);
     extern __shared__ float lds_coef[];
     __shared__ float ring_y[kRing][TY + 2 * kHalo];  // vy rows r-1, r, r+1 (8-cell halo)
     __shared__ float ring_x[kRing][TY + 2];          // vx rows r-1, r (1-cell halo)
@@ -321,7 +323,7 @@ hipError_t launch_2d(const DevSim& s, int chunk, bool velocity, hipStream_t st) 
     const bool u = s.ndmap == 1;
     const int lds = (u ? 1 : s.ndmap) * 2 * kNCoef * 4;
     const int64_t rows = s.nx - 2 * kHalo;
-    const dim3 g((unsigned)((s.ny + TY - 1) / TY), (unsigned)((rows + chunk - 1) / chunk), 1);
+    const dim3 g((unsigned)((s.ny + TY - 1) / TY), (unsigned)((rows + chunk - 1) / chunk), (unsigned)s.nshots);
     const dim3 b(TY);
     if (velocity) {
         if (u && s.isotropic) hipLaunchKernelGGL((k_velocity_2d<TY, true, true>), g, b, lds, st, s, chunk);

@@ -1,7 +1,7 @@
 // Streaming (x-marching) kernels for 3D grids on gfx950.
 //
 // Grid [nx][ny][nz], z contiguous. A workgroup owns a TY x TZ tile of (y, z) columns — TZ = 64
-// lanes of one wavefront along z, TY wavefronts along y — and a chunk of x planes that it
+// lanes of one wavefront along z, TY = 8 or 16 wavefronts along y — and a chunk of x planes that it
 // marches through one plane per iteration. The axial x stencil reads a per-lane register queue
 // of the last 16 planes of the input field. The y and z stencils and the transverse
 // corrections read the tile of the current planes staged in 4-slot LDS rings (8-cell halo
@@ -15,12 +15,11 @@ namespace fw {
 namespace {
 
 constexpr int TZ = 64;
-constexpr int TY = 8;
-constexpr int NT3 = TZ * TY;
+// TY (rows of the tile along y, one wavefront each) is a template parameter: 8 or 16
 constexpr int kRing3 = 4;
 
 // a staged tile: rows y in [j0 - HY, j0 + TY + HY), columns z in [k0 - HZ, k0 + TZ + HZ)
-template <int HY, int HZ>
+template <int TY, int HY, int HZ>
 struct Tile {
     static constexpr int PY = TY + 2 * HY;
     static constexpr int PZ = TZ + 2 * HZ;
@@ -32,7 +31,7 @@ struct Tile {
     __device__ void stage(const float* __restrict__ f, int64_t r, int64_t j0, int64_t k0, const DevSim& s) {
         float* dst = slot(r);
         const int64_t base = r * s.sx;
-        for (int t = threadIdx.x; t < N; t += NT3) {
+        for (int t = threadIdx.x; t < N; t += TZ * TY) {
             const int yy = t / PZ, zz = t - (t / PZ) * PZ;
             const int64_t jj = j0 - HY + yy, kk = k0 - HZ + zz;
             dst[t] = (jj >= 0 && jj < s.ny && kk >= 0 && kk < s.nz) ? f[base + jj * s.sy + kk] : 0.f;
@@ -53,18 +52,20 @@ __device__ inline const float* coef3(const float* lds_coef, const int32_t* dcmap
 __device__ inline const float* coef_base_3d(const DevSim& s, float* lds_coef, bool uniform) {
     const int nsets = uniform ? 1 : s.ndmap;
     if (nsets > kLdsSets3D) return s.coef;
-    for (int t = threadIdx.x; t < nsets * 3 * kNCoef; t += NT3) lds_coef[t] = s.coef[t];
+    for (int t = threadIdx.x; t < nsets * 3 * kNCoef; t += blockDim.x) lds_coef[t] = s.coef[t];
     return lds_coef;
 }
 
-template <bool UNIFORM, bool ISO>
-__global__ __launch_bounds__(NT3) void k_velocity_3d(DevSim s, int chunk) {
+template <int TY, bool UNIFORM, bool ISO>
+__global__ __launch_bounds__(TZ * TY) void k_velocity_3d(DevSim batch, int chunk) {
+    unsigned zl;
+    const DevSim s = shot_view(batch, shot_of_block(batch, zl));
     extern __shared__ float lds_coef[];
-    __shared__ Tile<kHalo, kHalo> P;  // p planes, 8-cell halo in y and z
+    __shared__ Tile<TY, kHalo, kHalo> P;  // p planes, 8-cell halo in y and z
     const int tz = threadIdx.x % TZ, ty = threadIdx.x / TZ;
     const int64_t k0 = (int64_t)blockIdx.x * TZ, j0 = (int64_t)blockIdx.y * TY;
     const int64_t k = k0 + tz, j = j0 + ty;
-    const int64_t i0 = kHalo + (int64_t)blockIdx.z * chunk;
+    const int64_t i0 = kHalo + (int64_t)zl * chunk;
     const int64_t i1 = min(i0 + chunk, s.nx - kHalo);
     const bool in_grid = j < s.ny && k < s.nz;
     const bool active = j >= kHalo && j < s.ny - kHalo && k >= kHalo && k < s.nz - kHalo;
@@ -94,7 +95,7 @@ __global__ __launch_bounds__(NT3) void k_velocity_3d(DevSim s, int chunk) {
             const float* pm = P.slot(i - 1);
             const float* p0 = P.slot(i);
             const float* pp = P.slot(i + 1);
-            auto A = [&](const float* pl, int dy, int dz) { return pl[Tile<kHalo, kHalo>::at(ty, tz, dy, dz)]; };
+            auto A = [&](const float* pl, int dy, int dz) { return pl[Tile<TY, kHalo, kHalo>::at(ty, tz, dy, dz)]; };
             // x
             {
                 const float* w = coef3<UNIFORM>(coefs, s.dcmap, c, 0);
@@ -153,16 +154,18 @@ __global__ __launch_bounds__(NT3) void k_velocity_3d(DevSim s, int chunk) {
     }
 }
 
-template <bool UNIFORM, bool ISO>
-__global__ __launch_bounds__(NT3) void k_pressure_3d(DevSim s, int chunk) {
+template <int TY, bool UNIFORM, bool ISO>
+__global__ __launch_bounds__(TZ * TY) void k_pressure_3d(DevSim batch, int chunk) {
+    unsigned zl;
+    const DevSim s = shot_view(batch, shot_of_block(batch, zl));
     extern __shared__ float lds_coef[];
-    __shared__ Tile<1, 1> VX;       // vx planes i-1, i
-    __shared__ Tile<kHalo, 1> VY;   // vy planes i-1, i, i+1; 8-cell halo in y
-    __shared__ Tile<1, kHalo> VZ;   // vz planes i-1, i, i+1; 8-cell halo in z
+    __shared__ Tile<TY, 1, 1> VX;       // vx planes i-1, i
+    __shared__ Tile<TY, kHalo, 1> VY;   // vy planes i-1, i, i+1; 8-cell halo in y
+    __shared__ Tile<TY, 1, kHalo> VZ;   // vz planes i-1, i, i+1; 8-cell halo in z
     const int tz = threadIdx.x % TZ, ty = threadIdx.x / TZ;
     const int64_t k0 = (int64_t)blockIdx.x * TZ, j0 = (int64_t)blockIdx.y * TY;
     const int64_t k = k0 + tz, j = j0 + ty;
-    const int64_t i0 = kHalo + (int64_t)blockIdx.z * chunk;
+    const int64_t i0 = kHalo + (int64_t)zl * chunk;
     const int64_t i1 = min(i0 + chunk, s.nx - kHalo);
     const bool in_grid = j < s.ny && k < s.nz;
     const bool active = j >= kHalo && j < s.ny - kHalo && k >= kHalo && k < s.nz - kHalo;
@@ -202,9 +205,9 @@ __global__ __launch_bounds__(NT3) void k_pressure_3d(DevSim s, int chunk) {
             const float* zm = VZ.slot(i - 1);
             const float* z0 = VZ.slot(i);
             const float* zp = VZ.slot(i + 1);
-            auto X = [&](const float* pl, int dy, int dz) { return pl[Tile<1, 1>::at(ty, tz, dy, dz)]; };
-            auto Y = [&](const float* pl, int dy, int dz) { return pl[Tile<kHalo, 1>::at(ty, tz, dy, dz)]; };
-            auto Z = [&](const float* pl, int dy, int dz) { return pl[Tile<1, kHalo>::at(ty, tz, dy, dz)]; };
+            auto X = [&](const float* pl, int dy, int dz) { return pl[Tile<TY, 1, 1>::at(ty, tz, dy, dz)]; };
+            auto Y = [&](const float* pl, int dy, int dz) { return pl[Tile<TY, kHalo, 1>::at(ty, tz, dy, dz)]; };
+            auto Z = [&](const float* pl, int dy, int dz) { return pl[Tile<TY, 1, kHalo>::at(ty, tz, dy, dz)]; };
             float div = 0.f;
             {
                 const float* w = coef3<UNIFORM>(coefs, s.dcmap, c, 0);
@@ -258,9 +261,9 @@ __global__ __launch_bounds__(NT3) void k_pressure_3d(DevSim s, int chunk) {
     }
 }
 
-dim3 grid_3d(const DevSim& s, int chunk) {
-    return dim3((unsigned)((s.nz + TZ - 1) / TZ), (unsigned)((s.ny + TY - 1) / TY),
-                (unsigned)((s.nx - 2 * kHalo + chunk - 1) / chunk));
+dim3 grid_3d(const DevSim& s, int ty, int chunk) {
+    return dim3((unsigned)((s.nz + TZ - 1) / TZ), (unsigned)((s.ny + ty - 1) / ty),
+                (unsigned)((s.nx - 2 * kHalo + chunk - 1) / chunk * s.nshots));
 }
 
 }  // namespace
@@ -270,27 +273,35 @@ static int coef_lds_bytes_3d(const DevSim& s) {
     return nsets > kLdsSets3D ? 0 : nsets * 3 * kNCoef * 4;
 }
 
-hipError_t launch_velocity_3d(const DevSim& s, const LaunchCfg& cfg, hipStream_t st) {
+template <int TY>
+void launch_3d(bool velocity, const DevSim& s, int lds, int chunk, hipStream_t st) {
     const bool u = s.ndmap == 1;
-    const int lds = coef_lds_bytes_3d(s);
+    const dim3 g = grid_3d(s, TY, chunk), b(TZ * TY);
+    if (velocity) {
+        if (u && s.isotropic) hipLaunchKernelGGL((k_velocity_3d<TY, true, true>), g, b, lds, st, s, chunk);
+        else if (u) hipLaunchKernelGGL((k_velocity_3d<TY, true, false>), g, b, lds, st, s, chunk);
+        else if (s.isotropic) hipLaunchKernelGGL((k_velocity_3d<TY, false, true>), g, b, lds, st, s, chunk);
+        else hipLaunchKernelGGL((k_velocity_3d<TY, false, false>), g, b, lds, st, s, chunk);
+    } else {
+        if (u && s.isotropic) hipLaunchKernelGGL((k_pressure_3d<TY, true, true>), g, b, lds, st, s, chunk);
+        else if (u) hipLaunchKernelGGL((k_pressure_3d<TY, true, false>), g, b, lds, st, s, chunk);
+        else if (s.isotropic) hipLaunchKernelGGL((k_pressure_3d<TY, false, true>), g, b, lds, st, s, chunk);
+        else hipLaunchKernelGGL((k_pressure_3d<TY, false, false>), g, b, lds, st, s, chunk);
+    }
+}
+
+// cfg.width is the workgroup size: 512 (tiles of 8 rows in y) or 1024 (16 rows)
+hipError_t launch_velocity_3d(const DevSim& s, const LaunchCfg& cfg, hipStream_t st) {
     const int chunk = cfg.chunk < 1 ? 1 : cfg.chunk;
-    const dim3 g = grid_3d(s, chunk);
-    if (u && s.isotropic) hipLaunchKernelGGL((k_velocity_3d<true, true>), g, dim3(NT3), lds, st, s, chunk);
-    else if (u) hipLaunchKernelGGL((k_velocity_3d<true, false>), g, dim3(NT3), lds, st, s, chunk);
-    else if (s.isotropic) hipLaunchKernelGGL((k_velocity_3d<false, true>), g, dim3(NT3), lds, st, s, chunk);
-    else hipLaunchKernelGGL((k_velocity_3d<false, false>), g, dim3(NT3), lds, st, s, chunk);
+    if (cfg.width == 1024) launch_3d<16>(true, s, coef_lds_bytes_3d(s), chunk, st);
+    else launch_3d<8>(true, s, coef_lds_bytes_3d(s), chunk, st);
     return hipGetLastError();
 }
 
 hipError_t launch_pressure_3d(const DevSim& s, const LaunchCfg& cfg, hipStream_t st) {
-    const bool u = s.ndmap == 1;
-    const int lds = coef_lds_bytes_3d(s);
     const int chunk = cfg.chunk < 1 ? 1 : cfg.chunk;
-    const dim3 g = grid_3d(s, chunk);
-    if (u && s.isotropic) hipLaunchKernelGGL((k_pressure_3d<true, true>), g, dim3(NT3), lds, st, s, chunk);
-    else if (u) hipLaunchKernelGGL((k_pressure_3d<true, false>), g, dim3(NT3), lds, st, s, chunk);
-    else if (s.isotropic) hipLaunchKernelGGL((k_pressure_3d<false, true>), g, dim3(NT3), lds, st, s, chunk);
-    else hipLaunchKernelGGL((k_pressure_3d<false, false>), g, dim3(NT3), lds, st, s, chunk);
+    if (cfg.width == 1024) launch_3d<16>(false, s, coef_lds_bytes_3d(s), chunk, st);
+    else launch_3d<8>(false, s, coef_lds_bytes_3d(s), chunk, st);
     return hipGetLastError();
 }
 

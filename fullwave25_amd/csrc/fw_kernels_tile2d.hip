@@ -58,7 +58,8 @@ __device__ inline const float* cset(const float* lds_coef, int dc, int axis) {
 
 // velocity: p tile rows i0-7 .. i0+TX+7 (TX + 15 rows), columns j0-8 .. j0+TW+8
 template <int TX, bool UNIFORM, bool ISO>
-__global__ __launch_bounds__(kLanes) void k_velocity_tile(DevSim s) {
+__global__ __launch_bounds__(kLanes) void k_velocity_tile(DevSim batch) {
+    const DevSim s = shot_view(batch, blockIdx.z);
     constexpr int PITCH = kTW + 16;
     constexpr int NROWS = TX + 15;
     __shared__ __attribute__((aligned(16))) float P[NROWS * PITCH];
@@ -156,7 +157,8 @@ __global__ __launch_bounds__(kLanes) void k_velocity_tile(DevSim s) {
 // pressure: vx tile rows i0-8 .. i0+TX+6 (TX + 15 rows), columns j0-4 .. j0+TW+4;
 //           vy tile rows i0-1 .. i0+TX (TX + 2 rows), columns j0-8 .. j0+TW+8
 template <int TX, bool UNIFORM, bool ISO>
-__global__ __launch_bounds__(kLanes) void k_pressure_tile(DevSim s) {
+__global__ __launch_bounds__(kLanes) void k_pressure_tile(DevSim batch) {
+    const DevSim s = shot_view(batch, blockIdx.z);
     constexpr int XP = kTW + 8;
     constexpr int XR = TX + 15;
     constexpr int YP = kTW + 16;
@@ -257,7 +259,7 @@ hipError_t launch_tile(const DevSim& s, bool velocity, hipStream_t st) {
     const bool u = s.ndmap == 1;
     const int lds = (u ? 1 : s.ndmap) * 2 * kNCoef * 4;
     const int64_t rows = s.nx - 2 * kHalo;
-    const dim3 g((unsigned)((s.ny + kTW - 1) / kTW), (unsigned)((rows + TX - 1) / TX), 1);
+    const dim3 g((unsigned)((s.ny + kTW - 1) / kTW), (unsigned)((rows + TX - 1) / TX), (unsigned)s.nshots);
     const dim3 b(kLanes);
     if (velocity) {
         if (u && s.isotropic) hipLaunchKernelGGL((k_velocity_tile<TX, true, true>), g, b, lds, st, s);

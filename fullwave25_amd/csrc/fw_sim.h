@@ -1,0 +1,46 @@
+// The engine handle, shared by the forward (fw_api.cpp) and adjoint (fw_adjoint.cpp) host code.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/fullwave_hip.h"
+#include "fw_internal.h"
+
+struct fw_sim {
+    fw_desc d{};
+
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int64_t ncell = 0;
+    int ndim = 2;
+    int nshots = 1;  // transmits sharing this medium, advanced together
+    fw::DevSim ds{};
+    fw::DevBoundary db{};
+    float* state = nullptr;  // p, v[ndim], psi[ndim][2], phi[ndim][2]
+    int64_t nstate = 0;
+    std::map<std::string, void*> dev;  // every other device allocation, by input name
+    std::map<std::string, bool> have;
+    std::vector<int32_t> icc, outc, zero;
+    std::vector<float> dmap;
+    bool ready = false;
+    int64_t step = 0;
+    int64_t rec_cap = 0;
+    fw::LaunchCfg cfg_v, cfg_p;  // velocity and pressure kernels, chosen independently
+    bool pinned[2] = {false, false};  // set by fw_sim_set_kernel_config: not autotuned
+    fw::LaunchCfg pin[2];
+    bool fields_set = false;  // a field was set before the first run: do not tune on it
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    std::vector<int64_t> out_lin_h;  // sensor voxels (linear indices), host copy
+};
+
+
+namespace fw {
+// host helpers of fw_api.cpp used by the adjoint
+int set_error(int code, const char* fmt, ...);
+int prepare(fw_sim* s);                    // check inputs, resolve coordinates, pick kernels
+int forward_step(fw_sim* s);               // one time step on the handle's stream
+}  // namespace fw

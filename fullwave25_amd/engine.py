@@ -19,7 +19,7 @@ LIB_NAME = "libfullwave_hip.so"
 LIB_PATH = Path(__file__).resolve().parent / LIB_NAME
 
 FW_OK = 0
-ABI_VERSION = 2
+ABI_VERSION = 3
 STATUS_NAMES = {
     0: "FW_OK", 1: "FW_ERR_INVALID", 2: "FW_ERR_DEVICE", 3: "FW_ERR_NOMEM",
     4: "FW_ERR_IO", 5: "FW_ERR_STATE", 6: "FW_ERR_UNSUPPORTED",
@@ -29,11 +29,13 @@ KERNEL_KINDS = ("velocity", "pressure", "boundary")
 # every symbol include/fullwave_hip.h declares
 EXPORTED_SYMBOLS = (
     "fw_abi_version", "fw_version_string", "fw_device_count", "fw_last_error", "fw_run_dir",
-    "fw_device_copy_bandwidth",
+    "fw_device_copy_bandwidth", "fw_device_memory",
     "fw_sim_create", "fw_sim_upload", "fw_sim_run", "fw_sim_reset", "fw_sim_step_count",
     "fw_sim_record_count", "fw_sim_download_sensors", "fw_sim_download_field",
-    "fw_sim_set_field", "fw_sim_kernel_config", "fw_sim_run_timed", "fw_sim_destroy",
+    "fw_sim_set_field", "fw_sim_kernel_config", "fw_sim_set_kernel_config", "fw_sim_run_timed",
+    "fw_sim_destroy", "fw_adj_create", "fw_adj_forward", "fw_adj_backward", "fw_adj_destroy",
 )
+VARIANTS = {"percell": 0, "stream": 1, "tile": 2}
 
 
 class EngineError(RuntimeError):
@@ -52,7 +54,7 @@ class FwDesc(ctypes.Structure):
         ("dy", ctypes.c_double), ("dz", ctypes.c_double), ("dt", ctypes.c_double),
         ("c0", ctypes.c_double), ("ncoords", ctypes.c_int64), ("ntic", ctypes.c_int64),
         ("ncoordsout", ctypes.c_int64), ("modt", ctypes.c_int64), ("ncoordszero", ctypes.c_int64),
-        ("ndmap", ctypes.c_int64), ("attenuation_model", ctypes.c_int32), ("reserved", ctypes.c_int32),
+        ("ndmap", ctypes.c_int64), ("attenuation_model", ctypes.c_int32), ("n_shots", ctypes.c_int32),
     ]
 
 
@@ -79,6 +81,7 @@ def load_library(path: Path | None = None) -> ctypes.CDLL:
         lib.fw_device_count.argtypes = [ctypes.POINTER(ctypes.c_int)]
         lib.fw_run_dir.argtypes = [ctypes.c_char_p, ctypes.c_int]
         lib.fw_device_copy_bandwidth.argtypes = [ctypes.c_int, sz, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+        lib.fw_device_memory.argtypes = [ctypes.c_int, ctypes.POINTER(sz), ctypes.POINTER(sz)]
         lib.fw_sim_create.argtypes = [ctypes.POINTER(FwDesc), ctypes.c_int, ctypes.POINTER(vp)]
         lib.fw_sim_upload.argtypes = [vp, ctypes.c_char_p, vp, sz]
         lib.fw_sim_run.argtypes = [vp, i64]
@@ -89,8 +92,13 @@ def load_library(path: Path | None = None) -> ctypes.CDLL:
         lib.fw_sim_download_field.argtypes = [vp, ctypes.c_char_p, vp, sz]
         lib.fw_sim_set_field.argtypes = [vp, ctypes.c_char_p, vp, sz]
         lib.fw_sim_kernel_config.argtypes = [vp, i32, ctypes.POINTER(i32), ctypes.POINTER(i32), ctypes.POINTER(i32)]
+        lib.fw_sim_set_kernel_config.argtypes = [vp, i32, i32, i32, i32]
         lib.fw_sim_run_timed.argtypes = [vp, i64, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(i32)]
         lib.fw_sim_destroy.argtypes = [vp]
+        lib.fw_adj_create.argtypes = [vp, i64, ctypes.POINTER(vp)]
+        lib.fw_adj_forward.argtypes = [vp]
+        lib.fw_adj_backward.argtypes = [vp, vp, sz, vp, vp, sz]
+        lib.fw_adj_destroy.argtypes = [vp]
         if lib.fw_abi_version() != ABI_VERSION:
             raise RuntimeError(f"{p}: ABI version {lib.fw_abi_version()}, expected {ABI_VERSION}")
         if path is None:
@@ -118,31 +126,77 @@ def copy_bandwidth(nbytes: int = 1 << 30, repeats: int = 10, device: int = 0) ->
     return g.value
 
 
+def device_memory(device: int = 0) -> tuple[int, int]:
+    """``(free, total)`` device memory in bytes."""
+    lib = load_library()
+    f, t = ctypes.c_size_t(0), ctypes.c_size_t(0)
+    _check(lib, lib.fw_device_memory(int(device), ctypes.byref(f), ctypes.byref(t)))
+    return f.value, t.value
+
+
+def device_bytes(inp: SimulationInputs, n_shots: int = 1) -> int:
+    """Device memory one handle for ``inp`` allocates (maps once, wave state per shot)."""
+    nd = inp.ndim
+    cells = inp.ncells
+    n_maps = 3 + len(inp.attenuation_map_stems())  # K, rho, dcmap + attenuation maps
+    state = (1 + nd + (0 if inp.exponential else 4 * nd)) * cells
+    per_shot = state + inp.ncoords * inp.ntic + inp.n_records * inp.ncoordsout
+    fixed = n_maps * cells + 2 * (inp.ncoords + inp.ncoordsout + inp.ncoordszero) + inp.ndmap * nd * 9
+    return 4 * (fixed + n_shots * per_shot)
+
+
 def run_dir(sim_dir: Path, device: int = 0) -> None:
     """Run the engine on a simulation directory; writes ``genout.dat`` there."""
     lib = load_library()
     _check(lib, lib.fw_run_dir(str(Path(sim_dir).absolute()).encode(), int(device)))
 
 
-def make_desc(inp: SimulationInputs) -> FwDesc:
+def make_desc(inp: SimulationInputs, n_shots: int = 1) -> FwDesc:
     return FwDesc(
         abi_version=ABI_VERSION, ndim=inp.ndim, isotropic=1 if inp.isotropic else 0, n_relax=inp.n_relax,
         nx=inp.nx, ny=inp.ny, nz=inp.nz if inp.ndim == 3 else 1, nt=inp.nt,
         dx=inp.dx, dy=inp.dy, dz=inp.dz, dt=inp.dt, c0=inp.c0,
         ncoords=inp.ncoords, ntic=inp.ntic, ncoordsout=inp.ncoordsout, modt=inp.modt,
         ncoordszero=inp.ncoordszero, ndmap=inp.ndmap,
-        attenuation_model=1 if inp.exponential else 0,
+        attenuation_model=1 if inp.exponential else 0, n_shots=int(n_shots),
     )
 
 
-class HipSimulation:
-    """One engine handle with its inputs resident in HBM."""
+def kernel_override_string(cfg: dict) -> str:
+    """``{"velocity": {...}, "pressure": {...}}`` (as ``HipSimulation.kernel_config`` returns)
+    -> ``"<velocity>/<pressure>"`` for ``FULLWAVE_HIP_KERNELS``."""
 
-    def __init__(self, inp: SimulationInputs, device: int = 0) -> None:
+    def one(c):
+        if c["variant"] == "percell":
+            return "percell"
+        if c["variant"] == "tile":
+            return f"tile:{c['chunk']}"
+        return f"stream:{c['width']}:{c['chunk']}"
+
+    return f"{one(cfg['velocity'])}/{one(cfg['pressure'])}"
+
+
+class HipSimulation:
+    """One engine handle with its inputs resident in HBM.
+
+    ``shots``: source matrices ``[n_shots, ncoords, ntic]`` of several transmits on this one
+    medium (same source points, different signals), advanced together with the maps stored
+    once; then :meth:`sensors` returns ``[n_shots, n_records, n_sensors]``. Without it the
+    handle runs ``inp.icmat`` alone.
+    """
+
+    def __init__(self, inp: SimulationInputs, device: int = 0, *, shots: np.ndarray | None = None) -> None:
         self.lib = load_library()
         self.inp = inp
+        self.batched = shots is not None
+        if shots is not None:
+            shots = np.ascontiguousarray(shots, dtype=np.float32)
+            if shots.ndim != 3 or shots.shape[1:] != (inp.ncoords, inp.ntic):
+                raise ValueError(f"shots must be [n_shots, {inp.ncoords}, {inp.ntic}], got {shots.shape}")
+        self.n_shots = 1 if shots is None else int(shots.shape[0])
+        self._shots = shots
         self._h = ctypes.c_void_p()
-        desc = make_desc(inp)
+        desc = make_desc(inp, self.n_shots)
         _check(self.lib, self.lib.fw_sim_create(ctypes.byref(desc), int(device), ctypes.byref(self._h)))
         try:
             self._upload_all()
@@ -161,10 +215,19 @@ class HipSimulation:
         self.upload("dcmap", inp.dcmap.astype(np.int32, copy=False))
         self.upload("dmap", inp.dmap.astype(np.float32, copy=False))
         self.upload("icc", inp.icc.astype(np.int32, copy=False))
-        self.upload("icmat", inp.icmat.astype(np.float32, copy=False))
+        self.upload("icmat", inp.icmat.astype(np.float32, copy=False) if self._shots is None else self._shots)
         self.upload("outc", inp.outc.astype(np.int32, copy=False))
         if inp.ncoordszero:
             self.upload("icczero", inp.icczero.astype(np.int32, copy=False))
+
+    def set_shots(self, shots: np.ndarray) -> None:
+        """New source matrices (``[n_shots, ncoords, ntic]``, or ``[ncoords, ntic]`` for a
+        single handle) on the same medium; call :meth:`reset` before running them."""
+        a = np.ascontiguousarray(shots, dtype=np.float32)
+        want = (self.n_shots, self.inp.ncoords, self.inp.ntic) if self.batched else (self.inp.ncoords, self.inp.ntic)
+        if a.shape != want:
+            raise ValueError(f"source matrices must be {want}, got {a.shape}")
+        self.upload("icmat", a)
 
     def run(self, n_steps: int) -> None:
         _check(self.lib, self.lib.fw_sim_run(self._h, int(n_steps)))
@@ -186,6 +249,18 @@ class HipSimulation:
             out[label] = {"variant": name, "width": w.value, "chunk": c.value}
         return out
 
+    def set_kernel_config(self, cfg: dict) -> None:
+        """Pin the kernels (a ``kernel_config()`` dict, either key optional) before the first run."""
+        for kind, label in ((0, "velocity"), (1, "pressure")):
+            if label in cfg:
+                c = cfg[label]
+                _check(self.lib, self.lib.fw_sim_set_kernel_config(self._h, kind, VARIANTS[c["variant"]],
+                                                                     int(c["width"]), int(c["chunk"])))
+
+    def kernel_override(self) -> str:
+        """The kernel configuration as a ``FULLWAVE_HIP_KERNELS`` value that pins it."""
+        return kernel_override_string(self.kernel_config())
+
     def reset(self) -> None:
         _check(self.lib, self.lib.fw_sim_reset(self._h))
 
@@ -196,15 +271,17 @@ class HipSimulation:
         return v.value
 
     def sensors(self) -> np.ndarray:
-        """Recorded traces ``[n_records, n_sensors]`` (genout.dat layout)."""
+        """Recorded traces ``[n_records, n_sensors]`` (genout.dat layout), or
+        ``[n_shots, n_records, n_sensors]`` for a batch."""
         n = ctypes.c_int64(0)
         _check(self.lib, self.lib.fw_sim_record_count(self._h, ctypes.byref(n)))
-        out = np.empty((n.value, self.inp.ncoordsout), dtype=np.float32)
+        out = np.empty((self.n_shots, n.value, self.inp.ncoordsout), dtype=np.float32)
         _check(self.lib, self.lib.fw_sim_download_sensors(self._h, out.ctypes.data_as(ctypes.c_void_p), out.nbytes))
-        return out
+        return out if self.batched else out[0]
 
-    def field(self, name: str) -> np.ndarray:
-        out = np.empty(self.inp.shape, dtype=np.float32)
+    def field(self, name: str, *, all_shots: bool = False) -> np.ndarray:
+        """Wave field ``name`` of the first shot, or ``[n_shots, ...]`` with ``all_shots``."""
+        out = np.empty(((self.n_shots,) if all_shots else ()) + tuple(self.inp.shape), dtype=np.float32)
         _check(self.lib, self.lib.fw_sim_download_field(self._h, name.encode(), out.ctypes.data_as(ctypes.c_void_p), out.nbytes))
         return out
 

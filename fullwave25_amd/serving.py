@@ -1,0 +1,266 @@
+"""Serving: many simulation jobs on one GPU, admitted by device-memory budget.
+
+The reference runs one simulation per engine call and has no server; this module has no
+counterpart there. A :class:`SimulationServer` accepts jobs (engine inputs, optionally a batch
+of source matrices on one medium) from any thread and returns a future per job. A scheduler
+thread admits waiting jobs in arrival order whenever the device memory they need fits the
+budget and a slot is free (a job that does not fit lets smaller later jobs pass it, up to
+``max_bypass`` times, then it waits for memory to free up); each admitted job runs on its own
+engine handle, which owns its own HIP stream, from its own worker thread, so the kernels of
+concurrent jobs overlap on the device. Inputs stay resident in HBM from admission to
+completion; the host only moves the inputs in and the traces out.
+
+Kernel shapes: the first handle of each grid shape times the candidates (the engine's
+autotuner); later jobs of that shape reuse the choice, so a job's numbers do not depend on
+what else is running.
+
+Latency of a job is measured from ``submit`` to its result being available.
+"""
+
+from __future__ import annotations
+
+import threading
+import time
+from collections import deque
+from concurrent.futures import Future
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import engine
+from .solver.inputs import SimulationInputs
+
+
+@dataclass
+class JobRecord:
+    """Timing of one job (seconds, ``time.perf_counter`` clock)."""
+
+    job_id: int
+    cells: int
+    steps: int
+    n_shots: int
+    device_bytes: int
+    submitted: float
+    started: float = 0.0
+    finished: float = 0.0
+    error: str | None = None
+
+    @property
+    def latency(self) -> float:
+        return self.finished - self.submitted
+
+    @property
+    def queued(self) -> float:
+        return self.started - self.submitted
+
+
+@dataclass
+class _Job:
+    record: JobRecord
+    inputs: SimulationInputs
+    shots: np.ndarray | None
+    future: Future
+    bypassed: int = 0
+
+
+@dataclass
+class ServerStats:
+    jobs: int
+    failed: int
+    wall_s: float
+    jobs_per_hour: float
+    latency_p50_s: float
+    latency_p99_s: float
+    queue_p50_s: float
+    glups: float
+    records: list = field(repr=False, default_factory=list)
+
+
+def _shape_key(inp: SimulationInputs, n_shots: int) -> tuple:
+    return (inp.ndim, inp.shape, bool(inp.isotropic), bool(inp.exponential), inp.ndmap == 1, n_shots)
+
+
+class SimulationServer:
+    """Concurrent simulation jobs on one device under a memory budget.
+
+    ``hbm_budget``: bytes of device memory jobs may hold at once (default: 90 % of the free
+    memory when the server starts). ``max_slots``: jobs running at once. ``chunk_steps``: steps
+    per engine call of a worker (a job's progress granularity).
+    """
+
+    def __init__(self, device: int = 0, *, hbm_budget: int | None = None, max_slots: int = 8,
+                 chunk_steps: int = 500, max_bypass: int = 8, runner=None) -> None:
+        self.device = device
+        # runner(server, inputs, shots) -> traces; the engine by default (tests of the
+        # scheduling logic substitute a host-only one)
+        self._runner = runner if runner is not None else SimulationServer._run_engine
+        if hbm_budget is None:
+            free, _ = engine.device_memory(device)
+            hbm_budget = int(0.9 * free)
+        self.budget = int(hbm_budget)
+        self.max_slots = int(max_slots)
+        self.chunk_steps = int(chunk_steps)
+        self.max_bypass = int(max_bypass)
+        self._lock = threading.Condition()
+        self._pending: deque[_Job] = deque()
+        self._in_use = 0
+        self.peak_in_use = 0
+        self.peak_running = 0
+        self._running = 0
+        self._next_id = 0
+        self._closed = False
+        self._records: list[JobRecord] = []
+        self._configs: dict = {}
+        self._config_locks: dict = {}
+        self._workers: list[threading.Thread] = []
+        self._t0 = time.perf_counter()
+        self._scheduler = threading.Thread(target=self._schedule, name="fullwave-scheduler", daemon=True)
+        self._scheduler.start()
+
+    # ---- public ---------------------------------------------------------------------------
+
+    def submit(self, inputs: SimulationInputs, *, shots: np.ndarray | None = None) -> Future:
+        """Queue a job; the future's result is the traces (``[n_records, n_sensors]``, or
+        ``[n_shots, n_records, n_sensors]`` with ``shots``) and ``future.record`` its timing."""
+        n_shots = 1 if shots is None else int(np.asarray(shots).shape[0])
+        need = engine.device_bytes(inputs, n_shots)
+        if need > self.budget:
+            raise ValueError(f"job needs {need} bytes of device memory, the budget is {self.budget}")
+        fut: Future = Future()
+        with self._lock:
+            if self._closed:
+                raise RuntimeError("server is closed")
+            rec = JobRecord(self._next_id, inputs.ncells, inputs.nt, n_shots, need, time.perf_counter())
+            self._next_id += 1
+            fut.record = rec
+            self._pending.append(_Job(rec, inputs, shots, fut))
+            self._lock.notify_all()
+        return fut
+
+    def drain(self) -> None:
+        """Block until every submitted job has finished."""
+        with self._lock:
+            while self._pending or self._running:
+                self._lock.wait()
+
+    def close(self) -> None:
+        self.drain()
+        with self._lock:
+            self._closed = True
+            self._lock.notify_all()
+        self._scheduler.join()
+        for w in self._workers:
+            w.join()
+
+    def __enter__(self) -> SimulationServer:
+        return self
+
+    def __exit__(self, *exc) -> None:
+        self.close()
+
+    def stats(self) -> ServerStats:
+        with self._lock:
+            recs = [r for r in self._records]
+        done = [r for r in recs if r.error is None]
+        if not done:
+            return ServerStats(0, len(recs), 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, recs)
+        t_first = min(r.submitted for r in recs)
+        t_last = max(r.finished for r in recs)
+        wall = max(t_last - t_first, 1e-9)
+        lat = np.array([r.latency for r in done])
+        queued = np.array([r.queued for r in done])
+        cell_steps = sum(r.cells * r.steps * r.n_shots for r in done)
+        return ServerStats(
+            jobs=len(done), failed=len(recs) - len(done), wall_s=wall, jobs_per_hour=len(done) / wall * 3600.0,
+            latency_p50_s=float(np.percentile(lat, 50)), latency_p99_s=float(np.percentile(lat, 99)),
+            queue_p50_s=float(np.percentile(queued, 50)), glups=cell_steps / wall / 1e9, records=recs,
+        )
+
+    # ---- scheduler and workers ------------------------------------------------------------
+
+    def _pick(self) -> _Job | None:
+        """First waiting job that fits, in arrival order; a job passed over too often blocks
+        the ones behind it until it fits."""
+        free = self.budget - self._in_use
+        for idx, job in enumerate(self._pending):
+            if job.record.device_bytes <= free:
+                for earlier in list(self._pending)[:idx]:
+                    earlier.bypassed += 1
+                del self._pending[idx]
+                return job
+            if job.bypassed >= self.max_bypass:
+                return None
+        return None
+
+    def _schedule(self) -> None:
+        while True:
+            with self._lock:
+                while True:
+                    if self._closed and not self._pending:
+                        return
+                    job = self._pick() if self._running < self.max_slots else None
+                    if job is not None:
+                        break
+                    self._lock.wait()
+                self._in_use += job.record.device_bytes
+                self._running += 1
+                self.peak_in_use = max(self.peak_in_use, self._in_use)
+                self.peak_running = max(self.peak_running, self._running)
+            w = threading.Thread(target=self._work, args=(job,), name=f"fullwave-job-{job.record.job_id}", daemon=True)
+            self._workers.append(w)
+            w.start()
+
+    def _kernel_config(self, key, sim: engine.HipSimulation):
+        """Reuse the kernel shapes tuned for this grid shape; the first handle tunes them."""
+        with self._lock:
+            lk = self._config_locks.setdefault(key, threading.Lock())
+        with lk:
+            cfg = self._configs.get(key)
+            if cfg is not None:
+                sim.set_kernel_config(cfg)
+            else:
+                self._configs[key] = sim.kernel_config()  # tunes now, on this handle's arrays
+
+    def _run_engine(self, inp: SimulationInputs, shots: np.ndarray | None) -> np.ndarray:
+        n_shots = 1 if shots is None else int(shots.shape[0])
+        with engine.HipSimulation(inp, self.device, shots=shots) as sim:
+            self._kernel_config(_shape_key(inp, n_shots), sim)
+            left = inp.nt
+            while left > 0:
+                n = min(self.chunk_steps, left)
+                sim.run(n)
+                left -= n
+            return sim.sensors()
+
+    def _work(self, job: _Job) -> None:
+        rec = job.record
+        rec.started = time.perf_counter()
+        try:
+            out = self._runner(self, job.inputs, job.shots)
+            rec.finished = time.perf_counter()
+            job.future.set_result(out)
+        except Exception as exc:  # the job fails, the server goes on
+            rec.finished = time.perf_counter()
+            rec.error = repr(exc)
+            job.future.set_exception(exc)
+        finally:
+            with self._lock:
+                self._in_use -= rec.device_bytes
+                self._running -= 1
+                self._records.append(rec)
+                self._lock.notify_all()
+
+
+def run_sequential(jobs: list[tuple[SimulationInputs, np.ndarray | None]], device: int = 0) -> tuple[list, float]:
+    """The plain path for comparison: one engine handle per job, one job at a time (create,
+    upload, tune, run, download). Returns ``(results, wall seconds)``."""
+    t0 = time.perf_counter()
+    out = []
+    for inp, shots in jobs:
+        with engine.HipSimulation(inp, device, shots=shots) as sim:
+            sim.run(inp.nt)
+            out.append(sim.sensors())
+    return out, time.perf_counter() - t0
+
+
+__all__ = ["JobRecord", "ServerStats", "SimulationServer", "run_sequential"]

@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define FW_ABI_VERSION 2
+#define FW_ABI_VERSION 3
 
 typedef enum fw_status {
     FW_OK = 0,
@@ -59,7 +59,12 @@ typedef struct fw_desc {
      * 1: exponential attenuation, one per-step decay factor per voxel ("a_exp", the reference's
      *    fullwave2_*_exponential_attenuation engines, fullwave/solver/solver.py:193-218) */
     int32_t attenuation_model;
-    int32_t reserved;
+    /* Transmits on this one medium advanced together (0 or 1: a single simulation). Every
+     * wave field, source matrix and record buffer holds n_shots copies; the medium maps are
+     * stored once and each launch covers every shot. The reference runs one transmit per
+     * engine call (fullwave/solver/solver.py:620-745, is_static_map reusing the maps on
+     * disk); this is that loop in one handle. */
+    int32_t n_shots;
 } fw_desc;
 
 typedef struct fw_sim fw_sim;
@@ -77,6 +82,9 @@ const char* fw_last_error(void);
  * `repeats` times and returns the achieved read + write rate in GB/s. Used to calibrate
  * hardware-counter byte counts against a kernel of known traffic. */
 int fw_device_copy_bandwidth(int device, size_t bytes, int repeats, double* gbs);
+/* Free and total device memory in bytes (hipMemGetInfo); no counterpart in the reference.
+ * Used to admit jobs by memory budget. */
+int fw_device_memory(int device, size_t* free_bytes, size_t* total_bytes);
 
 /* ---- directory contract: drop-in for running the engine executable ------------------- */
 /* Reads every input of sim_dir, runs nT steps on `device` and writes sim_dir/genout.dat
@@ -88,7 +96,8 @@ int fw_run_dir(const char* sim_dir, int device);
 int fw_sim_create(const fw_desc* desc, int device, fw_sim** out);
 /* Upload one input by its file stem: "K", "rho", "c", "beta", "dcmap" (int32 per voxel),
  * "dmap" (float32 [9, 2, ndmap]), "icc", "outc", "icczero" (int32 [n, ndim]),
- * "icmat" (float32 [ncoords, ntic]), and the relaxation maps "kappau", "kappax",
+ * "icmat" (float32 [n_shots, ncoords, ntic]; may be uploaded again between runs, e.g. after
+ * fw_sim_reset for the next transmits on the same medium), and the relaxation maps "kappau", "kappax",
  * "apmlu1", "bpmlu1", ... (isotropic) or additionally "kappaw", "kappay", "kappav", "kappaz",
  * "apmlw1", ... (anisotropic), or "a_exp" (exponential model). Per-voxel maps are float32
  * [nx, ny(, nz)]. `bytes` must match. */
@@ -100,12 +109,15 @@ int fw_sim_reset(fw_sim* sim);
 int fw_sim_step_count(const fw_sim* sim, int64_t* steps);
 /* Number of records written so far (ceil(steps / modt)). */
 int fw_sim_record_count(const fw_sim* sim, int64_t* n_records);
-/* Recorded traces, float32 [n_records, ncoordsout] time-major (genout.dat layout). */
+/* Recorded traces, float32 [n_shots, n_records, ncoordsout], each shot time-major (the
+ * genout.dat layout). */
 int fw_sim_download_sensors(fw_sim* sim, float* out, size_t bytes);
-/* Current wave field by name: "p", "vx", "vy", "vz"; float32 [nx, ny(, nz)]. */
+/* Current wave field by name: "p", "vx", "vy", "vz"; float32 [nx, ny(, nz)] of the first
+ * shot, or [n_shots, nx, ny(, nz)] when `bytes` covers every shot. */
 int fw_sim_download_field(fw_sim* sim, const char* name, float* out, size_t bytes);
-/* Overwrite a wave field by name ("p", "vx", "vy", "vz") from float32 [nx, ny(, nz)]; ghost
- * cells are left as given. Used to start from an initial condition or a checkpoint. */
+/* Overwrite a wave field by name ("p", "vx", "vy", "vz") from float32 [nx, ny(, nz)] (first
+ * shot) or [n_shots, nx, ny(, nz)]; ghost cells are left as given. Used to start from an
+ * initial condition or a checkpoint. */
 int fw_sim_set_field(fw_sim* sim, const char* name, const float* values, size_t bytes);
 /* Kernel family and shape the handle runs for `kind` (0 = velocity kernel, 1 = pressure
  * kernel). It has no counterpart in the reference, whose binary fixes its kernels.
@@ -115,14 +127,40 @@ int fw_sim_set_field(fw_sim* sim, const char* name, const float* values, size_t 
  * Chosen on the first run by timing the candidates on the handle's own arrays, separately for
  * the two kernels (or forced for both with
  * the FULLWAVE_HIP_KERNELS environment variable: "percell", "stream[:width[:chunk]]" or
- * "tile[:rows]").
+ * "tile[:rows]", or "<velocity>/<pressure>" to set each on its own).
  * Calling this before the first run makes that choice. */
 int fw_sim_kernel_config(fw_sim* sim, int32_t kind, int32_t* variant, int32_t* width, int32_t* chunk);
+/* Fix the kernel family and shape of `kind` (values as fw_sim_kernel_config reports them)
+ * before the first run, instead of timing the candidates; it overrides FULLWAVE_HIP_KERNELS.
+ * It has no counterpart in the reference. The first run fails with FW_ERR_UNSUPPORTED or
+ * FW_ERR_INVALID if the shape cannot run this grid. Lets a caller reuse a configuration tuned
+ * once for many handles of one grid shape (a server), and keeps results independent of
+ * timing noise. */
+int fw_sim_set_kernel_config(fw_sim* sim, int32_t kind, int32_t variant, int32_t width, int32_t chunk);
 /* Run n_steps with a HIP event pair around every launch on the handle's stream and add the
  * summed milliseconds per kernel kind to ms[0..n_kinds) (kinds: 0 velocity, 1 pressure,
  * 2 source/air/sensor); also returns the number of kinds in *n_kinds. */
 int fw_sim_run_timed(fw_sim* sim, int64_t n_steps, double* ms, int32_t* n_kinds);
 int fw_sim_destroy(fw_sim* sim);
+
+/* ---- adjoint: gradients of a functional of the traces ----------------------------------- */
+/* None of these has a counterpart in the reference, which runs forward simulations only.
+ * For J = sum_{n,j} w[n][j] * rec[n][j] (rec: the traces of a full run from rest, w: e.g. the
+ * residual of an L2 misfit), the sweep returns dJ/dK and dJ/drho per voxel, the coefficient
+ * sets (dcmap, dmap) held fixed. The forward run keeps its state every `checkpoint_every` steps;
+ * the backward sweep recomputes one segment at a time from its checkpoint. */
+typedef struct fw_adj fw_adj;
+/* Bind to a single-transmit handle whose inputs are uploaded (n_shots 1); allocates the
+ * checkpoints (ceil(nT / checkpoint_every) states), one segment of states and scratch. */
+int fw_adj_create(fw_sim* sim, int64_t checkpoint_every, fw_adj** out);
+/* Reset `sim`, run its nT steps from rest keeping checkpoints; the traces are then available
+ * through fw_sim_download_sensors. */
+int fw_adj_forward(fw_adj* adj);
+/* grad_rec: float32 [n_records, ncoordsout] (the weights w); grad_k, grad_rho: float32 grids
+ * [nx, ny(, nz)] (grid_bytes each) receiving dJ/dK and dJ/drho. */
+int fw_adj_backward(fw_adj* adj, const float* grad_rec, size_t grad_bytes, float* grad_k, float* grad_rho,
+                    size_t grid_bytes);
+int fw_adj_destroy(fw_adj* adj);
 
 #ifdef __cplusplus
 }

@@ -69,6 +69,7 @@ def load() -> ctypes.CDLL:
             getattr(lib, f"fwo_run_{sfx}").argtypes = [ctypes.c_void_p, ctypes.c_int64, _P]
             getattr(lib, f"fwo_get_field_{sfx}").argtypes = [ctypes.c_void_p, ctypes.c_int, _P]
             getattr(lib, f"fwo_set_field_{sfx}").argtypes = [ctypes.c_void_p, ctypes.c_int, _P]
+            getattr(lib, f"fwo_adjoint_{sfx}").argtypes = [ctypes.c_void_p, ctypes.c_int64, _P, _P, _P]
         _lib = lib
     return _lib
 
@@ -171,6 +172,23 @@ class OracleSimulation:
         which = {"p": 0, "vx": 1, "vy": 2, "vz": 3}[name]
         arr = np.ascontiguousarray(values, dtype=np.float64)
         getattr(self.lib, f"fwo_set_field_{self.sfx}")(self._h, which, arr.ctypes.data_as(_P))
+
+    def adjoint(self, grad_rec: np.ndarray, n_steps: int | None = None) -> tuple[np.ndarray, np.ndarray]:
+        """Reverse-mode product of ``grad_rec`` ([n_records, n_sensors]) with the Jacobian of
+        the traces of a full run from rest: returns ``(dK, drho)`` on the grid, the coefficient
+        sets held fixed. Runs the forward pass itself (the state is left at its end)."""
+        n = self.inp.nt if n_steps is None else int(n_steps)
+        g = np.ascontiguousarray(grad_rec, dtype=np.float64)
+        if g.shape != (-(-n // self.inp.modt), self.inp.ncoordsout):
+            raise ValueError(f"grad_rec must be [{-(-n // self.inp.modt)}, {self.inp.ncoordsout}]")
+        gk = np.zeros(self.inp.shape)
+        gr = np.zeros(self.inp.shape)
+        rc = getattr(self.lib, f"fwo_adjoint_{self.sfx}")(self._h, n, g.ctypes.data_as(_P), gk.ctypes.data_as(_P),
+                                                          gr.ctypes.data_as(_P))
+        if rc:
+            raise MemoryError("oracle adjoint: allocation failed")
+        self.step = n
+        return gk, gr
 
     def close(self) -> None:
         if self._h:

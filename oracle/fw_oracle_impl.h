@@ -223,3 +223,252 @@ int SFX(fwo_set_field)(void* handle, int which, const double* in) {
     for (int64_t i = 0; i < st->n; ++i) f[i] = (REAL)in[i];
     return 0;
 }
+
+/* ---- adjoint (reverse mode) ------------------------------------------------------------ *
+ * Vector-Jacobian product of the recorded traces with respect to the K and rho maps: runs
+ * nsteps from a zero state keeping every intermediate state, then sweeps back through the
+ * transposed step. grad_rec is [ceil(nsteps / modt)][nout]; gK and grho (length n) receive
+ * sum_{n,j} grad_rec[n][j] * d rec[n][j] / d K[c] (and / d rho[c]), with the coefficient sets
+ * (dcmap) held fixed. The transposed stencils are applied in scatter form (loops over the cells
+ * that read a value), the GPU kernels use the gather form. Single-threaded. */
+
+static inline int64_t SFX(fwo_nstate)(const SFX(fwo_state)* st) {
+    return st->n * (1 + st->P.ndim + (st->aexp ? 0 : 4 * st->P.ndim));
+}
+
+static void SFX(fwo_pack)(const SFX(fwo_state)* st, REAL* dst) {
+    const int nd = st->P.ndim;
+    const int64_t n = st->n;
+    memcpy(dst, st->p, sizeof(REAL) * n);
+    dst += n;
+    for (int a = 0; a < nd; ++a, dst += n) memcpy(dst, st->v[a], sizeof(REAL) * n);
+    if (st->aexp) return;
+    for (int a = 0; a < nd; ++a)
+        for (int nu = 0; nu < 2; ++nu, dst += n) memcpy(dst, st->psi[a][nu], sizeof(REAL) * n);
+    for (int a = 0; a < nd; ++a)
+        for (int nu = 0; nu < 2; ++nu, dst += n) memcpy(dst, st->phi[a][nu], sizeof(REAL) * n);
+}
+
+/* views into a packed state */
+static inline const REAL* SFX(fwo_sp)(const SFX(fwo_state)* st, const REAL* s) { (void)st; return s; }
+static inline const REAL* SFX(fwo_sv)(const SFX(fwo_state)* st, const REAL* s, int a) { return s + st->n * (1 + a); }
+static inline const REAL* SFX(fwo_spsi)(const SFX(fwo_state)* st, const REAL* s, int a, int nu) {
+    return s + st->n * (1 + st->P.ndim + 2 * a + nu);
+}
+static inline const REAL* SFX(fwo_sphi)(const SFX(fwo_state)* st, const REAL* s, int a, int nu) {
+    return s + st->n * (1 + 3 * st->P.ndim + 2 * a + nu);
+}
+
+/* D+_a p at active cell c (the velocity update's derivative) */
+static REAL SFX(fwo_dplus)(const SFX(fwo_state)* st, const REAL* p, int64_t c, int a) {
+    const REAL* w = st->coef + 9 * (int64_t)st->P.dcmap[c];
+    const int64_t sa = st->s[a];
+    REAL acc = 0;
+    for (int m = 1; m <= 8; ++m) acc += w[m - 1] * (p[c + m * sa] - p[c - (m - 1) * sa]);
+    REAL tr = 0;
+    for (int b = 0; b < st->P.ndim; ++b) {
+        if (b == a) continue;
+        const int64_t sb = st->s[b];
+        tr += (p[c + sa + sb] - p[c + sb]) + (p[c + sa - sb] - p[c - sb]);
+    }
+    return (acc + w[8] * tr) / st->h[a];
+}
+
+/* D-_a f at active cell c (the pressure update's derivative of v_a) */
+static REAL SFX(fwo_dminus)(const SFX(fwo_state)* st, const REAL* f, int64_t c, int a) {
+    const REAL* w = st->coef + 9 * (int64_t)st->P.dcmap[c];
+    const int64_t sa = st->s[a];
+    REAL acc = 0;
+    for (int m = 1; m <= 8; ++m) acc += w[m - 1] * (f[c + (m - 1) * sa] - f[c - m * sa]);
+    REAL tr = 0;
+    for (int b = 0; b < st->P.ndim; ++b) {
+        if (b == a) continue;
+        const int64_t sb = st->s[b];
+        tr += (f[c + sb] - f[c - sa + sb]) + (f[c - sb] - f[c - sa - sb]);
+    }
+    return (acc + w[8] * tr) / st->h[a];
+}
+
+/* out += (D+_a)^T g, scattered from active cell c with value g */
+static void SFX(fwo_dplus_t)(const SFX(fwo_state)* st, REAL* out, int64_t c, int a, REAL g) {
+    const REAL* w = st->coef + 9 * (int64_t)st->P.dcmap[c];
+    const int64_t sa = st->s[a];
+    const REAL gh = g / st->h[a];
+    for (int m = 1; m <= 8; ++m) {
+        out[c + m * sa] += w[m - 1] * gh;
+        out[c - (m - 1) * sa] -= w[m - 1] * gh;
+    }
+    for (int b = 0; b < st->P.ndim; ++b) {
+        if (b == a) continue;
+        const int64_t sb = st->s[b];
+        out[c + sa + sb] += w[8] * gh;
+        out[c + sb] -= w[8] * gh;
+        out[c + sa - sb] += w[8] * gh;
+        out[c - sb] -= w[8] * gh;
+    }
+}
+
+/* out += (D-_a)^T h, scattered from active cell c */
+static void SFX(fwo_dminus_t)(const SFX(fwo_state)* st, REAL* out, int64_t c, int a, REAL hv) {
+    const REAL* w = st->coef + 9 * (int64_t)st->P.dcmap[c];
+    const int64_t sa = st->s[a];
+    const REAL hh = hv / st->h[a];
+    for (int m = 1; m <= 8; ++m) {
+        out[c + (m - 1) * sa] += w[m - 1] * hh;
+        out[c - m * sa] -= w[m - 1] * hh;
+    }
+    for (int b = 0; b < st->P.ndim; ++b) {
+        if (b == a) continue;
+        const int64_t sb = st->s[b];
+        out[c + sb] += w[8] * hh;
+        out[c - sa + sb] -= w[8] * hh;
+        out[c - sb] += w[8] * hh;
+        out[c - sa - sb] -= w[8] * hh;
+    }
+}
+
+int SFX(fwo_adjoint)(void* handle, int64_t nsteps, const double* grad_rec, double* gK, double* grho) {
+    SFX(fwo_state)* st = (SFX(fwo_state)*)handle;
+    const fwo_problem* P = &st->P;
+    const int nd = P->ndim;
+    const int64_t n = st->n, ns = SFX(fwo_nstate)(st);
+    const int relax = st->aexp == 0;
+    const int64_t nx = P->nx, ny = P->ny, nz = nd == 3 ? P->nz : 1;
+
+    /* forward from zero, keeping the state after every step (snap[0] = initial) */
+    memset(st->p, 0, sizeof(REAL) * n);
+    for (int a = 0; a < nd; ++a) {
+        memset(st->v[a], 0, sizeof(REAL) * n);
+        for (int nu = 0; nu < 2; ++nu) {
+            memset(st->psi[a][nu], 0, sizeof(REAL) * n);
+            memset(st->phi[a][nu], 0, sizeof(REAL) * n);
+        }
+    }
+    st->step = 0;
+    REAL* snap = (REAL*)malloc(sizeof(REAL) * (size_t)(ns * (nsteps + 1)));
+    if (!snap) return 1;
+    SFX(fwo_pack)(st, snap);
+    for (int64_t it = 0; it < nsteps; ++it) {
+        SFX(fwo_run)(handle, 1, 0);
+        SFX(fwo_pack)(st, snap + ns * (it + 1));
+    }
+
+    REAL* lam = (REAL*)calloc((size_t)(ns + 2 * nd * n), sizeof(REAL)); /* adjoint state + H, G */
+    REAL* lp = lam;
+    REAL* lv[3];
+    REAL* lpsi[3][2];
+    REAL* lphi[3][2];
+    REAL* H = lam + ns;      /* [nd][n] */
+    REAL* G = H + nd * n;    /* [nd][n] */
+    for (int a = 0; a < nd; ++a) {
+        lv[a] = lam + n * (1 + a);
+        for (int nu = 0; nu < 2; ++nu) {
+            lpsi[a][nu] = relax ? lam + n * (1 + nd + 2 * a + nu) : 0;
+            lphi[a][nu] = relax ? lam + n * (1 + 3 * nd + 2 * a + nu) : 0;
+        }
+    }
+    for (int64_t c = 0; c < n; ++c) gK[c] = grho[c] = 0;
+
+    for (int64_t it = nsteps - 1; it >= 0; --it) {
+        const REAL* prev = snap + ns * it;       /* state before step it */
+        const REAL* cur = snap + ns * (it + 1);  /* state after step it */
+        /* records of step it, then the sources and pressure-release points (overwritten) */
+        if (P->modt > 0 && it % P->modt == 0) {
+            const double* row = grad_rec + (it / P->modt) * P->nout;
+            for (int64_t j = 0; j < P->nout; ++j) lp[P->outc[j]] += (REAL)row[j];
+        }
+        if (it < P->ntic)
+            for (int64_t q = 0; q < P->ncoords; ++q) lp[P->icc[q]] = 0;
+        for (int64_t q = 0; q < P->nzero; ++q) lp[P->zero[q]] = 0;
+
+        /* pressure update, transposed */
+        memset(H, 0, sizeof(REAL) * (size_t)(nd * n));
+        for (int64_t i = 0; i < nx; ++i)
+            for (int64_t j = 0; j < ny; ++j)
+                for (int64_t k = 0; k < nz; ++k) {
+                    if (!SFX(fwo_active)(st, i, j, k)) continue;
+                    const int64_t c = (i * ny + j) * nz + k;
+                    const REAL mu = lp[c];
+                    REAL div = 0;
+                    if (relax) {
+                        const REAL t = -st->dt * st->K[c] * mu;
+                        for (int a = 0; a < nd; ++a) {
+                            const REAL hfw = SFX(fwo_dminus)(st, SFX(fwo_sv)(st, cur, a), c, a);
+                            REAL lh = t / st->kp[a][c];
+                            for (int nu = 0; nu < 2; ++nu) {
+                                const REAL tot = lphi[a][nu][c] + t;
+                                lh += st->ap[a][nu][c] * tot;
+                                lphi[a][nu][c] = st->bp[a][nu][c] * tot;
+                                div += SFX(fwo_sphi)(st, cur, a, nu)[c];
+                            }
+                            div += hfw / st->kp[a][c];
+                            H[a * n + c] = lh;
+                        }
+                    } else {
+                        const REAL A = st->aexp[c];
+                        for (int a = 0; a < nd; ++a) {
+                            div += SFX(fwo_dminus)(st, SFX(fwo_sv)(st, cur, a), c, a);
+                            H[a * n + c] = -A * st->dt * st->K[c] * mu;
+                        }
+                        lp[c] = A * mu;
+                        gK[c] += (double)(mu * (-A * st->dt) * div);
+                        continue;
+                    }
+                    gK[c] += (double)(mu * (-st->dt) * div);
+                }
+        for (int64_t i = 0; i < nx; ++i)
+            for (int64_t j = 0; j < ny; ++j)
+                for (int64_t k = 0; k < nz; ++k) {
+                    if (!SFX(fwo_active)(st, i, j, k)) continue;
+                    const int64_t c = (i * ny + j) * nz + k;
+                    for (int a = 0; a < nd; ++a) SFX(fwo_dminus_t)(st, lv[a], c, a, H[a * n + c]);
+                }
+
+        /* velocity update, transposed */
+        memset(G, 0, sizeof(REAL) * (size_t)(nd * n));
+        for (int64_t i = 0; i < nx; ++i)
+            for (int64_t j = 0; j < ny; ++j)
+                for (int64_t k = 0; k < nz; ++k) {
+                    if (!SFX(fwo_active)(st, i, j, k)) continue;
+                    const int64_t c = (i * ny + j) * nz + k;
+                    for (int a = 0; a < nd; ++a) {
+                        const int64_t sa = st->s[a];
+                        const REAL nu_v = lv[a][c];
+                        const REAL rb = (REAL)0.5 * (st->rho[c] + st->rho[c + sa]);
+                        const REAL beta = st->dt / rb;
+                        const REAL g = SFX(fwo_dplus)(st, SFX(fwo_sp)(st, prev), c, a);
+                        REAL u, lg;
+                        if (relax) {
+                            u = g / st->kv[a][c];
+                            lg = -beta * nu_v / st->kv[a][c];
+                            for (int q = 0; q < 2; ++q) {
+                                const REAL tot = lpsi[a][q][c] - beta * nu_v;
+                                lg += st->av[a][q][c] * tot;
+                                lpsi[a][q][c] = st->bv[a][q][c] * tot;
+                                u += SFX(fwo_spsi)(st, cur, a, q)[c];
+                            }
+                        } else {
+                            const REAL A = st->aexp[c];
+                            u = A * g;
+                            lg = -A * beta * nu_v;
+                            lv[a][c] = A * nu_v;
+                        }
+                        G[a * n + c] = lg;
+                        /* v -= beta u with beta = dt / rho_bar, rho_bar = (rho_c + rho_c+e_a) / 2 */
+                        const double drb = (double)(nu_v * (-u)) * (double)(-st->dt / (rb * rb));
+                        grho[c] += 0.5 * drb;
+                        grho[c + sa] += 0.5 * drb;
+                    }
+                }
+        for (int64_t i = 0; i < nx; ++i)
+            for (int64_t j = 0; j < ny; ++j)
+                for (int64_t k = 0; k < nz; ++k) {
+                    if (!SFX(fwo_active)(st, i, j, k)) continue;
+                    const int64_t c = (i * ny + j) * nz + k;
+                    for (int a = 0; a < nd; ++a) SFX(fwo_dplus_t)(st, lp, c, a, G[a * n + c]);
+                }
+    }
+    free(lam);
+    free(snap);
+    return 0;
+}

@@ -51,6 +51,15 @@ def test_invalid_descriptor_rejected_before_device():
     assert b"ghost" in lib.fw_last_error()
 
 
+def test_too_many_shots_rejected_before_device():
+    lib = engine.load_library()
+    desc = engine.FwDesc(abi_version=engine.ABI_VERSION, ndim=3, isotropic=1, n_relax=2, nx=1000, ny=32, nz=32,
+                         nt=1, dx=1e-4, dy=1e-4, dz=1e-4, dt=1e-8, c0=1540, modt=1, ndmap=1, n_shots=70)
+    h = ctypes.c_void_p()
+    assert lib.fw_sim_create(ctypes.byref(desc), 0, ctypes.byref(h)) == 6  # FW_ERR_UNSUPPORTED
+    assert b"split the batch" in lib.fw_last_error()
+
+
 def test_null_arguments_rejected():
     lib = engine.load_library()
     assert lib.fw_sim_run(None, 1) == 1

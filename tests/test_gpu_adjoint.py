@@ -1,0 +1,106 @@
+"""The engine's adjoint (hand-written transposed-step kernels with checkpointed recomputation)
+on the GPU, against the FP64 host adjoint of the oracle (itself checked against finite
+differences in tests/test_oracle_adjoint.py), and through torch autograd against finite
+differences of the FP64 oracle.
+
+Tolerance: relative L2 error of the gradient maps <= 1e-3 (FP32 adjoint vs FP64 adjoint)."""
+
+import dataclasses
+
+import numpy as np
+import pytest
+
+import oracle
+from fullwave25_amd import workloads
+from fullwave25_amd.adjoint import HipAdjoint, TraceFunction, chain_to_c_rho
+from tests.helpers import rel_l2
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-3
+
+CASES = {
+    "2d_iso": lambda: workloads.random_medium_2d(40, 36, 150, seed=60, air=3).inputs,
+    "2d_aniso": lambda: workloads.random_medium_2d(34, 40, 120, seed=61, isotropic=False).inputs,
+    "2d_exp": lambda: workloads.random_medium_2d(40, 36, 150, seed=62, exponential=True).inputs,
+    "3d_iso": lambda: workloads.random_medium_3d(14, 12, 16, 60, seed=63, air=2).inputs,
+}
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_gpu_adjoint_matches_fp64_adjoint(case):
+    inp = CASES[case]()
+    rng = np.random.default_rng(64)
+    w = rng.standard_normal((inp.n_records, inp.ncoordsout)).astype(np.float32)
+    print(f"adjoint {case}: shape {inp.shape}, nt {inp.nt}", flush=True)
+    with HipAdjoint(inp, checkpoint_every=17) as adj:
+        traces = adj.forward()
+        gk, gr = adj.backward(w)
+        gk2, gr2 = adj.backward(w)  # the checkpoints serve any number of sweeps
+    o = oracle.OracleSimulation(inp, 64)
+    want_k, want_r = o.adjoint(w.astype(np.float64))
+    o2 = oracle.OracleSimulation(inp, 64)
+    o2.run(inp.nt)
+    assert rel_l2(traces, o2.sensors()) <= 1e-4
+    ek, er = rel_l2(gk, want_k), rel_l2(gr, want_r)
+    print(f"  rel-L2 dK {ek:.2e}, drho {er:.2e}", flush=True)
+    assert ek <= TOL and er <= TOL
+    np.testing.assert_array_equal(gk, gk2)
+    np.testing.assert_array_equal(gr, gr2)
+
+
+def test_checkpoint_spacing_does_not_change_the_gradient():
+    inp = CASES["2d_iso"]()
+    w = np.random.default_rng(65).standard_normal((inp.n_records, inp.ncoordsout)).astype(np.float32)
+    out = []
+    for every in (1, 23, inp.nt):
+        with HipAdjoint(inp, checkpoint_every=every, kernel_config={"velocity": {"variant": "percell", "width": 0, "chunk": 0},
+                                                                    "pressure": {"variant": "percell", "width": 0, "chunk": 0}}) as adj:
+            adj.forward()
+            out.append(adj.backward(w))
+    for gk, gr in out[1:]:
+        np.testing.assert_array_equal(gk, out[0][0])
+        np.testing.assert_array_equal(gr, out[0][1])
+
+
+def _oracle_loss(inp, K, rho, observed):
+    maps = dict(inp.maps)
+    maps["K"], maps["rho"] = K, rho  # float64: the oracle keeps them exact
+    o = oracle.OracleSimulation(dataclasses.replace(inp, maps=maps), 64)
+    o.run(inp.nt)
+    return 0.5 * float(np.sum((o.sensors() - observed) ** 2))
+
+
+def test_torch_misfit_gradient_matches_finite_differences():
+    """L2 misfit against traces of a perturbed ("true") medium: the gradient from
+    TraceFunction's backward, dotted with a smooth perturbation of c (and of rho), equals the
+    central difference of the FP64 oracle's loss along it."""
+    import torch
+
+    inp = workloads.random_medium_2d(40, 36, 160, seed=66).inputs
+    c0 = np.sqrt(inp.maps["K"].astype(np.float64) / inp.maps["rho"].astype(np.float64))
+    r0 = inp.maps["rho"].astype(np.float64)
+    idx = np.indices(inp.shape)
+    bump = np.exp(-(((idx[0] - inp.nx / 2) ** 2 + (idx[1] - inp.ny / 2) ** 2) / 40.0))
+    true = dataclasses.replace(inp, maps={**inp.maps, "K": (r0 * (c0 * (1 + 0.02 * bump)) ** 2).astype(np.float32)})
+    o = oracle.OracleSimulation(true, 64)
+    o.run(inp.nt)
+    observed = o.sensors()
+
+    c = torch.tensor(c0, dtype=torch.float64, requires_grad=True)
+    rho = torch.tensor(r0, dtype=torch.float64, requires_grad=True)
+    traces = TraceFunction.apply(c, rho, inp, 40)
+    loss = 0.5 * torch.sum((traces.double() - torch.from_numpy(observed)) ** 2)
+    loss.backward()
+    gc, grho = c.grad.numpy(), rho.grad.numpy()
+
+    d = bump * c0 * 1e-3
+    eps = 1e-3
+    fd_c = (_oracle_loss(inp, r0 * (c0 + eps * d) ** 2, r0, observed)
+            - _oracle_loss(inp, r0 * (c0 - eps * d) ** 2, r0, observed)) / (2 * eps)
+    dr = bump * r0 * 1e-3
+    fd_r = (_oracle_loss(inp, (r0 + eps * dr) * c0**2, r0 + eps * dr, observed)
+            - _oracle_loss(inp, (r0 - eps * dr) * c0**2, r0 - eps * dr, observed)) / (2 * eps)
+    ad_c, ad_r = float(np.sum(gc * d)), float(np.sum(grho * dr))
+    print(f"misfit: dc fd {fd_c:.6e} adj {ad_c:.6e}; drho fd {fd_r:.6e} adj {ad_r:.6e}", flush=True)
+    assert abs(ad_c - fd_c) <= 1e-3 * abs(fd_c)
+    assert abs(ad_r - fd_r) <= 1e-3 * abs(fd_r)

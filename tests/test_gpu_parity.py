@@ -188,7 +188,7 @@ def _three_speed_3d(shape, nt, seed):
                       outc=[[20, 20, 20], [15, 30, 40], [30, 22, 11]])
 
 
-@pytest.mark.parametrize("shape", ["stream:64:2", "stream:64:16", "stream:64:9"])
+@pytest.mark.parametrize("shape", ["stream:512:2", "stream:1024:16", "stream:512:9", "stream:1024:5"])
 def test_streaming_3d_matches_per_cell(monkeypatch, shape):
     for inp in (workloads.random_medium_3d(40, 30, 150, 60, seed=33, air=4).inputs,
                 workloads.random_medium_3d(26, 70, 36, 50, seed=34, isotropic=False).inputs,
@@ -242,3 +242,69 @@ def test_solver_end_to_end(tmp_path):
     assert path.name == "genout.dat" and (path.parent / "fw2_execution.log").exists()
     files = np.fromfile(path, np.float32).reshape(-1, sensor.n_sensors)
     np.testing.assert_array_equal(files, out.T)
+
+
+def _shot_stack(inp, n_shots, seed):
+    """Distinct source signals on the same source points: scaled, delayed copies."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for b in range(n_shots):
+        sh = np.roll(inp.icmat, int(rng.integers(0, 12)), axis=1) * float(rng.uniform(0.5, 2.0))
+        out.append(sh.astype(np.float32))
+    return np.stack(out)
+
+
+@pytest.mark.parametrize("case", ["2d_iso", "3d_aniso", "2d_exp"])
+def test_batched_shots_match_oracle(case):
+    """Several transmits on one medium in one handle: every shot against its own FP64 oracle run."""
+    inp = CASES[case]()
+    shots = _shot_stack(inp, 3, seed=40)
+    print(f"batch {case}: shape {inp.shape}, 3 shots", flush=True)
+    with engine.HipSimulation(inp, shots=shots) as sim:
+        sim.run(inp.nt)
+        got = sim.sensors().astype(np.float64)
+        p_all = sim.field("p", all_shots=True)
+    assert got.shape[0] == 3
+    for b in range(3):
+        one = dataclasses.replace(inp, icmat=shots[b])
+        want, p_want = _ref(one)
+        err = rel_l2(got[b], want)
+        print(f"  shot {b}: rel-L2 {err:.3e}", flush=True)
+        assert err <= TOL
+        assert rel_l2(p_all[b], p_want) <= TOL
+
+
+@pytest.mark.parametrize("shape", ["percell", "stream:128:16", "tile:16", "tile:8/stream:256:8"])
+def test_batched_shots_bitwise_equal_single_runs(monkeypatch, shape):
+    """With the kernel shape pinned, each shot of a batch is bit-identical to its own run."""
+    inp = workloads.random_medium_2d(52, 64, 160, seed=41, air=3).inputs
+    shots = _shot_stack(inp, 4, seed=42)
+    monkeypatch.setenv("FULLWAVE_HIP_KERNELS", shape)
+    with engine.HipSimulation(inp, shots=shots) as sim:
+        sim.run(inp.nt)
+        got = sim.sensors()
+    for b in (0, 3):
+        with engine.HipSimulation(dataclasses.replace(inp, icmat=shots[b])) as one:
+            one.run(inp.nt)
+            np.testing.assert_array_equal(got[b], one.sensors())
+
+
+def test_batched_shots_3d_bitwise_and_reuse(monkeypatch):
+    """3D batch (x-marching kernels) equals single runs; new source matrices on the same
+    handle after a reset give the same result as a fresh handle."""
+    inp = workloads.random_medium_3d(20, 22, 70, 60, seed=43).inputs
+    a, b = _shot_stack(inp, 2, seed=44), _shot_stack(inp, 2, seed=45)
+    monkeypatch.setenv("FULLWAVE_HIP_KERNELS", "stream:1024:16")
+    with engine.HipSimulation(inp, shots=a) as sim:
+        sim.run(inp.nt)
+        got_a = sim.sensors()
+        sim.set_shots(b)
+        sim.reset()
+        sim.run(inp.nt)
+        got_b = sim.sensors()
+    with engine.HipSimulation(dataclasses.replace(inp, icmat=a[1])) as one:
+        one.run(inp.nt)
+        np.testing.assert_array_equal(got_a[1], one.sensors())
+    with engine.HipSimulation(inp, shots=b) as fresh:
+        fresh.run(inp.nt)
+        np.testing.assert_array_equal(got_b, fresh.sensors())

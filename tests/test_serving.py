@@ -1,0 +1,85 @@
+"""Scheduling logic of the simulation server, with a host-only runner (no GPU): memory-budget
+admission, the slot limit, arrival order with bounded bypassing, failure isolation and the
+latency statistics."""
+
+import threading
+import time
+
+import numpy as np
+import pytest
+
+from fullwave25_amd import engine
+from fullwave25_amd.serving import SimulationServer
+from tests.helpers import raw_inputs
+
+
+def _inp(n, nt=10):
+    return raw_inputs((n, n), nt=nt, icc=[[20, 20]], icmat=np.ones((1, nt)), outc=[[21, 21]])
+
+
+class _Recorder:
+    def __init__(self, delay=0.05):
+        self.delay = delay
+        self.lock = threading.Lock()
+        self.order = []
+
+    def __call__(self, server, inp, shots):
+        with self.lock:
+            self.order.append(inp.nx)
+        time.sleep(self.delay)
+        if inp.nt == 13:
+            raise RuntimeError("planted failure")
+        return np.full((inp.n_records, inp.ncoordsout), float(inp.nx), np.float32)
+
+
+def test_budget_and_slots_respected():
+    jobs = [_inp(n) for n in (40, 60, 80, 40, 100, 60, 40, 80)]
+    need = [engine.device_bytes(j) for j in jobs]
+    budget = need[4] + need[0]  # the largest job fits only next to the smallest
+    rec = _Recorder()
+    with SimulationServer(hbm_budget=budget, max_slots=3, runner=rec) as srv:
+        futs = [srv.submit(j) for j in jobs]
+        outs = [f.result(timeout=30) for f in futs]
+    for j, o in zip(jobs, outs):
+        assert o[0, 0] == j.nx
+    assert srv.peak_in_use <= budget
+    assert srv.peak_running <= 3
+    st = srv.stats()
+    assert st.jobs == len(jobs) and st.failed == 0
+    assert st.latency_p99_s >= st.latency_p50_s > 0
+
+
+def test_too_large_job_rejected_at_submit():
+    with SimulationServer(hbm_budget=1000, runner=_Recorder()) as srv:
+        with pytest.raises(ValueError, match="budget"):
+            srv.submit(_inp(40))
+
+
+def test_bypass_is_bounded():
+    """A big job at the head lets small ones pass only max_bypass times."""
+    small, big = _inp(40), _inp(120)
+    budget = engine.device_bytes(big)
+    rec = _Recorder(delay=0.02)
+    with SimulationServer(hbm_budget=budget, max_slots=4, max_bypass=2, runner=rec) as srv:
+        first = srv.submit(small)  # occupies part of the budget, so `big` cannot start yet
+        time.sleep(0.005)
+        fb = srv.submit(big)
+        rest = [srv.submit(small) for _ in range(6)]
+        for f in [first, fb, *rest]:
+            f.result(timeout=30)
+    pos_big = rec.order.index(120)
+    assert pos_big <= 1 + 2  # after the first job and at most two bypassing ones
+
+
+def test_failure_is_isolated():
+    rec = _Recorder(delay=0.01)
+    with SimulationServer(hbm_budget=10**9, max_slots=2, runner=rec) as srv:
+        ok = srv.submit(_inp(40))
+        bad = srv.submit(_inp(40, nt=13))
+        ok2 = srv.submit(_inp(60))
+        assert ok.result(timeout=30)[0, 0] == 40
+        with pytest.raises(RuntimeError, match="planted"):
+            bad.result(timeout=30)
+        assert ok2.result(timeout=30)[0, 0] == 60
+    st = srv.stats()
+    assert st.jobs == 2 and st.failed == 1
