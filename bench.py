@@ -101,6 +101,150 @@ def measured_traffic(workload: str, kind: str, kernel_config: dict):
     return d["kernels"][kind]["bytes_per_launch"]
 
 
+def bench_batched(args) -> dict:
+    """Many focused transmits on one 2048 x 1536 medium: every transmit as its own plain engine
+    run (create, upload, tune, run, download), one after the other, against all of them as the
+    shots of one batched handle."""
+    import dataclasses
+
+    from fullwave25_amd import engine, workloads
+
+    t0 = time.time()
+    wl, shots = workloads.steered_transmits_2d(nx=args.nx, ny=args.ny, nt=args.steps, n_transmits=args.transmits)
+    inp = wl.inputs
+    print(f"# {wl.name} {inp.shape} nt={inp.nt}, {args.transmits} transmits, built in {time.time() - t0:.1f}s",
+          file=sys.stderr, flush=True)
+    n = shots.shape[0]
+    with engine.HipSimulation(inp) as warm:  # first-use costs (library load, code objects)
+        warm.run(min(10, inp.nt))
+    t1 = time.perf_counter()
+    seq = []
+    for b in range(n):
+        with engine.HipSimulation(dataclasses.replace(inp, icmat=shots[b])) as sim:
+            sim.run(inp.nt)
+            seq.append(sim.sensors())
+        print(f"# sequential transmit {b + 1}/{n}", file=sys.stderr, flush=True)
+    t_seq = time.perf_counter() - t1
+    t2 = time.perf_counter()
+    with engine.HipSimulation(inp, shots=shots) as sim:
+        sim.run(inp.nt)
+        bat = sim.sensors()
+        kcfg = sim.kernel_config()
+    t_bat = time.perf_counter() - t2
+    worst = max(float(np.linalg.norm(bat[b] - seq[b]) / max(np.linalg.norm(seq[b]), 1e-30)) for b in range(n))
+    cells = inp.ncells * inp.nt * n
+    return {
+        "metric": "batched transmits: jobs/hour vs sequential plain launches (1 GPU)",
+        "value": round(n / t_bat * 3600, 1), "unit": "jobs/hour", "n_gpus": 1, "steps": inp.nt,
+        "higher_is_better": True, "dtype": "f32", "data": "synthetic",
+        "sequential_jobs_per_hour": round(n / t_seq * 3600, 1), "speedup_vs_sequential": round(t_seq / t_bat, 3),
+        "glups_batched": round(cells / t_bat / 1e9, 3), "glups_sequential": round(cells / t_seq / 1e9, 3),
+        "max_rel_l2_batched_vs_sequential": worst, "kernel_config": kcfg,
+        "config": {"workload": f"{wl.name}: {args.nx}x{args.ny} interior ({inp.nx}x{inp.ny} extended), "
+                               f"{n} steered focused transmits, {inp.nt} steps each",
+                   "transmits": n, "cells": inp.ncells},
+    }
+
+
+def _served_jobs(args):
+    """The served mix: 2D forward jobs on 512^2..4096^2 grids with 1000..8000 steps (70 %),
+    misfit-gradient jobs on 1024^2 (20 %), small 3D forward jobs (10 %), seeded."""
+    from fullwave25_amd import workloads
+
+    import dataclasses
+
+    rng = np.random.default_rng(7)
+    sizes = (512, 1024, 2048, 4096)
+    media = {}
+
+    def medium(n, steps):
+        # one medium per grid size, built for the longest run; a job runs its own step count
+        if n not in media:
+            media[n] = workloads.plane_wave_2d(nx=n, ny=n, nt=8000, sensor_depth=n // 2).inputs
+        return dataclasses.replace(media[n], nt=steps)
+
+    jobs = []
+    adj_inp = None
+    for _ in range(args.jobs):
+        u = rng.random()
+        steps = int(rng.integers(2, 17)) * 500
+        if u < 0.7:
+            jobs.append(("forward", medium(int(rng.choice(sizes)), steps)))
+        elif u < 0.9:
+            adj_inp = adj_inp if adj_inp is not None else medium(1024, 2000)
+            jobs.append(("adjoint", adj_inp))
+        else:
+            jobs.append(("forward3d", None))
+    return jobs, media
+
+
+def bench_served(args) -> dict:
+    """Poisson arrivals of the served mix through the simulation server, against the same jobs
+    run one at a time on plain engine handles."""
+    from fullwave25_amd import adjoint, engine, workloads
+    from fullwave25_amd.serving import SimulationServer
+
+    t0 = time.time()
+    jobs, _ = _served_jobs(args)
+    w3 = workloads.plane_wave_3d(args.nz, args.nz, args.nz, nt=1000) if any(k == "forward3d" for k, _ in jobs) else None
+    jobs = [(k, w3.inputs if k == "forward3d" else inp) for k, inp in jobs]
+    observed = {}
+    for k, inp in jobs:
+        if k == "adjoint" and id(inp) not in observed:
+            # "observed" data: the same setup in a 2 % faster medium
+            true = adjoint.with_medium(inp, 1.02 * np.sqrt(inp.maps["K"] / inp.maps["rho"]), inp.maps["rho"])
+            with engine.HipSimulation(true) as sim:
+                sim.run(true.nt)
+                observed[id(inp)] = sim.sensors().astype(np.float64)
+    print(f"# served mix: {len(jobs)} jobs built in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+
+    def run_plain(kind, inp):
+        if kind == "adjoint":
+            with adjoint.HipAdjoint(inp) as adj:
+                tr = adj.forward()
+                adj.backward((tr - observed[id(inp)]).astype(np.float32))
+            return
+        with engine.HipSimulation(inp) as sim:
+            sim.run(inp.nt)
+            sim.sensors()
+
+    t1 = time.perf_counter()
+    for k, inp in jobs:
+        run_plain(k, inp)
+    t_seq = time.perf_counter() - t1
+    print(f"# sequential: {t_seq:.1f}s", file=sys.stderr, flush=True)
+
+    rng = np.random.default_rng(8)
+    rate = args.arrival_factor * len(jobs) / t_seq  # arrivals per second
+    gaps = rng.exponential(1.0 / rate, len(jobs))
+    with SimulationServer(max_slots=args.slots) as srv:
+        futs = []
+        t_next = time.perf_counter()
+        for (k, inp), gap in zip(jobs, gaps):
+            t_next += gap
+            while time.perf_counter() < t_next:
+                time.sleep(min(0.002, t_next - time.perf_counter()))
+            if k == "adjoint":
+                futs.append(srv.submit_adjoint(inp, observed[id(inp)]))
+            else:
+                futs.append(srv.submit(inp))
+        for f in futs:
+            f.result()
+    st = srv.stats()
+    return {
+        "metric": "served jobs/hour with p50/p99 latency (1 GPU)", "value": round(st.jobs_per_hour, 1),
+        "unit": "jobs/hour", "n_gpus": 1, "higher_is_better": True, "dtype": "f32", "data": "synthetic",
+        "latency_p50_s": round(st.latency_p50_s, 3), "latency_p99_s": round(st.latency_p99_s, 3),
+        "queue_p50_s": round(st.queue_p50_s, 3), "sequential_jobs_per_hour": round(len(jobs) / t_seq * 3600, 1),
+        "speedup_vs_sequential": round(st.jobs_per_hour / (len(jobs) / t_seq * 3600), 3),
+        "served_glups": round(st.glups, 3), "peak_concurrent_jobs": srv.peak_running,
+        "config": {"workload": f"{len(jobs)} jobs: 2D forward 512^2-4096^2 x 1000-8000 steps, 1024^2 misfit "
+                               f"gradients (2000 steps, checkpoint every 200), {args.nz}^3 3D forward (1000 steps); "
+                               f"Poisson arrivals at {args.arrival_factor} x the sequential service rate",
+                   "slots": args.slots, "jobs": len(jobs)},
+    }
+
+
 def cpu_baseline(inp, steps: int) -> dict:
     import oracle
 
@@ -144,7 +288,18 @@ def main() -> int:
                                                        "CONFIG_JSON"),
                     help="summarize rocprofv3 --pmc runs into per-launch HBM traffic (JSON) and exit")
     ap.add_argument("--copy-bytes", type=int, default=268435456)
+    ap.add_argument("--transmits", type=int, default=32, help="batched_2d: transmits on the shared medium")
+    ap.add_argument("--jobs", type=int, default=20, help="served_mix: number of jobs")
+    ap.add_argument("--slots", type=int, default=4, help="served_mix: concurrent jobs")
+    ap.add_argument("--arrival-factor", type=float, default=2.0,
+                    help="served_mix: arrival rate as a multiple of the sequential service rate")
     args = ap.parse_args()
+    if args.config in ("batched_2d", "served_mix"):
+        if args.gpus != 1:
+            print(json.dumps({"error": "this build measures one GPU; run with --gpus 1"}))
+            return 2
+        print(json.dumps(bench_batched(args) if args.config == "batched_2d" else bench_served(args)), flush=True)
+        return 0
     if args.pmc_summary:
         dirs = [Path(p) for p in args.pmc_summary]
         print(json.dumps(pmc_summary(*dirs[:4], args.copy_bytes, dirs[4]), indent=1))
@@ -172,7 +327,8 @@ def main() -> int:
         shapes = ["percell"] + [f"stream:{w}:{c}" for w in (128, 256) for c in (8, 16, 32)]
         shapes += [f"tile:{c}" for c in (8, 16, 32)]
         if inp.ndim == 3:
-            shapes = ["percell"] + [f"stream:{w}:{c}" for w in (512, 1024) for c in (8, 16, 32)]
+            shapes = ["percell"] + [f"{v}:{w}:{c}" for v in ("stream", "pipe", "pipelite") for w in (512, 1024)
+                                    for c in (8, 16, 32)]
         bpc = algorithmic_bytes_per_cell(inp.ndim, inp.isotropic)
         updated = int(np.prod([s - 16 for s in inp.shape]))
         for shape in shapes:

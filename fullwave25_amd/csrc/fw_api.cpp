@@ -155,6 +155,14 @@ int check_cfg(fw_sim* s, const fw::LaunchCfg& c) {
             return fail(FW_ERR_INVALID, "tile kernels: width 256 and rows 8, 16 or 32");
         return FW_OK;
     }
+    if (c.variant == 3 || c.variant == 4) {
+        if (s->ndim != 3 || !fw::streaming_supported(s->ds))
+            return fail(FW_ERR_UNSUPPORTED, "pipelined kernels need a 3D relaxation grid");
+        if (c.width != 512 && c.width != 1024)
+            return fail(FW_ERR_INVALID, "pipelined kernels: workgroup size (width) must be 512 or 1024");
+        if (c.chunk < 1 || c.chunk > 4096) return fail(FW_ERR_INVALID, "pipelined kernels: chunk out of range");
+        return FW_OK;
+    }
     if (c.variant == 1) {
         if (!fw::streaming_supported(s->ds)) return fail(FW_ERR_UNSUPPORTED, "this grid needs the per-cell kernels");
         if (s->ndim == 2 && c.width != 64 && c.width != 128 && c.width != 256)
@@ -164,7 +172,7 @@ int check_cfg(fw_sim* s, const fw::LaunchCfg& c) {
         if (c.chunk < 2 || c.chunk > 4096) return fail(FW_ERR_INVALID, "x-marching kernels: chunk out of range");
         return FW_OK;
     }
-    return fail(FW_ERR_INVALID, "kernel variant must be 0, 1 or 2");
+    return fail(FW_ERR_INVALID, "kernel variant must be 0 to 4");
 }
 
 int parse_kernel_env(fw_sim* s, const char* env, fw::LaunchCfg* out) {
@@ -183,6 +191,15 @@ int parse_kernel_env(fw_sim* s, const char* env, fw::LaunchCfg* out) {
         c.chunk = sscanf(env, "tile:%d", &ch) == 1 ? ch : 16;
         if (c.chunk != 8 && c.chunk != 16 && c.chunk != 32)
             return fail(FW_ERR_INVALID, "FULLWAVE_HIP_KERNELS: tile rows must be 8, 16 or 32");
+    } else if (!strncmp(env, "pipe", 4) || !strncmp(env, "pipelite", 8)) {
+        const bool lite = !strncmp(env, "pipelite", 8);
+        c.variant = lite ? 4 : 3;
+        int w = 0, ch = 0;
+        const int n = sscanf(env + (lite ? 8 : 4), ":%d:%d", &w, &ch);
+        if (n >= 1) c.width = w;
+        if (n >= 2) c.chunk = ch;
+        int rc = check_cfg(s, c);
+        if (rc != FW_OK) return rc;
     } else if (!strncmp(env, "stream", 6)) {
         if (!stream_ok) return fail(FW_ERR_UNSUPPORTED, "FULLWAVE_HIP_KERNELS=stream: this grid needs the per-cell kernels");
         c.variant = 1;
@@ -193,7 +210,8 @@ int parse_kernel_env(fw_sim* s, const char* env, fw::LaunchCfg* out) {
         int rc = check_cfg(s, c);
         if (rc != FW_OK) return rc;
     } else {
-        return fail(FW_ERR_INVALID, "FULLWAVE_HIP_KERNELS must be 'percell', 'stream[:width[:chunk]]' or 'tile[:rows]'");
+        return fail(FW_ERR_INVALID,
+                    "FULLWAVE_HIP_KERNELS must be 'percell', 'stream[:width[:chunk]]', 'tile[:rows]' or 'pipe[:width[:chunk]]'");
     }
     *out = c;
     return FW_OK;
@@ -258,13 +276,14 @@ int choose_kernels_unpinned(fw_sim* s) {
                 cands.push_back(c);
             }
         } else {
-            for (int w : {512, 1024}) {
-                fw::LaunchCfg c;
-                c.variant = 1;
-                c.width = w;
-                c.chunk = ch;
-                cands.push_back(c);
-            }
+            for (int v : {1, 3, 4})
+                for (int w : {512, 1024}) {
+                    fw::LaunchCfg c;
+                    c.variant = v;
+                    c.width = w;
+                    c.chunk = ch;
+                    cands.push_back(c);
+                }
         }
     }
     if (fw::tile2d_supported(s->ds)) {
@@ -432,6 +451,48 @@ int launch_step(fw_sim* s, bool timed, double* ms) {
     return FW_OK;
 }
 
+// Capture graph_steps steps (velocity, pressure, boundary pass with the device step counter,
+// counter advance) into one graph, once per handle.
+int capture_graph(fw_sim* s) {
+    if (!s->step_dev) FW_HIP(hipMalloc(&s->step_dev, sizeof(int64_t)));
+    hipGraph_t graph = nullptr;
+    FW_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+    hipError_t e = hipSuccess;
+    for (int64_t it = 0; it < s->graph_steps && e == hipSuccess; ++it) {
+        e = fw::launch_velocity(s->ds, s->cfg_v, s->stream);
+        if (e == hipSuccess) e = fw::launch_pressure(s->ds, s->cfg_p, s->stream);
+        if (e == hipSuccess) e = fw::launch_boundary_dev(s->ds, s->db, s->step_dev, s->d.modt, s->stream);
+        if (e == hipSuccess) e = fw::launch_tick(s->step_dev, -1, s->stream);
+    }
+    hipError_t e2 = hipStreamEndCapture(s->stream, &graph);
+    if (e != hipSuccess || e2 != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return fail(FW_ERR_DEVICE, "graph capture failed: %s", hipGetErrorString(e != hipSuccess ? e : e2));
+    }
+    e = hipGraphInstantiate(&s->graph, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+        s->graph = nullptr;
+        return fail(FW_ERR_DEVICE, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
+    }
+    return FW_OK;
+}
+
+// Replay the captured steps as often as they fit in *left; leaves the rest in *left.
+int run_graph(fw_sim* s, int64_t* left) {
+    if (!s->graph) {
+        int rc = capture_graph(s);
+        if (rc != FW_OK) return rc;
+    }
+    FW_HIP(fw::launch_tick(s->step_dev, s->step, s->stream));  // device counter = host step
+    while (*left >= s->graph_steps) {
+        FW_HIP(hipGraphLaunch(s->graph, s->stream));
+        s->step += s->graph_steps;
+        *left -= s->graph_steps;
+    }
+    return FW_OK;
+}
+
 int run_impl(fw_sim* s, int64_t n_steps, bool timed, double* ms) {
     if (!s) return fail(FW_ERR_INVALID, "null handle");
     if (n_steps < 0) return fail(FW_ERR_INVALID, "n_steps < 0");
@@ -443,7 +504,12 @@ int run_impl(fw_sim* s, int64_t n_steps, bool timed, double* ms) {
     if (s->step + n_steps > s->d.nt)
         return fail(FW_ERR_INVALID, "running %lld steps from step %lld exceeds nt = %lld", (long long)n_steps,
                     (long long)s->step, (long long)s->d.nt);
-    for (int64_t it = 0; it < n_steps; ++it) {
+    int64_t left = n_steps;
+    if (!timed && s->graph_steps > 0 && left >= s->graph_steps) {
+        int rc = run_graph(s, &left);
+        if (rc != FW_OK) return rc;
+    }
+    for (int64_t it = 0; it < left; ++it) {
         int rc = launch_step(s, timed, ms);
         if (rc != FW_OK) return rc;
     }
@@ -767,11 +833,24 @@ int fw_sim_kernel_config(fw_sim* s, int32_t kind, int32_t* variant, int32_t* wid
     return FW_OK;
 }
 
+int fw_sim_set_graph_steps(fw_sim* s, int64_t steps) {
+    if (!s) return fail(FW_ERR_INVALID, "null handle");
+    if (steps < 0 || steps > 100000) return fail(FW_ERR_INVALID, "graph steps must be in [0, 100000]");
+    DeviceGuard g(s->device);
+    if (s->graph) {
+        FW_HIP(hipStreamSynchronize(s->stream));
+        (void)hipGraphExecDestroy(s->graph);
+        s->graph = nullptr;
+    }
+    s->graph_steps = steps;
+    return FW_OK;
+}
+
 int fw_sim_set_kernel_config(fw_sim* s, int32_t kind, int32_t variant, int32_t width, int32_t chunk) {
     if (!s) return fail(FW_ERR_INVALID, "null handle");
     if (kind != 0 && kind != 1) return fail(FW_ERR_INVALID, "kind must be 0 (velocity) or 1 (pressure)");
     if (s->ready) return fail(FW_ERR_STATE, "the kernels are fixed at the first run; set them before it");
-    if (variant < 0 || variant > 2) return fail(FW_ERR_INVALID, "kernel variant must be 0, 1 or 2");
+    if (variant < 0 || variant > 4) return fail(FW_ERR_INVALID, "kernel variant must be 0 to 4");
     fw::LaunchCfg c;
     c.variant = variant;
     c.width = width;
@@ -785,6 +864,8 @@ int fw_sim_destroy(fw_sim* s) {
     if (!s) return FW_OK;
     DeviceGuard g(s->device);
     if (s->stream) (void)hipStreamSynchronize(s->stream);
+    if (s->graph) (void)hipGraphExecDestroy(s->graph);
+    if (s->step_dev) (void)hipFree(s->step_dev);
     for (auto& kv : s->dev) (void)hipFree(kv.second);
     if (s->state) (void)hipFree(s->state);
     for (auto& e : s->ev)

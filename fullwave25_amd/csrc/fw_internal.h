@@ -114,6 +114,9 @@ enum KernelKind { kVelocity = 0, kPressure = 1, kBoundary = 2, kNumKinds = 3 };
 //              (2D: 64, 128 or 256; 3D: fixed 64 x 8 tile) and `chunk` planes per workgroup
 //   variant 2: 2D tiled kernels with 4-wide vector access; tiles of `chunk` rows (8, 16, 32)
 //              by 256 columns (needs ny % 4 == 0)
+//   variant 3: 3D x-marching kernels with software pipelining (planes and per-cell operands
+//              loaded one iteration ahead); `width` = workgroup size 512 or 1024
+//   variant 4: as 3, with the per-cell operands loaded at the top of their own iteration
 struct LaunchCfg {
     int variant = 0;
     int width = 128;
@@ -123,9 +126,13 @@ struct LaunchCfg {
 bool streaming_supported(const DevSim& s);
 bool tile2d_supported(const DevSim& s);
 hipError_t launch_velocity(const DevSim& s, const LaunchCfg& cfg, hipStream_t st);
+hipError_t launch_pipe3d(const DevSim& s, const LaunchCfg& cfg, bool velocity, hipStream_t st);
 hipError_t launch_pressure(const DevSim& s, const LaunchCfg& cfg, hipStream_t st);
 hipError_t launch_boundary(const DevSim& s, const DevBoundary& b, int64_t step, int64_t modt,
                            hipStream_t st);
+hipError_t launch_boundary_dev(const DevSim& s, const DevBoundary& b, const int64_t* step, int64_t modt,
+                               hipStream_t st);
+hipError_t launch_tick(int64_t* step, int64_t set, hipStream_t st);
 hipError_t launch_reciprocal(float* data, int64_t n, hipStream_t st);
 hipError_t launch_copy(const void* src, void* dst, int64_t n16, hipStream_t st);
 hipError_t launch_adj_records(float* lp, const float* grow, const int64_t* vox, const int32_t* first,

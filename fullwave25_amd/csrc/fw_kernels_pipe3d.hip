@@ -1,0 +1,413 @@
+// Software-pipelined x-marching kernels for 3D grids (kernel variant 3).
+//
+// Same tiling and arithmetic as fw_kernels_stream3d.hip (a TY x 64 tile of (y, z) columns per
+// workgroup, 64 lanes along z, a chunk of x planes marched one per iteration, the axial x
+// stencil in a 16-deep register queue, the y/z stencils from planes staged in 4-slot LDS
+// rings), with the memory latency taken off the critical path:
+//   * the plane that enters the LDS ring at iteration i is loaded into registers at
+//     iteration i - 1 and only written to LDS at iteration i (each lane owns fixed elements of
+//     the staged tile; their in-plane offsets are computed once);
+//   * every per-cell operand of plane i + 1 (the next queue element, rho, the memory
+//     variables, a/b, 1/kappa, the velocities or pressure, the coefficient-set index) is loaded
+//     at iteration i, before the barrier, and consumed one iteration later.
+// One barrier per plane. A 4-slot ring is enough: at iteration i the lanes write slot i + 1
+// (mod 4) while lanes still in iteration i - 1 read slots i - 2 .. i.
+#include "fw_internal.h"
+
+namespace fw {
+
+namespace {
+
+constexpr int PZ3 = 64;  // lanes along z
+constexpr int kRingP = 4;
+constexpr int kLdsSetsP = 128;
+
+// The staged tile: rows y in [j0 - HY, j0 + TY + HY), columns z in [k0 - HZ, k0 + 64 + HZ).
+template <int TY, int HY, int HZ>
+struct PTile {
+    static constexpr int PY = TY + 2 * HY;
+    static constexpr int PZ = PZ3 + 2 * HZ;
+    static constexpr int N = PY * PZ;
+    static constexpr int NT = PZ3 * TY;
+    static constexpr int R = (N + NT - 1) / NT;  // elements per lane
+    float v[kRingP][N];
+    __device__ float* slot(int64_t r) { return v[r & (kRingP - 1)]; }
+    __device__ static int at(int ty, int tz, int dy, int dz) { return (ty + HY + dy) * PZ + (tz + HZ + dz); }
+};
+
+// A lane's share of one staged plane: fixed offsets within a plane, values in flight.
+template <int TY, int HY, int HZ>
+struct Stager {
+    using T = PTile<TY, HY, HZ>;
+    int32_t off[T::R];  // in-plane offset, or -1 outside the grid / past the tile
+    float val[T::R];
+    __device__ void init(const DevSim& s, int64_t j0, int64_t k0) {
+#pragma unroll
+        for (int r = 0; r < T::R; ++r) {
+            const int e = threadIdx.x + r * T::NT;
+            off[r] = -1;
+            if (e < T::N) {
+                const int yy = e / T::PZ, zz = e - (e / T::PZ) * T::PZ;
+                const int64_t jj = j0 - HY + yy, kk = k0 - HZ + zz;
+                if (jj >= 0 && jj < s.ny && kk >= 0 && kk < s.nz) off[r] = (int32_t)(jj * s.sy + kk);
+            }
+        }
+    }
+    __device__ void load(const float* __restrict__ f, int64_t plane, const DevSim& s) {
+        const float* base = f + plane * s.sx;
+#pragma unroll
+        for (int r = 0; r < T::R; ++r) val[r] = off[r] >= 0 ? base[off[r]] : 0.f;
+    }
+    __device__ void store(T& tile, int64_t plane) const {
+        float* dst = tile.slot(plane);
+#pragma unroll
+        for (int r = 0; r < T::R; ++r) {
+            const int e = threadIdx.x + r * T::NT;
+            if (r < T::R - 1 || e < T::N) dst[e] = val[r];
+        }
+    }
+};
+
+template <bool UNIFORM>
+__device__ inline const float* coefp(const float* coefs, int32_t dc, int axis) {
+    if (UNIFORM) return coefs + axis * kNCoef;
+    return coefs + ((int64_t)dc * 3 + axis) * kNCoef;
+}
+
+__device__ inline const float* coef_table(const DevSim& s, float* lds_coef, bool uniform) {
+    const int nsets = uniform ? 1 : s.ndmap;
+    if (nsets > kLdsSetsP) return s.coef;
+    for (int t = threadIdx.x; t < nsets * 3 * kNCoef; t += blockDim.x) lds_coef[t] = s.coef[t];
+    return lds_coef;
+}
+
+// per-cell operands of one plane for the velocity update
+template <bool ISO>
+struct VelOps {
+    static constexpr int NA = ISO ? 1 : 3;
+    float qnew;            // p(i + 8): the queue's next element
+    float rho_x, rho_y, rho_z;  // rho at c + e_x, c + e_y, c + e_z
+    float psi[3][2];
+    float v[3];
+    float a[NA][2], b[NA][2];
+    float ik[NA];
+    int32_t dc;
+};
+
+template <bool UNIFORM, bool ISO>
+__device__ inline void load_vel(VelOps<ISO>& o, const DevSim& s, int64_t i, int64_t col, bool in_grid, bool active) {
+    const int64_t c = i * s.sx + col;
+    o.qnew = in_grid ? s.p[c + 8 * s.sx] : 0.f;
+    o.rho_x = in_grid ? s.rho[c + s.sx] : 0.f;
+    if (!active) return;
+    o.rho_y = s.rho[c + s.sy];
+    o.rho_z = s.rho[c + 1];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        o.v[a] = s.v[a][c];
+        o.psi[a][0] = s.psi[a][0][c];
+        o.psi[a][1] = s.psi[a][1][c];
+    }
+#pragma unroll
+    for (int a = 0; a < VelOps<ISO>::NA; ++a) {
+        o.a[a][0] = s.av[a][0][c];
+        o.a[a][1] = s.av[a][1][c];
+        o.b[a][0] = s.bv[a][0][c];
+        o.b[a][1] = s.bv[a][1][c];
+        o.ik[a] = s.ikv[a][c];
+    }
+    o.dc = UNIFORM ? 0 : s.dcmap[c];
+}
+
+// AHEAD: per-cell operands loaded one iteration ahead (more registers); otherwise at the top
+// of their own iteration, before the barrier
+template <int TY, bool UNIFORM, bool ISO, bool AHEAD>
+__global__ __launch_bounds__(PZ3 * TY) void k_velocity_pipe3d(DevSim batch, int chunk) {
+    using TP = PTile<TY, kHalo, kHalo>;
+    unsigned zl;
+    const DevSim s = shot_view(batch, shot_of_block(batch, zl));
+    extern __shared__ float lds_coef[];
+    __shared__ TP P;
+    const int tz = threadIdx.x % PZ3, ty = threadIdx.x / PZ3;
+    const int64_t k0 = (int64_t)blockIdx.x * PZ3, j0 = (int64_t)blockIdx.y * TY;
+    const int64_t k = k0 + tz, j = j0 + ty;
+    const int64_t i0 = kHalo + (int64_t)zl * chunk;
+    const int64_t i1 = min(i0 + chunk, s.nx - kHalo);
+    const bool in_grid = j < s.ny && k < s.nz;
+    const bool active = j >= kHalo && j < s.ny - kHalo && k >= kHalo && k < s.nz - kHalo;
+    const int64_t sx = s.sx, col = j * s.sy + k;
+    const float* coefs = coef_table(s, lds_coef, UNIFORM);
+
+    Stager<TY, kHalo, kHalo> st;
+    st.init(s, j0, k0);
+    float q[16];  // q[m] = p(i - 7 + m)
+#pragma unroll
+    for (int m = 0; m < 15; ++m) q[m + 1] = in_grid ? s.p[(i0 - 7 + m) * sx + col] : 0.f;
+    st.load(s.p, i0 - 1, s);
+    st.store(P, i0 - 1);
+    st.load(s.p, i0, s);
+    st.store(P, i0);
+    st.load(s.p, i0 + 1, s);
+    float rho_i = in_grid ? s.rho[i0 * sx + col] : 0.f;
+    VelOps<ISO> cur, nxt;
+    if (AHEAD) load_vel<UNIFORM, ISO>(cur, s, i0, col, in_grid, active);
+
+    for (int64_t i = i0; i < i1; ++i) {
+        st.store(P, i + 1);
+        const bool more = i + 1 < i1;
+        if (!AHEAD) load_vel<UNIFORM, ISO>(cur, s, i, col, in_grid, active);
+        if (more) {
+            st.load(s.p, i + 2, s);
+            if (AHEAD) load_vel<UNIFORM, ISO>(nxt, s, i + 1, col, in_grid, active);
+        }
+#pragma unroll
+        for (int m = 0; m < 15; ++m) q[m] = q[m + 1];
+        q[15] = cur.qnew;
+        __syncthreads();
+        if (active) {
+            const int64_t c = i * sx + col;
+            const float* pm = P.slot(i - 1);
+            const float* p0 = P.slot(i);
+            const float* pp = P.slot(i + 1);
+            auto A = [&](const float* pl, int dy, int dz) { return pl[TP::at(ty, tz, dy, dz)]; };
+            float D[3];
+            {
+                const float* w = coefp<UNIFORM>(coefs, cur.dc, 0);
+                float acc = 0.f;
+#pragma unroll
+                for (int m = 1; m <= 8; ++m) acc += w[m - 1] * (q[7 + m] - q[8 - m]);
+                float tr = 0.f;
+                tr += (A(pp, 1, 0) - A(p0, 1, 0)) + (A(pp, -1, 0) - A(p0, -1, 0));
+                tr += (A(pp, 0, 1) - A(p0, 0, 1)) + (A(pp, 0, -1) - A(p0, 0, -1));
+                D[0] = acc + w[8] * tr;
+            }
+            {
+                const float* w = coefp<UNIFORM>(coefs, cur.dc, 1);
+                float acc = 0.f;
+#pragma unroll
+                for (int m = 1; m <= 8; ++m) acc += w[m - 1] * (A(p0, m, 0) - A(p0, -(m - 1), 0));
+                float tr = 0.f;
+                tr += (A(pp, 1, 0) - A(pp, 0, 0)) + (A(pm, 1, 0) - A(pm, 0, 0));
+                tr += (A(p0, 1, 1) - A(p0, 0, 1)) + (A(p0, 1, -1) - A(p0, 0, -1));
+                D[1] = acc + w[8] * tr;
+            }
+            {
+                const float* w = coefp<UNIFORM>(coefs, cur.dc, 2);
+                float acc = 0.f;
+#pragma unroll
+                for (int m = 1; m <= 8; ++m) acc += w[m - 1] * (A(p0, 0, m) - A(p0, 0, -(m - 1)));
+                float tr = 0.f;
+                tr += (A(pp, 0, 1) - A(pp, 0, 0)) + (A(pm, 0, 1) - A(pm, 0, 0));
+                tr += (A(p0, 1, 1) - A(p0, 1, 0)) + (A(p0, -1, 1) - A(p0, -1, 0));
+                D[2] = acc + w[8] * tr;
+            }
+            const float rnb[3] = {cur.rho_x, cur.rho_y, cur.rho_z};
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const int ax = ISO ? 0 : a;
+                const float qq1 = cur.b[ax][0] * cur.psi[a][0] + cur.a[ax][0] * D[a];
+                const float qq2 = cur.b[ax][1] * cur.psi[a][1] + cur.a[ax][1] * D[a];
+                s.psi[a][0][c] = qq1;
+                s.psi[a][1][c] = qq2;
+                const float rho_bar = 0.5f * (rho_i + rnb[a]);
+                s.v[a][c] = cur.v[a] - s.dt / rho_bar * (D[a] * cur.ik[ax] + (qq1 + qq2));
+            }
+        }
+        rho_i = cur.rho_x;
+        if (AHEAD) cur = nxt;
+    }
+}
+
+// per-cell operands of one plane for the pressure update
+template <bool ISO>
+struct PreOps {
+    static constexpr int NA = ISO ? 1 : 3;
+    float qnew;  // vx(i + 7)
+    float phi[3][2];
+    float a[NA][2], b[NA][2];
+    float ik[NA];
+    float K, p;
+    int32_t dc;
+};
+
+template <bool UNIFORM, bool ISO>
+__device__ inline void load_pre(PreOps<ISO>& o, const DevSim& s, int64_t i, int64_t col, bool in_grid, bool active) {
+    const int64_t c = i * s.sx + col;
+    o.qnew = in_grid ? s.v[0][c + 7 * s.sx] : 0.f;
+    if (!active) return;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        o.phi[a][0] = s.phi[a][0][c];
+        o.phi[a][1] = s.phi[a][1][c];
+    }
+#pragma unroll
+    for (int a = 0; a < PreOps<ISO>::NA; ++a) {
+        o.a[a][0] = s.ap[a][0][c];
+        o.a[a][1] = s.ap[a][1][c];
+        o.b[a][0] = s.bp[a][0][c];
+        o.b[a][1] = s.bp[a][1][c];
+        o.ik[a] = s.ikp[a][c];
+    }
+    o.K = s.K[c];
+    o.p = s.p[c];
+    o.dc = UNIFORM ? 0 : s.dcmap[c];
+}
+
+template <int TY, bool UNIFORM, bool ISO, bool AHEAD>
+__global__ __launch_bounds__(PZ3 * TY) void k_pressure_pipe3d(DevSim batch, int chunk) {
+    using TX_ = PTile<TY, 1, 1>;
+    using TY_ = PTile<TY, kHalo, 1>;
+    using TZ_ = PTile<TY, 1, kHalo>;
+    unsigned zl;
+    const DevSim s = shot_view(batch, shot_of_block(batch, zl));
+    extern __shared__ float lds_coef[];
+    __shared__ TX_ VX;  // vx planes i-1, i
+    __shared__ TY_ VY;  // vy planes i-1, i, i+1
+    __shared__ TZ_ VZ;  // vz planes i-1, i, i+1
+    const int tz = threadIdx.x % PZ3, ty = threadIdx.x / PZ3;
+    const int64_t k0 = (int64_t)blockIdx.x * PZ3, j0 = (int64_t)blockIdx.y * TY;
+    const int64_t k = k0 + tz, j = j0 + ty;
+    const int64_t i0 = kHalo + (int64_t)zl * chunk;
+    const int64_t i1 = min(i0 + chunk, s.nx - kHalo);
+    const bool in_grid = j < s.ny && k < s.nz;
+    const bool active = j >= kHalo && j < s.ny - kHalo && k >= kHalo && k < s.nz - kHalo;
+    const int64_t sx = s.sx, col = j * s.sy + k;
+    const float* coefs = coef_table(s, lds_coef, UNIFORM);
+
+    Stager<TY, 1, 1> sx_;
+    Stager<TY, kHalo, 1> sy_;
+    Stager<TY, 1, kHalo> sz_;
+    sx_.init(s, j0, k0);
+    sy_.init(s, j0, k0);
+    sz_.init(s, j0, k0);
+    float q[16];  // q[m] = vx(i - 8 + m)
+#pragma unroll
+    for (int m = 0; m < 15; ++m) q[m + 1] = in_grid ? s.v[0][(i0 - 8 + m) * sx + col] : 0.f;
+    sx_.load(s.v[0], i0 - 1, s);
+    sx_.store(VX, i0 - 1);
+    sy_.load(s.v[1], i0 - 1, s);
+    sy_.store(VY, i0 - 1);
+    sy_.load(s.v[1], i0, s);
+    sy_.store(VY, i0);
+    sz_.load(s.v[2], i0 - 1, s);
+    sz_.store(VZ, i0 - 1);
+    sz_.load(s.v[2], i0, s);
+    sz_.store(VZ, i0);
+    sx_.load(s.v[0], i0, s);
+    sy_.load(s.v[1], i0 + 1, s);
+    sz_.load(s.v[2], i0 + 1, s);
+    PreOps<ISO> cur, nxt;
+    if (AHEAD) load_pre<UNIFORM, ISO>(cur, s, i0, col, in_grid, active);
+
+    for (int64_t i = i0; i < i1; ++i) {
+        sx_.store(VX, i);
+        sy_.store(VY, i + 1);
+        sz_.store(VZ, i + 1);
+        const bool more = i + 1 < i1;
+        if (!AHEAD) load_pre<UNIFORM, ISO>(cur, s, i, col, in_grid, active);
+        if (more) {
+            sx_.load(s.v[0], i + 1, s);
+            sy_.load(s.v[1], i + 2, s);
+            sz_.load(s.v[2], i + 2, s);
+            if (AHEAD) load_pre<UNIFORM, ISO>(nxt, s, i + 1, col, in_grid, active);
+        }
+#pragma unroll
+        for (int m = 0; m < 15; ++m) q[m] = q[m + 1];
+        q[15] = cur.qnew;
+        __syncthreads();
+        if (active) {
+            const int64_t c = i * sx + col;
+            const float* xm = VX.slot(i - 1);
+            const float* x0 = VX.slot(i);
+            const float* ym = VY.slot(i - 1);
+            const float* y0 = VY.slot(i);
+            const float* yp = VY.slot(i + 1);
+            const float* zm = VZ.slot(i - 1);
+            const float* z0 = VZ.slot(i);
+            const float* zp = VZ.slot(i + 1);
+            auto X = [&](const float* pl, int dy, int dz) { return pl[TX_::at(ty, tz, dy, dz)]; };
+            auto Y = [&](const float* pl, int dy, int dz) { return pl[TY_::at(ty, tz, dy, dz)]; };
+            auto Z = [&](const float* pl, int dy, int dz) { return pl[TZ_::at(ty, tz, dy, dz)]; };
+            float D[3];
+            {
+                const float* w = coefp<UNIFORM>(coefs, cur.dc, 0);
+                float acc = 0.f;
+#pragma unroll
+                for (int m = 1; m <= 8; ++m) acc += w[m - 1] * (q[7 + m] - q[8 - m]);
+                float tr = 0.f;
+                tr += (X(x0, 1, 0) - X(xm, 1, 0)) + (X(x0, -1, 0) - X(xm, -1, 0));
+                tr += (X(x0, 0, 1) - X(xm, 0, 1)) + (X(x0, 0, -1) - X(xm, 0, -1));
+                D[0] = acc + w[8] * tr;
+            }
+            {
+                const float* w = coefp<UNIFORM>(coefs, cur.dc, 1);
+                float acc = 0.f;
+#pragma unroll
+                for (int m = 1; m <= 8; ++m) acc += w[m - 1] * (Y(y0, m - 1, 0) - Y(y0, -m, 0));
+                float tr = 0.f;
+                tr += (Y(yp, 0, 0) - Y(yp, -1, 0)) + (Y(ym, 0, 0) - Y(ym, -1, 0));
+                tr += (Y(y0, 0, 1) - Y(y0, -1, 1)) + (Y(y0, 0, -1) - Y(y0, -1, -1));
+                D[1] = acc + w[8] * tr;
+            }
+            {
+                const float* w = coefp<UNIFORM>(coefs, cur.dc, 2);
+                float acc = 0.f;
+#pragma unroll
+                for (int m = 1; m <= 8; ++m) acc += w[m - 1] * (Z(z0, 0, m - 1) - Z(z0, 0, -m));
+                float tr = 0.f;
+                tr += (Z(zp, 0, 0) - Z(zp, 0, -1)) + (Z(zm, 0, 0) - Z(zm, 0, -1));
+                tr += (Z(z0, 1, 0) - Z(z0, 1, -1)) + (Z(z0, -1, 0) - Z(z0, -1, -1));
+                D[2] = acc + w[8] * tr;
+            }
+            float div = 0.f;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const int ax = ISO ? 0 : a;
+                const float qq1 = cur.b[ax][0] * cur.phi[a][0] + cur.a[ax][0] * D[a];
+                const float qq2 = cur.b[ax][1] * cur.phi[a][1] + cur.a[ax][1] * D[a];
+                s.phi[a][0][c] = qq1;
+                s.phi[a][1][c] = qq2;
+                div += D[a] * cur.ik[ax] + (qq1 + qq2);
+            }
+            s.p[c] = cur.p - s.dt * cur.K * div;
+        }
+        if (AHEAD) cur = nxt;
+    }
+}
+
+template <int TY, bool AHEAD>
+void launch_pipe(bool velocity, const DevSim& s, int chunk, hipStream_t st) {
+    const bool u = s.ndmap == 1;
+    const int nsets = u ? 1 : s.ndmap;
+    const int lds = nsets > kLdsSetsP ? 0 : nsets * 3 * kNCoef * 4;
+    const dim3 g((unsigned)((s.nz + PZ3 - 1) / PZ3), (unsigned)((s.ny + TY - 1) / TY),
+                 (unsigned)((s.nx - 2 * kHalo + chunk - 1) / chunk * s.nshots));
+    const dim3 b(PZ3 * TY);
+    if (velocity) {
+        if (u && s.isotropic) hipLaunchKernelGGL((k_velocity_pipe3d<TY, true, true, AHEAD>), g, b, lds, st, s, chunk);
+        else if (u) hipLaunchKernelGGL((k_velocity_pipe3d<TY, true, false, AHEAD>), g, b, lds, st, s, chunk);
+        else if (s.isotropic) hipLaunchKernelGGL((k_velocity_pipe3d<TY, false, true, AHEAD>), g, b, lds, st, s, chunk);
+        else hipLaunchKernelGGL((k_velocity_pipe3d<TY, false, false, AHEAD>), g, b, lds, st, s, chunk);
+    } else {
+        if (u && s.isotropic) hipLaunchKernelGGL((k_pressure_pipe3d<TY, true, true, AHEAD>), g, b, lds, st, s, chunk);
+        else if (u) hipLaunchKernelGGL((k_pressure_pipe3d<TY, true, false, AHEAD>), g, b, lds, st, s, chunk);
+        else if (s.isotropic) hipLaunchKernelGGL((k_pressure_pipe3d<TY, false, true, AHEAD>), g, b, lds, st, s, chunk);
+        else hipLaunchKernelGGL((k_pressure_pipe3d<TY, false, false, AHEAD>), g, b, lds, st, s, chunk);
+    }
+}
+
+}  // namespace
+
+// cfg.width is the workgroup size: 512 (tiles of 8 rows in y) or 1024 (16 rows); variant 3
+// loads the per-cell operands one iteration ahead, variant 4 at the top of their iteration
+hipError_t launch_pipe3d(const DevSim& s, const LaunchCfg& cfg, bool velocity, hipStream_t st) {
+    const int chunk = cfg.chunk < 1 ? 1 : cfg.chunk;
+    const bool ahead = cfg.variant == 3;
+    if (cfg.width == 1024 && ahead) launch_pipe<16, true>(velocity, s, chunk, st);
+    else if (cfg.width == 1024) launch_pipe<16, false>(velocity, s, chunk, st);
+    else if (ahead) launch_pipe<8, true>(velocity, s, chunk, st);
+    else launch_pipe<8, false>(velocity, s, chunk, st);
+    return hipGetLastError();
+}
+
+}  // namespace fw

@@ -132,6 +132,37 @@ __global__ __launch_bounds__(256) void k_boundary(DevSim s, DevBoundary b, int64
     }
 }
 
+// The same pass with the step read from device memory (steps replayed from a captured graph).
+__global__ __launch_bounds__(256) void k_boundary_dev(DevSim s, DevBoundary b, const int64_t* step, int64_t modt) {
+    const int64_t n = *step;
+    const int64_t rec_row = (modt > 0 && n % modt == 0) ? n / modt : -1;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t shot = blockIdx.y;
+    float* p = s.p + shot * s.shot_stride;
+    const float* icmat_t = b.icmat_t + shot * b.icmat_stride;
+    const bool inject = n < b.ntic;
+    if (inject && t < b.ncoords) {
+        const int64_t lin = b.src_lin[t];
+        if (lin >= 0) p[lin] = icmat_t[n * b.ncoords + t];
+    }
+    if (t < b.nzero) p[b.zero_lin[t]] = 0.f;
+    if (rec_row >= 0 && t < b.nout) {
+        float val;
+        if (b.out_air[t]) {
+            val = 0.f;
+        } else {
+            const int32_t src = b.out_src[t];
+            val = (inject && src >= 0) ? icmat_t[n * b.ncoords + src] : p[b.out_lin[t]];
+        }
+        b.rec[shot * b.rec_stride + rec_row * b.nout + t] = val;
+    }
+}
+
+// Advance (or set, with set >= 0) the device step counter; one lane writes it.
+__global__ void k_tick(int64_t* step, int64_t set) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *step = set >= 0 ? set : *step + 1;
+}
+
 __global__ __launch_bounds__(256) void k_copy(const float4* __restrict__ src, float4* __restrict__ dst, int64_t n) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += stride) dst[t] = src[t];
@@ -177,6 +208,22 @@ hipError_t launch_boundary(const DevSim& s, const DevBoundary& b, int64_t step, 
     if (rec_row < 0 && step >= b.ntic && b.nzero == 0) return hipSuccess;
     hipLaunchKernelGGL(k_boundary, dim3((unsigned)((n + 255) / 256), (unsigned)s.nshots), dim3(256), 0, st, s, b,
                        step, rec_row);
+    return hipGetLastError();
+}
+
+hipError_t launch_boundary_dev(const DevSim& s, const DevBoundary& b, const int64_t* step, int64_t modt,
+                               hipStream_t st) {
+    int64_t n = b.ncoords;
+    if (b.nzero > n) n = b.nzero;
+    if (b.nout > n) n = b.nout;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_boundary_dev, dim3((unsigned)((n + 255) / 256), (unsigned)s.nshots), dim3(256), 0, st, s, b,
+                       step, modt);
+    return hipGetLastError();
+}
+
+hipError_t launch_tick(int64_t* step, int64_t set, hipStream_t st) {
+    hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, st, step, set);
     return hipGetLastError();
 }
 

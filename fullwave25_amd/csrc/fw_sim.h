@@ -35,6 +35,11 @@ struct fw_sim {
     bool fields_set = false;  // a field was set before the first run: do not tune on it
     hipEvent_t ev[2] = {nullptr, nullptr};
     std::vector<int64_t> out_lin_h;  // sensor voxels (linear indices), host copy
+    // steady steps replayed from a captured HIP graph (graph_steps > 0): the step number then
+    // lives in device memory (step_dev) and the graph advances it
+    int64_t graph_steps = 0;
+    int64_t* step_dev = nullptr;
+    hipGraphExec_t graph = nullptr;
 };
 
 

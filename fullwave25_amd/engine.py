@@ -32,10 +32,11 @@ EXPORTED_SYMBOLS = (
     "fw_device_copy_bandwidth", "fw_device_memory",
     "fw_sim_create", "fw_sim_upload", "fw_sim_run", "fw_sim_reset", "fw_sim_step_count",
     "fw_sim_record_count", "fw_sim_download_sensors", "fw_sim_download_field",
-    "fw_sim_set_field", "fw_sim_kernel_config", "fw_sim_set_kernel_config", "fw_sim_run_timed",
+    "fw_sim_set_field", "fw_sim_kernel_config", "fw_sim_set_kernel_config", "fw_sim_set_graph_steps",
+    "fw_sim_run_timed",
     "fw_sim_destroy", "fw_adj_create", "fw_adj_forward", "fw_adj_backward", "fw_adj_destroy",
 )
-VARIANTS = {"percell": 0, "stream": 1, "tile": 2}
+VARIANTS = {"percell": 0, "stream": 1, "tile": 2, "pipe": 3, "pipelite": 4}
 
 
 class EngineError(RuntimeError):
@@ -93,6 +94,7 @@ def load_library(path: Path | None = None) -> ctypes.CDLL:
         lib.fw_sim_set_field.argtypes = [vp, ctypes.c_char_p, vp, sz]
         lib.fw_sim_kernel_config.argtypes = [vp, i32, ctypes.POINTER(i32), ctypes.POINTER(i32), ctypes.POINTER(i32)]
         lib.fw_sim_set_kernel_config.argtypes = [vp, i32, i32, i32, i32]
+        lib.fw_sim_set_graph_steps.argtypes = [vp, i64]
         lib.fw_sim_run_timed.argtypes = [vp, i64, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(i32)]
         lib.fw_sim_destroy.argtypes = [vp]
         lib.fw_adj_create.argtypes = [vp, i64, ctypes.POINTER(vp)]
@@ -171,7 +173,7 @@ def kernel_override_string(cfg: dict) -> str:
             return "percell"
         if c["variant"] == "tile":
             return f"tile:{c['chunk']}"
-        return f"stream:{c['width']}:{c['chunk']}"
+        return f"{c['variant']}:{c['width']}:{c['chunk']}"
 
     return f"{one(cfg['velocity'])}/{one(cfg['pressure'])}"
 
@@ -245,7 +247,7 @@ class HipSimulation:
         for kind, label in ((0, "velocity"), (1, "pressure")):
             v, w, c = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
             _check(self.lib, self.lib.fw_sim_kernel_config(self._h, kind, ctypes.byref(v), ctypes.byref(w), ctypes.byref(c)))
-            name = {0: "percell", 1: "stream", 2: "tile"}.get(v.value, str(v.value))
+            name = {0: "percell", 1: "stream", 2: "tile", 3: "pipe", 4: "pipelite"}.get(v.value, str(v.value))
             out[label] = {"variant": name, "width": w.value, "chunk": c.value}
         return out
 
@@ -256,6 +258,10 @@ class HipSimulation:
                 c = cfg[label]
                 _check(self.lib, self.lib.fw_sim_set_kernel_config(self._h, kind, VARIANTS[c["variant"]],
                                                                      int(c["width"]), int(c["chunk"])))
+
+    def set_graph_steps(self, steps: int) -> None:
+        """Replay steady steps from a captured HIP graph of ``steps`` steps (0 turns it off)."""
+        _check(self.lib, self.lib.fw_sim_set_graph_steps(self._h, int(steps)))
 
     def kernel_override(self) -> str:
         """The kernel configuration as a ``FULLWAVE_HIP_KERNELS`` value that pins it."""

@@ -61,6 +61,19 @@ class _Job:
     shots: np.ndarray | None
     future: Future
     bypassed: int = 0
+    observed: np.ndarray | None = None  # adjoint jobs: the traces to fit
+    checkpoint_every: int = 0
+
+
+def adjoint_device_bytes(inp: SimulationInputs, checkpoint_every: int) -> int:
+    """Device memory of an adjoint job: the forward handle, its checkpoints, one segment of
+    states, the adjoint state and scratch."""
+    nd = inp.ndim
+    state = (1 + nd + (0 if inp.exponential else 4 * nd)) * inp.ncells
+    every = max(1, min(checkpoint_every, inp.nt))
+    n_ckpt = -(-inp.nt // every)
+    extra = (n_ckpt + every + 1) * state + (3 * nd + 2) * inp.ncells + inp.n_records * inp.ncoordsout
+    return engine.device_bytes(inp) + 4 * extra
 
 
 @dataclass
@@ -134,6 +147,26 @@ class SimulationServer:
             self._next_id += 1
             fut.record = rec
             self._pending.append(_Job(rec, inputs, shots, fut))
+            self._lock.notify_all()
+        return fut
+
+    def submit_adjoint(self, inputs: SimulationInputs, observed: np.ndarray, *,
+                       checkpoint_every: int = 200) -> Future:
+        """Queue a misfit-gradient job: the result is ``{"loss", "grad_K", "grad_rho",
+        "traces"}`` for ``0.5 * ||traces - observed||^2`` (gradients with respect to the K and
+        rho maps of ``inputs``; see :mod:`fullwave25_amd.adjoint`)."""
+        need = adjoint_device_bytes(inputs, checkpoint_every)
+        if need > self.budget:
+            raise ValueError(f"job needs {need} bytes of device memory, the budget is {self.budget}")
+        fut: Future = Future()
+        with self._lock:
+            if self._closed:
+                raise RuntimeError("server is closed")
+            rec = JobRecord(self._next_id, inputs.ncells, inputs.nt, 1, need, time.perf_counter())
+            self._next_id += 1
+            fut.record = rec
+            self._pending.append(_Job(rec, inputs, None, fut, observed=np.asarray(observed, np.float64),
+                                      checkpoint_every=int(checkpoint_every)))
             self._lock.notify_all()
         return fut
 
@@ -232,11 +265,31 @@ class SimulationServer:
                 left -= n
             return sim.sensors()
 
+    def _run_adjoint(self, job: _Job) -> dict:
+        from .adjoint import HipAdjoint
+
+        inp = job.inputs
+        key = _shape_key(inp, 1)
+        with self._lock:
+            cfg = self._configs.get(key)
+        # the adjoint handle fixes its kernels when it is created: pass a known configuration in
+        with HipAdjoint(inp, checkpoint_every=job.checkpoint_every, device=self.device, kernel_config=cfg) as adj:
+            if cfg is None:
+                with self._lock:
+                    self._configs.setdefault(key, adj.sim.kernel_config())
+            traces = adj.forward()
+            resid = traces.astype(np.float64) - job.observed
+            gk, gr = adj.backward(resid.astype(np.float32))
+        return {"loss": 0.5 * float(np.sum(resid**2)), "grad_K": gk, "grad_rho": gr, "traces": traces}
+
     def _work(self, job: _Job) -> None:
         rec = job.record
         rec.started = time.perf_counter()
         try:
-            out = self._runner(self, job.inputs, job.shots)
+            if job.observed is not None:
+                out = self._run_adjoint(job)
+            else:
+                out = self._runner(self, job.inputs, job.shots)
             rec.finished = time.perf_counter()
             job.future.set_result(out)
         except Exception as exc:  # the job fails, the server goes on

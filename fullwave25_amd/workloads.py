@@ -73,6 +73,24 @@ def plane_wave_2d(nx: int = 512, ny: int = 512, nt: int = 2000, *, n_pml: int = 
                                              n_transition=n_transition, isotropic=isotropic))
 
 
+def plane_wave_3d(nx: int = 256, ny: int = 256, nz: int = 256, nt: int = 1000, *, n_pml: int = 12,
+                  n_transition: int = 8, alpha0: float = 0.5, gamma: float = 1.1, f0: float = 3e6) -> Workload:
+    """Homogeneous 3D medium with a plane-wave transmit from the x = 0 face (a 3-cycle tone
+    burst on every point of it) and a receive plane at mid-depth."""
+    grid = _grid(nx, ny, nt=nt, f0=f0, nz=nz)
+    shape = (grid.nx, grid.ny, grid.nz)
+    medium = Medium(grid, np.full(shape, 1540.0), np.full(shape, 1000.0), np.full(shape, alpha0),
+                    np.full(shape, gamma), np.zeros(shape))
+    mask = np.zeros(shape, dtype=bool)
+    mask[0] = True
+    sig = gaussian_modulated_sinusoidal_signal(grid.nt, grid.duration, 3, 2, f0, 1e5)
+    source = Source(np.tile(sig, (int(mask.sum()), 1)), mask)
+    smask = np.zeros(shape, dtype=bool)
+    smask[nx // 2, ::4, ::4] = True
+    return Workload("plane_wave_3d", _inputs(grid, medium, source, Sensor(smask), n_pml=n_pml,
+                                             n_transition=n_transition))
+
+
 def layered_medium_2d(shape, rng, *, n_layers=6, amp_cells=10.0, dx=None, scatter_per_cell=15,
                       wavelength_cells=12):
     """Seeded layered medium: ``n_layers`` horizontal layers whose interfaces carry a random
@@ -105,6 +123,38 @@ def layered_medium_2d(shape, rng, *, n_layers=6, amp_cells=10.0, dx=None, scatte
     return c, rho, a0, gm
 
 
+def _array_geometry(ny: int, n_elements: int, pitch_cells: int):
+    width = pitch_cells - 1
+    aperture = n_elements * pitch_cells
+    y0 = (ny - aperture) // 2
+    elem_of_col = np.full(ny, -1)
+    for e in range(n_elements):
+        elem_of_col[y0 + e * pitch_cells : y0 + e * pitch_cells + width] = e
+    centres = np.array([y0 + e * pitch_cells + (width - 1) / 2 for e in range(n_elements)])
+    return elem_of_col, centres
+
+
+def focused_signals(grid, coords, elem_of_col, centres, *, focus_m: float, lateral_m: float = 0.0,
+                    f0: float = 3e6, ncycles: int = 3) -> np.ndarray:
+    """Source signals ``[n_points, nt]`` of a transmit focused at depth ``focus_m`` and lateral
+    offset ``lateral_m`` from the array centre, for source points ``coords`` (rows of a thick
+    source get the one-cell transit delay of their row)."""
+    yc = (grid.ny - 1) / 2
+    dist = np.sqrt(focus_m**2 + ((centres - yc) * grid.dx - lateral_m) ** 2)
+    delay = (dist.max() - dist) / grid.c0
+    p0 = np.zeros((coords.shape[0], grid.nt))
+    cache = {}
+    for r, (ix, iy) in enumerate(coords):
+        e = elem_of_col[iy]
+        key = (e, ix)
+        if key not in cache:
+            cache[key] = gaussian_modulated_sinusoidal_signal(
+                grid.nt, grid.duration, ncycles, 2, f0, 1e5, delay_sec=delay[e],
+                i_layer=int(ix), dt_for_layer_delay=grid.dt, cfl_for_layer_delay=grid.cfl)
+        p0[r] = cache[key]
+    return p0
+
+
 def focused_linear_array_2d(nx: int = 4096, ny: int = 2048, nt: int = 8000, *, n_elements: int = 128,
                             pitch_cells: int = 12, focus_m: float = 40e-3, seed: int = 2026,
                             n_pml: int = 36, layers: int = 3, isotropic: bool = True,
@@ -122,30 +172,12 @@ def focused_linear_array_2d(nx: int = 4096, ny: int = 2048, nt: int = 8000, *, n
     rng = np.random.default_rng(seed)
     c, rho, a0, gm = layered_medium_2d(shape, rng, wavelength_cells=int(round(grid.wavelength / grid.dx)))
     medium = Medium(grid, c, rho, a0, gm, np.zeros(shape), use_isotropic_relaxation=isotropic)
-    width = pitch_cells - 1
-    aperture = n_elements * pitch_cells
-    y0 = (ny - aperture) // 2
-    elem_of_col = np.full(ny, -1)
-    for e in range(n_elements):
-        elem_of_col[y0 + e * pitch_cells : y0 + e * pitch_cells + width] = e
+    elem_of_col, centres = _array_geometry(ny, n_elements, pitch_cells)
     cols = np.nonzero(elem_of_col >= 0)[0]
-    centres = np.array([y0 + e * pitch_cells + (width - 1) / 2 for e in range(n_elements)])
-    yc = (ny - 1) / 2
-    dist = np.sqrt(focus_m**2 + ((centres - yc) * grid.dx) ** 2)
-    delay = (dist.max() - dist) / grid.c0
     mask = np.zeros(shape, dtype=bool)
     mask[:layers, cols] = True
     coords = np.argwhere(mask)
-    p0 = np.zeros((coords.shape[0], grid.nt))
-    cache = {}
-    for r, (ix, iy) in enumerate(coords):
-        e = elem_of_col[iy]
-        key = (e, ix)
-        if key not in cache:
-            cache[key] = gaussian_modulated_sinusoidal_signal(
-                grid.nt, grid.duration, ncycles, 2, f0, 1e5, delay_sec=delay[e],
-                i_layer=int(ix), dt_for_layer_delay=grid.dt, cfl_for_layer_delay=grid.cfl)
-        p0[r] = cache[key]
+    p0 = focused_signals(grid, coords, elem_of_col, centres, focus_m=focus_m, f0=f0, ncycles=ncycles)
     source = Source(p0, mask)
     smask = np.zeros(shape, dtype=bool)
     smask[layers, cols] = True
@@ -153,6 +185,35 @@ def focused_linear_array_2d(nx: int = 4096, ny: int = 2048, nt: int = 8000, *, n
     inputs = _inputs(grid, medium, source, sensor, n_pml=n_pml, isotropic=isotropic)
     elem = elem_of_col[np.argwhere(smask)[:, 1]]
     return Workload("focused_linear_array_2d", inputs, element_of_sensor=elem, n_elements=n_elements)
+
+
+def steered_transmits_2d(nx: int = 2048, ny: int = 1536, nt: int = 3000, *, n_transmits: int = 128,
+                         n_elements: int = 128, pitch_cells: int = 12, focus_m: float = 30e-3,
+                         seed: int = 2027, n_pml: int = 36, layers: int = 3, f0: float = 3e6):
+    """Many focused transmits on one seeded heterogeneous medium (the batched-transmit
+    configuration: 2048 x 1536, 128 transmits, each focused at ``focus_m`` and steered to its own
+    lateral position, spread evenly across the aperture). Returns ``(workload, shots)``:
+    ``workload.inputs`` carries the first transmit, ``shots`` all source matrices
+    ``[n_transmits, n_points, nt]`` (float32) in the order of the inputs' source points."""
+    grid = _grid(nx, ny, nt=nt, f0=f0)
+    shape = (grid.nx, grid.ny)
+    rng = np.random.default_rng(seed)
+    c, rho, a0, gm = layered_medium_2d(shape, rng, wavelength_cells=int(round(grid.wavelength / grid.dx)))
+    medium = Medium(grid, c, rho, a0, gm, np.zeros(shape))
+    elem_of_col, centres = _array_geometry(ny, n_elements, pitch_cells)
+    cols = np.nonzero(elem_of_col >= 0)[0]
+    mask = np.zeros(shape, dtype=bool)
+    mask[:layers, cols] = True
+    coords = np.argwhere(mask)
+    half = 0.5 * n_elements * pitch_cells * grid.dx
+    laterals = np.linspace(-0.8 * half, 0.8 * half, n_transmits)
+    shots = np.stack([focused_signals(grid, coords, elem_of_col, centres, focus_m=focus_m, lateral_m=x, f0=f0)
+                      for x in laterals]).astype(np.float32)
+    smask = np.zeros(shape, dtype=bool)
+    smask[layers, cols] = True
+    inputs = _inputs(grid, medium, Source(shots[0].astype(np.float64), mask), Sensor(smask), n_pml=n_pml)
+    elem = elem_of_col[np.argwhere(smask)[:, 1]]
+    return Workload("steered_transmits_2d", inputs, element_of_sensor=elem, n_elements=n_elements), shots
 
 
 def element_traces(traces: np.ndarray, workload: Workload) -> np.ndarray:

@@ -137,6 +137,12 @@ int fw_sim_kernel_config(fw_sim* sim, int32_t kind, int32_t* variant, int32_t* w
  * once for many handles of one grid shape (a server), and keeps results independent of
  * timing noise. */
 int fw_sim_set_kernel_config(fw_sim* sim, int32_t kind, int32_t variant, int32_t width, int32_t chunk);
+/* Run steady steps from a captured HIP graph of `steps` time steps (0: plain launches, the
+ * default). It has no counterpart in the reference. fw_sim_run then replays the graph for
+ * whole multiples of `steps` and launches the remainder one step at a time; results are
+ * identical. The step number the source/sensor pass uses is then kept in device memory. Cuts
+ * the per-launch host cost, which dominates for small grids. fw_sim_run_timed never uses it. */
+int fw_sim_set_graph_steps(fw_sim* sim, int64_t steps);
 /* Run n_steps with a HIP event pair around every launch on the handle's stream and add the
  * summed milliseconds per kernel kind to ms[0..n_kinds) (kinds: 0 velocity, 1 pressure,
  * 2 source/air/sensor); also returns the number of kinds in *n_kinds. */

@@ -188,7 +188,8 @@ def _three_speed_3d(shape, nt, seed):
                       outc=[[20, 20, 20], [15, 30, 40], [30, 22, 11]])
 
 
-@pytest.mark.parametrize("shape", ["stream:512:2", "stream:1024:16", "stream:512:9", "stream:1024:5"])
+@pytest.mark.parametrize("shape", ["stream:512:2", "stream:1024:16", "stream:512:9", "stream:1024:5", "pipe:512:7",
+                                   "pipe:1024:16", "pipelite:512:2", "pipelite:1024:9"])
 def test_streaming_3d_matches_per_cell(monkeypatch, shape):
     for inp in (workloads.random_medium_3d(40, 30, 150, 60, seed=33, air=4).inputs,
                 workloads.random_medium_3d(26, 70, 36, 50, seed=34, isotropic=False).inputs,
@@ -308,3 +309,31 @@ def test_batched_shots_3d_bitwise_and_reuse(monkeypatch):
     with engine.HipSimulation(inp, shots=b) as fresh:
         fresh.run(inp.nt)
         np.testing.assert_array_equal(got_b, fresh.sensors())
+
+
+@pytest.mark.parametrize("pinned", ["percell", "tile:8/stream:256:8"])
+def test_graph_replay_matches_plain_launches(monkeypatch, pinned):
+    """Steady steps replayed from a captured graph (device step counter) give the same bits as
+    plain launches, across several replays with plain steps between them, after a reset, and
+    for a batch of shots."""
+    inp = workloads.random_medium_2d(60, 64, 300, seed=70, air=4).inputs
+    monkeypatch.setenv("FULLWAVE_HIP_KERNELS", pinned)
+    with engine.HipSimulation(inp) as sim:
+        sim.run(inp.nt)
+        want = sim.sensors()
+        p_want = sim.field("p")
+    with engine.HipSimulation(inp) as sim:
+        sim.set_graph_steps(16)
+        for n in (50, 37, 100, 113):
+            sim.run(n)
+        np.testing.assert_array_equal(sim.sensors(), want)
+        np.testing.assert_array_equal(sim.field("p"), p_want)
+        sim.reset()
+        sim.run(inp.nt)
+        np.testing.assert_array_equal(sim.sensors(), want)
+    shots = _shot_stack(inp, 3, seed=71)
+    with engine.HipSimulation(inp, shots=shots) as plain, engine.HipSimulation(inp, shots=shots) as graph:
+        plain.run(inp.nt)
+        graph.set_graph_steps(32)
+        graph.run(inp.nt)
+        np.testing.assert_array_equal(graph.sensors(), plain.sensors())
