@@ -160,9 +160,11 @@ int fw_adj_backward(fw_adj* a, const float* grad_rec, size_t grad_bytes, float* 
     fw_sim* f = a->f;
     if (!a->forward_done) return fw::set_error(FW_ERR_STATE, "run fw_adj_forward first");
     const int64_t nrec = f->rec_cap, nout = f->d.ncoordsout, n = f->ncell, nt = f->d.nt;
+    const int64_t n_c = f->ncell_c;
     if (grad_bytes != (size_t)(nrec * nout) * 4)
         return fw::set_error(FW_ERR_INVALID, "grad_rec must be [%lld, %lld] float32", (long long)nrec, (long long)nout);
-    if (grid_bytes != (size_t)n * 4) return fw::set_error(FW_ERR_INVALID, "gradient buffers must be %lld bytes", (long long)n * 4);
+    if (grid_bytes != (size_t)n_c * 4)
+        return fw::set_error(FW_ERR_INVALID, "gradient buffers must be %lld bytes", (long long)n_c * 4);
     Guard g(f->device);
     hipStream_t st = f->stream;
     const int nd = f->ndim;
@@ -216,9 +218,9 @@ int fw_adj_backward(fw_adj* a, const float* grad_rec, size_t grad_bytes, float* 
         }
     }
     ADJ_HIP(fw::launch_adj_rho(f->ds, w, grho, st));
-    ADJ_HIP(hipMemcpyAsync(grad_k, w.gK, grid_bytes, hipMemcpyDeviceToHost, st));
-    ADJ_HIP(hipMemcpyAsync(grad_rho, grho, grid_bytes, hipMemcpyDeviceToHost, st));
     ADJ_HIP(hipStreamSynchronize(st));
+    ADJ_HIP(fw::grid_to_host(f, grad_k, w.gK, 4));
+    ADJ_HIP(fw::grid_to_host(f, grad_rho, grho, 4));
     return FW_OK;
 }
 

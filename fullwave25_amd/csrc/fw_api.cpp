@@ -61,6 +61,22 @@ namespace {
 
 int64_t cells(const fw_desc& d) { return d.nx * d.ny * (d.ndim == 3 ? d.nz : 1); }
 
+// Device layout: 3D grids pad each z row to a multiple of 32 floats (128 bytes), so every row
+// starts on a cache-line boundary; 2D grids keep rows of ny floats.
+int64_t row_pitch(const fw_desc& d) { return d.ndim == 3 ? (d.nz + 31) / 32 * 32 : d.ny; }
+
+// compact host grid -> padded device grid (elements of `esize` bytes)
+hipError_t copy_grid_in(const fw_sim* s, void* dst, const void* src, size_t esize) {
+    return hipMemcpy2D(dst, (size_t)s->pitch * esize, src, (size_t)s->row_len * esize, (size_t)s->row_len * esize,
+                       (size_t)s->rows, hipMemcpyHostToDevice);
+}
+
+// padded device grid -> compact host grid
+hipError_t copy_grid_out(const fw_sim* s, void* dst, const void* src, size_t esize) {
+    return hipMemcpy2D(dst, (size_t)s->row_len * esize, src, (size_t)s->pitch * esize, (size_t)s->row_len * esize,
+                       (size_t)s->rows, hipMemcpyDeviceToHost);
+}
+
 std::vector<std::string> required_maps(const fw_desc& d) {
     std::vector<std::string> names = {"K", "rho", "dcmap", "dmap", "icmat", "icc", "outc"};
     if (d.ncoordszero > 0) names.push_back("icczero");
@@ -117,6 +133,7 @@ int check_coords(const fw_sim* s, const std::vector<int32_t>& c, int64_t n, cons
     const int nd = s->ndim;
     lin.resize((size_t)n);
     const int64_t dims[3] = {d.nx, d.ny, nd == 3 ? d.nz : 1};
+    const int64_t stride[3] = {s->ds.sx, s->ds.sy, 1};
     for (int64_t q = 0; q < n; ++q) {
         int64_t idx = 0;
         for (int a = 0; a < nd; ++a) {
@@ -124,7 +141,7 @@ int check_coords(const fw_sim* s, const std::vector<int32_t>& c, int64_t n, cons
             if (x < 0 || x >= dims[a])
                 return fail(FW_ERR_INVALID, "%s coordinate %lld (axis %d) = %lld outside [0, %lld)", what,
                             (long long)q, a, (long long)x, (long long)dims[a]);
-            idx = idx * dims[a] + x;
+            idx += x * (nd == 2 && a == 1 ? 1 : stride[a]);
         }
         lin[(size_t)q] = idx;
     }
@@ -387,8 +404,8 @@ int finalize(fw_sim* s) {
     // dcmap range check on the host would need a download; the kernels trust it, so check it
     // here once against ndmap
     {
-        std::vector<int32_t> dc((size_t)s->ncell);
-        FW_HIP(hipMemcpy(dc.data(), s->dev["dcmap"], dc.size() * 4, hipMemcpyDeviceToHost));
+        std::vector<int32_t> dc((size_t)s->ncell_c);
+        FW_HIP(copy_grid_out(s, dc.data(), s->dev["dcmap"], 4));
         for (int32_t v : dc)
             if (v < 0 || v >= d.ndmap) return fail(FW_ERR_INVALID, "dcmap value %d outside [0, %lld)", v, (long long)d.ndmap);
     }
@@ -521,6 +538,10 @@ int run_impl(fw_sim* s, int64_t n_steps, bool timed, double* ms) {
 
 namespace fw {
 
+hipError_t grid_to_host(const fw_sim* s, void* dst, const void* src, size_t esize) {
+    return copy_grid_out(s, dst, src, esize);
+}
+
 int set_error(int code, const char* fmt, ...) {
     char buf[1024];
     va_list ap;
@@ -632,7 +653,11 @@ int fw_sim_create(const fw_desc* desc, int device, fw_sim** out) {
     s->d.nz = nz;
     s->device = device;
     s->ndim = d.ndim;
-    s->ncell = cells(s->d);
+    s->ncell_c = cells(s->d);
+    s->pitch = row_pitch(s->d);
+    s->rows = d.ndim == 3 ? d.nx * d.ny : d.nx;
+    s->row_len = d.ndim == 3 ? nz : d.ny;
+    s->ncell = s->rows * s->pitch;
     s->nshots = (int)nshots;
     s->d.n_shots = (int32_t)nshots;
     DeviceGuard g(device);
@@ -666,8 +691,8 @@ int fw_sim_create(const fw_desc* desc, int device, fw_sim** out) {
     s->ds.nx = d.nx;
     s->ds.ny = d.ny;
     s->ds.nz = nz;
-    s->ds.sy = nz;
-    s->ds.sx = d.ny * nz;
+    s->ds.sy = d.ndim == 3 ? s->pitch : 1;
+    s->ds.sx = d.ndim == 3 ? d.ny * s->pitch : d.ny;
     s->ds.dt = (float)d.dt;
     *out = s;
     return FW_OK;
@@ -684,15 +709,16 @@ int fw_sim_upload(fw_sim* s, const char* name_c, const void* host, size_t bytes)
     int rc;
     if (name == "c" || name == "beta") {
         // c enters only through dcmap; beta (nonlinearity) is not modelled by this engine
-        if (bytes != (size_t)s->ncell * 4) return fail(FW_ERR_INVALID, "'%s' must be %lld bytes", name_c, (long long)s->ncell * 4);
+        if (bytes != (size_t)s->ncell_c * 4) return fail(FW_ERR_INVALID, "'%s' must be %lld bytes", name_c, (long long)s->ncell_c * 4);
         s->have[name] = true;
         return FW_OK;
     }
     if (is_voxel_map(d, name)) {
-        if (bytes != (size_t)s->ncell * 4)
-            return fail(FW_ERR_INVALID, "'%s' must be %lld bytes, got %zu", name_c, (long long)s->ncell * 4, bytes);
-        if ((rc = alloc(s, name, bytes, &p)) != FW_OK) return rc;
-        FW_HIP(hipMemcpy(p, host, bytes, hipMemcpyHostToDevice));
+        if (bytes != (size_t)s->ncell_c * 4)
+            return fail(FW_ERR_INVALID, "'%s' must be %lld bytes, got %zu", name_c, (long long)s->ncell_c * 4, bytes);
+        if ((rc = alloc(s, name, (size_t)s->ncell * 4, &p)) != FW_OK) return rc;
+        if (s->pitch != s->row_len) FW_HIP(hipMemset(p, 0, (size_t)s->ncell * 4));  // row padding
+        FW_HIP(copy_grid_in(s, p, host, 4));
         if (name.rfind("kappa", 0) == 0) {
             FW_HIP(fw::launch_reciprocal((float*)p, s->ncell, s->stream));
             FW_HIP(hipStreamSynchronize(s->stream));
@@ -793,11 +819,13 @@ int fw_sim_download_field(fw_sim* s, const char* name, float* out, size_t bytes)
     else if (!strcmp(name, "vy")) src = s->ds.v[1];
     else if (!strcmp(name, "vz") && s->ndim == 3) src = s->ds.v[2];
     if (!src) return fail(FW_ERR_INVALID, "unknown field '%s'", name);
-    if (bytes != (size_t)s->ncell * 4 && bytes != (size_t)(s->ncell * s->nshots) * 4)
-        return fail(FW_ERR_INVALID, "field buffer must be %lld bytes (first shot) or %lld (every shot)",
-                    (long long)s->ncell * 4, (long long)(s->ncell * s->nshots) * 4);
+    const size_t one = (size_t)s->ncell_c * 4;
+    if (bytes != one && bytes != one * (size_t)s->nshots)
+        return fail(FW_ERR_INVALID, "field buffer must be %zu bytes (first shot) or %zu (every shot)", one,
+                    one * (size_t)s->nshots);
     DeviceGuard g(s->device);
-    FW_HIP(hipMemcpy(out, src, bytes, hipMemcpyDeviceToHost));
+    for (size_t b = 0; b * one < bytes; ++b)
+        FW_HIP(copy_grid_out(s, (char*)out + b * one, src + b * s->ncell, 4));
     return FW_OK;
 }
 
@@ -809,11 +837,13 @@ int fw_sim_set_field(fw_sim* s, const char* name, const float* values, size_t by
     else if (!strcmp(name, "vy")) dst = s->ds.v[1];
     else if (!strcmp(name, "vz") && s->ndim == 3) dst = s->ds.v[2];
     if (!dst) return fail(FW_ERR_INVALID, "unknown field '%s'", name);
-    if (bytes != (size_t)s->ncell * 4 && bytes != (size_t)(s->ncell * s->nshots) * 4)
-        return fail(FW_ERR_INVALID, "field buffer must be %lld bytes (first shot) or %lld (every shot)",
-                    (long long)s->ncell * 4, (long long)(s->ncell * s->nshots) * 4);
+    const size_t one = (size_t)s->ncell_c * 4;
+    if (bytes != one && bytes != one * (size_t)s->nshots)
+        return fail(FW_ERR_INVALID, "field buffer must be %zu bytes (first shot) or %zu (every shot)", one,
+                    one * (size_t)s->nshots);
     DeviceGuard g(s->device);
-    FW_HIP(hipMemcpy(dst, values, bytes, hipMemcpyHostToDevice));
+    for (size_t b = 0; b * one < bytes; ++b)
+        FW_HIP(copy_grid_in(s, dst + b * s->ncell, (const char*)values + b * one, 4));
     if (!s->ready) s->fields_set = true;
     return FW_OK;
 }

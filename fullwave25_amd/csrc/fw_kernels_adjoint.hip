@@ -231,12 +231,17 @@ __global__ __launch_bounds__(256) void k_adj_dplus_t(DevSim s, AdjView w) {
     w.lp[x] += acc;
 }
 
-// d/d rho[x] = sum_a (g_rb_a[x] + g_rb_a[x - e_a]) / 2, over the whole grid
+// d/d rho[x] = sum_a (g_rb_a[x] + g_rb_a[x - e_a]) / 2, over the whole (padded) grid
 __global__ __launch_bounds__(256) void k_adj_rho(DevSim s, AdjView w, float* grho) {
     const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t n = s.nx * s.ny * s.nz;
+    const int64_t n = s.nx * s.sx;
     if (x >= n) return;
-    const int64_t i = x / s.sx, j = (x / s.sy) % s.ny, k = x % s.nz;
+    const int64_t i = x / s.sx, rem = x - (x / s.sx) * s.sx;
+    const int64_t j = s.ndim == 3 ? rem / s.sy : rem, k = s.ndim == 3 ? rem - (rem / s.sy) * s.sy : 0;
+    if (j >= s.ny || k >= s.nz) {  // row padding
+        grho[x] = 0.f;
+        return;
+    }
     const int64_t coord[3] = {i, j, k};
     const int64_t st[3] = {s.sx, s.sy, 1};
     float acc = 0.f;
@@ -293,7 +298,7 @@ hipError_t launch_adj_step(const DevSim& s, const AdjView& w, hipStream_t st) {
 }
 
 hipError_t launch_adj_rho(const DevSim& s, const AdjView& w, float* grho, hipStream_t st) {
-    const int64_t n = s.nx * s.ny * s.nz;
+    const int64_t n = s.nx * s.sx;
     hipLaunchKernelGGL(k_adj_rho, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, s, w, grho);
     return hipGetLastError();
 }

@@ -170,7 +170,7 @@ __global__ __launch_bounds__(256) void k_copy(const float4* __restrict__ src, fl
 
 __global__ __launch_bounds__(256) void k_reciprocal(float* data, int64_t n) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < n) data[t] = 1.0f / data[t];
+    if (t < n) data[t] = data[t] != 0.f ? 1.0f / data[t] : 0.f;  // zero: row padding
 }
 
 }  // namespace

@@ -15,7 +15,11 @@ struct fw_sim {
 
     int device = 0;
     hipStream_t stream = nullptr;
-    int64_t ncell = 0;
+    int64_t ncell = 0;    // cells of one device grid (3D: z rows padded to `pitch`)
+    int64_t ncell_c = 0;  // cells of the grid as the caller sees it (nx * ny * nz)
+    int64_t pitch = 0;    // floats per contiguous row on the device (3D: nz rounded up to 32)
+    int64_t rows = 0;     // contiguous rows (2D: nx, 3D: nx * ny)
+    int64_t row_len = 0;  // cells per row as the caller sees it (2D: ny, 3D: nz)
     int ndim = 2;
     int nshots = 1;  // transmits sharing this medium, advanced together
     fw::DevSim ds{};
@@ -48,4 +52,5 @@ namespace fw {
 int set_error(int code, const char* fmt, ...);
 int prepare(fw_sim* s);                    // check inputs, resolve coordinates, pick kernels
 int forward_step(fw_sim* s);               // one time step on the handle's stream
+hipError_t grid_to_host(const fw_sim* s, void* dst, const void* src, size_t esize);  // padded -> compact
 }  // namespace fw

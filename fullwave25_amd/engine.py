@@ -139,7 +139,8 @@ def device_memory(device: int = 0) -> tuple[int, int]:
 def device_bytes(inp: SimulationInputs, n_shots: int = 1) -> int:
     """Device memory one handle for ``inp`` allocates (maps once, wave state per shot)."""
     nd = inp.ndim
-    cells = inp.ncells
+    # 3D rows are padded to a multiple of 32 floats on the device
+    cells = inp.nx * inp.ny * (-(-inp.nz // 32) * 32) if nd == 3 else inp.ncells
     n_maps = 3 + len(inp.attenuation_map_stems())  # K, rho, dcmap + attenuation maps
     state = (1 + nd + (0 if inp.exponential else 4 * nd)) * cells
     per_shot = state + inp.ncoords * inp.ntic + inp.n_records * inp.ncoordsout

@@ -68,6 +68,24 @@ def fw2_map_names(*, is_3d: bool, isotropic: bool, n_relax: int = 2) -> list[str
     return names
 
 
+def sort_mechanisms(params: dict, n: int) -> dict:
+    """Per voxel and side (x1 = velocity side, x2 = pressure side), the mechanisms ordered by
+    ``d / kappa + alpha``; ties keep their input order (fullwave/medium.py:121-222)."""
+    out = {"kappa_x1": np.asarray(params["kappa_x1"]), "kappa_x2": np.asarray(params["kappa_x2"])}
+    for side in ("x1", "x2"):
+        kappa = np.asarray(params[f"kappa_{side}"], dtype=np.float64)
+        d = np.stack([np.asarray(params[f"d_{side}_nu{nu}"], dtype=np.float64) for nu in range(1, n + 1)], -1)
+        al = np.stack([np.asarray(params[f"alpha_{side}_nu{nu}"], dtype=np.float64) for nu in range(1, n + 1)], -1)
+        tau = d / kappa[..., None] + al
+        order = np.argsort(tau, axis=-1, kind="stable")
+        d = np.take_along_axis(d, order, -1)
+        al = np.take_along_axis(al, order, -1)
+        for nu in range(1, n + 1):
+            out[f"d_{side}_nu{nu}"] = d[..., nu - 1]
+            out[f"alpha_{side}_nu{nu}"] = al[..., nu - 1]
+    return {k: out[k] for k in relaxation_param_keys(n)}
+
+
 class MediumRelaxationMaps:
     """Medium described by per-voxel relaxation parameters (the engine's native input)."""
 
@@ -124,20 +142,8 @@ class MediumRelaxationMaps:
         ``d / kappa + alpha``; ties keep their input order (fullwave/medium.py:121-222).
         """
         self.check_relaxation_param_dict(updates, self.sound_speed.shape, self.n_relaxation_mechanisms)
-        n = self.n_relaxation_mechanisms
-        out = {"kappa_x1": np.atleast_2d(updates["kappa_x1"]), "kappa_x2": np.atleast_2d(updates["kappa_x2"])}
-        for side in ("x1", "x2"):
-            kappa = np.asarray(updates[f"kappa_{side}"], dtype=np.float64)
-            d = np.stack([np.asarray(updates[f"d_{side}_nu{nu}"], dtype=np.float64) for nu in range(1, n + 1)], -1)
-            al = np.stack([np.asarray(updates[f"alpha_{side}_nu{nu}"], dtype=np.float64) for nu in range(1, n + 1)], -1)
-            tau = d / kappa[..., None] + al
-            order = np.argsort(tau, axis=-1, kind="stable")
-            d = np.take_along_axis(d, order, -1)
-            al = np.take_along_axis(al, order, -1)
-            for nu in range(1, n + 1):
-                out[f"d_{side}_nu{nu}"] = np.atleast_2d(d[..., nu - 1])
-                out[f"alpha_{side}_nu{nu}"] = np.atleast_2d(al[..., nu - 1])
-        self.relaxation_param_dict = {k: out[k] for k in relaxation_param_keys(n)}
+        out = sort_mechanisms(updates, self.n_relaxation_mechanisms)
+        self.relaxation_param_dict = {k: np.atleast_2d(v) for k, v in out.items()}
 
     def _calculate_relaxation_coefficients(self) -> dict:
         r = self.relaxation_param_dict

@@ -69,10 +69,11 @@ def adjoint_device_bytes(inp: SimulationInputs, checkpoint_every: int) -> int:
     """Device memory of an adjoint job: the forward handle, its checkpoints, one segment of
     states, the adjoint state and scratch."""
     nd = inp.ndim
-    state = (1 + nd + (0 if inp.exponential else 4 * nd)) * inp.ncells
+    cells = inp.nx * inp.ny * (-(-inp.nz // 32) * 32) if nd == 3 else inp.ncells
+    state = (1 + nd + (0 if inp.exponential else 4 * nd)) * cells
     every = max(1, min(checkpoint_every, inp.nt))
     n_ckpt = -(-inp.nt // every)
-    extra = (n_ckpt + every + 1) * state + (3 * nd + 2) * inp.ncells + inp.n_records * inp.ncoordsout
+    extra = (n_ckpt + every + 1) * state + (3 * nd + 2) * cells + inp.n_records * inp.ncoordsout
     return engine.device_bytes(inp) + 4 * extra
 
 

@@ -91,6 +91,68 @@ def plane_wave_3d(nx: int = 256, ny: int = 256, nz: int = 256, nt: int = 1000, *
                                              n_transition=n_transition))
 
 
+def inclusions_3d(nx: int = 768, ny: int = 768, nz: int = 640, nt: int = 6000, *, n_inclusions: int = 20,
+                  seed: int = 2028, n_pml: int = 24, n_transition: int = 36, f0: float = 2.5e6,
+                  n_el: int = 32, ntic: int = 400, record_every: int = 4, radii=(20, 120)) -> Workload:
+    """3D heterogeneous medium with ellipsoidal inclusions and a matrix-array plane-wave transmit
+    (the 768 x 768 x 640 / 6000-step configuration at default sizes; 904 x 904 x 776 cells with
+    the 24-cell absorbing layer, 36-cell transition and ghost border).
+
+    Background c = 1540 m/s, rho = 1000 kg/m^3, alpha0 = 0.5 dB/(MHz cm); gamma varies with depth
+    from 0.7 to 1.5 (steps of 0.1); ``n_inclusions`` seeded ellipsoids (radii ``radii`` cells)
+    with c ~ U[1450, 1620], alpha0 ~ U[0.3, 1.5] (steps of 0.1) and rho within +-3 %. The
+    transmit: a ``n_el`` x ``n_el`` array (pitch one wavelength, one-cell kerf) on the first
+    interior plane, every element firing the same 3-cycle pulse (``ntic`` samples). Receivers:
+    the array face one plane deeper (every point) and three planes at 1/4, 1/2 and 3/4 depth
+    (every 8th point in y and z), recorded every ``record_every`` steps. Built plane by plane
+    (:func:`~fullwave25_amd.solver.fast_inputs.build_inputs_planewise`).
+    """
+    from .solver.fast_inputs import build_inputs_planewise
+
+    grid = _grid(nx, ny, nt=nt, f0=f0, nz=nz)
+    shape = (grid.nx, grid.ny, grid.nz)
+    rng = np.random.default_rng(seed)
+    c = np.full(shape, 1540.0)
+    rho = np.full(shape, 1000.0)
+    a0 = np.full(shape, 0.5)
+    gm = np.empty(shape)
+    gm[:] = np.round(0.7 + 0.8 * np.arange(nx) / max(nx - 1, 1), 1)[:, None, None]
+    for _ in range(n_inclusions):
+        ctr = [rng.uniform(0.1, 0.9) * n for n in shape]
+        rad = [rng.uniform(*radii) for _ in range(3)]
+        lo = [max(0, int(ctr[a] - rad[a])) for a in range(3)]
+        hi = [min(shape[a], int(ctr[a] + rad[a]) + 1) for a in range(3)]
+        if any(h <= l for l, h in zip(lo, hi)):
+            continue
+        ax = [((np.arange(lo[a], hi[a]) - ctr[a]) / rad[a]) ** 2 for a in range(3)]
+        inside = ax[0][:, None, None] + ax[1][None, :, None] + ax[2][None, None, :] <= 1.0
+        box = tuple(slice(lo[a], hi[a]) for a in range(3))
+        c[box] = np.where(inside, rng.uniform(1450, 1620), c[box])
+        rho[box] = np.where(inside, 1000.0 * (1 + rng.uniform(-0.03, 0.03)), rho[box])
+        a0[box] = np.where(inside, np.round(rng.uniform(0.3, 1.5), 1), a0[box])
+    medium = Medium(grid, c, rho, a0, gm, np.zeros(shape))
+    pitch = int(round(grid.wavelength / grid.dx))
+    width = pitch - 1
+    el = np.zeros(ny, bool)
+    el_z = np.zeros(nz, bool)
+    y0, z0 = (ny - n_el * pitch) // 2, (nz - n_el * pitch) // 2
+    for e in range(n_el):
+        el[max(0, y0 + e * pitch): max(0, y0 + e * pitch + width)] = True
+        el_z[max(0, z0 + e * pitch): max(0, z0 + e * pitch + width)] = True
+    face = el[:, None] & el_z[None, :]
+    mask = np.zeros(shape, bool)
+    mask[0] = face
+    sig = gaussian_modulated_sinusoidal_signal(ntic, ntic * grid.dt, 3, 2, f0, 1e5)
+    source = Source(np.broadcast_to(sig, (int(face.sum()), ntic)), mask)
+    smask = np.zeros(shape, bool)
+    smask[1] = face
+    for frac in (0.25, 0.5, 0.75):
+        smask[int(frac * nx), ::8, ::8] = True
+    sensor = Sensor(smask, sampling_modulus_time=record_every)
+    inputs = build_inputs_planewise(grid, medium, source, sensor, n_pml=n_pml, n_transition=n_transition)
+    return Workload("inclusions_3d", inputs)
+
+
 def layered_medium_2d(shape, rng, *, n_layers=6, amp_cells=10.0, dx=None, scatter_per_cell=15,
                       wavelength_cells=12):
     """Seeded layered medium: ``n_layers`` horizontal layers whose interfaces carry a random
