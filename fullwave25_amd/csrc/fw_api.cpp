@@ -175,6 +175,8 @@ int check_cfg(fw_sim* s, const fw::LaunchCfg& c) {
     if (c.variant == 3 || c.variant == 4) {
         if (s->ndim != 3 || !fw::streaming_supported(s->ds))
             return fail(FW_ERR_UNSUPPORTED, "pipelined kernels need a 3D relaxation grid");
+        if (s->ncell >= ((int64_t)1 << 30))  // 32-bit byte offsets per array
+            return fail(FW_ERR_UNSUPPORTED, "pipelined kernels take grids of fewer than 2^30 cells");
         if (c.width != 512 && c.width != 1024)
             return fail(FW_ERR_INVALID, "pipelined kernels: workgroup size (width) must be 512 or 1024");
         if (c.chunk < 1 || c.chunk > 4096) return fail(FW_ERR_INVALID, "pipelined kernels: chunk out of range");
@@ -299,7 +301,7 @@ int choose_kernels_unpinned(fw_sim* s) {
                     c.variant = v;
                     c.width = w;
                     c.chunk = ch;
-                    cands.push_back(c);
+                    if (v == 1 || s->ncell < ((int64_t)1 << 30)) cands.push_back(c);
                 }
         }
     }

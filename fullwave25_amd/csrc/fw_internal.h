@@ -47,6 +47,18 @@ struct DevSim {
     int64_t shot_stride;
 };
 
+// Global memory at a 32-bit element index: the array base stays in scalar registers and each
+// lane supplies a 32-bit byte offset, which the hardware adds (one VGPR per access instead of a
+// 64-bit address). Only for arrays of fewer than 2^30 elements; the launchers check the grid.
+template <class T>
+__device__ __forceinline__ T gld32(const T* __restrict__ base, uint32_t i) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + (i << 2));
+}
+template <class T>
+__device__ __forceinline__ void gst32(T* base, uint32_t i, T v) {
+    *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + (i << 2)) = v;
+}
+
 // The fields of one shot of the batch.
 __host__ __device__ inline DevSim shot_view(const DevSim& s, int shot) {
     DevSim t = s;

@@ -22,6 +22,12 @@ constexpr int PZ3 = 64;  // lanes along z
 constexpr int kRingP = 4;
 constexpr int kLdsSetsP = 128;
 
+// global memory at 32-bit element indices (gld32/gst32 in fw_internal.h)
+template <class T>
+__device__ __forceinline__ T gld(const T* __restrict__ base, uint32_t i) { return gld32(base, i); }
+template <class T>
+__device__ __forceinline__ void gst(T* base, uint32_t i, T v) { gst32(base, i, v); }
+
 // The staged tile: rows y in [j0 - HY, j0 + TY + HY), columns z in [k0 - HZ, k0 + 64 + HZ).
 template <int TY, int HY, int HZ>
 struct PTile {
@@ -56,7 +62,7 @@ struct Stager {
     __device__ void load(const float* __restrict__ f, int64_t plane, const DevSim& s) {
         const float* base = f + plane * s.sx;
 #pragma unroll
-        for (int r = 0; r < T::R; ++r) val[r] = off[r] >= 0 ? base[off[r]] : 0.f;
+        for (int r = 0; r < T::R; ++r) val[r] = off[r] >= 0 ? gld(base, (uint32_t)off[r]) : 0.f;
     }
     __device__ void store(T& tile, int64_t plane) const {
         float* dst = tile.slot(plane);
@@ -95,28 +101,28 @@ struct VelOps {
 };
 
 template <bool UNIFORM, bool ISO>
-__device__ inline void load_vel(VelOps<ISO>& o, const DevSim& s, int64_t i, int64_t col, bool in_grid, bool active) {
-    const int64_t c = i * s.sx + col;
-    o.qnew = in_grid ? s.p[c + 8 * s.sx] : 0.f;
-    o.rho_x = in_grid ? s.rho[c + s.sx] : 0.f;
+__device__ inline void load_vel(VelOps<ISO>& o, const DevSim& s, uint32_t c, bool in_grid, bool active) {
+    const uint32_t sx = (uint32_t)s.sx;
+    o.qnew = in_grid ? gld(s.p, c + 8 * sx) : 0.f;
+    o.rho_x = in_grid ? gld(s.rho, c + sx) : 0.f;
     if (!active) return;
-    o.rho_y = s.rho[c + s.sy];
-    o.rho_z = s.rho[c + 1];
+    o.rho_y = gld(s.rho, c + (uint32_t)s.sy);
+    o.rho_z = gld(s.rho, c + 1);
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        o.v[a] = s.v[a][c];
-        o.psi[a][0] = s.psi[a][0][c];
-        o.psi[a][1] = s.psi[a][1][c];
+        o.v[a] = gld((const float*)s.v[a], c);
+        o.psi[a][0] = gld((const float*)s.psi[a][0], c);
+        o.psi[a][1] = gld((const float*)s.psi[a][1], c);
     }
 #pragma unroll
     for (int a = 0; a < VelOps<ISO>::NA; ++a) {
-        o.a[a][0] = s.av[a][0][c];
-        o.a[a][1] = s.av[a][1][c];
-        o.b[a][0] = s.bv[a][0][c];
-        o.b[a][1] = s.bv[a][1][c];
-        o.ik[a] = s.ikv[a][c];
+        o.a[a][0] = gld(s.av[a][0], c);
+        o.a[a][1] = gld(s.av[a][1], c);
+        o.b[a][0] = gld(s.bv[a][0], c);
+        o.b[a][1] = gld(s.bv[a][1], c);
+        o.ik[a] = gld(s.ikv[a], c);
     }
-    o.dc = UNIFORM ? 0 : s.dcmap[c];
+    o.dc = UNIFORM ? 0 : gld(s.dcmap, c);
 }
 
 // AHEAD: per-cell operands loaded one iteration ahead (more registers); otherwise at the top
@@ -135,37 +141,37 @@ __global__ __launch_bounds__(PZ3 * TY) void k_velocity_pipe3d(DevSim batch, int 
     const int64_t i1 = min(i0 + chunk, s.nx - kHalo);
     const bool in_grid = j < s.ny && k < s.nz;
     const bool active = j >= kHalo && j < s.ny - kHalo && k >= kHalo && k < s.nz - kHalo;
-    const int64_t sx = s.sx, col = j * s.sy + k;
+    const uint32_t sx = (uint32_t)s.sx, col = (uint32_t)(j * s.sy + k);
     const float* coefs = coef_table(s, lds_coef, UNIFORM);
 
     Stager<TY, kHalo, kHalo> st;
     st.init(s, j0, k0);
     float q[16];  // q[m] = p(i - 7 + m)
 #pragma unroll
-    for (int m = 0; m < 15; ++m) q[m + 1] = in_grid ? s.p[(i0 - 7 + m) * sx + col] : 0.f;
+    for (int m = 0; m < 15; ++m) q[m + 1] = in_grid ? gld((const float*)s.p, (uint32_t)(i0 - 7 + m) * sx + col) : 0.f;
     st.load(s.p, i0 - 1, s);
     st.store(P, i0 - 1);
     st.load(s.p, i0, s);
     st.store(P, i0);
     st.load(s.p, i0 + 1, s);
-    float rho_i = in_grid ? s.rho[i0 * sx + col] : 0.f;
+    float rho_i = in_grid ? gld(s.rho, (uint32_t)i0 * sx + col) : 0.f;
     VelOps<ISO> cur, nxt;
-    if (AHEAD) load_vel<UNIFORM, ISO>(cur, s, i0, col, in_grid, active);
+    if (AHEAD) load_vel<UNIFORM, ISO>(cur, s, (uint32_t)i0 * sx + col, in_grid, active);
 
     for (int64_t i = i0; i < i1; ++i) {
         st.store(P, i + 1);
         const bool more = i + 1 < i1;
-        if (!AHEAD) load_vel<UNIFORM, ISO>(cur, s, i, col, in_grid, active);
+        if (!AHEAD) load_vel<UNIFORM, ISO>(cur, s, (uint32_t)i * sx + col, in_grid, active);
         if (more) {
             st.load(s.p, i + 2, s);
-            if (AHEAD) load_vel<UNIFORM, ISO>(nxt, s, i + 1, col, in_grid, active);
+            if (AHEAD) load_vel<UNIFORM, ISO>(nxt, s, (uint32_t)(i + 1) * sx + col, in_grid, active);
         }
 #pragma unroll
         for (int m = 0; m < 15; ++m) q[m] = q[m + 1];
         q[15] = cur.qnew;
         __syncthreads();
         if (active) {
-            const int64_t c = i * sx + col;
+            const uint32_t c = (uint32_t)i * sx + col;
             const float* pm = P.slot(i - 1);
             const float* p0 = P.slot(i);
             const float* pp = P.slot(i + 1);
@@ -207,10 +213,10 @@ __global__ __launch_bounds__(PZ3 * TY) void k_velocity_pipe3d(DevSim batch, int 
                 const int ax = ISO ? 0 : a;
                 const float qq1 = cur.b[ax][0] * cur.psi[a][0] + cur.a[ax][0] * D[a];
                 const float qq2 = cur.b[ax][1] * cur.psi[a][1] + cur.a[ax][1] * D[a];
-                s.psi[a][0][c] = qq1;
-                s.psi[a][1][c] = qq2;
+                gst(s.psi[a][0], c, qq1);
+                gst(s.psi[a][1], c, qq2);
                 const float rho_bar = 0.5f * (rho_i + rnb[a]);
-                s.v[a][c] = cur.v[a] - s.dt / rho_bar * (D[a] * cur.ik[ax] + (qq1 + qq2));
+                gst(s.v[a], c, cur.v[a] - s.dt / rho_bar * (D[a] * cur.ik[ax] + (qq1 + qq2)));
             }
         }
         rho_i = cur.rho_x;
@@ -231,26 +237,25 @@ struct PreOps {
 };
 
 template <bool UNIFORM, bool ISO>
-__device__ inline void load_pre(PreOps<ISO>& o, const DevSim& s, int64_t i, int64_t col, bool in_grid, bool active) {
-    const int64_t c = i * s.sx + col;
-    o.qnew = in_grid ? s.v[0][c + 7 * s.sx] : 0.f;
+__device__ inline void load_pre(PreOps<ISO>& o, const DevSim& s, uint32_t c, bool in_grid, bool active) {
+    o.qnew = in_grid ? gld((const float*)s.v[0], c + 7 * (uint32_t)s.sx) : 0.f;
     if (!active) return;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        o.phi[a][0] = s.phi[a][0][c];
-        o.phi[a][1] = s.phi[a][1][c];
+        o.phi[a][0] = gld((const float*)s.phi[a][0], c);
+        o.phi[a][1] = gld((const float*)s.phi[a][1], c);
     }
 #pragma unroll
     for (int a = 0; a < PreOps<ISO>::NA; ++a) {
-        o.a[a][0] = s.ap[a][0][c];
-        o.a[a][1] = s.ap[a][1][c];
-        o.b[a][0] = s.bp[a][0][c];
-        o.b[a][1] = s.bp[a][1][c];
-        o.ik[a] = s.ikp[a][c];
+        o.a[a][0] = gld(s.ap[a][0], c);
+        o.a[a][1] = gld(s.ap[a][1], c);
+        o.b[a][0] = gld(s.bp[a][0], c);
+        o.b[a][1] = gld(s.bp[a][1], c);
+        o.ik[a] = gld(s.ikp[a], c);
     }
-    o.K = s.K[c];
-    o.p = s.p[c];
-    o.dc = UNIFORM ? 0 : s.dcmap[c];
+    o.K = gld(s.K, c);
+    o.p = gld((const float*)s.p, c);
+    o.dc = UNIFORM ? 0 : gld(s.dcmap, c);
 }
 
 template <int TY, bool UNIFORM, bool ISO, bool AHEAD>
@@ -271,7 +276,7 @@ __global__ __launch_bounds__(PZ3 * TY) void k_pressure_pipe3d(DevSim batch, int 
     const int64_t i1 = min(i0 + chunk, s.nx - kHalo);
     const bool in_grid = j < s.ny && k < s.nz;
     const bool active = j >= kHalo && j < s.ny - kHalo && k >= kHalo && k < s.nz - kHalo;
-    const int64_t sx = s.sx, col = j * s.sy + k;
+    const uint32_t sx = (uint32_t)s.sx, col = (uint32_t)(j * s.sy + k);
     const float* coefs = coef_table(s, lds_coef, UNIFORM);
 
     Stager<TY, 1, 1> sx_;
@@ -282,7 +287,7 @@ __global__ __launch_bounds__(PZ3 * TY) void k_pressure_pipe3d(DevSim batch, int 
     sz_.init(s, j0, k0);
     float q[16];  // q[m] = vx(i - 8 + m)
 #pragma unroll
-    for (int m = 0; m < 15; ++m) q[m + 1] = in_grid ? s.v[0][(i0 - 8 + m) * sx + col] : 0.f;
+    for (int m = 0; m < 15; ++m) q[m + 1] = in_grid ? gld((const float*)s.v[0], (uint32_t)(i0 - 8 + m) * sx + col) : 0.f;
     sx_.load(s.v[0], i0 - 1, s);
     sx_.store(VX, i0 - 1);
     sy_.load(s.v[1], i0 - 1, s);
@@ -297,26 +302,26 @@ __global__ __launch_bounds__(PZ3 * TY) void k_pressure_pipe3d(DevSim batch, int 
     sy_.load(s.v[1], i0 + 1, s);
     sz_.load(s.v[2], i0 + 1, s);
     PreOps<ISO> cur, nxt;
-    if (AHEAD) load_pre<UNIFORM, ISO>(cur, s, i0, col, in_grid, active);
+    if (AHEAD) load_pre<UNIFORM, ISO>(cur, s, (uint32_t)i0 * sx + col, in_grid, active);
 
     for (int64_t i = i0; i < i1; ++i) {
         sx_.store(VX, i);
         sy_.store(VY, i + 1);
         sz_.store(VZ, i + 1);
         const bool more = i + 1 < i1;
-        if (!AHEAD) load_pre<UNIFORM, ISO>(cur, s, i, col, in_grid, active);
+        if (!AHEAD) load_pre<UNIFORM, ISO>(cur, s, (uint32_t)i * sx + col, in_grid, active);
         if (more) {
             sx_.load(s.v[0], i + 1, s);
             sy_.load(s.v[1], i + 2, s);
             sz_.load(s.v[2], i + 2, s);
-            if (AHEAD) load_pre<UNIFORM, ISO>(nxt, s, i + 1, col, in_grid, active);
+            if (AHEAD) load_pre<UNIFORM, ISO>(nxt, s, (uint32_t)(i + 1) * sx + col, in_grid, active);
         }
 #pragma unroll
         for (int m = 0; m < 15; ++m) q[m] = q[m + 1];
         q[15] = cur.qnew;
         __syncthreads();
         if (active) {
-            const int64_t c = i * sx + col;
+            const uint32_t c = (uint32_t)i * sx + col;
             const float* xm = VX.slot(i - 1);
             const float* x0 = VX.slot(i);
             const float* ym = VY.slot(i - 1);
@@ -365,11 +370,11 @@ __global__ __launch_bounds__(PZ3 * TY) void k_pressure_pipe3d(DevSim batch, int 
                 const int ax = ISO ? 0 : a;
                 const float qq1 = cur.b[ax][0] * cur.phi[a][0] + cur.a[ax][0] * D[a];
                 const float qq2 = cur.b[ax][1] * cur.phi[a][1] + cur.a[ax][1] * D[a];
-                s.phi[a][0][c] = qq1;
-                s.phi[a][1][c] = qq2;
+                gst(s.phi[a][0], c, qq1);
+                gst(s.phi[a][1], c, qq2);
                 div += D[a] * cur.ik[ax] + (qq1 + qq2);
             }
-            s.p[c] = cur.p - s.dt * cur.K * div;
+            gst(s.p, c, cur.p - s.dt * cur.K * div);
         }
         if (AHEAD) cur = nxt;
     }

@@ -70,39 +70,39 @@ __global__ __launch_bounds__(TY) void k_velocity_2d(DevSim batch, int chunk) {
     for (int t = tid; t < ncoef; t += TY) lds_coef[t] = s.coef[t];
 
     auto load = [&](VelRow<ISO>& R, int64_t r) {
-        const int64_t c = r * ny + j;
-        R.pq = col_ok ? p[c + 8 * ny] : 0.f;
-        R.rho_next = col_ok ? s.rho[c + ny] : 0.f;
-        R.halo = halo_ok ? p[(r + 1) * ny + hj] : 0.f;
+        const uint32_t c = (uint32_t)(r * ny + j);
+        R.pq = col_ok ? gld32((const float*)p, c + (uint32_t)(8 * ny)) : 0.f;
+        R.rho_next = col_ok ? gld32((const float*)s.rho, c + (uint32_t)(ny)) : 0.f;
+        R.halo = halo_ok ? gld32((const float*)p, (uint32_t)((r + 1) * ny + hj)) : 0.f;
         if (active) {
-            R.rho_right = s.rho[c + 1];
-            R.dc = UNIFORM ? 0 : s.dcmap[c];
+            R.rho_right = gld32((const float*)s.rho, c + (uint32_t)(1));
+            R.dc = UNIFORM ? 0 : gld32((const int32_t*)s.dcmap, c);
 #pragma unroll
             for (int a = 0; a < VelRow<ISO>::NA; ++a) {
-                R.ik[a] = s.ikv[a][c];
-                R.b1[a] = s.bv[a][0][c];
-                R.b2[a] = s.bv[a][1][c];
-                R.a1[a] = s.av[a][0][c];
-                R.a2[a] = s.av[a][1][c];
+                R.ik[a] = gld32((const float*)s.ikv[a], c);
+                R.b1[a] = gld32((const float*)s.bv[a][0], c);
+                R.b2[a] = gld32((const float*)s.bv[a][1], c);
+                R.a1[a] = gld32((const float*)s.av[a][0], c);
+                R.a2[a] = gld32((const float*)s.av[a][1], c);
             }
 #pragma unroll
             for (int a = 0; a < 2; ++a) {
-                R.ps1[a] = s.psi[a][0][c];
-                R.ps2[a] = s.psi[a][1][c];
-                R.v[a] = s.v[a][c];
+                R.ps1[a] = gld32((const float*)s.psi[a][0], c);
+                R.ps2[a] = gld32((const float*)s.psi[a][1], c);
+                R.v[a] = gld32((const float*)s.v[a], c);
             }
         }
     };
 
     float q[16];  // q[m] = p(r - 7 + m, j)
 #pragma unroll
-    for (int m = 0; m < 16; ++m) q[m] = col_ok ? p[(i0 - 7 + m) * ny + j] : 0.f;
+    for (int m = 0; m < 16; ++m) q[m] = col_ok ? gld32((const float*)p, (uint32_t)((i0 - 7 + m) * ny + j)) : 0.f;
     for (int64_t r = i0 - 1; r <= i0; ++r) {
         float* dst = ring[r & (kRing - 1)];
         dst[kHalo + tid] = r < i0 ? q[6] : q[7];
-        if (halo_lane) dst[halo_slot] = halo_ok ? p[r * ny + hj] : 0.f;
+        if (halo_lane) dst[halo_slot] = halo_ok ? gld32((const float*)p, (uint32_t)(r * ny + hj)) : 0.f;
     }
-    float rho_i = col_ok ? s.rho[i0 * ny + j] : 0.f;
+    float rho_i = col_ok ? gld32((const float*)s.rho, (uint32_t)(i0 * ny + j)) : 0.f;
 
     auto step = [&](int64_t r, const VelRow<ISO>& R) {
         if (r > i0) {
@@ -117,7 +117,7 @@ __global__ __launch_bounds__(TY) void k_velocity_2d(DevSim batch, int chunk) {
         }
         __syncthreads();
         if (active) {
-            const int64_t c = r * ny + j;
+            const uint32_t c = (uint32_t)(r * ny + j);
             const float* rm = ring[(r - 1) & (kRing - 1)] + kHalo + tid;
             const float* r0 = ring[r & (kRing - 1)] + kHalo + tid;
             const float* rp = ring[(r + 1) & (kRing - 1)] + kHalo + tid;
@@ -131,10 +131,10 @@ __global__ __launch_bounds__(TY) void k_velocity_2d(DevSim batch, int chunk) {
                 const float D = acc + w[8] * tr;
                 const float qq1 = R.b1[0] * R.ps1[0] + R.a1[0] * D;
                 const float qq2 = R.b2[0] * R.ps2[0] + R.a2[0] * D;
-                s.psi[0][0][c] = qq1;
-                s.psi[0][1][c] = qq2;
+                gst32(s.psi[0][0], c, qq1);
+                gst32(s.psi[0][1], c, qq2);
                 const float rho_bar = 0.5f * (rho_i + R.rho_next);
-                s.v[0][c] = R.v[0] - s.dt / rho_bar * (D * R.ik[0] + (qq1 + qq2));
+                gst32(s.v[0], c, R.v[0] - s.dt / rho_bar * (D * R.ik[0] + (qq1 + qq2)));
             }
             {  // y: axial from row r in LDS, transverse from rows r-1, r+1 at j, j+1
                 constexpr int ax = ISO ? 0 : 1;
@@ -147,10 +147,10 @@ __global__ __launch_bounds__(TY) void k_velocity_2d(DevSim batch, int chunk) {
                 const float D = acc + w[8] * tr;
                 const float qq1 = R.b1[ax] * R.ps1[1] + R.a1[ax] * D;
                 const float qq2 = R.b2[ax] * R.ps2[1] + R.a2[ax] * D;
-                s.psi[1][0][c] = qq1;
-                s.psi[1][1][c] = qq2;
+                gst32(s.psi[1][0], c, qq1);
+                gst32(s.psi[1][1], c, qq2);
                 const float rho_bar = 0.5f * (rho_i + R.rho_right);
-                s.v[1][c] = R.v[1] - s.dt / rho_bar * (D * R.ik[ax] + (qq1 + qq2));
+                gst32(s.v[1], c, R.v[1] - s.dt / rho_bar * (D * R.ik[ax] + (qq1 + qq2)));
             }
         }
         rho_i = R.rho_next;
@@ -212,43 +212,43 @@ __global__ __launch_bounds__(TY) void k_pressure_2d(DevSim batch, int chunk) {
     for (int t = tid; t < ncoef; t += TY) lds_coef[t] = s.coef[t];
 
     auto load = [&](PreRow<ISO>& R, int64_t r) {
-        const int64_t c = r * ny + j;
-        R.vxq = col_ok ? vx[c + 7 * ny] : 0.f;
-        R.vy_own = col_ok ? vy[c + ny] : 0.f;
-        R.vy_halo = halo_ok ? vy[(r + 1) * ny + hj] : 0.f;
-        R.vx_halo = xh_ok ? vx[r * ny + xhj] : 0.f;
+        const uint32_t c = (uint32_t)(r * ny + j);
+        R.vxq = col_ok ? gld32((const float*)vx, c + (uint32_t)(7 * ny)) : 0.f;
+        R.vy_own = col_ok ? gld32((const float*)vy, c + (uint32_t)(ny)) : 0.f;
+        R.vy_halo = halo_ok ? gld32((const float*)vy, (uint32_t)((r + 1) * ny + hj)) : 0.f;
+        R.vx_halo = xh_ok ? gld32((const float*)vx, (uint32_t)(r * ny + xhj)) : 0.f;
         if (active) {
-            R.K = s.K[c];
-            R.p = s.p[c];
-            R.dc = UNIFORM ? 0 : s.dcmap[c];
+            R.K = gld32((const float*)s.K, c);
+            R.p = gld32((const float*)s.p, c);
+            R.dc = UNIFORM ? 0 : gld32((const int32_t*)s.dcmap, c);
 #pragma unroll
             for (int a = 0; a < PreRow<ISO>::NA; ++a) {
-                R.ik[a] = s.ikp[a][c];
-                R.b1[a] = s.bp[a][0][c];
-                R.b2[a] = s.bp[a][1][c];
-                R.a1[a] = s.ap[a][0][c];
-                R.a2[a] = s.ap[a][1][c];
+                R.ik[a] = gld32((const float*)s.ikp[a], c);
+                R.b1[a] = gld32((const float*)s.bp[a][0], c);
+                R.b2[a] = gld32((const float*)s.bp[a][1], c);
+                R.a1[a] = gld32((const float*)s.ap[a][0], c);
+                R.a2[a] = gld32((const float*)s.ap[a][1], c);
             }
 #pragma unroll
             for (int a = 0; a < 2; ++a) {
-                R.ph1[a] = s.phi[a][0][c];
-                R.ph2[a] = s.phi[a][1][c];
+                R.ph1[a] = gld32((const float*)s.phi[a][0], c);
+                R.ph2[a] = gld32((const float*)s.phi[a][1], c);
             }
         }
     };
 
     float q[16];  // q[m] = vx(r - 8 + m, j)
 #pragma unroll
-    for (int m = 0; m < 16; ++m) q[m] = col_ok ? vx[(i0 - 8 + m) * ny + j] : 0.f;
+    for (int m = 0; m < 16; ++m) q[m] = col_ok ? gld32((const float*)vx, (uint32_t)((i0 - 8 + m) * ny + j)) : 0.f;
     for (int64_t r = i0 - 1; r <= i0; ++r) {
         float* dst = ring_y[r & (kRing - 1)];
-        dst[kHalo + tid] = col_ok ? vy[r * ny + j] : 0.f;
-        if (halo_lane) dst[halo_slot] = halo_ok ? vy[r * ny + hj] : 0.f;
+        dst[kHalo + tid] = col_ok ? gld32((const float*)vy, (uint32_t)(r * ny + j)) : 0.f;
+        if (halo_lane) dst[halo_slot] = halo_ok ? gld32((const float*)vy, (uint32_t)(r * ny + hj)) : 0.f;
     }
     {
         float* dst = ring_x[(i0 - 1) & (kRing - 1)];
         dst[1 + tid] = q[7];
-        if (xh_lane) dst[xh_slot] = xh_ok ? vx[(i0 - 1) * ny + xhj] : 0.f;
+        if (xh_lane) dst[xh_slot] = xh_ok ? gld32((const float*)vx, (uint32_t)((i0 - 1) * ny + xhj)) : 0.f;
     }
 
     auto step = [&](int64_t r, const PreRow<ISO>& R) {
@@ -267,7 +267,7 @@ __global__ __launch_bounds__(TY) void k_pressure_2d(DevSim batch, int chunk) {
         }
         __syncthreads();
         if (active) {
-            const int64_t c = r * ny + j;
+            const uint32_t c = (uint32_t)(r * ny + j);
             const float* xm = ring_x[(r - 1) & (kRing - 1)] + 1 + tid;
             const float* x0 = ring_x[r & (kRing - 1)] + 1 + tid;
             const float* ym = ring_y[(r - 1) & (kRing - 1)] + kHalo + tid;
@@ -284,8 +284,8 @@ __global__ __launch_bounds__(TY) void k_pressure_2d(DevSim batch, int chunk) {
                 const float D = acc + w[8] * tr;
                 const float qq1 = R.b1[0] * R.ph1[0] + R.a1[0] * D;
                 const float qq2 = R.b2[0] * R.ph2[0] + R.a2[0] * D;
-                s.phi[0][0][c] = qq1;
-                s.phi[0][1][c] = qq2;
+                gst32(s.phi[0][0], c, qq1);
+                gst32(s.phi[0][1], c, qq2);
                 div += D * R.ik[0] + (qq1 + qq2);
             }
             {
@@ -299,11 +299,11 @@ __global__ __launch_bounds__(TY) void k_pressure_2d(DevSim batch, int chunk) {
                 const float D = acc + w[8] * tr;
                 const float qq1 = R.b1[ax] * R.ph1[1] + R.a1[ax] * D;
                 const float qq2 = R.b2[ax] * R.ph2[1] + R.a2[ax] * D;
-                s.phi[1][0][c] = qq1;
-                s.phi[1][1][c] = qq2;
+                gst32(s.phi[1][0], c, qq1);
+                gst32(s.phi[1][1], c, qq2);
                 div += D * R.ik[ax] + (qq1 + qq2);
             }
-            s.p[c] = R.p - s.dt * R.K * div;
+            gst32(s.p, c, R.p - s.dt * R.K * div);
         }
     };
 
@@ -357,7 +357,8 @@ constexpr int kMaxLdsCoefSets2D = 1024;
 bool streaming_supported(const DevSim& s) {
     if (s.aexp) return false;  // the exponential model runs the per-cell kernels
     if (s.ndim == 3) return true;  // large 3D tables are read from global memory
-    return s.ndmap <= kMaxLdsCoefSets2D;
+    // 2D: the coefficient table fits LDS, and 32-bit byte offsets cover every array
+    return s.ndmap <= kMaxLdsCoefSets2D && s.nx * s.ny < ((int64_t)1 << 30);
 }
 
 bool tile2d_supported(const DevSim& s);

@@ -29,9 +29,16 @@ struct alignas(16) I4 {
     __device__ int operator[](int i) const { return (&x)[i]; }
 };
 
-__device__ inline F4 ld4(const float* p) { return *reinterpret_cast<const F4*>(p); }
-__device__ inline I4 ldi4(const int32_t* p) { return *reinterpret_cast<const I4*>(p); }
-__device__ inline void st4(float* p, const F4& v) { *reinterpret_cast<F4*>(p) = v; }
+// Vector access at a 32-bit element index: the array base stays in scalar registers and each
+// lane supplies a 32-bit byte offset (grids here have fewer than 2^30 cells per array, checked
+// on the host), which saves a 64-bit address per access.
+__device__ inline const char* at_(const void* base, uint32_t i) { return reinterpret_cast<const char*>(base) + (i << 2); }
+__device__ inline F4 ld4(const float* base, uint32_t i) { return *reinterpret_cast<const F4*>(at_(base, i)); }
+__device__ inline I4 ldi4(const int32_t* base, uint32_t i) { return *reinterpret_cast<const I4*>(at_(base, i)); }
+__device__ inline float ld1(const float* base, uint32_t i) { return *reinterpret_cast<const float*>(at_(base, i)); }
+__device__ inline void st4(float* base, uint32_t i, const F4& v) {
+    *reinterpret_cast<F4*>(reinterpret_cast<char*>(base) + (i << 2)) = v;
+}
 
 // Stage rows [r0, r0 + nrows) x columns [j0 - hl, j0 + kTW + hr) of f into lds (row pitch
 // `pitch` floats, column j0 - hl at offset 0). hl and hr are multiples of 4; ny % 4 == 0, so
@@ -46,7 +53,7 @@ __device__ inline void stage_tile(float* lds, const float* __restrict__ f, int64
         const int64_t r = r0 + row;
         const int64_t jj = j0 - hl + 4 * qd;
         F4 v{0.f, 0.f, 0.f, 0.f};
-        if (r >= 0 && r < nx && jj >= 0 && jj < ny) v = ld4(f + r * ny + jj);
+        if (r >= 0 && r < nx && jj >= 0 && jj < ny) v = ld4(f, (uint32_t)(r * ny + jj));
         *reinterpret_cast<F4*>(lds + row * pitch + 4 * qd) = v;
     }
 }
@@ -82,27 +89,27 @@ __global__ __launch_bounds__(kLanes) void k_velocity_tile(DevSim batch) {
         const int lr = ty + 4 * m;  // row within the tile
         const int64_t i = i0 + lr;
         if (i >= nx - kHalo) break;
-        const int64_t c = i * ny + jq;
-        const F4 rho0 = ld4(s.rho + c);
-        const F4 rho1 = ld4(s.rho + c + ny);
-        const float rho_r = (jq + 4 < ny) ? s.rho[c + 4] : 0.f;
-        const I4 dc = UNIFORM ? I4{0, 0, 0, 0} : ldi4(s.dcmap + c);
+        const uint32_t c = (uint32_t)(i * ny + jq);
+        const F4 rho0 = ld4(s.rho, c);
+        const F4 rho1 = ld4(s.rho, c + (uint32_t)ny);
+        const float rho_r = (jq + 4 < ny) ? ld1(s.rho, c + 4) : 0.f;
+        const I4 dc = UNIFORM ? I4{0, 0, 0, 0} : ldi4(s.dcmap, c);
         F4 ik[2], b1[2], b2[2], a1[2], a2[2];
         constexpr int NA = ISO ? 1 : 2;
 #pragma unroll
         for (int a = 0; a < NA; ++a) {
-            ik[a] = ld4(s.ikv[a] + c);
-            b1[a] = ld4(s.bv[a][0] + c);
-            b2[a] = ld4(s.bv[a][1] + c);
-            a1[a] = ld4(s.av[a][0] + c);
-            a2[a] = ld4(s.av[a][1] + c);
+            ik[a] = ld4(s.ikv[a], c);
+            b1[a] = ld4(s.bv[a][0], c);
+            b2[a] = ld4(s.bv[a][1], c);
+            a1[a] = ld4(s.av[a][0], c);
+            a2[a] = ld4(s.av[a][1], c);
         }
         F4 ps1[2], ps2[2], v[2];
 #pragma unroll
         for (int a = 0; a < 2; ++a) {
-            ps1[a] = ld4(s.psi[a][0] + c);
-            ps2[a] = ld4(s.psi[a][1] + c);
-            v[a] = ld4(s.v[a] + c);
+            ps1[a] = ld4(s.psi[a][0], c);
+            ps2[a] = ld4(s.psi[a][1], c);
+            v[a] = ld4(s.v[a], c);
         }
         // LDS row of tile row lr is lr + 7; column j is at (j - j0 + 8)
         const float* row0 = P + (lr + 7) * PITCH + 4 * qd + 8;
@@ -147,9 +154,9 @@ __global__ __launch_bounds__(kLanes) void k_velocity_tile(DevSim batch) {
         // cells outside the active columns keep their loaded values, so whole quads store back
 #pragma unroll
         for (int a = 0; a < 2; ++a) {
-            st4(s.psi[a][0] + c, ps1[a]);
-            st4(s.psi[a][1] + c, ps2[a]);
-            st4(s.v[a] + c, v[a]);
+            st4(s.psi[a][0], c, ps1[a]);
+            st4(s.psi[a][1], c, ps2[a]);
+            st4(s.v[a], c, v[a]);
         }
     }
 }
@@ -185,25 +192,25 @@ __global__ __launch_bounds__(kLanes) void k_pressure_tile(DevSim batch) {
         const int lr = ty + 4 * m;
         const int64_t i = i0 + lr;
         if (i >= nx - kHalo) break;
-        const int64_t c = i * ny + jq;
-        F4 pv = ld4(s.p + c);
-        const F4 K = ld4(s.K + c);
-        const I4 dc = UNIFORM ? I4{0, 0, 0, 0} : ldi4(s.dcmap + c);
+        const uint32_t c = (uint32_t)(i * ny + jq);
+        F4 pv = ld4(s.p, c);
+        const F4 K = ld4(s.K, c);
+        const I4 dc = UNIFORM ? I4{0, 0, 0, 0} : ldi4(s.dcmap, c);
         constexpr int NA = ISO ? 1 : 2;
         F4 ik[2], b1[2], b2[2], a1[2], a2[2];
 #pragma unroll
         for (int a = 0; a < NA; ++a) {
-            ik[a] = ld4(s.ikp[a] + c);
-            b1[a] = ld4(s.bp[a][0] + c);
-            b2[a] = ld4(s.bp[a][1] + c);
-            a1[a] = ld4(s.ap[a][0] + c);
-            a2[a] = ld4(s.ap[a][1] + c);
+            ik[a] = ld4(s.ikp[a], c);
+            b1[a] = ld4(s.bp[a][0], c);
+            b2[a] = ld4(s.bp[a][1], c);
+            a1[a] = ld4(s.ap[a][0], c);
+            a2[a] = ld4(s.ap[a][1], c);
         }
         F4 ph1[2], ph2[2];
 #pragma unroll
         for (int a = 0; a < 2; ++a) {
-            ph1[a] = ld4(s.phi[a][0] + c);
-            ph2[a] = ld4(s.phi[a][1] + c);
+            ph1[a] = ld4(s.phi[a][0], c);
+            ph2[a] = ld4(s.phi[a][1], c);
         }
         const float* xrow = VX + (lr + 8) * XP + 4 * qd + 4;  // vx(i, jq)
         const float* yrow = VY + (lr + 1) * YP + 4 * qd + 8;  // vy(i, jq)
@@ -245,11 +252,11 @@ __global__ __launch_bounds__(kLanes) void k_pressure_tile(DevSim batch) {
             }
             pv[e] -= dt * K[e] * div;
         }
-        st4(s.p + c, pv);
+        st4(s.p, c, pv);
 #pragma unroll
         for (int a = 0; a < 2; ++a) {
-            st4(s.phi[a][0] + c, ph1[a]);
-            st4(s.phi[a][1] + c, ph2[a]);
+            st4(s.phi[a][0], c, ph1[a]);
+            st4(s.phi[a][1], c, ph2[a]);
         }
     }
 }
@@ -277,7 +284,9 @@ hipError_t launch_tile(const DevSim& s, bool velocity, hipStream_t st) {
 
 }  // namespace
 
-bool tile2d_supported(const DevSim& s) { return !s.aexp && s.ndim == 2 && s.ny % 4 == 0 && s.ndmap <= 1024; }
+bool tile2d_supported(const DevSim& s) {
+    return !s.aexp && s.ndim == 2 && s.ny % 4 == 0 && s.ndmap <= 1024 && s.nx * s.ny < ((int64_t)1 << 30);
+}
 
 hipError_t launch_tile2d(const DevSim& s, const LaunchCfg& cfg, bool velocity, hipStream_t st) {
     switch (cfg.chunk) {
