@@ -160,6 +160,9 @@ def bench_3d(args) -> dict:
     steps = min(args.steps, inp.nt)
     with engine.HipSimulation(inp) as sim:
         kcfg = sim.kernel_config()
+        if args.profile_steps:
+            sim.run(args.profile_steps)
+            return {"kernel_config": kcfg, "kernels": engine.kernel_override_string(kcfg)}
         sim.run(min(args.warmup, inp.nt - steps))
         sim.reset()
         t1 = time.perf_counter()

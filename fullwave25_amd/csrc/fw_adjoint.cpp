@@ -128,13 +128,14 @@ int fw_adj_create(fw_sim* f, int64_t checkpoint_every, fw_adj** out) {
     if (!dalloc((void**)&a->vox, vox.size() * 8) || !dalloc((void**)&a->first, first.size() * 4) ||
         !dalloc((void**)&a->order, idx.size() * 4))
         return cleanup(fw::set_error(FW_ERR_NOMEM, "adjoint sensor tables"));
-    if (!vox.empty()) {
-        if (hipMemcpy(a->vox, vox.data(), vox.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
-            hipMemcpy(a->order, idx.data(), idx.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
-            return cleanup(fw::set_error(FW_ERR_DEVICE, "upload of the sensor tables failed"));
-    }
-    if (hipMemcpy(a->first, first.data(), first.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
-        return cleanup(fw::set_error(FW_ERR_DEVICE, "upload of the sensor tables failed"));
+    // on the handle's stream, waited for (its kernels do not order after the null stream)
+    bool ok = true;
+    if (!vox.empty())
+        ok = hipMemcpyAsync(a->vox, vox.data(), vox.size() * 8, hipMemcpyHostToDevice, f->stream) == hipSuccess &&
+             hipMemcpyAsync(a->order, idx.data(), idx.size() * 4, hipMemcpyHostToDevice, f->stream) == hipSuccess;
+    ok = ok && hipMemcpyAsync(a->first, first.data(), first.size() * 4, hipMemcpyHostToDevice, f->stream) == hipSuccess;
+    ok = ok && hipStreamSynchronize(f->stream) == hipSuccess;
+    if (!ok) return cleanup(fw::set_error(FW_ERR_DEVICE, "upload of the sensor tables failed"));
     *out = a;
     return FW_OK;
 }
@@ -169,6 +170,7 @@ int fw_adj_backward(fw_adj* a, const float* grad_rec, size_t grad_bytes, float* 
     hipStream_t st = f->stream;
     const int nd = f->ndim;
     ADJ_HIP(hipMemcpyAsync(a->grec, grad_rec, grad_bytes, hipMemcpyHostToDevice, st));
+    ADJ_HIP(hipStreamSynchronize(st));  // grad_rec is the caller's buffer, read during the call only
     ADJ_HIP(hipMemsetAsync(a->lam, 0, (size_t)f->nstate * 4, st));
     ADJ_HIP(hipMemsetAsync(a->work, 0, (size_t)((3 * nd + 2) * n) * 4, st));
 

@@ -65,16 +65,32 @@ int64_t cells(const fw_desc& d) { return d.nx * d.ny * (d.ndim == 3 ? d.nz : 1);
 // starts on a cache-line boundary; 2D grids keep rows of ny floats.
 int64_t row_pitch(const fw_desc& d) { return d.ndim == 3 ? (d.nz + 31) / 32 * 32 : d.ny; }
 
+// Every copy and fill goes through the handle's own stream and is waited for: the handle's
+// kernels run on that (non-blocking) stream, which does not order itself after work on the
+// null stream, so a copy issued there could still be in flight when a kernel reads the data.
+
 // compact host grid -> padded device grid (elements of `esize` bytes)
 hipError_t copy_grid_in(const fw_sim* s, void* dst, const void* src, size_t esize) {
-    return hipMemcpy2D(dst, (size_t)s->pitch * esize, src, (size_t)s->row_len * esize, (size_t)s->row_len * esize,
-                       (size_t)s->rows, hipMemcpyHostToDevice);
+    hipError_t e = hipMemcpy2DAsync(dst, (size_t)s->pitch * esize, src, (size_t)s->row_len * esize,
+                                    (size_t)s->row_len * esize, (size_t)s->rows, hipMemcpyHostToDevice, s->stream);
+    return e != hipSuccess ? e : hipStreamSynchronize(s->stream);
 }
 
 // padded device grid -> compact host grid
 hipError_t copy_grid_out(const fw_sim* s, void* dst, const void* src, size_t esize) {
-    return hipMemcpy2D(dst, (size_t)s->row_len * esize, src, (size_t)s->pitch * esize, (size_t)s->row_len * esize,
-                       (size_t)s->rows, hipMemcpyDeviceToHost);
+    hipError_t e = hipMemcpy2DAsync(dst, (size_t)s->row_len * esize, src, (size_t)s->pitch * esize,
+                                    (size_t)s->row_len * esize, (size_t)s->rows, hipMemcpyDeviceToHost, s->stream);
+    return e != hipSuccess ? e : hipStreamSynchronize(s->stream);
+}
+
+hipError_t copy_sync(const fw_sim* s, void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, s->stream);
+    return e != hipSuccess ? e : hipStreamSynchronize(s->stream);
+}
+
+hipError_t fill_zero_sync(const fw_sim* s, void* dst, size_t bytes) {
+    hipError_t e = hipMemsetAsync(dst, 0, bytes, s->stream);
+    return e != hipSuccess ? e : hipStreamSynchronize(s->stream);
 }
 
 std::vector<std::string> required_maps(const fw_desc& d) {
@@ -369,7 +385,7 @@ int finalize(fw_sim* s) {
     auto up = [&](const char* name, const void* h, size_t bytes) -> int {
         int r = alloc(s, name, bytes, &p);
         if (r != FW_OK) return r;
-        if (bytes) FW_HIP(hipMemcpy(p, h, bytes, hipMemcpyHostToDevice));
+        if (bytes) FW_HIP(copy_sync(s, p, h, bytes, hipMemcpyHostToDevice));
         return FW_OK;
     };
     if ((rc = up("_src_lin", src_lin.data(), src_lin.size() * 8)) != FW_OK) return rc;
@@ -417,7 +433,7 @@ int finalize(fw_sim* s) {
     if ((rc = alloc(s, "_rec", rec_bytes, &p)) != FW_OK) return rc;
     s->db.rec = (float*)p;
     s->db.rec_stride = s->rec_cap * d.ncoordsout;
-    FW_HIP(hipMemset(p, 0, rec_bytes));
+    FW_HIP(fill_zero_sync(s, p, rec_bytes));
 
     // map pointers
     const int nax = d.isotropic ? 1 : nd;
@@ -677,7 +693,7 @@ int fw_sim_create(const fw_desc* desc, int device, fw_sim** out) {
     s->nstate = (int64_t)(1 + nd + (relax ? 4 * nd : 0)) * field;
     if (hipMalloc(&s->state, (size_t)s->nstate * 4) != hipSuccess)
         return cleanup(fail(FW_ERR_NOMEM, "hipMalloc of %lld state floats failed", (long long)s->nstate));
-    if (hipMemset(s->state, 0, (size_t)s->nstate * 4) != hipSuccess)
+    if (fill_zero_sync(s, s->state, (size_t)s->nstate * 4) != hipSuccess)
         return cleanup(fail(FW_ERR_DEVICE, "hipMemset failed"));
     float* f = s->state;
     s->ds.p = f;
@@ -719,7 +735,7 @@ int fw_sim_upload(fw_sim* s, const char* name_c, const void* host, size_t bytes)
         if (bytes != (size_t)s->ncell_c * 4)
             return fail(FW_ERR_INVALID, "'%s' must be %lld bytes, got %zu", name_c, (long long)s->ncell_c * 4, bytes);
         if ((rc = alloc(s, name, (size_t)s->ncell * 4, &p)) != FW_OK) return rc;
-        if (s->pitch != s->row_len) FW_HIP(hipMemset(p, 0, (size_t)s->ncell * 4));  // row padding
+        if (s->pitch != s->row_len) FW_HIP(fill_zero_sync(s, p, (size_t)s->ncell * 4));  // row padding
         FW_HIP(copy_grid_in(s, p, host, 4));
         if (name.rfind("kappa", 0) == 0) {
             FW_HIP(fw::launch_reciprocal((float*)p, s->ncell, s->stream));
@@ -757,7 +773,7 @@ int fw_sim_upload(fw_sim* s, const char* name_c, const void* host, size_t bytes)
                     t[(size_t)(b * per_shot + n * d.ncoords + q)] = src[b * per_shot + q * d.ntic + n];
         s->db.icmat_stride = per_shot;
         if ((rc = alloc(s, name, bytes, &p)) != FW_OK) return rc;
-        if (bytes) FW_HIP(hipMemcpy(p, t.data(), bytes, hipMemcpyHostToDevice));
+        if (bytes) FW_HIP(copy_sync(s, p, t.data(), bytes, hipMemcpyHostToDevice));
         s->db.icmat_t = (const float*)p;
         s->have[name] = true;
         return FW_OK;
@@ -808,8 +824,9 @@ int fw_sim_download_sensors(fw_sim* s, float* out, size_t bytes) {
                     s->nshots, (long long)nrec, bytes);
     if (!want) return FW_OK;
     DeviceGuard g(s->device);
-    FW_HIP(hipMemcpy2D(out, row, s->db.rec, (size_t)s->db.rec_stride * 4, row, (size_t)s->nshots,
-                       hipMemcpyDeviceToHost));
+    FW_HIP(hipMemcpy2DAsync(out, row, s->db.rec, (size_t)s->db.rec_stride * 4, row, (size_t)s->nshots,
+                            hipMemcpyDeviceToHost, s->stream));
+    FW_HIP(hipStreamSynchronize(s->stream));
     return FW_OK;
 }
 

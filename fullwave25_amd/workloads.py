@@ -43,6 +43,15 @@ def _grid(nx: int, ny: int, *, f0=3e6, c0=1540.0, ppw=12, cfl=0.4, nt=2000, nz: 
 
 def _inputs(grid, medium, source, sensor, *, n_pml: int, n_transition: int | None = None,
             isotropic: bool = True, exponential: bool = False):
+    n_tr = grid.ppw * 3 if n_transition is None else n_transition
+    if not exponential and n_pml > 0 and n_tr > 0:
+        # the plane-wise builder gives the Solver path's inputs byte for byte, faster and in
+        # far less memory on large grids
+        from .solver.fast_inputs import build_inputs_planewise
+
+        medium.use_isotropic_relaxation = isotropic
+        return build_inputs_planewise(grid, medium, source, sensor, n_pml=n_pml, n_transition=n_tr,
+                                      isotropic=isotropic)
     with tempfile.TemporaryDirectory() as tmp:
         solver = Solver(Path(tmp), grid, medium, source, sensor, pml_layer_thickness_px=n_pml,
                         n_transition_layer=n_transition, use_isotropic_relaxation=isotropic,
