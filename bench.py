@@ -28,18 +28,37 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+# bytes per lattice update of a whole isotropic time step in the step-level accounting the
+# throughput targets are stated in (2D: 140, 3D: 180); GLUPS x this / peak = step roofline
+STEP_BYTES_PER_LUP = {2: 140, 3: 180}
 METRIC = "GLUPS + % HBM roofline (1 GPU); served jobs/hr with p50/p99 latency"
 
 
-def algorithmic_bytes_per_cell(ndim: int, isotropic: bool) -> dict:
-    """Bytes each kernel must move per updated cell (float32 words and the int32 dcmap)."""
+def algorithmic_bytes_per_cell(ndim: int, isotropic: bool, *, uniform: bool = False,
+                               coded_fraction: float | None = None) -> dict:
+    """Bytes each kernel must move per updated cell.
+
+    Wave state and medium, float32 words: velocity reads p, rho, psi (2 per axis), v and writes
+    v, psi; pressure reads v, K, p, phi (2 per axis) and writes p, phi. Relaxation coefficients:
+    1/kappa, a1, a2, b1, b2 per map set (one set when isotropic) and the int32 dcmap (not read
+    when every cell uses one coefficient set). With coded coefficients (``coded_fraction`` =
+    share of cells carrying a code) every cell reads a 2-byte code instead, and the cells
+    without a code also read the maps."""
     nd = ndim
     nax = 1 if isotropic else nd
-    # velocity: reads p, rho, dcmap, 1/kappa (nax), a/b (4 nax), psi (2 nd), v (nd); writes v, psi
-    vel = 4 * ((1 + 1 + 1 + nax + 4 * nax + 2 * nd + nd) + (nd + 2 * nd))
-    # pressure: reads v (nd), K, dcmap, 1/kappa (nax), a/b (4 nax), phi (2 nd), p; writes p, phi
-    pre = 4 * ((nd + 1 + 1 + nax + 4 * nax + 2 * nd + 1) + (1 + 2 * nd))
-    return {"velocity": vel, "pressure": pre}
+    coef = 4 * (5 * nax + (0 if uniform else 1))
+    if coded_fraction is not None:
+        coef = 2 + (1.0 - coded_fraction) * coef
+    vel = 4 * (2 + 6 * nd) + coef
+    pre = 4 * (3 + 5 * nd) + coef
+    return {"velocity": round(vel, 2), "pressure": round(pre, 2)}
+
+
+def coefficient_layout(sim, inp) -> dict:
+    """How the handle reads relaxation coefficients, for the byte accounting."""
+    n, frac = sim.relaxation_coding()
+    ndmap = int(np.asarray(inp.dmap).shape[-1])
+    return {"uniform": ndmap == 1, "coded_fraction": frac if n else None, "codes": n}
 
 
 def build_workload(args):
@@ -76,8 +95,8 @@ def pmc_summary(fetch_dir: Path, write_dir: Path, copy_fetch_dir: Path, copy_wri
     w_scale = copy_bytes / 1024.0 / copy_kib(copy_write_dir)
     fetch, write = per_kernel(fetch_dir), per_kernel(write_dir)
     cfg = json.loads(Path(config_file).read_text())
-    out = {"kernel_config": cfg["kernel_config"], "fetch_scale": round(f_scale, 4), "write_scale": round(w_scale, 4),
-           "kernels": {}}
+    out = {"kernel_config": cfg["kernel_config"], "codes": cfg.get("codes", 0), "fetch_scale": round(f_scale, 4),
+           "write_scale": round(w_scale, 4), "kernels": {}}
     for kind, tag in (("velocity", "k_velocity"), ("pressure", "k_pressure")):
         # the pinned kernel is the one dispatched every step; others ran only once or twice
         name = max((k for k in fetch if tag in k), key=lambda k: len(fetch[k]))
@@ -89,14 +108,14 @@ def pmc_summary(fetch_dir: Path, write_dir: Path, copy_fetch_dir: Path, copy_wri
     return out
 
 
-def measured_traffic(workload: str, kind: str, kernel_config: dict):
+def measured_traffic(workload: str, kind: str, kernel_config: dict, codes: int):
     """HBM bytes per launch of ``kind`` from the committed counter summary for this workload, if
-    it was measured with the same kernel configuration; else None."""
+    it was measured with the same kernel configuration and coefficient coding; else None."""
     path = ROOT / "profiles" / f"pmc_traffic_{workload}.json"
     if not path.exists():
         return None
     d = json.loads(path.read_text())
-    if d.get("kernel_config", {}).get(kind) != kernel_config.get(kind):
+    if d.get("kernel_config", {}).get(kind) != kernel_config.get(kind) or bool(d.get("codes", 0)) != bool(codes):
         return None
     return d["kernels"][kind]["bytes_per_launch"]
 
@@ -160,9 +179,10 @@ def bench_3d(args) -> dict:
     steps = min(args.steps, inp.nt)
     with engine.HipSimulation(inp) as sim:
         kcfg = sim.kernel_config()
+        layout = coefficient_layout(sim, inp)
         if args.profile_steps:
             sim.run(args.profile_steps)
-            return {"kernel_config": kcfg, "kernels": engine.kernel_override_string(kcfg)}
+            return {"kernel_config": kcfg, "kernels": engine.kernel_override_string(kcfg), "codes": layout["codes"]}
         sim.run(min(args.warmup, inp.nt - steps))
         sim.reset()
         t1 = time.perf_counter()
@@ -172,7 +192,8 @@ def bench_3d(args) -> dict:
         sim.reset()
         kms = sim.run_timed(steps)
     updated = int(np.prod([s - 16 for s in inp.shape]))
-    bpc = algorithmic_bytes_per_cell(inp.ndim, inp.isotropic)
+    bpc = algorithmic_bytes_per_cell(inp.ndim, inp.isotropic, uniform=layout["uniform"],
+                                     coded_fraction=layout["coded_fraction"])
     dominant = max(("velocity", "pressure"), key=lambda k: kms[k])
     avg_s = kms[dominant] / 1e3 / steps
     achieved = bpc[dominant] * updated / avg_s / 1e9
@@ -184,10 +205,14 @@ def bench_3d(args) -> dict:
                                f"extended), {steps} of the job's 6000 steps timed, isotropic relaxation",
                    "cells": inp.ncells, "sources": inp.ncoords, "sensors": inp.ncoordsout, "finite_output": finite},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "bytes_per_cell": bpc[dominant],
-                     "avg_launch_us": round(avg_s * 1e6, 2),
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic(wl.name, dominant, kcfg, layout["codes"]),
+                     "bytes_per_cell": bpc[dominant], "avg_launch_us": round(avg_s * 1e6, 2),
                      "kernel_ms_per_step": {k: round(v / steps, 4) for k, v in kms.items()},
-                     "kernel_config": kcfg, "kernels": engine.kernel_override_string(kcfg)},
+                     "kernel_config": kcfg, "kernels": engine.kernel_override_string(kcfg),
+                     "relaxation_codes": layout["codes"], "coded_fraction": layout["coded_fraction"],
+                     "step_bytes_per_lup": STEP_BYTES_PER_LUP[3],
+                     "step_frac": round(inp.ncells * steps / elapsed * STEP_BYTES_PER_LUP[3] / 1e9 / HBM_PEAK_GBS, 4)},
     }
 
 
@@ -375,11 +400,13 @@ def main() -> int:
         if inp.ndim == 3:
             shapes = ["percell"] + [f"{v}:{w}:{c}" for v in ("stream", "pipe", "pipelite") for w in (512, 1024)
                                     for c in (8, 16, 32)]
-        bpc = algorithmic_bytes_per_cell(inp.ndim, inp.isotropic)
         updated = int(np.prod([s - 16 for s in inp.shape]))
         for shape in shapes:
             os.environ["FULLWAVE_HIP_KERNELS"] = shape
             with engine.HipSimulation(inp, device=0) as sim:
+                layout = coefficient_layout(sim, inp)
+                bpc = algorithmic_bytes_per_cell(inp.ndim, inp.isotropic, uniform=layout["uniform"],
+                                                 coded_fraction=layout["coded_fraction"])
                 sim.run_timed(min(20, args.sweep))
                 sim.reset()
                 kms = sim.run_timed(args.sweep)
@@ -390,10 +417,12 @@ def main() -> int:
         return 0
     sim = engine.HipSimulation(inp, device=0)
     kcfg = sim.kernel_config()
+    layout = coefficient_layout(sim, inp)
     if args.profile_steps:
         sim.run(args.profile_steps)
         sim.close()
-        print(json.dumps({"kernel_config": kcfg, "kernels": engine.kernel_override_string(kcfg)}), flush=True)
+        print(json.dumps({"kernel_config": kcfg, "kernels": engine.kernel_override_string(kcfg),
+                          "codes": layout["codes"]}), flush=True)
         return 0
     sim.run(min(args.warmup, inp.nt))
     sim.reset()
@@ -413,7 +442,8 @@ def main() -> int:
     cells = inp.ncells
     updated = int(np.prod([s - 16 for s in inp.shape]))
     glups = cells * args.steps / elapsed / 1e9
-    bpc = algorithmic_bytes_per_cell(inp.ndim, inp.isotropic)
+    bpc = algorithmic_bytes_per_cell(inp.ndim, inp.isotropic, uniform=layout["uniform"],
+                                     coded_fraction=layout["coded_fraction"])
     dominant = max(("velocity", "pressure"), key=lambda k: kms[k])
     avg_s = kms[dominant] / 1e3 / args.steps
     achieved = bpc[dominant] * updated / avg_s / 1e9
@@ -445,13 +475,17 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": measured_traffic(wl.name, dominant, kcfg),
+            "traffic": measured_traffic(wl.name, dominant, kcfg, layout["codes"]),
             "bytes_per_cell": bpc[dominant],
             "avg_launch_us": round(avg_s * 1e6, 2),
             "kernel_ms_per_step": {k: round(v / args.steps, 5) for k, v in kms.items()},
             "copy_kernel_gbs": round(copy_gbs, 1),
             "kernel_config": kcfg,
             "kernels": engine.kernel_override_string(kcfg),
+            "relaxation_codes": layout["codes"],
+            "coded_fraction": layout["coded_fraction"],
+            "step_bytes_per_lup": STEP_BYTES_PER_LUP[inp.ndim],
+            "step_frac": round(glups * STEP_BYTES_PER_LUP[inp.ndim] / HBM_PEAK_GBS, 4),
         },
     }
     if not args.no_cpu_baseline:

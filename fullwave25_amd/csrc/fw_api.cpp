@@ -346,6 +346,124 @@ int choose_kernels_unpinned(fw_sim* s) {
     return FW_OK;
 }
 
+bool codes_wanted(const fw_sim* s) {
+    if (s->codes_mode >= 0) return s->codes_mode == 1;
+    const char* env = getenv("FULLWAVE_HIP_CODES");
+    return !(env && env[0] == '0');
+}
+
+void drop_codes(fw_sim* s) {
+    for (const char* name : {"_rcode", "_rtab_v", "_rtab_p"}) {
+        auto it = s->dev.find(name);
+        if (it == s->dev.end()) continue;
+        (void)hipFree(it->second);
+        s->dev.erase(it);
+    }
+    s->ds.rcode = nullptr;
+    s->ds.rtab_v = nullptr;
+    s->ds.rtab_p = nullptr;
+    s->ncodes = 0;
+    s->coded_cells = 0;
+}
+
+// Code the isotropic relaxation maps and dcmap (kernels and the steps in fw_kernels_codes.hip).
+// Leaves the handle on the per-cell maps when the model is not isotropic relaxation, when codes
+// are switched off, when the codes would cover less than half of the cells, or when any check
+// fails; device errors are errors.
+int encode_relaxation(fw_sim* s) {
+    drop_codes(s);
+    const fw_desc& d = s->d;
+    if (d.attenuation_model != 0 || !d.isotropic || !codes_wanted(s)) return FW_OK;
+    if (s->ncell >= ((int64_t)1 << 30)) return FW_OK;  // the marching kernels' 32-bit offsets
+    fw::CodeMaps m{};
+    const char* names[fw::kCodeVals - 1] = {"kappau", "apmlu1", "apmlu2", "bpmlu1", "bpmlu2",
+                                            "kappax", "apmlx1", "apmlx2", "bpmlx1", "bpmlx2"};
+    for (int k = 0; k < fw::kCodeVals - 1; ++k) m.f[k] = (const float*)s->dev[names[k]];
+    m.dcmap = (const int32_t*)s->dev["dcmap"];
+
+    const size_t nslots = fw::kCodeSlots;
+    unsigned long long* keys = nullptr;
+    uint32_t* count = nullptr;
+    int32_t* rep = nullptr;
+    int32_t* status = nullptr;
+    uint16_t* slot_code_d = nullptr;
+    int32_t* rep_of_code_d = nullptr;
+    auto run = [&]() -> int {
+        FW_HIP(hipMalloc((void**)&keys, nslots * 8));
+        FW_HIP(hipMalloc((void**)&count, nslots * 4));
+        FW_HIP(hipMalloc((void**)&rep, nslots * 4));
+        FW_HIP(hipMalloc((void**)&status, 16));
+        FW_HIP(hipMemsetAsync(keys, 0, nslots * 8, s->stream));
+        FW_HIP(hipMemsetAsync(count, 0, nslots * 4, s->stream));
+        FW_HIP(hipMemsetAsync(status, 0, 16, s->stream));
+        FW_HIP(fw::launch_code_insert(m, s->ncell, keys, count, rep, status, s->stream));
+        int32_t st[4] = {0, 0, 0, 0};
+        FW_HIP(copy_sync(s, st, status, 16, hipMemcpyDeviceToHost));
+        if (st[0]) return FW_OK;  // probe limit: far too many distinct tuples
+        std::vector<unsigned long long> hk(nslots);
+        std::vector<uint32_t> hc(nslots);
+        std::vector<int32_t> hr(nslots);
+        FW_HIP(copy_sync(s, hk.data(), keys, nslots * 8, hipMemcpyDeviceToHost));
+        FW_HIP(copy_sync(s, hc.data(), count, nslots * 4, hipMemcpyDeviceToHost));
+        FW_HIP(copy_sync(s, hr.data(), rep, nslots * 4, hipMemcpyDeviceToHost));
+        std::vector<uint32_t> occupied;
+        for (size_t q = 0; q < nslots; ++q)
+            if (hk[q]) occupied.push_back((uint32_t)q);
+        if (occupied.empty()) return FW_OK;
+        // the most frequent tuples get codes (ties: lower slot first, so the choice is fixed)
+        std::sort(occupied.begin(), occupied.end(), [&](uint32_t x, uint32_t y) {
+            return hc[x] != hc[y] ? hc[x] > hc[y] : x < y;
+        });
+        if (occupied.size() > (size_t)fw::kMaxCodes) occupied.resize((size_t)fw::kMaxCodes);
+        int64_t covered = 0;
+        for (uint32_t q : occupied) covered += hc[q];
+        if (2 * covered < s->ncell) return FW_OK;
+        std::vector<uint16_t> slot_code(nslots, (uint16_t)fw::kCodeEscape);
+        std::vector<int32_t> rep_of_code(occupied.size());
+        for (size_t i = 0; i < occupied.size(); ++i) {
+            slot_code[occupied[i]] = (uint16_t)i;
+            rep_of_code[i] = hr[occupied[i]];
+        }
+        const int32_t n = (int32_t)occupied.size();
+        FW_HIP(hipMalloc((void**)&slot_code_d, nslots * 2));
+        FW_HIP(hipMalloc((void**)&rep_of_code_d, (size_t)n * 4));
+        FW_HIP(copy_sync(s, slot_code_d, slot_code.data(), nslots * 2, hipMemcpyHostToDevice));
+        FW_HIP(copy_sync(s, rep_of_code_d, rep_of_code.data(), (size_t)n * 4, hipMemcpyHostToDevice));
+        void *code, *tv, *tp;
+        int r;
+        if ((r = alloc(s, "_rcode", (size_t)s->ncell * 2, &code)) != FW_OK) return r;
+        if ((r = alloc(s, "_rtab_v", (size_t)n * fw::kCodeStride * 4, &tv)) != FW_OK) return r;
+        if ((r = alloc(s, "_rtab_p", (size_t)n * fw::kCodeStride * 4, &tp)) != FW_OK) return r;
+        FW_HIP(fw::launch_code_assign(m, s->ncell, keys, slot_code_d, (uint16_t*)code, status, s->stream));
+        FW_HIP(fw::launch_code_table(m, n, rep_of_code_d, (float*)tv, (float*)tp, s->stream));
+        FW_HIP(fw::launch_code_verify(m, s->ncell, (const uint16_t*)code, n, (const float*)tv, (const float*)tp,
+                                      status, s->stream));
+        FW_HIP(copy_sync(s, st, status, 16, hipMemcpyDeviceToHost));
+        if (st[1] || st[2]) return FW_OK;  // a key not found again, or a collision
+        s->ds.rcode = (const uint16_t*)code;
+        s->ds.rtab_v = (const float*)tv;
+        s->ds.rtab_p = (const float*)tp;
+        s->ncodes = n;
+        s->coded_cells = covered;
+        return FW_OK;
+    };
+    const int rc = run();
+    for (void* q : {(void*)keys, (void*)count, (void*)rep, (void*)status, (void*)slot_code_d, (void*)rep_of_code_d})
+        if (q) (void)hipFree(q);
+    if (rc != FW_OK || s->ncodes == 0) drop_codes(s);
+    return rc;
+}
+
+// New codes for a handle that has run: a captured graph holds the old table addresses.
+int recode(fw_sim* s) {
+    FW_HIP(hipStreamSynchronize(s->stream));
+    if (s->graph) {
+        (void)hipGraphExecDestroy(s->graph);
+        s->graph = nullptr;
+    }
+    return encode_relaxation(s);
+}
+
 // Resolve coordinate lists into device index arrays and build the coefficient table.
 int finalize(fw_sim* s) {
     const fw_desc& d = s->d;
@@ -455,6 +573,7 @@ int finalize(fw_sim* s) {
     s->ds.dcmap = (const int32_t*)s->dev["dcmap"];
     s->ds.ndmap = (int32_t)d.ndmap;
     s->ds.isotropic = d.isotropic;
+    if ((rc = encode_relaxation(s)) != FW_OK) return rc;
     if ((rc = choose_kernels(s)) != FW_OK) return rc;
     s->ready = true;
     return FW_OK;
@@ -742,6 +861,9 @@ int fw_sim_upload(fw_sim* s, const char* name_c, const void* host, size_t bytes)
             FW_HIP(hipStreamSynchronize(s->stream));
         }
         s->have[name] = true;
+        const bool relaxation_map = name.rfind("kappa", 0) == 0 || name.rfind("apml", 0) == 0 ||
+                                    name.rfind("bpml", 0) == 0 || name == "dcmap";
+        if (s->ready && relaxation_map && d.isotropic) return recode(s);
         return FW_OK;
     }
     if (name == "dmap") {
@@ -892,6 +1014,27 @@ int fw_sim_set_graph_steps(fw_sim* s, int64_t steps) {
         s->graph = nullptr;
     }
     s->graph_steps = steps;
+    return FW_OK;
+}
+
+int fw_sim_set_relaxation_codes(fw_sim* s, int32_t enable) {
+    if (!s) return fail(FW_ERR_INVALID, "null handle");
+    if (enable != 0 && enable != 1) return fail(FW_ERR_INVALID, "enable must be 0 or 1");
+    s->codes_mode = enable;
+    if (!s->ready) return FW_OK;
+    DeviceGuard g(s->device);
+    return recode(s);
+}
+
+int fw_sim_relaxation_codes(fw_sim* s, int32_t* n_codes, double* coded_fraction) {
+    if (!s || !n_codes) return fail(FW_ERR_INVALID, "null argument");
+    if (!s->ready) {
+        DeviceGuard g(s->device);
+        int rc = finalize(s);
+        if (rc != FW_OK) return rc;
+    }
+    *n_codes = s->ncodes;
+    if (coded_fraction) *coded_fraction = s->ncell ? (double)s->coded_cells / (double)s->ncell : 0.0;
     return FW_OK;
 }
 

@@ -45,6 +45,29 @@ struct DevSim {
     // nshots consecutive grids, shot_stride floats apart; the maps are shared
     int32_t nshots;
     int64_t shot_stride;
+
+    // isotropic relaxation model, coded coefficients (fw_kernels_codes.hip): rcode[c] selects
+    // row rcode[c] of rtab_v / rtab_p, kCodeStride floats {1/kappa, a1, a2, b1, b2, dcmap (int
+    // bits), 0, 0}; rcode[c] == kCodeEscape: the cell reads the per-cell maps and dcmap. The
+    // marching kernels read these instead of ikv/av/bv, ikp/ap/bp and dcmap (rcode == nullptr:
+    // per-cell maps only). The per-cell maps stay valid for every other kernel.
+    const uint16_t* rcode;
+    const float* rtab_v;
+    const float* rtab_p;
+};
+
+constexpr int kCodeVals = 11;          // values per tuple: velocity side, pressure side, dcmap
+constexpr int kCodeStride = 8;         // floats per table row (one 16-byte and one 8-byte load)
+constexpr uint32_t kCodeEscape = 0xffff;  // the cell's tuple has no code
+constexpr int kMaxCodes = 65535;       // codes 0 .. 65534
+constexpr uint32_t kCodeSlots = 1u << 22;  // hash table slots
+constexpr int kCodeMaxProbe = 128;
+static_assert(kCodeStride == 8, "coded_coef addresses rows as code << 5 bytes");
+
+// the maps of a tuple: ikv, av1, av2, bv1, bv2, ikp, ap1, ap2, bp1, bp2 and dcmap (int32)
+struct CodeMaps {
+    const float* f[kCodeVals - 1];
+    const int32_t* dcmap;
 };
 
 // Global memory at a 32-bit element index: the array base stays in scalar registers and each
@@ -57,6 +80,40 @@ __device__ __forceinline__ T gld32(const T* __restrict__ base, uint32_t i) {
 template <class T>
 __device__ __forceinline__ void gst32(T* base, uint32_t i, T v) {
     *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + (i << 2)) = v;
+}
+
+template <class T>
+__device__ __forceinline__ T gld16(const T* __restrict__ base, uint32_t i) {  // 2-byte elements
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + (i << 1));
+}
+
+// Relaxation coefficients and coefficient-set index of one cell.
+struct CodedCoef {
+    float ik, a1, a2, b1, b2;
+    int32_t dc;
+};
+// from row `code` of a code table
+__device__ __forceinline__ CodedCoef coded_coef(const float* __restrict__ tab, uint32_t code) {
+    const char* row = reinterpret_cast<const char*>(tab) + (code << 5);
+    const float4 q = *reinterpret_cast<const float4*>(row);
+    const float2 r = *reinterpret_cast<const float2*>(row + 16);
+    return CodedCoef{q.x, q.y, q.z, q.w, r.x, __float_as_int(r.y)};
+}
+// from the per-cell maps (a cell without a code); dc only where the set index varies
+template <bool UNIFORM>
+__device__ __forceinline__ CodedCoef mapped_coef(const float* ik, const float* a1, const float* a2,
+                                                 const float* b1, const float* b2, const int32_t* dcmap,
+                                                 uint32_t c) {
+    return CodedCoef{gld32(ik, c), gld32(a1, c), gld32(a2, c), gld32(b1, c), gld32(b2, c),
+                     UNIFORM ? 0 : gld32(dcmap, c)};
+}
+// the coefficients of cell c for the velocity (VEL) or pressure update of a coded grid
+template <bool VEL, bool UNIFORM>
+__device__ __forceinline__ CodedCoef cell_coef(const DevSim& s, uint32_t c) {
+    const uint32_t code = gld16(s.rcode, c);
+    if (code != kCodeEscape) return coded_coef(VEL ? s.rtab_v : s.rtab_p, code);
+    if (VEL) return mapped_coef<UNIFORM>(s.ikv[0], s.av[0][0], s.av[0][1], s.bv[0][0], s.bv[0][1], s.dcmap, c);
+    return mapped_coef<UNIFORM>(s.ikp[0], s.ap[0][0], s.ap[0][1], s.bp[0][0], s.bp[0][1], s.dcmap, c);
 }
 
 // The fields of one shot of the batch.
@@ -147,6 +204,14 @@ hipError_t launch_boundary_dev(const DevSim& s, const DevBoundary& b, const int6
 hipError_t launch_tick(int64_t* step, int64_t set, hipStream_t st);
 hipError_t launch_reciprocal(float* data, int64_t n, hipStream_t st);
 hipError_t launch_copy(const void* src, void* dst, int64_t n16, hipStream_t st);
+hipError_t launch_code_insert(const CodeMaps& m, int64_t n, unsigned long long* keys, uint32_t* count, int32_t* rep,
+                              int32_t* status, hipStream_t st);
+hipError_t launch_code_assign(const CodeMaps& m, int64_t n, const unsigned long long* keys, const uint16_t* slot_code,
+                              uint16_t* code, int32_t* status, hipStream_t st);
+hipError_t launch_code_table(const CodeMaps& m, int32_t ncodes, const int32_t* rep_of_code, float* tab_v, float* tab_p,
+                             hipStream_t st);
+hipError_t launch_code_verify(const CodeMaps& m, int64_t n, const uint16_t* code, int32_t ncodes, const float* tab_v,
+                              const float* tab_p, int32_t* status, hipStream_t st);
 hipError_t launch_adj_records(float* lp, const float* grow, const int64_t* vox, const int32_t* first,
                               const int32_t* order, int64_t nvox, hipStream_t st);
 hipError_t launch_adj_mask(float* lp, const int64_t* src_lin, int64_t nsrc, const int64_t* zero_lin, int64_t nzero,

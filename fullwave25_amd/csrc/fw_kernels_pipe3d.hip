@@ -100,7 +100,8 @@ struct VelOps {
     int32_t dc;
 };
 
-template <bool UNIFORM, bool ISO>
+// CODED (isotropic only): the relaxation coefficients come from the code table (rcode, rtab_v/p)
+template <bool UNIFORM, bool ISO, bool CODED>
 __device__ inline void load_vel(VelOps<ISO>& o, const DevSim& s, uint32_t c, bool in_grid, bool active) {
     const uint32_t sx = (uint32_t)s.sx;
     o.qnew = in_grid ? gld(s.p, c + 8 * sx) : 0.f;
@@ -114,20 +115,30 @@ __device__ inline void load_vel(VelOps<ISO>& o, const DevSim& s, uint32_t c, boo
         o.psi[a][0] = gld((const float*)s.psi[a][0], c);
         o.psi[a][1] = gld((const float*)s.psi[a][1], c);
     }
+    if (CODED) {
+        const CodedCoef k = cell_coef<true, UNIFORM>(s, c);
+        o.a[0][0] = k.a1;
+        o.a[0][1] = k.a2;
+        o.b[0][0] = k.b1;
+        o.b[0][1] = k.b2;
+        o.ik[0] = k.ik;
+        o.dc = UNIFORM ? 0 : k.dc;
+    } else {
+        o.dc = UNIFORM ? 0 : gld(s.dcmap, c);
 #pragma unroll
-    for (int a = 0; a < VelOps<ISO>::NA; ++a) {
-        o.a[a][0] = gld(s.av[a][0], c);
-        o.a[a][1] = gld(s.av[a][1], c);
-        o.b[a][0] = gld(s.bv[a][0], c);
-        o.b[a][1] = gld(s.bv[a][1], c);
-        o.ik[a] = gld(s.ikv[a], c);
+        for (int a = 0; a < VelOps<ISO>::NA; ++a) {
+            o.a[a][0] = gld(s.av[a][0], c);
+            o.a[a][1] = gld(s.av[a][1], c);
+            o.b[a][0] = gld(s.bv[a][0], c);
+            o.b[a][1] = gld(s.bv[a][1], c);
+            o.ik[a] = gld(s.ikv[a], c);
+        }
     }
-    o.dc = UNIFORM ? 0 : gld(s.dcmap, c);
 }
 
 // AHEAD: per-cell operands loaded one iteration ahead (more registers); otherwise at the top
 // of their own iteration, before the barrier
-template <int TY, bool UNIFORM, bool ISO, bool AHEAD>
+template <int TY, bool UNIFORM, bool ISO, bool AHEAD, bool CODED>
 __global__ __launch_bounds__(PZ3 * TY) void k_velocity_pipe3d(DevSim batch, int chunk) {
     using TP = PTile<TY, kHalo, kHalo>;
     unsigned zl;
@@ -156,15 +167,15 @@ __global__ __launch_bounds__(PZ3 * TY) void k_velocity_pipe3d(DevSim batch, int 
     st.load(s.p, i0 + 1, s);
     float rho_i = in_grid ? gld(s.rho, (uint32_t)i0 * sx + col) : 0.f;
     VelOps<ISO> cur, nxt;
-    if (AHEAD) load_vel<UNIFORM, ISO>(cur, s, (uint32_t)i0 * sx + col, in_grid, active);
+    if (AHEAD) load_vel<UNIFORM, ISO, CODED>(cur, s, (uint32_t)i0 * sx + col, in_grid, active);
 
     for (int64_t i = i0; i < i1; ++i) {
         st.store(P, i + 1);
         const bool more = i + 1 < i1;
-        if (!AHEAD) load_vel<UNIFORM, ISO>(cur, s, (uint32_t)i * sx + col, in_grid, active);
+        if (!AHEAD) load_vel<UNIFORM, ISO, CODED>(cur, s, (uint32_t)i * sx + col, in_grid, active);
         if (more) {
             st.load(s.p, i + 2, s);
-            if (AHEAD) load_vel<UNIFORM, ISO>(nxt, s, (uint32_t)(i + 1) * sx + col, in_grid, active);
+            if (AHEAD) load_vel<UNIFORM, ISO, CODED>(nxt, s, (uint32_t)(i + 1) * sx + col, in_grid, active);
         }
 #pragma unroll
         for (int m = 0; m < 15; ++m) q[m] = q[m + 1];
@@ -236,7 +247,7 @@ struct PreOps {
     int32_t dc;
 };
 
-template <bool UNIFORM, bool ISO>
+template <bool UNIFORM, bool ISO, bool CODED>
 __device__ inline void load_pre(PreOps<ISO>& o, const DevSim& s, uint32_t c, bool in_grid, bool active) {
     o.qnew = in_grid ? gld((const float*)s.v[0], c + 7 * (uint32_t)s.sx) : 0.f;
     if (!active) return;
@@ -245,20 +256,30 @@ __device__ inline void load_pre(PreOps<ISO>& o, const DevSim& s, uint32_t c, boo
         o.phi[a][0] = gld((const float*)s.phi[a][0], c);
         o.phi[a][1] = gld((const float*)s.phi[a][1], c);
     }
+    if (CODED) {
+        const CodedCoef k = cell_coef<false, UNIFORM>(s, c);
+        o.a[0][0] = k.a1;
+        o.a[0][1] = k.a2;
+        o.b[0][0] = k.b1;
+        o.b[0][1] = k.b2;
+        o.ik[0] = k.ik;
+        o.dc = UNIFORM ? 0 : k.dc;
+    } else {
+        o.dc = UNIFORM ? 0 : gld(s.dcmap, c);
 #pragma unroll
-    for (int a = 0; a < PreOps<ISO>::NA; ++a) {
-        o.a[a][0] = gld(s.ap[a][0], c);
-        o.a[a][1] = gld(s.ap[a][1], c);
-        o.b[a][0] = gld(s.bp[a][0], c);
-        o.b[a][1] = gld(s.bp[a][1], c);
-        o.ik[a] = gld(s.ikp[a], c);
+        for (int a = 0; a < PreOps<ISO>::NA; ++a) {
+            o.a[a][0] = gld(s.ap[a][0], c);
+            o.a[a][1] = gld(s.ap[a][1], c);
+            o.b[a][0] = gld(s.bp[a][0], c);
+            o.b[a][1] = gld(s.bp[a][1], c);
+            o.ik[a] = gld(s.ikp[a], c);
+        }
     }
     o.K = gld(s.K, c);
     o.p = gld((const float*)s.p, c);
-    o.dc = UNIFORM ? 0 : gld(s.dcmap, c);
 }
 
-template <int TY, bool UNIFORM, bool ISO, bool AHEAD>
+template <int TY, bool UNIFORM, bool ISO, bool AHEAD, bool CODED>
 __global__ __launch_bounds__(PZ3 * TY) void k_pressure_pipe3d(DevSim batch, int chunk) {
     using TX_ = PTile<TY, 1, 1>;
     using TY_ = PTile<TY, kHalo, 1>;
@@ -302,19 +323,19 @@ __global__ __launch_bounds__(PZ3 * TY) void k_pressure_pipe3d(DevSim batch, int 
     sy_.load(s.v[1], i0 + 1, s);
     sz_.load(s.v[2], i0 + 1, s);
     PreOps<ISO> cur, nxt;
-    if (AHEAD) load_pre<UNIFORM, ISO>(cur, s, (uint32_t)i0 * sx + col, in_grid, active);
+    if (AHEAD) load_pre<UNIFORM, ISO, CODED>(cur, s, (uint32_t)i0 * sx + col, in_grid, active);
 
     for (int64_t i = i0; i < i1; ++i) {
         sx_.store(VX, i);
         sy_.store(VY, i + 1);
         sz_.store(VZ, i + 1);
         const bool more = i + 1 < i1;
-        if (!AHEAD) load_pre<UNIFORM, ISO>(cur, s, (uint32_t)i * sx + col, in_grid, active);
+        if (!AHEAD) load_pre<UNIFORM, ISO, CODED>(cur, s, (uint32_t)i * sx + col, in_grid, active);
         if (more) {
             sx_.load(s.v[0], i + 1, s);
             sy_.load(s.v[1], i + 2, s);
             sz_.load(s.v[2], i + 2, s);
-            if (AHEAD) load_pre<UNIFORM, ISO>(nxt, s, (uint32_t)(i + 1) * sx + col, in_grid, active);
+            if (AHEAD) load_pre<UNIFORM, ISO, CODED>(nxt, s, (uint32_t)(i + 1) * sx + col, in_grid, active);
         }
 #pragma unroll
         for (int m = 0; m < 15; ++m) q[m] = q[m + 1];
@@ -388,16 +409,21 @@ void launch_pipe(bool velocity, const DevSim& s, int chunk, hipStream_t st) {
     const dim3 g((unsigned)((s.nz + PZ3 - 1) / PZ3), (unsigned)((s.ny + TY - 1) / TY),
                  (unsigned)((s.nx - 2 * kHalo + chunk - 1) / chunk * s.nshots));
     const dim3 b(PZ3 * TY);
+    const bool coded = s.isotropic && s.rcode;
     if (velocity) {
-        if (u && s.isotropic) hipLaunchKernelGGL((k_velocity_pipe3d<TY, true, true, AHEAD>), g, b, lds, st, s, chunk);
-        else if (u) hipLaunchKernelGGL((k_velocity_pipe3d<TY, true, false, AHEAD>), g, b, lds, st, s, chunk);
-        else if (s.isotropic) hipLaunchKernelGGL((k_velocity_pipe3d<TY, false, true, AHEAD>), g, b, lds, st, s, chunk);
-        else hipLaunchKernelGGL((k_velocity_pipe3d<TY, false, false, AHEAD>), g, b, lds, st, s, chunk);
+        if (u && coded) hipLaunchKernelGGL((k_velocity_pipe3d<TY, true, true, AHEAD, true>), g, b, lds, st, s, chunk);
+        else if (u && s.isotropic) hipLaunchKernelGGL((k_velocity_pipe3d<TY, true, true, AHEAD, false>), g, b, lds, st, s, chunk);
+        else if (u) hipLaunchKernelGGL((k_velocity_pipe3d<TY, true, false, AHEAD, false>), g, b, lds, st, s, chunk);
+        else if (coded) hipLaunchKernelGGL((k_velocity_pipe3d<TY, false, true, AHEAD, true>), g, b, lds, st, s, chunk);
+        else if (s.isotropic) hipLaunchKernelGGL((k_velocity_pipe3d<TY, false, true, AHEAD, false>), g, b, lds, st, s, chunk);
+        else hipLaunchKernelGGL((k_velocity_pipe3d<TY, false, false, AHEAD, false>), g, b, lds, st, s, chunk);
     } else {
-        if (u && s.isotropic) hipLaunchKernelGGL((k_pressure_pipe3d<TY, true, true, AHEAD>), g, b, lds, st, s, chunk);
-        else if (u) hipLaunchKernelGGL((k_pressure_pipe3d<TY, true, false, AHEAD>), g, b, lds, st, s, chunk);
-        else if (s.isotropic) hipLaunchKernelGGL((k_pressure_pipe3d<TY, false, true, AHEAD>), g, b, lds, st, s, chunk);
-        else hipLaunchKernelGGL((k_pressure_pipe3d<TY, false, false, AHEAD>), g, b, lds, st, s, chunk);
+        if (u && coded) hipLaunchKernelGGL((k_pressure_pipe3d<TY, true, true, AHEAD, true>), g, b, lds, st, s, chunk);
+        else if (u && s.isotropic) hipLaunchKernelGGL((k_pressure_pipe3d<TY, true, true, AHEAD, false>), g, b, lds, st, s, chunk);
+        else if (u) hipLaunchKernelGGL((k_pressure_pipe3d<TY, true, false, AHEAD, false>), g, b, lds, st, s, chunk);
+        else if (coded) hipLaunchKernelGGL((k_pressure_pipe3d<TY, false, true, AHEAD, true>), g, b, lds, st, s, chunk);
+        else if (s.isotropic) hipLaunchKernelGGL((k_pressure_pipe3d<TY, false, true, AHEAD, false>), g, b, lds, st, s, chunk);
+        else hipLaunchKernelGGL((k_pressure_pipe3d<TY, false, false, AHEAD, false>), g, b, lds, st, s, chunk);
     }
 }
 

@@ -47,7 +47,8 @@ struct VelRow {
     float ps1[2], ps2[2], v[2];
 };
 
-template <int TY, bool UNIFORM, bool ISO>
+// CODED (isotropic only): the relaxation coefficients come from the code table (rcode, rtab_v)
+template <int TY, bool UNIFORM, bool ISO, bool CODED>
 __global__ __launch_bounds__(TY) void k_velocity_2d(DevSim batch, int chunk) {
     const DevSim s = shot_view(batch, blockIdx.z);
     extern __shared__ float lds_coef[];
@@ -76,14 +77,24 @@ __global__ __launch_bounds__(TY) void k_velocity_2d(DevSim batch, int chunk) {
         R.halo = halo_ok ? gld32((const float*)p, (uint32_t)((r + 1) * ny + hj)) : 0.f;
         if (active) {
             R.rho_right = gld32((const float*)s.rho, c + (uint32_t)(1));
-            R.dc = UNIFORM ? 0 : gld32((const int32_t*)s.dcmap, c);
+            if (CODED) {
+                const CodedCoef k = cell_coef<true, UNIFORM>(s, c);
+                R.ik[0] = k.ik;
+                R.a1[0] = k.a1;
+                R.a2[0] = k.a2;
+                R.b1[0] = k.b1;
+                R.b2[0] = k.b2;
+                R.dc = UNIFORM ? 0 : k.dc;
+            } else {
+                R.dc = UNIFORM ? 0 : gld32((const int32_t*)s.dcmap, c);
 #pragma unroll
-            for (int a = 0; a < VelRow<ISO>::NA; ++a) {
-                R.ik[a] = gld32((const float*)s.ikv[a], c);
-                R.b1[a] = gld32((const float*)s.bv[a][0], c);
-                R.b2[a] = gld32((const float*)s.bv[a][1], c);
-                R.a1[a] = gld32((const float*)s.av[a][0], c);
-                R.a2[a] = gld32((const float*)s.av[a][1], c);
+                for (int a = 0; a < VelRow<ISO>::NA; ++a) {
+                    R.ik[a] = gld32((const float*)s.ikv[a], c);
+                    R.b1[a] = gld32((const float*)s.bv[a][0], c);
+                    R.b2[a] = gld32((const float*)s.bv[a][1], c);
+                    R.a1[a] = gld32((const float*)s.av[a][0], c);
+                    R.a2[a] = gld32((const float*)s.av[a][1], c);
+                }
             }
 #pragma unroll
             for (int a = 0; a < 2; ++a) {
@@ -183,7 +194,7 @@ struct PreRow {
     float ph1[2], ph2[2];
 };
 
-template <int TY, bool UNIFORM, bool ISO>
+template <int TY, bool UNIFORM, bool ISO, bool CODED>
 __global__ __launch_bounds__(TY) void k_pressure_2d(DevSim batch, int chunk) {
     const DevSim s = shot_view(batch, blockIdx.z);
     extern __shared__ float lds_coef[];
@@ -220,14 +231,24 @@ __global__ __launch_bounds__(TY) void k_pressure_2d(DevSim batch, int chunk) {
         if (active) {
             R.K = gld32((const float*)s.K, c);
             R.p = gld32((const float*)s.p, c);
-            R.dc = UNIFORM ? 0 : gld32((const int32_t*)s.dcmap, c);
+            if (CODED) {
+                const CodedCoef k = cell_coef<false, UNIFORM>(s, c);
+                R.ik[0] = k.ik;
+                R.a1[0] = k.a1;
+                R.a2[0] = k.a2;
+                R.b1[0] = k.b1;
+                R.b2[0] = k.b2;
+                R.dc = UNIFORM ? 0 : k.dc;
+            } else {
+                R.dc = UNIFORM ? 0 : gld32((const int32_t*)s.dcmap, c);
 #pragma unroll
-            for (int a = 0; a < PreRow<ISO>::NA; ++a) {
-                R.ik[a] = gld32((const float*)s.ikp[a], c);
-                R.b1[a] = gld32((const float*)s.bp[a][0], c);
-                R.b2[a] = gld32((const float*)s.bp[a][1], c);
-                R.a1[a] = gld32((const float*)s.ap[a][0], c);
-                R.a2[a] = gld32((const float*)s.ap[a][1], c);
+                for (int a = 0; a < PreRow<ISO>::NA; ++a) {
+                    R.ik[a] = gld32((const float*)s.ikp[a], c);
+                    R.b1[a] = gld32((const float*)s.bp[a][0], c);
+                    R.b2[a] = gld32((const float*)s.bp[a][1], c);
+                    R.a1[a] = gld32((const float*)s.ap[a][0], c);
+                    R.a2[a] = gld32((const float*)s.ap[a][1], c);
+                }
             }
 #pragma unroll
             for (int a = 0; a < 2; ++a) {
@@ -325,16 +346,21 @@ hipError_t launch_2d(const DevSim& s, int chunk, bool velocity, hipStream_t st) 
     const int64_t rows = s.nx - 2 * kHalo;
     const dim3 g((unsigned)((s.ny + TY - 1) / TY), (unsigned)((rows + chunk - 1) / chunk), (unsigned)s.nshots);
     const dim3 b(TY);
+    const bool coded = s.isotropic && s.rcode;
     if (velocity) {
-        if (u && s.isotropic) hipLaunchKernelGGL((k_velocity_2d<TY, true, true>), g, b, lds, st, s, chunk);
-        else if (u) hipLaunchKernelGGL((k_velocity_2d<TY, true, false>), g, b, lds, st, s, chunk);
-        else if (s.isotropic) hipLaunchKernelGGL((k_velocity_2d<TY, false, true>), g, b, lds, st, s, chunk);
-        else hipLaunchKernelGGL((k_velocity_2d<TY, false, false>), g, b, lds, st, s, chunk);
+        if (u && coded) hipLaunchKernelGGL((k_velocity_2d<TY, true, true, true>), g, b, lds, st, s, chunk);
+        else if (u && s.isotropic) hipLaunchKernelGGL((k_velocity_2d<TY, true, true, false>), g, b, lds, st, s, chunk);
+        else if (u) hipLaunchKernelGGL((k_velocity_2d<TY, true, false, false>), g, b, lds, st, s, chunk);
+        else if (coded) hipLaunchKernelGGL((k_velocity_2d<TY, false, true, true>), g, b, lds, st, s, chunk);
+        else if (s.isotropic) hipLaunchKernelGGL((k_velocity_2d<TY, false, true, false>), g, b, lds, st, s, chunk);
+        else hipLaunchKernelGGL((k_velocity_2d<TY, false, false, false>), g, b, lds, st, s, chunk);
     } else {
-        if (u && s.isotropic) hipLaunchKernelGGL((k_pressure_2d<TY, true, true>), g, b, lds, st, s, chunk);
-        else if (u) hipLaunchKernelGGL((k_pressure_2d<TY, true, false>), g, b, lds, st, s, chunk);
-        else if (s.isotropic) hipLaunchKernelGGL((k_pressure_2d<TY, false, true>), g, b, lds, st, s, chunk);
-        else hipLaunchKernelGGL((k_pressure_2d<TY, false, false>), g, b, lds, st, s, chunk);
+        if (u && coded) hipLaunchKernelGGL((k_pressure_2d<TY, true, true, true>), g, b, lds, st, s, chunk);
+        else if (u && s.isotropic) hipLaunchKernelGGL((k_pressure_2d<TY, true, true, false>), g, b, lds, st, s, chunk);
+        else if (u) hipLaunchKernelGGL((k_pressure_2d<TY, true, false, false>), g, b, lds, st, s, chunk);
+        else if (coded) hipLaunchKernelGGL((k_pressure_2d<TY, false, true, true>), g, b, lds, st, s, chunk);
+        else if (s.isotropic) hipLaunchKernelGGL((k_pressure_2d<TY, false, true, false>), g, b, lds, st, s, chunk);
+        else hipLaunchKernelGGL((k_pressure_2d<TY, false, false, false>), g, b, lds, st, s, chunk);
     }
     return hipGetLastError();
 }

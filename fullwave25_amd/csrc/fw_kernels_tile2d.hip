@@ -36,8 +36,51 @@ __device__ inline const char* at_(const void* base, uint32_t i) { return reinter
 __device__ inline F4 ld4(const float* base, uint32_t i) { return *reinterpret_cast<const F4*>(at_(base, i)); }
 __device__ inline I4 ldi4(const int32_t* base, uint32_t i) { return *reinterpret_cast<const I4*>(at_(base, i)); }
 __device__ inline float ld1(const float* base, uint32_t i) { return *reinterpret_cast<const float*>(at_(base, i)); }
+// four 16-bit codes at element index i (i % 4 == 0: 8-byte aligned)
+__device__ inline uint2 ldc4(const uint16_t* base, uint32_t i) {
+    return *reinterpret_cast<const uint2*>(reinterpret_cast<const char*>(base) + (i << 1));
+}
+
 __device__ inline void st4(float* base, uint32_t i, const F4& v) {
     *reinterpret_cast<F4*>(reinterpret_cast<char*>(base) + (i << 2)) = v;
+}
+
+// the relaxation coefficients and coefficient-set indices of a quad: from the code table
+// (CODED; a cell without a code reads the per-cell maps) or from the per-cell maps
+template <bool CODED, bool UNIFORM, bool VEL, int NA>
+__device__ inline void quad_coef(const DevSim& s, uint32_t c, F4* ik, F4* b1, F4* b2, F4* a1, F4* a2, I4& dc) {
+    const float* const* ik_m = VEL ? s.ikv : s.ikp;
+    const float* const (*a_m)[kNRelax] = VEL ? s.av : s.ap;
+    const float* const (*b_m)[kNRelax] = VEL ? s.bv : s.bp;
+    if (CODED) {
+        const uint2 cw = ldc4(s.rcode, c);
+        const uint32_t cs[4] = {cw.x & 0xffffu, cw.x >> 16, cw.y & 0xffffu, cw.y >> 16};
+        int d[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const CodedCoef k = cs[e] != kCodeEscape
+                                    ? coded_coef(VEL ? s.rtab_v : s.rtab_p, cs[e])
+                                    : mapped_coef<UNIFORM>(ik_m[0], a_m[0][0], a_m[0][1], b_m[0][0], b_m[0][1],
+                                                           s.dcmap, c + e);
+            ik[0][e] = k.ik;
+            a1[0][e] = k.a1;
+            a2[0][e] = k.a2;
+            b1[0][e] = k.b1;
+            b2[0][e] = k.b2;
+            d[e] = UNIFORM ? 0 : k.dc;
+        }
+        dc = I4{d[0], d[1], d[2], d[3]};
+        return;
+    }
+    dc = UNIFORM ? I4{0, 0, 0, 0} : ldi4(s.dcmap, c);
+#pragma unroll
+    for (int a = 0; a < NA; ++a) {
+        ik[a] = ld4(ik_m[a], c);
+        b1[a] = ld4(b_m[a][0], c);
+        b2[a] = ld4(b_m[a][1], c);
+        a1[a] = ld4(a_m[a][0], c);
+        a2[a] = ld4(a_m[a][1], c);
+    }
 }
 
 // Stage rows [r0, r0 + nrows) x columns [j0 - hl, j0 + kTW + hr) of f into lds (row pitch
@@ -64,7 +107,7 @@ __device__ inline const float* cset(const float* lds_coef, int dc, int axis) {
 }
 
 // velocity: p tile rows i0-7 .. i0+TX+7 (TX + 15 rows), columns j0-8 .. j0+TW+8
-template <int TX, bool UNIFORM, bool ISO>
+template <int TX, bool UNIFORM, bool ISO, bool CODED>
 __global__ __launch_bounds__(kLanes) void k_velocity_tile(DevSim batch) {
     const DevSim s = shot_view(batch, blockIdx.z);
     constexpr int PITCH = kTW + 16;
@@ -93,17 +136,10 @@ __global__ __launch_bounds__(kLanes) void k_velocity_tile(DevSim batch) {
         const F4 rho0 = ld4(s.rho, c);
         const F4 rho1 = ld4(s.rho, c + (uint32_t)ny);
         const float rho_r = (jq + 4 < ny) ? ld1(s.rho, c + 4) : 0.f;
-        const I4 dc = UNIFORM ? I4{0, 0, 0, 0} : ldi4(s.dcmap, c);
+        I4 dc;
         F4 ik[2], b1[2], b2[2], a1[2], a2[2];
         constexpr int NA = ISO ? 1 : 2;
-#pragma unroll
-        for (int a = 0; a < NA; ++a) {
-            ik[a] = ld4(s.ikv[a], c);
-            b1[a] = ld4(s.bv[a][0], c);
-            b2[a] = ld4(s.bv[a][1], c);
-            a1[a] = ld4(s.av[a][0], c);
-            a2[a] = ld4(s.av[a][1], c);
-        }
+        quad_coef<CODED, UNIFORM, true, NA>(s, c, ik, b1, b2, a1, a2, dc);
         F4 ps1[2], ps2[2], v[2];
 #pragma unroll
         for (int a = 0; a < 2; ++a) {
@@ -163,7 +199,7 @@ __global__ __launch_bounds__(kLanes) void k_velocity_tile(DevSim batch) {
 
 // pressure: vx tile rows i0-8 .. i0+TX+6 (TX + 15 rows), columns j0-4 .. j0+TW+4;
 //           vy tile rows i0-1 .. i0+TX (TX + 2 rows), columns j0-8 .. j0+TW+8
-template <int TX, bool UNIFORM, bool ISO>
+template <int TX, bool UNIFORM, bool ISO, bool CODED>
 __global__ __launch_bounds__(kLanes) void k_pressure_tile(DevSim batch) {
     const DevSim s = shot_view(batch, blockIdx.z);
     constexpr int XP = kTW + 8;
@@ -195,17 +231,10 @@ __global__ __launch_bounds__(kLanes) void k_pressure_tile(DevSim batch) {
         const uint32_t c = (uint32_t)(i * ny + jq);
         F4 pv = ld4(s.p, c);
         const F4 K = ld4(s.K, c);
-        const I4 dc = UNIFORM ? I4{0, 0, 0, 0} : ldi4(s.dcmap, c);
+        I4 dc;
         constexpr int NA = ISO ? 1 : 2;
         F4 ik[2], b1[2], b2[2], a1[2], a2[2];
-#pragma unroll
-        for (int a = 0; a < NA; ++a) {
-            ik[a] = ld4(s.ikp[a], c);
-            b1[a] = ld4(s.bp[a][0], c);
-            b2[a] = ld4(s.bp[a][1], c);
-            a1[a] = ld4(s.ap[a][0], c);
-            a2[a] = ld4(s.ap[a][1], c);
-        }
+        quad_coef<CODED, UNIFORM, false, NA>(s, c, ik, b1, b2, a1, a2, dc);
         F4 ph1[2], ph2[2];
 #pragma unroll
         for (int a = 0; a < 2; ++a) {
@@ -268,16 +297,21 @@ hipError_t launch_tile(const DevSim& s, bool velocity, hipStream_t st) {
     const int64_t rows = s.nx - 2 * kHalo;
     const dim3 g((unsigned)((s.ny + kTW - 1) / kTW), (unsigned)((rows + TX - 1) / TX), (unsigned)s.nshots);
     const dim3 b(kLanes);
+    const bool coded = s.isotropic && s.rcode;
     if (velocity) {
-        if (u && s.isotropic) hipLaunchKernelGGL((k_velocity_tile<TX, true, true>), g, b, lds, st, s);
-        else if (u) hipLaunchKernelGGL((k_velocity_tile<TX, true, false>), g, b, lds, st, s);
-        else if (s.isotropic) hipLaunchKernelGGL((k_velocity_tile<TX, false, true>), g, b, lds, st, s);
-        else hipLaunchKernelGGL((k_velocity_tile<TX, false, false>), g, b, lds, st, s);
+        if (u && coded) hipLaunchKernelGGL((k_velocity_tile<TX, true, true, true>), g, b, lds, st, s);
+        else if (u && s.isotropic) hipLaunchKernelGGL((k_velocity_tile<TX, true, true, false>), g, b, lds, st, s);
+        else if (u) hipLaunchKernelGGL((k_velocity_tile<TX, true, false, false>), g, b, lds, st, s);
+        else if (coded) hipLaunchKernelGGL((k_velocity_tile<TX, false, true, true>), g, b, lds, st, s);
+        else if (s.isotropic) hipLaunchKernelGGL((k_velocity_tile<TX, false, true, false>), g, b, lds, st, s);
+        else hipLaunchKernelGGL((k_velocity_tile<TX, false, false, false>), g, b, lds, st, s);
     } else {
-        if (u && s.isotropic) hipLaunchKernelGGL((k_pressure_tile<TX, true, true>), g, b, lds, st, s);
-        else if (u) hipLaunchKernelGGL((k_pressure_tile<TX, true, false>), g, b, lds, st, s);
-        else if (s.isotropic) hipLaunchKernelGGL((k_pressure_tile<TX, false, true>), g, b, lds, st, s);
-        else hipLaunchKernelGGL((k_pressure_tile<TX, false, false>), g, b, lds, st, s);
+        if (u && coded) hipLaunchKernelGGL((k_pressure_tile<TX, true, true, true>), g, b, lds, st, s);
+        else if (u && s.isotropic) hipLaunchKernelGGL((k_pressure_tile<TX, true, true, false>), g, b, lds, st, s);
+        else if (u) hipLaunchKernelGGL((k_pressure_tile<TX, true, false, false>), g, b, lds, st, s);
+        else if (coded) hipLaunchKernelGGL((k_pressure_tile<TX, false, true, true>), g, b, lds, st, s);
+        else if (s.isotropic) hipLaunchKernelGGL((k_pressure_tile<TX, false, true, false>), g, b, lds, st, s);
+        else hipLaunchKernelGGL((k_pressure_tile<TX, false, false, false>), g, b, lds, st, s);
     }
     return hipGetLastError();
 }

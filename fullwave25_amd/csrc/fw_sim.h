@@ -44,6 +44,12 @@ struct fw_sim {
     int64_t graph_steps = 0;
     int64_t* step_dev = nullptr;
     hipGraphExec_t graph = nullptr;
+    // coded relaxation coefficients: -1 = as FULLWAVE_HIP_CODES says (default on), 0 off, 1 on;
+    // ncodes = codes in use (0: the marching kernels read the per-cell maps), covering
+    // coded_cells of the ncell device cells (the others read the per-cell maps)
+    int codes_mode = -1;
+    int32_t ncodes = 0;
+    int64_t coded_cells = 0;
 };
 
 

@@ -19,7 +19,7 @@ LIB_NAME = "libfullwave_hip.so"
 LIB_PATH = Path(__file__).resolve().parent / LIB_NAME
 
 FW_OK = 0
-ABI_VERSION = 3
+ABI_VERSION = 4
 STATUS_NAMES = {
     0: "FW_OK", 1: "FW_ERR_INVALID", 2: "FW_ERR_DEVICE", 3: "FW_ERR_NOMEM",
     4: "FW_ERR_IO", 5: "FW_ERR_STATE", 6: "FW_ERR_UNSUPPORTED",
@@ -33,7 +33,7 @@ EXPORTED_SYMBOLS = (
     "fw_sim_create", "fw_sim_upload", "fw_sim_run", "fw_sim_reset", "fw_sim_step_count",
     "fw_sim_record_count", "fw_sim_download_sensors", "fw_sim_download_field",
     "fw_sim_set_field", "fw_sim_kernel_config", "fw_sim_set_kernel_config", "fw_sim_set_graph_steps",
-    "fw_sim_run_timed",
+    "fw_sim_set_relaxation_codes", "fw_sim_relaxation_codes", "fw_sim_run_timed",
     "fw_sim_destroy", "fw_adj_create", "fw_adj_forward", "fw_adj_backward", "fw_adj_destroy",
 )
 VARIANTS = {"percell": 0, "stream": 1, "tile": 2, "pipe": 3, "pipelite": 4}
@@ -95,6 +95,8 @@ def load_library(path: Path | None = None) -> ctypes.CDLL:
         lib.fw_sim_kernel_config.argtypes = [vp, i32, ctypes.POINTER(i32), ctypes.POINTER(i32), ctypes.POINTER(i32)]
         lib.fw_sim_set_kernel_config.argtypes = [vp, i32, i32, i32, i32]
         lib.fw_sim_set_graph_steps.argtypes = [vp, i64]
+        lib.fw_sim_set_relaxation_codes.argtypes = [vp, i32]
+        lib.fw_sim_relaxation_codes.argtypes = [vp, ctypes.POINTER(i32), ctypes.POINTER(ctypes.c_double)]
         lib.fw_sim_run_timed.argtypes = [vp, i64, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(i32)]
         lib.fw_sim_destroy.argtypes = [vp]
         lib.fw_adj_create.argtypes = [vp, i64, ctypes.POINTER(vp)]
@@ -263,6 +265,22 @@ class HipSimulation:
     def set_graph_steps(self, steps: int) -> None:
         """Replay steady steps from a captured HIP graph of ``steps`` steps (0 turns it off)."""
         _check(self.lib, self.lib.fw_sim_set_graph_steps(self._h, int(steps)))
+
+    def set_relaxation_codes(self, enable: bool) -> None:
+        """Let the marching kernels read coded relaxation coefficients (the default) or the
+        ten per-cell maps; results are the same either way."""
+        _check(self.lib, self.lib.fw_sim_set_relaxation_codes(self._h, 1 if enable else 0))
+
+    def relaxation_codes(self) -> int:
+        """Relaxation tuples coded for this medium (0: per-cell maps in use)."""
+        return self.relaxation_coding()[0]
+
+    def relaxation_coding(self) -> tuple[int, float]:
+        """(codes in use, fraction of device cells carrying a code)."""
+        n = ctypes.c_int32(0)
+        f = ctypes.c_double(0.0)
+        _check(self.lib, self.lib.fw_sim_relaxation_codes(self._h, ctypes.byref(n), ctypes.byref(f)))
+        return n.value, f.value
 
     def kernel_override(self) -> str:
         """The kernel configuration as a ``FULLWAVE_HIP_KERNELS`` value that pins it."""

@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define FW_ABI_VERSION 3
+#define FW_ABI_VERSION 4
 
 typedef enum fw_status {
     FW_OK = 0,
@@ -143,6 +143,22 @@ int fw_sim_set_kernel_config(fw_sim* sim, int32_t kind, int32_t variant, int32_t
  * identical. The step number the source/sensor pass uses is then kept in device memory. Cuts
  * the per-launch host cost, which dominates for small grids. fw_sim_run_timed never uses it. */
 int fw_sim_set_graph_steps(fw_sim* sim, int64_t steps);
+/* Coded relaxation coefficients (isotropic relaxation model). No counterpart in the reference,
+ * which reads its ten coefficient maps (kappau/x, apml/bpml u/x 1-2) and dcmap for every cell
+ * and step. On the first run the handle counts the distinct tuples of those eleven maps on the
+ * device and gives the (at most 65535) most frequent a 16-bit code; the x-marching and tiled
+ * kernels then read a code per cell and that code's row of a small table instead of the maps
+ * (2 bytes per cell and kernel instead of 24). Cells of the remaining tuples (PML edges and
+ * corners of 3D grids) read the maps as before. The values read are the maps' own, bit for
+ * bit, so results do not change. Codes are not used when they would cover less than half of
+ * the cells. fw_sim_set_relaxation_codes(sim, 0) keeps the per-cell maps (as does
+ * FULLWAVE_HIP_CODES=0 in the environment); called after the first run it re-codes at once.
+ * Uploading a relaxation map or dcmap after the first run re-codes too.
+ * fw_sim_relaxation_codes returns the number of codes in use (0: per-cell maps) and, if
+ * coded_fraction is not NULL, the fraction of device cells that carry a code (3D device rows
+ * are padded to 32 cells); it runs the first-run set-up if it has not happened yet. */
+int fw_sim_set_relaxation_codes(fw_sim* sim, int32_t enable);
+int fw_sim_relaxation_codes(fw_sim* sim, int32_t* n_codes, double* coded_fraction);
 /* Run n_steps with a HIP event pair around every launch on the handle's stream and add the
  * summed milliseconds per kernel kind to ms[0..n_kinds) (kinds: 0 velocity, 1 pressure,
  * 2 source/air/sensor); also returns the number of kinds in *n_kinds. */
