@@ -364,8 +364,6 @@ def main() -> int:
     ap.add_argument("--arrival-factor", type=float, default=2.0,
                     help="served_mix: arrival rate as a multiple of the sequential service rate")
     args = ap.parse_args()
-    if args.kernels:
-        os.environ["FULLWAVE_HIP_KERNELS"] = args.kernels
     if args.config in ("batched_2d", "served_mix", "hetero_3d"):
         if args.gpus != 1:
             print(json.dumps({"error": "this build measures one GPU; run with --gpus 1"}))
@@ -377,6 +375,8 @@ def main() -> int:
         dirs = [Path(p) for p in args.pmc_summary]
         print(json.dumps(pmc_summary(*dirs[:4], args.copy_bytes, dirs[4]), indent=1))
         return 0
+    if args.kernels:
+        os.environ["FULLWAVE_HIP_KERNELS"] = args.kernels
     if args.copy_probe:
         from fullwave25_amd import engine
 
@@ -398,8 +398,8 @@ def main() -> int:
         shapes = ["percell"] + [f"stream:{w}:{c}" for w in (128, 256) for c in (8, 16, 32)]
         shapes += [f"tile:{c}" for c in (8, 16, 32)]
         if inp.ndim == 3:
-            shapes = ["percell"] + [f"{v}:{w}:{c}" for v in ("stream", "pipe", "pipelite", "xpipe", "xpipelite")
-                                    for w in (512, 1024) for c in (8, 16, 32)]
+            shapes = ["percell"] + [f"{v}:{w}:{c}" for v in ("stream", "pipe", "pipelite") for w in (512, 1024)
+                                    for c in (8, 16, 32)]
         updated = int(np.prod([s - 16 for s in inp.shape]))
         for shape in shapes:
             os.environ["FULLWAVE_HIP_KERNELS"] = shape

@@ -188,7 +188,7 @@ int check_cfg(fw_sim* s, const fw::LaunchCfg& c) {
             return fail(FW_ERR_INVALID, "tile kernels: width 256 and rows 8, 16 or 32");
         return FW_OK;
     }
-    if (c.variant >= 3 && c.variant <= 6) {
+    if (c.variant == 3 || c.variant == 4) {
         if (s->ndim != 3 || !fw::streaming_supported(s->ds))
             return fail(FW_ERR_UNSUPPORTED, "pipelined kernels need a 3D relaxation grid");
         if (s->ncell >= ((int64_t)1 << 30))  // 32-bit byte offsets per array
@@ -207,7 +207,7 @@ int check_cfg(fw_sim* s, const fw::LaunchCfg& c) {
         if (c.chunk < 2 || c.chunk > 4096) return fail(FW_ERR_INVALID, "x-marching kernels: chunk out of range");
         return FW_OK;
     }
-    return fail(FW_ERR_INVALID, "kernel variant must be 0 to 6");
+    return fail(FW_ERR_INVALID, "kernel variant must be 0 to 4");
 }
 
 int parse_kernel_env(fw_sim* s, const char* env, fw::LaunchCfg* out) {
@@ -226,13 +226,11 @@ int parse_kernel_env(fw_sim* s, const char* env, fw::LaunchCfg* out) {
         c.chunk = sscanf(env, "tile:%d", &ch) == 1 ? ch : 16;
         if (c.chunk != 8 && c.chunk != 16 && c.chunk != 32)
             return fail(FW_ERR_INVALID, "FULLWAVE_HIP_KERNELS: tile rows must be 8, 16 or 32");
-    } else if (!strncmp(env, "pipe", 4) || !strncmp(env, "xpipe", 5)) {
-        const bool xcd = env[0] == 'x';
-        const char* e = env + (xcd ? 1 : 0);
-        const bool lite = !strncmp(e, "pipelite", 8);
-        c.variant = (lite ? 4 : 3) + (xcd ? 2 : 0);
+    } else if (!strncmp(env, "pipe", 4) || !strncmp(env, "pipelite", 8)) {
+        const bool lite = !strncmp(env, "pipelite", 8);
+        c.variant = lite ? 4 : 3;
         int w = 0, ch = 0;
-        const int n = sscanf(e + (lite ? 8 : 4), ":%d:%d", &w, &ch);
+        const int n = sscanf(env + (lite ? 8 : 4), ":%d:%d", &w, &ch);
         if (n >= 1) c.width = w;
         if (n >= 2) c.chunk = ch;
         int rc = check_cfg(s, c);
@@ -248,8 +246,7 @@ int parse_kernel_env(fw_sim* s, const char* env, fw::LaunchCfg* out) {
         if (rc != FW_OK) return rc;
     } else {
         return fail(FW_ERR_INVALID,
-                    "FULLWAVE_HIP_KERNELS must be 'percell', 'stream[:width[:chunk]]', 'tile[:rows]', "
-                    "'[x]pipe[:width[:chunk]]' or '[x]pipelite[:width[:chunk]]'");
+                    "FULLWAVE_HIP_KERNELS must be 'percell', 'stream[:width[:chunk]]', 'tile[:rows]' or 'pipe[:width[:chunk]]'");
     }
     *out = c;
     return FW_OK;
@@ -314,7 +311,7 @@ int choose_kernels_unpinned(fw_sim* s) {
                 cands.push_back(c);
             }
         } else {
-            for (int v : {1, 3, 4, 5, 6})
+            for (int v : {1, 3, 4})
                 for (int w : {512, 1024}) {
                     fw::LaunchCfg c;
                     c.variant = v;
@@ -1045,7 +1042,7 @@ int fw_sim_set_kernel_config(fw_sim* s, int32_t kind, int32_t variant, int32_t w
     if (!s) return fail(FW_ERR_INVALID, "null handle");
     if (kind != 0 && kind != 1) return fail(FW_ERR_INVALID, "kind must be 0 (velocity) or 1 (pressure)");
     if (s->ready) return fail(FW_ERR_STATE, "the kernels are fixed at the first run; set them before it");
-    if (variant < 0 || variant > 6) return fail(FW_ERR_INVALID, "kernel variant must be 0 to 6");
+    if (variant < 0 || variant > 4) return fail(FW_ERR_INVALID, "kernel variant must be 0 to 4");
     fw::LaunchCfg c;
     c.variant = variant;
     c.width = width;

@@ -143,32 +143,6 @@ __device__ inline int shot_of_block(const DevSim& s, unsigned& zl) {
 // Largest z extent of a launch grid.
 constexpr unsigned kMaxGridZ = 65535;
 
-// The hardware hands the workgroups of a launch to the 8 XCDs in turn (linear workgroup id
-// modulo 8), and each XCD has its own L2, so tiles that are neighbours in the launch grid run
-// on different XCDs and read each other's stencil halos through their own L2s. With XCD = true
-// the workgroups one XCD receives take a contiguous run of tiles instead (x-fastest order):
-// neighbouring tiles, which read the same halo rows at about the same time, share an L2.
-// Returns the tile coordinates in the launch grid; the mapping is a bijection for any grid.
-constexpr unsigned kNumXcd = 8;
-template <bool XCD>
-__device__ inline uint3 tile_of_block() {
-    if (!XCD) return make_uint3(blockIdx.x, blockIdx.y, blockIdx.z);
-    const unsigned gx = gridDim.x, gy = gridDim.y;
-    const unsigned w = gx * gy * gridDim.z;
-    const unsigned l = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
-    const unsigned k = l % kNumXcd, q = w / kNumXcd, r = w % kNumXcd;
-    const unsigned t = (k < r ? k * (q + 1) : r * (q + 1) + (k - r) * q) + l / kNumXcd;
-    return make_uint3(t % gx, (t / gx) % gy, t / (gx * gy));
-}
-
-// As shot_of_block, for a tile's z coordinate bz.
-__device__ inline int shot_of_tile(const DevSim& s, unsigned bz, unsigned& zl) {
-    const unsigned zper = gridDim.z / (unsigned)s.nshots;
-    const unsigned shot = bz / zper;
-    zl = bz - shot * zper;
-    return (int)shot;
-}
-
 // Source injection, pressure release and sensor gather of one step.
 struct DevBoundary {
     int64_t ncoords, nzero, nout, ntic;
@@ -212,7 +186,6 @@ enum KernelKind { kVelocity = 0, kPressure = 1, kBoundary = 2, kNumKinds = 3 };
 //   variant 3: 3D x-marching kernels with software pipelining (planes and per-cell operands
 //              loaded one iteration ahead); `width` = workgroup size 512 or 1024
 //   variant 4: as 3, with the per-cell operands loaded at the top of their own iteration
-//   variants 5, 6: as 3, 4 with XCD-grouped tiles (tile_of_block)
 struct LaunchCfg {
     int variant = 0;
     int width = 128;

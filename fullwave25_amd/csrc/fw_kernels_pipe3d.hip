@@ -138,16 +138,15 @@ __device__ inline void load_vel(VelOps<ISO>& o, const DevSim& s, uint32_t c, boo
 
 // AHEAD: per-cell operands loaded one iteration ahead (more registers); otherwise at the top
 // of their own iteration, before the barrier
-template <int TY, bool UNIFORM, bool ISO, bool AHEAD, bool CODED, bool XCD>
+template <int TY, bool UNIFORM, bool ISO, bool AHEAD, bool CODED>
 __global__ __launch_bounds__(PZ3 * TY) void k_velocity_pipe3d(DevSim batch, int chunk) {
     using TP = PTile<TY, kHalo, kHalo>;
     unsigned zl;
-    const uint3 tb = tile_of_block<XCD>();
-    const DevSim s = shot_view(batch, shot_of_tile(batch, tb.z, zl));
+    const DevSim s = shot_view(batch, shot_of_block(batch, zl));
     extern __shared__ float lds_coef[];
     __shared__ TP P;
     const int tz = threadIdx.x % PZ3, ty = threadIdx.x / PZ3;
-    const int64_t k0 = (int64_t)tb.x * PZ3, j0 = (int64_t)tb.y * TY;
+    const int64_t k0 = (int64_t)blockIdx.x * PZ3, j0 = (int64_t)blockIdx.y * TY;
     const int64_t k = k0 + tz, j = j0 + ty;
     const int64_t i0 = kHalo + (int64_t)zl * chunk;
     const int64_t i1 = min(i0 + chunk, s.nx - kHalo);
@@ -280,20 +279,19 @@ __device__ inline void load_pre(PreOps<ISO>& o, const DevSim& s, uint32_t c, boo
     o.p = gld((const float*)s.p, c);
 }
 
-template <int TY, bool UNIFORM, bool ISO, bool AHEAD, bool CODED, bool XCD>
+template <int TY, bool UNIFORM, bool ISO, bool AHEAD, bool CODED>
 __global__ __launch_bounds__(PZ3 * TY) void k_pressure_pipe3d(DevSim batch, int chunk) {
     using TX_ = PTile<TY, 1, 1>;
     using TY_ = PTile<TY, kHalo, 1>;
     using TZ_ = PTile<TY, 1, kHalo>;
     unsigned zl;
-    const uint3 tb = tile_of_block<XCD>();
-    const DevSim s = shot_view(batch, shot_of_tile(batch, tb.z, zl));
+    const DevSim s = shot_view(batch, shot_of_block(batch, zl));
     extern __shared__ float lds_coef[];
     __shared__ TX_ VX;  // vx planes i-1, i
     __shared__ TY_ VY;  // vy planes i-1, i, i+1
     __shared__ TZ_ VZ;  // vz planes i-1, i, i+1
     const int tz = threadIdx.x % PZ3, ty = threadIdx.x / PZ3;
-    const int64_t k0 = (int64_t)tb.x * PZ3, j0 = (int64_t)tb.y * TY;
+    const int64_t k0 = (int64_t)blockIdx.x * PZ3, j0 = (int64_t)blockIdx.y * TY;
     const int64_t k = k0 + tz, j = j0 + ty;
     const int64_t i0 = kHalo + (int64_t)zl * chunk;
     const int64_t i1 = min(i0 + chunk, s.nx - kHalo);
@@ -403,7 +401,7 @@ __global__ __launch_bounds__(PZ3 * TY) void k_pressure_pipe3d(DevSim batch, int 
     }
 }
 
-template <int TY, bool AHEAD, bool XCD>
+template <int TY, bool AHEAD>
 void launch_pipe(bool velocity, const DevSim& s, int chunk, hipStream_t st) {
     const bool u = s.ndmap == 1;
     const int nsets = u ? 1 : s.ndmap;
@@ -413,40 +411,33 @@ void launch_pipe(bool velocity, const DevSim& s, int chunk, hipStream_t st) {
     const dim3 b(PZ3 * TY);
     const bool coded = s.isotropic && s.rcode;
     if (velocity) {
-        if (u && coded) hipLaunchKernelGGL((k_velocity_pipe3d<TY, true, true, AHEAD, true, XCD>), g, b, lds, st, s, chunk);
-        else if (u && s.isotropic) hipLaunchKernelGGL((k_velocity_pipe3d<TY, true, true, AHEAD, false, XCD>), g, b, lds, st, s, chunk);
-        else if (u) hipLaunchKernelGGL((k_velocity_pipe3d<TY, true, false, AHEAD, false, XCD>), g, b, lds, st, s, chunk);
-        else if (coded) hipLaunchKernelGGL((k_velocity_pipe3d<TY, false, true, AHEAD, true, XCD>), g, b, lds, st, s, chunk);
-        else if (s.isotropic) hipLaunchKernelGGL((k_velocity_pipe3d<TY, false, true, AHEAD, false, XCD>), g, b, lds, st, s, chunk);
-        else hipLaunchKernelGGL((k_velocity_pipe3d<TY, false, false, AHEAD, false, XCD>), g, b, lds, st, s, chunk);
+        if (u && coded) hipLaunchKernelGGL((k_velocity_pipe3d<TY, true, true, AHEAD, true>), g, b, lds, st, s, chunk);
+        else if (u && s.isotropic) hipLaunchKernelGGL((k_velocity_pipe3d<TY, true, true, AHEAD, false>), g, b, lds, st, s, chunk);
+        else if (u) hipLaunchKernelGGL((k_velocity_pipe3d<TY, true, false, AHEAD, false>), g, b, lds, st, s, chunk);
+        else if (coded) hipLaunchKernelGGL((k_velocity_pipe3d<TY, false, true, AHEAD, true>), g, b, lds, st, s, chunk);
+        else if (s.isotropic) hipLaunchKernelGGL((k_velocity_pipe3d<TY, false, true, AHEAD, false>), g, b, lds, st, s, chunk);
+        else hipLaunchKernelGGL((k_velocity_pipe3d<TY, false, false, AHEAD, false>), g, b, lds, st, s, chunk);
     } else {
-        if (u && coded) hipLaunchKernelGGL((k_pressure_pipe3d<TY, true, true, AHEAD, true, XCD>), g, b, lds, st, s, chunk);
-        else if (u && s.isotropic) hipLaunchKernelGGL((k_pressure_pipe3d<TY, true, true, AHEAD, false, XCD>), g, b, lds, st, s, chunk);
-        else if (u) hipLaunchKernelGGL((k_pressure_pipe3d<TY, true, false, AHEAD, false, XCD>), g, b, lds, st, s, chunk);
-        else if (coded) hipLaunchKernelGGL((k_pressure_pipe3d<TY, false, true, AHEAD, true, XCD>), g, b, lds, st, s, chunk);
-        else if (s.isotropic) hipLaunchKernelGGL((k_pressure_pipe3d<TY, false, true, AHEAD, false, XCD>), g, b, lds, st, s, chunk);
-        else hipLaunchKernelGGL((k_pressure_pipe3d<TY, false, false, AHEAD, false, XCD>), g, b, lds, st, s, chunk);
+        if (u && coded) hipLaunchKernelGGL((k_pressure_pipe3d<TY, true, true, AHEAD, true>), g, b, lds, st, s, chunk);
+        else if (u && s.isotropic) hipLaunchKernelGGL((k_pressure_pipe3d<TY, true, true, AHEAD, false>), g, b, lds, st, s, chunk);
+        else if (u) hipLaunchKernelGGL((k_pressure_pipe3d<TY, true, false, AHEAD, false>), g, b, lds, st, s, chunk);
+        else if (coded) hipLaunchKernelGGL((k_pressure_pipe3d<TY, false, true, AHEAD, true>), g, b, lds, st, s, chunk);
+        else if (s.isotropic) hipLaunchKernelGGL((k_pressure_pipe3d<TY, false, true, AHEAD, false>), g, b, lds, st, s, chunk);
+        else hipLaunchKernelGGL((k_pressure_pipe3d<TY, false, false, AHEAD, false>), g, b, lds, st, s, chunk);
     }
-}
-
-template <bool XCD>
-void launch_pipe_shape(bool velocity, const DevSim& s, int width, int chunk, bool ahead, hipStream_t st) {
-    if (width == 1024 && ahead) launch_pipe<16, true, XCD>(velocity, s, chunk, st);
-    else if (width == 1024) launch_pipe<16, false, XCD>(velocity, s, chunk, st);
-    else if (ahead) launch_pipe<8, true, XCD>(velocity, s, chunk, st);
-    else launch_pipe<8, false, XCD>(velocity, s, chunk, st);
 }
 
 }  // namespace
 
-// cfg.width is the workgroup size: 512 (tiles of 8 rows in y) or 1024 (16 rows); variants 3
-// and 5 load the per-cell operands one iteration ahead, 4 and 6 at the top of their iteration;
-// 5 and 6 group the tiles by XCD
+// cfg.width is the workgroup size: 512 (tiles of 8 rows in y) or 1024 (16 rows); variant 3
+// loads the per-cell operands one iteration ahead, variant 4 at the top of their iteration
 hipError_t launch_pipe3d(const DevSim& s, const LaunchCfg& cfg, bool velocity, hipStream_t st) {
     const int chunk = cfg.chunk < 1 ? 1 : cfg.chunk;
-    const bool ahead = cfg.variant == 3 || cfg.variant == 5;
-    if (cfg.variant >= 5) launch_pipe_shape<true>(velocity, s, cfg.width, chunk, ahead, st);
-    else launch_pipe_shape<false>(velocity, s, cfg.width, chunk, ahead, st);
+    const bool ahead = cfg.variant == 3;
+    if (cfg.width == 1024 && ahead) launch_pipe<16, true>(velocity, s, chunk, st);
+    else if (cfg.width == 1024) launch_pipe<16, false>(velocity, s, chunk, st);
+    else if (ahead) launch_pipe<8, true>(velocity, s, chunk, st);
+    else launch_pipe<8, false>(velocity, s, chunk, st);
     return hipGetLastError();
 }
 

@@ -392,7 +392,7 @@ hipError_t launch_tile2d(const DevSim& s, const LaunchCfg& cfg, bool velocity, h
 
 hipError_t launch_velocity(const DevSim& s, const LaunchCfg& cfg, hipStream_t st) {
     if (cfg.variant == 2 && tile2d_supported(s)) return launch_tile2d(s, cfg, true, st);
-    if (cfg.variant >= 3 && cfg.variant <= 6 && s.ndim == 3 && streaming_supported(s))
+    if ((cfg.variant == 3 || cfg.variant == 4) && s.ndim == 3 && streaming_supported(s))
         return launch_pipe3d(s, cfg, true, st);
     if (cfg.variant == 0 || !streaming_supported(s)) return launch_velocity_simple(s, st);
     if (s.ndim == 3) return launch_velocity_3d(s, cfg, st);
@@ -401,7 +401,7 @@ hipError_t launch_velocity(const DevSim& s, const LaunchCfg& cfg, hipStream_t st
 
 hipError_t launch_pressure(const DevSim& s, const LaunchCfg& cfg, hipStream_t st) {
     if (cfg.variant == 2 && tile2d_supported(s)) return launch_tile2d(s, cfg, false, st);
-    if (cfg.variant >= 3 && cfg.variant <= 6 && s.ndim == 3 && streaming_supported(s))
+    if ((cfg.variant == 3 || cfg.variant == 4) && s.ndim == 3 && streaming_supported(s))
         return launch_pipe3d(s, cfg, false, st);
     if (cfg.variant == 0 || !streaming_supported(s)) return launch_pressure_simple(s, st);
     if (s.ndim == 3) return launch_pressure_3d(s, cfg, st);

@@ -36,7 +36,7 @@ EXPORTED_SYMBOLS = (
     "fw_sim_set_relaxation_codes", "fw_sim_relaxation_codes", "fw_sim_run_timed",
     "fw_sim_destroy", "fw_adj_create", "fw_adj_forward", "fw_adj_backward", "fw_adj_destroy",
 )
-VARIANTS = {"percell": 0, "stream": 1, "tile": 2, "pipe": 3, "pipelite": 4, "xpipe": 5, "xpipelite": 6}
+VARIANTS = {"percell": 0, "stream": 1, "tile": 2, "pipe": 3, "pipelite": 4}
 
 
 class EngineError(RuntimeError):
@@ -250,7 +250,7 @@ class HipSimulation:
         for kind, label in ((0, "velocity"), (1, "pressure")):
             v, w, c = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
             _check(self.lib, self.lib.fw_sim_kernel_config(self._h, kind, ctypes.byref(v), ctypes.byref(w), ctypes.byref(c)))
-            name = {v_: k_ for k_, v_ in VARIANTS.items()}.get(v.value, str(v.value))
+            name = {0: "percell", 1: "stream", 2: "tile", 3: "pipe", 4: "pipelite"}.get(v.value, str(v.value))
             out[label] = {"variant": name, "width": w.value, "chunk": c.value}
         return out
 
