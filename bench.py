@@ -364,6 +364,8 @@ def main() -> int:
     ap.add_argument("--arrival-factor", type=float, default=2.0,
                     help="served_mix: arrival rate as a multiple of the sequential service rate")
     args = ap.parse_args()
+    if args.kernels:
+        os.environ["FULLWAVE_HIP_KERNELS"] = args.kernels
     if args.config in ("batched_2d", "served_mix", "hetero_3d"):
         if args.gpus != 1:
             print(json.dumps({"error": "this build measures one GPU; run with --gpus 1"}))
@@ -375,8 +377,6 @@ def main() -> int:
         dirs = [Path(p) for p in args.pmc_summary]
         print(json.dumps(pmc_summary(*dirs[:4], args.copy_bytes, dirs[4]), indent=1))
         return 0
-    if args.kernels:
-        os.environ["FULLWAVE_HIP_KERNELS"] = args.kernels
     if args.copy_probe:
         from fullwave25_amd import engine
 

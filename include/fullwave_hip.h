@@ -123,11 +123,14 @@ int fw_sim_set_field(fw_sim* sim, const char* name, const float* values, size_t 
  * kernel). It has no counterpart in the reference, whose binary fixes its kernels.
  * variant 0 = per-cell kernels, 1 = x-marching kernels with
  * `width` lanes per workgroup along the contiguous axis and `chunk` planes per workgroup,
- * 2 = 2D tiled kernels with 4-wide vector access on tiles of `chunk` rows x 256 columns.
+ * 2 = 2D tiled kernels with 4-wide vector access on tiles of `chunk` rows x 256 columns,
+ * 3 / 4 = 3D software-pipelined x-marching kernels (workgroups of `width` = 512 or 1024 lanes;
+ * per-cell operands loaded one iteration ahead / at the top of their iteration).
  * Chosen on the first run by timing the candidates on the handle's own arrays, separately for
  * the two kernels (or forced for both with
- * the FULLWAVE_HIP_KERNELS environment variable: "percell", "stream[:width[:chunk]]" or
- * "tile[:rows]", or "<velocity>/<pressure>" to set each on its own).
+ * the FULLWAVE_HIP_KERNELS environment variable: "percell", "stream[:width[:chunk]]",
+ * "tile[:rows]", "pipe[:width[:chunk]]" or "pipelite[:width[:chunk]]", or
+ * "<velocity>/<pressure>" to set each on its own).
  * Calling this before the first run makes that choice. */
 int fw_sim_kernel_config(fw_sim* sim, int32_t kind, int32_t* variant, int32_t* width, int32_t* chunk);
 /* Fix the kernel family and shape of `kind` (values as fw_sim_kernel_config reports them)

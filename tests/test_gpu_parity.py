@@ -290,12 +290,13 @@ def test_batched_shots_bitwise_equal_single_runs(monkeypatch, shape):
             np.testing.assert_array_equal(got[b], one.sensors())
 
 
-def test_batched_shots_3d_bitwise_and_reuse(monkeypatch):
+@pytest.mark.parametrize("shape", ["stream:1024:16", "pipelite:512:8"])
+def test_batched_shots_3d_bitwise_and_reuse(monkeypatch, shape):
     """3D batch (x-marching kernels) equals single runs; new source matrices on the same
     handle after a reset give the same result as a fresh handle."""
     inp = workloads.random_medium_3d(20, 22, 70, 60, seed=43).inputs
     a, b = _shot_stack(inp, 2, seed=44), _shot_stack(inp, 2, seed=45)
-    monkeypatch.setenv("FULLWAVE_HIP_KERNELS", "stream:1024:16")
+    monkeypatch.setenv("FULLWAVE_HIP_KERNELS", shape)
     with engine.HipSimulation(inp, shots=a) as sim:
         sim.run(inp.nt)
         got_a = sim.sensors()
