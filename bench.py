@@ -70,6 +70,8 @@ def build_workload(args):
         return workloads.plane_wave_2d(nx=args.nx, ny=args.ny, nt=args.steps)
     if args.config == "random_3d":
         return workloads.random_medium_3d(args.nx, args.ny, args.nz, args.steps, seed=5)
+    if args.config == "inclusions_3d":  # the hetero_3d workload, for --sweep and --profile-steps
+        return workloads.inclusions_3d(args.nx, args.ny, args.nz, nt=max(args.steps, 100))
     raise SystemExit(f"unknown config {args.config}")
 
 
@@ -217,14 +219,18 @@ def bench_3d(args) -> dict:
 
 
 def _served_jobs(args):
-    """The served mix: 2D forward jobs on 512^2..4096^2 grids with 1000..8000 steps (70 %),
-    misfit-gradient jobs on 1024^2 (20 %), small 3D forward jobs (10 %), seeded."""
+    """The served mix, seeded. ``--mix standard``: 2D forward jobs on 512^2..4096^2 grids with
+    1000..8000 steps (70 %), misfit-gradient jobs on 1024^2 (20 %), small 3D forward jobs
+    (10 %). ``--mix small``: the same shares on 256^2..1024^2 grids with 500..4000 steps, 512^2
+    misfit gradients and 96^3 3D jobs: requests none of which fills the GPU alone."""
     from fullwave25_amd import workloads
 
     import dataclasses
 
     rng = np.random.default_rng(7)
-    sizes = (512, 1024, 2048, 4096)
+    small = args.mix == "small"
+    sizes = (256, 512, 1024) if small else (512, 1024, 2048, 4096)
+    adj_n, adj_steps = (512, 1000) if small else (1024, 2000)
     media = {}
 
     def medium(n, steps):
@@ -237,11 +243,11 @@ def _served_jobs(args):
     adj_inp = None
     for _ in range(args.jobs):
         u = rng.random()
-        steps = int(rng.integers(2, 17)) * 500
+        steps = int(rng.integers(2, 17)) * (250 if small else 500)
         if u < 0.7:
             jobs.append(("forward", medium(int(rng.choice(sizes)), steps)))
         elif u < 0.9:
-            adj_inp = adj_inp if adj_inp is not None else medium(1024, 2000)
+            adj_inp = adj_inp if adj_inp is not None else medium(adj_n, adj_steps)
             jobs.append(("adjoint", adj_inp))
         else:
             jobs.append(("forward3d", None))
@@ -256,7 +262,8 @@ def bench_served(args) -> dict:
 
     t0 = time.time()
     jobs, _ = _served_jobs(args)
-    w3 = workloads.plane_wave_3d(args.nz, args.nz, args.nz, nt=1000) if any(k == "forward3d" for k, _ in jobs) else None
+    n3 = 96 if args.mix == "small" else args.nz
+    w3 = workloads.plane_wave_3d(n3, n3, n3, nt=1000) if any(k == "forward3d" for k, _ in jobs) else None
     jobs = [(k, w3.inputs if k == "forward3d" else inp) for k, inp in jobs]
     observed = {}
     for k, inp in jobs:
@@ -308,10 +315,12 @@ def bench_served(args) -> dict:
         "queue_p50_s": round(st.queue_p50_s, 3), "sequential_jobs_per_hour": round(len(jobs) / t_seq * 3600, 1),
         "speedup_vs_sequential": round(st.jobs_per_hour / (len(jobs) / t_seq * 3600), 3),
         "served_glups": round(st.glups, 3), "peak_concurrent_jobs": srv.peak_running,
-        "config": {"workload": f"{len(jobs)} jobs: 2D forward 512^2-4096^2 x 1000-8000 steps, 1024^2 misfit "
-                               f"gradients (2000 steps, checkpoint every 200), {args.nz}^3 3D forward (1000 steps); "
-                               f"Poisson arrivals at {args.arrival_factor} x the sequential service rate",
-                   "slots": args.slots, "jobs": len(jobs)},
+        "config": {"workload": (f"{len(jobs)} jobs: 2D forward 256^2-1024^2 x 500-4000 steps, 512^2 misfit "
+                                f"gradients (1000 steps), 96^3 3D forward (1000 steps)" if args.mix == "small" else
+                                f"{len(jobs)} jobs: 2D forward 512^2-4096^2 x 1000-8000 steps, 1024^2 misfit "
+                                f"gradients (2000 steps, checkpoint every 200), {args.nz}^3 3D forward (1000 steps)")
+                               + f"; Poisson arrivals at {args.arrival_factor} x the sequential service rate",
+                   "mix": args.mix, "slots": args.slots, "jobs": len(jobs)},
     }
 
 
@@ -350,6 +359,7 @@ def main() -> int:
                     help="run only this many steps and exit (for rocprofv3)")
     ap.add_argument("--sweep", type=int, default=0,
                     help="time every kernel family/shape over this many steps and exit")
+    ap.add_argument("--sweep-shapes", default="", help="comma-separated kernel shapes for --sweep")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run the plain copy kernel over this many bytes (counter calibration)")
     ap.add_argument("--kernels", default="",
@@ -361,6 +371,8 @@ def main() -> int:
     ap.add_argument("--transmits", type=int, default=32, help="batched_2d: transmits on the shared medium")
     ap.add_argument("--jobs", type=int, default=20, help="served_mix: number of jobs")
     ap.add_argument("--slots", type=int, default=4, help="served_mix: concurrent jobs")
+    ap.add_argument("--mix", choices=("standard", "small"), default="standard",
+                    help="served_mix: job sizes (small: none fills the GPU alone)")
     ap.add_argument("--arrival-factor", type=float, default=2.0,
                     help="served_mix: arrival rate as a multiple of the sequential service rate")
     args = ap.parse_args()
@@ -400,6 +412,8 @@ def main() -> int:
         if inp.ndim == 3:
             shapes = ["percell"] + [f"{v}:{w}:{c}" for v in ("stream", "pipe", "pipelite") for w in (512, 1024)
                                     for c in (8, 16, 32)]
+        if args.sweep_shapes:
+            shapes = args.sweep_shapes.split(",")
         updated = int(np.prod([s - 16 for s in inp.shape]))
         for shape in shapes:
             os.environ["FULLWAVE_HIP_KERNELS"] = shape

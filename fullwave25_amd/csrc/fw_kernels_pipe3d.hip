@@ -74,6 +74,60 @@ struct Stager {
     }
 };
 
+// As Stager, for a field whose interior values (one per lane, the lane's own column) are
+// already in the lanes' registers (the axial register queue): the lanes load only the halo ring
+// of the staged plane, and each lane writes its own interior element from a register. Saves
+// the interior's second read of every plane (a quarter to a third of the staging loads).
+template <int TY, int HY, int HZ>
+struct HaloStager {
+    using T = PTile<TY, HY, HZ>;
+    static constexpr int NH = T::N - TY * PZ3;          // halo elements of a plane
+    static constexpr int R = (NH + T::NT - 1) / T::NT;  // halo elements per lane
+    int32_t off[R];  // in-plane offset, or -1 outside the grid / past the halo
+    int32_t idx[R];  // element of the staged plane, or -1 past the halo
+    float val[R];
+    // the h-th halo element: HY whole rows above the interior rows, the 2 HZ side cells of
+    // each interior row, HY whole rows below
+    __device__ static int halo_element(int h) {
+        constexpr int top = HY * T::PZ, side = TY * 2 * HZ;
+        if (h < top) return h;
+        h -= top;
+        if (h < side) {
+            const int zz = h % (2 * HZ);
+            return (HY + h / (2 * HZ)) * T::PZ + (zz < HZ ? zz : zz + PZ3);
+        }
+        return (HY + TY) * T::PZ + (h - side);
+    }
+    __device__ void init(const DevSim& s, int64_t j0, int64_t k0) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int h = threadIdx.x + r * T::NT;
+            off[r] = -1;
+            idx[r] = -1;
+            if (h < NH) {
+                const int e = halo_element(h);
+                idx[r] = e;
+                const int yy = e / T::PZ, zz = e - (e / T::PZ) * T::PZ;
+                const int64_t jj = j0 - HY + yy, kk = k0 - HZ + zz;
+                if (jj >= 0 && jj < s.ny && kk >= 0 && kk < s.nz) off[r] = (int32_t)(jj * s.sy + kk);
+            }
+        }
+    }
+    __device__ void load(const float* __restrict__ f, int64_t plane, const DevSim& s) {
+        const float* base = f + plane * s.sx;
+#pragma unroll
+        for (int r = 0; r < R; ++r) val[r] = off[r] >= 0 ? gld(base, (uint32_t)off[r]) : 0.f;
+    }
+    // own: this lane's interior value of the plane (0 outside the grid)
+    __device__ void store(T& tile, int64_t plane, float own, int ty, int tz) const {
+        float* dst = tile.slot(plane);
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            if (idx[r] >= 0) dst[idx[r]] = val[r];
+        dst[T::at(ty, tz, 0, 0)] = own;
+    }
+};
+
 template <bool UNIFORM>
 __device__ inline const float* coefp(const float* coefs, int32_t dc, int axis) {
     if (UNIFORM) return coefs + axis * kNCoef;
@@ -155,22 +209,22 @@ __global__ __launch_bounds__(PZ3 * TY) void k_velocity_pipe3d(DevSim batch, int 
     const uint32_t sx = (uint32_t)s.sx, col = (uint32_t)(j * s.sy + k);
     const float* coefs = coef_table(s, lds_coef, UNIFORM);
 
-    Stager<TY, kHalo, kHalo> st;
+    HaloStager<TY, kHalo, kHalo> st;
     st.init(s, j0, k0);
-    float q[16];  // q[m] = p(i - 7 + m)
+    float q[16];  // q[m] = p(i - 7 + m) in the stencil; p(i - 8 + m) at the top of iteration i
 #pragma unroll
     for (int m = 0; m < 15; ++m) q[m + 1] = in_grid ? gld((const float*)s.p, (uint32_t)(i0 - 7 + m) * sx + col) : 0.f;
     st.load(s.p, i0 - 1, s);
-    st.store(P, i0 - 1);
+    st.store(P, i0 - 1, q[7], ty, tz);
     st.load(s.p, i0, s);
-    st.store(P, i0);
+    st.store(P, i0, q[8], ty, tz);
     st.load(s.p, i0 + 1, s);
     float rho_i = in_grid ? gld(s.rho, (uint32_t)i0 * sx + col) : 0.f;
     VelOps<ISO> cur, nxt;
     if (AHEAD) load_vel<UNIFORM, ISO, CODED>(cur, s, (uint32_t)i0 * sx + col, in_grid, active);
 
     for (int64_t i = i0; i < i1; ++i) {
-        st.store(P, i + 1);
+        st.store(P, i + 1, q[9], ty, tz);
         const bool more = i + 1 < i1;
         if (!AHEAD) load_vel<UNIFORM, ISO, CODED>(cur, s, (uint32_t)i * sx + col, in_grid, active);
         if (more) {
@@ -300,17 +354,17 @@ __global__ __launch_bounds__(PZ3 * TY) void k_pressure_pipe3d(DevSim batch, int 
     const uint32_t sx = (uint32_t)s.sx, col = (uint32_t)(j * s.sy + k);
     const float* coefs = coef_table(s, lds_coef, UNIFORM);
 
-    Stager<TY, 1, 1> sx_;
+    HaloStager<TY, 1, 1> sx_;
     Stager<TY, kHalo, 1> sy_;
     Stager<TY, 1, kHalo> sz_;
     sx_.init(s, j0, k0);
     sy_.init(s, j0, k0);
     sz_.init(s, j0, k0);
-    float q[16];  // q[m] = vx(i - 8 + m)
+    float q[16];  // q[m] = vx(i - 8 + m) in the stencil; vx(i - 9 + m) at the top of iteration i
 #pragma unroll
     for (int m = 0; m < 15; ++m) q[m + 1] = in_grid ? gld((const float*)s.v[0], (uint32_t)(i0 - 8 + m) * sx + col) : 0.f;
     sx_.load(s.v[0], i0 - 1, s);
-    sx_.store(VX, i0 - 1);
+    sx_.store(VX, i0 - 1, q[8], ty, tz);
     sy_.load(s.v[1], i0 - 1, s);
     sy_.store(VY, i0 - 1);
     sy_.load(s.v[1], i0, s);
@@ -326,7 +380,7 @@ __global__ __launch_bounds__(PZ3 * TY) void k_pressure_pipe3d(DevSim batch, int 
     if (AHEAD) load_pre<UNIFORM, ISO, CODED>(cur, s, (uint32_t)i0 * sx + col, in_grid, active);
 
     for (int64_t i = i0; i < i1; ++i) {
-        sx_.store(VX, i);
+        sx_.store(VX, i, q[9], ty, tz);
         sy_.store(VY, i + 1);
         sz_.store(VZ, i + 1);
         const bool more = i + 1 < i1;
