@@ -167,6 +167,67 @@ def bench_batched(args) -> dict:
     }
 
 
+def bench_grouped(args) -> dict:
+    """Jobs on uneven media in shared launches: ``--jobs`` 2D jobs on 1024^2, 1536x1024, 2048^2
+    and 3072x2048 grids (one seeded heterogeneous medium per size) with 3000..6000 steps, run
+    one after the other as plain engine runs (create, upload, tune, run, download) against all
+    of them in one group of handles (fw_group_run: one velocity, one pressure and one
+    source/sensor launch per step for every job still running; jobs leave the group as they
+    finish)."""
+    import dataclasses
+
+    from fullwave25_amd import engine, workloads
+
+    t0 = time.time()
+    rng = np.random.default_rng(2028)
+    sizes = [(1024, 1024), (1536, 1024), (2048, 2048), (3072, 2048)]
+    media = {sz: workloads.focused_linear_array_2d(nx=sz[0], ny=sz[1], nt=6000).inputs for sz in sizes}
+    jobs = []
+    for _ in range(args.jobs):
+        sz = sizes[int(rng.integers(len(sizes)))]
+        jobs.append(dataclasses.replace(media[sz], nt=int(rng.integers(3, 7)) * 1000))
+    print(f"# grouped: {len(jobs)} jobs built in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+    with engine.HipSimulation(jobs[0]) as warm:  # first-use costs (library load, code objects)
+        warm.run(10)
+    t1 = time.perf_counter()
+    seq = []
+    for inp in jobs:
+        with engine.HipSimulation(inp) as sim:
+            sim.run(inp.nt)
+            seq.append(sim.sensors())
+    t_seq = time.perf_counter() - t1
+    print(f"# sequential: {t_seq:.1f}s", file=sys.stderr, flush=True)
+    t2 = time.perf_counter()
+    sims = [engine.HipSimulation(inp) for inp in jobs]
+    done = 0
+    got = [None] * len(jobs)
+    live = list(range(len(jobs)))
+    while live:
+        end = min(jobs[i].nt for i in live)
+        with engine.HipGroup([sims[i] for i in live], width=args.group_width, chunk=args.group_chunk) as grp:
+            grp.run(end - done)
+        done = end
+        for i in [i for i in live if jobs[i].nt == end]:
+            got[i] = sims[i].sensors()
+            sims[i].close()
+        live = [i for i in live if jobs[i].nt > end]
+    t_grp = time.perf_counter() - t2
+    worst = max(float(np.linalg.norm(g - q) / max(np.linalg.norm(q), 1e-30)) for g, q in zip(got, seq))
+    cells = sum(inp.ncells * inp.nt for inp in jobs)
+    return {
+        "metric": "grouped uneven jobs: jobs/hour vs sequential plain launches (1 GPU)",
+        "value": round(len(jobs) / t_grp * 3600, 1), "unit": "jobs/hour", "n_gpus": 1,
+        "higher_is_better": True, "dtype": "f32", "data": "synthetic",
+        "sequential_jobs_per_hour": round(len(jobs) / t_seq * 3600, 1),
+        "speedup_vs_sequential": round(t_seq / t_grp, 3),
+        "glups_grouped": round(cells / t_grp / 1e9, 3), "glups_sequential": round(cells / t_seq / 1e9, 3),
+        "max_rel_l2_grouped_vs_sequential": worst,
+        "config": {"workload": f"{len(jobs)} focused-transmit jobs on 1024^2, 1536x1024, 2048^2 and 3072x2048 "
+                               f"interiors, 3000-6000 steps; group kernel stream:{args.group_width}:{args.group_chunk}",
+                   "jobs": len(jobs)},
+    }
+
+
 def bench_3d(args) -> dict:
     """The 3D heterogeneous configuration (768 x 768 x 640 interior, 904 x 904 x 776 cells,
     ellipsoidal inclusions, 32 x 32 matrix-array plane wave, 6000-step job): the per-step rate
@@ -369,7 +430,9 @@ def main() -> int:
                     help="summarize rocprofv3 --pmc runs into per-launch HBM traffic (JSON) and exit")
     ap.add_argument("--copy-bytes", type=int, default=268435456)
     ap.add_argument("--transmits", type=int, default=32, help="batched_2d: transmits on the shared medium")
-    ap.add_argument("--jobs", type=int, default=20, help="served_mix: number of jobs")
+    ap.add_argument("--jobs", type=int, default=20, help="served_mix, grouped_2d: number of jobs")
+    ap.add_argument("--group-width", type=int, default=256, help="grouped_2d: lanes per workgroup")
+    ap.add_argument("--group-chunk", type=int, default=16, help="grouped_2d: rows per workgroup")
     ap.add_argument("--slots", type=int, default=4, help="served_mix: concurrent jobs")
     ap.add_argument("--mix", choices=("standard", "small"), default="standard",
                     help="served_mix: job sizes (small: none fills the GPU alone)")
@@ -378,11 +441,12 @@ def main() -> int:
     args = ap.parse_args()
     if args.kernels:
         os.environ["FULLWAVE_HIP_KERNELS"] = args.kernels
-    if args.config in ("batched_2d", "served_mix", "hetero_3d"):
+    if args.config in ("batched_2d", "served_mix", "hetero_3d", "grouped_2d"):
         if args.gpus != 1:
             print(json.dumps({"error": "this build measures one GPU; run with --gpus 1"}))
             return 2
-        fn = {"batched_2d": bench_batched, "served_mix": bench_served, "hetero_3d": bench_3d}[args.config]
+        fn = {"batched_2d": bench_batched, "served_mix": bench_served, "hetero_3d": bench_3d,
+              "grouped_2d": bench_grouped}[args.config]
         print(json.dumps(fn(args)), flush=True)
         return 0
     if args.pmc_summary:

@@ -381,25 +381,37 @@ int encode_relaxation(fw_sim* s) {
     for (int k = 0; k < fw::kCodeVals - 1; ++k) m.f[k] = (const float*)s->dev[names[k]];
     m.dcmap = (const int32_t*)s->dev["dcmap"];
 
-    const size_t nslots = fw::kCodeSlots;
+    size_t nslots = fw::kCodeSlotsSmall;
     unsigned long long* keys = nullptr;
     uint32_t* count = nullptr;
     int32_t* rep = nullptr;
     int32_t* status = nullptr;
     uint16_t* slot_code_d = nullptr;
     int32_t* rep_of_code_d = nullptr;
+    auto free_table = [&] {
+        for (void* q : {(void*)keys, (void*)count, (void*)rep})
+            if (q) (void)hipFree(q);
+        keys = nullptr;
+        count = nullptr;
+        rep = nullptr;
+    };
     auto run = [&]() -> int {
-        FW_HIP(hipMalloc((void**)&keys, nslots * 8));
-        FW_HIP(hipMalloc((void**)&count, nslots * 4));
-        FW_HIP(hipMalloc((void**)&rep, nslots * 4));
         FW_HIP(hipMalloc((void**)&status, 16));
-        FW_HIP(hipMemsetAsync(keys, 0, nslots * 8, s->stream));
-        FW_HIP(hipMemsetAsync(count, 0, nslots * 4, s->stream));
-        FW_HIP(hipMemsetAsync(status, 0, 16, s->stream));
-        FW_HIP(fw::launch_code_insert(m, s->ncell, keys, count, rep, status, s->stream));
         int32_t st[4] = {0, 0, 0, 0};
-        FW_HIP(copy_sync(s, st, status, 16, hipMemcpyDeviceToHost));
-        if (st[0]) return FW_OK;  // probe limit: far too many distinct tuples
+        for (;;) {
+            FW_HIP(hipMalloc((void**)&keys, nslots * 8));
+            FW_HIP(hipMalloc((void**)&count, nslots * 4));
+            FW_HIP(hipMalloc((void**)&rep, nslots * 4));
+            FW_HIP(hipMemsetAsync(keys, 0, nslots * 8, s->stream));
+            FW_HIP(hipMemsetAsync(count, 0, nslots * 4, s->stream));
+            FW_HIP(hipMemsetAsync(status, 0, 16, s->stream));
+            FW_HIP(fw::launch_code_insert(m, s->ncell, keys, (uint32_t)nslots, count, rep, status, s->stream));
+            FW_HIP(copy_sync(s, st, status, 16, hipMemcpyDeviceToHost));
+            if (!st[0]) break;
+            if (nslots == fw::kCodeSlots) return FW_OK;  // probe limit: far too many distinct tuples
+            free_table();
+            nslots = fw::kCodeSlots;
+        }
         std::vector<unsigned long long> hk(nslots);
         std::vector<uint32_t> hc(nslots);
         std::vector<int32_t> hr(nslots);
@@ -434,7 +446,8 @@ int encode_relaxation(fw_sim* s) {
         if ((r = alloc(s, "_rcode", (size_t)s->ncell * 2, &code)) != FW_OK) return r;
         if ((r = alloc(s, "_rtab_v", (size_t)n * fw::kCodeStride * 4, &tv)) != FW_OK) return r;
         if ((r = alloc(s, "_rtab_p", (size_t)n * fw::kCodeStride * 4, &tp)) != FW_OK) return r;
-        FW_HIP(fw::launch_code_assign(m, s->ncell, keys, slot_code_d, (uint16_t*)code, status, s->stream));
+        FW_HIP(fw::launch_code_assign(m, s->ncell, keys, (uint32_t)nslots, slot_code_d, (uint16_t*)code, status,
+                                      s->stream));
         FW_HIP(fw::launch_code_table(m, n, rep_of_code_d, (float*)tv, (float*)tp, s->stream));
         FW_HIP(fw::launch_code_verify(m, s->ncell, (const uint16_t*)code, n, (const float*)tv, (const float*)tp,
                                       status, s->stream));
@@ -448,7 +461,8 @@ int encode_relaxation(fw_sim* s) {
         return FW_OK;
     };
     const int rc = run();
-    for (void* q : {(void*)keys, (void*)count, (void*)rep, (void*)status, (void*)slot_code_d, (void*)rep_of_code_d})
+    free_table();
+    for (void* q : {(void*)status, (void*)slot_code_d, (void*)rep_of_code_d})
         if (q) (void)hipFree(q);
     if (rc != FW_OK || s->ncodes == 0) drop_codes(s);
     return rc;

@@ -1,0 +1,184 @@
+// Grouped time steps: several engine handles (different grids and media) advanced together.
+// Every step is one velocity launch, one pressure launch and one source/air/sensor launch over
+// the tiles of all members (kernels in fw_kernels_stream.hip and fw_kernels_simple.hip), so
+// many small or uneven jobs fill the GPU as one job does, without one launch per job and kernel.
+#include <hip/hip_runtime.h>
+
+#include <set>
+#include <vector>
+
+#include "../../include/fullwave_hip.h"
+#include "fw_internal.h"
+#include "fw_sim.h"
+
+struct fw_group {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::vector<fw_sim*> sims;
+    std::vector<fw::DevSim> views;  // each member's device view when the group was made
+    std::vector<int> shots_of;        // boundary entries per member (its shots)
+    std::vector<fw::GroupBoundary> bounds_h;
+    fw::GroupTile* tiles = nullptr;
+    fw::GroupBoundary* bounds = nullptr;
+    int ntiles = 0;
+    uint32_t blocks = 0;
+    int64_t max_points = 0;
+    int width = 256, chunk = 16, max_ndmap = 1;
+};
+
+namespace {
+
+#define GR_HIP(expr)                                                                            \
+    do {                                                                                        \
+        hipError_t e_ = (expr);                                                                 \
+        if (e_ != hipSuccess)                                                                   \
+            return fw::set_error(FW_ERR_DEVICE, "%s failed: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+struct Guard {
+    int prev = -1;
+    explicit Guard(int dev) {
+        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+        (void)hipSetDevice(dev);
+    }
+    ~Guard() {
+        if (prev >= 0) (void)hipSetDevice(prev);
+    }
+};
+
+void release(fw_group* g) {
+    if (!g) return;
+    if (g->stream) (void)hipStreamSynchronize(g->stream);
+    if (g->tiles) (void)hipFree(g->tiles);
+    if (g->bounds) (void)hipFree(g->bounds);
+    if (g->stream) (void)hipStreamDestroy(g->stream);
+    delete g;
+}
+
+int build(fw_group* g) {
+    std::vector<fw::GroupTile> tiles;
+    std::vector<fw::GroupBoundary> bounds;
+    uint64_t blocks = 0;
+    for (fw_sim* s : g->sims) {
+        const unsigned gx = fw::group_tiles_x(s->ds.ny, g->width);
+        const unsigned gy = fw::group_tiles_y(s->ds.nx, g->chunk);
+        for (int b = 0; b < s->nshots; ++b) {
+            fw::GroupTile t{};
+            t.s = fw::shot_view(s->ds, b);
+            t.start = (uint32_t)blocks;
+            t.gx = gx;
+            tiles.push_back(t);
+            blocks += (uint64_t)gx * gy;
+            fw::GroupBoundary e{};
+            e.p = t.s.p;
+            e.b = s->db;
+            e.b.icmat_t = s->db.icmat_t + (int64_t)b * s->db.icmat_stride;
+            e.b.rec = s->db.rec + (int64_t)b * s->db.rec_stride;
+            e.modt = s->d.modt;
+            e.step0 = s->step;
+            bounds.push_back(e);
+            int64_t pts = s->db.ncoords;
+            if (s->db.nzero > pts) pts = s->db.nzero;
+            if (s->db.nout > pts) pts = s->db.nout;
+            if (pts > g->max_points) g->max_points = pts;
+        }
+        if (s->ds.ndmap > g->max_ndmap) g->max_ndmap = s->ds.ndmap;
+        g->shots_of.push_back(s->nshots);
+    }
+    if (blocks >= (1ull << 32)) return fw::set_error(FW_ERR_INVALID, "the group has too many tiles for one launch");
+    if (bounds.size() > fw::kMaxGridZ) return fw::set_error(FW_ERR_INVALID, "a group holds at most 65535 grids");
+    g->blocks = (uint32_t)blocks;
+    g->ntiles = (int)tiles.size();
+    GR_HIP(hipMalloc((void**)&g->tiles, tiles.size() * sizeof(fw::GroupTile)));
+    GR_HIP(hipMalloc((void**)&g->bounds, bounds.size() * sizeof(fw::GroupBoundary)));
+    GR_HIP(hipMemcpyAsync(g->tiles, tiles.data(), tiles.size() * sizeof(fw::GroupTile), hipMemcpyHostToDevice,
+                          g->stream));
+    GR_HIP(hipMemcpyAsync(g->bounds, bounds.data(), bounds.size() * sizeof(fw::GroupBoundary), hipMemcpyHostToDevice,
+                          g->stream));
+    GR_HIP(hipStreamSynchronize(g->stream));
+    g->bounds_h = bounds;
+    return FW_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fw_group_create(fw_sim* const* sims, int32_t n, int32_t width, int32_t chunk, fw_group** out) {
+    if (!sims || !out || n < 1) return fw::set_error(FW_ERR_INVALID, "null argument or empty group");
+    *out = nullptr;
+    if (width != 64 && width != 128 && width != 256)
+        return fw::set_error(FW_ERR_INVALID, "group width must be 64, 128 or 256");
+    if (chunk < 2 || chunk > 4096) return fw::set_error(FW_ERR_INVALID, "group chunk must be in [2, 4096]");
+    std::set<fw_sim*> seen;
+    for (int32_t i = 0; i < n; ++i) {
+        fw_sim* s = sims[i];
+        if (!s) return fw::set_error(FW_ERR_INVALID, "null handle in the group");
+        if (!seen.insert(s).second) return fw::set_error(FW_ERR_INVALID, "a handle appears twice in the group");
+        if (s->device != sims[0]->device) return fw::set_error(FW_ERR_INVALID, "group members must share one device");
+        if (s->ndim != 2 || s->d.attenuation_model != 0 || !s->d.isotropic)
+            return fw::set_error(FW_ERR_UNSUPPORTED, "groups take 2D isotropic relaxation simulations");
+    }
+    Guard guard(sims[0]->device);
+    for (int32_t i = 0; i < n; ++i) {
+        int rc = fw::prepare(sims[i]);
+        if (rc != FW_OK) return rc;
+        if (!fw::streaming_supported(sims[i]->ds))
+            return fw::set_error(FW_ERR_UNSUPPORTED, "group member %d needs the per-cell kernels", (int)i);
+    }
+    fw_group* g = new fw_group();
+    g->device = sims[0]->device;
+    g->width = width;
+    g->chunk = chunk;
+    g->sims.assign(sims, sims + n);
+    for (fw_sim* s : g->sims) g->views.push_back(s->ds);
+    hipError_t e = hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        release(g);
+        return fw::set_error(FW_ERR_DEVICE, "hipStreamCreateWithFlags failed: %s", hipGetErrorString(e));
+    }
+    int rc = build(g);
+    if (rc != FW_OK) {
+        release(g);
+        return rc;
+    }
+    *out = g;
+    return FW_OK;
+}
+
+int fw_group_run(fw_group* g, int64_t n_steps) {
+    if (!g) return fw::set_error(FW_ERR_INVALID, "null group");
+    if (n_steps < 0) return fw::set_error(FW_ERR_INVALID, "n_steps < 0");
+    size_t e = 0;
+    for (size_t i = 0; i < g->sims.size(); ++i) {
+        fw_sim* s = g->sims[i];
+        const fw::DevSim& v = g->views[i];
+        if (s->ds.rcode != v.rcode || s->ds.rtab_v != v.rtab_v || s->ds.rtab_p != v.rtab_p || s->ds.p != v.p)
+            return fw::set_error(FW_ERR_STATE, "group member %d changed since the group was made", (int)i);
+        if (s->step + n_steps > s->d.nt)
+            return fw::set_error(FW_ERR_INVALID, "group member %d has %lld steps left", (int)i,
+                                 (long long)(s->d.nt - s->step));
+        for (int b = 0; b < g->shots_of[i]; ++b) g->bounds_h[e++].step0 = s->step;  // members may differ
+    }
+    Guard guard(g->device);
+    for (fw_sim* s : g->sims) GR_HIP(hipStreamSynchronize(s->stream));  // members idle
+    GR_HIP(hipMemcpyAsync(g->bounds, g->bounds_h.data(), g->bounds_h.size() * sizeof(fw::GroupBoundary),
+                          hipMemcpyHostToDevice, g->stream));
+    for (int64_t it = 0; it < n_steps; ++it) {
+        GR_HIP(fw::launch_group_2d(g->tiles, g->ntiles, g->blocks, g->width, g->chunk, g->max_ndmap, true, g->stream));
+        GR_HIP(fw::launch_group_2d(g->tiles, g->ntiles, g->blocks, g->width, g->chunk, g->max_ndmap, false, g->stream));
+        GR_HIP(fw::launch_group_boundary(g->bounds, g->ntiles, g->max_points, it, g->stream));
+    }
+    GR_HIP(hipStreamSynchronize(g->stream));
+    for (fw_sim* s : g->sims) s->step += n_steps;
+    return FW_OK;
+}
+
+int fw_group_destroy(fw_group* g) {
+    if (!g) return FW_OK;
+    Guard guard(g->device);
+    release(g);
+    return FW_OK;
+}
+
+}  // extern "C"

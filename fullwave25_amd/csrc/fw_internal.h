@@ -60,7 +60,8 @@ constexpr int kCodeVals = 11;          // values per tuple: velocity side, press
 constexpr int kCodeStride = 8;         // floats per table row (one 16-byte and one 8-byte load)
 constexpr uint32_t kCodeEscape = 0xffff;  // the cell's tuple has no code
 constexpr int kMaxCodes = 65535;       // codes 0 .. 65534
-constexpr uint32_t kCodeSlots = 1u << 22;  // hash table slots
+constexpr uint32_t kCodeSlotsSmall = 1u << 16;  // hash table slots, first try
+constexpr uint32_t kCodeSlots = 1u << 22;       // hash table slots, when the first try overflows
 constexpr int kCodeMaxProbe = 128;
 static_assert(kCodeStride == 8, "coded_coef addresses rows as code << 5 bytes");
 
@@ -175,6 +176,22 @@ struct AdjView {
     float* gRb[kMaxDim];     // d/d rho_bar_a, summed over steps
 };
 
+// One grid (or one shot of a batch) of a grouped launch: its workgroups are
+// [start, start + gx * gy) of the launch, gx tiles across the contiguous axis.
+struct GroupTile {
+    DevSim s;
+    uint32_t start;
+    uint32_t gx;
+};
+
+// One grid (or shot) of a grouped source/air/sensor pass.
+struct GroupBoundary {
+    float* p;             // this shot's pressure field
+    DevBoundary b;        // icmat_t and rec already offset to this shot
+    int64_t modt;
+    int64_t step0;        // the grid's step when the current fw_group_run call began
+};
+
 enum KernelKind { kVelocity = 0, kPressure = 1, kBoundary = 2, kNumKinds = 3 };
 
 // Kernel family and shape for the two stencil kernels of a step.
@@ -202,12 +219,18 @@ hipError_t launch_boundary(const DevSim& s, const DevBoundary& b, int64_t step, 
 hipError_t launch_boundary_dev(const DevSim& s, const DevBoundary& b, const int64_t* step, int64_t modt,
                                hipStream_t st);
 hipError_t launch_tick(int64_t* step, int64_t set, hipStream_t st);
+// grouped 2D x-marching kernels (isotropic relaxation); `width` 64/128/256 lanes, `chunk` rows
+unsigned group_tiles_x(int64_t ny, int width);
+unsigned group_tiles_y(int64_t nx, int chunk);
+hipError_t launch_group_2d(const GroupTile* t, int n, uint32_t blocks, int width, int chunk, int max_ndmap,
+                           bool velocity, hipStream_t st);
+hipError_t launch_group_boundary(const GroupBoundary* g, int n, int64_t max_points, int64_t it, hipStream_t st);
 hipError_t launch_reciprocal(float* data, int64_t n, hipStream_t st);
 hipError_t launch_copy(const void* src, void* dst, int64_t n16, hipStream_t st);
-hipError_t launch_code_insert(const CodeMaps& m, int64_t n, unsigned long long* keys, uint32_t* count, int32_t* rep,
-                              int32_t* status, hipStream_t st);
-hipError_t launch_code_assign(const CodeMaps& m, int64_t n, const unsigned long long* keys, const uint16_t* slot_code,
-                              uint16_t* code, int32_t* status, hipStream_t st);
+hipError_t launch_code_insert(const CodeMaps& m, int64_t n, unsigned long long* keys, uint32_t nslots, uint32_t* count,
+                              int32_t* rep, int32_t* status, hipStream_t st);
+hipError_t launch_code_assign(const CodeMaps& m, int64_t n, const unsigned long long* keys, uint32_t nslots,
+                              const uint16_t* slot_code, uint16_t* code, int32_t* status, hipStream_t st);
 hipError_t launch_code_table(const CodeMaps& m, int32_t ncodes, const int32_t* rep_of_code, float* tab_v, float* tab_p,
                              hipStream_t st);
 hipError_t launch_code_verify(const CodeMaps& m, int64_t n, const uint16_t* code, int32_t ncodes, const float* tab_v,

@@ -14,7 +14,7 @@
 //   1. insert: every cell hashes its tuple (the raw bits of the eleven words) into an
 //      open-addressing table of 64-bit keys (linear probing, bounded probe count) and counts
 //      it; the lane whose compare-and-swap claims an empty slot records its cell as the slot's
-//      representative;
+//      representative (a table of 2^16 slots first, 2^22 if that one overflows);
 //   2. the host numbers the most frequent occupied slots 0 .. n-1 (n <= 65535);
 //   3. assign: every cell finds its key again and stores the slot's code (or the escape);
 //   4. table: each code copies its representative cell's eleven words into the tables;
@@ -49,7 +49,7 @@ __device__ inline uint64_t tuple_hash(const uint32_t t[kCodeVals]) {
     return h == 0 ? 1 : h;  // 0 marks an empty slot
 }
 
-__global__ __launch_bounds__(256) void k_code_insert(CodeMaps m, int64_t n, unsigned long long* keys,
+__global__ __launch_bounds__(256) void k_code_insert(CodeMaps m, int64_t n, unsigned long long* keys, uint32_t mask,
                                                      uint32_t* count, int32_t* rep, int32_t* status) {
     for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x) {
         uint32_t t[kCodeVals];
@@ -62,7 +62,7 @@ __global__ __launch_bounds__(256) void k_code_insert(CodeMaps m, int64_t n, unsi
         const unsigned long long group = __ballot(same);
         const bool leader = same && (int)__lane_id() == __ffsll((long long)group) - 1;
         if (same && !leader) continue;
-        uint32_t slot = (uint32_t)h & (kCodeSlots - 1);
+        uint32_t slot = (uint32_t)h & mask;
         bool placed = false;
         for (int probe = 0; probe < kCodeMaxProbe; ++probe) {
             unsigned long long k = __hip_atomic_load(&keys[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -77,7 +77,7 @@ __global__ __launch_bounds__(256) void k_code_insert(CodeMaps m, int64_t n, unsi
                 placed = true;
                 break;
             }
-            slot = (slot + 1) & (kCodeSlots - 1);
+            slot = (slot + 1) & mask;
         }
         if (placed) atomicAdd(&count[slot], same ? (uint32_t)__popcll(group) : 1u);
         else status[0] = 1;  // every writer stores the same value
@@ -85,12 +85,13 @@ __global__ __launch_bounds__(256) void k_code_insert(CodeMaps m, int64_t n, unsi
 }
 
 __global__ __launch_bounds__(256) void k_code_assign(CodeMaps m, int64_t n, const unsigned long long* keys,
-                                                     const uint16_t* slot_code, uint16_t* code, int32_t* status) {
+                                                     uint32_t mask, const uint16_t* slot_code, uint16_t* code,
+                                                     int32_t* status) {
     for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x) {
         uint32_t t[kCodeVals];
         tuple_of(m, c, t);
         const unsigned long long h = tuple_hash(t);
-        uint32_t slot = (uint32_t)h & (kCodeSlots - 1);
+        uint32_t slot = (uint32_t)h & mask;
         bool found = false;
         for (int probe = 0; probe < kCodeMaxProbe; ++probe) {
             const unsigned long long k = keys[slot];
@@ -99,7 +100,7 @@ __global__ __launch_bounds__(256) void k_code_assign(CodeMaps m, int64_t n, cons
                 break;
             }
             if (k == 0) break;
-            slot = (slot + 1) & (kCodeSlots - 1);
+            slot = (slot + 1) & mask;
         }
         code[c] = found ? slot_code[slot] : (uint16_t)kCodeEscape;
         if (!found) status[1] = 1;
@@ -151,15 +152,16 @@ dim3 code_grid(int64_t n) {
 
 }  // namespace
 
-hipError_t launch_code_insert(const CodeMaps& m, int64_t n, unsigned long long* keys, uint32_t* count, int32_t* rep,
-                              int32_t* status, hipStream_t st) {
-    hipLaunchKernelGGL(k_code_insert, code_grid(n), dim3(256), 0, st, m, n, keys, count, rep, status);
+// nslots: a power of two
+hipError_t launch_code_insert(const CodeMaps& m, int64_t n, unsigned long long* keys, uint32_t nslots, uint32_t* count,
+                              int32_t* rep, int32_t* status, hipStream_t st) {
+    hipLaunchKernelGGL(k_code_insert, code_grid(n), dim3(256), 0, st, m, n, keys, nslots - 1, count, rep, status);
     return hipGetLastError();
 }
 
-hipError_t launch_code_assign(const CodeMaps& m, int64_t n, const unsigned long long* keys, const uint16_t* slot_code,
-                              uint16_t* code, int32_t* status, hipStream_t st) {
-    hipLaunchKernelGGL(k_code_assign, code_grid(n), dim3(256), 0, st, m, n, keys, slot_code, code, status);
+hipError_t launch_code_assign(const CodeMaps& m, int64_t n, const unsigned long long* keys, uint32_t nslots,
+                              const uint16_t* slot_code, uint16_t* code, int32_t* status, hipStream_t st) {
+    hipLaunchKernelGGL(k_code_assign, code_grid(n), dim3(256), 0, st, m, n, keys, nslots - 1, slot_code, code, status);
     return hipGetLastError();
 }
 

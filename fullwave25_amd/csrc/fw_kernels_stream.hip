@@ -47,17 +47,17 @@ struct VelRow {
     float ps1[2], ps2[2], v[2];
 };
 
+// The velocity update of one workgroup's tile: columns [bx * TY, bx * TY + TY), rows chunk `by`.
 // CODED (isotropic only): the relaxation coefficients come from the code table (rcode, rtab_v)
 template <int TY, bool UNIFORM, bool ISO, bool CODED>
-__global__ __launch_bounds__(TY) void k_velocity_2d(DevSim batch, int chunk) {
-    const DevSim s = shot_view(batch, blockIdx.z);
-    extern __shared__ float lds_coef[];
+__device__ __forceinline__ void velocity_2d_tile(const DevSim& s, unsigned bx, unsigned by, int chunk,
+                                                 float* lds_coef) {
     __shared__ float ring[kRing][TY + 2 * kHalo];
     const int tid = threadIdx.x;
     const int64_t ny = s.ny, nx = s.nx;
-    const int64_t j0 = (int64_t)blockIdx.x * TY;
+    const int64_t j0 = (int64_t)bx * TY;
     const int64_t j = j0 + tid;
-    const int64_t i0 = kHalo + (int64_t)blockIdx.y * chunk;
+    const int64_t i0 = kHalo + (int64_t)by * chunk;
     const int64_t i1 = min(i0 + chunk, nx - kHalo);
     const bool col_ok = j < ny;
     const bool active = j >= kHalo && j < ny - kHalo;
@@ -178,6 +178,12 @@ __global__ __launch_bounds__(TY) void k_velocity_2d(DevSim batch, int chunk) {
     }
 }
 
+template <int TY, bool UNIFORM, bool ISO, bool CODED>
+__global__ __launch_bounds__(TY) void k_velocity_2d(DevSim batch, int chunk) {
+    extern __shared__ float lds_coef[];
+    velocity_2d_tile<TY, UNIFORM, ISO, CODED>(shot_view(batch, blockIdx.z), blockIdx.x, blockIdx.y, chunk, lds_coef);
+}
+
 // ---------------------------------------------------------------------------------------------
 // pressure, 2D
 
@@ -194,19 +200,17 @@ struct PreRow {
     float ph1[2], ph2[2];
 };
 
-// at most 96 VGPRs (five waves per SIMD instead of four): measured 3-15 % faster across the
-// 128/256-lane shapes, for a few bytes of spill per lane
+// The pressure update of one workgroup's tile (as velocity_2d_tile).
 template <int TY, bool UNIFORM, bool ISO, bool CODED>
-__global__ __launch_bounds__(TY) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_pressure_2d(DevSim batch, int chunk) {
-    const DevSim s = shot_view(batch, blockIdx.z);
-    extern __shared__ float lds_coef[];
+__device__ __forceinline__ void pressure_2d_tile(const DevSim& s, unsigned bx, unsigned by, int chunk,
+                                                 float* lds_coef) {
     __shared__ float ring_y[kRing][TY + 2 * kHalo];  // vy rows r-1, r, r+1 (8-cell halo)
     __shared__ float ring_x[kRing][TY + 2];          // vx rows r-1, r (1-cell halo)
     const int tid = threadIdx.x;
     const int64_t ny = s.ny, nx = s.nx;
-    const int64_t j0 = (int64_t)blockIdx.x * TY;
+    const int64_t j0 = (int64_t)bx * TY;
     const int64_t j = j0 + tid;
-    const int64_t i0 = kHalo + (int64_t)blockIdx.y * chunk;
+    const int64_t i0 = kHalo + (int64_t)by * chunk;
     const int64_t i1 = min(i0 + chunk, nx - kHalo);
     const bool col_ok = j < ny;
     const bool active = j >= kHalo && j < ny - kHalo;
@@ -341,6 +345,51 @@ __global__ __launch_bounds__(TY) __attribute__((amdgpu_waves_per_eu(5, 8))) void
     }
 }
 
+// at most 96 VGPRs (five waves per SIMD instead of four): measured 3-15 % faster across the
+// 128/256-lane shapes, for a few bytes of spill per lane
+template <int TY, bool UNIFORM, bool ISO, bool CODED>
+__global__ __launch_bounds__(TY) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_pressure_2d(DevSim batch, int chunk) {
+    extern __shared__ float lds_coef[];
+    pressure_2d_tile<TY, UNIFORM, ISO, CODED>(shot_view(batch, blockIdx.z), blockIdx.x, blockIdx.y, chunk, lds_coef);
+}
+
+// ---------------------------------------------------------------------------------------------
+// grouped launches: the tiles of several grids (different sizes and media, isotropic
+// relaxation) in one launch. Workgroup b belongs to the last entry with start <= b.
+
+__device__ inline int group_entry(const GroupTile* __restrict__ t, int n) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (t[mid].start <= blockIdx.x) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <int TY>
+__global__ __launch_bounds__(TY) void k_velocity_2d_group(const GroupTile* __restrict__ t, int n, int chunk) {
+    extern __shared__ float lds_coef[];
+    const int m = group_entry(t, n);
+    const DevSim s = t[m].s;
+    const unsigned local = blockIdx.x - t[m].start;
+    const unsigned bx = local % t[m].gx, by = local / t[m].gx;
+    if (s.rcode) velocity_2d_tile<TY, false, true, true>(s, bx, by, chunk, lds_coef);
+    else velocity_2d_tile<TY, false, true, false>(s, bx, by, chunk, lds_coef);
+}
+
+template <int TY>
+__global__ __launch_bounds__(TY) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_pressure_2d_group(
+    const GroupTile* __restrict__ t, int n, int chunk) {
+    extern __shared__ float lds_coef[];
+    const int m = group_entry(t, n);
+    const DevSim s = t[m].s;
+    const unsigned local = blockIdx.x - t[m].start;
+    const unsigned bx = local % t[m].gx, by = local / t[m].gx;
+    if (s.rcode) pressure_2d_tile<TY, false, true, true>(s, bx, by, chunk, lds_coef);
+    else pressure_2d_tile<TY, false, true, false>(s, bx, by, chunk, lds_coef);
+}
+
 template <int TY>
 hipError_t launch_2d(const DevSim& s, int chunk, bool velocity, hipStream_t st) {
     const bool u = s.ndmap == 1;
@@ -376,7 +425,28 @@ hipError_t launch_2d_any(const DevSim& s, const LaunchCfg& cfg, bool velocity, h
     }
 }
 
+template <int TY>
+hipError_t launch_group(const GroupTile* t, int n, uint32_t blocks, int chunk, int lds, bool velocity, hipStream_t st) {
+    if (velocity) hipLaunchKernelGGL((k_velocity_2d_group<TY>), dim3(blocks), dim3(TY), lds, st, t, n, chunk);
+    else hipLaunchKernelGGL((k_pressure_2d_group<TY>), dim3(blocks), dim3(TY), lds, st, t, n, chunk);
+    return hipGetLastError();
+}
+
 }  // namespace
+
+unsigned group_tiles_x(int64_t ny, int width) { return (unsigned)((ny + width - 1) / width); }
+unsigned group_tiles_y(int64_t nx, int chunk) { return (unsigned)((nx - 2 * kHalo + chunk - 1) / chunk); }
+
+hipError_t launch_group_2d(const GroupTile* t, int n, uint32_t blocks, int width, int chunk, int max_ndmap,
+                           bool velocity, hipStream_t st) {
+    if (n <= 0 || blocks == 0) return hipSuccess;
+    const int lds = max_ndmap * 2 * kNCoef * 4;
+    switch (width) {
+        case 64: return launch_group<64>(t, n, blocks, chunk, lds, velocity, st);
+        case 128: return launch_group<128>(t, n, blocks, chunk, lds, velocity, st);
+        default: return launch_group<256>(t, n, blocks, chunk, lds, velocity, st);
+    }
+}
 
 // the per-voxel coefficient table goes to LDS; beyond this many sets the 2D grids run the
 // per-cell kernels

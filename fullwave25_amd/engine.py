@@ -34,7 +34,8 @@ EXPORTED_SYMBOLS = (
     "fw_sim_record_count", "fw_sim_download_sensors", "fw_sim_download_field",
     "fw_sim_set_field", "fw_sim_kernel_config", "fw_sim_set_kernel_config", "fw_sim_set_graph_steps",
     "fw_sim_set_relaxation_codes", "fw_sim_relaxation_codes", "fw_sim_run_timed",
-    "fw_sim_destroy", "fw_adj_create", "fw_adj_forward", "fw_adj_backward", "fw_adj_destroy",
+    "fw_sim_destroy", "fw_group_create", "fw_group_run", "fw_group_destroy",
+    "fw_adj_create", "fw_adj_forward", "fw_adj_backward", "fw_adj_destroy",
 )
 VARIANTS = {"percell": 0, "stream": 1, "tile": 2, "pipe": 3, "pipelite": 4}
 
@@ -99,6 +100,9 @@ def load_library(path: Path | None = None) -> ctypes.CDLL:
         lib.fw_sim_relaxation_codes.argtypes = [vp, ctypes.POINTER(i32), ctypes.POINTER(ctypes.c_double)]
         lib.fw_sim_run_timed.argtypes = [vp, i64, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(i32)]
         lib.fw_sim_destroy.argtypes = [vp]
+        lib.fw_group_create.argtypes = [ctypes.POINTER(vp), i32, i32, i32, ctypes.POINTER(vp)]
+        lib.fw_group_run.argtypes = [vp, i64]
+        lib.fw_group_destroy.argtypes = [vp]
         lib.fw_adj_create.argtypes = [vp, i64, ctypes.POINTER(vp)]
         lib.fw_adj_forward.argtypes = [vp]
         lib.fw_adj_backward.argtypes = [vp, vp, sz, vp, vp, sz]
@@ -322,6 +326,50 @@ class HipSimulation:
             self._h = ctypes.c_void_p()
 
     def __enter__(self) -> HipSimulation:
+        return self
+
+    def __exit__(self, *exc) -> None:
+        self.close()
+
+    def __del__(self) -> None:
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class HipGroup:
+    """Several 2D isotropic simulations on one device (any grid sizes, media and shot counts)
+    advanced together: each time step is one velocity, one pressure and one source/sensor
+    launch over all of them (``fw_group_*``). Members keep their own records
+    (``member.sensors()``), bit-identical to ``member.run`` with the kernel shape
+    ``stream:<width>:<chunk>``. Members not yet run are pinned to that shape (no autotuning).
+    Members may be at different steps (a job that joins a running group) and may run alone
+    between group runs; they must stay open while the group exists."""
+
+    def __init__(self, sims: list[HipSimulation], *, width: int = 256, chunk: int = 16) -> None:
+        self.lib = load_library()
+        self.sims = list(sims)
+        self._h = ctypes.c_void_p()
+        shape = {"variant": "stream", "width": int(width), "chunk": int(chunk)}
+        for sim in self.sims:
+            try:
+                sim.set_kernel_config({"velocity": shape, "pressure": shape})
+            except EngineError as e:
+                if e.status != 5:  # FW_ERR_STATE: already set up, keeps its own kernels
+                    raise
+        arr = (ctypes.c_void_p * len(self.sims))(*[sim._h.value for sim in self.sims])
+        _check(self.lib, self.lib.fw_group_create(arr, len(self.sims), int(width), int(chunk), ctypes.byref(self._h)))
+
+    def run(self, n_steps: int) -> None:
+        _check(self.lib, self.lib.fw_group_run(self._h, int(n_steps)))
+
+    def close(self) -> None:
+        if self._h:
+            self.lib.fw_group_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __enter__(self) -> HipGroup:
         return self
 
     def __exit__(self, *exc) -> None:

@@ -168,6 +168,22 @@ int fw_sim_relaxation_codes(fw_sim* sim, int32_t* n_codes, double* coded_fractio
 int fw_sim_run_timed(fw_sim* sim, int64_t n_steps, double* ms, int32_t* n_kinds);
 int fw_sim_destroy(fw_sim* sim);
 
+/* ---- groups: several simulations advanced by shared launches ----------------------------- */
+/* No counterpart in the reference, which runs one simulation per process. A group holds
+ * handles of 2D isotropic relaxation simulations on one device, of any grid sizes and media
+ * (and any n_shots), each at its own step. fw_group_run advances every member by n_steps; each
+ * step is one velocity launch, one pressure launch and one source/air/sensor launch over the
+ * tiles of all members (x-marching kernels, `width` lanes of 64, 128 or 256 and `chunk` rows
+ * per workgroup), so many small or uneven jobs fill the GPU together. Each member's records are
+ * those of fw_sim_run with the same kernel shape, bit for bit; read them with
+ * fw_sim_download_sensors. Creating the group runs each member's first-run set-up. Between
+ * group runs a member may be run alone, reset or read; new maps make fw_group_run fail with
+ * FW_ERR_STATE. Destroy the group before destroying its members. */
+typedef struct fw_group fw_group;
+int fw_group_create(fw_sim* const* sims, int32_t n, int32_t width, int32_t chunk, fw_group** out);
+int fw_group_run(fw_group* group, int64_t n_steps);
+int fw_group_destroy(fw_group* group);
+
 /* ---- adjoint: gradients of a functional of the traces ----------------------------------- */
 /* None of these has a counterpart in the reference, which runs forward simulations only.
  * For J = sum_{n,j} w[n][j] * rec[n][j] (rec: the traces of a full run from rest, w: e.g. the

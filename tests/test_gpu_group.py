@@ -1,0 +1,143 @@
+"""Grouped launches: several 2D simulations of different sizes and media advanced by shared
+kernel launches (fw_group_*), against each simulation run on its own with the same kernel shape
+(bit for bit), and the state checks of the group API."""
+
+import numpy as np
+import pytest
+
+from fullwave25_amd import engine, workloads
+from tests.helpers import rel_l2
+
+pytestmark = pytest.mark.gpu
+SHAPE = "stream:256:16"
+
+
+def _media():
+    return [
+        workloads.random_medium_2d(60, 52, 200, seed=61, air=4).inputs,
+        workloads.plane_wave_2d(60, 280, 150, n_pml=10, sensor_depth=40).inputs,
+        workloads.random_medium_2d(150, 92, 260, seed=62, n_pml=7, n_transition=5).inputs,
+        workloads.random_medium_2d(37, 131, 180, seed=63, n_src=9, n_sensors=40).inputs,
+    ]
+
+
+def _alone(inp, n, monkeypatch, codes=True):
+    monkeypatch.setenv("FULLWAVE_HIP_KERNELS", SHAPE)
+    with engine.HipSimulation(inp) as sim:
+        sim.set_relaxation_codes(codes)
+        sim.run(n)
+        out = sim.sensors().copy()
+    monkeypatch.delenv("FULLWAVE_HIP_KERNELS")
+    return out
+
+
+def test_group_matches_members_run_alone(monkeypatch):
+    media = _media()
+    k = min(m.nt for m in media)
+    sims = [engine.HipSimulation(m) for m in media]
+    sims[2].set_relaxation_codes(False)  # one member on the per-cell maps, the rest coded
+    try:
+        with engine.HipGroup(sims, width=256, chunk=16) as grp:
+            grp.run(k // 2)
+            grp.run(k - k // 2)
+        got = [s.sensors().copy() for s in sims]
+        for i, (m, g) in enumerate(zip(media, got)):
+            want = _alone(m, k, monkeypatch, codes=(i != 2))
+            print(f"member {i}: shape {m.shape}, {k} steps, max |rec| {np.abs(g).max():.3e}", flush=True)
+            assert np.abs(want).max() > 0
+            np.testing.assert_array_equal(g, want)
+        # after the group, each member finishes its own run alone
+        for i, (m, s) in enumerate(zip(media, sims)):
+            s.run(m.nt - k)
+            np.testing.assert_array_equal(s.sensors(), _alone(m, m.nt, monkeypatch, codes=(i != 2)))
+    finally:
+        for s in sims:
+            s.close()
+
+
+def test_group_with_batched_member(monkeypatch):
+    a = workloads.random_medium_2d(60, 52, 120, seed=64).inputs
+    b = workloads.random_medium_2d(80, 60, 120, seed=65).inputs
+    rng = np.random.default_rng(66)
+    shots = (a.icmat[None] * (1 + rng.random((3, 1, 1)))).astype(np.float32)
+    sa = engine.HipSimulation(a, shots=shots)
+    sb = engine.HipSimulation(b)
+    try:
+        with engine.HipGroup([sa, sb]) as grp:
+            grp.run(a.nt)
+        monkeypatch.setenv("FULLWAVE_HIP_KERNELS", SHAPE)
+        with engine.HipSimulation(a, shots=shots) as alone:
+            alone.run(a.nt)
+            np.testing.assert_array_equal(sa.sensors(), alone.sensors())
+        monkeypatch.delenv("FULLWAVE_HIP_KERNELS")
+        np.testing.assert_array_equal(sb.sensors(), _alone(b, b.nt, monkeypatch))
+    finally:
+        sa.close()
+        sb.close()
+
+
+def test_group_state_checks():
+    m2 = workloads.random_medium_2d(40, 40, 50, seed=67).inputs
+    m3 = workloads.random_medium_3d(20, 18, 22, 20, seed=68).inputs
+    with engine.HipSimulation(m2) as a, engine.HipSimulation(m2) as b:
+        with pytest.raises(engine.EngineError, match="twice"):
+            engine.HipGroup([a, a])
+        with engine.HipSimulation(m3) as c, pytest.raises(engine.EngineError, match="2D isotropic"):
+            engine.HipGroup([a, c])
+        b.run(3)
+        grp = engine.HipGroup([a, b])
+        grp.run(5)
+        assert (a.step, b.step) == (5, 8)
+        with pytest.raises(engine.EngineError, match="steps left"):
+            grp.run(m2.nt - 7)
+        a.set_relaxation_codes(False)  # new coefficient layout: the group's view is stale
+        with pytest.raises(engine.EngineError, match="changed"):
+            grp.run(1)
+        grp.close()
+
+
+def test_members_at_different_steps(monkeypatch):
+    """Members join at different steps (a job that arrives while others run): each member's
+    records equal its own run alone, step for step."""
+    ma = workloads.random_medium_2d(60, 52, 200, seed=69).inputs
+    mb = workloads.random_medium_2d(44, 90, 160, seed=70).inputs
+    a, b = engine.HipSimulation(ma), engine.HipSimulation(mb)
+    try:
+        with engine.HipGroup([a, b]) as grp:
+            a.run(37)  # a runs ahead on its own, then both advance together
+            grp.run(60)
+            b.run(11)
+            grp.run(50)
+        assert (a.step, b.step) == (147, 121)
+        np.testing.assert_array_equal(a.sensors(), _alone_partial(ma, 147, monkeypatch))
+        np.testing.assert_array_equal(b.sensors(), _alone_partial(mb, 121, monkeypatch))
+    finally:
+        a.close()
+        b.close()
+
+
+def _alone_partial(inp, n, monkeypatch):
+    return _alone(inp, n, monkeypatch)
+
+
+def test_group_against_oracle():
+    """Two members of different sizes, grouped for the first 150 steps; the longer one then
+    finishes alone. Both full records against the FP64 oracle."""
+    import oracle
+
+    media = [workloads.random_medium_2d(60, 52, 300, seed=11, air=6).inputs,
+             workloads.random_medium_2d(37, 131, 150, seed=13, n_src=9, n_sensors=40).inputs]
+    sims = [engine.HipSimulation(m) for m in media]
+    try:
+        with engine.HipGroup(sims) as grp:
+            grp.run(150)
+        sims[0].run(150)
+        for m, s in zip(media, sims):
+            o = oracle.OracleSimulation(m, 64)
+            o.run(m.nt)
+            err = rel_l2(s.sensors().astype(np.float64), o.sensors())
+            print(f"{m.shape}: rel-L2 {err:.3e}", flush=True)
+            assert err <= 1e-4
+    finally:
+        for s in sims:
+            s.close()
