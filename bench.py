@@ -54,6 +54,34 @@ def algorithmic_bytes_per_cell(ndim: int, isotropic: bool, *, uniform: bool = Fa
     return {"velocity": round(vel, 2), "pressure": round(pre, 2)}
 
 
+def profiled_kernels(workload: str) -> str | None:
+    """The kernel configuration (FULLWAVE_HIP_KERNELS syntax) that the committed counter summary
+    of this workload measured, if there is one."""
+    path = ROOT / "profiles" / f"pmc_traffic_{workload}.json"
+    if not path.exists():
+        return None
+    from fullwave25_amd import engine
+
+    return engine.kernel_override_string(json.loads(path.read_text())["kernel_config"])
+
+
+def pin_kernels(args, workload: str, default_size: bool) -> str:
+    """Kernel selection for a bench run: ``--kernels SHAPES`` pins them, ``--kernels tune``
+    lets the engine's autotuner choose; by default the full-size workloads run the configuration
+    their committed counter summary measured (the autotuner's choices among shapes within a few
+    per cent of each other vary from run to run, and the traffic figure belongs to one
+    configuration), and anything else is autotuned. Returns how the kernels were chosen."""
+    if args.kernels and args.kernels != "tune":
+        return "pinned"
+    if not args.kernels and default_size:
+        k = profiled_kernels(workload)
+        if k:
+            os.environ["FULLWAVE_HIP_KERNELS"] = k
+            return "profiled"
+    os.environ.pop("FULLWAVE_HIP_KERNELS", None)
+    return "autotuned"
+
+
 def coefficient_layout(sim, inp) -> dict:
     """How the handle reads relaxation coefficients, for the byte accounting."""
     n, frac = sim.relaxation_coding()
@@ -180,12 +208,16 @@ def bench_grouped(args) -> dict:
 
     t0 = time.time()
     rng = np.random.default_rng(2028)
-    sizes = [(1024, 1024), (1536, 1024), (2048, 2048), (3072, 2048)]
-    media = {sz: workloads.focused_linear_array_2d(nx=sz[0], ny=sz[1], nt=6000).inputs for sz in sizes}
+    small = args.mix == "small"
+    sizes = [(256, 256), (384, 256), (512, 512), (768, 512)] if small else \
+        [(1024, 1024), (1536, 1024), (2048, 2048), (3072, 2048)]
+    unit = 250 if small else 1000  # jobs run 3..6 units of steps
+    media = {sz: workloads.focused_linear_array_2d(nx=sz[0], ny=sz[1], nt=6 * unit,
+                                                   n_elements=sz[1] // 12 if small else 128).inputs for sz in sizes}
     jobs = []
     for _ in range(args.jobs):
         sz = sizes[int(rng.integers(len(sizes)))]
-        jobs.append(dataclasses.replace(media[sz], nt=int(rng.integers(3, 7)) * 1000))
+        jobs.append(dataclasses.replace(media[sz], nt=int(rng.integers(3, 7)) * unit))
     print(f"# grouped: {len(jobs)} jobs built in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
     with engine.HipSimulation(jobs[0]) as warm:  # first-use costs (library load, code objects)
         warm.run(10)
@@ -222,9 +254,11 @@ def bench_grouped(args) -> dict:
         "speedup_vs_sequential": round(t_seq / t_grp, 3),
         "glups_grouped": round(cells / t_grp / 1e9, 3), "glups_sequential": round(cells / t_seq / 1e9, 3),
         "max_rel_l2_grouped_vs_sequential": worst,
-        "config": {"workload": f"{len(jobs)} focused-transmit jobs on 1024^2, 1536x1024, 2048^2 and 3072x2048 "
-                               f"interiors, 3000-6000 steps; group kernel stream:{args.group_width}:{args.group_chunk}",
-                   "jobs": len(jobs)},
+        "config": {"workload": f"{len(jobs)} focused-transmit jobs on "
+                               + ", ".join(f"{a}x{b}" for a, b in sizes)
+                               + f" interiors, {3 * unit}-{6 * unit} steps; group kernel "
+                               f"stream:{args.group_width}:{args.group_chunk}",
+                   "mix": args.mix, "jobs": len(jobs)},
     }
 
 
@@ -240,6 +274,7 @@ def bench_3d(args) -> dict:
     inp = wl.inputs
     print(f"# {wl.name} {inp.shape} built in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
     steps = min(args.steps, inp.nt)
+    source = pin_kernels(args, wl.name, (args.nx, args.ny, args.nz) == (768, 768, 640))
     with engine.HipSimulation(inp) as sim:
         kcfg = sim.kernel_config()
         layout = coefficient_layout(sim, inp)
@@ -272,7 +307,7 @@ def bench_3d(args) -> dict:
                      "traffic": measured_traffic(wl.name, dominant, kcfg, layout["codes"]),
                      "bytes_per_cell": bpc[dominant], "avg_launch_us": round(avg_s * 1e6, 2),
                      "kernel_ms_per_step": {k: round(v / steps, 4) for k, v in kms.items()},
-                     "kernel_config": kcfg, "kernels": engine.kernel_override_string(kcfg),
+                     "kernel_config": kcfg, "kernels": engine.kernel_override_string(kcfg), "kernel_source": source,
                      "relaxation_codes": layout["codes"], "coded_fraction": layout["coded_fraction"],
                      "step_bytes_per_lup": STEP_BYTES_PER_LUP[3],
                      "step_frac": round(inp.ncells * steps / elapsed * STEP_BYTES_PER_LUP[3] / 1e9 / HBM_PEAK_GBS, 4)},
@@ -424,7 +459,8 @@ def main() -> int:
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run the plain copy kernel over this many bytes (counter calibration)")
     ap.add_argument("--kernels", default="",
-                    help="pin the kernel configuration (FULLWAVE_HIP_KERNELS syntax) instead of autotuning")
+                    help="pin the kernel configuration (FULLWAVE_HIP_KERNELS syntax), or 'tune' to autotune; "
+                         "default: the profiled configuration for the full-size workloads, else autotune")
     ap.add_argument("--pmc-summary", nargs=5, metavar=("FETCH_DIR", "WRITE_DIR", "COPY_FETCH_DIR", "COPY_WRITE_DIR",
                                                        "CONFIG_JSON"),
                     help="summarize rocprofv3 --pmc runs into per-launch HBM traffic (JSON) and exit")
@@ -435,11 +471,11 @@ def main() -> int:
     ap.add_argument("--group-chunk", type=int, default=16, help="grouped_2d: rows per workgroup")
     ap.add_argument("--slots", type=int, default=4, help="served_mix: concurrent jobs")
     ap.add_argument("--mix", choices=("standard", "small"), default="standard",
-                    help="served_mix: job sizes (small: none fills the GPU alone)")
+                    help="served_mix, grouped_2d: job sizes (small: none fills the GPU alone)")
     ap.add_argument("--arrival-factor", type=float, default=2.0,
                     help="served_mix: arrival rate as a multiple of the sequential service rate")
     args = ap.parse_args()
-    if args.kernels:
+    if args.kernels and args.kernels != "tune":
         os.environ["FULLWAVE_HIP_KERNELS"] = args.kernels
     if args.config in ("batched_2d", "served_mix", "hetero_3d", "grouped_2d"):
         if args.gpus != 1:
@@ -493,6 +529,7 @@ def main() -> int:
                               "GBs": {k: round(v, 1) for k, v in gbs.items()}}), flush=True)
         os.environ.pop("FULLWAVE_HIP_KERNELS", None)
         return 0
+    source = pin_kernels(args, wl.name, args.config == "focused_2d" and (args.nx, args.ny) == (4096, 2048))
     sim = engine.HipSimulation(inp, device=0)
     kcfg = sim.kernel_config()
     layout = coefficient_layout(sim, inp)
@@ -560,6 +597,7 @@ def main() -> int:
             "copy_kernel_gbs": round(copy_gbs, 1),
             "kernel_config": kcfg,
             "kernels": engine.kernel_override_string(kcfg),
+            "kernel_source": source,
             "relaxation_codes": layout["codes"],
             "coded_fraction": layout["coded_fraction"],
             "step_bytes_per_lup": STEP_BYTES_PER_LUP[inp.ndim],

@@ -10,9 +10,17 @@ engine handle, which owns its own HIP stream, from its own worker thread, so the
 concurrent jobs overlap on the device. Inputs stay resident in HBM from admission to
 completion; the host only moves the inputs in and the traces out.
 
+Continuous batching (``batching=True``, the default): 2D isotropic forward jobs do not run
+their own launch loop. Once a job's handle holds its inputs, it joins a shared group
+(:class:`~fullwave25_amd.engine.HipGroup`), and a batcher thread advances the whole group
+``quantum_steps`` at a time with one velocity, one pressure and one source/sensor launch per
+step for all members; jobs that arrive meanwhile join at the next quantum, at their own step 0,
+and finished jobs leave. A grouped job's traces are bit-identical to its own run with the
+group's kernel shape.
+
 Kernel shapes: the first handle of each grid shape times the candidates (the engine's
 autotuner); later jobs of that shape reuse the choice, so a job's numbers do not depend on
-what else is running.
+what else is running. Grouped jobs use the group's shape (``stream:256:16``).
 
 Latency of a job is measured from ``submit`` to its result being available.
 """
@@ -90,6 +98,9 @@ class ServerStats:
     records: list = field(repr=False, default_factory=list)
 
 
+GROUP_WIDTH, GROUP_CHUNK = 256, 16  # kernel shape of grouped jobs
+
+
 def _shape_key(inp: SimulationInputs, n_shots: int) -> tuple:
     return (inp.ndim, inp.shape, bool(inp.isotropic), bool(inp.exponential), inp.ndmap == 1, n_shots)
 
@@ -103,8 +114,15 @@ class SimulationServer:
     """
 
     def __init__(self, device: int = 0, *, hbm_budget: int | None = None, max_slots: int = 8,
-                 chunk_steps: int = 500, max_bypass: int = 8, runner=None) -> None:
+                 chunk_steps: int = 500, max_bypass: int = 8, runner=None, batching: bool = True,
+                 quantum_steps: int = 250) -> None:
         self.device = device
+        self.batching = bool(batching) and runner is None
+        self.quantum_steps = int(quantum_steps)
+        self._batch_lock = threading.Condition()
+        self._batch_in: list = []
+        self._batch_closed = False
+        self._batcher = None
         # runner(server, inputs, shots) -> traces; the engine by default (tests of the
         # scheduling logic substitute a host-only one)
         self._runner = runner if runner is not None else SimulationServer._run_engine
@@ -130,6 +148,9 @@ class SimulationServer:
         self._t0 = time.perf_counter()
         self._scheduler = threading.Thread(target=self._schedule, name="fullwave-scheduler", daemon=True)
         self._scheduler.start()
+        if self.batching:
+            self._batcher = threading.Thread(target=self._batch_loop, name="fullwave-batcher", daemon=True)
+            self._batcher.start()
 
     # ---- public ---------------------------------------------------------------------------
 
@@ -185,6 +206,11 @@ class SimulationServer:
         self._scheduler.join()
         for w in self._workers:
             w.join()
+        if self._batcher is not None:
+            with self._batch_lock:
+                self._batch_closed = True
+                self._batch_lock.notify_all()
+            self._batcher.join()
 
     def __enter__(self) -> SimulationServer:
         return self
@@ -255,8 +281,38 @@ class SimulationServer:
             else:
                 self._configs[key] = sim.kernel_config()  # tunes now, on this handle's arrays
 
+    def kernel_config_for(self, inp: SimulationInputs, n_shots: int = 1) -> dict | None:
+        """The kernel configuration this server runs (or ran) jobs of this shape with: the group
+        shape for grouped jobs, else the one tuned on the first handle of the shape (None if no
+        such job has run yet)."""
+        if self.batching and self._groupable(inp):
+            shape = {"variant": "stream", "width": GROUP_WIDTH, "chunk": GROUP_CHUNK}
+            return {"velocity": shape, "pressure": shape}
+        with self._lock:
+            return self._configs.get(_shape_key(inp, n_shots))
+
+    @staticmethod
+    def _groupable(inp: SimulationInputs) -> bool:
+        """Jobs the grouped kernels take: 2D isotropic relaxation, coefficient sets in LDS."""
+        return (inp.ndim == 2 and bool(inp.isotropic) and not bool(inp.exponential) and inp.ndmap <= 1024
+                and inp.ncells < (1 << 30))
+
     def _run_engine(self, inp: SimulationInputs, shots: np.ndarray | None) -> np.ndarray:
         n_shots = 1 if shots is None else int(shots.shape[0])
+        if self.batching and self._groupable(inp):
+            with engine.HipSimulation(inp, self.device, shots=shots) as sim:
+                shape = {"variant": "stream", "width": GROUP_WIDTH, "chunk": GROUP_CHUNK}
+                sim.set_kernel_config({"velocity": shape, "pressure": shape})
+                sim.relaxation_codes()  # first-run set-up here, on this worker's thread
+                done = threading.Event()
+                failure: list = []
+                with self._batch_lock:
+                    self._batch_in.append((sim, inp.nt, done, failure))
+                    self._batch_lock.notify_all()
+                done.wait()
+                if failure:
+                    raise failure[0]
+                return sim.sensors()
         with engine.HipSimulation(inp, self.device, shots=shots) as sim:
             self._kernel_config(_shape_key(inp, n_shots), sim)
             left = inp.nt
@@ -265,6 +321,47 @@ class SimulationServer:
                 sim.run(n)
                 left -= n
             return sim.sensors()
+
+    def _batch_loop(self) -> None:
+        """Advance the grouped jobs: a quantum of steps (or fewer, up to the next job to finish)
+        per round; arrivals join and finished jobs leave between rounds."""
+        members: list = []  # [sim, steps left, done event, failure list]
+        group = None
+        while True:
+            with self._batch_lock:
+                while not self._batch_in and not members:
+                    if self._batch_closed:
+                        return
+                    self._batch_lock.wait()
+                new, self._batch_in = self._batch_in, []
+            if new:
+                members += [list(m) for m in new]
+                if group is not None:
+                    group.close()
+                    group = None
+            try:
+                if group is None:
+                    group = engine.HipGroup([m[0] for m in members], width=GROUP_WIDTH, chunk=GROUP_CHUNK)
+                q = min(self.quantum_steps, min(m[1] for m in members))
+                group.run(q)
+            except Exception as exc:  # every member of the failed round fails; the server goes on
+                if group is not None:
+                    group.close()
+                    group = None
+                for m in members:
+                    m[3].append(exc)
+                    m[2].set()
+                members = []
+                continue
+            for m in members:
+                m[1] -= q
+            finished = [m for m in members if m[1] == 0]
+            if finished:
+                group.close()
+                group = None
+                members = [m for m in members if m[1] > 0]
+                for m in finished:
+                    m[2].set()
 
     def _run_adjoint(self, job: _Job) -> dict:
         from .adjoint import HipAdjoint

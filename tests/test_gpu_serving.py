@@ -10,7 +10,7 @@ import pytest
 
 import oracle
 from fullwave25_amd import engine, workloads
-from fullwave25_amd.serving import SimulationServer, _shape_key
+from fullwave25_amd.serving import SimulationServer
 from tests.helpers import rel_l2
 
 pytestmark = pytest.mark.gpu
@@ -27,21 +27,24 @@ def _jobs():
     return [(a, None), (b, None), (c, None), (d, None), (a, shots), (b, None), (c, None), (d, None), (a, None)]
 
 
-def test_served_mix_equals_alone_and_oracle():
+@pytest.mark.parametrize("batching", [True, False])
+def test_served_mix_equals_alone_and_oracle(batching):
+    """With batching, the 2D isotropic jobs (a, b and the batch on a's medium) run as members
+    of the server's shared group while the 3D and exponential jobs run on their own handles."""
     jobs = _jobs()
     budget = 3 * max(engine.device_bytes(inp, 1 if s is None else s.shape[0]) for inp, s in jobs)
-    with SimulationServer(hbm_budget=budget, max_slots=4, chunk_steps=64) as srv:
+    with SimulationServer(hbm_budget=budget, max_slots=4, chunk_steps=64, batching=batching,
+                          quantum_steps=48) as srv:
         futs = [srv.submit(inp, shots=s) for inp, s in jobs]
         results = [f.result(timeout=300) for f in futs]
-        configs = dict(srv._configs)
+        configs = [srv.kernel_config_for(inp, 1 if s is None else s.shape[0]) for inp, s in jobs]
     st = srv.stats()
     print(f"served {st.jobs} jobs: p50 {st.latency_p50_s * 1e3:.1f} ms, p99 {st.latency_p99_s * 1e3:.1f} ms, "
           f"peak running {srv.peak_running}, peak bytes {srv.peak_in_use}", flush=True)
     assert st.failed == 0 and srv.peak_in_use <= budget and srv.peak_running >= 2
-    for (inp, shots), got in zip(jobs, results):
-        n_shots = 1 if shots is None else shots.shape[0]
+    for (inp, shots), got, cfg in zip(jobs, results, configs):
         with engine.HipSimulation(inp, shots=shots) as alone:
-            alone.set_kernel_config(configs[_shape_key(inp, n_shots)])
+            alone.set_kernel_config(cfg)
             alone.run(inp.nt)
             np.testing.assert_array_equal(got, alone.sensors())
     for inp, shots in jobs[:4]:
