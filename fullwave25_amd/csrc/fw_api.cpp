@@ -69,18 +69,73 @@ int64_t row_pitch(const fw_desc& d) { return d.ndim == 3 ? (d.nz + 31) / 32 * 32
 // kernels run on that (non-blocking) stream, which does not order itself after work on the
 // null stream, so a copy issued there could still be in flight when a kernel reads the data.
 
+// Grids whose device rows are padded (3D) move through a device staging buffer: host <->
+// staging as one contiguous copy, staging <-> the padded rows as a device-side pitched copy. A
+// pitched copy straight from pageable host memory moves row by row, at a few hundred MB/s.
+constexpr size_t kStageBytes = size_t(64) << 20;
+
+struct Stage {
+    void* p = nullptr;
+    bool async = false;
+    hipError_t get(const fw_sim* s, size_t bytes) {
+        hipError_t e = hipMallocAsync(&p, bytes, s->stream);
+        if (e == hipSuccess) {
+            async = true;
+            return e;
+        }
+        (void)hipGetLastError();
+        return hipMalloc(&p, bytes);
+    }
+    hipError_t release(const fw_sim* s) {
+        if (!p) return hipSuccess;
+        if (async) return hipFreeAsync(p, s->stream);
+        hipError_t e = hipStreamSynchronize(s->stream);
+        hipError_t e2 = hipFree(p);
+        return e != hipSuccess ? e : e2;
+    }
+};
+
 // compact host grid -> padded device grid (elements of `esize` bytes)
 hipError_t copy_grid_in(const fw_sim* s, void* dst, const void* src, size_t esize) {
-    hipError_t e = hipMemcpy2DAsync(dst, (size_t)s->pitch * esize, src, (size_t)s->row_len * esize,
-                                    (size_t)s->row_len * esize, (size_t)s->rows, hipMemcpyHostToDevice, s->stream);
-    return e != hipSuccess ? e : hipStreamSynchronize(s->stream);
+    const size_t row = (size_t)s->row_len * esize, prow = (size_t)s->pitch * esize, rows = (size_t)s->rows;
+    hipError_t e;
+    if (row == prow) {
+        e = hipMemcpyAsync(dst, src, row * rows, hipMemcpyHostToDevice, s->stream);
+        return e != hipSuccess ? e : hipStreamSynchronize(s->stream);
+    }
+    const size_t per = kStageBytes / row > 0 ? kStageBytes / row : 1;
+    Stage st;
+    e = st.get(s, per * row);
+    for (size_t r0 = 0; r0 < rows && e == hipSuccess; r0 += per) {
+        const size_t n = per < rows - r0 ? per : rows - r0;
+        e = hipMemcpyAsync(st.p, (const char*)src + r0 * row, n * row, hipMemcpyHostToDevice, s->stream);
+        if (e == hipSuccess)
+            e = hipMemcpy2DAsync((char*)dst + r0 * prow, prow, st.p, row, row, n, hipMemcpyDeviceToDevice, s->stream);
+    }
+    const hipError_t e2 = st.release(s);
+    const hipError_t e3 = hipStreamSynchronize(s->stream);
+    return e != hipSuccess ? e : e2 != hipSuccess ? e2 : e3;
 }
 
 // padded device grid -> compact host grid
 hipError_t copy_grid_out(const fw_sim* s, void* dst, const void* src, size_t esize) {
-    hipError_t e = hipMemcpy2DAsync(dst, (size_t)s->row_len * esize, src, (size_t)s->pitch * esize,
-                                    (size_t)s->row_len * esize, (size_t)s->rows, hipMemcpyDeviceToHost, s->stream);
-    return e != hipSuccess ? e : hipStreamSynchronize(s->stream);
+    const size_t row = (size_t)s->row_len * esize, prow = (size_t)s->pitch * esize, rows = (size_t)s->rows;
+    hipError_t e;
+    if (row == prow) {
+        e = hipMemcpyAsync(dst, src, row * rows, hipMemcpyDeviceToHost, s->stream);
+        return e != hipSuccess ? e : hipStreamSynchronize(s->stream);
+    }
+    const size_t per = kStageBytes / row > 0 ? kStageBytes / row : 1;
+    Stage st;
+    e = st.get(s, per * row);
+    for (size_t r0 = 0; r0 < rows && e == hipSuccess; r0 += per) {
+        const size_t n = per < rows - r0 ? per : rows - r0;
+        e = hipMemcpy2DAsync(st.p, row, (const char*)src + r0 * prow, prow, row, n, hipMemcpyDeviceToDevice, s->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync((char*)dst + r0 * row, st.p, n * row, hipMemcpyDeviceToHost, s->stream);
+    }
+    const hipError_t e2 = st.release(s);
+    const hipError_t e3 = hipStreamSynchronize(s->stream);
+    return e != hipSuccess ? e : e2 != hipSuccess ? e2 : e3;
 }
 
 hipError_t copy_sync(const fw_sim* s, void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
@@ -381,7 +436,8 @@ int encode_relaxation(fw_sim* s) {
     for (int k = 0; k < fw::kCodeVals - 1; ++k) m.f[k] = (const float*)s->dev[names[k]];
     m.dcmap = (const int32_t*)s->dev["dcmap"];
 
-    size_t nslots = fw::kCodeSlotsSmall;
+    // large grids (3D) hold more tuples than the small table takes: start with the large one
+    size_t nslots = s->ncell > ((int64_t)1 << 24) ? fw::kCodeSlots : fw::kCodeSlotsSmall;
     unsigned long long* keys = nullptr;
     uint32_t* count = nullptr;
     int32_t* rep = nullptr;

@@ -61,12 +61,19 @@ int build(fw_group* g) {
     uint64_t blocks = 0;
     for (fw_sim* s : g->sims) {
         const unsigned gx = fw::group_tiles_x(s->ds.ny, g->width);
-        const unsigned gy = fw::group_tiles_y(s->ds.nx, g->chunk);
+        // a grid too small to give ~128 workgroups at the group's row count takes fewer rows
+        // per workgroup (no fewer than 2), so that its tiles do not march alone at the end
+        const int64_t rows = s->ds.nx - 2 * fw::kHalo;
+        int64_t chunk = rows * gx / 128;
+        if (chunk > g->chunk) chunk = g->chunk;
+        if (chunk < 2) chunk = 2;
+        const unsigned gy = fw::group_tiles_y(s->ds.nx, (int)chunk);
         for (int b = 0; b < s->nshots; ++b) {
             fw::GroupTile t{};
             t.s = fw::shot_view(s->ds, b);
             t.start = (uint32_t)blocks;
             t.gx = gx;
+            t.chunk = (int32_t)chunk;
             tiles.push_back(t);
             blocks += (uint64_t)gx * gy;
             fw::GroupBoundary e{};
@@ -165,8 +172,8 @@ int fw_group_run(fw_group* g, int64_t n_steps) {
     GR_HIP(hipMemcpyAsync(g->bounds, g->bounds_h.data(), g->bounds_h.size() * sizeof(fw::GroupBoundary),
                           hipMemcpyHostToDevice, g->stream));
     for (int64_t it = 0; it < n_steps; ++it) {
-        GR_HIP(fw::launch_group_2d(g->tiles, g->ntiles, g->blocks, g->width, g->chunk, g->max_ndmap, true, g->stream));
-        GR_HIP(fw::launch_group_2d(g->tiles, g->ntiles, g->blocks, g->width, g->chunk, g->max_ndmap, false, g->stream));
+        GR_HIP(fw::launch_group_2d(g->tiles, g->ntiles, g->blocks, g->width, g->max_ndmap, true, g->stream));
+        GR_HIP(fw::launch_group_2d(g->tiles, g->ntiles, g->blocks, g->width, g->max_ndmap, false, g->stream));
         GR_HIP(fw::launch_group_boundary(g->bounds, g->ntiles, g->max_points, it, g->stream));
     }
     GR_HIP(hipStreamSynchronize(g->stream));

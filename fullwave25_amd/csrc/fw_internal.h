@@ -182,6 +182,7 @@ struct GroupTile {
     DevSim s;
     uint32_t start;
     uint32_t gx;
+    int32_t chunk;  // rows per workgroup for this grid
 };
 
 // One grid (or shot) of a grouped source/air/sensor pass.
@@ -222,8 +223,8 @@ hipError_t launch_tick(int64_t* step, int64_t set, hipStream_t st);
 // grouped 2D x-marching kernels (isotropic relaxation); `width` 64/128/256 lanes, `chunk` rows
 unsigned group_tiles_x(int64_t ny, int width);
 unsigned group_tiles_y(int64_t nx, int chunk);
-hipError_t launch_group_2d(const GroupTile* t, int n, uint32_t blocks, int width, int chunk, int max_ndmap,
-                           bool velocity, hipStream_t st);
+hipError_t launch_group_2d(const GroupTile* t, int n, uint32_t blocks, int width, int max_ndmap, bool velocity,
+                           hipStream_t st);
 hipError_t launch_group_boundary(const GroupBoundary* g, int n, int64_t max_points, int64_t it, hipStream_t st);
 hipError_t launch_reciprocal(float* data, int64_t n, hipStream_t st);
 hipError_t launch_copy(const void* src, void* dst, int64_t n16, hipStream_t st);

@@ -14,7 +14,8 @@
 //   1. insert: every cell hashes its tuple (the raw bits of the eleven words) into an
 //      open-addressing table of 64-bit keys (linear probing, bounded probe count) and counts
 //      it; the lane whose compare-and-swap claims an empty slot records its cell as the slot's
-//      representative (a table of 2^16 slots first, 2^22 if that one overflows);
+//      representative (for grids up to 2^24 cells a table of 2^16 slots first, 2^22 if that one
+//      overflows);
 //   2. the host numbers the most frequent occupied slots 0 .. n-1 (n <= 65535);
 //   3. assign: every cell finds its key again and stores the slot's code (or the escape);
 //   4. table: each code copies its representative cell's eleven words into the tables;

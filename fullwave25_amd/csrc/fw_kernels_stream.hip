@@ -368,10 +368,11 @@ __device__ inline int group_entry(const GroupTile* __restrict__ t, int n) {
 }
 
 template <int TY>
-__global__ __launch_bounds__(TY) void k_velocity_2d_group(const GroupTile* __restrict__ t, int n, int chunk) {
+__global__ __launch_bounds__(TY) void k_velocity_2d_group(const GroupTile* __restrict__ t, int n) {
     extern __shared__ float lds_coef[];
     const int m = group_entry(t, n);
     const DevSim s = t[m].s;
+    const int chunk = t[m].chunk;
     const unsigned local = blockIdx.x - t[m].start;
     const unsigned bx = local % t[m].gx, by = local / t[m].gx;
     if (s.rcode) velocity_2d_tile<TY, false, true, true>(s, bx, by, chunk, lds_coef);
@@ -380,10 +381,11 @@ __global__ __launch_bounds__(TY) void k_velocity_2d_group(const GroupTile* __res
 
 template <int TY>
 __global__ __launch_bounds__(TY) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_pressure_2d_group(
-    const GroupTile* __restrict__ t, int n, int chunk) {
+    const GroupTile* __restrict__ t, int n) {
     extern __shared__ float lds_coef[];
     const int m = group_entry(t, n);
     const DevSim s = t[m].s;
+    const int chunk = t[m].chunk;
     const unsigned local = blockIdx.x - t[m].start;
     const unsigned bx = local % t[m].gx, by = local / t[m].gx;
     if (s.rcode) pressure_2d_tile<TY, false, true, true>(s, bx, by, chunk, lds_coef);
@@ -426,9 +428,9 @@ hipError_t launch_2d_any(const DevSim& s, const LaunchCfg& cfg, bool velocity, h
 }
 
 template <int TY>
-hipError_t launch_group(const GroupTile* t, int n, uint32_t blocks, int chunk, int lds, bool velocity, hipStream_t st) {
-    if (velocity) hipLaunchKernelGGL((k_velocity_2d_group<TY>), dim3(blocks), dim3(TY), lds, st, t, n, chunk);
-    else hipLaunchKernelGGL((k_pressure_2d_group<TY>), dim3(blocks), dim3(TY), lds, st, t, n, chunk);
+hipError_t launch_group(const GroupTile* t, int n, uint32_t blocks, int lds, bool velocity, hipStream_t st) {
+    if (velocity) hipLaunchKernelGGL((k_velocity_2d_group<TY>), dim3(blocks), dim3(TY), lds, st, t, n);
+    else hipLaunchKernelGGL((k_pressure_2d_group<TY>), dim3(blocks), dim3(TY), lds, st, t, n);
     return hipGetLastError();
 }
 
@@ -437,14 +439,14 @@ hipError_t launch_group(const GroupTile* t, int n, uint32_t blocks, int chunk, i
 unsigned group_tiles_x(int64_t ny, int width) { return (unsigned)((ny + width - 1) / width); }
 unsigned group_tiles_y(int64_t nx, int chunk) { return (unsigned)((nx - 2 * kHalo + chunk - 1) / chunk); }
 
-hipError_t launch_group_2d(const GroupTile* t, int n, uint32_t blocks, int width, int chunk, int max_ndmap,
-                           bool velocity, hipStream_t st) {
+hipError_t launch_group_2d(const GroupTile* t, int n, uint32_t blocks, int width, int max_ndmap, bool velocity,
+                           hipStream_t st) {
     if (n <= 0 || blocks == 0) return hipSuccess;
     const int lds = max_ndmap * 2 * kNCoef * 4;
     switch (width) {
-        case 64: return launch_group<64>(t, n, blocks, chunk, lds, velocity, st);
-        case 128: return launch_group<128>(t, n, blocks, chunk, lds, velocity, st);
-        default: return launch_group<256>(t, n, blocks, chunk, lds, velocity, st);
+        case 64: return launch_group<64>(t, n, blocks, lds, velocity, st);
+        case 128: return launch_group<128>(t, n, blocks, lds, velocity, st);
+        default: return launch_group<256>(t, n, blocks, lds, velocity, st);
     }
 }
 

@@ -174,8 +174,9 @@ int fw_sim_destroy(fw_sim* sim);
  * (and any n_shots), each at its own step. fw_group_run advances every member by n_steps; each
  * step is one velocity launch, one pressure launch and one source/air/sensor launch over the
  * tiles of all members (x-marching kernels, `width` lanes of 64, 128 or 256 and `chunk` rows
- * per workgroup), so many small or uneven jobs fill the GPU together. Each member's records are
- * those of fw_sim_run with the same kernel shape, bit for bit; read them with
+ * per workgroup; a member too small to give about 128 workgroups takes fewer rows, down to 2),
+ * so many small or uneven jobs fill the GPU together. Each member's records are those of
+ * fw_sim_run with the same kernel shape, bit for bit; read them with
  * fw_sim_download_sensors. Creating the group runs each member's first-run set-up. Between
  * group runs a member may be run alone, reset or read; new maps make fw_group_run fail with
  * FW_ERR_STATE. Destroy the group before destroying its members. */
