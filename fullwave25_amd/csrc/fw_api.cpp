@@ -87,10 +87,12 @@ struct Stage {
         return hipMalloc(&p, bytes);
     }
     hipError_t release(const fw_sim* s) {
-        if (!p) return hipSuccess;
-        if (async) return hipFreeAsync(p, s->stream);
+        void* q = p;
+        p = nullptr;
+        if (!q) return hipSuccess;
+        if (async) return hipFreeAsync(q, s->stream);
         hipError_t e = hipStreamSynchronize(s->stream);
-        hipError_t e2 = hipFree(p);
+        hipError_t e2 = hipFree(q);
         return e != hipSuccess ? e : e2;
     }
 };
@@ -438,6 +440,9 @@ int encode_relaxation(fw_sim* s) {
 
     // large grids (3D) hold more tuples than the small table takes: start with the large one
     size_t nslots = s->ncell > ((int64_t)1 << 24) ? fw::kCodeSlots : fw::kCodeSlotsSmall;
+    // temporaries are stream-ordered (no device-wide wait when they are freed, which would
+    // stall a server's other jobs)
+    Stage keys_b, count_b, rep_b, status_b, slot_b, repc_b;
     unsigned long long* keys = nullptr;
     uint32_t* count = nullptr;
     int32_t* rep = nullptr;
@@ -445,19 +450,21 @@ int encode_relaxation(fw_sim* s) {
     uint16_t* slot_code_d = nullptr;
     int32_t* rep_of_code_d = nullptr;
     auto free_table = [&] {
-        for (void* q : {(void*)keys, (void*)count, (void*)rep})
-            if (q) (void)hipFree(q);
-        keys = nullptr;
-        count = nullptr;
-        rep = nullptr;
+        (void)keys_b.release(s);
+        (void)count_b.release(s);
+        (void)rep_b.release(s);
     };
     auto run = [&]() -> int {
-        FW_HIP(hipMalloc((void**)&status, 16));
+        FW_HIP(status_b.get(s, 16));
+        status = (int32_t*)status_b.p;
         int32_t st[4] = {0, 0, 0, 0};
         for (;;) {
-            FW_HIP(hipMalloc((void**)&keys, nslots * 8));
-            FW_HIP(hipMalloc((void**)&count, nslots * 4));
-            FW_HIP(hipMalloc((void**)&rep, nslots * 4));
+            FW_HIP(keys_b.get(s, nslots * 8));
+            FW_HIP(count_b.get(s, nslots * 4));
+            FW_HIP(rep_b.get(s, nslots * 4));
+            keys = (unsigned long long*)keys_b.p;
+            count = (uint32_t*)count_b.p;
+            rep = (int32_t*)rep_b.p;
             FW_HIP(hipMemsetAsync(keys, 0, nslots * 8, s->stream));
             FW_HIP(hipMemsetAsync(count, 0, nslots * 4, s->stream));
             FW_HIP(hipMemsetAsync(status, 0, 16, s->stream));
@@ -493,8 +500,10 @@ int encode_relaxation(fw_sim* s) {
             rep_of_code[i] = hr[occupied[i]];
         }
         const int32_t n = (int32_t)occupied.size();
-        FW_HIP(hipMalloc((void**)&slot_code_d, nslots * 2));
-        FW_HIP(hipMalloc((void**)&rep_of_code_d, (size_t)n * 4));
+        FW_HIP(slot_b.get(s, nslots * 2));
+        FW_HIP(repc_b.get(s, (size_t)n * 4));
+        slot_code_d = (uint16_t*)slot_b.p;
+        rep_of_code_d = (int32_t*)repc_b.p;
         FW_HIP(copy_sync(s, slot_code_d, slot_code.data(), nslots * 2, hipMemcpyHostToDevice));
         FW_HIP(copy_sync(s, rep_of_code_d, rep_of_code.data(), (size_t)n * 4, hipMemcpyHostToDevice));
         void *code, *tv, *tp;
@@ -518,9 +527,12 @@ int encode_relaxation(fw_sim* s) {
     };
     const int rc = run();
     free_table();
-    for (void* q : {(void*)status, (void*)slot_code_d, (void*)rep_of_code_d})
-        if (q) (void)hipFree(q);
+    (void)status_b.release(s);
+    (void)slot_b.release(s);
+    (void)repc_b.release(s);
+    const hipError_t e = hipStreamSynchronize(s->stream);
     if (rc != FW_OK || s->ncodes == 0) drop_codes(s);
+    if (rc == FW_OK && e != hipSuccess) return fail(FW_ERR_DEVICE, "coding: %s", hipGetErrorString(e));
     return rc;
 }
 

@@ -46,12 +46,32 @@ struct Guard {
     }
 };
 
+// the group's tables are stream-ordered allocations on its own stream: freeing them does not
+// wait for the whole device (other handles' work)
+void free_table(fw_group* g, void* p) {
+    if (!p) return;
+    if (hipFreeAsync(p, g->stream) != hipSuccess) {
+        (void)hipGetLastError();
+        (void)hipStreamSynchronize(g->stream);
+        (void)hipFree(p);
+    }
+}
+
+hipError_t alloc_table(fw_group* g, void** p, size_t bytes) {
+    hipError_t e = hipMallocAsync(p, bytes, g->stream);
+    if (e == hipSuccess) return e;
+    (void)hipGetLastError();
+    return hipMalloc(p, bytes);
+}
+
 void release(fw_group* g) {
     if (!g) return;
-    if (g->stream) (void)hipStreamSynchronize(g->stream);
-    if (g->tiles) (void)hipFree(g->tiles);
-    if (g->bounds) (void)hipFree(g->bounds);
-    if (g->stream) (void)hipStreamDestroy(g->stream);
+    if (g->stream) {
+        free_table(g, g->tiles);
+        free_table(g, g->bounds);
+        (void)hipStreamSynchronize(g->stream);
+        (void)hipStreamDestroy(g->stream);
+    }
     delete g;
 }
 
@@ -96,8 +116,8 @@ int build(fw_group* g) {
     if (bounds.size() > fw::kMaxGridZ) return fw::set_error(FW_ERR_INVALID, "a group holds at most 65535 grids");
     g->blocks = (uint32_t)blocks;
     g->ntiles = (int)tiles.size();
-    GR_HIP(hipMalloc((void**)&g->tiles, tiles.size() * sizeof(fw::GroupTile)));
-    GR_HIP(hipMalloc((void**)&g->bounds, bounds.size() * sizeof(fw::GroupBoundary)));
+    GR_HIP(alloc_table(g, (void**)&g->tiles, tiles.size() * sizeof(fw::GroupTile)));
+    GR_HIP(alloc_table(g, (void**)&g->bounds, bounds.size() * sizeof(fw::GroupBoundary)));
     GR_HIP(hipMemcpyAsync(g->tiles, tiles.data(), tiles.size() * sizeof(fw::GroupTile), hipMemcpyHostToDevice,
                           g->stream));
     GR_HIP(hipMemcpyAsync(g->bounds, bounds.data(), bounds.size() * sizeof(fw::GroupBoundary), hipMemcpyHostToDevice,

@@ -166,8 +166,8 @@ __device__ inline void load_vel(VelOps<ISO>& o, const DevSim& s, uint32_t c, boo
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         o.v[a] = gld((const float*)s.v[a], c);
-        o.psi[a][0] = gld((const float*)s.psi[a][0], c);
-        o.psi[a][1] = gld((const float*)s.psi[a][1], c);
+        o.psi[a][0] = gldnt(s.psi[a][0], c);
+        o.psi[a][1] = gldnt(s.psi[a][1], c);
     }
     if (CODED) {
         const CodedCoef k = cell_coef<true, UNIFORM>(s, c);
@@ -278,8 +278,8 @@ __global__ __launch_bounds__(PZ3 * TY) void k_velocity_pipe3d(DevSim batch, int 
                 const int ax = ISO ? 0 : a;
                 const float qq1 = cur.b[ax][0] * cur.psi[a][0] + cur.a[ax][0] * D[a];
                 const float qq2 = cur.b[ax][1] * cur.psi[a][1] + cur.a[ax][1] * D[a];
-                gst(s.psi[a][0], c, qq1);
-                gst(s.psi[a][1], c, qq2);
+                gstnt(s.psi[a][0], c, qq1);
+                gstnt(s.psi[a][1], c, qq2);
                 const float rho_bar = 0.5f * (rho_i + rnb[a]);
                 gst(s.v[a], c, cur.v[a] - s.dt / rho_bar * (D[a] * cur.ik[ax] + (qq1 + qq2)));
             }
@@ -307,8 +307,8 @@ __device__ inline void load_pre(PreOps<ISO>& o, const DevSim& s, uint32_t c, boo
     if (!active) return;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        o.phi[a][0] = gld((const float*)s.phi[a][0], c);
-        o.phi[a][1] = gld((const float*)s.phi[a][1], c);
+        o.phi[a][0] = gldnt(s.phi[a][0], c);
+        o.phi[a][1] = gldnt(s.phi[a][1], c);
     }
     if (CODED) {
         const CodedCoef k = cell_coef<false, UNIFORM>(s, c);
@@ -445,8 +445,8 @@ __global__ __launch_bounds__(PZ3 * TY) void k_pressure_pipe3d(DevSim batch, int 
                 const int ax = ISO ? 0 : a;
                 const float qq1 = cur.b[ax][0] * cur.phi[a][0] + cur.a[ax][0] * D[a];
                 const float qq2 = cur.b[ax][1] * cur.phi[a][1] + cur.a[ax][1] * D[a];
-                gst(s.phi[a][0], c, qq1);
-                gst(s.phi[a][1], c, qq2);
+                gstnt(s.phi[a][0], c, qq1);
+                gstnt(s.phi[a][1], c, qq2);
                 div += D[a] * cur.ik[ax] + (qq1 + qq2);
             }
             gst(s.p, c, cur.p - s.dt * cur.K * div);

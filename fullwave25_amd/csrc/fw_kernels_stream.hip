@@ -98,8 +98,8 @@ __device__ __forceinline__ void velocity_2d_tile(const DevSim& s, unsigned bx, u
             }
 #pragma unroll
             for (int a = 0; a < 2; ++a) {
-                R.ps1[a] = gld32((const float*)s.psi[a][0], c);
-                R.ps2[a] = gld32((const float*)s.psi[a][1], c);
+                R.ps1[a] = gldnt(s.psi[a][0], c);
+                R.ps2[a] = gldnt(s.psi[a][1], c);
                 R.v[a] = gld32((const float*)s.v[a], c);
             }
         }
@@ -142,8 +142,8 @@ __device__ __forceinline__ void velocity_2d_tile(const DevSim& s, unsigned bx, u
                 const float D = acc + w[8] * tr;
                 const float qq1 = R.b1[0] * R.ps1[0] + R.a1[0] * D;
                 const float qq2 = R.b2[0] * R.ps2[0] + R.a2[0] * D;
-                gst32(s.psi[0][0], c, qq1);
-                gst32(s.psi[0][1], c, qq2);
+                gstnt(s.psi[0][0], c, qq1);
+                gstnt(s.psi[0][1], c, qq2);
                 const float rho_bar = 0.5f * (rho_i + R.rho_next);
                 gst32(s.v[0], c, R.v[0] - s.dt / rho_bar * (D * R.ik[0] + (qq1 + qq2)));
             }
@@ -158,8 +158,8 @@ __device__ __forceinline__ void velocity_2d_tile(const DevSim& s, unsigned bx, u
                 const float D = acc + w[8] * tr;
                 const float qq1 = R.b1[ax] * R.ps1[1] + R.a1[ax] * D;
                 const float qq2 = R.b2[ax] * R.ps2[1] + R.a2[ax] * D;
-                gst32(s.psi[1][0], c, qq1);
-                gst32(s.psi[1][1], c, qq2);
+                gstnt(s.psi[1][0], c, qq1);
+                gstnt(s.psi[1][1], c, qq2);
                 const float rho_bar = 0.5f * (rho_i + R.rho_right);
                 gst32(s.v[1], c, R.v[1] - s.dt / rho_bar * (D * R.ik[ax] + (qq1 + qq2)));
             }
@@ -258,8 +258,8 @@ __device__ __forceinline__ void pressure_2d_tile(const DevSim& s, unsigned bx, u
             }
 #pragma unroll
             for (int a = 0; a < 2; ++a) {
-                R.ph1[a] = gld32((const float*)s.phi[a][0], c);
-                R.ph2[a] = gld32((const float*)s.phi[a][1], c);
+                R.ph1[a] = gldnt(s.phi[a][0], c);
+                R.ph2[a] = gldnt(s.phi[a][1], c);
             }
         }
     };
@@ -311,8 +311,8 @@ __device__ __forceinline__ void pressure_2d_tile(const DevSim& s, unsigned bx, u
                 const float D = acc + w[8] * tr;
                 const float qq1 = R.b1[0] * R.ph1[0] + R.a1[0] * D;
                 const float qq2 = R.b2[0] * R.ph2[0] + R.a2[0] * D;
-                gst32(s.phi[0][0], c, qq1);
-                gst32(s.phi[0][1], c, qq2);
+                gstnt(s.phi[0][0], c, qq1);
+                gstnt(s.phi[0][1], c, qq2);
                 div += D * R.ik[0] + (qq1 + qq2);
             }
             {
@@ -326,8 +326,8 @@ __device__ __forceinline__ void pressure_2d_tile(const DevSim& s, unsigned bx, u
                 const float D = acc + w[8] * tr;
                 const float qq1 = R.b1[ax] * R.ph1[1] + R.a1[ax] * D;
                 const float qq2 = R.b2[ax] * R.ph2[1] + R.a2[ax] * D;
-                gst32(s.phi[1][0], c, qq1);
-                gst32(s.phi[1][1], c, qq2);
+                gstnt(s.phi[1][0], c, qq1);
+                gstnt(s.phi[1][1], c, qq2);
                 div += D * R.ik[ax] + (qq1 + qq2);
             }
             gst32(s.p, c, R.p - s.dt * R.K * div);

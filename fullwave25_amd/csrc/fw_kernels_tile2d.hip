@@ -44,6 +44,16 @@ __device__ inline uint2 ldc4(const uint16_t* base, uint32_t i) {
 __device__ inline void st4(float* base, uint32_t i, const F4& v) {
     *reinterpret_cast<F4*>(reinterpret_cast<char*>(base) + (i << 2)) = v;
 }
+// non-temporal 16-byte access for the memory variables (read and written once per step)
+typedef float f4v __attribute__((ext_vector_type(4)));
+__device__ inline F4 ld4nt(const float* base, uint32_t i) {
+    const f4v v = __builtin_nontemporal_load(reinterpret_cast<const f4v*>(at_(base, i)));
+    return F4{v.x, v.y, v.z, v.w};
+}
+__device__ inline void st4nt(float* base, uint32_t i, const F4& v) {
+    const f4v w = {v.x, v.y, v.z, v.w};
+    __builtin_nontemporal_store(w, reinterpret_cast<f4v*>(reinterpret_cast<char*>(base) + (i << 2)));
+}
 
 // the relaxation coefficients and coefficient-set indices of a quad: from the code table
 // (CODED; a cell without a code reads the per-cell maps) or from the per-cell maps
@@ -143,8 +153,8 @@ __global__ __launch_bounds__(kLanes) void k_velocity_tile(DevSim batch) {
         F4 ps1[2], ps2[2], v[2];
 #pragma unroll
         for (int a = 0; a < 2; ++a) {
-            ps1[a] = ld4(s.psi[a][0], c);
-            ps2[a] = ld4(s.psi[a][1], c);
+            ps1[a] = ld4nt(s.psi[a][0], c);
+            ps2[a] = ld4nt(s.psi[a][1], c);
             v[a] = ld4(s.v[a], c);
         }
         // LDS row of tile row lr is lr + 7; column j is at (j - j0 + 8)
@@ -190,8 +200,8 @@ __global__ __launch_bounds__(kLanes) void k_velocity_tile(DevSim batch) {
         // cells outside the active columns keep their loaded values, so whole quads store back
 #pragma unroll
         for (int a = 0; a < 2; ++a) {
-            st4(s.psi[a][0], c, ps1[a]);
-            st4(s.psi[a][1], c, ps2[a]);
+            st4nt(s.psi[a][0], c, ps1[a]);
+            st4nt(s.psi[a][1], c, ps2[a]);
             st4(s.v[a], c, v[a]);
         }
     }
@@ -238,8 +248,8 @@ __global__ __launch_bounds__(kLanes) void k_pressure_tile(DevSim batch) {
         F4 ph1[2], ph2[2];
 #pragma unroll
         for (int a = 0; a < 2; ++a) {
-            ph1[a] = ld4(s.phi[a][0], c);
-            ph2[a] = ld4(s.phi[a][1], c);
+            ph1[a] = ld4nt(s.phi[a][0], c);
+            ph2[a] = ld4nt(s.phi[a][1], c);
         }
         const float* xrow = VX + (lr + 8) * XP + 4 * qd + 4;  // vx(i, jq)
         const float* yrow = VY + (lr + 1) * YP + 4 * qd + 8;  // vy(i, jq)
@@ -284,8 +294,8 @@ __global__ __launch_bounds__(kLanes) void k_pressure_tile(DevSim batch) {
         st4(s.p, c, pv);
 #pragma unroll
         for (int a = 0; a < 2; ++a) {
-            st4(s.phi[a][0], c, ph1[a]);
-            st4(s.phi[a][1], c, ph2[a]);
+            st4nt(s.phi[a][0], c, ph1[a]);
+            st4nt(s.phi[a][1], c, ph2[a]);
         }
     }
 }
