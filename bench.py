@@ -252,9 +252,12 @@ def bench_grouped(args) -> dict:
     done = 0
     got = [None] * len(jobs)
     live = list(range(len(jobs)))
+    width, chunk = args.group_width, args.group_chunk  # 0: timed on the first (fresh) group
     while live:
         end = min(jobs[i].nt for i in live)
-        with engine.HipGroup([sims[i] for i in live], width=args.group_width, chunk=args.group_chunk) as grp:
+        with engine.HipGroup([sims[i] for i in live], width=width, chunk=chunk) as grp:
+            shape = grp.kernel_config()["velocity"]
+            width, chunk = shape["width"], shape["chunk"]
             grp.run(end - done)
         done = end
         for i in [i for i in live if jobs[i].nt == end]:
@@ -274,8 +277,7 @@ def bench_grouped(args) -> dict:
         "max_rel_l2_grouped_vs_sequential": worst,
         "config": {"workload": f"{len(jobs)} focused-transmit jobs on "
                                + ", ".join(f"{a}x{b}" for a, b in sizes)
-                               + f" interiors, {3 * unit}-{6 * unit} steps; group kernel "
-                               f"stream:{args.group_width}:{args.group_chunk}",
+                               + f" interiors, {3 * unit}-{6 * unit} steps; group kernel stream:{width}:{chunk}",
                    "mix": args.mix, "jobs": len(jobs)},
     }
 
@@ -485,8 +487,8 @@ def main() -> int:
     ap.add_argument("--copy-bytes", type=int, default=268435456)
     ap.add_argument("--transmits", type=int, default=32, help="batched_2d: transmits on the shared medium")
     ap.add_argument("--jobs", type=int, default=20, help="served_mix, grouped_2d: number of jobs")
-    ap.add_argument("--group-width", type=int, default=256, help="grouped_2d: lanes per workgroup")
-    ap.add_argument("--group-chunk", type=int, default=16, help="grouped_2d: rows per workgroup")
+    ap.add_argument("--group-width", type=int, default=0, help="grouped_2d: lanes per workgroup (0: timed)")
+    ap.add_argument("--group-chunk", type=int, default=0, help="grouped_2d: rows per workgroup (0: timed)")
     ap.add_argument("--slots", type=int, default=4, help="served_mix: concurrent jobs")
     ap.add_argument("--mix", choices=("standard", "small"), default="standard",
                     help="served_mix, grouped_2d: job sizes (small: none fills the GPU alone)")

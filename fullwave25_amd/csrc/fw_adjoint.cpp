@@ -143,6 +143,7 @@ int fw_adj_create(fw_sim* f, int64_t checkpoint_every, fw_adj** out) {
 int fw_adj_forward(fw_adj* a) {
     if (!a) return fw::set_error(FW_ERR_INVALID, "null handle");
     fw_sim* f = a->f;
+    f->state_zero = false;
     Guard g(f->device);
     int rc = fw_sim_reset(f);
     if (rc != FW_OK) return rc;
@@ -159,6 +160,7 @@ int fw_adj_backward(fw_adj* a, const float* grad_rec, size_t grad_bytes, float* 
                     size_t grid_bytes) {
     if (!a || !grad_rec || !grad_k || !grad_rho) return fw::set_error(FW_ERR_INVALID, "null argument");
     fw_sim* f = a->f;
+    f->state_zero = false;
     if (!a->forward_done) return fw::set_error(FW_ERR_STATE, "run fw_adj_forward first");
     const int64_t nrec = f->rec_cap, nout = f->d.ncoordsout, n = f->ncell, nt = f->d.nt;
     const int64_t n_c = f->ncell_c;

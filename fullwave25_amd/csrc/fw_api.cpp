@@ -684,6 +684,7 @@ int launch_step(fw_sim* s, bool timed, double* ms) {
     rc = timed_launch(fw::kBoundary, [&] { return fw::launch_boundary(s->ds, s->db, s->step, s->d.modt, s->stream); });
     if (rc != FW_OK) return rc;
     s->step += 1;
+    s->state_zero = false;
     return FW_OK;
 }
 
@@ -724,6 +725,7 @@ int run_graph(fw_sim* s, int64_t* left) {
     while (*left >= s->graph_steps) {
         FW_HIP(hipGraphLaunch(s->graph, s->stream));
         s->step += s->graph_steps;
+        s->state_zero = false;
         *left -= s->graph_steps;
     }
     return FW_OK;
@@ -1002,6 +1004,7 @@ int fw_sim_reset(fw_sim* s) {
         FW_HIP(hipMemsetAsync(s->db.rec, 0, (size_t)(s->nshots * s->rec_cap * s->d.ncoordsout) * 4, s->stream));
     FW_HIP(hipStreamSynchronize(s->stream));
     s->step = 0;
+    s->state_zero = true;
     return FW_OK;
 }
 
@@ -1068,6 +1071,7 @@ int fw_sim_set_field(fw_sim* s, const char* name, const float* values, size_t by
     for (size_t b = 0; b * one < bytes; ++b)
         FW_HIP(copy_grid_in(s, dst + b * s->ncell, (const char*)values + b * one, 4));
     if (!s->ready) s->fields_set = true;
+    s->state_zero = false;
     return FW_OK;
 }
 

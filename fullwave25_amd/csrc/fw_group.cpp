@@ -127,6 +127,73 @@ int build(fw_group* g) {
     return FW_OK;
 }
 
+void free_tables(fw_group* g) {
+    free_table(g, g->tiles);
+    free_table(g, g->bounds);
+    g->tiles = nullptr;
+    g->bounds = nullptr;
+}
+
+// Milliseconds of `reps` grouped velocity + pressure launches with the current tables.
+int time_shape(fw_group* g, int reps, float* ms) {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    GR_HIP(hipEventCreate(&e0));
+    hipError_t e = hipEventCreate(&e1);
+    auto launch = [&] {
+        hipError_t r = fw::launch_group_2d(g->tiles, g->ntiles, g->blocks, g->width, g->max_ndmap, true, g->stream);
+        return r != hipSuccess ? r : fw::launch_group_2d(g->tiles, g->ntiles, g->blocks, g->width, g->max_ndmap, false,
+                                                         g->stream);
+    };
+    if (e == hipSuccess) e = launch();
+    if (e == hipSuccess) e = hipEventRecord(e0, g->stream);
+    for (int r = 0; r < reps && e == hipSuccess; ++r) e = launch();
+    if (e == hipSuccess) e = hipEventRecord(e1, g->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    if (e == hipSuccess) e = hipEventElapsedTime(ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    GR_HIP(e);
+    return FW_OK;
+}
+
+// Pick the group's lanes per workgroup and rows per workgroup by timing the candidates, when
+// every member's wave fields are known to be zero (the update then leaves them zero, so the
+// timing launches change nothing); otherwise 256 lanes and 16 rows.
+int choose_shape(fw_group* g, bool tune_width, bool tune_chunk) {
+    bool zero = true;
+    for (fw_sim* s : g->sims) zero = zero && s->state_zero;
+    if (!tune_width && !tune_chunk) return build(g);
+    if (!zero) {
+        if (tune_width) g->width = 256;
+        if (tune_chunk) g->chunk = 16;
+        return build(g);
+    }
+    const int w0 = g->width, c0 = g->chunk;
+    float best = 1e30f;
+    int bw = 256, bc = 16;
+    for (int w : {64, 128, 256}) {
+        if (!tune_width && w != w0) continue;
+        for (int c : {8, 16, 32}) {
+            if (!tune_chunk && c != c0) continue;
+            g->width = w;
+            g->chunk = c;
+            int rc = build(g);
+            float ms = 0.f;
+            if (rc == FW_OK) rc = time_shape(g, 3, &ms);
+            free_tables(g);
+            if (rc != FW_OK) return rc;
+            if (ms < best) {
+                best = ms;
+                bw = w;
+                bc = c;
+            }
+        }
+    }
+    g->width = bw;
+    g->chunk = bc;
+    return build(g);
+}
+
 }  // namespace
 
 extern "C" {
@@ -134,9 +201,10 @@ extern "C" {
 int fw_group_create(fw_sim* const* sims, int32_t n, int32_t width, int32_t chunk, fw_group** out) {
     if (!sims || !out || n < 1) return fw::set_error(FW_ERR_INVALID, "null argument or empty group");
     *out = nullptr;
-    if (width != 64 && width != 128 && width != 256)
-        return fw::set_error(FW_ERR_INVALID, "group width must be 64, 128 or 256");
-    if (chunk < 2 || chunk > 4096) return fw::set_error(FW_ERR_INVALID, "group chunk must be in [2, 4096]");
+    if (width != 0 && width != 64 && width != 128 && width != 256)
+        return fw::set_error(FW_ERR_INVALID, "group width must be 64, 128 or 256 (0: timed)");
+    if (chunk != 0 && (chunk < 2 || chunk > 4096))
+        return fw::set_error(FW_ERR_INVALID, "group chunk must be in [2, 4096] (0: timed)");
     std::set<fw_sim*> seen;
     for (int32_t i = 0; i < n; ++i) {
         fw_sim* s = sims[i];
@@ -164,12 +232,19 @@ int fw_group_create(fw_sim* const* sims, int32_t n, int32_t width, int32_t chunk
         release(g);
         return fw::set_error(FW_ERR_DEVICE, "hipStreamCreateWithFlags failed: %s", hipGetErrorString(e));
     }
-    int rc = build(g);
+    int rc = choose_shape(g, width == 0, chunk == 0);
     if (rc != FW_OK) {
         release(g);
         return rc;
     }
     *out = g;
+    return FW_OK;
+}
+
+int fw_group_config(const fw_group* g, int32_t* width, int32_t* chunk) {
+    if (!g || !width || !chunk) return fw::set_error(FW_ERR_INVALID, "null argument");
+    *width = g->width;
+    *chunk = g->chunk;
     return FW_OK;
 }
 
@@ -197,7 +272,10 @@ int fw_group_run(fw_group* g, int64_t n_steps) {
         GR_HIP(fw::launch_group_boundary(g->bounds, g->ntiles, g->max_points, it, g->stream));
     }
     GR_HIP(hipStreamSynchronize(g->stream));
-    for (fw_sim* s : g->sims) s->step += n_steps;
+    for (fw_sim* s : g->sims) {
+        s->step += n_steps;
+        if (n_steps > 0) s->state_zero = false;
+    }
     return FW_OK;
 }
 

@@ -37,6 +37,7 @@ struct fw_sim {
     bool pinned[2] = {false, false};  // set by fw_sim_set_kernel_config: not autotuned
     fw::LaunchCfg pin[2];
     bool fields_set = false;  // a field was set before the first run: do not tune on it
+    bool state_zero = true;   // every wave field is known to be zero (created or reset, not run since)
     hipEvent_t ev[2] = {nullptr, nullptr};
     std::vector<int64_t> out_lin_h;  // sensor voxels (linear indices), host copy
     // steady steps replayed from a captured HIP graph (graph_steps > 0): the step number then

@@ -175,13 +175,16 @@ int fw_sim_destroy(fw_sim* sim);
  * step is one velocity launch, one pressure launch and one source/air/sensor launch over the
  * tiles of all members (x-marching kernels, `width` lanes of 64, 128 or 256 and `chunk` rows
  * per workgroup; a member too small to give about 128 workgroups takes fewer rows, down to 2),
- * so many small or uneven jobs fill the GPU together. Each member's records are those of
+ * so many small or uneven jobs fill the GPU together. width = 0 or chunk = 0: timed on the
+ * group when every member's wave fields are zero (created or reset, not run since), else 256
+ * and 16; fw_group_config reports the shape in use. Each member's records are those of
  * fw_sim_run with the same kernel shape, bit for bit; read them with
  * fw_sim_download_sensors. Creating the group runs each member's first-run set-up. Between
  * group runs a member may be run alone, reset or read; new maps make fw_group_run fail with
  * FW_ERR_STATE. Destroy the group before destroying its members. */
 typedef struct fw_group fw_group;
 int fw_group_create(fw_sim* const* sims, int32_t n, int32_t width, int32_t chunk, fw_group** out);
+int fw_group_config(const fw_group* group, int32_t* width, int32_t* chunk);
 int fw_group_run(fw_group* group, int64_t n_steps);
 int fw_group_destroy(fw_group* group);
 

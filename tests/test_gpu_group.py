@@ -21,8 +21,8 @@ def _media():
     ]
 
 
-def _alone(inp, n, monkeypatch, codes=True):
-    monkeypatch.setenv("FULLWAVE_HIP_KERNELS", SHAPE)
+def _alone(inp, n, monkeypatch, codes=True, shape=SHAPE):
+    monkeypatch.setenv("FULLWAVE_HIP_KERNELS", shape)
     with engine.HipSimulation(inp) as sim:
         sim.set_relaxation_codes(codes)
         sim.run(n)
@@ -63,14 +63,17 @@ def test_group_with_batched_member(monkeypatch):
     sa = engine.HipSimulation(a, shots=shots)
     sb = engine.HipSimulation(b)
     try:
-        with engine.HipGroup([sa, sb]) as grp:
+        with engine.HipGroup([sa, sb]) as grp:  # shape timed on the fresh members
+            cfg = grp.kernel_config()["velocity"]
             grp.run(a.nt)
-        monkeypatch.setenv("FULLWAVE_HIP_KERNELS", SHAPE)
+        shape = f"stream:{cfg['width']}:{cfg['chunk']}"
+        print(f"timed group shape {shape}", flush=True)
+        monkeypatch.setenv("FULLWAVE_HIP_KERNELS", shape)
         with engine.HipSimulation(a, shots=shots) as alone:
             alone.run(a.nt)
             np.testing.assert_array_equal(sa.sensors(), alone.sensors())
         monkeypatch.delenv("FULLWAVE_HIP_KERNELS")
-        np.testing.assert_array_equal(sb.sensors(), _alone(b, b.nt, monkeypatch))
+        np.testing.assert_array_equal(sb.sensors(), _alone(b, b.nt, monkeypatch, shape=shape))
     finally:
         sa.close()
         sb.close()
@@ -85,7 +88,8 @@ def test_group_state_checks():
         with engine.HipSimulation(m3) as c, pytest.raises(engine.EngineError, match="2D isotropic"):
             engine.HipGroup([a, c])
         b.run(3)
-        grp = engine.HipGroup([a, b])
+        grp = engine.HipGroup([a, b])  # b has run: no timing, the default shape
+        assert grp.kernel_config()["velocity"] == {"variant": "stream", "width": 256, "chunk": 16}
         grp.run(5)
         assert (a.step, b.step) == (5, 8)
         with pytest.raises(engine.EngineError, match="steps left"):
@@ -103,7 +107,7 @@ def test_members_at_different_steps(monkeypatch):
     mb = workloads.random_medium_2d(44, 90, 160, seed=70).inputs
     a, b = engine.HipSimulation(ma), engine.HipSimulation(mb)
     try:
-        with engine.HipGroup([a, b]) as grp:
+        with engine.HipGroup([a, b], width=256, chunk=16) as grp:
             a.run(37)  # a runs ahead on its own, then both advance together
             grp.run(60)
             b.run(11)
