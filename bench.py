@@ -282,6 +282,41 @@ def bench_grouped(args) -> dict:
     }
 
 
+def bench_adjoint(args) -> dict:
+    """The misfit-gradient job of the served mix on its own: 1024^2 interior, 2000 steps,
+    checkpoints every 200 steps; the time of the forward run with checkpoints and of the
+    reverse sweep (recomputing each segment from its checkpoint), against a plain forward run
+    of the same job."""
+    from fullwave25_amd import adjoint, engine, workloads
+
+    inp = workloads.plane_wave_2d(nx=args.nx, ny=args.ny, nt=args.steps, sensor_depth=args.nx // 2).inputs
+    observed = np.zeros((inp.n_records, inp.ncoordsout))
+    with engine.HipSimulation(inp) as sim:  # first-use costs, and the plain run's kernels
+        sim.run(10)
+    t0 = time.perf_counter()
+    with engine.HipSimulation(inp) as sim:
+        sim.run(inp.nt)
+        sim.sensors()
+    t_plain = time.perf_counter() - t0
+    with adjoint.HipAdjoint(inp, checkpoint_every=args.checkpoint_every) as adj:
+        t1 = time.perf_counter()
+        tr = adj.forward()
+        t2 = time.perf_counter()
+        gk, gr = adj.backward((tr - observed).astype(np.float32))
+        t3 = time.perf_counter()
+    cells = inp.ncells * inp.nt
+    return {
+        "metric": "misfit gradient (dJ/dK, dJ/drho): time vs a plain forward run (1 GPU)",
+        "value": round((t3 - t1) / t_plain, 3), "unit": "forward-run equivalents", "n_gpus": 1,
+        "higher_is_better": False, "dtype": "f32", "data": "synthetic",
+        "plain_forward_s": round(t_plain, 4), "forward_with_checkpoints_s": round(t2 - t1, 4),
+        "reverse_sweep_s": round(t3 - t2, 4), "gradient_jobs_per_hour": round(3600 / (t3 - t1), 1),
+        "forward_glups": round(cells / t_plain / 1e9, 3), "finite": bool(np.isfinite(gk).all() and np.isfinite(gr).all()),
+        "config": {"workload": f"plane_wave_2d: {args.nx}x{args.ny} interior ({inp.nx}x{inp.ny} extended), "
+                               f"{inp.nt} steps, checkpoint every {args.checkpoint_every}", "cells": inp.ncells},
+    }
+
+
 def bench_3d(args) -> dict:
     """The 3D heterogeneous configuration (768 x 768 x 640 interior, 904 x 904 x 776 cells,
     ellipsoidal inclusions, 32 x 32 matrix-array plane wave, 6000-step job): the per-step rate
@@ -487,6 +522,7 @@ def main() -> int:
     ap.add_argument("--copy-bytes", type=int, default=268435456)
     ap.add_argument("--transmits", type=int, default=32, help="batched_2d: transmits on the shared medium")
     ap.add_argument("--jobs", type=int, default=20, help="served_mix, grouped_2d: number of jobs")
+    ap.add_argument("--checkpoint-every", type=int, default=200, help="adjoint_2d: checkpoint spacing")
     ap.add_argument("--group-width", type=int, default=0, help="grouped_2d: lanes per workgroup (0: timed)")
     ap.add_argument("--group-chunk", type=int, default=0, help="grouped_2d: rows per workgroup (0: timed)")
     ap.add_argument("--slots", type=int, default=4, help="served_mix: concurrent jobs")
@@ -497,12 +533,12 @@ def main() -> int:
     args = ap.parse_args()
     if args.kernels and args.kernels != "tune":
         os.environ["FULLWAVE_HIP_KERNELS"] = args.kernels
-    if args.config in ("batched_2d", "served_mix", "hetero_3d", "grouped_2d"):
+    if args.config in ("batched_2d", "served_mix", "hetero_3d", "grouped_2d", "adjoint_2d"):
         if args.gpus != 1:
             print(json.dumps({"error": "this build measures one GPU; run with --gpus 1"}))
             return 2
         fn = {"batched_2d": bench_batched, "served_mix": bench_served, "hetero_3d": bench_3d,
-              "grouped_2d": bench_grouped}[args.config]
+              "grouped_2d": bench_grouped, "adjoint_2d": bench_adjoint}[args.config]
         print(json.dumps(fn(args)), flush=True)
         return 0
     if args.pmc_summary:
