@@ -480,8 +480,13 @@ def cpu_baseline(inp, steps: int) -> dict:
 
     oracle.build()
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    model = ""
+    try:
+        model = next(l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo") if l.startswith("model name"))
+    except (OSError, StopIteration):
+        pass
     sim = oracle.OracleSimulation(inp, precision=32)
-    sim.run(1)  # first-touch of the state arrays
+    sim.run(5)  # warm-up (first touch of the state arrays)
     t0 = time.perf_counter()
     sim.run(steps)
     dt = time.perf_counter() - t0
@@ -491,7 +496,9 @@ def cpu_baseline(inp, steps: int) -> dict:
         "unit": "GLUPS",
         "cores": threads,
         "kind": "port",
-        "sample": f"FP32 oracle (C, OpenMP, {threads} threads), steps 1..{steps + 1} of the same workload",
+        "cpu_model": model,
+        "sample": f"FP32 oracle (C, OpenMP, {threads} threads), steps 6..{steps + 5} of the same workload "
+                  f"after 5 warm-up steps",
     }
 
 
