@@ -93,10 +93,15 @@ __device__ inline void quad_coef(const DevSim& s, uint32_t c, F4* ik, F4* b1, F4
     }
 }
 
-// Stage rows [r0, r0 + nrows) x columns [j0 - hl, j0 + kTW + hr) of f into lds (row pitch
-// `pitch` floats, column j0 - hl at offset 0). hl and hr are multiples of 4; ny % 4 == 0, so
-// every quad is either entirely inside [0, ny) or entirely outside (then zero). Rows outside
-// [0, nx) are zero.
+// LDS tile layout: each staged row of `pitch` floats is split into four planes of pitch / 4
+// floats by tile column % 4, each plane holding its columns in order. The lanes of a wavefront
+// own consecutive column quads, so when they all read the same element of their quads (at the
+// same stencil offset) they touch consecutive words: the LDS serves these single-float reads
+// without bank conflicts, where a row-major tile (lanes 4 floats apart) conflicts four ways.
+//
+// Stage rows [r0, r0 + nrows) x columns [j0 - hl, j0 + kTW + hr) of f into lds (tile column 0
+// is j0 - hl). hl and hr are multiples of 4; ny % 4 == 0, so every quad is either entirely
+// inside [0, ny) or entirely outside (then zero). Rows outside [0, nx) are zero.
 __device__ inline void stage_tile(float* lds, const float* __restrict__ f, int64_t r0, int nrows, int64_t j0,
                                   int hl, int hr, int pitch, int64_t nx, int64_t ny) {
     const int qpr = (kTW + hl + hr) / 4;  // quads per row
@@ -107,8 +112,22 @@ __device__ inline void stage_tile(float* lds, const float* __restrict__ f, int64
         const int64_t jj = j0 - hl + 4 * qd;
         F4 v{0.f, 0.f, 0.f, 0.f};
         if (r >= 0 && r < nx && jj >= 0 && jj < ny) v = ld4(f, (uint32_t)(r * ny + jj));
-        *reinterpret_cast<F4*>(lds + row * pitch + 4 * qd) = v;
+        float* dst = lds + row * pitch + qd;
+        const int plane = pitch / 4;
+        dst[0] = v.x;
+        dst[plane] = v.y;
+        dst[2 * plane] = v.z;
+        dst[3 * plane] = v.w;
     }
+}
+
+// Element at row offset dr and column offset dc from tile row `row` of the lane's quad base
+// `lb` (= lds + row * pitch + qd): K = hl + e + dc is the column within the quad grid (hl + e
+// for the lane's own cell e); every index part is a compile-time constant after unrolling.
+template <int PITCH, int HL>
+__device__ inline float tl(const float* lb, int dr, int col) {
+    const int K = HL + col;  // >= 0 for every offset used
+    return lb[dr * PITCH + (K & 3) * (PITCH / 4) + (K >> 2)];
 }
 
 template <bool UNIFORM>
@@ -157,20 +176,20 @@ __global__ __launch_bounds__(kLanes) void k_velocity_tile(DevSim batch) {
             ps2[a] = ld4nt(s.psi[a][1], c);
             v[a] = ld4(s.v[a], c);
         }
-        // LDS row of tile row lr is lr + 7; column j is at (j - j0 + 8)
-        const float* row0 = P + (lr + 7) * PITCH + 4 * qd + 8;
+        // LDS row of tile row lr is lr + 7; column j is tile column j - j0 + 8
+        const float* lb = P + (lr + 7) * PITCH + qd;
+        auto pa = [&](int e, int dr, int dc_) { return tl<PITCH, 8>(lb, dr, e + dc_); };
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
             const int64_t j = jq + e;
             if (j < kHalo || j >= ny - kHalo) continue;
-            const float* pc = row0 + e;  // p(i, j)
             {
                 const float* w = cset<UNIFORM>(lds_coef, dc[e], 0);
                 float acc = 0.f;
 #pragma unroll
-                for (int k = 1; k <= 8; ++k) acc += w[k - 1] * (pc[k * PITCH] - pc[-(k - 1) * PITCH]);
+                for (int k = 1; k <= 8; ++k) acc += w[k - 1] * (pa(e, k, 0) - pa(e, -(k - 1), 0));
                 float tr = 0.f;
-                tr += (pc[PITCH + 1] - pc[1]) + (pc[PITCH - 1] - pc[-1]);
+                tr += (pa(e, 1, 1) - pa(e, 0, 1)) + (pa(e, 1, -1) - pa(e, 0, -1));
                 const float D = acc + w[8] * tr;
                 const float q1 = b1[0][e] * ps1[0][e] + a1[0][e] * D;
                 const float q2 = b2[0][e] * ps2[0][e] + a2[0][e] * D;
@@ -184,9 +203,9 @@ __global__ __launch_bounds__(kLanes) void k_velocity_tile(DevSim batch) {
                 const float* w = cset<UNIFORM>(lds_coef, dc[e], 1);
                 float acc = 0.f;
 #pragma unroll
-                for (int k = 1; k <= 8; ++k) acc += w[k - 1] * (pc[k] - pc[-(k - 1)]);
+                for (int k = 1; k <= 8; ++k) acc += w[k - 1] * (pa(e, 0, k) - pa(e, 0, -(k - 1)));
                 float tr = 0.f;
-                tr += (pc[PITCH + 1] - pc[PITCH]) + (pc[-PITCH + 1] - pc[-PITCH]);
+                tr += (pa(e, 1, 1) - pa(e, 1, 0)) + (pa(e, -1, 1) - pa(e, -1, 0));
                 const float D = acc + w[8] * tr;
                 const float q1 = b1[ax][e] * ps1[1][e] + a1[ax][e] * D;
                 const float q2 = b2[ax][e] * ps2[1][e] + a2[ax][e] * D;
@@ -251,21 +270,22 @@ __global__ __launch_bounds__(kLanes) void k_pressure_tile(DevSim batch) {
             ph1[a] = ld4nt(s.phi[a][0], c);
             ph2[a] = ld4nt(s.phi[a][1], c);
         }
-        const float* xrow = VX + (lr + 8) * XP + 4 * qd + 4;  // vx(i, jq)
-        const float* yrow = VY + (lr + 1) * YP + 4 * qd + 8;  // vy(i, jq)
+        const float* xb = VX + (lr + 8) * XP + qd;  // vx(i, .): tile column j - j0 + 4
+        const float* yb = VY + (lr + 1) * YP + qd;  // vy(i, .): tile column j - j0 + 8
+        auto xa = [&](int e, int dr, int dc_) { return tl<XP, 4>(xb, dr, e + dc_); };
+        auto ya = [&](int e, int dr, int dc_) { return tl<YP, 8>(yb, dr, e + dc_); };
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
             const int64_t j = jq + e;
             if (j < kHalo || j >= ny - kHalo) continue;
             float div = 0.f;
             {
-                const float* vc = xrow + e;
                 const float* w = cset<UNIFORM>(lds_coef, dc[e], 0);
                 float acc = 0.f;
 #pragma unroll
-                for (int k = 1; k <= 8; ++k) acc += w[k - 1] * (vc[(k - 1) * XP] - vc[-k * XP]);
+                for (int k = 1; k <= 8; ++k) acc += w[k - 1] * (xa(e, k - 1, 0) - xa(e, -k, 0));
                 float tr = 0.f;
-                tr += (vc[1] - vc[-XP + 1]) + (vc[-1] - vc[-XP - 1]);
+                tr += (xa(e, 0, 1) - xa(e, -1, 1)) + (xa(e, 0, -1) - xa(e, -1, -1));
                 const float D = acc + w[8] * tr;
                 const float q1 = b1[0][e] * ph1[0][e] + a1[0][e] * D;
                 const float q2 = b2[0][e] * ph2[0][e] + a2[0][e] * D;
@@ -275,13 +295,12 @@ __global__ __launch_bounds__(kLanes) void k_pressure_tile(DevSim batch) {
             }
             {
                 constexpr int ax = ISO ? 0 : 1;
-                const float* vc = yrow + e;
                 const float* w = cset<UNIFORM>(lds_coef, dc[e], 1);
                 float acc = 0.f;
 #pragma unroll
-                for (int k = 1; k <= 8; ++k) acc += w[k - 1] * (vc[k - 1] - vc[-k]);
+                for (int k = 1; k <= 8; ++k) acc += w[k - 1] * (ya(e, 0, k - 1) - ya(e, 0, -k));
                 float tr = 0.f;
-                tr += (vc[YP] - vc[YP - 1]) + (vc[-YP] - vc[-YP - 1]);
+                tr += (ya(e, 1, 0) - ya(e, 1, -1)) + (ya(e, -1, 0) - ya(e, -1, -1));
                 const float D = acc + w[8] * tr;
                 const float q1 = b1[ax][e] * ph1[1][e] + a1[ax][e] * D;
                 const float q2 = b2[ax][e] * ph2[1][e] + a2[ax][e] * D;
