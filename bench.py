@@ -450,8 +450,8 @@ def bench_served(args) -> dict:
         t_next = time.perf_counter()
         for (k, inp), gap in zip(jobs, gaps):
             t_next += gap
-            while time.perf_counter() < t_next:
-                time.sleep(min(0.002, t_next - time.perf_counter()))
+            while (wait := t_next - time.perf_counter()) > 0:
+                time.sleep(min(0.002, wait))
             if k == "adjoint":
                 futs.append(srv.submit_adjoint(inp, observed[id(inp)]))
             else:

@@ -970,16 +970,20 @@ int fw_sim_upload(fw_sim* s, const char* name_c, const void* host, size_t bytes)
         if (bytes != (size_t)(s->nshots * per_shot) * 4)
             return fail(FW_ERR_INVALID, "'icmat' must be [n_shots, ncoords, ntic] float32 (%lld bytes)",
                         (long long)(s->nshots * per_shot * 4));
-        // device layout [shot][ntic][ncoords]: one coalesced row per step
-        std::vector<float> t((size_t)(s->nshots * per_shot));
-        const float* src = (const float*)host;
-        for (int64_t b = 0; b < s->nshots; ++b)
-            for (int64_t q = 0; q < d.ncoords; ++q)
-                for (int64_t n = 0; n < d.ntic; ++n)
-                    t[(size_t)(b * per_shot + n * d.ncoords + q)] = src[b * per_shot + q * d.ntic + n];
+        // device layout [shot][ntic][ncoords] (one coalesced row per step): the caller's
+        // source-major matrix is copied as it is and transposed on the device
         s->db.icmat_stride = per_shot;
         if ((rc = alloc(s, name, bytes, &p)) != FW_OK) return rc;
-        if (bytes) FW_HIP(copy_sync(s, p, t.data(), bytes, hipMemcpyHostToDevice));
+        if (bytes) {
+            Stage src;
+            hipError_t e = src.get(s, bytes);
+            if (e == hipSuccess) e = hipMemcpyAsync(src.p, host, bytes, hipMemcpyHostToDevice, s->stream);
+            if (e == hipSuccess)
+                e = fw::launch_transpose_icmat((const float*)src.p, (float*)p, s->nshots, d.ncoords, d.ntic, s->stream);
+            const hipError_t e2 = src.release(s);
+            const hipError_t e3 = hipStreamSynchronize(s->stream);
+            FW_HIP(e != hipSuccess ? e : e2 != hipSuccess ? e2 : e3);
+        }
         s->db.icmat_t = (const float*)p;
         s->have[name] = true;
         return FW_OK;

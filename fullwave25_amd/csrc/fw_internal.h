@@ -237,6 +237,9 @@ hipError_t launch_group_2d(const GroupTile* t, int n, uint32_t blocks, int width
                            hipStream_t st);
 hipError_t launch_group_boundary(const GroupBoundary* g, int n, int64_t max_points, int64_t it, hipStream_t st);
 hipError_t launch_reciprocal(float* data, int64_t n, hipStream_t st);
+// [nshots][ncoords][ntic] -> [nshots][ntic][ncoords]
+hipError_t launch_transpose_icmat(const float* in, float* out, int64_t nshots, int64_t ncoords, int64_t ntic,
+                                  hipStream_t st);
 hipError_t launch_copy(const void* src, void* dst, int64_t n16, hipStream_t st);
 hipError_t launch_code_insert(const CodeMaps& m, int64_t n, unsigned long long* keys, uint32_t nslots, uint32_t* count,
                               int32_t* rep, int32_t* status, hipStream_t st);

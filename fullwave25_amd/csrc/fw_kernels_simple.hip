@@ -200,6 +200,33 @@ __global__ __launch_bounds__(256) void k_reciprocal(float* data, int64_t n) {
     if (t < n) data[t] = data[t] != 0.f ? 1.0f / data[t] : 0.f;  // zero: row padding
 }
 
+// Source signals from the file layout [shot][ncoords][ntic] (source-major) to the device
+// layout [shot][ntic][ncoords] (one coalesced row per step), in 32 x 32 tiles through LDS:
+// both the reads and the writes of a tile are row-contiguous. The grid strides over the tiles
+// (every lane of a workgroup runs the same number of iterations, so the barriers are uniform).
+__global__ __launch_bounds__(256) void k_transpose_icmat(const float* __restrict__ in, float* __restrict__ out,
+                                                         int64_t nq, int64_t nt, int64_t ntiles) {
+    __shared__ float tile[32][33];
+    const int64_t tq = (nq + 31) / 32, tn = (nt + 31) / 32, per = tq * tn;
+    const int tx = threadIdx.x % 32, ty = threadIdx.x / 32;  // ty: 0 .. 7
+    for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int64_t b = t / per, r = t - b * per;
+        const int64_t q0 = (r / tn) * 32, n0 = (r % tn) * 32;
+        const float* src = in + b * nq * nt;
+        float* dst = out + b * nq * nt;
+        for (int k = ty; k < 32; k += 8) {
+            const int64_t q = q0 + k, n = n0 + tx;
+            if (q < nq && n < nt) tile[k][tx] = src[q * nt + n];
+        }
+        __syncthreads();
+        for (int k = ty; k < 32; k += 8) {
+            const int64_t n = n0 + k, q = q0 + tx;
+            if (q < nq && n < nt) dst[n * nq + q] = tile[tx][k];
+        }
+        __syncthreads();
+    }
+}
+
 }  // namespace
 
 static dim3 simple_grid(const DevSim& s) {
@@ -266,6 +293,15 @@ hipError_t launch_copy(const void* src, void* dst, int64_t n16, hipStream_t st) 
     int64_t blocks = (n16 + 255) / 256;
     if (blocks > 256 * 64) blocks = 256 * 64;
     hipLaunchKernelGGL(k_copy, dim3((unsigned)blocks), dim3(256), 0, st, (const float4*)src, (float4*)dst, n16);
+    return hipGetLastError();
+}
+
+hipError_t launch_transpose_icmat(const float* in, float* out, int64_t nshots, int64_t ncoords, int64_t ntic,
+                                  hipStream_t st) {
+    const int64_t ntiles = nshots * ((ncoords + 31) / 32) * ((ntic + 31) / 32);
+    if (ntiles == 0) return hipSuccess;
+    const int64_t blocks = ntiles < 256 * 64 ? ntiles : 256 * 64;
+    hipLaunchKernelGGL(k_transpose_icmat, dim3((unsigned)blocks), dim3(256), 0, st, in, out, ncoords, ntic, ntiles);
     return hipGetLastError();
 }
 

@@ -10,8 +10,9 @@ engine handle, which owns its own HIP stream, from its own worker thread, so the
 concurrent jobs overlap on the device. Inputs stay resident in HBM from admission to
 completion; the host only moves the inputs in and the traces out.
 
-Continuous batching (``batching=True``, the default): 2D isotropic forward jobs do not run
-their own launch loop. Once a job's handle holds its inputs, it joins a shared group
+Continuous batching (``batching=True``, the default): 2D isotropic forward jobs of at most
+``group_max_cells`` cells (all shots together; larger ones fill the GPU alone and keep their
+own tuned kernels) do not run their own launch loop. Once a job's handle holds its inputs, it joins a shared group
 (:class:`~fullwave25_amd.engine.HipGroup`), and a batcher thread advances the whole group
 ``quantum_steps`` at a time with one velocity, one pressure and one source/sensor launch per
 step for all members; jobs that arrive meanwhile join at the next quantum, at their own step 0,
@@ -99,6 +100,9 @@ class ServerStats:
 
 
 GROUP_WIDTH, GROUP_CHUNK = 256, 16  # kernel shape of grouped jobs
+# Jobs of more cells (all shots together) than this fill the GPU on their own, where their own
+# tuned kernels (the float4 tiles) outrun the grouped marching kernels: they run alone.
+GROUP_MAX_CELLS = 1 << 21
 
 
 def _shape_key(inp: SimulationInputs, n_shots: int) -> tuple:
@@ -115,8 +119,9 @@ class SimulationServer:
 
     def __init__(self, device: int = 0, *, hbm_budget: int | None = None, max_slots: int = 8,
                  chunk_steps: int = 500, max_bypass: int = 8, runner=None, batching: bool = True,
-                 quantum_steps: int = 250) -> None:
+                 quantum_steps: int = 250, group_max_cells: int = GROUP_MAX_CELLS) -> None:
         self.device = device
+        self.group_max_cells = int(group_max_cells)
         self.batching = bool(batching) and runner is None
         self.quantum_steps = int(quantum_steps)
         self._batch_lock = threading.Condition()
@@ -285,7 +290,7 @@ class SimulationServer:
         """The kernel configuration this server runs (or ran) jobs of this shape with: the group
         shape for grouped jobs, else the one tuned on the first handle of the shape (None if no
         such job has run yet)."""
-        if self.batching and self._groupable(inp):
+        if self._grouped(inp, n_shots):
             shape = {"variant": "stream", "width": GROUP_WIDTH, "chunk": GROUP_CHUNK}
             return {"velocity": shape, "pressure": shape}
         with self._lock:
@@ -297,9 +302,14 @@ class SimulationServer:
         return (inp.ndim == 2 and bool(inp.isotropic) and not bool(inp.exponential) and inp.ndmap <= 1024
                 and inp.ncells < (1 << 30))
 
+    def _grouped(self, inp: SimulationInputs, n_shots: int) -> bool:
+        """Whether a forward job joins the shared group: batching on, a groupable model, and
+        small enough that its own launches would leave the GPU partly idle."""
+        return self.batching and self._groupable(inp) and inp.ncells * n_shots <= self.group_max_cells
+
     def _run_engine(self, inp: SimulationInputs, shots: np.ndarray | None) -> np.ndarray:
         n_shots = 1 if shots is None else int(shots.shape[0])
-        if self.batching and self._groupable(inp):
+        if self._grouped(inp, n_shots):
             with engine.HipSimulation(inp, self.device, shots=shots) as sim:
                 shape = {"variant": "stream", "width": GROUP_WIDTH, "chunk": GROUP_CHUNK}
                 sim.set_kernel_config({"velocity": shape, "pressure": shape})

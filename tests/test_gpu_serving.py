@@ -27,17 +27,26 @@ def _jobs():
     return [(a, None), (b, None), (c, None), (d, None), (a, shots), (b, None), (c, None), (d, None), (a, None)]
 
 
-@pytest.mark.parametrize("batching", [True, False])
+@pytest.mark.parametrize("batching", ["group", "mixed", "off"])
 def test_served_mix_equals_alone_and_oracle(batching):
     """With batching, the 2D isotropic jobs (a, b and the batch on a's medium) run as members
-    of the server's shared group while the 3D and exponential jobs run on their own handles."""
+    of the server's shared group while the 3D and exponential jobs run on their own handles;
+    "mixed" caps the group at a's size, so that b and the batch run alone beside it."""
     jobs = _jobs()
+    a, b = jobs[0][0], jobs[1][0]
+    limit = a.ncells if batching == "mixed" else 1 << 21
+    assert b.ncells > a.ncells  # b (and the batch of three) exceed the "mixed" cap
     budget = 3 * max(engine.device_bytes(inp, 1 if s is None else s.shape[0]) for inp, s in jobs)
-    with SimulationServer(hbm_budget=budget, max_slots=4, chunk_steps=64, batching=batching,
-                          quantum_steps=48) as srv:
+    with SimulationServer(hbm_budget=budget, max_slots=4, chunk_steps=64, batching=batching != "off",
+                          quantum_steps=48, group_max_cells=limit) as srv:
         futs = [srv.submit(inp, shots=s) for inp, s in jobs]
         results = [f.result(timeout=300) for f in futs]
         configs = [srv.kernel_config_for(inp, 1 if s is None else s.shape[0]) for inp, s in jobs]
+        grouped = [srv._grouped(inp, 1 if s is None else s.shape[0]) for inp, s in jobs]
+    want = {"group": [True, True, False, False, True, True, False, False, True],
+            "mixed": [True, False, False, False, False, False, False, False, True],
+            "off": [False] * 9}[batching]
+    assert grouped == want
     st = srv.stats()
     print(f"served {st.jobs} jobs: p50 {st.latency_p50_s * 1e3:.1f} ms, p99 {st.latency_p99_s * 1e3:.1f} ms, "
           f"peak running {srv.peak_running}, peak bytes {srv.peak_in_use}", flush=True)
