@@ -618,6 +618,12 @@ int finalize(fw_sim* s) {
     }
     if ((rc = up("_coef", coef.data(), coef.size() * 4)) != FW_OK) return rc;
     s->ds.coef = (const float*)s->dev["_coef"];
+    s->ds.coef_axes_same = 1;
+    for (int64_t m = 0; m < d.ndmap && s->ds.coef_axes_same; ++m)
+        for (int a = 1; a < nd; ++a)
+            if (memcmp(&coef[(size_t)(m * nd * fw::kNCoef)], &coef[(size_t)((m * nd + a) * fw::kNCoef)],
+                       fw::kNCoef * sizeof(float)) != 0)
+                s->ds.coef_axes_same = 0;
 
     // dcmap range check on the host would need a download; the kernels trust it, so check it
     // here once against ndmap

@@ -36,6 +36,7 @@ struct DevSim {
     const int32_t* dcmap;
     const float* coef;  // [ndmap][ndim][9], w and e already divided by the axis spacing
     int32_t ndmap;
+    int32_t coef_axes_same;  // 1: every axis has the same set for each dcmap value (equal spacings)
     int32_t isotropic;
     // exponential-attenuation model: no memory variables; v and p are scaled by aexp after
     // their update (aexp == nullptr for the relaxation model)

@@ -130,9 +130,27 @@ __device__ inline float tl(const float* lb, int dr, int col) {
     return lb[dr * PITCH + (K & 3) * (PITCH / 4) + (K >> 2)];
 }
 
+// LDS coefficient sets: [set][axis][9], or [set][9] when both axes share their sets (equal
+// spacings; `axes` = 1), which halves the table and lets one more workgroup fit on a CU.
 template <bool UNIFORM>
-__device__ inline const float* cset(const float* lds_coef, int dc, int axis) {
-    return UNIFORM ? lds_coef + axis * kNCoef : lds_coef + (dc * 2 + axis) * kNCoef;
+__device__ inline const float* cset(const float* lds_coef, int dc, int axis, int axes) {
+    return lds_coef + ((UNIFORM ? 0 : dc) * axes + (axes > 1 ? axis : 0)) * kNCoef;
+}
+
+// Stage the coefficient sets into LDS (all axes, or axis 0 only when they are the same).
+template <bool UNIFORM>
+__device__ inline int stage_coef(float* lds_coef, const DevSim& s) {
+    const int axes = s.coef_axes_same ? 1 : 2;
+    const int nsets = UNIFORM ? 1 : s.ndmap;
+    if (axes == 2) {
+        for (int t = threadIdx.x; t < nsets * 2 * kNCoef; t += kLanes) lds_coef[t] = s.coef[t];
+    } else {
+        for (int t = threadIdx.x; t < nsets * kNCoef; t += kLanes) {
+            const int set = t / kNCoef;
+            lds_coef[t] = s.coef[(set * 2) * kNCoef + (t - set * kNCoef)];
+        }
+    }
+    return axes;
 }
 
 // velocity: p tile rows i0-7 .. i0+TX+7 (TX + 15 rows), columns j0-8 .. j0+TW+8
@@ -146,8 +164,7 @@ __global__ __launch_bounds__(kLanes) void k_velocity_tile(DevSim batch) {
     const int64_t nx = s.nx, ny = s.ny;
     const int64_t j0 = (int64_t)blockIdx.x * kTW;
     const int64_t i0 = kHalo + (int64_t)blockIdx.y * TX;
-    const int ncoef = (UNIFORM ? 1 : s.ndmap) * 2 * kNCoef;
-    for (int t = threadIdx.x; t < ncoef; t += kLanes) lds_coef[t] = s.coef[t];
+    const int axes = stage_coef<UNIFORM>(lds_coef, s);
     stage_tile(P, s.p, i0 - 7, NROWS, j0, 8, 8, PITCH, nx, ny);
     __syncthreads();
 
@@ -184,7 +201,7 @@ __global__ __launch_bounds__(kLanes) void k_velocity_tile(DevSim batch) {
             const int64_t j = jq + e;
             if (j < kHalo || j >= ny - kHalo) continue;
             {
-                const float* w = cset<UNIFORM>(lds_coef, dc[e], 0);
+                const float* w = cset<UNIFORM>(lds_coef, dc[e], 0, axes);
                 float acc = 0.f;
 #pragma unroll
                 for (int k = 1; k <= 8; ++k) acc += w[k - 1] * (pa(e, k, 0) - pa(e, -(k - 1), 0));
@@ -200,7 +217,7 @@ __global__ __launch_bounds__(kLanes) void k_velocity_tile(DevSim batch) {
             }
             {
                 constexpr int ax = ISO ? 0 : 1;
-                const float* w = cset<UNIFORM>(lds_coef, dc[e], 1);
+                const float* w = cset<UNIFORM>(lds_coef, dc[e], 1, axes);
                 float acc = 0.f;
 #pragma unroll
                 for (int k = 1; k <= 8; ++k) acc += w[k - 1] * (pa(e, 0, k) - pa(e, 0, -(k - 1)));
@@ -241,8 +258,7 @@ __global__ __launch_bounds__(kLanes) void k_pressure_tile(DevSim batch) {
     const int64_t nx = s.nx, ny = s.ny;
     const int64_t j0 = (int64_t)blockIdx.x * kTW;
     const int64_t i0 = kHalo + (int64_t)blockIdx.y * TX;
-    const int ncoef = (UNIFORM ? 1 : s.ndmap) * 2 * kNCoef;
-    for (int t = threadIdx.x; t < ncoef; t += kLanes) lds_coef[t] = s.coef[t];
+    const int axes = stage_coef<UNIFORM>(lds_coef, s);
     stage_tile(VX, s.v[0], i0 - 8, XR, j0, 4, 4, XP, nx, ny);
     stage_tile(VY, s.v[1], i0 - 1, YR, j0, 8, 8, YP, nx, ny);
     __syncthreads();
@@ -280,7 +296,7 @@ __global__ __launch_bounds__(kLanes) void k_pressure_tile(DevSim batch) {
             if (j < kHalo || j >= ny - kHalo) continue;
             float div = 0.f;
             {
-                const float* w = cset<UNIFORM>(lds_coef, dc[e], 0);
+                const float* w = cset<UNIFORM>(lds_coef, dc[e], 0, axes);
                 float acc = 0.f;
 #pragma unroll
                 for (int k = 1; k <= 8; ++k) acc += w[k - 1] * (xa(e, k - 1, 0) - xa(e, -k, 0));
@@ -295,7 +311,7 @@ __global__ __launch_bounds__(kLanes) void k_pressure_tile(DevSim batch) {
             }
             {
                 constexpr int ax = ISO ? 0 : 1;
-                const float* w = cset<UNIFORM>(lds_coef, dc[e], 1);
+                const float* w = cset<UNIFORM>(lds_coef, dc[e], 1, axes);
                 float acc = 0.f;
 #pragma unroll
                 for (int k = 1; k <= 8; ++k) acc += w[k - 1] * (ya(e, 0, k - 1) - ya(e, 0, -k));
@@ -322,7 +338,7 @@ __global__ __launch_bounds__(kLanes) void k_pressure_tile(DevSim batch) {
 template <int TX>
 hipError_t launch_tile(const DevSim& s, bool velocity, hipStream_t st) {
     const bool u = s.ndmap == 1;
-    const int lds = (u ? 1 : s.ndmap) * 2 * kNCoef * 4;
+    const int lds = (u ? 1 : s.ndmap) * (s.coef_axes_same ? 1 : 2) * kNCoef * 4;
     const int64_t rows = s.nx - 2 * kHalo;
     const dim3 g((unsigned)((s.ny + kTW - 1) / kTW), (unsigned)((rows + TX - 1) / TX), (unsigned)s.nshots);
     const dim3 b(kLanes);

@@ -175,6 +175,26 @@ def test_streaming_kernels_match_per_cell_kernels(monkeypatch, shape):
         assert rel_l2(pa, pb) <= 1e-6
 
 
+@pytest.mark.parametrize("shape", ["tile:8", "tile:16", "stream:256:8"])
+def test_unequal_spacing(monkeypatch, shape):
+    """dy != dx: the x and y stencil weights differ, so the tiled kernels stage both axes'
+    coefficient sets (with equal spacings they stage one set per sound speed). Against the
+    per-cell kernels and the FP64 oracle."""
+    base = workloads.random_medium_2d(70, 300, 120, seed=38).inputs
+    inp = dataclasses.replace(base, dy=base.dy * 1.25)
+    monkeypatch.setenv("FULLWAVE_HIP_KERNELS", shape)
+    a, pa = _gpu(inp)
+    monkeypatch.setenv("FULLWAVE_HIP_KERNELS", "percell")
+    b, pb = _gpu(inp)
+    monkeypatch.delenv("FULLWAVE_HIP_KERNELS")
+    assert rel_l2(a, b) <= 1e-6 and rel_l2(pa, pb) <= 1e-6
+    ref, _ = _ref(inp)
+    err = rel_l2(a, ref)
+    print(f"{shape}, dy = 1.25 dx: rel-L2 {err:.2e} against the oracle", flush=True)
+    assert err <= 1e-4
+    assert rel_l2(a, _gpu(base)[0]) > 1e-3  # the spacing reaches the stencil
+
+
 def _three_speed_3d(shape, nt, seed):
     """3D inputs whose sound speed takes three values (61 coefficient sets, LDS-resident)."""
     rng = np.random.default_rng(seed)
