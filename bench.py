@@ -252,12 +252,13 @@ def bench_grouped(args) -> dict:
     done = 0
     got = [None] * len(jobs)
     live = list(range(len(jobs)))
-    width, chunk = args.group_width, args.group_chunk  # 0: timed on the first (fresh) group
+    # kernel family and shape: timed on the first (fresh) group where left open, then kept
+    variant, width, chunk = args.group_variant, args.group_width, args.group_chunk
     while live:
         end = min(jobs[i].nt for i in live)
-        with engine.HipGroup([sims[i] for i in live], width=width, chunk=chunk) as grp:
+        with engine.HipGroup([sims[i] for i in live], width=width, chunk=chunk, variant=variant) as grp:
             shape = grp.kernel_config()["velocity"]
-            width, chunk = shape["width"], shape["chunk"]
+            variant, width, chunk = shape["variant"], shape["width"], shape["chunk"]
             grp.run(end - done)
         done = end
         for i in [i for i in live if jobs[i].nt == end]:
@@ -277,7 +278,8 @@ def bench_grouped(args) -> dict:
         "max_rel_l2_grouped_vs_sequential": worst,
         "config": {"workload": f"{len(jobs)} focused-transmit jobs on "
                                + ", ".join(f"{a}x{b}" for a, b in sizes)
-                               + f" interiors, {3 * unit}-{6 * unit} steps; group kernel stream:{width}:{chunk}",
+                               + f" interiors, {3 * unit}-{6 * unit} steps; group kernel "
+                               + (f"tile:{chunk}" if variant == "tile" else f"stream:{width}:{chunk}"),
                    "mix": args.mix, "jobs": len(jobs)},
     }
 
@@ -532,6 +534,8 @@ def main() -> int:
     ap.add_argument("--checkpoint-every", type=int, default=200, help="adjoint_2d: checkpoint spacing")
     ap.add_argument("--group-width", type=int, default=0, help="grouped_2d: lanes per workgroup (0: timed)")
     ap.add_argument("--group-chunk", type=int, default=0, help="grouped_2d: rows per workgroup (0: timed)")
+    ap.add_argument("--group-variant", choices=("auto", "stream", "tile"), default="auto",
+                    help="grouped_2d: kernel family of the group (auto: timed)")
     ap.add_argument("--slots", type=int, default=4, help="served_mix: concurrent jobs")
     ap.add_argument("--mix", choices=("standard", "small"), default="standard",
                     help="served_mix, grouped_2d: job sizes (small: none fills the GPU alone)")

@@ -1,7 +1,8 @@
 // Grouped time steps: several engine handles (different grids and media) advanced together.
 // Every step is one velocity launch, one pressure launch and one source/air/sensor launch over
-// the tiles of all members (kernels in fw_kernels_stream.hip and fw_kernels_simple.hip), so
-// many small or uneven jobs fill the GPU as one job does, without one launch per job and kernel.
+// the tiles of all members (marching kernels in fw_kernels_stream.hip or tiled kernels in
+// fw_kernels_tile2d.hip; fw_kernels_simple.hip), so many small or uneven jobs fill the GPU as
+// one job does, without one launch per job and kernel.
 #include <hip/hip_runtime.h>
 
 #include <set>
@@ -23,7 +24,10 @@ struct fw_group {
     int ntiles = 0;
     uint32_t blocks = 0;
     int64_t max_points = 0;
+    int variant = 1;  // 1: marching kernels (width lanes, chunk rows); 2: tiled (256 x chunk)
     int width = 256, chunk = 16, max_ndmap = 1;
+    int coef_sets = 1;      // LDS coefficient sets of the tiled kernels (largest member)
+    bool all_coded = true;  // every member reads coded relaxation coefficients
 };
 
 namespace {
@@ -80,14 +84,21 @@ int build(fw_group* g) {
     std::vector<fw::GroupBoundary> bounds;
     uint64_t blocks = 0;
     for (fw_sim* s : g->sims) {
-        const unsigned gx = fw::group_tiles_x(s->ds.ny, g->width);
-        // a grid too small to give ~128 workgroups at the group's row count takes fewer rows
-        // per workgroup (no fewer than 2), so that its tiles do not march alone at the end
-        const int64_t rows = s->ds.nx - 2 * fw::kHalo;
-        int64_t chunk = rows * gx / 128;
-        if (chunk > g->chunk) chunk = g->chunk;
-        if (chunk < 2) chunk = 2;
-        const unsigned gy = fw::group_tiles_y(s->ds.nx, (int)chunk);
+        unsigned gx, gy;
+        int64_t chunk = g->chunk;
+        if (g->variant == 2) {
+            gx = fw::group_tiles_x(s->ds.ny, 256);
+            gy = fw::group_tile_rows(s->ds.nx, g->chunk);
+        } else {
+            gx = fw::group_tiles_x(s->ds.ny, g->width);
+            // a grid too small to give ~128 workgroups at the group's row count takes fewer rows
+            // per workgroup (no fewer than 2), so that its tiles do not march alone at the end
+            const int64_t rows = s->ds.nx - 2 * fw::kHalo;
+            chunk = rows * gx / 128;
+            if (chunk > g->chunk) chunk = g->chunk;
+            if (chunk < 2) chunk = 2;
+            gy = fw::group_tiles_y(s->ds.nx, (int)chunk);
+        }
         for (int b = 0; b < s->nshots; ++b) {
             fw::GroupTile t{};
             t.s = fw::shot_view(s->ds, b);
@@ -110,6 +121,9 @@ int build(fw_group* g) {
             if (pts > g->max_points) g->max_points = pts;
         }
         if (s->ds.ndmap > g->max_ndmap) g->max_ndmap = s->ds.ndmap;
+        const int sets = s->ds.ndmap * (s->ds.coef_axes_same ? 1 : 2);
+        if (sets > g->coef_sets) g->coef_sets = sets;
+        if (!s->ds.rcode) g->all_coded = false;
         g->shots_of.push_back(s->nshots);
     }
     if (blocks >= (1ull << 32)) return fw::set_error(FW_ERR_INVALID, "the group has too many tiles for one launch");
@@ -134,15 +148,21 @@ void free_tables(fw_group* g) {
     g->bounds = nullptr;
 }
 
+hipError_t launch_stencil(const fw_group* g, bool velocity) {
+    if (g->variant == 2)
+        return fw::launch_group_tile2d(g->tiles, g->ntiles, g->blocks, g->chunk, g->coef_sets, g->all_coded, velocity,
+                                       g->stream);
+    return fw::launch_group_2d(g->tiles, g->ntiles, g->blocks, g->width, g->max_ndmap, velocity, g->stream);
+}
+
 // Milliseconds of `reps` grouped velocity + pressure launches with the current tables.
 int time_shape(fw_group* g, int reps, float* ms) {
     hipEvent_t e0 = nullptr, e1 = nullptr;
     GR_HIP(hipEventCreate(&e0));
     hipError_t e = hipEventCreate(&e1);
     auto launch = [&] {
-        hipError_t r = fw::launch_group_2d(g->tiles, g->ntiles, g->blocks, g->width, g->max_ndmap, true, g->stream);
-        return r != hipSuccess ? r : fw::launch_group_2d(g->tiles, g->ntiles, g->blocks, g->width, g->max_ndmap, false,
-                                                         g->stream);
+        hipError_t r = launch_stencil(g, true);
+        return r != hipSuccess ? r : launch_stencil(g, false);
     };
     if (e == hipSuccess) e = launch();
     if (e == hipSuccess) e = hipEventRecord(e0, g->stream);
@@ -156,41 +176,58 @@ int time_shape(fw_group* g, int reps, float* ms) {
     return FW_OK;
 }
 
-// Pick the group's lanes per workgroup and rows per workgroup by timing the candidates, when
-// every member's wave fields are known to be zero (the update then leaves them zero, so the
-// timing launches change nothing); otherwise 256 lanes and 16 rows.
-int choose_shape(fw_group* g, bool tune_width, bool tune_chunk) {
+bool tiles_supported(const fw_group* g) {
+    for (fw_sim* s : g->sims)
+        if (!fw::tile2d_supported(s->ds)) return false;
+    return true;
+}
+
+// Pick the group's kernel family, lanes per workgroup and rows per workgroup by timing the
+// candidates (those left open: `tune_variant`, `tune_width`, `tune_chunk`), when every member's
+// wave fields are known to be zero (the update then leaves them zero, so the timing launches
+// change nothing); otherwise the marching kernels at 256 lanes and 16 rows, or the tiled ones
+// at 16 rows.
+int choose_shape(fw_group* g, bool tune_variant, bool tune_width, bool tune_chunk) {
     bool zero = true;
     for (fw_sim* s : g->sims) zero = zero && s->state_zero;
-    if (!tune_width && !tune_chunk) return build(g);
+    if (!tune_variant && !tune_width && !tune_chunk) return build(g);
     if (!zero) {
+        if (tune_variant) g->variant = 1;
         if (tune_width) g->width = 256;
         if (tune_chunk) g->chunk = 16;
         return build(g);
     }
-    const int w0 = g->width, c0 = g->chunk;
+    struct Shape {
+        int variant, width, chunk;
+    };
+    std::vector<Shape> cands;
+    const int v0 = g->variant, w0 = g->width, c0 = g->chunk;
+    if (tune_variant || v0 == 1)
+        for (int w : {64, 128, 256})
+            for (int c : {8, 16, 32})
+                if ((tune_width || w == w0) && (tune_chunk || c == c0)) cands.push_back({1, w, c});
+    if ((tune_variant && tiles_supported(g)) || v0 == 2)
+        for (int c : {8, 16, 32})
+            if (tune_chunk || c == c0) cands.push_back({2, 256, c});
     float best = 1e30f;
-    int bw = 256, bc = 16;
-    for (int w : {64, 128, 256}) {
-        if (!tune_width && w != w0) continue;
-        for (int c : {8, 16, 32}) {
-            if (!tune_chunk && c != c0) continue;
-            g->width = w;
-            g->chunk = c;
-            int rc = build(g);
-            float ms = 0.f;
-            if (rc == FW_OK) rc = time_shape(g, 3, &ms);
-            free_tables(g);
-            if (rc != FW_OK) return rc;
-            if (ms < best) {
-                best = ms;
-                bw = w;
-                bc = c;
-            }
+    Shape pick = cands.empty() ? Shape{1, 256, 16} : cands[0];
+    for (const Shape& c : cands) {
+        g->variant = c.variant;
+        g->width = c.width;
+        g->chunk = c.chunk;
+        int rc = build(g);
+        float ms = 0.f;
+        if (rc == FW_OK) rc = time_shape(g, 3, &ms);
+        free_tables(g);
+        if (rc != FW_OK) return rc;
+        if (ms < best) {
+            best = ms;
+            pick = c;
         }
     }
-    g->width = bw;
-    g->chunk = bc;
+    g->variant = pick.variant;
+    g->width = pick.width;
+    g->chunk = pick.chunk;
     return build(g);
 }
 
@@ -198,13 +235,24 @@ int choose_shape(fw_group* g, bool tune_width, bool tune_chunk) {
 
 extern "C" {
 
-int fw_group_create(fw_sim* const* sims, int32_t n, int32_t width, int32_t chunk, fw_group** out) {
+int fw_group_create_shaped(fw_sim* const* sims, int32_t n, int32_t variant, int32_t width, int32_t chunk,
+                           fw_group** out) {
     if (!sims || !out || n < 1) return fw::set_error(FW_ERR_INVALID, "null argument or empty group");
     *out = nullptr;
-    if (width != 0 && width != 64 && width != 128 && width != 256)
-        return fw::set_error(FW_ERR_INVALID, "group width must be 64, 128 or 256 (0: timed)");
-    if (chunk != 0 && (chunk < 2 || chunk > 4096))
-        return fw::set_error(FW_ERR_INVALID, "group chunk must be in [2, 4096] (0: timed)");
+    if (variant < 0 || variant > 2)
+        return fw::set_error(FW_ERR_INVALID, "group variant must be 1 (marching), 2 (tiled) or 0 (timed)");
+    if (variant == 2) {
+        if (width != 0 && width != 256) return fw::set_error(FW_ERR_INVALID, "tiled groups are 256 columns wide");
+        if (chunk != 0 && chunk != 8 && chunk != 16 && chunk != 32)
+            return fw::set_error(FW_ERR_INVALID, "tiled group rows must be 8, 16 or 32 (0: timed)");
+    } else {
+        if (width != 0 && width != 64 && width != 128 && width != 256)
+            return fw::set_error(FW_ERR_INVALID, "group width must be 64, 128 or 256 (0: timed)");
+        if (chunk != 0 && (chunk < 2 || chunk > 4096))
+            return fw::set_error(FW_ERR_INVALID, "group chunk must be in [2, 4096] (0: timed)");
+        if (variant == 0 && (width != 0 || chunk != 0))
+            return fw::set_error(FW_ERR_INVALID, "a timed kernel family takes a timed width and chunk (0, 0)");
+    }
     std::set<fw_sim*> seen;
     for (int32_t i = 0; i < n; ++i) {
         fw_sim* s = sims[i];
@@ -220,10 +268,14 @@ int fw_group_create(fw_sim* const* sims, int32_t n, int32_t width, int32_t chunk
         if (rc != FW_OK) return rc;
         if (!fw::streaming_supported(sims[i]->ds))
             return fw::set_error(FW_ERR_UNSUPPORTED, "group member %d needs the per-cell kernels", (int)i);
+        if (variant == 2 && !fw::tile2d_supported(sims[i]->ds))
+            return fw::set_error(FW_ERR_UNSUPPORTED, "group member %d cannot run the tiled kernels (ny %% 4 != 0)",
+                                 (int)i);
     }
     fw_group* g = new fw_group();
     g->device = sims[0]->device;
-    g->width = width;
+    g->variant = variant == 0 ? 1 : variant;
+    g->width = variant == 2 ? 256 : width;
     g->chunk = chunk;
     g->sims.assign(sims, sims + n);
     for (fw_sim* s : g->sims) g->views.push_back(s->ds);
@@ -232,7 +284,7 @@ int fw_group_create(fw_sim* const* sims, int32_t n, int32_t width, int32_t chunk
         release(g);
         return fw::set_error(FW_ERR_DEVICE, "hipStreamCreateWithFlags failed: %s", hipGetErrorString(e));
     }
-    int rc = choose_shape(g, width == 0, chunk == 0);
+    int rc = choose_shape(g, variant == 0, variant != 2 && width == 0, chunk == 0);
     if (rc != FW_OK) {
         release(g);
         return rc;
@@ -241,8 +293,20 @@ int fw_group_create(fw_sim* const* sims, int32_t n, int32_t width, int32_t chunk
     return FW_OK;
 }
 
+int fw_group_create(fw_sim* const* sims, int32_t n, int32_t width, int32_t chunk, fw_group** out) {
+    return fw_group_create_shaped(sims, n, 1, width, chunk, out);
+}
+
 int fw_group_config(const fw_group* g, int32_t* width, int32_t* chunk) {
     if (!g || !width || !chunk) return fw::set_error(FW_ERR_INVALID, "null argument");
+    *width = g->width;
+    *chunk = g->chunk;
+    return FW_OK;
+}
+
+int fw_group_shape(const fw_group* g, int32_t* variant, int32_t* width, int32_t* chunk) {
+    if (!g || !variant || !width || !chunk) return fw::set_error(FW_ERR_INVALID, "null argument");
+    *variant = g->variant;
     *width = g->width;
     *chunk = g->chunk;
     return FW_OK;
@@ -267,8 +331,8 @@ int fw_group_run(fw_group* g, int64_t n_steps) {
     GR_HIP(hipMemcpyAsync(g->bounds, g->bounds_h.data(), g->bounds_h.size() * sizeof(fw::GroupBoundary),
                           hipMemcpyHostToDevice, g->stream));
     for (int64_t it = 0; it < n_steps; ++it) {
-        GR_HIP(fw::launch_group_2d(g->tiles, g->ntiles, g->blocks, g->width, g->max_ndmap, true, g->stream));
-        GR_HIP(fw::launch_group_2d(g->tiles, g->ntiles, g->blocks, g->width, g->max_ndmap, false, g->stream));
+        GR_HIP(launch_stencil(g, true));
+        GR_HIP(launch_stencil(g, false));
         GR_HIP(fw::launch_group_boundary(g->bounds, g->ntiles, g->max_points, it, g->stream));
     }
     GR_HIP(hipStreamSynchronize(g->stream));

@@ -196,6 +196,18 @@ struct GroupTile {
     int32_t chunk;  // rows per workgroup for this grid
 };
 
+// The entry of a grouped launch that owns workgroup blockIdx.x: the last with start <= it
+// (binary search over the table; every lane of the workgroup reads the same entries).
+__device__ inline int group_entry(const GroupTile* __restrict__ t, int n) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (t[mid].start <= blockIdx.x) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
 // One grid (or shot) of a grouped source/air/sensor pass.
 struct GroupBoundary {
     float* p;             // this shot's pressure field
@@ -236,6 +248,13 @@ unsigned group_tiles_x(int64_t ny, int width);
 unsigned group_tiles_y(int64_t nx, int chunk);
 hipError_t launch_group_2d(const GroupTile* t, int n, uint32_t blocks, int width, int max_ndmap, bool velocity,
                            hipStream_t st);
+// grouped tiled 2D kernels (isotropic relaxation, ny % 4 == 0): 256 columns x `rows` rows
+// (8, 16 or 32) per workgroup; coef_sets: the most coefficient sets any member stages in LDS
+// (ndmap, times 2 unless its axes share their sets); all_coded: every member has codes (the
+// kernel then holds the coded path only, fewer registers)
+unsigned group_tile_rows(int64_t nx, int rows);
+hipError_t launch_group_tile2d(const GroupTile* t, int n, uint32_t blocks, int rows, int coef_sets, bool all_coded,
+                               bool velocity, hipStream_t st);
 hipError_t launch_group_boundary(const GroupBoundary* g, int n, int64_t max_points, int64_t it, hipStream_t st);
 hipError_t launch_reciprocal(float* data, int64_t n, hipStream_t st);
 // [nshots][ncoords][ntic] -> [nshots][ntic][ncoords]

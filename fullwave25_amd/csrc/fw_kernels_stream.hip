@@ -357,16 +357,6 @@ __global__ __launch_bounds__(TY) __attribute__((amdgpu_waves_per_eu(5, 8))) void
 // grouped launches: the tiles of several grids (different sizes and media, isotropic
 // relaxation) in one launch. Workgroup b belongs to the last entry with start <= b.
 
-__device__ inline int group_entry(const GroupTile* __restrict__ t, int n) {
-    int lo = 0, hi = n - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (t[mid].start <= blockIdx.x) lo = mid;
-        else hi = mid - 1;
-    }
-    return lo;
-}
-
 template <int TY>
 __global__ __launch_bounds__(TY) void k_velocity_2d_group(const GroupTile* __restrict__ t, int n) {
     extern __shared__ float lds_coef[];

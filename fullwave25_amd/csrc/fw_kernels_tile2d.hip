@@ -154,16 +154,23 @@ __device__ inline int stage_coef(float* lds_coef, const DevSim& s) {
 }
 
 // velocity: p tile rows i0-7 .. i0+TX+7 (TX + 15 rows), columns j0-8 .. j0+TW+8
+template <int TX>
+struct VelTile {
+    static constexpr int PITCH = kTW + 16;
+    static constexpr int NROWS = TX + 15;
+    float P[NROWS * PITCH];
+};
+
+// The velocity update of tile (bx, by) of grid s: columns [bx * 256, bx * 256 + 256), rows
+// [8 + by * TX, 8 + by * TX + TX).
 template <int TX, bool UNIFORM, bool ISO, bool CODED>
-__global__ __launch_bounds__(kLanes) void k_velocity_tile(DevSim batch) {
-    const DevSim s = shot_view(batch, blockIdx.z);
-    constexpr int PITCH = kTW + 16;
-    constexpr int NROWS = TX + 15;
-    __shared__ __attribute__((aligned(16))) float P[NROWS * PITCH];
-    extern __shared__ float lds_coef[];
+__device__ inline void velocity_tile(const DevSim& s, unsigned bx, unsigned by, VelTile<TX>& L, float* lds_coef) {
+    constexpr int PITCH = VelTile<TX>::PITCH;
+    constexpr int NROWS = VelTile<TX>::NROWS;
+    float* P = L.P;
     const int64_t nx = s.nx, ny = s.ny;
-    const int64_t j0 = (int64_t)blockIdx.x * kTW;
-    const int64_t i0 = kHalo + (int64_t)blockIdx.y * TX;
+    const int64_t j0 = (int64_t)bx * kTW;
+    const int64_t i0 = kHalo + (int64_t)by * TX;
     const int axes = stage_coef<UNIFORM>(lds_coef, s);
     stage_tile(P, s.p, i0 - 7, NROWS, j0, 8, 8, PITCH, nx, ny);
     __syncthreads();
@@ -245,19 +252,28 @@ __global__ __launch_bounds__(kLanes) void k_velocity_tile(DevSim batch) {
 
 // pressure: vx tile rows i0-8 .. i0+TX+6 (TX + 15 rows), columns j0-4 .. j0+TW+4;
 //           vy tile rows i0-1 .. i0+TX (TX + 2 rows), columns j0-8 .. j0+TW+8
+template <int TX>
+struct PreTile {
+    static constexpr int XP = kTW + 8;
+    static constexpr int XR = TX + 15;
+    static constexpr int YP = kTW + 16;
+    static constexpr int YR = TX + 2;
+    float VX[XR * XP];
+    float VY[YR * YP];
+};
+
+// The pressure update of tile (bx, by) of grid s (as velocity_tile).
 template <int TX, bool UNIFORM, bool ISO, bool CODED>
-__global__ __launch_bounds__(kLanes) void k_pressure_tile(DevSim batch) {
-    const DevSim s = shot_view(batch, blockIdx.z);
-    constexpr int XP = kTW + 8;
-    constexpr int XR = TX + 15;
-    constexpr int YP = kTW + 16;
-    constexpr int YR = TX + 2;
-    __shared__ __attribute__((aligned(16))) float VX[XR * XP];
-    __shared__ __attribute__((aligned(16))) float VY[YR * YP];
-    extern __shared__ float lds_coef[];
+__device__ inline void pressure_tile(const DevSim& s, unsigned bx, unsigned by, PreTile<TX>& L, float* lds_coef) {
+    constexpr int XP = PreTile<TX>::XP;
+    constexpr int XR = PreTile<TX>::XR;
+    constexpr int YP = PreTile<TX>::YP;
+    constexpr int YR = PreTile<TX>::YR;
+    float* VX = L.VX;
+    float* VY = L.VY;
     const int64_t nx = s.nx, ny = s.ny;
-    const int64_t j0 = (int64_t)blockIdx.x * kTW;
-    const int64_t i0 = kHalo + (int64_t)blockIdx.y * TX;
+    const int64_t j0 = (int64_t)bx * kTW;
+    const int64_t i0 = kHalo + (int64_t)by * TX;
     const int axes = stage_coef<UNIFORM>(lds_coef, s);
     stage_tile(VX, s.v[0], i0 - 8, XR, j0, 4, 4, XP, nx, ny);
     stage_tile(VY, s.v[1], i0 - 1, YR, j0, 8, 8, YP, nx, ny);
@@ -335,6 +351,61 @@ __global__ __launch_bounds__(kLanes) void k_pressure_tile(DevSim batch) {
     }
 }
 
+template <int TX, bool UNIFORM, bool ISO, bool CODED>
+__global__ __launch_bounds__(kLanes) void k_velocity_tile(DevSim batch) {
+    __shared__ __attribute__((aligned(16))) VelTile<TX> L;
+    extern __shared__ float lds_coef[];
+    velocity_tile<TX, UNIFORM, ISO, CODED>(shot_view(batch, blockIdx.z), blockIdx.x, blockIdx.y, L, lds_coef);
+}
+
+template <int TX, bool UNIFORM, bool ISO, bool CODED>
+__global__ __launch_bounds__(kLanes) void k_pressure_tile(DevSim batch) {
+    __shared__ __attribute__((aligned(16))) PreTile<TX> L;
+    extern __shared__ float lds_coef[];
+    pressure_tile<TX, UNIFORM, ISO, CODED>(shot_view(batch, blockIdx.z), blockIdx.x, blockIdx.y, L, lds_coef);
+}
+
+// grouped launches: the tiles of several grids (different sizes and media, isotropic
+// relaxation) in one 1D launch. ALL_CODED: every grid reads coded coefficients; otherwise each
+// grid takes the coded or the per-cell-map path by its own handle.
+template <int TX, bool ALL_CODED>
+__global__ __launch_bounds__(kLanes) void k_velocity_tile_group(const GroupTile* __restrict__ t, int n) {
+    __shared__ __attribute__((aligned(16))) VelTile<TX> L;
+    extern __shared__ float lds_coef[];
+    const int m = group_entry(t, n);
+    const DevSim s = t[m].s;
+    const unsigned local = blockIdx.x - t[m].start;
+    const unsigned bx = local % t[m].gx, by = local / t[m].gx;
+    if (ALL_CODED || s.rcode) velocity_tile<TX, false, true, true>(s, bx, by, L, lds_coef);
+    else velocity_tile<TX, false, true, false>(s, bx, by, L, lds_coef);
+}
+
+template <int TX, bool ALL_CODED>
+__global__ __launch_bounds__(kLanes) void k_pressure_tile_group(const GroupTile* __restrict__ t, int n) {
+    __shared__ __attribute__((aligned(16))) PreTile<TX> L;
+    extern __shared__ float lds_coef[];
+    const int m = group_entry(t, n);
+    const DevSim s = t[m].s;
+    const unsigned local = blockIdx.x - t[m].start;
+    const unsigned bx = local % t[m].gx, by = local / t[m].gx;
+    if (ALL_CODED || s.rcode) pressure_tile<TX, false, true, true>(s, bx, by, L, lds_coef);
+    else pressure_tile<TX, false, true, false>(s, bx, by, L, lds_coef);
+}
+
+template <int TX, bool ALL_CODED>
+hipError_t launch_group_tile(const GroupTile* t, int n, uint32_t blocks, int lds, bool velocity, hipStream_t st) {
+    if (velocity) hipLaunchKernelGGL((k_velocity_tile_group<TX, ALL_CODED>), dim3(blocks), dim3(kLanes), lds, st, t, n);
+    else hipLaunchKernelGGL((k_pressure_tile_group<TX, ALL_CODED>), dim3(blocks), dim3(kLanes), lds, st, t, n);
+    return hipGetLastError();
+}
+
+template <int TX>
+hipError_t launch_group_tile_any(const GroupTile* t, int n, uint32_t blocks, int lds, bool all_coded, bool velocity,
+                                 hipStream_t st) {
+    return all_coded ? launch_group_tile<TX, true>(t, n, blocks, lds, velocity, st)
+                     : launch_group_tile<TX, false>(t, n, blocks, lds, velocity, st);
+}
+
 template <int TX>
 hipError_t launch_tile(const DevSim& s, bool velocity, hipStream_t st) {
     const bool u = s.ndmap == 1;
@@ -365,6 +436,19 @@ hipError_t launch_tile(const DevSim& s, bool velocity, hipStream_t st) {
 
 bool tile2d_supported(const DevSim& s) {
     return !s.aexp && s.ndim == 2 && s.ny % 4 == 0 && s.ndmap <= 1024 && s.nx * s.ny < ((int64_t)1 << 30);
+}
+
+unsigned group_tile_rows(int64_t nx, int rows) { return (unsigned)((nx - 2 * kHalo + rows - 1) / rows); }
+
+hipError_t launch_group_tile2d(const GroupTile* t, int n, uint32_t blocks, int rows, int coef_sets, bool all_coded,
+                               bool velocity, hipStream_t st) {
+    if (n <= 0 || blocks == 0) return hipSuccess;
+    const int lds = coef_sets * kNCoef * 4;  // the largest member's staged sets (stage_coef)
+    switch (rows) {
+        case 8: return launch_group_tile_any<8>(t, n, blocks, lds, all_coded, velocity, st);
+        case 32: return launch_group_tile_any<32>(t, n, blocks, lds, all_coded, velocity, st);
+        default: return launch_group_tile_any<16>(t, n, blocks, lds, all_coded, velocity, st);
+    }
 }
 
 hipError_t launch_tile2d(const DevSim& s, const LaunchCfg& cfg, bool velocity, hipStream_t st) {

@@ -34,7 +34,8 @@ EXPORTED_SYMBOLS = (
     "fw_sim_record_count", "fw_sim_download_sensors", "fw_sim_download_field",
     "fw_sim_set_field", "fw_sim_kernel_config", "fw_sim_set_kernel_config", "fw_sim_set_graph_steps",
     "fw_sim_set_relaxation_codes", "fw_sim_relaxation_codes", "fw_sim_run_timed",
-    "fw_sim_destroy", "fw_group_create", "fw_group_config", "fw_group_run", "fw_group_destroy",
+    "fw_sim_destroy", "fw_group_create", "fw_group_config", "fw_group_create_shaped", "fw_group_shape",
+    "fw_group_run", "fw_group_destroy",
     "fw_adj_create", "fw_adj_forward", "fw_adj_backward", "fw_adj_destroy",
 )
 VARIANTS = {"percell": 0, "stream": 1, "tile": 2, "pipe": 3, "pipelite": 4}
@@ -103,6 +104,8 @@ def load_library(path: Path | None = None) -> ctypes.CDLL:
         lib.fw_group_create.argtypes = [ctypes.POINTER(vp), i32, i32, i32, ctypes.POINTER(vp)]
         lib.fw_group_run.argtypes = [vp, i64]
         lib.fw_group_config.argtypes = [vp, ctypes.POINTER(i32), ctypes.POINTER(i32)]
+        lib.fw_group_create_shaped.argtypes = [ctypes.POINTER(vp), i32, i32, i32, i32, ctypes.POINTER(vp)]
+        lib.fw_group_shape.argtypes = [vp, ctypes.POINTER(i32), ctypes.POINTER(i32), ctypes.POINTER(i32)]
         lib.fw_group_destroy.argtypes = [vp]
         lib.fw_adj_create.argtypes = [vp, i64, ctypes.POINTER(vp)]
         lib.fw_adj_forward.argtypes = [vp]
@@ -343,18 +346,27 @@ class HipGroup:
     """Several 2D isotropic simulations on one device (any grid sizes, media and shot counts)
     advanced together: each time step is one velocity, one pressure and one source/sensor
     launch over all of them (``fw_group_*``). Members keep their own records
-    (``member.sensors()``), bit-identical to ``member.run`` with the kernel shape
-    ``stream:<width>:<chunk>`` (``kernel_config()``; ``width``/``chunk`` 0: timed on the group
-    when every member is fresh). Members not yet set up skip their own autotuning.
+    (``member.sensors()``), bit-identical to ``member.run`` with the group's kernel shape
+    (``kernel_config()``): ``variant="stream"``, the x-marching kernels ``stream:<width>:<chunk>``;
+    ``variant="tile"``, the tiled kernels ``tile:<chunk>`` (members with ny % 4 == 0);
+    ``variant="auto"``, whichever family and shape runs fastest. ``width``/``chunk`` 0: timed on
+    the group when every member is fresh. Members not yet set up skip their own autotuning.
     Members may be at different steps (a job that joins a running group) and may run alone
     between group runs; they must stay open while the group exists."""
 
-    def __init__(self, sims: list[HipSimulation], *, width: int = 0, chunk: int = 0) -> None:
+    _FAMILIES = {"auto": 0, "stream": 1, "tile": 2}
+
+    def __init__(self, sims: list[HipSimulation], *, width: int = 0, chunk: int = 0, variant: str = "stream") -> None:
         self.lib = load_library()
         self.sims = list(sims)
         self._h = ctypes.c_void_p()
+        if variant not in self._FAMILIES:
+            raise ValueError(f"variant must be one of {sorted(self._FAMILIES)}, got {variant!r}")
         # members not yet set up skip their own autotuning (the group runs its own shape)
-        shape = {"variant": "stream", "width": int(width) or 256, "chunk": int(chunk) or 16}
+        if variant == "tile":
+            shape = {"variant": "tile", "width": 256, "chunk": int(chunk) or 16}
+        else:
+            shape = {"variant": "stream", "width": int(width) or 256, "chunk": int(chunk) or 16}
         for sim in self.sims:
             try:
                 sim.set_kernel_config({"velocity": shape, "pressure": shape})
@@ -362,13 +374,14 @@ class HipGroup:
                 if e.status != 5:  # FW_ERR_STATE: already set up, keeps its own kernels
                     raise
         arr = (ctypes.c_void_p * len(self.sims))(*[sim._h.value for sim in self.sims])
-        _check(self.lib, self.lib.fw_group_create(arr, len(self.sims), int(width), int(chunk), ctypes.byref(self._h)))
+        _check(self.lib, self.lib.fw_group_create_shaped(arr, len(self.sims), self._FAMILIES[variant], int(width),
+                                                         int(chunk), ctypes.byref(self._h)))
 
     def kernel_config(self) -> dict:
         """The group's kernel shape, as ``HipSimulation.kernel_config`` reports one."""
-        w, c = ctypes.c_int32(), ctypes.c_int32()
-        _check(self.lib, self.lib.fw_group_config(self._h, ctypes.byref(w), ctypes.byref(c)))
-        shape = {"variant": "stream", "width": w.value, "chunk": c.value}
+        v, w, c = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        _check(self.lib, self.lib.fw_group_shape(self._h, ctypes.byref(v), ctypes.byref(w), ctypes.byref(c)))
+        shape = {"variant": "tile" if v.value == 2 else "stream", "width": w.value, "chunk": c.value}
         return {"velocity": shape, "pressure": shape}
 
     def run(self, n_steps: int) -> None:

@@ -145,3 +145,71 @@ def test_group_against_oracle():
     finally:
         for s in sims:
             s.close()
+
+
+def _tile_media():
+    """Four media with ny % 4 == 0 (the tiled kernels' row alignment)."""
+    return _media()[:3] + [workloads.random_medium_2d(37, 128, 180, seed=63, n_src=9, n_sensors=40).inputs]
+
+
+@pytest.mark.parametrize("rows,uncoded", [(8, 1), (16, None), (32, 1), (8, None)])
+def test_tiled_group_matches_members_run_alone(monkeypatch, rows, uncoded):
+    """The tiled group kernels: each member's records equal its own run with tile:<rows>, bit
+    for bit; every member coded (the kernels then hold the coded path only), or one member on
+    its per-cell maps."""
+    media = _tile_media()
+    assert all(m.ny % 4 == 0 for m in media)
+    k = min(m.nt for m in media)
+    sims = [engine.HipSimulation(m) for m in media]
+    if uncoded is not None:
+        sims[uncoded].set_relaxation_codes(False)
+    try:
+        with engine.HipGroup(sims, variant="tile", chunk=rows) as grp:
+            assert grp.kernel_config()["pressure"] == {"variant": "tile", "width": 256, "chunk": rows}
+            grp.run(k // 3)
+            grp.run(k - k // 3)
+        for i, (m, s) in enumerate(zip(media, sims)):
+            want = _alone(m, k, monkeypatch, codes=(i != uncoded), shape=f"tile:{rows}")
+            print(f"member {i}: shape {m.shape}, {k} steps, max |rec| {np.abs(want).max():.3e}", flush=True)
+            assert np.abs(want).max() > 0
+            np.testing.assert_array_equal(s.sensors(), want)
+    finally:
+        for s in sims:
+            s.close()
+
+
+def test_timed_family_with_batched_member(monkeypatch):
+    """variant="auto" times both families on fresh members (one of them a batch of three
+    shots); every member equals its own run with the chosen family and shape."""
+    a = workloads.random_medium_2d(60, 52, 120, seed=64).inputs
+    b = workloads.random_medium_2d(80, 60, 120, seed=65).inputs
+    rng = np.random.default_rng(66)
+    shots = (a.icmat[None] * (1 + rng.random((3, 1, 1)))).astype(np.float32)
+    sa = engine.HipSimulation(a, shots=shots)
+    sb = engine.HipSimulation(b)
+    try:
+        with engine.HipGroup([sa, sb], variant="auto") as grp:
+            cfg = grp.kernel_config()["velocity"]
+            grp.run(a.nt)
+        shape = f"tile:{cfg['chunk']}" if cfg["variant"] == "tile" else f"stream:{cfg['width']}:{cfg['chunk']}"
+        print(f"timed group shape {shape}", flush=True)
+        monkeypatch.setenv("FULLWAVE_HIP_KERNELS", shape)
+        with engine.HipSimulation(a, shots=shots) as alone:
+            alone.run(a.nt)
+            np.testing.assert_array_equal(sa.sensors(), alone.sensors())
+        monkeypatch.delenv("FULLWAVE_HIP_KERNELS")
+        np.testing.assert_array_equal(sb.sensors(), _alone(b, b.nt, monkeypatch, shape=shape))
+    finally:
+        sa.close()
+        sb.close()
+
+
+def test_tiled_group_rejects_unaligned_member():
+    ok = workloads.random_medium_2d(60, 52, 50, seed=61).inputs
+    bad = workloads.random_medium_2d(37, 131, 50, seed=63).inputs  # ny % 4 == 3
+    with engine.HipSimulation(ok) as a, engine.HipSimulation(bad) as b:
+        with pytest.raises(engine.EngineError, match="ny % 4"):
+            engine.HipGroup([a, b], variant="tile")
+        with engine.HipGroup([a, b], variant="auto") as grp:  # times the marching family only
+            assert grp.kernel_config()["velocity"]["variant"] == "stream"
+            grp.run(5)
