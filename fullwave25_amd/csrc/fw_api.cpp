@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
+#include <set>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -186,6 +188,49 @@ bool is_voxel_map(const fw_desc& d, const std::string& name) {
     return false;
 }
 
+// Device memory of a handle is stream-ordered on the handle's stream: a free does not wait
+// for the whole device (other handles' kernels), and the device's memory pool keeps freed
+// blocks (up to kPoolKeep bytes) for the next handle, so a server creating and destroying many
+// small handles does not pay an allocation and a device-wide wait per array. Where the
+// runtime has no stream-ordered allocator, hipMalloc.
+constexpr uint64_t kPoolKeep = 8ull << 30;
+
+hipError_t dev_malloc(fw_sim* s, void** p, size_t bytes) {
+    static std::mutex mu;
+    static std::set<int> pools_set;
+    {
+        std::lock_guard<std::mutex> lock(mu);
+        if (pools_set.insert(s->device).second) {
+            hipMemPool_t pool = nullptr;
+            if (hipDeviceGetDefaultMemPool(&pool, s->device) == hipSuccess && pool) {
+                uint64_t keep = kPoolKeep;
+                (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+            }
+            (void)hipGetLastError();
+        }
+    }
+    hipError_t e = hipMallocAsync(p, bytes, s->stream);
+    if (e == hipSuccess) return e;
+    (void)hipGetLastError();
+    e = hipMalloc(p, bytes);
+    if (e == hipSuccess) s->plain_alloc.insert(*p);
+    return e;
+}
+
+void dev_free(fw_sim* s, void* p) {
+    if (!p) return;
+    auto it = s->plain_alloc.find(p);
+    if (it != s->plain_alloc.end()) {
+        s->plain_alloc.erase(it);
+        (void)hipStreamSynchronize(s->stream);
+        (void)hipFree(p);
+    } else if (hipFreeAsync(p, s->stream) != hipSuccess) {
+        (void)hipGetLastError();
+        (void)hipStreamSynchronize(s->stream);
+        (void)hipFree(p);
+    }
+}
+
 int alloc(fw_sim* s, const std::string& name, size_t bytes, void** out) {
     auto it = s->dev.find(name);
     if (it != s->dev.end()) {
@@ -193,7 +238,7 @@ int alloc(fw_sim* s, const std::string& name, size_t bytes, void** out) {
         return FW_OK;
     }
     void* p = nullptr;
-    hipError_t e = hipMalloc(&p, bytes ? bytes : 4);
+    hipError_t e = dev_malloc(s, &p, bytes ? bytes : 4);
     if (e != hipSuccess) return fail(FW_ERR_NOMEM, "hipMalloc(%zu) for '%s' failed: %s", bytes, name.c_str(), hipGetErrorString(e));
     s->dev[name] = p;
     *out = p;
@@ -413,7 +458,7 @@ void drop_codes(fw_sim* s) {
     for (const char* name : {"_rcode", "_rtab_v", "_rtab_p"}) {
         auto it = s->dev.find(name);
         if (it == s->dev.end()) continue;
-        (void)hipFree(it->second);
+        dev_free(s, it->second);
         s->dev.erase(it);
     }
     s->ds.rcode = nullptr;
@@ -697,7 +742,7 @@ int launch_step(fw_sim* s, bool timed, double* ms) {
 // Capture graph_steps steps (velocity, pressure, boundary pass with the device step counter,
 // counter advance) into one graph, once per handle.
 int capture_graph(fw_sim* s) {
-    if (!s->step_dev) FW_HIP(hipMalloc(&s->step_dev, sizeof(int64_t)));
+    if (!s->step_dev) FW_HIP(dev_malloc(s, (void**)&s->step_dev, sizeof(int64_t)));
     hipGraph_t graph = nullptr;
     FW_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
     hipError_t e = hipSuccess;
@@ -900,7 +945,7 @@ int fw_sim_create(const fw_desc* desc, int device, fw_sim** out) {
     const bool relax = d.attenuation_model == 0;  // the exponential model has no memory variables
     const int64_t field = s->ncell * nshots;  // one wave field of every shot
     s->nstate = (int64_t)(1 + nd + (relax ? 4 * nd : 0)) * field;
-    if (hipMalloc(&s->state, (size_t)s->nstate * 4) != hipSuccess)
+    if (dev_malloc(s, (void**)&s->state, (size_t)s->nstate * 4) != hipSuccess)
         return cleanup(fail(FW_ERR_NOMEM, "hipMalloc of %lld state floats failed", (long long)s->nstate));
     if (fill_zero_sync(s, s->state, (size_t)s->nstate * 4) != hipSuccess)
         return cleanup(fail(FW_ERR_DEVICE, "hipMemset failed"));
@@ -1153,9 +1198,10 @@ int fw_sim_destroy(fw_sim* s) {
     DeviceGuard g(s->device);
     if (s->stream) (void)hipStreamSynchronize(s->stream);
     if (s->graph) (void)hipGraphExecDestroy(s->graph);
-    if (s->step_dev) (void)hipFree(s->step_dev);
-    for (auto& kv : s->dev) (void)hipFree(kv.second);
-    if (s->state) (void)hipFree(s->state);
+    dev_free(s, s->step_dev);
+    for (auto& kv : s->dev) dev_free(s, kv.second);
+    dev_free(s, s->state);
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
     for (auto& e : s->ev)
         if (e) (void)hipEventDestroy(e);
     if (s->stream) (void)hipStreamDestroy(s->stream);

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <map>
+#include <set>
 #include <string>
 #include <vector>
 
@@ -27,6 +28,7 @@ struct fw_sim {
     float* state = nullptr;  // p, v[ndim], psi[ndim][2], phi[ndim][2]
     int64_t nstate = 0;
     std::map<std::string, void*> dev;  // every other device allocation, by input name
+    std::set<void*> plain_alloc;       // allocations made with hipMalloc (see dev_malloc)
     std::map<std::string, bool> have;
     std::vector<int32_t> icc, outc, zero;
     std::vector<float> dmap;
