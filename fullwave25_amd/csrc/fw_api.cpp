@@ -99,20 +99,93 @@ struct Stage {
     }
 };
 
+hipError_t copy_sync(const fw_sim* s, void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, s->stream);
+    return e != hipSuccess ? e : hipStreamSynchronize(s->stream);
+}
+
+// Host-to-device copies go through pinned staging buffers: a copy from pageable memory pays a
+// fixed cost per call in the runtime (about half a millisecond for a map of a few MB), and a
+// handle's maps are a dozen such copies. Buffers of kPinnedBytes come from a process-wide pool
+// (allocated on first use and kept, at most kPinnedKeep); a copy takes two, so the host fills
+// one while the DMA drains the other. Without pinned memory: the pageable copy.
+constexpr size_t kPinnedBytes = size_t(4) << 20;
+constexpr size_t kPinnedKeep = 16;
+
+std::mutex g_pinned_mu;
+std::vector<void*> g_pinned_free;
+
+void* pinned_get() {
+    {
+        std::lock_guard<std::mutex> lock(g_pinned_mu);
+        if (!g_pinned_free.empty()) {
+            void* b = g_pinned_free.back();
+            g_pinned_free.pop_back();
+            return b;
+        }
+    }
+    void* b = nullptr;
+    if (hipHostMalloc(&b, kPinnedBytes, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    return b;
+}
+
+void pinned_put(void* b) {
+    if (!b) return;
+    {
+        std::lock_guard<std::mutex> lock(g_pinned_mu);
+        if (g_pinned_free.size() < kPinnedKeep) {
+            g_pinned_free.push_back(b);
+            return;
+        }
+    }
+    (void)hipHostFree(b);
+}
+
+// synchronous: returns when the data is on the device
+hipError_t h2d(const fw_sim* s, void* dst, const void* src, size_t bytes) {
+    if (bytes < (size_t(256) << 10)) return copy_sync(s, dst, src, bytes, hipMemcpyHostToDevice);
+    void* buf[2] = {pinned_get(), pinned_get()};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    hipError_t e = buf[0] && buf[1] ? hipSuccess : hipErrorOutOfMemory;
+    for (int i = 0; i < 2 && e == hipSuccess; ++i) e = hipEventCreateWithFlags(&ev[i], hipEventDisableTiming);
+    bool used[2] = {false, false};
+    size_t off = 0;
+    for (int i = 0; off < bytes && e == hipSuccess; i ^= 1) {
+        const size_t n = bytes - off < kPinnedBytes ? bytes - off : kPinnedBytes;
+        if (used[i]) e = hipEventSynchronize(ev[i]);  // the DMA out of buf[i] has finished
+        if (e != hipSuccess) break;
+        memcpy(buf[i], (const char*)src + off, n);
+        e = hipMemcpyAsync((char*)dst + off, buf[i], n, hipMemcpyHostToDevice, s->stream);
+        if (e == hipSuccess) e = hipEventRecord(ev[i], s->stream);
+        used[i] = true;
+        off += n;
+    }
+    const hipError_t e2 = hipStreamSynchronize(s->stream);
+    for (hipEvent_t x : ev)
+        if (x) (void)hipEventDestroy(x);
+    pinned_put(buf[0]);
+    pinned_put(buf[1]);
+    if (!used[0] && e == hipErrorOutOfMemory) {  // no pinned memory: the pageable copy
+        (void)hipGetLastError();
+        return copy_sync(s, dst, src, bytes, hipMemcpyHostToDevice);
+    }
+    return e != hipSuccess ? e : e2;
+}
+
 // compact host grid -> padded device grid (elements of `esize` bytes)
 hipError_t copy_grid_in(const fw_sim* s, void* dst, const void* src, size_t esize) {
     const size_t row = (size_t)s->row_len * esize, prow = (size_t)s->pitch * esize, rows = (size_t)s->rows;
     hipError_t e;
-    if (row == prow) {
-        e = hipMemcpyAsync(dst, src, row * rows, hipMemcpyHostToDevice, s->stream);
-        return e != hipSuccess ? e : hipStreamSynchronize(s->stream);
-    }
+    if (row == prow) return h2d(s, dst, src, row * rows);
     const size_t per = kStageBytes / row > 0 ? kStageBytes / row : 1;
     Stage st;
     e = st.get(s, per * row);
     for (size_t r0 = 0; r0 < rows && e == hipSuccess; r0 += per) {
         const size_t n = per < rows - r0 ? per : rows - r0;
-        e = hipMemcpyAsync(st.p, (const char*)src + r0 * row, n * row, hipMemcpyHostToDevice, s->stream);
+        e = h2d(s, st.p, (const char*)src + r0 * row, n * row);
         if (e == hipSuccess)
             e = hipMemcpy2DAsync((char*)dst + r0 * prow, prow, st.p, row, row, n, hipMemcpyDeviceToDevice, s->stream);
     }
@@ -140,11 +213,6 @@ hipError_t copy_grid_out(const fw_sim* s, void* dst, const void* src, size_t esi
     const hipError_t e2 = st.release(s);
     const hipError_t e3 = hipStreamSynchronize(s->stream);
     return e != hipSuccess ? e : e2 != hipSuccess ? e2 : e3;
-}
-
-hipError_t copy_sync(const fw_sim* s, void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
-    hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, s->stream);
-    return e != hipSuccess ? e : hipStreamSynchronize(s->stream);
 }
 
 hipError_t fill_zero_sync(const fw_sim* s, void* dst, size_t bytes) {
@@ -235,6 +303,12 @@ int alloc(fw_sim* s, const std::string& name, size_t bytes, void** out) {
     auto it = s->dev.find(name);
     if (it != s->dev.end()) {
         *out = it->second;
+        return FW_OK;
+    }
+    auto a = s->arena_slot.find(name);
+    if (a != s->arena_slot.end() && bytes == (size_t)s->ncell * 4) {
+        s->dev[name] = a->second;
+        *out = a->second;
         return FW_OK;
     }
     void* p = nullptr;
@@ -949,6 +1023,15 @@ int fw_sim_create(const fw_desc* desc, int device, fw_sim** out) {
         return cleanup(fail(FW_ERR_NOMEM, "hipMalloc of %lld state floats failed", (long long)s->nstate));
     if (fill_zero_sync(s, s->state, (size_t)s->nstate * 4) != hipSuccess)
         return cleanup(fail(FW_ERR_DEVICE, "hipMemset failed"));
+    {  // the per-cell input maps in one allocation, whose slots alloc() hands out
+        const size_t slot = ((size_t)s->ncell * 4 + 255) / 256 * 256;
+        std::vector<std::string> grid_maps;
+        for (const std::string& n : required_maps(d))
+            if (n != "dmap" && n != "icmat" && n != "icc" && n != "outc" && n != "icczero") grid_maps.push_back(n);
+        if (dev_malloc(s, &s->arena, slot * grid_maps.size()) != hipSuccess)
+            return cleanup(fail(FW_ERR_NOMEM, "device allocation of %zu input maps failed", grid_maps.size()));
+        for (size_t i = 0; i < grid_maps.size(); ++i) s->arena_slot[grid_maps[i]] = (char*)s->arena + i * slot;
+    }
     float* f = s->state;
     s->ds.p = f;
     f += field;
@@ -1028,7 +1111,7 @@ int fw_sim_upload(fw_sim* s, const char* name_c, const void* host, size_t bytes)
         if (bytes) {
             Stage src;
             hipError_t e = src.get(s, bytes);
-            if (e == hipSuccess) e = hipMemcpyAsync(src.p, host, bytes, hipMemcpyHostToDevice, s->stream);
+            if (e == hipSuccess) e = h2d(s, src.p, host, bytes);
             if (e == hipSuccess)
                 e = fw::launch_transpose_icmat((const float*)src.p, (float*)p, s->nshots, d.ncoords, d.ntic, s->stream);
             const hipError_t e2 = src.release(s);
@@ -1199,7 +1282,9 @@ int fw_sim_destroy(fw_sim* s) {
     if (s->stream) (void)hipStreamSynchronize(s->stream);
     if (s->graph) (void)hipGraphExecDestroy(s->graph);
     dev_free(s, s->step_dev);
-    for (auto& kv : s->dev) dev_free(s, kv.second);
+    for (auto& kv : s->dev)
+        if (!s->arena_slot.count(kv.first)) dev_free(s, kv.second);
+    dev_free(s, s->arena);
     dev_free(s, s->state);
     if (s->stream) (void)hipStreamSynchronize(s->stream);
     for (auto& e : s->ev)

@@ -29,6 +29,8 @@ struct fw_sim {
     int64_t nstate = 0;
     std::map<std::string, void*> dev;  // every other device allocation, by input name
     std::set<void*> plain_alloc;       // allocations made with hipMalloc (see dev_malloc)
+    void* arena = nullptr;             // every per-cell input map, one allocation
+    std::map<std::string, void*> arena_slot;
     std::map<std::string, bool> have;
     std::vector<int32_t> icc, outc, zero;
     std::vector<float> dmap;
