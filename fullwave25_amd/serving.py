@@ -21,7 +21,9 @@ group's kernel shape.
 
 Kernel shapes: the first handle of each grid shape times the candidates (the engine's
 autotuner); later jobs of that shape reuse the choice, so a job's numbers do not depend on
-what else is running. Grouped jobs use the group's shape (``stream:256:16``).
+what else is running. Grouped jobs use a fixed group shape chosen by the job's own grid: the
+tiled kernels ``tile:8`` where the rows are 16-byte aligned (ny % 4 == 0), else the marching
+kernels ``stream:256:16``; the batcher keeps one group per shape.
 
 Latency of a job is measured from ``submit`` to its result being available.
 """
@@ -99,7 +101,15 @@ class ServerStats:
     records: list = field(repr=False, default_factory=list)
 
 
-GROUP_WIDTH, GROUP_CHUNK = 256, 16  # kernel shape of grouped jobs
+GROUP_WIDTH, GROUP_CHUNK = 256, 16  # marching kernel shape of grouped jobs
+GROUP_TILE_ROWS = 8  # tiled kernel shape of grouped jobs with ny % 4 == 0
+
+
+def _group_shape(inp: SimulationInputs) -> dict:
+    """The kernel shape a grouped job runs with (depends on the job's own grid only)."""
+    if inp.ny % 4 == 0:
+        return {"variant": "tile", "width": 256, "chunk": GROUP_TILE_ROWS}
+    return {"variant": "stream", "width": GROUP_WIDTH, "chunk": GROUP_CHUNK}
 # Jobs of more cells (all shots together) than this fill the GPU on their own, where their own
 # tuned kernels (the float4 tiles) outrun the grouped marching kernels: they run alone.
 GROUP_MAX_CELLS = 1 << 21
@@ -291,7 +301,7 @@ class SimulationServer:
         shape for grouped jobs, else the one tuned on the first handle of the shape (None if no
         such job has run yet)."""
         if self._grouped(inp, n_shots):
-            shape = {"variant": "stream", "width": GROUP_WIDTH, "chunk": GROUP_CHUNK}
+            shape = _group_shape(inp)
             return {"velocity": shape, "pressure": shape}
         with self._lock:
             return self._configs.get(_shape_key(inp, n_shots))
@@ -311,7 +321,7 @@ class SimulationServer:
         n_shots = 1 if shots is None else int(shots.shape[0])
         if self._grouped(inp, n_shots):
             with engine.HipSimulation(inp, self.device, shots=shots) as sim:
-                shape = {"variant": "stream", "width": GROUP_WIDTH, "chunk": GROUP_CHUNK}
+                shape = _group_shape(inp)
                 sim.set_kernel_config({"velocity": shape, "pressure": shape})
                 sim.relaxation_codes()  # first-run set-up here, on this worker's thread
                 done = threading.Event()
@@ -334,9 +344,10 @@ class SimulationServer:
 
     def _batch_loop(self) -> None:
         """Advance the grouped jobs: a quantum of steps (or fewer, up to the next job to finish)
-        per round; arrivals join and finished jobs leave between rounds."""
-        members: list = []  # [sim, steps left, done event, failure list]
-        group = None
+        per round; arrivals join and finished jobs leave between rounds. Jobs of each kernel
+        shape form their own group; a round advances every group by the same steps."""
+        members: list = []  # [sim, steps left, done event, failure list, shape]
+        groups: dict = {}   # shape variant -> HipGroup of the members with that shape
         while True:
             with self._batch_lock:
                 while not self._batch_in and not members:
@@ -344,20 +355,27 @@ class SimulationServer:
                         return
                     self._batch_lock.wait()
                 new, self._batch_in = self._batch_in, []
-            if new:
-                members += [list(m) for m in new]
-                if group is not None:
-                    group.close()
-                    group = None
+            changed = set()
+            for sim, steps, done, failure in new:
+                shape = sim.kernel_config()["velocity"]
+                members.append([sim, steps, done, failure, shape["variant"]])
+                changed.add(shape["variant"])
             try:
-                if group is None:
-                    group = engine.HipGroup([m[0] for m in members], width=GROUP_WIDTH, chunk=GROUP_CHUNK)
+                for v in changed:
+                    if v in groups:
+                        groups.pop(v).close()
+                for v in {m[4] for m in members}:
+                    if v not in groups:
+                        mine = [m[0] for m in members if m[4] == v]
+                        groups[v] = (engine.HipGroup(mine, variant="tile", chunk=GROUP_TILE_ROWS) if v == "tile" else
+                                     engine.HipGroup(mine, width=GROUP_WIDTH, chunk=GROUP_CHUNK))
                 q = min(self.quantum_steps, min(m[1] for m in members))
-                group.run(q)
+                for grp in groups.values():
+                    grp.run(q)
             except Exception as exc:  # every member of the failed round fails; the server goes on
-                if group is not None:
-                    group.close()
-                    group = None
+                for grp in groups.values():
+                    grp.close()
+                groups = {}
                 for m in members:
                     m[3].append(exc)
                     m[2].set()
@@ -367,8 +385,8 @@ class SimulationServer:
                 m[1] -= q
             finished = [m for m in members if m[1] == 0]
             if finished:
-                group.close()
-                group = None
+                for v in {m[4] for m in finished}:
+                    groups.pop(v).close()
                 members = [m for m in members if m[1] > 0]
                 for m in finished:
                     m[2].set()

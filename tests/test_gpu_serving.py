@@ -23,15 +23,18 @@ def _jobs():
     d = workloads.random_medium_2d(50, 46, 180, seed=53, exponential=True).inputs
     rng = np.random.default_rng(54)
     shots = np.stack([a.icmat * rng.uniform(0.5, 2.0) for _ in range(3)]).astype(np.float32)
+    e = workloads.random_medium_2d(40, 131, 150, seed=56).inputs  # ny % 4 != 0: the marching group
     # each job twice, interleaved, plus a batch of transmits on a's medium
-    return [(a, None), (b, None), (c, None), (d, None), (a, shots), (b, None), (c, None), (d, None), (a, None)]
+    return [(a, None), (b, None), (c, None), (d, None), (a, shots), (b, None), (c, None), (d, None), (a, None),
+            (e, None)]
 
 
 @pytest.mark.parametrize("batching", ["group", "mixed", "off"])
 def test_served_mix_equals_alone_and_oracle(batching):
-    """With batching, the 2D isotropic jobs (a, b and the batch on a's medium) run as members
-    of the server's shared group while the 3D and exponential jobs run on their own handles;
-    "mixed" caps the group at a's size, so that b and the batch run alone beside it."""
+    """With batching, the 2D isotropic jobs (a, b and the batch on a's medium in the tiled
+    group, e in the marching group) run as group members while the 3D and exponential jobs run
+    on their own handles; "mixed" caps the groups at a's size, so that b, e and the batch run
+    alone beside them."""
     jobs = _jobs()
     a, b = jobs[0][0], jobs[1][0]
     limit = a.ncells if batching == "mixed" else 1 << 21
@@ -43,10 +46,12 @@ def test_served_mix_equals_alone_and_oracle(batching):
         results = [f.result(timeout=300) for f in futs]
         configs = [srv.kernel_config_for(inp, 1 if s is None else s.shape[0]) for inp, s in jobs]
         grouped = [srv._grouped(inp, 1 if s is None else s.shape[0]) for inp, s in jobs]
-    want = {"group": [True, True, False, False, True, True, False, False, True],
-            "mixed": [True, False, False, False, False, False, False, False, True],
-            "off": [False] * 9}[batching]
+    want = {"group": [True, True, False, False, True, True, False, False, True, True],
+            "mixed": [True, False, False, False, False, False, False, False, True, False],
+            "off": [False] * 10}[batching]
     assert grouped == want
+    if batching == "group":  # tiled group for aligned rows, marching group for the rest
+        assert configs[0]["velocity"]["variant"] == "tile" and configs[9]["velocity"]["variant"] == "stream"
     st = srv.stats()
     print(f"served {st.jobs} jobs: p50 {st.latency_p50_s * 1e3:.1f} ms, p99 {st.latency_p99_s * 1e3:.1f} ms, "
           f"peak running {srv.peak_running}, peak bytes {srv.peak_in_use}", flush=True)
