@@ -83,3 +83,22 @@ def test_failure_is_isolated():
         assert ok2.result(timeout=30)[0, 0] == 60
     st = srv.stats()
     assert st.jobs == 2 and st.failed == 1
+
+
+def test_group_membership_and_shapes():
+    """Which forward jobs join the batcher's groups, and with which fixed kernel shape: 2D
+    isotropic jobs up to the cell cap (all shots together) join, the tiled shape for rows of a
+    multiple of 4 cells and the marching shape otherwise; larger, 3D and exponential jobs run
+    alone on the shape tuned for them (none before their first handle)."""
+    aligned, unaligned, larger = _inp(40), _inp(41), _inp(48)
+    cube = raw_inputs((20, 20, 20), nt=10, icc=[[10, 10, 10]], icmat=np.ones((1, 10)), outc=[[11, 11, 11]])
+    expo = raw_inputs((40, 40), nt=10, icc=[[20, 20]], icmat=np.ones((1, 10)), outc=[[21, 21]], a_exp=0.999)
+    with SimulationServer(hbm_budget=10**9, batching=True, group_max_cells=45 * 45) as srv:
+        assert srv._grouped(aligned, 1) and srv._grouped(unaligned, 1)
+        assert not srv._grouped(aligned, 2)  # two shots: 3200 cells
+        assert not srv._grouped(larger, 1) and not srv._grouped(cube, 1) and not srv._grouped(expo, 1)
+        assert srv.kernel_config_for(aligned)["velocity"] == {"variant": "tile", "width": 256, "chunk": 8}
+        assert srv.kernel_config_for(unaligned)["pressure"] == {"variant": "stream", "width": 256, "chunk": 16}
+        assert srv.kernel_config_for(larger) is None
+    with SimulationServer(hbm_budget=10**9, batching=False) as srv:
+        assert not srv._grouped(aligned, 1) and srv.kernel_config_for(aligned) is None
