@@ -195,6 +195,26 @@ def test_unequal_spacing(monkeypatch, shape):
     assert rel_l2(a, _gpu(base)[0]) > 1e-3  # the spacing reaches the stencil
 
 
+@pytest.mark.parametrize("shape", ["pipelite:512:8", "pipe:1024:7", "stream:512:9"])
+def test_unequal_spacing_3d(monkeypatch, shape):
+    """dz != dx = dy in 3D: the three axes' stencil weights differ. The 3D kernels read them
+    from LDS for the three-speed medium (61 sound speeds) and from the global table for 171."""
+    for base in (_three_speed_3d((44, 50, 90), 60, seed=36), workloads.random_medium_3d(30, 26, 40, 50, seed=39).inputs):
+        inp = dataclasses.replace(base, dz=base.dz * 1.2)
+        monkeypatch.setenv("FULLWAVE_HIP_KERNELS", shape)
+        a, pa = _gpu(inp)
+        monkeypatch.setenv("FULLWAVE_HIP_KERNELS", "percell")
+        b, pb = _gpu(inp)
+        monkeypatch.delenv("FULLWAVE_HIP_KERNELS")
+        assert rel_l2(a, b) <= 1e-6 and rel_l2(pa, pb) <= 1e-6
+        ref, _ = _ref(inp)
+        err = rel_l2(a, ref)
+        print(f"{shape}, {inp.shape}, {inp.ndmap} sound speeds, dz = 1.2 dx: rel-L2 {err:.2e} against the oracle",
+              flush=True)
+        assert err <= 1e-4
+        assert rel_l2(a, _gpu(base)[0]) > 1e-3
+
+
 def _three_speed_3d(shape, nt, seed):
     """3D inputs whose sound speed takes three values (61 coefficient sets, LDS-resident)."""
     rng = np.random.default_rng(seed)
