@@ -130,11 +130,22 @@ __device__ inline float tl(const float* lb, int dr, int col) {
     return lb[dr * PITCH + (K & 3) * (PITCH / 4) + (K >> 2)];
 }
 
-// LDS coefficient sets: [set][axis][9], or [set][9] when both axes share their sets (equal
-// spacings; `axes` = 1), which halves the table and lets one more workgroup fit on a CU.
+// LDS coefficient sets: [set][axis][kCS], or [set][kCS] when both axes share their sets (equal
+// spacings; `axes` = 1), which halves the table and lets one more workgroup fit on a CU. Each
+// set of 9 weights is padded to 12 floats, so a cell reads its weights as two 16-byte loads
+// and one 4-byte load.
+constexpr int kCS = 12;
 template <bool UNIFORM>
 __device__ inline const float* cset(const float* lds_coef, int dc, int axis, int axes) {
-    return lds_coef + ((UNIFORM ? 0 : dc) * axes + (axes > 1 ? axis : 0)) * kNCoef;
+    return lds_coef + ((UNIFORM ? 0 : dc) * axes + (axes > 1 ? axis : 0)) * kCS;
+}
+
+struct W9 {
+    float w[9];
+};
+__device__ inline W9 weights(const float* c) {
+    const F4 a = *reinterpret_cast<const F4*>(c), b = *reinterpret_cast<const F4*>(c + 4);
+    return W9{{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c[8]}};
 }
 
 // Stage the coefficient sets into LDS (all axes, or axis 0 only when they are the same).
@@ -142,13 +153,9 @@ template <bool UNIFORM>
 __device__ inline int stage_coef(float* lds_coef, const DevSim& s) {
     const int axes = s.coef_axes_same ? 1 : 2;
     const int nsets = UNIFORM ? 1 : s.ndmap;
-    if (axes == 2) {
-        for (int t = threadIdx.x; t < nsets * 2 * kNCoef; t += kLanes) lds_coef[t] = s.coef[t];
-    } else {
-        for (int t = threadIdx.x; t < nsets * kNCoef; t += kLanes) {
-            const int set = t / kNCoef;
-            lds_coef[t] = s.coef[(set * 2) * kNCoef + (t - set * kNCoef)];
-        }
+    for (int t = threadIdx.x; t < nsets * axes * kNCoef; t += kLanes) {
+        const int set = t / kNCoef, k = t - set * kNCoef;  // set: (sound speed) * axes + axis
+        lds_coef[set * kCS + k] = s.coef[(axes == 2 ? set : set * 2) * kNCoef + k];
     }
     return axes;
 }
@@ -208,13 +215,13 @@ __device__ inline void velocity_tile(const DevSim& s, unsigned bx, unsigned by, 
             const int64_t j = jq + e;
             if (j < kHalo || j >= ny - kHalo) continue;
             {
-                const float* w = cset<UNIFORM>(lds_coef, dc[e], 0, axes);
+                const W9 wv = weights(cset<UNIFORM>(lds_coef, dc[e], 0, axes));
                 float acc = 0.f;
 #pragma unroll
-                for (int k = 1; k <= 8; ++k) acc += w[k - 1] * (pa(e, k, 0) - pa(e, -(k - 1), 0));
+                for (int k = 1; k <= 8; ++k) acc += wv.w[k - 1] * (pa(e, k, 0) - pa(e, -(k - 1), 0));
                 float tr = 0.f;
                 tr += (pa(e, 1, 1) - pa(e, 0, 1)) + (pa(e, 1, -1) - pa(e, 0, -1));
-                const float D = acc + w[8] * tr;
+                const float D = acc + wv.w[8] * tr;
                 const float q1 = b1[0][e] * ps1[0][e] + a1[0][e] * D;
                 const float q2 = b2[0][e] * ps2[0][e] + a2[0][e] * D;
                 ps1[0][e] = q1;
@@ -224,13 +231,13 @@ __device__ inline void velocity_tile(const DevSim& s, unsigned bx, unsigned by, 
             }
             {
                 constexpr int ax = ISO ? 0 : 1;
-                const float* w = cset<UNIFORM>(lds_coef, dc[e], 1, axes);
+                const W9 wv = weights(cset<UNIFORM>(lds_coef, dc[e], 1, axes));
                 float acc = 0.f;
 #pragma unroll
-                for (int k = 1; k <= 8; ++k) acc += w[k - 1] * (pa(e, 0, k) - pa(e, 0, -(k - 1)));
+                for (int k = 1; k <= 8; ++k) acc += wv.w[k - 1] * (pa(e, 0, k) - pa(e, 0, -(k - 1)));
                 float tr = 0.f;
                 tr += (pa(e, 1, 1) - pa(e, 1, 0)) + (pa(e, -1, 1) - pa(e, -1, 0));
-                const float D = acc + w[8] * tr;
+                const float D = acc + wv.w[8] * tr;
                 const float q1 = b1[ax][e] * ps1[1][e] + a1[ax][e] * D;
                 const float q2 = b2[ax][e] * ps2[1][e] + a2[ax][e] * D;
                 ps1[1][e] = q1;
@@ -312,13 +319,13 @@ __device__ inline void pressure_tile(const DevSim& s, unsigned bx, unsigned by, 
             if (j < kHalo || j >= ny - kHalo) continue;
             float div = 0.f;
             {
-                const float* w = cset<UNIFORM>(lds_coef, dc[e], 0, axes);
+                const W9 wv = weights(cset<UNIFORM>(lds_coef, dc[e], 0, axes));
                 float acc = 0.f;
 #pragma unroll
-                for (int k = 1; k <= 8; ++k) acc += w[k - 1] * (xa(e, k - 1, 0) - xa(e, -k, 0));
+                for (int k = 1; k <= 8; ++k) acc += wv.w[k - 1] * (xa(e, k - 1, 0) - xa(e, -k, 0));
                 float tr = 0.f;
                 tr += (xa(e, 0, 1) - xa(e, -1, 1)) + (xa(e, 0, -1) - xa(e, -1, -1));
-                const float D = acc + w[8] * tr;
+                const float D = acc + wv.w[8] * tr;
                 const float q1 = b1[0][e] * ph1[0][e] + a1[0][e] * D;
                 const float q2 = b2[0][e] * ph2[0][e] + a2[0][e] * D;
                 ph1[0][e] = q1;
@@ -327,13 +334,13 @@ __device__ inline void pressure_tile(const DevSim& s, unsigned bx, unsigned by, 
             }
             {
                 constexpr int ax = ISO ? 0 : 1;
-                const float* w = cset<UNIFORM>(lds_coef, dc[e], 1, axes);
+                const W9 wv = weights(cset<UNIFORM>(lds_coef, dc[e], 1, axes));
                 float acc = 0.f;
 #pragma unroll
-                for (int k = 1; k <= 8; ++k) acc += w[k - 1] * (ya(e, 0, k - 1) - ya(e, 0, -k));
+                for (int k = 1; k <= 8; ++k) acc += wv.w[k - 1] * (ya(e, 0, k - 1) - ya(e, 0, -k));
                 float tr = 0.f;
                 tr += (ya(e, 1, 0) - ya(e, 1, -1)) + (ya(e, -1, 0) - ya(e, -1, -1));
-                const float D = acc + w[8] * tr;
+                const float D = acc + wv.w[8] * tr;
                 const float q1 = b1[ax][e] * ph1[1][e] + a1[ax][e] * D;
                 const float q2 = b2[ax][e] * ph2[1][e] + a2[ax][e] * D;
                 ph1[1][e] = q1;
@@ -354,14 +361,14 @@ __device__ inline void pressure_tile(const DevSim& s, unsigned bx, unsigned by, 
 template <int TX, bool UNIFORM, bool ISO, bool CODED>
 __global__ __launch_bounds__(kLanes) void k_velocity_tile(DevSim batch) {
     __shared__ __attribute__((aligned(16))) VelTile<TX> L;
-    extern __shared__ float lds_coef[];
+    extern __shared__ __attribute__((aligned(16))) float lds_coef[];
     velocity_tile<TX, UNIFORM, ISO, CODED>(shot_view(batch, blockIdx.z), blockIdx.x, blockIdx.y, L, lds_coef);
 }
 
 template <int TX, bool UNIFORM, bool ISO, bool CODED>
 __global__ __launch_bounds__(kLanes) void k_pressure_tile(DevSim batch) {
     __shared__ __attribute__((aligned(16))) PreTile<TX> L;
-    extern __shared__ float lds_coef[];
+    extern __shared__ __attribute__((aligned(16))) float lds_coef[];
     pressure_tile<TX, UNIFORM, ISO, CODED>(shot_view(batch, blockIdx.z), blockIdx.x, blockIdx.y, L, lds_coef);
 }
 
@@ -371,7 +378,7 @@ __global__ __launch_bounds__(kLanes) void k_pressure_tile(DevSim batch) {
 template <int TX, bool ALL_CODED>
 __global__ __launch_bounds__(kLanes) void k_velocity_tile_group(const GroupTile* __restrict__ t, int n) {
     __shared__ __attribute__((aligned(16))) VelTile<TX> L;
-    extern __shared__ float lds_coef[];
+    extern __shared__ __attribute__((aligned(16))) float lds_coef[];
     const int m = group_entry(t, n);
     const DevSim s = t[m].s;
     const unsigned local = blockIdx.x - t[m].start;
@@ -383,7 +390,7 @@ __global__ __launch_bounds__(kLanes) void k_velocity_tile_group(const GroupTile*
 template <int TX, bool ALL_CODED>
 __global__ __launch_bounds__(kLanes) void k_pressure_tile_group(const GroupTile* __restrict__ t, int n) {
     __shared__ __attribute__((aligned(16))) PreTile<TX> L;
-    extern __shared__ float lds_coef[];
+    extern __shared__ __attribute__((aligned(16))) float lds_coef[];
     const int m = group_entry(t, n);
     const DevSim s = t[m].s;
     const unsigned local = blockIdx.x - t[m].start;
@@ -409,7 +416,7 @@ hipError_t launch_group_tile_any(const GroupTile* t, int n, uint32_t blocks, int
 template <int TX>
 hipError_t launch_tile(const DevSim& s, bool velocity, hipStream_t st) {
     const bool u = s.ndmap == 1;
-    const int lds = (u ? 1 : s.ndmap) * (s.coef_axes_same ? 1 : 2) * kNCoef * 4;
+    const int lds = (u ? 1 : s.ndmap) * (s.coef_axes_same ? 1 : 2) * kCS * 4;
     const int64_t rows = s.nx - 2 * kHalo;
     const dim3 g((unsigned)((s.ny + kTW - 1) / kTW), (unsigned)((rows + TX - 1) / TX), (unsigned)s.nshots);
     const dim3 b(kLanes);
@@ -435,7 +442,8 @@ hipError_t launch_tile(const DevSim& s, bool velocity, hipStream_t st) {
 }  // namespace
 
 bool tile2d_supported(const DevSim& s) {
-    return !s.aexp && s.ndim == 2 && s.ny % 4 == 0 && s.ndmap <= 1024 && s.nx * s.ny < ((int64_t)1 << 30);
+    // ndmap: two padded sets of every sound speed next to the largest tile fit in LDS
+    return !s.aexp && s.ndim == 2 && s.ny % 4 == 0 && s.ndmap <= 768 && s.nx * s.ny < ((int64_t)1 << 30);
 }
 
 unsigned group_tile_rows(int64_t nx, int rows) { return (unsigned)((nx - 2 * kHalo + rows - 1) / rows); }
@@ -443,7 +451,7 @@ unsigned group_tile_rows(int64_t nx, int rows) { return (unsigned)((nx - 2 * kHa
 hipError_t launch_group_tile2d(const GroupTile* t, int n, uint32_t blocks, int rows, int coef_sets, bool all_coded,
                                bool velocity, hipStream_t st) {
     if (n <= 0 || blocks == 0) return hipSuccess;
-    const int lds = coef_sets * kNCoef * 4;  // the largest member's staged sets (stage_coef)
+    const int lds = coef_sets * kCS * 4;  // the largest member's staged sets (stage_coef)
     switch (rows) {
         case 8: return launch_group_tile_any<8>(t, n, blocks, lds, all_coded, velocity, st);
         case 32: return launch_group_tile_any<32>(t, n, blocks, lds, all_coded, velocity, st);
