@@ -22,8 +22,9 @@ group's kernel shape.
 Kernel shapes: the first handle of each grid shape times the candidates (the engine's
 autotuner); later jobs of that shape reuse the choice, so a job's numbers do not depend on
 what else is running. Grouped jobs use a fixed group shape chosen by the job's own grid: the
-tiled kernels ``tile:8`` where the rows are 16-byte aligned (ny % 4 == 0), else the marching
-kernels ``stream:256:16``; the batcher keeps one group per shape.
+tiled kernels ``tile:8`` where the rows are 16-byte aligned (ny % 4 == 0) and the grid has at
+most 768 sound speeds, else the marching kernels ``stream:256:16``; the batcher keeps one group
+per shape.
 
 Latency of a job is measured from ``submit`` to its result being available.
 """
@@ -103,11 +104,12 @@ class ServerStats:
 
 GROUP_WIDTH, GROUP_CHUNK = 256, 16  # marching kernel shape of grouped jobs
 GROUP_TILE_ROWS = 8  # tiled kernel shape of grouped jobs with ny % 4 == 0
+TILE_MAX_NDMAP = 768  # sound speeds the tiled kernels take (their coefficient sets sit in LDS)
 
 
 def _group_shape(inp: SimulationInputs) -> dict:
     """The kernel shape a grouped job runs with (depends on the job's own grid only)."""
-    if inp.ny % 4 == 0:
+    if inp.ny % 4 == 0 and inp.ndmap <= TILE_MAX_NDMAP:
         return {"variant": "tile", "width": 256, "chunk": GROUP_TILE_ROWS}
     return {"variant": "stream", "width": GROUP_WIDTH, "chunk": GROUP_CHUNK}
 # Jobs of more cells (all shots together) than this fill the GPU on their own, where their own

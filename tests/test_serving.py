@@ -100,5 +100,8 @@ def test_group_membership_and_shapes():
         assert srv.kernel_config_for(aligned)["velocity"] == {"variant": "tile", "width": 256, "chunk": 8}
         assert srv.kernel_config_for(unaligned)["pressure"] == {"variant": "stream", "width": 256, "chunk": 16}
         assert srv.kernel_config_for(larger) is None
+        many = _inp(40)
+        many.dmap = np.zeros((9, 2, 800), np.float32)  # more sound speeds than the tiled kernels hold
+        assert srv.kernel_config_for(many)["velocity"]["variant"] == "stream"
     with SimulationServer(hbm_budget=10**9, batching=False) as srv:
         assert not srv._grouped(aligned, 1) and srv.kernel_config_for(aligned) is None
