@@ -359,7 +359,8 @@ int check_cfg(fw_sim* s, const fw::LaunchCfg& c) {
     if (c.variant == 0) return FW_OK;
     if (c.variant == 2) {
         if (!fw::tile2d_supported(s->ds))
-            return fail(FW_ERR_UNSUPPORTED, "tile kernels need a 2D relaxation grid with ny %% 4 == 0");
+            return fail(FW_ERR_UNSUPPORTED,
+                        "tile kernels need a 2D relaxation grid with ny %% 4 == 0 and at most 768 sound speeds");
         if (c.width != 256 || (c.chunk != 8 && c.chunk != 16 && c.chunk != 32))
             return fail(FW_ERR_INVALID, "tile kernels: width 256 and rows 8, 16 or 32");
         return FW_OK;

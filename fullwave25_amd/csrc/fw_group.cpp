@@ -269,7 +269,9 @@ int fw_group_create_shaped(fw_sim* const* sims, int32_t n, int32_t variant, int3
         if (!fw::streaming_supported(sims[i]->ds))
             return fw::set_error(FW_ERR_UNSUPPORTED, "group member %d needs the per-cell kernels", (int)i);
         if (variant == 2 && !fw::tile2d_supported(sims[i]->ds))
-            return fw::set_error(FW_ERR_UNSUPPORTED, "group member %d cannot run the tiled kernels (ny %% 4 != 0)",
+            return fw::set_error(FW_ERR_UNSUPPORTED,
+                                 "group member %d cannot run the tiled kernels (ny %% 4 != 0 or more than 768 sound "
+                                 "speeds)",
                                  (int)i);
     }
     fw_group* g = new fw_group();

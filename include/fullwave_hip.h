@@ -187,9 +187,9 @@ int fw_group_create(fw_sim* const* sims, int32_t n, int32_t width, int32_t chunk
 int fw_group_config(const fw_group* group, int32_t* width, int32_t* chunk);
 /* The same with the kernel family chosen: variant 1 is the x-marching kernels above
  * (fw_group_create); variant 2 the tiled float4 kernels, 256 columns x `chunk` rows (8, 16 or
- * 32; 0: timed) per workgroup, for members with ny % 4 == 0 (FW_ERR_UNSUPPORTED otherwise);
- * variant 0 times both families (the tiled one when every member takes it) with width and
- * chunk 0. Each member's records equal fw_sim_run with the same family and shape bit for bit
+ * 32; 0: timed) per workgroup, for members with ny % 4 == 0 and at most 768 sound speeds
+ * (FW_ERR_UNSUPPORTED otherwise); variant 0 times both families (the tiled one when every
+ * member takes it) with width and chunk 0. Each member's records equal fw_sim_run with the same family and shape bit for bit
  * ("tile:<chunk>" for variant 2). fw_group_shape reports the family and shape in use. */
 int fw_group_create_shaped(fw_sim* const* sims, int32_t n, int32_t variant, int32_t width, int32_t chunk,
                            fw_group** out);
