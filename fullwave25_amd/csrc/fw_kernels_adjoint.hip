@@ -40,13 +40,17 @@ __device__ inline bool adj_cell(const DevSim& s, int64_t& i, int64_t& j, int64_t
     return i >= kHalo && i < s.nx - kHalo && j >= kHalo && j < s.ny - kHalo && k >= kHalo && k < s.nz - kHalo;
 }
 
+// The coefficient set of cell c on axis a. UNI: one sound speed (ndmap == 1), the same set
+// everywhere, so the gathers skip the per-neighbour dcmap load.
+template <bool UNI>
 __device__ inline const float* wts(const DevSim& s, int64_t c, int nd, int a) {
+    if (UNI) return s.coef + a * kNCoef;
     return s.coef + ((int64_t)s.dcmap[c] * nd + a) * kNCoef;
 }
 
-template <int ND>
+template <int ND, bool UNI>
 __device__ inline float dplus(const DevSim& s, const float* p, int64_t c, int a, const int64_t* st) {
-    const float* w = wts(s, c, ND, a);
+    const float* w = wts<UNI>(s, c, ND, a);
     const int64_t sa = st[a];
     float acc = 0.f;
 #pragma unroll
@@ -61,9 +65,9 @@ __device__ inline float dplus(const DevSim& s, const float* p, int64_t c, int a,
     return acc + w[8] * tr;
 }
 
-template <int ND>
+template <int ND, bool UNI>
 __device__ inline float dminus(const DevSim& s, const float* f, int64_t c, int a, const int64_t* st) {
-    const float* w = wts(s, c, ND, a);
+    const float* w = wts<UNI>(s, c, ND, a);
     const int64_t sa = st[a];
     float acc = 0.f;
 #pragma unroll
@@ -95,7 +99,7 @@ __global__ __launch_bounds__(256) void k_adj_mask(float* lp, const int64_t* src_
     if (t < nzero) lp[zero_lin[t]] = 0.f;
 }
 
-template <int ND, bool EXPO>
+template <int ND, bool EXPO, bool UNI>
 __global__ __launch_bounds__(256) void k_adj_pressure(DevSim s, AdjView w) {
     int64_t i, j, k;
     if (!adj_cell<ND>(s, i, j, k)) return;
@@ -108,7 +112,7 @@ __global__ __launch_bounds__(256) void k_adj_pressure(DevSim s, AdjView w) {
         const float t = -A * s.dt * s.K[c] * mu;
 #pragma unroll
         for (int a = 0; a < ND; ++a) {
-            div += dminus<ND>(s, w.fv[a], c, a, st);
+            div += dminus<ND, UNI>(s, w.fv[a], c, a, st);
             w.H[a][c] = t;
         }
         w.lp[c] = A * mu;
@@ -118,7 +122,7 @@ __global__ __launch_bounds__(256) void k_adj_pressure(DevSim s, AdjView w) {
     const float t = -s.dt * s.K[c] * mu;
 #pragma unroll
     for (int a = 0; a < ND; ++a) {
-        const float h = dminus<ND>(s, w.fv[a], c, a, st);
+        const float h = dminus<ND, UNI>(s, w.fv[a], c, a, st);
         const float ik = s.ikp[a][c];
         float lh = t * ik;
 #pragma unroll
@@ -134,7 +138,7 @@ __global__ __launch_bounds__(256) void k_adj_pressure(DevSim s, AdjView w) {
 }
 
 // lambda_v_a += (D-_a)^T H_a at active cells
-template <int ND>
+template <int ND, bool UNI>
 __global__ __launch_bounds__(256) void k_adj_dminus_t(DevSim s, AdjView w) {
     int64_t i, j, k;
     if (!adj_cell<ND>(s, i, j, k)) return;
@@ -148,7 +152,7 @@ __global__ __launch_bounds__(256) void k_adj_dminus_t(DevSim s, AdjView w) {
 #pragma unroll
         for (int m = 1; m <= 8; ++m) {
             const int64_t c1 = x - (m - 1) * sa, c2 = x + m * sa;
-            acc += wts(s, c1, ND, a)[m - 1] * H[c1] - wts(s, c2, ND, a)[m - 1] * H[c2];
+            acc += wts<UNI>(s, c1, ND, a)[m - 1] * H[c1] - wts<UNI>(s, c2, ND, a)[m - 1] * H[c2];
         }
 #pragma unroll
         for (int b = 0; b < ND; ++b) {
@@ -157,14 +161,14 @@ __global__ __launch_bounds__(256) void k_adj_dminus_t(DevSim s, AdjView w) {
 #pragma unroll
             for (int sg = -1; sg <= 1; sg += 2) {
                 const int64_t c1 = x - sg * sb, c2 = x + sa - sg * sb;
-                acc += wts(s, c1, ND, a)[8] * H[c1] - wts(s, c2, ND, a)[8] * H[c2];
+                acc += wts<UNI>(s, c1, ND, a)[8] * H[c1] - wts<UNI>(s, c2, ND, a)[8] * H[c2];
             }
         }
         w.lv[a][x] += acc;
     }
 }
 
-template <int ND, bool EXPO>
+template <int ND, bool EXPO, bool UNI>
 __global__ __launch_bounds__(256) void k_adj_velocity(DevSim s, AdjView w) {
     int64_t i, j, k;
     if (!adj_cell<ND>(s, i, j, k)) return;
@@ -177,7 +181,7 @@ __global__ __launch_bounds__(256) void k_adj_velocity(DevSim s, AdjView w) {
         const float nu = w.lv[a][c];
         const float rb = 0.5f * (rho_c + s.rho[c + sa]);
         const float beta = s.dt / rb;
-        const float g = dplus<ND>(s, w.fp_prev, c, a, st);
+        const float g = dplus<ND, UNI>(s, w.fp_prev, c, a, st);
         float u, lg;
         if (EXPO) {
             const float A = s.aexp[c];
@@ -201,7 +205,7 @@ __global__ __launch_bounds__(256) void k_adj_velocity(DevSim s, AdjView w) {
 }
 
 // lambda_p += sum_a (D+_a)^T G_a at active cells
-template <int ND>
+template <int ND, bool UNI>
 __global__ __launch_bounds__(256) void k_adj_dplus_t(DevSim s, AdjView w) {
     int64_t i, j, k;
     if (!adj_cell<ND>(s, i, j, k)) return;
@@ -215,7 +219,7 @@ __global__ __launch_bounds__(256) void k_adj_dplus_t(DevSim s, AdjView w) {
 #pragma unroll
         for (int m = 1; m <= 8; ++m) {
             const int64_t c1 = x - m * sa, c2 = x + (m - 1) * sa;
-            acc += wts(s, c1, ND, a)[m - 1] * G[c1] - wts(s, c2, ND, a)[m - 1] * G[c2];
+            acc += wts<UNI>(s, c1, ND, a)[m - 1] * G[c1] - wts<UNI>(s, c2, ND, a)[m - 1] * G[c2];
         }
 #pragma unroll
         for (int b = 0; b < ND; ++b) {
@@ -224,7 +228,7 @@ __global__ __launch_bounds__(256) void k_adj_dplus_t(DevSim s, AdjView w) {
 #pragma unroll
             for (int sg = -1; sg <= 1; sg += 2) {
                 const int64_t c1 = x - sa - sg * sb, c2 = x - sg * sb;
-                acc += wts(s, c1, ND, a)[8] * G[c1] - wts(s, c2, ND, a)[8] * G[c2];
+                acc += wts<UNI>(s, c1, ND, a)[8] * G[c1] - wts<UNI>(s, c2, ND, a)[8] * G[c2];
             }
         }
     }
@@ -276,23 +280,25 @@ hipError_t launch_adj_mask(float* lp, const int64_t* src_lin, int64_t nsrc, cons
     return hipGetLastError();
 }
 
+template <int ND, bool UNI>
+void launch_adj_step_nd(const DevSim& s, const AdjView& w, dim3 g, dim3 b, hipStream_t st) {
+    if (s.aexp) hipLaunchKernelGGL((k_adj_pressure<ND, true, UNI>), g, b, 0, st, s, w);
+    else hipLaunchKernelGGL((k_adj_pressure<ND, false, UNI>), g, b, 0, st, s, w);
+    hipLaunchKernelGGL((k_adj_dminus_t<ND, UNI>), g, b, 0, st, s, w);
+    if (s.aexp) hipLaunchKernelGGL((k_adj_velocity<ND, true, UNI>), g, b, 0, st, s, w);
+    else hipLaunchKernelGGL((k_adj_velocity<ND, false, UNI>), g, b, 0, st, s, w);
+    hipLaunchKernelGGL((k_adj_dplus_t<ND, UNI>), g, b, 0, st, s, w);
+}
+
 hipError_t launch_adj_step(const DevSim& s, const AdjView& w, hipStream_t st) {
     const dim3 g = adj_grid(s), b(64, 4);
-    const bool e = s.aexp != nullptr;
+    const bool u = s.ndmap == 1;
     if (s.ndim == 2) {
-        if (e) hipLaunchKernelGGL((k_adj_pressure<2, true>), g, b, 0, st, s, w);
-        else hipLaunchKernelGGL((k_adj_pressure<2, false>), g, b, 0, st, s, w);
-        hipLaunchKernelGGL((k_adj_dminus_t<2>), g, b, 0, st, s, w);
-        if (e) hipLaunchKernelGGL((k_adj_velocity<2, true>), g, b, 0, st, s, w);
-        else hipLaunchKernelGGL((k_adj_velocity<2, false>), g, b, 0, st, s, w);
-        hipLaunchKernelGGL((k_adj_dplus_t<2>), g, b, 0, st, s, w);
+        if (u) launch_adj_step_nd<2, true>(s, w, g, b, st);
+        else launch_adj_step_nd<2, false>(s, w, g, b, st);
     } else {
-        if (e) hipLaunchKernelGGL((k_adj_pressure<3, true>), g, b, 0, st, s, w);
-        else hipLaunchKernelGGL((k_adj_pressure<3, false>), g, b, 0, st, s, w);
-        hipLaunchKernelGGL((k_adj_dminus_t<3>), g, b, 0, st, s, w);
-        if (e) hipLaunchKernelGGL((k_adj_velocity<3, true>), g, b, 0, st, s, w);
-        else hipLaunchKernelGGL((k_adj_velocity<3, false>), g, b, 0, st, s, w);
-        hipLaunchKernelGGL((k_adj_dplus_t<3>), g, b, 0, st, s, w);
+        if (u) launch_adj_step_nd<3, true>(s, w, g, b, st);
+        else launch_adj_step_nd<3, false>(s, w, g, b, st);
     }
     return hipGetLastError();
 }

@@ -23,6 +23,7 @@ CASES = {
     "2d_aniso": lambda: workloads.random_medium_2d(34, 40, 120, seed=61, isotropic=False).inputs,
     "2d_exp": lambda: workloads.random_medium_2d(40, 36, 150, seed=62, exponential=True).inputs,
     "3d_iso": lambda: workloads.random_medium_3d(14, 12, 16, 60, seed=63, air=2).inputs,
+    "2d_one_speed": lambda: workloads.plane_wave_2d(48, 40, 150, n_pml=8, sensor_depth=20).inputs,
 }
 
 
@@ -104,3 +105,26 @@ def test_torch_misfit_gradient_matches_finite_differences():
     print(f"misfit: dc fd {fd_c:.6e} adj {ad_c:.6e}; drho fd {fd_r:.6e} adj {ad_r:.6e}", flush=True)
     assert abs(ad_c - fd_c) <= 1e-3 * abs(fd_c)
     assert abs(ad_r - fd_r) <= 1e-3 * abs(fd_r)
+
+
+def test_one_speed_kernels_equal_general_kernels():
+    """A medium of one sound speed takes the adjoint kernels that skip the per-neighbour
+    coefficient lookup; the same medium with an unused second coefficient set (every dcmap
+    value still 0) takes the general kernels. Same weights, same arithmetic: the gradients
+    agree to FP32 rounding of the forward kernels' shapes (bit for bit when those match)."""
+    inp = CASES["2d_one_speed"]()
+    assert inp.ndmap == 1
+    two = dataclasses.replace(inp, dmap=np.concatenate([inp.dmap, inp.dmap], axis=2))
+    assert two.ndmap == 2
+    rng = np.random.default_rng(65)
+    w = rng.standard_normal((inp.n_records, inp.ncoordsout)).astype(np.float32)
+    out = []
+    for case in (inp, two):
+        with HipAdjoint(case, checkpoint_every=25) as adj:
+            adj.forward()
+            out.append(adj.backward(w))
+    (gk1, gr1), (gk2, gr2) = out
+    ek, er = rel_l2(gk1, gk2), rel_l2(gr1, gr2)
+    print(f"one-speed vs general adjoint kernels: rel-L2 dK {ek:.2e}, drho {er:.2e}", flush=True)
+    assert np.abs(gk1).max() > 0
+    assert ek <= 1e-5 and er <= 1e-5
