@@ -229,7 +229,7 @@ def _three_speed_3d(shape, nt, seed):
 
 
 @pytest.mark.parametrize("shape", ["stream:512:2", "stream:1024:16", "stream:512:9", "stream:1024:5", "pipe:512:7",
-                                   "pipe:1024:16", "pipelite:512:2", "pipelite:1024:9"])
+                                   "pipe:1024:16", "pipelite:512:2", "pipelite:1024:9", "pipelite:512:64"])
 def test_streaming_3d_matches_per_cell(monkeypatch, shape):
     for inp in (workloads.random_medium_3d(40, 30, 150, 60, seed=33, air=4).inputs,
                 workloads.random_medium_3d(26, 70, 36, 50, seed=34, isotropic=False).inputs,
