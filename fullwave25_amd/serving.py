@@ -329,7 +329,7 @@ class SimulationServer:
                 done = threading.Event()
                 failure: list = []
                 with self._batch_lock:
-                    self._batch_in.append((sim, inp.nt, done, failure))
+                    self._batch_in.append((sim, inp.nt, done, failure, shape["variant"]))
                     self._batch_lock.notify_all()
                 done.wait()
                 if failure:
@@ -358,10 +358,9 @@ class SimulationServer:
                     self._batch_lock.wait()
                 new, self._batch_in = self._batch_in, []
             changed = set()
-            for sim, steps, done, failure in new:
-                shape = sim.kernel_config()["velocity"]
-                members.append([sim, steps, done, failure, shape["variant"]])
-                changed.add(shape["variant"])
+            for sim, steps, done, failure, variant in new:
+                members.append([sim, steps, done, failure, variant])
+                changed.add(variant)
             try:
                 for v in changed:
                     if v in groups:
