@@ -68,6 +68,24 @@ def test_null_arguments_rejected():
     assert lib.fw_sim_destroy(None) == 0
 
 
+def test_group_arguments_rejected_before_device():
+    """fw_group_create_shaped checks the kernel family, width and chunk before it reads any
+    handle (the pointer below is never dereferenced)."""
+    lib = engine.load_library()
+    sims = (ctypes.c_void_p * 1)(ctypes.c_void_p(1))
+    g = ctypes.c_void_p()
+    for (variant, width, chunk), msg in [((3, 0, 0), b"variant"), ((2, 128, 0), b"256 columns"),
+                                         ((2, 0, 12), b"8, 16 or 32"), ((0, 256, 0), b"timed kernel family"),
+                                         ((1, 100, 16), b"64, 128 or 256"), ((1, 256, 1), b"chunk")]:
+        assert lib.fw_group_create_shaped(sims, 1, variant, width, chunk, ctypes.byref(g)) == 1, (variant, width, chunk)
+        assert msg in lib.fw_last_error(), lib.fw_last_error()
+        assert not g.value
+    assert lib.fw_group_create_shaped(None, 1, 1, 0, 0, ctypes.byref(g)) == 1
+    v, w, c = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+    assert lib.fw_group_shape(None, ctypes.byref(v), ctypes.byref(w), ctypes.byref(c)) == 1
+    assert lib.fw_group_destroy(None) == 0
+
+
 def test_missing_library_fails_loudly(tmp_path):
     with pytest.raises(RuntimeError, match="missing"):
         engine.load_library(tmp_path / "libfullwave_hip.so")
