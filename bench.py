@@ -1,16 +1,21 @@
-"""Benchmark: FDTD time steps of the 2D heterogeneous focused-transmit workload on one GPU.
+"""Benchmark: FDTD time steps of the Fullwave 2.5 engine on one GPU.
 
-Workload: 4096 x 2048 interior (4256 x 2208 with the 36-cell PML, 36-cell transition layer and
-8 ghost cells per side), seeded six-layer medium with sub-resolution scatterers, 128-element
-linear array focused at 40 mm, 3 MHz, per-element receive, 8000 steps. A "step" is one time step
-(velocity update, pressure update, source/sensor pass) over the whole grid; the default timed
-region is the full 8000-step transmit with every input resident in HBM.
+Default workload: the 3D heterogeneous inclusion configuration, 768 x 768 x 640 interior
+(904 x 904 x 776 cells with the 24-cell absorbing layer, 36-cell transition layer and 8 ghost
+cells per side), 20 seeded ellipsoidal inclusions, spatially varying gamma, 32 x 32 matrix-array
+plane-wave transmit at 2.5 MHz, a 6000-step job whose full-aperture receive plane and three
+orthogonal mid-planes (2.06 M sensors) are recorded every step. A "step" is one time step
+(velocity update, pressure update, source/air/sensor pass) over the whole grid with every input
+resident in HBM; ``--steps`` of the job are timed. ``--config focused_2d`` runs the 4096 x 2048
+2D heterogeneous focused-transmit workload instead.
 
-Prints one JSON line. ``value`` is GLUPS (extended-grid cells x steps / s). ``roofline`` is for
-the dominant kernel, timed with HIP events around each launch on the engine's stream during a
-second pass over the same steps; its bytes are the algorithmic bytes per updated cell (each
-array element the kernel must read or write, once). ``cpu_baseline`` times the FP32 build of the
-oracle (OpenMP) on the first steps of the same workload.
+Prints one JSON line. ``value`` is GLUPS over the updated cells (the ghost border is never
+updated). ``roofline`` is for the dominant kernel, timed with HIP events around each launch on
+the engine's stream during a second pass over the same steps; its bytes are the algorithmic
+bytes per updated cell (each array element the kernel must read or write, once).
+``cpu_baseline`` times the FP32 build of the oracle (OpenMP) on a bounded sample of the same
+grid (an x-slab of it in 3D). Other ``--config`` values print the batching, grouping, serving
+and gradient figures.
 """
 
 from __future__ import annotations
@@ -87,20 +92,6 @@ def coefficient_layout(sim, inp) -> dict:
     n, frac = sim.relaxation_coding()
     ndmap = int(np.asarray(inp.dmap).shape[-1])
     return {"uniform": ndmap == 1, "coded_fraction": frac if n else None, "codes": n}
-
-
-def build_workload(args):
-    from fullwave25_amd import workloads
-
-    if args.config == "focused_2d":
-        return workloads.focused_linear_array_2d(nx=args.nx, ny=args.ny, nt=args.steps)
-    if args.config == "plane_wave_2d":
-        return workloads.plane_wave_2d(nx=args.nx, ny=args.ny, nt=args.steps)
-    if args.config == "random_3d":
-        return workloads.random_medium_3d(args.nx, args.ny, args.nz, args.steps, seed=5)
-    if args.config == "inclusions_3d":  # the hetero_3d workload, for --sweep and --profile-steps
-        return workloads.inclusions_3d(args.nx, args.ny, args.nz, nt=max(args.steps, 100))
-    raise SystemExit(f"unknown config {args.config}")
 
 
 def pmc_summary(fetch_dir: Path, write_dir: Path, copy_fetch_dir: Path, copy_write_dir: Path,
@@ -319,58 +310,6 @@ def bench_adjoint(args) -> dict:
     }
 
 
-def bench_3d(args) -> dict:
-    """The 3D heterogeneous configuration (768 x 768 x 640 interior, 904 x 904 x 776 cells,
-    ellipsoidal inclusions, 32 x 32 matrix-array plane wave, 6000-step job): the per-step rate
-    over ``--steps`` timed steps of the full grid (the rate is steady after warm-up; the job's
-    6000 steps would take minutes), with the dominant kernel's roofline."""
-    from fullwave25_amd import engine, workloads
-
-    t0 = time.time()
-    wl = workloads.inclusions_3d(args.nx, args.ny, args.nz, nt=6000)
-    inp = wl.inputs
-    print(f"# {wl.name} {inp.shape} built in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
-    steps = min(args.steps, inp.nt)
-    source = pin_kernels(args, wl.name, (args.nx, args.ny, args.nz) == (768, 768, 640))
-    with engine.HipSimulation(inp) as sim:
-        kcfg = sim.kernel_config()
-        layout = coefficient_layout(sim, inp)
-        if args.profile_steps:
-            sim.run(args.profile_steps)
-            return {"kernel_config": kcfg, "kernels": engine.kernel_override_string(kcfg), "codes": layout["codes"]}
-        sim.run(min(args.warmup, inp.nt - steps))
-        sim.reset()
-        t1 = time.perf_counter()
-        sim.run(steps)
-        elapsed = time.perf_counter() - t1
-        finite = bool(np.isfinite(sim.sensors()).all())
-        sim.reset()
-        kms = sim.run_timed(steps)
-    updated = int(np.prod([s - 16 for s in inp.shape]))
-    bpc = algorithmic_bytes_per_cell(inp.ndim, inp.isotropic, uniform=layout["uniform"],
-                                     coded_fraction=layout["coded_fraction"])
-    dominant = max(("velocity", "pressure"), key=lambda k: kms[k])
-    avg_s = kms[dominant] / 1e3 / steps
-    achieved = bpc[dominant] * updated / avg_s / 1e9
-    return {
-        "metric": "3D FP32 step: GLUPS + % HBM roofline (1 GPU)", "value": round(inp.ncells * steps / elapsed / 1e9, 3),
-        "unit": "GLUPS", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
-        "ms_per_step": round(elapsed / steps * 1e3, 4), "higher_is_better": True, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": f"{wl.name}: {args.nx}x{args.ny}x{args.nz} interior ({inp.nx}x{inp.ny}x{inp.nz} "
-                               f"extended), {steps} of the job's 6000 steps timed, isotropic relaxation",
-                   "cells": inp.ncells, "sources": inp.ncoords, "sensors": inp.ncoordsout, "finite_output": finite},
-        "roofline": {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": measured_traffic(wl.name, dominant, kcfg, layout["codes"]),
-                     "bytes_per_cell": bpc[dominant], "avg_launch_us": round(avg_s * 1e6, 2),
-                     "kernel_ms_per_step": {k: round(v / steps, 4) for k, v in kms.items()},
-                     "kernel_config": kcfg, "kernels": engine.kernel_override_string(kcfg), "kernel_source": source,
-                     "relaxation_codes": layout["codes"], "coded_fraction": layout["coded_fraction"],
-                     "step_bytes_per_lup": STEP_BYTES_PER_LUP[3],
-                     "step_frac": round(inp.ncells * steps / elapsed * STEP_BYTES_PER_LUP[3] / 1e9 / HBM_PEAK_GBS, 4)},
-    }
-
-
 def _served_jobs(args):
     """The served mix, seeded. ``--mix standard``: 2D forward jobs on 512^2..4096^2 grids with
     1000..8000 steps (70 %), misfit-gradient jobs on 1024^2 (20 %), small 3D forward jobs
@@ -477,7 +416,48 @@ def bench_served(args) -> dict:
     }
 
 
-def cpu_baseline(inp, steps: int) -> dict:
+
+
+def x_slab(inp, n_planes: int, x0: int | None = None):
+    """Planes ``[x0, x0 + n_planes)`` of ``inp`` as inputs of their own: every map sliced along x
+    (the slowest axis, so each slice is contiguous), the source, sensor and air points inside the
+    slab kept and shifted. By default the slab is centred on the first source plane, so it holds
+    the transmit and the receive plane. Used to time the CPU oracle on a bounded sample of a grid
+    too large for it to step in seconds."""
+    import dataclasses
+
+    n = min(int(n_planes), inp.nx)
+    if x0 is None:
+        first = int(inp.icc[:, 0].min()) if inp.ncoords else inp.nx // 2
+        x0 = first - n // 2
+    x0 = max(0, min(int(x0), inp.nx - n))
+    sl = slice(x0, x0 + n)
+
+    def keep(coords):
+        c = np.asarray(coords).reshape(-1, inp.ndim)
+        m = (c[:, 0] >= x0) & (c[:, 0] < x0 + n)
+        out = c[m].astype(np.int32)
+        out[:, 0] -= x0
+        return out, m
+
+    icc, m = keep(inp.icc)
+    outc, _ = keep(inp.outc)
+    zero = keep(inp.icczero)[0] if inp.ncoordszero else np.zeros((0, inp.ndim), np.int32)
+    return dataclasses.replace(
+        inp, nx=n, maps={k: np.ascontiguousarray(v[sl]) for k, v in inp.maps.items()},
+        dcmap=np.ascontiguousarray(inp.dcmap[sl]), icc=icc, icmat=np.ascontiguousarray(inp.icmat[m]),
+        outc=outc, icczero=zero)
+
+
+def updated_cells(inp) -> int:
+    """Cells the engine updates: every coordinate in [8, n - 8) (the 8-cell ghost border stays 0)."""
+    return int(np.prod([s - 16 for s in inp.shape]))
+
+
+def cpu_baseline(inp, steps: int, *, slab_planes: int = 0) -> dict:
+    """The FP32 oracle (C, OpenMP) timed on this host's cores over ``steps`` steps after 5
+    warm-up steps, on ``inp`` or, with ``slab_planes``, on an x-slab of it. The steps are clamped
+    to what the inputs' ``nt`` leaves after the warm-up. GLUPS over updated cells."""
     import oracle
 
     oracle.build()
@@ -487,33 +467,188 @@ def cpu_baseline(inp, steps: int) -> dict:
         model = next(l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo") if l.startswith("model name"))
     except (OSError, StopIteration):
         pass
-    sim = oracle.OracleSimulation(inp, precision=32)
-    sim.run(5)  # warm-up (first touch of the state arrays)
-    t0 = time.perf_counter()
-    sim.run(steps)
-    dt = time.perf_counter() - t0
-    sim.close()
+    sample = inp if not slab_planes else x_slab(inp, slab_planes)
+    warm = min(5, max(sample.nt - 1, 0))
+    steps = max(1, min(int(steps), sample.nt - warm))
+    sim = oracle.OracleSimulation(sample, precision=32)
+    try:
+        sim.run(warm)  # first touch of the state arrays
+        t0 = time.perf_counter()
+        sim.run(steps)
+        dt = time.perf_counter() - t0
+    finally:
+        sim.close()
+    where = (f"an x-slab of {sample.nx} planes ({sample.nx}x{sample.ny}" + (f"x{sample.nz}" if sample.ndim == 3 else "")
+             + f" cells, {sample.ncoords} sources, {sample.ncoordsout} sensors) of the same grid") if slab_planes \
+        else "the same workload"
     return {
-        "value": inp.ncells * steps / dt / 1e9,
+        "value": round(updated_cells(sample) * steps / dt / 1e9, 4),
         "unit": "GLUPS",
         "cores": threads,
         "kind": "port",
         "cpu_model": model,
-        "sample": f"FP32 oracle (C, OpenMP, {threads} threads), steps 6..{steps + 5} of the same workload "
-                  f"after 5 warm-up steps",
+        "seconds": round(dt, 2),
+        "sample": f"FP32 oracle (C, OpenMP, {threads} threads) on {where}: steps {warm + 1}..{warm + steps} "
+                  f"after {warm} warm-up steps; GLUPS over updated cells",
+    }
+
+
+# Headline single-grid workloads and their full sizes and step counts.
+SINGLE_CONFIGS = {
+    "hetero_3d": {"workload": "inclusions_3d", "nx": 768, "ny": 768, "nz": 640, "nt": 6000,
+                  "desc": "3D heterogeneous, ellipsoidal inclusions, 32x32 matrix-array plane wave, 2.5 MHz"},
+    "focused_2d": {"workload": "focused_linear_array_2d", "nx": 4096, "ny": 2048, "nz": 1, "nt": 8000,
+                   "desc": "2D heterogeneous six-layer medium with scatterers, 128-element array focused at 40 mm, 3 MHz"},
+    "plane_wave_2d": {"workload": "plane_wave_2d", "nx": 512, "ny": 512, "nz": 1, "nt": 2000,
+                      "desc": "2D homogeneous plane wave, 3 MHz"},
+    "random_3d": {"workload": "random_medium_3d", "nx": 256, "ny": 256, "nz": 256, "nt": 1000,
+                  "desc": "3D seeded random smooth medium"},
+}
+# the CPU sample for grids the oracle cannot step in seconds: an x-slab of this many planes
+CPU_SLAB_PLANES = {"hetero_3d": 40}
+# CPU baseline steps (about 10-30 s of oracle time on the GPU host's cores)
+CPU_STEPS = {"hetero_3d": 30, "random_3d": 30}
+
+
+def build_single(args):
+    """The workload of a single-grid config at ``args``' sizes (the config's own by default) with
+    the config's full step count (at least ``--warmup`` and ``--steps``)."""
+    from fullwave25_amd import workloads
+
+    cfg = SINGLE_CONFIGS[args.config]
+    nt = max(cfg["nt"], args.steps, args.warmup)
+    if args.config == "hetero_3d":
+        return workloads.inclusions_3d(args.nx, args.ny, args.nz, nt=nt)
+    if args.config == "focused_2d":
+        return workloads.focused_linear_array_2d(nx=args.nx, ny=args.ny, nt=nt)
+    if args.config == "plane_wave_2d":
+        return workloads.plane_wave_2d(nx=args.nx, ny=args.ny, nt=nt)
+    return workloads.random_medium_3d(args.nx, args.ny, args.nz, nt, seed=5)
+
+
+def bench_single(args) -> dict:
+    """One grid, one transmit: ``--warmup`` untimed steps, reset, then exactly ``--steps`` timed
+    steps (fw_sim_run synchronizes the engine's stream before it returns, so the wall clock
+    brackets the steps); a second pass over the same steps with HIP events around every launch
+    gives the dominant kernel's roofline; a third with the relaxation coefficients read from
+    the per-cell maps instead of codes gives the uncoded rate."""
+    from fullwave25_amd import engine
+
+    cfg = SINGLE_CONFIGS[args.config]
+    t0 = time.time()
+    wl = build_single(args)
+    inp = wl.inputs
+    print(f"# workload {wl.name} {inp.shape} nt={inp.nt}, {inp.ncoordsout} sensors, built in {time.time() - t0:.1f}s",
+          file=sys.stderr, flush=True)
+    full_size = (args.nx, args.ny, args.nz if inp.ndim == 3 else 1) == (cfg["nx"], cfg["ny"], cfg["nz"])
+    source = pin_kernels(args, wl.name, full_size)
+    upd = updated_cells(inp)
+    with engine.HipSimulation(inp, device=0) as sim:
+        kcfg = sim.kernel_config()
+        layout = coefficient_layout(sim, inp)
+        print(f"# handle ready: kernels {engine.kernel_override_string(kcfg)}, {layout['codes']} codes", file=sys.stderr,
+              flush=True)
+        if args.profile_steps:
+            sim.run(args.profile_steps)
+            return {"kernel_config": kcfg, "kernels": engine.kernel_override_string(kcfg), "codes": layout["codes"]}
+        sim.run(args.warmup)
+        sim.reset()
+        t1 = time.perf_counter()
+        sim.run(args.steps)
+        elapsed = time.perf_counter() - t1
+        finite = bool(np.isfinite(sim.sensors()).all())
+        print(f"# timed {args.steps} steps: {elapsed * 1e3 / args.steps:.3f} ms/step", file=sys.stderr, flush=True)
+        sim.reset()
+        kms = sim.run_timed(args.steps)
+        uncoded = None
+        if layout["codes"]:
+            # the same kernels reading the ten relaxation maps and dcmap per cell
+            sim.set_relaxation_codes(False)
+            sim.reset()
+            sim.run(min(args.warmup, 5))
+            sim.reset()
+            t2 = time.perf_counter()
+            sim.run(args.steps)
+            t_unc = time.perf_counter() - t2
+            uncoded = {"glups": round(upd * args.steps / t_unc / 1e9, 3), "ms_per_step": round(t_unc / args.steps * 1e3, 4),
+                       "step_frac": round(upd * args.steps / t_unc * STEP_BYTES_PER_LUP[inp.ndim] / 1e9 / HBM_PEAK_GBS, 4)}
+            print(f"# uncoded: {t_unc * 1e3 / args.steps:.3f} ms/step", file=sys.stderr, flush=True)
+    copy_gbs = engine.copy_bandwidth(1 << 30, repeats=10)
+    glups = upd * args.steps / elapsed / 1e9
+    bpc = algorithmic_bytes_per_cell(inp.ndim, inp.isotropic, uniform=layout["uniform"],
+                                     coded_fraction=layout["coded_fraction"])
+    bpc_unc = algorithmic_bytes_per_cell(inp.ndim, inp.isotropic, uniform=layout["uniform"])
+    dominant = max(("velocity", "pressure"), key=lambda k: kms[k])
+    avg_s = kms[dominant] / 1e3 / args.steps
+    achieved = bpc[dominant] * upd / avg_s / 1e9
+    dims = f"{args.nx}x{args.ny}" + (f"x{args.nz}" if inp.ndim == 3 else "")
+    ext = "x".join(str(s) for s in inp.shape)
+    return {
+        "metric": METRIC,
+        "value": round(glups, 3),
+        "unit": "GLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{wl.name}: {cfg['desc']}; {dims} interior ({ext} extended), {args.steps} timed steps "
+                        f"of the {inp.nt}-step job, isotropic relaxation, {inp.ncoords} source points, "
+                        f"{inp.ncoordsout} sensors recorded every {inp.modt} step(s); GLUPS over the {upd} updated "
+                        f"cells (the 8-cell ghost border is not updated)",
+            "cells_extended": inp.ncells,
+            "cells_updated": upd,
+            "sources": inp.ncoords,
+            "sensors": inp.ncoordsout,
+            "record_every": inp.modt,
+            "nt": inp.nt,
+            "finite_output": finite,
+        },
+        "glups_extended_grid": round(inp.ncells * args.steps / elapsed / 1e9, 3),
+        "uncoded": uncoded,
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dominant,
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": measured_traffic(wl.name, dominant, kcfg, layout["codes"]),
+            "bytes_per_cell": bpc[dominant],
+            "bytes_per_cell_uncoded": bpc_unc[dominant],
+            "updated_cells_per_launch": upd,
+            "avg_launch_us": round(avg_s * 1e6, 2),
+            "kernel_ms_per_step": {k: round(v / args.steps, 5) for k, v in kms.items()},
+            "copy_kernel_gbs": round(copy_gbs, 1),
+            "kernel_config": kcfg,
+            "kernels": engine.kernel_override_string(kcfg),
+            "kernel_source": source,
+            "relaxation_codes": layout["codes"],
+            "coded_fraction": layout["coded_fraction"],
+            "step_bytes_per_lup": STEP_BYTES_PER_LUP[inp.ndim],
+            "step_frac": round(glups * STEP_BYTES_PER_LUP[inp.ndim] / HBM_PEAK_GBS, 4),
+        },
+        "_inputs": inp,
     }
 
 
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=8000)
-    ap.add_argument("--warmup", type=int, default=100)
-    ap.add_argument("--config", default="focused_2d")
-    ap.add_argument("--nx", type=int, default=4096)
-    ap.add_argument("--ny", type=int, default=2048)
-    ap.add_argument("--nz", type=int, default=256)
-    ap.add_argument("--cpu-steps", type=int, default=150)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--config", default="hetero_3d",
+                    choices=(*SINGLE_CONFIGS, "batched_2d", "served_mix", "grouped_2d", "adjoint_2d", "inclusions_3d"))
+    ap.add_argument("--nx", type=int, default=None)
+    ap.add_argument("--ny", type=int, default=None)
+    ap.add_argument("--nz", type=int, default=None)
+    ap.add_argument("--cpu-steps", type=int, default=0,
+                    help="steps of the CPU baseline (default: 30 on the 3D slab, 150 on 2D grids)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=0,
                     help="run only this many steps and exit (for rocprofv3)")
@@ -542,138 +677,83 @@ def main() -> int:
     ap.add_argument("--arrival-factor", type=float, default=2.0,
                     help="served_mix: arrival rate as a multiple of the sequential service rate")
     args = ap.parse_args()
+    if args.config == "inclusions_3d":  # the workload's own name
+        args.config = "hetero_3d"
+    # grid sizes: the config's own unless given
+    own = {"batched_2d": (2048, 1536, 1), "grouped_2d": (2048, 2048, 1), "adjoint_2d": (1024, 1024, 1),
+           "served_mix": (1024, 1024, 256)}
+    dflt = own.get(args.config) or tuple(SINGLE_CONFIGS[args.config][k] for k in ("nx", "ny", "nz"))
+    args.nx = dflt[0] if args.nx is None else args.nx
+    args.ny = dflt[1] if args.ny is None else args.ny
+    args.nz = dflt[2] if args.nz is None else args.nz
     if args.kernels and args.kernels != "tune":
         os.environ["FULLWAVE_HIP_KERNELS"] = args.kernels
-    if args.config in ("batched_2d", "served_mix", "hetero_3d", "grouped_2d", "adjoint_2d"):
-        if args.gpus != 1:
-            print(json.dumps({"error": "this build measures one GPU; run with --gpus 1"}))
-            return 2
-        fn = {"batched_2d": bench_batched, "served_mix": bench_served, "hetero_3d": bench_3d,
-              "grouped_2d": bench_grouped, "adjoint_2d": bench_adjoint}[args.config]
-        print(json.dumps(fn(args)), flush=True)
-        return 0
     if args.pmc_summary:
         dirs = [Path(p) for p in args.pmc_summary]
         print(json.dumps(pmc_summary(*dirs[:4], args.copy_bytes, dirs[4]), indent=1))
         return 0
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU (one simulation or batch per GPU); "
+                                   "run with --gpus 1"}), flush=True)
+        return 2
     if args.copy_probe:
         from fullwave25_amd import engine
 
         gbs = engine.copy_bandwidth(args.copy_probe, repeats=5)
         print(json.dumps({"copy_bytes": args.copy_probe, "copy_gbs": round(gbs, 1)}), flush=True)
         return 0
-    if args.gpus != 1:
-        print(json.dumps({"error": "this engine runs one simulation per GPU and this build measures one GPU; "
-                          "run with --gpus 1"}))
-        return 2
+    if args.config in ("batched_2d", "served_mix", "grouped_2d", "adjoint_2d"):
+        if args.config in ("batched_2d", "adjoint_2d") and "--steps" not in sys.argv:
+            args.steps = 3000 if args.config == "batched_2d" else 2000
+        fn = {"batched_2d": bench_batched, "served_mix": bench_served, "grouped_2d": bench_grouped,
+              "adjoint_2d": bench_adjoint}[args.config]
+        print(json.dumps(fn(args)), flush=True)
+        return 0
+    if args.sweep:
+        return sweep(args)
+    line = bench_single(args)
+    inp = line.pop("_inputs")
+    if "metric" not in line:  # --profile-steps
+        print(json.dumps(line), flush=True)
+        return 0
+    if not args.no_cpu_baseline:
+        # the GPU line is printed whatever happens here
+        try:
+            steps = args.cpu_steps or CPU_STEPS.get(args.config, 150)
+            line["cpu_baseline"] = cpu_baseline(inp, steps, slab_planes=CPU_SLAB_PLANES.get(args.config, 0))
+        except Exception as exc:  # noqa: BLE001
+            line["cpu_baseline"] = {"value": None, "unit": "GLUPS", "error": f"{type(exc).__name__}: {exc}"}
+    print(json.dumps(line), flush=True)
+    return 0
 
+
+def sweep(args) -> int:
+    """Time every kernel family and shape over ``--sweep`` steps on the config's workload."""
     from fullwave25_amd import engine
 
-    t0 = time.time()
-    wl = build_workload(args)
+    wl = build_single(args)
     inp = wl.inputs
-    print(f"# workload {wl.name} {inp.shape} nt={inp.nt} built in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
-    if args.sweep:
-        shapes = ["percell"] + [f"stream:{w}:{c}" for w in (128, 256) for c in (8, 16, 32)]
-        shapes += [f"tile:{c}" for c in (8, 16, 32)]
-        if inp.ndim == 3:
-            shapes = ["percell"] + [f"{v}:{w}:{c}" for v in ("stream", "pipe", "pipelite") for w in (512, 1024)
-                                    for c in (8, 16, 32)]
-        if args.sweep_shapes:
-            shapes = args.sweep_shapes.split(",")
-        updated = int(np.prod([s - 16 for s in inp.shape]))
-        for shape in shapes:
-            os.environ["FULLWAVE_HIP_KERNELS"] = shape
-            with engine.HipSimulation(inp, device=0) as sim:
-                layout = coefficient_layout(sim, inp)
-                bpc = algorithmic_bytes_per_cell(inp.ndim, inp.isotropic, uniform=layout["uniform"],
-                                                 coded_fraction=layout["coded_fraction"])
-                sim.run_timed(min(20, args.sweep))
-                sim.reset()
-                kms = sim.run_timed(args.sweep)
-            gbs = {k: bpc[k] * updated / (kms[k] / 1e3 / args.sweep) / 1e9 for k in ("velocity", "pressure")}
-            print(json.dumps({"shape": shape, "us_per_step": {k: round(v / args.sweep * 1e3, 2) for k, v in kms.items()},
-                              "GBs": {k: round(v, 1) for k, v in gbs.items()}}), flush=True)
-        os.environ.pop("FULLWAVE_HIP_KERNELS", None)
-        return 0
-    source = pin_kernels(args, wl.name, args.config == "focused_2d" and (args.nx, args.ny) == (4096, 2048))
-    sim = engine.HipSimulation(inp, device=0)
-    kcfg = sim.kernel_config()
-    layout = coefficient_layout(sim, inp)
-    if args.profile_steps:
-        sim.run(args.profile_steps)
-        sim.close()
-        print(json.dumps({"kernel_config": kcfg, "kernels": engine.kernel_override_string(kcfg),
-                          "codes": layout["codes"]}), flush=True)
-        return 0
-    sim.run(min(args.warmup, inp.nt))
-    sim.reset()
-    # fw_sim_run returns after synchronizing the engine's stream, so the wall clock around it
-    # brackets exactly the K steps (one GPU: no barrier to take)
-    t1 = time.perf_counter()
-    sim.run(args.steps)
-    elapsed = time.perf_counter() - t1
-    traces = sim.sensors()
-    finite = bool(np.isfinite(traces).all())
-    # second pass over the same steps with HIP events around every launch
-    sim.reset()
-    kms = sim.run_timed(args.steps)
-    sim.close()
-    copy_gbs = engine.copy_bandwidth(1 << 30, repeats=10)
-
-    cells = inp.ncells
-    updated = int(np.prod([s - 16 for s in inp.shape]))
-    glups = cells * args.steps / elapsed / 1e9
-    bpc = algorithmic_bytes_per_cell(inp.ndim, inp.isotropic, uniform=layout["uniform"],
-                                     coded_fraction=layout["coded_fraction"])
-    dominant = max(("velocity", "pressure"), key=lambda k: kms[k])
-    avg_s = kms[dominant] / 1e3 / args.steps
-    achieved = bpc[dominant] * updated / avg_s / 1e9
-    line = {
-        "metric": METRIC,
-        "value": round(glups, 3),
-        "unit": "GLUPS",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(elapsed / args.steps * 1e3, 5),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic",
-        "config": {
-            "workload": f"{wl.name}: 2D heterogeneous {args.nx}x{args.ny} interior "
-                        f"({inp.nx}x{inp.ny} extended), {args.steps} steps, isotropic relaxation",
-            "cells": cells,
-            "sources": inp.ncoords,
-            "sensors": inp.ncoordsout,
-            "finite_output": finite,
-        },
-        "roofline": {
-            "bound": "hbm",
-            "kernel": dominant,
-            "achieved": round(achieved, 1),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": measured_traffic(wl.name, dominant, kcfg, layout["codes"]),
-            "bytes_per_cell": bpc[dominant],
-            "avg_launch_us": round(avg_s * 1e6, 2),
-            "kernel_ms_per_step": {k: round(v / args.steps, 5) for k, v in kms.items()},
-            "copy_kernel_gbs": round(copy_gbs, 1),
-            "kernel_config": kcfg,
-            "kernels": engine.kernel_override_string(kcfg),
-            "kernel_source": source,
-            "relaxation_codes": layout["codes"],
-            "coded_fraction": layout["coded_fraction"],
-            "step_bytes_per_lup": STEP_BYTES_PER_LUP[inp.ndim],
-            "step_frac": round(glups * STEP_BYTES_PER_LUP[inp.ndim] / HBM_PEAK_GBS, 4),
-        },
-    }
-    if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(inp, args.cpu_steps)
-    print(json.dumps(line), flush=True)
+    shapes = ["percell"] + [f"stream:{w}:{c}" for w in (128, 256) for c in (8, 16, 32)]
+    shapes += [f"tile:{c}" for c in (8, 16, 32)]
+    if inp.ndim == 3:
+        shapes = ["percell"] + [f"{v}:{w}:{c}" for v in ("stream", "pipe", "pipelite") for w in (512, 1024)
+                                for c in (8, 16, 32)]
+    if args.sweep_shapes:
+        shapes = args.sweep_shapes.split(",")
+    upd = updated_cells(inp)
+    for shape in shapes:
+        os.environ["FULLWAVE_HIP_KERNELS"] = shape
+        with engine.HipSimulation(inp, device=0) as sim:
+            layout = coefficient_layout(sim, inp)
+            bpc = algorithmic_bytes_per_cell(inp.ndim, inp.isotropic, uniform=layout["uniform"],
+                                             coded_fraction=layout["coded_fraction"])
+            sim.run_timed(min(20, args.sweep))
+            sim.reset()
+            kms = sim.run_timed(args.sweep)
+        gbs = {k: bpc[k] * upd / (kms[k] / 1e3 / args.sweep) / 1e9 for k in ("velocity", "pressure")}
+        print(json.dumps({"shape": shape, "us_per_step": {k: round(v / args.sweep * 1e3, 2) for k, v in kms.items()},
+                          "GBs": {k: round(v, 1) for k, v in gbs.items()}}), flush=True)
+    os.environ.pop("FULLWAVE_HIP_KERNELS", None)
     return 0
 
 

@@ -102,7 +102,7 @@ def plane_wave_3d(nx: int = 256, ny: int = 256, nz: int = 256, nt: int = 1000, *
 
 def inclusions_3d(nx: int = 768, ny: int = 768, nz: int = 640, nt: int = 6000, *, n_inclusions: int = 20,
                   seed: int = 2028, n_pml: int = 24, n_transition: int = 36, f0: float = 2.5e6,
-                  n_el: int = 32, ntic: int = 400, record_every: int = 4, radii=(20, 120)) -> Workload:
+                  n_el: int = 32, ntic: int = 400, record_every: int = 1, radii=(20, 120)) -> Workload:
     """3D heterogeneous medium with ellipsoidal inclusions and a matrix-array plane-wave transmit
     (the 768 x 768 x 640 / 6000-step configuration at default sizes; 904 x 904 x 776 cells with
     the 24-cell absorbing layer, 36-cell transition and ghost border).
@@ -112,8 +112,9 @@ def inclusions_3d(nx: int = 768, ny: int = 768, nz: int = 640, nt: int = 6000, *
     with c ~ U[1450, 1620], alpha0 ~ U[0.3, 1.5] (steps of 0.1) and rho within +-3 %. The
     transmit: a ``n_el`` x ``n_el`` array (pitch one wavelength, one-cell kerf) on the first
     interior plane, every element firing the same 3-cycle pulse (``ntic`` samples). Receivers:
-    the array face one plane deeper (every point) and three planes at 1/4, 1/2 and 3/4 depth
-    (every 8th point in y and z), recorded every ``record_every`` steps. Built plane by plane
+    the full-aperture receive plane one plane deeper (every y-z point of the interior) and the
+    three orthogonal mid-planes x = nx/2, y = ny/2, z = nz/2 (every point; 2,060,802 distinct
+    sensors at the default size), recorded every ``record_every`` steps. Built plane by plane
     (:func:`~fullwave25_amd.solver.fast_inputs.build_inputs_planewise`).
     """
     from .solver.fast_inputs import build_inputs_planewise
@@ -154,9 +155,10 @@ def inclusions_3d(nx: int = 768, ny: int = 768, nz: int = 640, nt: int = 6000, *
     sig = gaussian_modulated_sinusoidal_signal(ntic, ntic * grid.dt, 3, 2, f0, 1e5)
     source = Source(np.broadcast_to(sig, (int(face.sum()), ntic)), mask)
     smask = np.zeros(shape, bool)
-    smask[1] = face
-    for frac in (0.25, 0.5, 0.75):
-        smask[int(frac * nx), ::8, ::8] = True
+    smask[1] = True
+    smask[nx // 2] = True
+    smask[:, ny // 2] = True
+    smask[:, :, nz // 2] = True
     sensor = Sensor(smask, sampling_modulus_time=record_every)
     inputs = build_inputs_planewise(grid, medium, source, sensor, n_pml=n_pml, n_transition=n_transition)
     return Workload("inclusions_3d", inputs)
