@@ -341,10 +341,20 @@ def generate_relaxation_params(
     (same signature and clipping as fullwave/utils/relaxation_parameters.py:78-112).
 
     ``dt`` fits the engine's discrete-time model for that time step (``Medium.build`` passes
-    its grid's); ``path_database`` uses a saved table instead of fitting."""
+    its grid's); ``path_database`` uses a saved table instead of fitting. A saved table fitted
+    for another time step is refused (its discrete-time fit would give another attenuation on
+    this grid); one saved without a time step (the reference's format, a continuous-time fit)
+    is used with a warning."""
     if n_relaxation_mechanisms != 2:
         raise ValueError("Only n_relaxation_mechanisms=2 are supported currently.")
     table = RelaxationTable.load(path_database) if path_database is not None else default_table(dt, band)
+    if path_database is not None and dt is not None:
+        if table.dt is None:
+            logger.warning("relaxation table %s has no time step: it is a continuous-time fit, not the "
+                           "discrete-time fit for dt = %.6g s", path_database, dt)
+        elif abs(table.dt - dt) > 1e-9 * abs(dt):
+            raise ValueError(f"relaxation table {path_database} was fitted for dt = {table.dt:.9g} s, this grid's "
+                             f"time step is {dt:.9g} s; fit a table for it (RelaxationTable(dt=...).fill())")
     a = np.asarray(alpha_coeff, dtype=np.float64)
     g = np.asarray(alpha_power, dtype=np.float64)
     if np.any(a < table.alpha_min) or np.any(g < table.power_min):

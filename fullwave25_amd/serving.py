@@ -298,11 +298,11 @@ class SimulationServer:
             else:
                 self._configs[key] = sim.kernel_config()  # tunes now, on this handle's arrays
 
-    def kernel_config_for(self, inp: SimulationInputs, n_shots: int = 1) -> dict | None:
+    def kernel_config_for(self, inp: SimulationInputs, n_shots: int = 1, *, adjoint: bool = False) -> dict | None:
         """The kernel configuration this server runs (or ran) jobs of this shape with: the group
-        shape for grouped jobs, else the one tuned on the first handle of the shape (None if no
-        such job has run yet)."""
-        if self._grouped(inp, n_shots):
+        shape for grouped forward jobs, else the one tuned on the first handle of the shape (None
+        if no such job has run yet). Misfit-gradient jobs (``adjoint``) never join a group."""
+        if not adjoint and self._grouped(inp, n_shots):
             shape = _group_shape(inp)
             return {"velocity": shape, "pressure": shape}
         with self._lock:

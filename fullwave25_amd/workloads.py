@@ -262,12 +262,14 @@ def focused_linear_array_2d(nx: int = 4096, ny: int = 2048, nt: int = 8000, *, n
 
 def steered_transmits_2d(nx: int = 2048, ny: int = 1536, nt: int = 3000, *, n_transmits: int = 128,
                          n_elements: int = 128, pitch_cells: int = 12, focus_m: float = 30e-3,
-                         seed: int = 2027, n_pml: int = 36, layers: int = 3, f0: float = 3e6):
+                         seed: int = 2027, n_pml: int = 36, layers: int = 3, f0: float = 3e6,
+                         transmits=None):
     """Many focused transmits on one seeded heterogeneous medium (the batched-transmit
     configuration: 2048 x 1536, 128 transmits, each focused at ``focus_m`` and steered to its own
     lateral position, spread evenly across the aperture). Returns ``(workload, shots)``:
-    ``workload.inputs`` carries the first transmit, ``shots`` all source matrices
-    ``[n_transmits, n_points, nt]`` (float32) in the order of the inputs' source points."""
+    ``workload.inputs`` carries the first transmit built, ``shots`` the source matrices
+    ``[n, n_points, nt]`` (float32) in the order of the inputs' source points, of every transmit
+    or, with ``transmits`` (indices into the ``n_transmits``), of those only."""
     grid = _grid(nx, ny, nt=nt, f0=f0)
     shape = (grid.nx, grid.ny)
     rng = np.random.default_rng(seed)
@@ -280,6 +282,8 @@ def steered_transmits_2d(nx: int = 2048, ny: int = 1536, nt: int = 3000, *, n_tr
     coords = np.argwhere(mask)
     half = 0.5 * n_elements * pitch_cells * grid.dx
     laterals = np.linspace(-0.8 * half, 0.8 * half, n_transmits)
+    if transmits is not None:
+        laterals = laterals[np.asarray(transmits, dtype=np.int64)]
     shots = np.stack([focused_signals(grid, coords, elem_of_col, centres, focus_m=focus_m, lateral_m=x, f0=f0)
                       for x in laterals]).astype(np.float32)
     smask = np.zeros(shape, dtype=bool)
@@ -287,6 +291,34 @@ def steered_transmits_2d(nx: int = 2048, ny: int = 1536, nt: int = 3000, *, n_tr
     inputs = _inputs(grid, medium, Source(shots[0].astype(np.float64), mask), Sensor(smask), n_pml=n_pml)
     elem = elem_of_col[np.argwhere(smask)[:, 1]]
     return Workload("steered_transmits_2d", inputs, element_of_sensor=elem, n_elements=n_elements), shots
+
+
+def misfit_media_2d(nx: int = 1024, ny: int = 1024, nt: int = 2000, *, seed: int = 2029,
+                    smooth_cells: float = 12.0, n_pml: int = 20, f0: float = 3e6):
+    """The misfit-gradient job of the served mix: a seeded "true" heterogeneous medium
+    (:func:`layered_medium_2d`) and a smoothed initial model of it (c and rho filtered with a
+    Gaussian of ``smooth_cells``; the attenuation maps are the true ones), both with a 3-cycle
+    tone burst on the top row and a line of receivers two rows below it (reflection geometry, as
+    with an array on the skin). Returns
+    ``(initial, true)`` workloads; the true medium's traces are the observed data the initial
+    model is fitted to."""
+    from scipy.ndimage import gaussian_filter
+
+    grid = _grid(nx, ny, nt=nt, f0=f0)
+    shape = (grid.nx, grid.ny)
+    rng = np.random.default_rng(seed)
+    c, rho, a0, gm = layered_medium_2d(shape, rng, wavelength_cells=int(round(grid.wavelength / grid.dx)))
+    mask = np.zeros(shape, dtype=bool)
+    mask[0, :] = True
+    sig = gaussian_modulated_sinusoidal_signal(grid.nt, grid.duration, 3, 2, f0, 1e5)
+    smask = np.zeros(shape, dtype=bool)
+    smask[2, :] = True
+    out = []
+    for cc, rr in ((gaussian_filter(c, smooth_cells), gaussian_filter(rho, smooth_cells)), (c, rho)):
+        medium = Medium(grid, cc, rr, a0, gm, np.zeros(shape))
+        source = Source(np.tile(sig, (int(mask.sum()), 1)), mask)
+        out.append(Workload("misfit_media_2d", _inputs(grid, medium, source, Sensor(smask), n_pml=n_pml)))
+    return out[0], out[1]
 
 
 def element_traces(traces: np.ndarray, workload: Workload) -> np.ndarray:

@@ -226,11 +226,17 @@ int SFX(fwo_set_field)(void* handle, int which, const double* in) {
 
 /* ---- adjoint (reverse mode) ------------------------------------------------------------ *
  * Vector-Jacobian product of the recorded traces with respect to the K and rho maps: runs
- * nsteps from a zero state keeping every intermediate state, then sweeps back through the
- * transposed step. grad_rec is [ceil(nsteps / modt)][nout]; gK and grho (length n) receive
- * sum_{n,j} grad_rec[n][j] * d rec[n][j] / d K[c] (and / d rho[c]), with the coefficient sets
- * (dcmap) held fixed. The transposed stencils are applied in scatter form (loops over the cells
- * that read a value), the GPU kernels use the gather form. Single-threaded. */
+ * nsteps from a zero state keeping a checkpoint every ceil(sqrt(nsteps)) steps, then sweeps back
+ * one segment at a time: the segment's states are recomputed from its checkpoint, then the
+ * transposed steps run from its last step to its first. grad_rec is [ceil(nsteps / modt)][nout];
+ * gK and grho (length n) receive sum_{n,j} grad_rec[n][j] * d rec[n][j] / d K[c] (and / d rho[c]),
+ * with the coefficient sets (dcmap) held fixed. The transposed stencils are applied in scatter
+ * form (loops over the cells that read a value), the GPU kernels use the gather form. The
+ * per-cell loops run in parallel over x rows; a scatter from row i writes rows i - 8 .. i + 8,
+ * so the scatter loops take the rows in blocks of 17 and run the even blocks in parallel, then
+ * the odd ones (no two threads write one value). The state is left at the end of the run. */
+
+#define FWO_ROWBLK 17
 
 static inline int64_t SFX(fwo_nstate)(const SFX(fwo_state)* st) {
     return st->n * (1 + st->P.ndim + (st->aexp ? 0 : 4 * st->P.ndim));
@@ -247,6 +253,19 @@ static void SFX(fwo_pack)(const SFX(fwo_state)* st, REAL* dst) {
         for (int nu = 0; nu < 2; ++nu, dst += n) memcpy(dst, st->psi[a][nu], sizeof(REAL) * n);
     for (int a = 0; a < nd; ++a)
         for (int nu = 0; nu < 2; ++nu, dst += n) memcpy(dst, st->phi[a][nu], sizeof(REAL) * n);
+}
+
+static void SFX(fwo_unpack)(SFX(fwo_state)* st, const REAL* src) {
+    const int nd = st->P.ndim;
+    const int64_t n = st->n;
+    memcpy(st->p, src, sizeof(REAL) * n);
+    src += n;
+    for (int a = 0; a < nd; ++a, src += n) memcpy(st->v[a], src, sizeof(REAL) * n);
+    if (st->aexp) return;
+    for (int a = 0; a < nd; ++a)
+        for (int nu = 0; nu < 2; ++nu, src += n) memcpy(st->psi[a][nu], src, sizeof(REAL) * n);
+    for (int a = 0; a < nd; ++a)
+        for (int nu = 0; nu < 2; ++nu, src += n) memcpy(st->phi[a][nu], src, sizeof(REAL) * n);
 }
 
 /* views into a packed state */
@@ -327,15 +346,118 @@ static void SFX(fwo_dminus_t)(const SFX(fwo_state)* st, REAL* out, int64_t c, in
     }
 }
 
+/* pressure update, transposed, at every active cell of x row i: H, the phi adjoints, gK */
+static void SFX(fwo_adj_pressure_row)(SFX(fwo_state)* st, int64_t i, const REAL* cur, REAL* lp, REAL* lphi[3][2],
+                                      REAL* H, double* gK) {
+    const fwo_problem* P = &st->P;
+    const int nd = P->ndim;
+    const int64_t n = st->n, ny = P->ny, nz = nd == 3 ? P->nz : 1;
+    for (int64_t j = 0; j < ny; ++j)
+        for (int64_t k = 0; k < nz; ++k) {
+            if (!SFX(fwo_active)(st, i, j, k)) continue;
+            const int64_t c = (i * ny + j) * nz + k;
+            const REAL mu = lp[c];
+            REAL div = 0;
+            if (!st->aexp) {
+                const REAL t = -st->dt * st->K[c] * mu;
+                for (int a = 0; a < nd; ++a) {
+                    const REAL hfw = SFX(fwo_dminus)(st, SFX(fwo_sv)(st, cur, a), c, a);
+                    REAL lh = t / st->kp[a][c];
+                    for (int nu = 0; nu < 2; ++nu) {
+                        const REAL tot = lphi[a][nu][c] + t;
+                        lh += st->ap[a][nu][c] * tot;
+                        lphi[a][nu][c] = st->bp[a][nu][c] * tot;
+                        div += SFX(fwo_sphi)(st, cur, a, nu)[c];
+                    }
+                    div += hfw / st->kp[a][c];
+                    H[a * n + c] = lh;
+                }
+                gK[c] += (double)(mu * (-st->dt) * div);
+            } else {
+                const REAL A = st->aexp[c];
+                for (int a = 0; a < nd; ++a) {
+                    div += SFX(fwo_dminus)(st, SFX(fwo_sv)(st, cur, a), c, a);
+                    H[a * n + c] = -A * st->dt * st->K[c] * mu;
+                }
+                lp[c] = A * mu;
+                gK[c] += (double)(mu * (-A * st->dt) * div);
+            }
+        }
+}
+
+/* velocity update, transposed, at every active cell of x row i: G, the psi adjoints, grho
+ * (which also receives at row i + 1) */
+static void SFX(fwo_adj_velocity_row)(SFX(fwo_state)* st, int64_t i, const REAL* prev, const REAL* cur, REAL* lv[3],
+                                      REAL* lpsi[3][2], REAL* G, double* grho) {
+    const fwo_problem* P = &st->P;
+    const int nd = P->ndim;
+    const int64_t n = st->n, ny = P->ny, nz = nd == 3 ? P->nz : 1;
+    for (int64_t j = 0; j < ny; ++j)
+        for (int64_t k = 0; k < nz; ++k) {
+            if (!SFX(fwo_active)(st, i, j, k)) continue;
+            const int64_t c = (i * ny + j) * nz + k;
+            for (int a = 0; a < nd; ++a) {
+                const int64_t sa = st->s[a];
+                const REAL nu_v = lv[a][c];
+                const REAL rb = (REAL)0.5 * (st->rho[c] + st->rho[c + sa]);
+                const REAL beta = st->dt / rb;
+                const REAL g = SFX(fwo_dplus)(st, SFX(fwo_sp)(st, prev), c, a);
+                REAL u, lg;
+                if (!st->aexp) {
+                    u = g / st->kv[a][c];
+                    lg = -beta * nu_v / st->kv[a][c];
+                    for (int q = 0; q < 2; ++q) {
+                        const REAL tot = lpsi[a][q][c] - beta * nu_v;
+                        lg += st->av[a][q][c] * tot;
+                        lpsi[a][q][c] = st->bv[a][q][c] * tot;
+                        u += SFX(fwo_spsi)(st, cur, a, q)[c];
+                    }
+                } else {
+                    const REAL A = st->aexp[c];
+                    u = A * g;
+                    lg = -A * beta * nu_v;
+                    lv[a][c] = A * nu_v;
+                }
+                G[a * n + c] = lg;
+                /* v -= beta u with beta = dt / rho_bar, rho_bar = (rho_c + rho_c+e_a) / 2 */
+                const double drb = (double)(nu_v * (-u)) * (double)(-st->dt / (rb * rb));
+                grho[c] += 0.5 * drb;
+                grho[c + sa] += 0.5 * drb;
+            }
+        }
+}
+
+/* the transposed derivatives of x row i scattered: kind 0 = (D-_a)^T H into lv, 1 = (D+_a)^T G
+ * into lp */
+static void SFX(fwo_adj_scatter_row)(SFX(fwo_state)* st, int64_t i, int kind, REAL* lp, REAL* lv[3], const REAL* H,
+                                     const REAL* G) {
+    const fwo_problem* P = &st->P;
+    const int nd = P->ndim;
+    const int64_t n = st->n, ny = P->ny, nz = nd == 3 ? P->nz : 1;
+    for (int64_t j = 0; j < ny; ++j)
+        for (int64_t k = 0; k < nz; ++k) {
+            if (!SFX(fwo_active)(st, i, j, k)) continue;
+            const int64_t c = (i * ny + j) * nz + k;
+            for (int a = 0; a < nd; ++a) {
+                if (kind == 0) SFX(fwo_dminus_t)(st, lv[a], c, a, H[a * n + c]);
+                else SFX(fwo_dplus_t)(st, lp, c, a, G[a * n + c]);
+            }
+        }
+}
+
 int SFX(fwo_adjoint)(void* handle, int64_t nsteps, const double* grad_rec, double* gK, double* grho) {
     SFX(fwo_state)* st = (SFX(fwo_state)*)handle;
     const fwo_problem* P = &st->P;
     const int nd = P->ndim;
     const int64_t n = st->n, ns = SFX(fwo_nstate)(st);
     const int relax = st->aexp == 0;
-    const int64_t nx = P->nx, ny = P->ny, nz = nd == 3 ? P->nz : 1;
+    const int64_t nx = P->nx;
+    const int64_t nblk = (nx + FWO_ROWBLK - 1) / FWO_ROWBLK;
+    int64_t every = 1;
+    while (every * every < nsteps) ++every;
+    const int64_t nck = nsteps > 0 ? (nsteps + every - 1) / every : 0;
 
-    /* forward from zero, keeping the state after every step (snap[0] = initial) */
+    /* forward from zero, a checkpoint before every `every`-th step, and the final state */
     memset(st->p, 0, sizeof(REAL) * n);
     for (int a = 0; a < nd; ++a) {
         memset(st->v[a], 0, sizeof(REAL) * n);
@@ -345,19 +467,26 @@ int SFX(fwo_adjoint)(void* handle, int64_t nsteps, const double* grad_rec, doubl
         }
     }
     st->step = 0;
-    REAL* snap = (REAL*)malloc(sizeof(REAL) * (size_t)(ns * (nsteps + 1)));
-    if (!snap) return 1;
-    SFX(fwo_pack)(st, snap);
-    for (int64_t it = 0; it < nsteps; ++it) {
-        SFX(fwo_run)(handle, 1, 0);
-        SFX(fwo_pack)(st, snap + ns * (it + 1));
-    }
-
+    REAL* ck = (REAL*)malloc(sizeof(REAL) * (size_t)(ns * (nck + 1)));
+    REAL* seg = (REAL*)malloc(sizeof(REAL) * (size_t)(ns * (every + 1)));
     REAL* lam = (REAL*)calloc((size_t)(ns + 2 * nd * n), sizeof(REAL)); /* adjoint state + H, G */
+    if (!ck || !seg || !lam) {
+        free(ck);
+        free(seg);
+        free(lam);
+        return 1;
+    }
+    REAL* fin = ck + ns * nck;
+    for (int64_t it = 0; it < nsteps; ++it) {
+        if (it % every == 0) SFX(fwo_pack)(st, ck + ns * (it / every));
+        SFX(fwo_run)(handle, 1, 0);
+    }
+    SFX(fwo_pack)(st, fin);
+
     REAL* lp = lam;
-    REAL* lv[3];
-    REAL* lpsi[3][2];
-    REAL* lphi[3][2];
+    REAL* lv[3] = {0, 0, 0};
+    REAL* lpsi[3][2] = {{0, 0}, {0, 0}, {0, 0}};
+    REAL* lphi[3][2] = {{0, 0}, {0, 0}, {0, 0}};
     REAL* H = lam + ns;      /* [nd][n] */
     REAL* G = H + nd * n;    /* [nd][n] */
     for (int a = 0; a < nd; ++a) {
@@ -369,106 +498,58 @@ int SFX(fwo_adjoint)(void* handle, int64_t nsteps, const double* grad_rec, doubl
     }
     for (int64_t c = 0; c < n; ++c) gK[c] = grho[c] = 0;
 
-    for (int64_t it = nsteps - 1; it >= 0; --it) {
-        const REAL* prev = snap + ns * it;       /* state before step it */
-        const REAL* cur = snap + ns * (it + 1);  /* state after step it */
-        /* records of step it, then the sources and pressure-release points (overwritten) */
-        if (P->modt > 0 && it % P->modt == 0) {
-            const double* row = grad_rec + (it / P->modt) * P->nout;
-            for (int64_t j = 0; j < P->nout; ++j) lp[P->outc[j]] += (REAL)row[j];
+    for (int64_t sgi = nck - 1; sgi >= 0; --sgi) {
+        const int64_t s0 = sgi * every, s1 = s0 + every < nsteps ? s0 + every : nsteps;
+        SFX(fwo_unpack)(st, ck + ns * sgi);
+        st->step = s0;
+        SFX(fwo_pack)(st, seg);
+        for (int64_t t = s0; t < s1; ++t) {
+            SFX(fwo_run)(handle, 1, 0);
+            SFX(fwo_pack)(st, seg + ns * (t - s0 + 1));
         }
-        if (it < P->ntic)
-            for (int64_t q = 0; q < P->ncoords; ++q) lp[P->icc[q]] = 0;
-        for (int64_t q = 0; q < P->nzero; ++q) lp[P->zero[q]] = 0;
+        for (int64_t it = s1 - 1; it >= s0; --it) {
+            const REAL* prev = seg + ns * (it - s0);     /* state before step it */
+            const REAL* cur = seg + ns * (it - s0 + 1);  /* state after step it */
+            /* records of step it, then the sources and pressure-release points (overwritten) */
+            if (P->modt > 0 && it % P->modt == 0) {
+                const double* row = grad_rec + (it / P->modt) * P->nout;
+                for (int64_t j = 0; j < P->nout; ++j) lp[P->outc[j]] += (REAL)row[j];
+            }
+            if (it < P->ntic)
+                for (int64_t q = 0; q < P->ncoords; ++q) lp[P->icc[q]] = 0;
+            for (int64_t q = 0; q < P->nzero; ++q) lp[P->zero[q]] = 0;
 
-        /* pressure update, transposed */
-        memset(H, 0, sizeof(REAL) * (size_t)(nd * n));
-        for (int64_t i = 0; i < nx; ++i)
-            for (int64_t j = 0; j < ny; ++j)
-                for (int64_t k = 0; k < nz; ++k) {
-                    if (!SFX(fwo_active)(st, i, j, k)) continue;
-                    const int64_t c = (i * ny + j) * nz + k;
-                    const REAL mu = lp[c];
-                    REAL div = 0;
-                    if (relax) {
-                        const REAL t = -st->dt * st->K[c] * mu;
-                        for (int a = 0; a < nd; ++a) {
-                            const REAL hfw = SFX(fwo_dminus)(st, SFX(fwo_sv)(st, cur, a), c, a);
-                            REAL lh = t / st->kp[a][c];
-                            for (int nu = 0; nu < 2; ++nu) {
-                                const REAL tot = lphi[a][nu][c] + t;
-                                lh += st->ap[a][nu][c] * tot;
-                                lphi[a][nu][c] = st->bp[a][nu][c] * tot;
-                                div += SFX(fwo_sphi)(st, cur, a, nu)[c];
-                            }
-                            div += hfw / st->kp[a][c];
-                            H[a * n + c] = lh;
-                        }
-                    } else {
-                        const REAL A = st->aexp[c];
-                        for (int a = 0; a < nd; ++a) {
-                            div += SFX(fwo_dminus)(st, SFX(fwo_sv)(st, cur, a), c, a);
-                            H[a * n + c] = -A * st->dt * st->K[c] * mu;
-                        }
-                        lp[c] = A * mu;
-                        gK[c] += (double)(mu * (-A * st->dt) * div);
-                        continue;
-                    }
-                    gK[c] += (double)(mu * (-st->dt) * div);
-                }
-        for (int64_t i = 0; i < nx; ++i)
-            for (int64_t j = 0; j < ny; ++j)
-                for (int64_t k = 0; k < nz; ++k) {
-                    if (!SFX(fwo_active)(st, i, j, k)) continue;
-                    const int64_t c = (i * ny + j) * nz + k;
-                    for (int a = 0; a < nd; ++a) SFX(fwo_dminus_t)(st, lv[a], c, a, H[a * n + c]);
-                }
+            /* pressure update, transposed */
+            memset(H, 0, sizeof(REAL) * (size_t)(nd * n));
+#pragma omp parallel for schedule(static)
+            for (int64_t i = 0; i < nx; ++i) SFX(fwo_adj_pressure_row)(st, i, cur, lp, lphi, H, gK);
+            for (int color = 0; color < 2; ++color) {
+#pragma omp parallel for schedule(static)
+                for (int64_t b = color; b < nblk; b += 2)
+                    for (int64_t i = b * FWO_ROWBLK; i < nx && i < (b + 1) * FWO_ROWBLK; ++i)
+                        SFX(fwo_adj_scatter_row)(st, i, 0, lp, lv, H, G);
+            }
 
-        /* velocity update, transposed */
-        memset(G, 0, sizeof(REAL) * (size_t)(nd * n));
-        for (int64_t i = 0; i < nx; ++i)
-            for (int64_t j = 0; j < ny; ++j)
-                for (int64_t k = 0; k < nz; ++k) {
-                    if (!SFX(fwo_active)(st, i, j, k)) continue;
-                    const int64_t c = (i * ny + j) * nz + k;
-                    for (int a = 0; a < nd; ++a) {
-                        const int64_t sa = st->s[a];
-                        const REAL nu_v = lv[a][c];
-                        const REAL rb = (REAL)0.5 * (st->rho[c] + st->rho[c + sa]);
-                        const REAL beta = st->dt / rb;
-                        const REAL g = SFX(fwo_dplus)(st, SFX(fwo_sp)(st, prev), c, a);
-                        REAL u, lg;
-                        if (relax) {
-                            u = g / st->kv[a][c];
-                            lg = -beta * nu_v / st->kv[a][c];
-                            for (int q = 0; q < 2; ++q) {
-                                const REAL tot = lpsi[a][q][c] - beta * nu_v;
-                                lg += st->av[a][q][c] * tot;
-                                lpsi[a][q][c] = st->bv[a][q][c] * tot;
-                                u += SFX(fwo_spsi)(st, cur, a, q)[c];
-                            }
-                        } else {
-                            const REAL A = st->aexp[c];
-                            u = A * g;
-                            lg = -A * beta * nu_v;
-                            lv[a][c] = A * nu_v;
-                        }
-                        G[a * n + c] = lg;
-                        /* v -= beta u with beta = dt / rho_bar, rho_bar = (rho_c + rho_c+e_a) / 2 */
-                        const double drb = (double)(nu_v * (-u)) * (double)(-st->dt / (rb * rb));
-                        grho[c] += 0.5 * drb;
-                        grho[c + sa] += 0.5 * drb;
-                    }
-                }
-        for (int64_t i = 0; i < nx; ++i)
-            for (int64_t j = 0; j < ny; ++j)
-                for (int64_t k = 0; k < nz; ++k) {
-                    if (!SFX(fwo_active)(st, i, j, k)) continue;
-                    const int64_t c = (i * ny + j) * nz + k;
-                    for (int a = 0; a < nd; ++a) SFX(fwo_dplus_t)(st, lp, c, a, G[a * n + c]);
-                }
+            /* velocity update, transposed */
+            memset(G, 0, sizeof(REAL) * (size_t)(nd * n));
+            for (int color = 0; color < 2; ++color) {
+#pragma omp parallel for schedule(static)
+                for (int64_t b = color; b < nblk; b += 2)
+                    for (int64_t i = b * FWO_ROWBLK; i < nx && i < (b + 1) * FWO_ROWBLK; ++i)
+                        SFX(fwo_adj_velocity_row)(st, i, prev, cur, lv, lpsi, G, grho);
+            }
+            for (int color = 0; color < 2; ++color) {
+#pragma omp parallel for schedule(static)
+                for (int64_t b = color; b < nblk; b += 2)
+                    for (int64_t i = b * FWO_ROWBLK; i < nx && i < (b + 1) * FWO_ROWBLK; ++i)
+                        SFX(fwo_adj_scatter_row)(st, i, 1, lp, lv, H, G);
+            }
+        }
     }
+    SFX(fwo_unpack)(st, fin);
+    st->step = nsteps;
     free(lam);
-    free(snap);
+    free(seg);
+    free(ck);
     return 0;
 }

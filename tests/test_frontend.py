@@ -169,3 +169,23 @@ def test_medium_build_uses_grid_time_step():
     assert isinstance(built, MediumRelaxationMaps)
     assert np.all(built.relaxation_param_dict["kappa_x1"] >= 1.0)
     assert np.all(np.isfinite(built.relaxation_param_dict_for_fw2["a_pml_u1"]))
+
+
+def test_saved_table_refused_for_another_time_step(tmp_path):
+    """A table is a fit of the engine's discrete-time recursion at one time step: it records the
+    step, and a grid with another step refuses it; a table without a step (the reference's
+    continuous-time format) is taken with a warning."""
+    dt = 0.4 * (1540 / 3e6 / 12) / 1540
+    t = RelaxationTable(alpha_list=[0.5], power_list=[1.1], dt=dt)
+    t.fill()
+    t.save(tmp_path / "db.mat")
+    a, g = np.full((2, 2), 0.5), np.full((2, 2), 1.1)
+    out = generate_relaxation_params(a, g, path_database=tmp_path / "db.mat", dt=dt)
+    np.testing.assert_allclose(out["kappa_x1"], t.database[0, 0, 0])
+    with pytest.raises(ValueError, match="fitted for dt"):
+        generate_relaxation_params(a, g, path_database=tmp_path / "db.mat", dt=1.5 * dt)
+    u = RelaxationTable(alpha_list=[0.5], power_list=[1.1])
+    u.fill()
+    u.save(tmp_path / "ct.npz")
+    assert RelaxationTable.load(tmp_path / "ct.npz").dt is None
+    generate_relaxation_params(a, g, path_database=tmp_path / "ct.npz", dt=dt)  # warns, does not raise
