@@ -441,9 +441,19 @@ hipError_t launch_tile(const DevSim& s, bool velocity, hipStream_t st) {
 
 }  // namespace
 
+// The most sound speeds (coefficient sets) the tiled kernels stage: two padded sets (one per
+// axis, unequal spacings) of every sound speed next to the largest tile must fit in the 160 KiB of
+// LDS a workgroup can have on gfx950, for both kernels and for the group kernels (which stage the
+// largest member's sets next to the same tiles).
+constexpr int kMaxTileSets = 768;
+constexpr size_t kLdsPerWorkgroup = size_t(160) << 10;
+static_assert(sizeof(PreTile<32>) + size_t(2) * kMaxTileSets * kCS * sizeof(float) <= kLdsPerWorkgroup,
+              "tiled pressure kernel: 32-row tile plus the staged sets exceeds the LDS of a workgroup");
+static_assert(sizeof(VelTile<32>) + size_t(2) * kMaxTileSets * kCS * sizeof(float) <= kLdsPerWorkgroup,
+              "tiled velocity kernel: 32-row tile plus the staged sets exceeds the LDS of a workgroup");
+
 bool tile2d_supported(const DevSim& s) {
-    // ndmap: two padded sets of every sound speed next to the largest tile fit in LDS
-    return !s.aexp && s.ndim == 2 && s.ny % 4 == 0 && s.ndmap <= 768 && s.nx * s.ny < ((int64_t)1 << 30);
+    return !s.aexp && s.ndim == 2 && s.ny % 4 == 0 && s.ndmap <= kMaxTileSets && s.nx * s.ny < ((int64_t)1 << 30);
 }
 
 unsigned group_tile_rows(int64_t nx, int rows) { return (unsigned)((nx - 2 * kHalo + rows - 1) / rows); }

@@ -175,7 +175,7 @@ def test_streaming_kernels_match_per_cell_kernels(monkeypatch, shape):
         assert rel_l2(pa, pb) <= 1e-6
 
 
-@pytest.mark.parametrize("shape", ["tile:8", "tile:16", "stream:256:8"])
+@pytest.mark.parametrize("shape", ["tile:8", "tile:16", "tile:32", "stream:256:8"])
 def test_unequal_spacing(monkeypatch, shape):
     """dy != dx: the x and y stencil weights differ, so the tiled kernels stage both axes'
     coefficient sets (with equal spacings they stage one set per sound speed). Against the
@@ -193,6 +193,46 @@ def test_unequal_spacing(monkeypatch, shape):
     print(f"{shape}, dy = 1.25 dx: rel-L2 {err:.2e} against the oracle", flush=True)
     assert err <= 1e-4
     assert rel_l2(a, _gpu(base)[0]) > 1e-3  # the spacing reaches the stencil
+
+
+def _many_speeds_2d(shape, nt, seed, n_speeds=767):
+    """2D inputs whose sound speed spans n_speeds integer values (the tiled kernels' limit is
+    768 coefficient sets), smooth in space, with dy = 1.25 dx so both axes' sets are staged."""
+    from scipy.ndimage import gaussian_filter
+
+    rng = np.random.default_rng(seed)
+    f = gaussian_filter(rng.random(shape), 6.0)
+    c = 1000.0 + (n_speeds - 1) * (f - f.min()) / np.ptp(f)
+    src = [[shape[0] // 2, shape[1] // 2], [20, 40]]
+    sig = np.sin(np.arange(nt) * 0.3)[None, :] * np.array([[1.0], [0.7]])
+    inp = raw_inputs(shape, c=c, rho=1000 + 50 * rng.random(shape), nt=nt, icc=src, icmat=sig,
+                     outc=[[30, 30], [50, 150], [12, 180]])
+    return dataclasses.replace(inp, dy=inp.dy * 1.25)
+
+
+@pytest.mark.parametrize("shape", ["tile:8", "tile:32"])
+def test_tiled_kernels_at_the_sound_speed_limit(monkeypatch, shape):
+    """767 sound speeds with unequal spacings: the most coefficient sets the tiled kernels (and
+    the tiled group kernels) stage in LDS, next to their largest tile; against the per-cell
+    kernels and the FP64 oracle."""
+    inp = _many_speeds_2d((72, 200), 120, seed=41)
+    assert inp.ndmap == 767 and inp.ny % 4 == 0
+    monkeypatch.setenv("FULLWAVE_HIP_KERNELS", shape)
+    a, pa = _gpu(inp)
+    monkeypatch.setenv("FULLWAVE_HIP_KERNELS", "percell")
+    b, pb = _gpu(inp)
+    monkeypatch.delenv("FULLWAVE_HIP_KERNELS")
+    assert rel_l2(a, b) <= 1e-6 and rel_l2(pa, pb) <= 1e-6
+    ref, _ = _ref(inp)
+    err = rel_l2(a, ref)
+    print(f"{shape}, {inp.ndmap} sound speeds, dy = 1.25 dx: rel-L2 {err:.2e} against the oracle", flush=True)
+    assert err <= 1e-4
+    rows = int(shape.split(":")[1])
+    with engine.HipSimulation(inp) as sim:
+        with engine.HipGroup([sim], variant="tile", chunk=rows) as grp:
+            assert grp.kernel_config()["velocity"]["variant"] == "tile"
+            grp.run(inp.nt)
+        assert rel_l2(sim.sensors().astype(np.float64), ref) <= 1e-4
 
 
 @pytest.mark.parametrize("shape", ["pipelite:512:8", "pipe:1024:7", "stream:512:9"])
