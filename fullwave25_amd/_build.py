@@ -15,7 +15,8 @@ PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 OUT = PKG / "libfullwave_hip.so"
 SOURCES = ["fw_api.cpp", "fw_adjoint.cpp", "fw_group.cpp", "fw_kernels_simple.hip", "fw_kernels_stream.hip", "fw_kernels_stream3d.hip",
-           "fw_kernels_tile2d.hip", "fw_kernels_pipe3d.hip", "fw_kernels_adjoint.hip", "fw_kernels_codes.hip"]
+           "fw_kernels_tile2d.hip", "fw_kernels_pipe3d.hip", "fw_kernels_adjoint.hip", "fw_kernels_codes.hip",
+           "fw_noalloc.hip"]
 
 HIPCC_FLAGS = [
     "-O3",

@@ -75,11 +75,32 @@ int64_t row_pitch(const fw_desc& d) { return d.ndim == 3 ? (d.nz + 31) / 32 * 32
 // staging as one contiguous copy, staging <-> the padded rows as a device-side pitched copy. A
 // pitched copy straight from pageable host memory moves row by row, at a few hundred MB/s.
 constexpr size_t kStageBytes = size_t(64) << 20;
+constexpr size_t kFixedStageBytes = size_t(4) << 20;  // the same, for a fixed-region handle
 
+size_t stage_cap(const fw_sim* s) { return s->fixed_base ? kFixedStageBytes : kStageBytes; }
+
+// The next 256-byte aligned `bytes` of a fixed-region handle's region.
+hipError_t fixed_carve(const fw_sim* s, void** p, size_t bytes) {
+    const size_t off = (s->fixed_used + 255) / 256 * 256;
+    if (off > s->fixed_size || bytes > s->fixed_size - off) return hipErrorOutOfMemory;
+    *p = s->fixed_base + off;
+    s->fixed_used = off + bytes;
+    return hipSuccess;
+}
+
+// A temporary device buffer. On a fixed-region handle it is carved after everything else and
+// handed back on release (temporaries are released in the reverse order of their creation).
 struct Stage {
     void* p = nullptr;
     bool async = false;
+    bool fixed = false;
+    size_t mark = 0;
     hipError_t get(const fw_sim* s, size_t bytes) {
+        if (s->fixed_base) {
+            fixed = true;
+            mark = s->fixed_used;
+            return fixed_carve(s, &p, bytes);
+        }
         hipError_t e = hipMallocAsync(&p, bytes, s->stream);
         if (e == hipSuccess) {
             async = true;
@@ -92,6 +113,11 @@ struct Stage {
         void* q = p;
         p = nullptr;
         if (!q) return hipSuccess;
+        if (fixed) {  // later work on the stream may still read it: wait before handing it back
+            const hipError_t e = hipStreamSynchronize(s->stream);
+            s->fixed_used = mark;
+            return e;
+        }
         if (async) return hipFreeAsync(q, s->stream);
         hipError_t e = hipStreamSynchronize(s->stream);
         hipError_t e2 = hipFree(q);
@@ -146,7 +172,8 @@ void pinned_put(void* b) {
 
 // synchronous: returns when the data is on the device
 hipError_t h2d(const fw_sim* s, void* dst, const void* src, size_t bytes) {
-    if (bytes < (size_t(256) << 10)) return copy_sync(s, dst, src, bytes, hipMemcpyHostToDevice);
+    // a fixed-region handle takes no pinned buffers: the pageable copy
+    if (s->fixed_base || bytes < (size_t(256) << 10)) return copy_sync(s, dst, src, bytes, hipMemcpyHostToDevice);
     void* buf[2] = {pinned_get(), pinned_get()};
     hipEvent_t ev[2] = {nullptr, nullptr};
     hipError_t e = buf[0] && buf[1] ? hipSuccess : hipErrorOutOfMemory;
@@ -180,7 +207,7 @@ hipError_t copy_grid_in(const fw_sim* s, void* dst, const void* src, size_t esiz
     const size_t row = (size_t)s->row_len * esize, prow = (size_t)s->pitch * esize, rows = (size_t)s->rows;
     hipError_t e;
     if (row == prow) return h2d(s, dst, src, row * rows);
-    const size_t per = kStageBytes / row > 0 ? kStageBytes / row : 1;
+    const size_t per = stage_cap(s) / row > 0 ? stage_cap(s) / row : 1;
     Stage st;
     e = st.get(s, per * row);
     for (size_t r0 = 0; r0 < rows && e == hipSuccess; r0 += per) {
@@ -202,7 +229,7 @@ hipError_t copy_grid_out(const fw_sim* s, void* dst, const void* src, size_t esi
         e = hipMemcpyAsync(dst, src, row * rows, hipMemcpyDeviceToHost, s->stream);
         return e != hipSuccess ? e : hipStreamSynchronize(s->stream);
     }
-    const size_t per = kStageBytes / row > 0 ? kStageBytes / row : 1;
+    const size_t per = stage_cap(s) / row > 0 ? stage_cap(s) / row : 1;
     Stage st;
     e = st.get(s, per * row);
     for (size_t r0 = 0; r0 < rows && e == hipSuccess; r0 += per) {
@@ -264,6 +291,7 @@ bool is_voxel_map(const fw_desc& d, const std::string& name) {
 constexpr uint64_t kPoolKeep = 8ull << 30;
 
 hipError_t dev_malloc(fw_sim* s, void** p, size_t bytes) {
+    if (s->fixed_base) return fixed_carve(s, p, bytes);
     static std::mutex mu;
     static std::set<int> pools_set;
     {
@@ -286,7 +314,7 @@ hipError_t dev_malloc(fw_sim* s, void** p, size_t bytes) {
 }
 
 void dev_free(fw_sim* s, void* p) {
-    if (!p) return;
+    if (!p || s->fixed_base) return;
     auto it = s->plain_alloc.find(p);
     if (it != s->plain_alloc.end()) {
         s->plain_alloc.erase(it);
@@ -313,6 +341,9 @@ int alloc(fw_sim* s, const std::string& name, size_t bytes, void** out) {
     }
     void* p = nullptr;
     hipError_t e = dev_malloc(s, &p, bytes ? bytes : 4);
+    if (e != hipSuccess && s->fixed_base)
+        return fail(FW_ERR_NOMEM, "'%s' (%zu bytes) does not fit in the %zu-byte fixed device region (%zu used)",
+                    name.c_str(), bytes, s->fixed_size, s->fixed_used);
     if (e != hipSuccess) return fail(FW_ERR_NOMEM, "hipMalloc(%zu) for '%s' failed: %s", bytes, name.c_str(), hipGetErrorString(e));
     s->dev[name] = p;
     *out = p;
@@ -473,7 +504,8 @@ int choose_kernels_unpinned(fw_sim* s) {
     def.width = s->ndim == 3 ? 512 : 256;
     def.chunk = 8;
     s->cfg_v = s->cfg_p = def;
-    if (s->fields_set || !stream_ok || (s->pinned[0] && s->pinned[1])) return FW_OK;
+    // a fixed-region handle has no timing events: the default shapes
+    if (s->fields_set || s->fixed_base || !stream_ok || (s->pinned[0] && s->pinned[1])) return FW_OK;
     std::vector<fw::LaunchCfg> cands;
     fw::LaunchCfg pc;
     pc.variant = 0;
@@ -525,6 +557,7 @@ int choose_kernels_unpinned(fw_sim* s) {
 }
 
 bool codes_wanted(const fw_sim* s) {
+    if (s->fixed_base) return false;  // coding needs temporary tables of up to 100 MB
     if (s->codes_mode >= 0) return s->codes_mode == 1;
     const char* env = getenv("FULLWAVE_HIP_CODES");
     return !(env && env[0] == '0');
@@ -969,7 +1002,12 @@ int fw_device_memory(int device, size_t* free_bytes, size_t* total_bytes) {
     return FW_OK;
 }
 
-int fw_sim_create(const fw_desc* desc, int device, fw_sim** out) {
+}  // extern "C"
+
+namespace {
+
+// fw_sim_create; with `fixed` not null, a fixed-region handle on [fixed, fixed + fixed_bytes)
+int create_impl(const fw_desc* desc, int device, void* fixed, size_t fixed_bytes, fw_sim** out) {
     if (!desc || !out) return fail(FW_ERR_INVALID, "null argument");
     *out = nullptr;
     const fw_desc& d = *desc;
@@ -1013,10 +1051,16 @@ int fw_sim_create(const fw_desc* desc, int device, fw_sim** out) {
         fw_sim_destroy(s);
         return rc;
     };
-    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
-        return cleanup(fail(FW_ERR_DEVICE, "hipStreamCreate failed"));
-    if (hipEventCreate(&s->ev[0]) != hipSuccess || hipEventCreate(&s->ev[1]) != hipSuccess)
-        return cleanup(fail(FW_ERR_DEVICE, "hipEventCreate failed"));
+    if (fixed) {
+        s->fixed_base = (char*)fixed;
+        s->fixed_size = fixed_bytes;
+        s->stream = hipStreamPerThread;
+    } else {
+        if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
+            return cleanup(fail(FW_ERR_DEVICE, "hipStreamCreate failed"));
+        if (hipEventCreate(&s->ev[0]) != hipSuccess || hipEventCreate(&s->ev[1]) != hipSuccess)
+            return cleanup(fail(FW_ERR_DEVICE, "hipEventCreate failed"));
+    }
     const int nd = d.ndim;
     const bool relax = d.attenuation_model == 0;  // the exponential model has no memory variables
     const int64_t field = s->ncell * nshots;  // one wave field of every shot
@@ -1053,6 +1097,14 @@ int fw_sim_create(const fw_desc* desc, int device, fw_sim** out) {
     s->ds.dt = (float)d.dt;
     *out = s;
     return FW_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fw_sim_create(const fw_desc* desc, int device, fw_sim** out) {
+    return create_impl(desc, device, nullptr, 0, out);
 }
 
 int fw_sim_upload(fw_sim* s, const char* name_c, const void* host, size_t bytes) {
@@ -1291,7 +1343,7 @@ int fw_sim_destroy(fw_sim* s) {
     if (s->stream) (void)hipStreamSynchronize(s->stream);
     for (auto& e : s->ev)
         if (e) (void)hipEventDestroy(e);
-    if (s->stream) (void)hipStreamDestroy(s->stream);
+    if (s->stream && !s->fixed_base) (void)hipStreamDestroy(s->stream);  // not the per-thread stream
     delete s;
     return FW_OK;
 }
@@ -1312,7 +1364,11 @@ static bool file_exists(const std::string& path) {
     return (bool)f;
 }
 
-int fw_run_dir(const char* sim_dir, int device) {
+}  // extern "C"
+
+namespace fw {
+
+int read_dir_desc(const char* sim_dir, fw_desc* out) {
     if (!sim_dir) return fail(FW_ERR_INVALID, "null directory");
     const std::string dir = std::string(sim_dir) + "/";
     std::vector<char> buf;
@@ -1348,8 +1404,46 @@ int fw_run_dir(const char* sim_dir, int device) {
     }
     d.nz = 1;
     if (d.ndim == 3 && ((rc = scalar_i("nZ", &d.nz)) || (rc = scalar_f("dZ", &d.dz)))) return rc;
+    *out = d;
+    return FW_OK;
+}
+
+size_t fixed_footprint(const fw_desc& d) {
+    // in floating point first: the sizes come from files and may be absurd
+    const int nd = d.ndim == 3 ? 3 : 2;
+    const double nz = nd == 3 ? (double)d.nz : 1.0;
+    const double pitch = nd == 3 ? std::ceil(nz / 32.0) * 32.0 : (double)d.ny;
+    const double rows = nd == 3 ? (double)d.nx * (double)d.ny : (double)d.nx;
+    const double ncell = rows * pitch;
+    const double nshots = d.n_shots < 1 ? 1.0 : (double)d.n_shots;
+    const bool relax = d.attenuation_model == 0;
+    double nmaps = 0;
+    for (const std::string& n : required_maps(d))
+        if (n != "dmap" && n != "icmat" && n != "icc" && n != "outc" && n != "icczero") nmaps += 1;
+    const double rec = d.modt > 0 ? std::ceil((double)d.nt / (double)d.modt) : 0.0;
+    const double icmat = nshots * (double)d.ncoords * (double)d.ntic * 4;
+    double b = (1 + nd + (relax ? 4 * nd : 0)) * ncell * nshots * 4  // wave state
+               + nmaps * ncell * 4                                   // input maps
+               + icmat                                               // source matrix
+               + (double)d.ncoords * 8 + (double)d.ncoordszero * 8 + (double)d.ncoordsout * 13  // index lists
+               + (double)d.ndmap * nd * kNCoef * 4                   // coefficient table
+               + nshots * rec * (double)d.ncoordsout * 4;            // records
+    // largest temporary: the source matrix before its transpose, or a row-staging buffer
+    const double stage = pitch != (nd == 3 ? nz : (double)d.ny) ? (double)kFixedStageBytes : 0.0;
+    b += std::max(icmat, stage);
+    b += (nmaps + 16) * 256;  // alignment of every array
+    if (!(b < 1e18)) return (size_t)-1;
+    return (size_t)b;
+}
+
+int run_dir_impl(const char* sim_dir, int device, void* fixed, size_t fixed_bytes) {
+    fw_desc d{};
+    int rc;
+    if ((rc = read_dir_desc(sim_dir, &d)) != FW_OK) return rc;
+    const std::string dir = std::string(sim_dir) + "/";
+    std::vector<char> buf;
     fw_sim* s = nullptr;
-    if ((rc = fw_sim_create(&d, device, &s)) != FW_OK) return rc;
+    if ((rc = create_impl(&d, device, fixed, fixed_bytes, &s)) != FW_OK) return rc;
     std::vector<std::string> names = required_maps(d);
     for (const auto& name : names) {
         if (name == "icczero" && d.ncoordszero == 0) continue;
@@ -1383,5 +1477,11 @@ int fw_run_dir(const char* sim_dir, int device) {
     if (!out) return fail(FW_ERR_IO, "short write to %sgenout.dat", dir.c_str());
     return FW_OK;
 }
+
+}  // namespace fw
+
+extern "C" {
+
+int fw_run_dir(const char* sim_dir, int device) { return fw::run_dir_impl(sim_dir, device, nullptr, 0); }
 
 }  // extern "C"

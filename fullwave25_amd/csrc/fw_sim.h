@@ -55,6 +55,12 @@ struct fw_sim {
     int codes_mode = -1;
     int32_t ncodes = 0;
     int64_t coded_cells = 0;
+    // fixed-region handle (fw_run_dir_noalloc): every device array is carved out of
+    // [fixed_base, fixed_base + fixed_size) in order and nothing is freed; no device allocation,
+    // pinned host buffer, stream or event is created (the handle runs on the per-thread stream)
+    char* fixed_base = nullptr;
+    size_t fixed_size = 0;
+    mutable size_t fixed_used = 0;
 };
 
 
@@ -64,4 +70,12 @@ int set_error(int code, const char* fmt, ...);
 int prepare(fw_sim* s);                    // check inputs, resolve coordinates, pick kernels
 int forward_step(fw_sim* s);               // one time step on the handle's stream
 hipError_t grid_to_host(const fw_sim* s, void* dst, const void* src, size_t esize);  // padded -> compact
+// directory contract (fw_run_dir): the scalar files of sim_dir as a descriptor
+int read_dir_desc(const char* sim_dir, fw_desc* d);
+// device bytes a fixed-region handle of `d` needs (input maps, wave state, records, source
+// matrix, index lists, coefficient table and the largest transient staging buffer)
+size_t fixed_footprint(const fw_desc& d);
+// fw_run_dir with every device array of the run in [fixed, fixed + fixed_bytes) when fixed is
+// not null (the caller has checked fixed_footprint); else on ordinary device allocations
+int run_dir_impl(const char* sim_dir, int device, void* fixed, size_t fixed_bytes);
 }  // namespace fw

@@ -28,7 +28,7 @@ KERNEL_KINDS = ("velocity", "pressure", "boundary")
 
 # every symbol include/fullwave_hip.h declares
 EXPORTED_SYMBOLS = (
-    "fw_abi_version", "fw_version_string", "fw_device_count", "fw_last_error", "fw_run_dir",
+    "fw_abi_version", "fw_version_string", "fw_device_count", "fw_last_error", "fw_run_dir", "fw_run_dir_noalloc",
     "fw_device_copy_bandwidth", "fw_device_memory",
     "fw_sim_create", "fw_sim_upload", "fw_sim_run", "fw_sim_reset", "fw_sim_step_count",
     "fw_sim_record_count", "fw_sim_download_sensors", "fw_sim_download_field",
@@ -83,6 +83,7 @@ def load_library(path: Path | None = None) -> ctypes.CDLL:
         lib.fw_last_error.restype = ctypes.c_char_p
         lib.fw_device_count.argtypes = [ctypes.POINTER(ctypes.c_int)]
         lib.fw_run_dir.argtypes = [ctypes.c_char_p, ctypes.c_int]
+        lib.fw_run_dir_noalloc.argtypes = [ctypes.c_char_p, ctypes.c_int]
         lib.fw_device_copy_bandwidth.argtypes = [ctypes.c_int, sz, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
         lib.fw_device_memory.argtypes = [ctypes.c_int, ctypes.POINTER(sz), ctypes.POINTER(sz)]
         lib.fw_sim_create.argtypes = [ctypes.POINTER(FwDesc), ctypes.c_int, ctypes.POINTER(vp)]
@@ -162,6 +163,18 @@ def run_dir(sim_dir: Path, device: int = 0) -> None:
     """Run the engine on a simulation directory; writes ``genout.dat`` there."""
     lib = load_library()
     _check(lib, lib.fw_run_dir(str(Path(sim_dir).absolute()).encode(), int(device)))
+
+
+# device bytes fw_run_dir_noalloc works in (FW_NOALLOC_SCRATCH_BYTES)
+NOALLOC_SCRATCH_BYTES = 256 << 20
+
+
+def run_dir_noalloc(sim_dir: Path, device: int = 0) -> None:
+    """:func:`run_dir` without device allocation: every device array of the run sits in the
+    library's static per-GPU array of :data:`NOALLOC_SCRATCH_BYTES` (runs that need more fail
+    with FW_ERR_UNSUPPORTED); kernel shapes are the defaults or FULLWAVE_HIP_KERNELS's."""
+    lib = load_library()
+    _check(lib, lib.fw_run_dir_noalloc(str(Path(sim_dir).absolute()).encode(), int(device)))
 
 
 def make_desc(inp: SimulationInputs, n_shots: int = 1) -> FwDesc:

@@ -92,6 +92,24 @@ int fw_device_memory(int device, size_t* free_bytes, size_t* total_bytes);
  * fullwave/solver/launcher.py:196-215. */
 int fw_run_dir(const char* sim_dir, int device);
 
+/* Device bytes that fw_run_dir_noalloc works in: one static device array per GPU, held from the
+ * first such call on that GPU until the process ends. */
+#define FW_NOALLOC_SCRATCH_BYTES (256ull << 20)
+/* fw_run_dir without any device allocation. It has no counterpart in the reference; it is for
+ * callers that cannot allocate device memory (a serving loop with a fixed device footprint, code
+ * that must not call hipMalloc or hipFree). Same files, same genout.dat, same kernels as
+ * fw_run_dir, with two differences: the kernel shapes are FULLWAVE_HIP_KERNELS's or the defaults
+ * (not timed), and the relaxation coefficients are read from the per-cell maps (no coding; the
+ * results are the same). Every device array of the run (input maps, wave fields, memory
+ * variables, records, index lists, coefficient table, staging) is placed in the static array;
+ * nothing is allocated on the device, no pinned host memory, stream or event is created, and the
+ * run uses the calling thread's per-thread stream. Host memory for the files is allocated as
+ * usual. A run that needs more than FW_NOALLOC_SCRATCH_BYTES (2D isotropic: about 2.5 M cells)
+ * fails with FW_ERR_UNSUPPORTED before any device work. Each call clears the wave state and the
+ * records it uses; calls on one GPU share the array, so this process serializes them (a second
+ * call waits for the first), while another process has its own array. */
+int fw_run_dir_noalloc(const char* sim_dir, int device);
+
 /* ---- in-memory handle ------------------------------------------------------------------ */
 int fw_sim_create(const fw_desc* desc, int device, fw_sim** out);
 /* Upload one input by its file stem: "K", "rho", "c", "beta", "dcmap" (int32 per voxel),

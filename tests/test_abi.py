@@ -89,3 +89,23 @@ def test_group_arguments_rejected_before_device():
 def test_missing_library_fails_loudly(tmp_path):
     with pytest.raises(RuntimeError, match="missing"):
         engine.load_library(tmp_path / "libfullwave_hip.so")
+
+
+def test_noalloc_rejects_oversized_run_before_device(tmp_path):
+    """fw_run_dir_noalloc checks the run's device footprint against its fixed array before any
+    device call or map read: a directory of scalar files alone is enough to be refused."""
+    import numpy as np
+
+    ints = {"nX": 4000, "nY": 4000, "nT": 100, "ncoords": 1, "ncoordsout": 1, "nTic": 100, "modT": 1, "ndmap": 1}
+    floats = {"dX": 1e-4, "dY": 1e-4, "dT": 1e-8, "c0": 1540.0}
+    for stem, v in ints.items():
+        np.array([v], np.int32).tofile(tmp_path / f"{stem}.dat")
+    for stem, v in floats.items():
+        np.array([v], np.float32).tofile(tmp_path / f"{stem}.dat")
+    lib = engine.load_library()
+    assert lib.fw_run_dir_noalloc(str(tmp_path).encode(), 0) == 6  # FW_ERR_UNSUPPORTED
+    msg = lib.fw_last_error()
+    assert b"allocation-free" in msg and str(engine.NOALLOC_SCRATCH_BYTES).encode() in msg, msg
+    assert lib.fw_run_dir_noalloc(None, 0) == 1
+    (tmp_path / "nX.dat").unlink()
+    assert lib.fw_run_dir_noalloc(str(tmp_path).encode(), 0) == 4  # FW_ERR_IO
