@@ -350,13 +350,13 @@ int alloc(fw_sim* s, const std::string& name, size_t bytes, void** out) {
     return FW_OK;
 }
 
-int check_coords(const fw_sim* s, const std::vector<int32_t>& c, int64_t n, const char* what,
-                 std::vector<int64_t>& lin) {
-    const fw_desc& d = s->d;
-    const int nd = s->ndim;
+int check_coords(const fw_desc& d, int64_t sx, int64_t sy, const std::vector<int32_t>& c, int64_t n,
+                 const char* what, std::vector<int64_t>& lin) {
+    const int nd = d.ndim;
+    if ((int64_t)c.size() < n * nd) return fail(FW_ERR_INVALID, "%s list holds fewer than %lld points", what, (long long)n);
     lin.resize((size_t)n);
     const int64_t dims[3] = {d.nx, d.ny, nd == 3 ? d.nz : 1};
-    const int64_t stride[3] = {s->ds.sx, s->ds.sy, 1};
+    const int64_t stride[3] = {sx, nd == 3 ? sy : 1, 1};
     for (int64_t q = 0; q < n; ++q) {
         int64_t idx = 0;
         for (int a = 0; a < nd; ++a) {
@@ -364,7 +364,7 @@ int check_coords(const fw_sim* s, const std::vector<int32_t>& c, int64_t n, cons
             if (x < 0 || x >= dims[a])
                 return fail(FW_ERR_INVALID, "%s coordinate %lld (axis %d) = %lld outside [0, %lld)", what,
                             (long long)q, a, (long long)x, (long long)dims[a]);
-            idx += x * (nd == 2 && a == 1 ? 1 : stride[a]);
+            idx += x * stride[a];
         }
         lin[(size_t)q] = idx;
     }
@@ -706,34 +706,14 @@ int finalize(fw_sim* s) {
     for (const auto& name : required_maps(d))
         if (!s->have.count(name)) return fail(FW_ERR_STATE, "input '%s' has not been uploaded", name.c_str());
 
-    std::vector<int64_t> src, zero, out;
     int rc;
-    if ((rc = check_coords(s, s->icc, d.ncoords, "source", src)) != FW_OK) return rc;
-    if ((rc = check_coords(s, s->outc, d.ncoordsout, "sensor", out)) != FW_OK) return rc;
-    if ((rc = check_coords(s, s->zero, d.ncoordszero, "pressure-release", zero)) != FW_OK) return rc;
-
-    // pressure release wins over injection; a repeated source point keeps its last signal
-    std::unordered_map<int64_t, int32_t> air, last_src;
-    for (int64_t q = 0; q < d.ncoordszero; ++q) air[zero[(size_t)q]] = 1;
-    for (int64_t q = 0; q < d.ncoords; ++q) last_src[src[(size_t)q]] = (int32_t)q;
-    std::vector<int64_t> src_lin((size_t)d.ncoords);
-    for (int64_t q = 0; q < d.ncoords; ++q) {
-        const int64_t l = src[(size_t)q];
-        src_lin[(size_t)q] = (air.count(l) || last_src[l] != q) ? -1 : l;
-    }
-    std::vector<int32_t> out_src((size_t)d.ncoordsout);
-    std::vector<int8_t> out_air((size_t)d.ncoordsout);
-    for (int64_t q = 0; q < d.ncoordsout; ++q) {
-        const int64_t l = out[(size_t)q];
-        out_air[(size_t)q] = air.count(l) ? 1 : 0;
-        auto it = last_src.find(l);
-        out_src[(size_t)q] = it == last_src.end() ? -1 : it->second;
-    }
-
-    // deduplicate zero list (same address written twice with the same value is harmless,
-    // but keep one writer per address)
-    std::sort(zero.begin(), zero.end());
-    zero.erase(std::unique(zero.begin(), zero.end()), zero.end());
+    fw::BoundaryLists bl;
+    if ((rc = fw::resolve_boundary(d, s->ds.sx, s->ds.sy, s->icc, s->outc, s->zero, &bl)) != FW_OK) return rc;
+    const std::vector<int64_t>& src_lin = bl.src_lin;
+    const std::vector<int64_t>& zero = bl.zero_lin;
+    const std::vector<int64_t>& out = bl.out_lin;
+    const std::vector<int32_t>& out_src = bl.out_src;
+    const std::vector<int8_t>& out_air = bl.out_air;
 
     void* p;
     auto up = [&](const char* name, const void* h, size_t bytes) -> int {
@@ -758,26 +738,11 @@ int finalize(fw_sim* s) {
     s->db.nout = d.ncoordsout;
     s->db.ntic = d.ntic;
 
-    // coefficient table [ndmap][ndim][9] from dmap [9][2][ndmap]; spacing folded in
-    const int nd = s->ndim;
-    const double h[3] = {d.dx, d.dy, d.dz};
-    std::vector<float> coef((size_t)d.ndmap * nd * fw::kNCoef);
-    for (int64_t m = 0; m < d.ndmap; ++m) {
-        for (int a = 0; a < nd; ++a) {
-            float* row = &coef[(size_t)((m * nd + a) * fw::kNCoef)];
-            for (int k = 1; k <= 8; ++k)
-                row[k - 1] = (float)((double)s->dmap[(size_t)((k * 2 + 0) * d.ndmap + m)] / h[a]);
-            row[8] = (float)((double)s->dmap[(size_t)((1 * 2 + 1) * d.ndmap + m)] / h[a]);
-        }
-    }
+    int axes_same = 1;
+    const std::vector<float> coef = fw::coefficient_table(d, s->dmap, &axes_same);
     if ((rc = up("_coef", coef.data(), coef.size() * 4)) != FW_OK) return rc;
     s->ds.coef = (const float*)s->dev["_coef"];
-    s->ds.coef_axes_same = 1;
-    for (int64_t m = 0; m < d.ndmap && s->ds.coef_axes_same; ++m)
-        for (int a = 1; a < nd; ++a)
-            if (memcmp(&coef[(size_t)(m * nd * fw::kNCoef)], &coef[(size_t)((m * nd + a) * fw::kNCoef)],
-                       fw::kNCoef * sizeof(float)) != 0)
-                s->ds.coef_axes_same = 0;
+    s->ds.coef_axes_same = axes_same;
 
     // dcmap range check on the host would need a download; the kernels trust it, so check it
     // here once against ndmap
@@ -796,6 +761,7 @@ int finalize(fw_sim* s) {
     FW_HIP(fill_zero_sync(s, p, rec_bytes));
 
     // map pointers
+    const int nd = s->ndim;
     const int nax = d.isotropic ? 1 : nd;
     s->ds.aexp = d.attenuation_model == 1 ? (const float*)s->dev["a_exp"] : nullptr;
     for (int a = 0; a < nd && d.attenuation_model == 0; ++a) {
@@ -918,6 +884,59 @@ int run_impl(fw_sim* s, int64_t n_steps, bool timed, double* ms) {
 }  // namespace
 
 namespace fw {
+
+int resolve_boundary(const fw_desc& d, int64_t sx, int64_t sy, const std::vector<int32_t>& icc,
+                     const std::vector<int32_t>& outc, const std::vector<int32_t>& zero_c, BoundaryLists* out) {
+    std::vector<int64_t> src, zero, sen;
+    int rc;
+    if ((rc = check_coords(d, sx, sy, icc, d.ncoords, "source", src)) != FW_OK) return rc;
+    if ((rc = check_coords(d, sx, sy, outc, d.ncoordsout, "sensor", sen)) != FW_OK) return rc;
+    if ((rc = check_coords(d, sx, sy, zero_c, d.ncoordszero, "pressure-release", zero)) != FW_OK) return rc;
+    // pressure release wins over injection; a repeated source point keeps its last signal
+    std::unordered_map<int64_t, int32_t> air, last_src;
+    for (int64_t q = 0; q < d.ncoordszero; ++q) air[zero[(size_t)q]] = 1;
+    for (int64_t q = 0; q < d.ncoords; ++q) last_src[src[(size_t)q]] = (int32_t)q;
+    out->src_lin.assign((size_t)d.ncoords, -1);
+    for (int64_t q = 0; q < d.ncoords; ++q) {
+        const int64_t l = src[(size_t)q];
+        out->src_lin[(size_t)q] = (air.count(l) || last_src[l] != q) ? -1 : l;
+    }
+    out->out_src.assign((size_t)d.ncoordsout, -1);
+    out->out_air.assign((size_t)d.ncoordsout, 0);
+    for (int64_t q = 0; q < d.ncoordsout; ++q) {
+        const int64_t l = sen[(size_t)q];
+        out->out_air[(size_t)q] = air.count(l) ? 1 : 0;
+        auto it = last_src.find(l);
+        out->out_src[(size_t)q] = it == last_src.end() ? -1 : it->second;
+    }
+    // one writer per pressure-release address
+    std::sort(zero.begin(), zero.end());
+    zero.erase(std::unique(zero.begin(), zero.end()), zero.end());
+    out->zero_lin = std::move(zero);
+    out->out_lin = std::move(sen);
+    return FW_OK;
+}
+
+std::vector<float> coefficient_table(const fw_desc& d, const std::vector<float>& dmap, int* axes_same) {
+    const int nd = d.ndim == 3 ? 3 : 2;
+    const double h[3] = {d.dx, d.dy, d.dz};
+    std::vector<float> coef((size_t)d.ndmap * nd * kNCoef);
+    if (dmap.size() < (size_t)(9 * 2 * d.ndmap)) return {};
+    for (int64_t m = 0; m < d.ndmap; ++m) {
+        for (int a = 0; a < nd; ++a) {
+            float* row = &coef[(size_t)((m * nd + a) * kNCoef)];
+            for (int k = 1; k <= 8; ++k) row[k - 1] = (float)((double)dmap[(size_t)((k * 2 + 0) * d.ndmap + m)] / h[a]);
+            row[8] = (float)((double)dmap[(size_t)((1 * 2 + 1) * d.ndmap + m)] / h[a]);
+        }
+    }
+    int same = 1;
+    for (int64_t m = 0; m < d.ndmap && same; ++m)
+        for (int a = 1; a < nd; ++a)
+            if (memcmp(&coef[(size_t)(m * nd * kNCoef)], &coef[(size_t)((m * nd + a) * kNCoef)], kNCoef * sizeof(float)) != 0)
+                same = 0;
+    if (axes_same) *axes_same = same;
+    return coef;
+}
 
 hipError_t grid_to_host(const fw_sim* s, void* dst, const void* src, size_t esize) {
     return copy_grid_out(s, dst, src, esize);

@@ -70,6 +70,24 @@ int set_error(int code, const char* fmt, ...);
 int prepare(fw_sim* s);                    // check inputs, resolve coordinates, pick kernels
 int forward_step(fw_sim* s);               // one time step on the handle's stream
 hipError_t grid_to_host(const fw_sim* s, void* dst, const void* src, size_t esize);  // padded -> compact
+// Host-side set-up of a handle, without device access.
+// Source, pressure-release and sensor points as one time step applies them.
+struct BoundaryLists {
+    std::vector<int64_t> src_lin;   // per source point: device index, or -1 (repeated later, or air)
+    std::vector<int64_t> zero_lin;  // distinct pressure-release device indices, ascending
+    std::vector<int64_t> out_lin;   // per sensor: device index
+    std::vector<int32_t> out_src;   // per sensor: the source point last written there, or -1
+    std::vector<int8_t> out_air;    // per sensor: 1 on a pressure-release point
+};
+// Coordinate lists (int32 [n, ndim]) of the grid of `d` to device indices (x stride sx, y stride
+// sy, z stride 1; in 2D the y stride is 1 and sy is ignored); out-of-grid coordinates fail with
+// FW_ERR_INVALID. Air wins over a source at the same point; a repeated source point keeps its last
+// row; a sensor on a source or air point reads the written value.
+int resolve_boundary(const fw_desc& d, int64_t sx, int64_t sy, const std::vector<int32_t>& icc,
+                     const std::vector<int32_t>& outc, const std::vector<int32_t>& zero, BoundaryLists* out);
+// Coefficient table [ndmap][ndim][9] (w_1..w_8, e) from dmap [9][2][ndmap] with each axis's spacing
+// folded in; *axes_same = 1 when every axis has the same set for each sound speed.
+std::vector<float> coefficient_table(const fw_desc& d, const std::vector<float>& dmap, int* axes_same);
 // directory contract (fw_run_dir): the scalar files of sim_dir as a descriptor
 int read_dir_desc(const char* sim_dir, fw_desc* d);
 // device bytes a fixed-region handle of `d` needs (input maps, wave state, records, source
