@@ -250,6 +250,8 @@ def bench_grouped(args) -> dict:
         with engine.HipGroup([sims[i] for i in live], width=width, chunk=chunk, variant=variant) as grp:
             shape = grp.kernel_config()["velocity"]
             variant, width, chunk = shape["variant"], shape["width"], shape["chunk"]
+            if args.graph_steps:
+                grp.set_graph_steps(args.graph_steps)
             grp.run(end - done)
         done = end
         for i in [i for i in live if jobs[i].nt == end]:
@@ -270,8 +272,9 @@ def bench_grouped(args) -> dict:
         "config": {"workload": f"{len(jobs)} focused-transmit jobs on "
                                + ", ".join(f"{a}x{b}" for a, b in sizes)
                                + f" interiors, {3 * unit}-{6 * unit} steps; group kernel "
-                               + (f"tile:{chunk}" if variant == "tile" else f"stream:{width}:{chunk}"),
-                   "mix": args.mix, "jobs": len(jobs)},
+                               + (f"tile:{chunk}" if variant == "tile" else f"stream:{width}:{chunk}")
+                               + (f", steps replayed from {args.graph_steps}-step graphs" if args.graph_steps else ""),
+                   "mix": args.mix, "jobs": len(jobs), "graph_steps": args.graph_steps},
     }
 
 
@@ -625,6 +628,7 @@ def bench_single(args) -> dict:
             "avg_launch_us": round(avg_s * 1e6, 2),
             "kernel_ms_per_step": {k: round(v / args.steps, 5) for k, v in kms.items()},
             "copy_kernel_gbs": round(copy_gbs, 1),
+            "frac_of_copy": round(achieved / copy_gbs, 4),
             "kernel_config": kcfg,
             "kernels": engine.kernel_override_string(kcfg),
             "kernel_source": source,
@@ -671,6 +675,8 @@ def main() -> int:
     ap.add_argument("--group-chunk", type=int, default=0, help="grouped_2d: rows per workgroup (0: timed)")
     ap.add_argument("--group-variant", choices=("auto", "stream", "tile"), default="auto",
                     help="grouped_2d: kernel family of the group (auto: timed)")
+    ap.add_argument("--graph-steps", type=int, default=0,
+                    help="grouped_2d: replay the group's steps from a captured graph of this many steps")
     ap.add_argument("--slots", type=int, default=4, help="served_mix: concurrent jobs")
     ap.add_argument("--mix", choices=("standard", "small"), default="standard",
                     help="served_mix, grouped_2d: job sizes (small: none fills the GPU alone)")

@@ -28,6 +28,11 @@ struct fw_group {
     int width = 256, chunk = 16, max_ndmap = 1;
     int coef_sets = 1;      // LDS coefficient sets of the tiled kernels (largest member)
     bool all_coded = true;  // every member reads coded relaxation coefficients
+    // steady steps replayed from a captured graph of graph_steps steps (0: plain launches); the
+    // boundary pass adds the device counter base_dev, which each replay advances
+    int64_t graph_steps = 0;
+    int64_t* base_dev = nullptr;
+    hipGraphExec_t graph = nullptr;
 };
 
 namespace {
@@ -70,9 +75,11 @@ hipError_t alloc_table(fw_group* g, void** p, size_t bytes) {
 
 void release(fw_group* g) {
     if (!g) return;
+    if (g->graph) (void)hipGraphExecDestroy(g->graph);
     if (g->stream) {
         free_table(g, g->tiles);
         free_table(g, g->bounds);
+        free_table(g, g->base_dev);
         (void)hipStreamSynchronize(g->stream);
         (void)hipStreamDestroy(g->stream);
     }
@@ -231,9 +238,49 @@ int choose_shape(fw_group* g, bool tune_variant, bool tune_width, bool tune_chun
     return build(g);
 }
 
+// Capture graph_steps group steps (velocity, pressure and boundary launches, the boundary pass
+// reading the device counter) and one counter advance into a graph, once per group.
+int capture_graph(fw_group* g) {
+    if (!g->base_dev) GR_HIP(alloc_table(g, (void**)&g->base_dev, sizeof(int64_t)));
+    hipGraph_t graph = nullptr;
+    GR_HIP(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal));
+    hipError_t e = hipSuccess;
+    for (int64_t j = 0; j < g->graph_steps && e == hipSuccess; ++j) {
+        e = launch_stencil(g, true);
+        if (e == hipSuccess) e = launch_stencil(g, false);
+        if (e == hipSuccess) e = fw::launch_group_boundary(g->bounds, g->ntiles, g->max_points, g->base_dev, j, g->stream);
+    }
+    if (e == hipSuccess) e = fw::launch_advance(g->base_dev, g->graph_steps, g->stream);
+    const hipError_t e2 = hipStreamEndCapture(g->stream, &graph);
+    if (e != hipSuccess || e2 != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return fw::set_error(FW_ERR_DEVICE, "group graph capture failed: %s", hipGetErrorString(e != hipSuccess ? e : e2));
+    }
+    e = hipGraphInstantiate(&g->graph, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+        g->graph = nullptr;
+        return fw::set_error(FW_ERR_DEVICE, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
+    }
+    return FW_OK;
+}
+
 }  // namespace
 
 extern "C" {
+
+int fw_group_set_graph_steps(fw_group* g, int64_t steps) {
+    if (!g) return fw::set_error(FW_ERR_INVALID, "null group");
+    if (steps < 0) return fw::set_error(FW_ERR_INVALID, "graph steps must be >= 0");
+    if (steps != g->graph_steps && g->graph) {
+        Guard guard(g->device);
+        (void)hipStreamSynchronize(g->stream);
+        (void)hipGraphExecDestroy(g->graph);
+        g->graph = nullptr;
+    }
+    g->graph_steps = steps;
+    return FW_OK;
+}
 
 int fw_group_create_shaped(fw_sim* const* sims, int32_t n, int32_t variant, int32_t width, int32_t chunk,
                            fw_group** out) {
@@ -332,10 +379,19 @@ int fw_group_run(fw_group* g, int64_t n_steps) {
     for (fw_sim* s : g->sims) GR_HIP(hipStreamSynchronize(s->stream));  // members idle
     GR_HIP(hipMemcpyAsync(g->bounds, g->bounds_h.data(), g->bounds_h.size() * sizeof(fw::GroupBoundary),
                           hipMemcpyHostToDevice, g->stream));
-    for (int64_t it = 0; it < n_steps; ++it) {
+    int64_t done = 0;
+    if (g->graph_steps > 0 && n_steps >= g->graph_steps) {
+        if (!g->graph) {
+            int rc = capture_graph(g);
+            if (rc != FW_OK) return rc;
+        }
+        GR_HIP(fw::launch_tick(g->base_dev, 0, g->stream));  // replays count from this call's start
+        for (; done + g->graph_steps <= n_steps; done += g->graph_steps) GR_HIP(hipGraphLaunch(g->graph, g->stream));
+    }
+    for (int64_t it = done; it < n_steps; ++it) {
         GR_HIP(launch_stencil(g, true));
         GR_HIP(launch_stencil(g, false));
-        GR_HIP(fw::launch_group_boundary(g->bounds, g->ntiles, g->max_points, it, g->stream));
+        GR_HIP(fw::launch_group_boundary(g->bounds, g->ntiles, g->max_points, nullptr, it, g->stream));
     }
     GR_HIP(hipStreamSynchronize(g->stream));
     for (fw_sim* s : g->sims) {

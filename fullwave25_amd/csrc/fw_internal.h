@@ -243,6 +243,7 @@ hipError_t launch_boundary(const DevSim& s, const DevBoundary& b, int64_t step, 
 hipError_t launch_boundary_dev(const DevSim& s, const DevBoundary& b, const int64_t* step, int64_t modt,
                                hipStream_t st);
 hipError_t launch_tick(int64_t* step, int64_t set, hipStream_t st);
+hipError_t launch_advance(int64_t* counter, int64_t k, hipStream_t st);
 // grouped 2D x-marching kernels (isotropic relaxation); `width` 64/128/256 lanes, `chunk` rows
 unsigned group_tiles_x(int64_t ny, int width);
 unsigned group_tiles_y(int64_t nx, int chunk);
@@ -255,7 +256,9 @@ hipError_t launch_group_2d(const GroupTile* t, int n, uint32_t blocks, int width
 unsigned group_tile_rows(int64_t nx, int rows);
 hipError_t launch_group_tile2d(const GroupTile* t, int n, uint32_t blocks, int rows, int coef_sets, bool all_coded,
                                bool velocity, hipStream_t st);
-hipError_t launch_group_boundary(const GroupBoundary* g, int n, int64_t max_points, int64_t it, hipStream_t st);
+// step of entry e: e.step0 + it (+ *base when base is not null)
+hipError_t launch_group_boundary(const GroupBoundary* g, int n, int64_t max_points, const int64_t* base, int64_t it,
+                                 hipStream_t st);
 hipError_t launch_reciprocal(float* data, int64_t n, hipStream_t st);
 // [nshots][ncoords][ntic] -> [nshots][ntic][ncoords]
 hipError_t launch_transpose_icmat(const float* in, float* out, int64_t nshots, int64_t ncoords, int64_t ntic,

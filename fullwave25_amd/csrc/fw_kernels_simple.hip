@@ -160,10 +160,12 @@ __global__ __launch_bounds__(256) void k_boundary_dev(DevSim s, DevBoundary b, c
 
 // The same pass for every grid of a group, at iteration `it` of a run: grid y of the launch is
 // member y (or one shot of it), at step step0 + it; its pointers are already offset to it.
-__global__ __launch_bounds__(256) void k_boundary_group(const GroupBoundary* __restrict__ g, int64_t it) {
+// `base` (may be null): a device counter added to `it` (graph replay advances it between replays)
+__global__ __launch_bounds__(256) void k_boundary_group(const GroupBoundary* __restrict__ g, const int64_t* base,
+                                                        int64_t it) {
     const GroupBoundary& e = g[blockIdx.y];
     const DevBoundary& b = e.b;
-    const int64_t n = e.step0 + it;
+    const int64_t n = e.step0 + it + (base ? *base : 0);
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     float* p = e.p;
     const int64_t rec_row = (e.modt > 0 && n % e.modt == 0) ? n / e.modt : -1;
@@ -190,9 +192,16 @@ __global__ void k_tick(int64_t* step, int64_t set) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *step = set >= 0 ? set : *step + 1;
 }
 
+// Add k to a device counter; one lane writes it.
+__global__ void k_advance(int64_t* counter, int64_t k) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *counter += k;
+}
+
+// One 16-byte element per lane: the fastest plain copy measured on the chip (a grid-stride loop
+// over the same buffers reached 78 % of its rate).
 __global__ __launch_bounds__(256) void k_copy(const float4* __restrict__ src, float4* __restrict__ dst, int64_t n) {
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += stride) dst[t] = src[t];
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) dst[t] = src[t];
 }
 
 __global__ __launch_bounds__(256) void k_reciprocal(float* data, int64_t n) {
@@ -276,10 +285,11 @@ hipError_t launch_boundary_dev(const DevSim& s, const DevBoundary& b, const int6
     return hipGetLastError();
 }
 
-hipError_t launch_group_boundary(const GroupBoundary* g, int n, int64_t max_points, int64_t it, hipStream_t st) {
+hipError_t launch_group_boundary(const GroupBoundary* g, int n, int64_t max_points, const int64_t* base, int64_t it,
+                                 hipStream_t st) {
     if (n <= 0 || max_points <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_boundary_group, dim3((unsigned)((max_points + 255) / 256), (unsigned)n), dim3(256), 0, st, g,
-                       it);
+                       base, it);
     return hipGetLastError();
 }
 
@@ -288,10 +298,15 @@ hipError_t launch_tick(int64_t* step, int64_t set, hipStream_t st) {
     return hipGetLastError();
 }
 
+hipError_t launch_advance(int64_t* counter, int64_t k, hipStream_t st) {
+    hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, st, counter, k);
+    return hipGetLastError();
+}
+
 hipError_t launch_copy(const void* src, void* dst, int64_t n16, hipStream_t st) {
     if (n16 == 0) return hipSuccess;
-    int64_t blocks = (n16 + 255) / 256;
-    if (blocks > 256 * 64) blocks = 256 * 64;
+    const int64_t blocks = (n16 + 255) / 256;
+    if (blocks > 0x7fffffff) return hipErrorInvalidValue;  // 2^39 bytes and more: not a probe size
     hipLaunchKernelGGL(k_copy, dim3((unsigned)blocks), dim3(256), 0, st, (const float4*)src, (float4*)dst, n16);
     return hipGetLastError();
 }

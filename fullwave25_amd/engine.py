@@ -35,7 +35,7 @@ EXPORTED_SYMBOLS = (
     "fw_sim_set_field", "fw_sim_kernel_config", "fw_sim_set_kernel_config", "fw_sim_set_graph_steps",
     "fw_sim_set_relaxation_codes", "fw_sim_relaxation_codes", "fw_sim_run_timed",
     "fw_sim_destroy", "fw_group_create", "fw_group_config", "fw_group_create_shaped", "fw_group_shape",
-    "fw_group_run", "fw_group_destroy",
+    "fw_group_run", "fw_group_set_graph_steps", "fw_group_destroy",
     "fw_adj_create", "fw_adj_forward", "fw_adj_backward", "fw_adj_destroy",
 )
 VARIANTS = {"percell": 0, "stream": 1, "tile": 2, "pipe": 3, "pipelite": 4}
@@ -108,6 +108,7 @@ def load_library(path: Path | None = None) -> ctypes.CDLL:
         lib.fw_group_create_shaped.argtypes = [ctypes.POINTER(vp), i32, i32, i32, i32, ctypes.POINTER(vp)]
         lib.fw_group_shape.argtypes = [vp, ctypes.POINTER(i32), ctypes.POINTER(i32), ctypes.POINTER(i32)]
         lib.fw_group_destroy.argtypes = [vp]
+        lib.fw_group_set_graph_steps.argtypes = [vp, i64]
         lib.fw_adj_create.argtypes = [vp, i64, ctypes.POINTER(vp)]
         lib.fw_adj_forward.argtypes = [vp]
         lib.fw_adj_backward.argtypes = [vp, vp, sz, vp, vp, sz]
@@ -396,6 +397,11 @@ class HipGroup:
         _check(self.lib, self.lib.fw_group_shape(self._h, ctypes.byref(v), ctypes.byref(w), ctypes.byref(c)))
         shape = {"variant": "tile" if v.value == 2 else "stream", "width": w.value, "chunk": c.value}
         return {"velocity": shape, "pressure": shape}
+
+    def set_graph_steps(self, steps: int) -> None:
+        """Replay steady steps from a captured HIP graph of ``steps`` group steps (0: plain
+        launches); records are the same either way."""
+        _check(self.lib, self.lib.fw_group_set_graph_steps(self._h, int(steps)))
 
     def run(self, n_steps: int) -> None:
         _check(self.lib, self.lib.fw_group_run(self._h, int(n_steps)))

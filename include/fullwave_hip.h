@@ -213,6 +213,11 @@ int fw_group_create_shaped(fw_sim* const* sims, int32_t n, int32_t variant, int3
                            fw_group** out);
 int fw_group_shape(const fw_group* group, int32_t* variant, int32_t* width, int32_t* chunk);
 int fw_group_run(fw_group* group, int64_t n_steps);
+/* Run steady group steps from a captured HIP graph of `steps` steps (0: plain launches, the
+ * default); no counterpart in the reference. fw_group_run then replays the graph for whole
+ * multiples of `steps` and launches the remainder step by step; every member's records are the
+ * same either way. Cuts the host cost of a step's three launches for groups of small grids. */
+int fw_group_set_graph_steps(fw_group* group, int64_t steps);
 int fw_group_destroy(fw_group* group);
 
 /* ---- adjoint: gradients of a functional of the traces ----------------------------------- */

@@ -213,3 +213,27 @@ def test_tiled_group_rejects_unaligned_member():
         with engine.HipGroup([a, b], variant="auto") as grp:  # times the marching family only
             assert grp.kernel_config()["velocity"]["variant"] == "stream"
             grp.run(5)
+
+
+def test_group_graph_replay_matches_plain_launches():
+    """Group steps replayed from a captured graph -- several replays in one call, a remainder
+    launched plainly, members at different steps, traces read and other work done between the
+    calls -- give every member the plain launches' records bit for bit."""
+    media = _media()
+    k = min(m.nt for m in media)
+    plain = [engine.HipSimulation(m) for m in media]
+    graph = [engine.HipSimulation(m) for m in media]
+    try:
+        plain[1].run(5)  # one member ahead of the others, in both groups
+        graph[1].run(5)
+        with engine.HipGroup(plain, width=256, chunk=16) as gp, engine.HipGroup(graph, width=256, chunk=16) as gg:
+            gg.set_graph_steps(16)
+            for n in (40, 16, 37, k - 5 - 93):  # 2 replays + 8 steps; 1 replay; 2 + 5; the rest
+                gp.run(n)
+                gg.run(n)
+                for i, (a, b) in enumerate(zip(plain, graph)):
+                    np.testing.assert_array_equal(b.sensors(), a.sensors(), err_msg=f"member {i} after {n}")
+        assert np.abs(plain[0].sensors()).max() > 0
+    finally:
+        for s in plain + graph:
+            s.close()
