@@ -278,16 +278,87 @@ def bench_grouped(args) -> dict:
     }
 
 
+def bench_batched_mix(args) -> dict:
+    """The batched-transmit configuration as one workload: 128 steered focused transmits on one
+    2048 x 1536 medium (3000 steps each) and 32 jobs on uneven grids (1024^2, 1536x1024, 2048^2,
+    3072x2048; 3000..6000 steps), all submitted at once to the simulation server (the transmits
+    as batched jobs of ``--shots-per-job`` shots on one handle each, the uneven jobs as jobs of
+    their own, ``--slots`` running at once), against every transmit and job run one after the
+    other as a plain engine run (create, upload, tune, run, download). Jobs/hour counts each
+    transmit and each uneven job as one job; the latency percentiles are over the submitted jobs."""
+    import dataclasses
+
+    from fullwave25_amd import engine, workloads
+    from fullwave25_amd.serving import SimulationServer
+
+    t0 = time.time()
+    wl, shots = workloads.steered_transmits_2d(nt=3000, n_transmits=args.transmits)
+    inp = wl.inputs
+    rng = np.random.default_rng(2031)
+    sizes = [(1024, 1024), (1536, 1024), (2048, 2048), (3072, 2048)]
+    media = {sz: workloads.focused_linear_array_2d(nx=sz[0], ny=sz[1], nt=6000, seed=2032 + i).inputs
+             for i, sz in enumerate(sizes)}
+    uneven = [dataclasses.replace(media[sizes[int(rng.integers(len(sizes)))]], nt=int(rng.integers(3, 7)) * 1000)
+              for _ in range(args.jobs)]
+    n_jobs = shots.shape[0] + len(uneven)
+    print(f"# batched mix: {shots.shape[0]} transmits on {inp.shape} + {len(uneven)} uneven jobs, built in "
+          f"{time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+    with engine.HipSimulation(inp) as warm:  # first-use costs (library load, code objects)
+        warm.run(10)
+    t1 = time.perf_counter()
+    for b in range(shots.shape[0]):
+        with engine.HipSimulation(dataclasses.replace(inp, icmat=shots[b])) as sim:
+            sim.run(inp.nt)
+            sim.sensors()
+        if (b + 1) % 16 == 0:
+            print(f"# sequential transmit {b + 1}/{shots.shape[0]}", file=sys.stderr, flush=True)
+    t_tx = time.perf_counter() - t1
+    for q, job in enumerate(uneven):
+        with engine.HipSimulation(job) as sim:
+            sim.run(job.nt)
+            sim.sensors()
+        if (q + 1) % 8 == 0:
+            print(f"# sequential uneven job {q + 1}/{len(uneven)}", file=sys.stderr, flush=True)
+    t_seq = time.perf_counter() - t1
+    print(f"# sequential: {t_seq:.1f}s ({t_tx:.1f}s of transmits)", file=sys.stderr, flush=True)
+    k = max(1, int(args.shots_per_job))
+    t2 = time.perf_counter()
+    with SimulationServer(max_slots=args.slots) as srv:
+        futs = [srv.submit(inp, shots=shots[b:b + k]) for b in range(0, shots.shape[0], k)]
+        futs += [srv.submit(job) for job in uneven]
+        for f in futs:
+            f.result()
+    t_srv = time.perf_counter() - t2
+    st = srv.stats()
+    return {
+        "metric": "batched transmits + uneven jobs: jobs/hour vs sequential plain launches (1 GPU)",
+        "value": round(n_jobs / t_srv * 3600, 1), "unit": "jobs/hour", "n_gpus": 1, "higher_is_better": True,
+        "dtype": "f32", "data": "synthetic",
+        "sequential_jobs_per_hour": round(n_jobs / t_seq * 3600, 1), "speedup_vs_sequential": round(t_seq / t_srv, 3),
+        "wall_s": round(t_srv, 2), "sequential_wall_s": round(t_seq, 2),
+        "latency_p50_s": round(st.latency_p50_s, 3), "latency_p99_s": round(st.latency_p99_s, 3),
+        "submitted_jobs": len(futs), "peak_concurrent_jobs": srv.peak_running,
+        "config": {"workload": f"{shots.shape[0]} steered focused transmits on {inp.nx}x{inp.ny} cells (2048x1536 "
+                               f"interior, 3000 steps, {k} shots per batched job) + {len(uneven)} focused-transmit "
+                               f"jobs on 1024^2..3072x2048 interiors with 3000-6000 steps, submitted together",
+                   "transmits": int(shots.shape[0]), "uneven_jobs": len(uneven), "slots": args.slots},
+    }
+
+
 def bench_adjoint(args) -> dict:
     """The misfit-gradient job of the served mix on its own: 1024^2 interior, 2000 steps,
-    checkpoints every 200 steps; the time of the forward run with checkpoints and of the
-    reverse sweep (recomputing each segment from its checkpoint), against a plain forward run
-    of the same job."""
+    checkpoints every 200 steps, the L2 misfit of a smoothed initial model against the traces
+    of a seeded heterogeneous true medium; the time of the forward run with checkpoints and of
+    the reverse sweep (recomputing each segment from its checkpoint), against a plain forward
+    run of the same job."""
     from fullwave25_amd import adjoint, engine, workloads
 
-    inp = workloads.plane_wave_2d(nx=args.nx, ny=args.ny, nt=args.steps, sensor_depth=args.nx // 2).inputs
-    observed = np.zeros((inp.n_records, inp.ncoordsout))
-    with engine.HipSimulation(inp) as sim:  # first-use costs, and the plain run's kernels
+    initial, true = workloads.misfit_media_2d(nx=args.nx, ny=args.ny, nt=args.steps)
+    inp = initial.inputs
+    with engine.HipSimulation(true.inputs) as sim:  # the observed data; also first-use costs
+        sim.run(inp.nt)
+        observed = sim.sensors().astype(np.float64)
+    with engine.HipSimulation(inp) as sim:
         sim.run(10)
     t0 = time.perf_counter()
     with engine.HipSimulation(inp) as sim:
@@ -308,7 +379,8 @@ def bench_adjoint(args) -> dict:
         "plain_forward_s": round(t_plain, 4), "forward_with_checkpoints_s": round(t2 - t1, 4),
         "reverse_sweep_s": round(t3 - t2, 4), "gradient_jobs_per_hour": round(3600 / (t3 - t1), 1),
         "forward_glups": round(cells / t_plain / 1e9, 3), "finite": bool(np.isfinite(gk).all() and np.isfinite(gr).all()),
-        "config": {"workload": f"plane_wave_2d: {args.nx}x{args.ny} interior ({inp.nx}x{inp.ny} extended), "
+        "config": {"workload": f"misfit_media_2d: {args.nx}x{args.ny} interior ({inp.nx}x{inp.ny} extended), "
+                               f"smoothed initial model of a seeded layered medium ({inp.ndmap} sound speeds), "
                                f"{inp.nt} steps, checkpoint every {args.checkpoint_every}", "cells": inp.ncells},
     }
 
@@ -647,7 +719,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--config", default="hetero_3d",
-                    choices=(*SINGLE_CONFIGS, "batched_2d", "served_mix", "grouped_2d", "adjoint_2d", "inclusions_3d"))
+                    choices=(*SINGLE_CONFIGS, "batched_2d", "batched_mix", "served_mix", "grouped_2d", "adjoint_2d",
+                             "inclusions_3d"))
     ap.add_argument("--nx", type=int, default=None)
     ap.add_argument("--ny", type=int, default=None)
     ap.add_argument("--nz", type=int, default=None)
@@ -668,7 +741,9 @@ def main() -> int:
                                                        "CONFIG_JSON"),
                     help="summarize rocprofv3 --pmc runs into per-launch HBM traffic (JSON) and exit")
     ap.add_argument("--copy-bytes", type=int, default=268435456)
-    ap.add_argument("--transmits", type=int, default=32, help="batched_2d: transmits on the shared medium")
+    ap.add_argument("--transmits", type=int, default=32,
+                    help="batched_2d, batched_mix: transmits on the shared medium (batched_mix: 128)")
+    ap.add_argument("--shots-per-job", type=int, default=32, help="batched_mix: transmits per batched job")
     ap.add_argument("--jobs", type=int, default=20, help="served_mix, grouped_2d: number of jobs")
     ap.add_argument("--checkpoint-every", type=int, default=200, help="adjoint_2d: checkpoint spacing")
     ap.add_argument("--group-width", type=int, default=0, help="grouped_2d: lanes per workgroup (0: timed)")
@@ -686,8 +761,8 @@ def main() -> int:
     if args.config == "inclusions_3d":  # the workload's own name
         args.config = "hetero_3d"
     # grid sizes: the config's own unless given
-    own = {"batched_2d": (2048, 1536, 1), "grouped_2d": (2048, 2048, 1), "adjoint_2d": (1024, 1024, 1),
-           "served_mix": (1024, 1024, 256)}
+    own = {"batched_2d": (2048, 1536, 1), "batched_mix": (2048, 1536, 1), "grouped_2d": (2048, 2048, 1),
+           "adjoint_2d": (1024, 1024, 1), "served_mix": (1024, 1024, 256)}
     dflt = own.get(args.config) or tuple(SINGLE_CONFIGS[args.config][k] for k in ("nx", "ny", "nz"))
     args.nx = dflt[0] if args.nx is None else args.nx
     args.ny = dflt[1] if args.ny is None else args.ny
@@ -708,17 +783,22 @@ def main() -> int:
         gbs = engine.copy_bandwidth(args.copy_probe, repeats=5)
         print(json.dumps({"copy_bytes": args.copy_probe, "copy_gbs": round(gbs, 1)}), flush=True)
         return 0
-    if args.config in ("batched_2d", "served_mix", "grouped_2d", "adjoint_2d"):
+    if args.config in ("batched_2d", "batched_mix", "served_mix", "grouped_2d", "adjoint_2d"):
         if args.config in ("batched_2d", "adjoint_2d") and "--steps" not in sys.argv:
             args.steps = 3000 if args.config == "batched_2d" else 2000
-        fn = {"batched_2d": bench_batched, "served_mix": bench_served, "grouped_2d": bench_grouped,
-              "adjoint_2d": bench_adjoint}[args.config]
+        if args.config == "batched_mix":
+            if "--transmits" not in sys.argv:
+                args.transmits = 128
+            if "--jobs" not in sys.argv:
+                args.jobs = 32
+        fn = {"batched_2d": bench_batched, "batched_mix": bench_batched_mix, "served_mix": bench_served,
+              "grouped_2d": bench_grouped, "adjoint_2d": bench_adjoint}[args.config]
         print(json.dumps(fn(args)), flush=True)
         return 0
     if args.sweep:
         return sweep(args)
     line = bench_single(args)
-    inp = line.pop("_inputs")
+    inp = line.pop("_inputs", None)
     if "metric" not in line:  # --profile-steps
         print(json.dumps(line), flush=True)
         return 0

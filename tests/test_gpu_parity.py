@@ -418,3 +418,34 @@ def test_graph_replay_matches_plain_launches(monkeypatch, pinned):
         graph.set_graph_steps(32)
         graph.run(inp.nt)
         np.testing.assert_array_equal(graph.sensors(), plain.sensors())
+
+
+def test_empty_lists_and_degenerate_counts():
+    """No source points (the wave starts from an initial pressure field instead), no sensors,
+    a recording interval longer than the run, and a run of zero steps: against the oracle
+    started from the same field."""
+    shape = (48, 52)
+    rng = np.random.default_rng(42)
+    p0 = np.zeros(shape)
+    p0[20:28, 22:30] = rng.random((8, 8))
+    cases = {
+        "no sources": raw_inputs(shape, nt=40, outc=[[10, 10], [30, 40]]),
+        "no sensors": raw_inputs(shape, nt=40, icc=[[24, 26]], icmat=np.ones((1, 5))),
+        "modT > nT": raw_inputs(shape, nt=30, modt=45, outc=[[24, 26]]),
+    }
+    for name, inp in cases.items():
+        with engine.HipSimulation(inp) as sim:
+            sim.set_field("p", p0.astype(np.float32))
+            sim.run(0)
+            assert sim.step == 0 and sim.sensors().shape == (0, inp.ncoordsout)
+            sim.run(inp.nt)
+            got, p_got = sim.sensors().astype(np.float64), sim.field("p").astype(np.float64)
+        o = oracle.OracleSimulation(inp, 64)
+        o.set_field("p", p0)
+        o.run(inp.nt)
+        assert got.shape == o.sensors().shape == (-(-inp.nt // inp.modt), inp.ncoordsout), name
+        if inp.ncoordsout:
+            assert np.abs(got).max() > 0, name
+            assert rel_l2(got, o.sensors()) <= TOL, name
+        assert rel_l2(p_got, o.field("p")) <= TOL, name
+        print(f"{name}: records {got.shape}, field rel-L2 {rel_l2(p_got, o.field('p')):.2e}", flush=True)
