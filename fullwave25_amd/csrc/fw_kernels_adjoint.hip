@@ -20,6 +20,9 @@
 //   6. lambda_p += sum_a (D+_a)^T G_a   (k_adj_dplus_t, gather form)
 // The coefficient sets (dcmap) are constants. The transposed stencils gather: the value at x
 // collects every active cell whose stencil reads x, with that cell's coefficients.
+#include <cstdlib>
+#include <cstring>
+
 #include "fw_internal.h"
 
 namespace fw {
@@ -99,7 +102,9 @@ __global__ __launch_bounds__(256) void k_adj_mask(float* lp, const int64_t* src_
     if (t < nzero) lp[zero_lin[t]] = 0.f;
 }
 
-template <int ND, bool EXPO, bool UNI>
+// CODED (isotropic relaxation with codes): the relaxation coefficients of the cell come from its
+// code's table row (2 bytes a cell) instead of the five per-cell maps
+template <int ND, bool EXPO, bool UNI, bool CODED>
 __global__ __launch_bounds__(256) void k_adj_pressure(DevSim s, AdjView w) {
     int64_t i, j, k;
     if (!adj_cell<ND>(s, i, j, k)) return;
@@ -120,16 +125,20 @@ __global__ __launch_bounds__(256) void k_adj_pressure(DevSim s, AdjView w) {
         return;
     }
     const float t = -s.dt * s.K[c] * mu;
+    CodedCoef kc{};
+    if (CODED) kc = cell_coef<false, UNI>(s, (uint32_t)c);
 #pragma unroll
     for (int a = 0; a < ND; ++a) {
         const float h = dminus<ND, UNI>(s, w.fv[a], c, a, st);
-        const float ik = s.ikp[a][c];
+        const float ik = CODED ? kc.ik : s.ikp[a][c];
         float lh = t * ik;
 #pragma unroll
         for (int nu = 0; nu < kNRelax; ++nu) {
             const float tot = w.lphi[a][nu][c] + t;
-            lh += s.ap[a][nu][c] * tot;
-            w.lphi[a][nu][c] = s.bp[a][nu][c] * tot;
+            const float an = CODED ? (nu ? kc.a2 : kc.a1) : s.ap[a][nu][c];
+            const float bn = CODED ? (nu ? kc.b2 : kc.b1) : s.bp[a][nu][c];
+            lh += an * tot;
+            w.lphi[a][nu][c] = bn * tot;
         }
         div += h * ik + (w.fphi[a][0][c] + w.fphi[a][1][c]);
         w.H[a][c] = lh;
@@ -168,13 +177,15 @@ __global__ __launch_bounds__(256) void k_adj_dminus_t(DevSim s, AdjView w) {
     }
 }
 
-template <int ND, bool EXPO, bool UNI>
+template <int ND, bool EXPO, bool UNI, bool CODED>
 __global__ __launch_bounds__(256) void k_adj_velocity(DevSim s, AdjView w) {
     int64_t i, j, k;
     if (!adj_cell<ND>(s, i, j, k)) return;
     const int64_t c = i * s.sx + j * s.sy + k;
     const int64_t st[3] = {s.sx, s.sy, 1};
     const float rho_c = s.rho[c];
+    CodedCoef kc{};
+    if (CODED) kc = cell_coef<true, UNI>(s, (uint32_t)c);
 #pragma unroll
     for (int a = 0; a < ND; ++a) {
         const int64_t sa = st[a];
@@ -189,14 +200,16 @@ __global__ __launch_bounds__(256) void k_adj_velocity(DevSim s, AdjView w) {
             lg = -A * beta * nu;
             w.lv[a][c] = A * nu;
         } else {
-            const float ik = s.ikv[a][c];
+            const float ik = CODED ? kc.ik : s.ikv[a][c];
             u = g * ik + (w.fpsi[a][0][c] + w.fpsi[a][1][c]);
             lg = -beta * nu * ik;
 #pragma unroll
             for (int q = 0; q < kNRelax; ++q) {
                 const float tot = w.lpsi[a][q][c] - beta * nu;
-                lg += s.av[a][q][c] * tot;
-                w.lpsi[a][q][c] = s.bv[a][q][c] * tot;
+                const float aq = CODED ? (q ? kc.a2 : kc.a1) : s.av[a][q][c];
+                const float bq = CODED ? (q ? kc.b2 : kc.b1) : s.bv[a][q][c];
+                lg += aq * tot;
+                w.lpsi[a][q][c] = bq * tot;
             }
         }
         w.G[a][c] = lg;
@@ -233,6 +246,114 @@ __global__ __launch_bounds__(256) void k_adj_dplus_t(DevSim s, AdjView w) {
         }
     }
     w.lp[x] += acc;
+}
+
+// The two transposed gathers of a 2D step on tiles, with the gathered field (H or G), the
+// coefficient-set index of every cell it comes from and the coefficient table staged in LDS (the
+// per-cell kernels above read each neighbour's index and weights from global memory). A
+// workgroup of 256 lanes owns a tile of ATX rows (x) by ATY columns (y); lane (r, j) updates
+// rows r, r + 4, ... of column j. Axis x's field is staged over rows [i0 - 8, i0 + ATX + 8) and
+// columns [j0 - 1, j0 + ATY + 1), axis y's over rows [i0 - 1, i0 + ATX + 1) and columns
+// [j0 - 8, j0 + ATY + 8): every offset either gather reads (along its axis -8..+8, across it
+// -1..+1). Cells outside the grid stage 0 (H and G are 0 off the updated cells anyway). The
+// arithmetic and its order are the per-cell kernels', so the results are the same bit for bit.
+constexpr int ATX = 16, ATY = 64;
+constexpr int AXR = ATX + 16, AXC = ATY + 2;  // axis-x staged region
+constexpr int AYR = ATX + 2, AYC = ATY + 16;  // axis-y staged region
+constexpr int kAdjLdsSets = 256;              // coefficient sets staged (ndmap beyond: per-cell kernels)
+
+// DMINUS: lambda_v_a += (D-_a)^T H_a (k_adj_dminus_t); else lambda_p += sum_a (D+_a)^T G_a
+template <bool UNI, bool DMINUS>
+__global__ __launch_bounds__(256) void k_adj_gather_tile2d(DevSim s, AdjView w) {
+    __shared__ float FX[AXR * AXC];
+    __shared__ float FY[AYR * AYC];
+    __shared__ int16_t DX[UNI ? 1 : AXR * AXC];
+    __shared__ int16_t DY[UNI ? 1 : AYR * AYC];
+    __shared__ float CT[UNI ? 2 * kNCoef : kAdjLdsSets * 2 * kNCoef];  // [set][axis][9]
+    const float* fx = DMINUS ? w.H[0] : w.G[0];
+    const float* fy = DMINUS ? w.H[1] : w.G[1];
+    const int64_t i0 = kHalo + (int64_t)blockIdx.y * ATX, j0 = kHalo + (int64_t)blockIdx.x * ATY;
+    const int nsets = UNI ? 1 : s.ndmap;
+    for (int t = threadIdx.x; t < nsets * 2 * kNCoef; t += 256) CT[t] = s.coef[t];
+    for (int t = threadIdx.x; t < AXR * AXC; t += 256) {
+        const int r = t / AXC, cc = t - r * AXC;
+        const int64_t i = i0 - 8 + r, j = j0 - 1 + cc;
+        const bool in = i >= 0 && i < s.nx && j >= 0 && j < s.ny;
+        FX[t] = in ? fx[i * s.sx + j] : 0.f;
+        if (!UNI) DX[t] = in ? (int16_t)s.dcmap[i * s.sx + j] : 0;
+    }
+    for (int t = threadIdx.x; t < AYR * AYC; t += 256) {
+        const int r = t / AYC, cc = t - r * AYC;
+        const int64_t i = i0 - 1 + r, j = j0 - 8 + cc;
+        const bool in = i >= 0 && i < s.nx && j >= 0 && j < s.ny;
+        FY[t] = in ? fy[i * s.sx + j] : 0.f;
+        if (!UNI) DY[t] = in ? (int16_t)s.dcmap[i * s.sx + j] : 0;
+    }
+    __syncthreads();
+    const int jl = threadIdx.x % ATY, rl = threadIdx.x / ATY;
+    const int64_t j = j0 + jl;
+    if (j >= s.ny - kHalo) return;
+    // staged element at (di, dj) from cell (il, jl) of the tile, for axis x's and axis y's field
+    auto ex = [&](int il, int di, int dj) { return (il + 8 + di) * AXC + (jl + 1 + dj); };
+    auto ey = [&](int il, int di, int dj) { return (il + 1 + di) * AYC + (jl + 8 + dj); };
+    auto wx = [&](int e, int k) { return CT[(UNI ? 0 : (int)DX[e] * 2) * kNCoef + k]; };             // axis 0 set
+    auto wy = [&](int e, int k) { return CT[(UNI ? kNCoef : ((int)DY[e] * 2 + 1) * kNCoef) + k]; };  // axis 1 set
+    for (int il = rl; il < ATX; il += 4) {
+        const int64_t i = i0 + il;
+        if (i >= s.nx - kHalo) break;
+        const int64_t x = i * s.sx + j;
+        if (DMINUS) {
+            // axis x: c1 = x - (m-1) e_x, c2 = x + m e_x; transverse c1 = x -+ e_y, c2 = x + e_x -+ e_y
+            float acc = 0.f;
+#pragma unroll
+            for (int m = 1; m <= 8; ++m) {
+                const int e1 = ex(il, -(m - 1), 0), e2 = ex(il, m, 0);
+                acc += wx(e1, m - 1) * FX[e1] - wx(e2, m - 1) * FX[e2];
+            }
+#pragma unroll
+            for (int sg = -1; sg <= 1; sg += 2) {
+                const int e1 = ex(il, 0, -sg), e2 = ex(il, 1, -sg);
+                acc += wx(e1, 8) * FX[e1] - wx(e2, 8) * FX[e2];
+            }
+            w.lv[0][x] += acc;
+            acc = 0.f;
+#pragma unroll
+            for (int m = 1; m <= 8; ++m) {
+                const int e1 = ey(il, 0, -(m - 1)), e2 = ey(il, 0, m);
+                acc += wy(e1, m - 1) * FY[e1] - wy(e2, m - 1) * FY[e2];
+            }
+#pragma unroll
+            for (int sg = -1; sg <= 1; sg += 2) {
+                const int e1 = ey(il, -sg, 0), e2 = ey(il, -sg, 1);
+                acc += wy(e1, 8) * FY[e1] - wy(e2, 8) * FY[e2];
+            }
+            w.lv[1][x] += acc;
+        } else {
+            // axis x: c1 = x - m e_x, c2 = x + (m-1) e_x; transverse c1 = x - e_x -+ e_y, c2 = x -+ e_y
+            float acc = 0.f;
+#pragma unroll
+            for (int m = 1; m <= 8; ++m) {
+                const int e1 = ex(il, -m, 0), e2 = ex(il, m - 1, 0);
+                acc += wx(e1, m - 1) * FX[e1] - wx(e2, m - 1) * FX[e2];
+            }
+#pragma unroll
+            for (int sg = -1; sg <= 1; sg += 2) {
+                const int e1 = ex(il, -1, -sg), e2 = ex(il, 0, -sg);
+                acc += wx(e1, 8) * FX[e1] - wx(e2, 8) * FX[e2];
+            }
+#pragma unroll
+            for (int m = 1; m <= 8; ++m) {
+                const int e1 = ey(il, 0, -m), e2 = ey(il, 0, m - 1);
+                acc += wy(e1, m - 1) * FY[e1] - wy(e2, m - 1) * FY[e2];
+            }
+#pragma unroll
+            for (int sg = -1; sg <= 1; sg += 2) {
+                const int e1 = ey(il, -sg, -1), e2 = ey(il, -sg, 0);
+                acc += wy(e1, 8) * FY[e1] - wy(e2, 8) * FY[e2];
+            }
+            w.lp[x] += acc;
+        }
+    }
 }
 
 // d/d rho[x] = sum_a (g_rb_a[x] + g_rb_a[x - e_a]) / 2, over the whole (padded) grid
@@ -280,14 +401,37 @@ hipError_t launch_adj_mask(float* lp, const int64_t* src_lin, int64_t nsrc, cons
     return hipGetLastError();
 }
 
+// the transposed gathers on LDS tiles: 2D grids of at most kAdjLdsSets coefficient sets (the
+// staged set indices are 16-bit)
+bool adj_tiles(const DevSim& s) { return s.ndim == 2 && s.ndmap <= kAdjLdsSets; }
+
+// FULLWAVE_HIP_ADJ_GATHERS=percell keeps the per-cell gathers (to compare the two forms)
+bool adj_percell_gathers() {
+    const char* e = getenv("FULLWAVE_HIP_ADJ_GATHERS");
+    return e && !strncmp(e, "percell", 7);
+}
+
+template <bool DMINUS>
+void launch_gather_tile2d(const DevSim& s, const AdjView& w, hipStream_t st) {
+    const dim3 g((unsigned)((s.ny - 2 * kHalo + ATY - 1) / ATY), (unsigned)((s.nx - 2 * kHalo + ATX - 1) / ATX));
+    if (s.ndmap == 1) hipLaunchKernelGGL((k_adj_gather_tile2d<true, DMINUS>), g, dim3(256), 0, st, s, w);
+    else hipLaunchKernelGGL((k_adj_gather_tile2d<false, DMINUS>), g, dim3(256), 0, st, s, w);
+}
+
 template <int ND, bool UNI>
 void launch_adj_step_nd(const DevSim& s, const AdjView& w, dim3 g, dim3 b, hipStream_t st) {
-    if (s.aexp) hipLaunchKernelGGL((k_adj_pressure<ND, true, UNI>), g, b, 0, st, s, w);
-    else hipLaunchKernelGGL((k_adj_pressure<ND, false, UNI>), g, b, 0, st, s, w);
-    hipLaunchKernelGGL((k_adj_dminus_t<ND, UNI>), g, b, 0, st, s, w);
-    if (s.aexp) hipLaunchKernelGGL((k_adj_velocity<ND, true, UNI>), g, b, 0, st, s, w);
-    else hipLaunchKernelGGL((k_adj_velocity<ND, false, UNI>), g, b, 0, st, s, w);
-    hipLaunchKernelGGL((k_adj_dplus_t<ND, UNI>), g, b, 0, st, s, w);
+    const bool coded = !s.aexp && s.isotropic && s.rcode;
+    const bool tiles = adj_tiles(s) && !adj_percell_gathers();
+    if (s.aexp) hipLaunchKernelGGL((k_adj_pressure<ND, true, UNI, false>), g, b, 0, st, s, w);
+    else if (coded) hipLaunchKernelGGL((k_adj_pressure<ND, false, UNI, true>), g, b, 0, st, s, w);
+    else hipLaunchKernelGGL((k_adj_pressure<ND, false, UNI, false>), g, b, 0, st, s, w);
+    if (tiles) launch_gather_tile2d<true>(s, w, st);
+    else hipLaunchKernelGGL((k_adj_dminus_t<ND, UNI>), g, b, 0, st, s, w);
+    if (s.aexp) hipLaunchKernelGGL((k_adj_velocity<ND, true, UNI, false>), g, b, 0, st, s, w);
+    else if (coded) hipLaunchKernelGGL((k_adj_velocity<ND, false, UNI, true>), g, b, 0, st, s, w);
+    else hipLaunchKernelGGL((k_adj_velocity<ND, false, UNI, false>), g, b, 0, st, s, w);
+    if (tiles) launch_gather_tile2d<false>(s, w, st);
+    else hipLaunchKernelGGL((k_adj_dplus_t<ND, UNI>), g, b, 0, st, s, w);
 }
 
 hipError_t launch_adj_step(const DevSim& s, const AdjView& w, hipStream_t st) {

@@ -128,3 +128,26 @@ def test_one_speed_kernels_equal_general_kernels():
     print(f"one-speed vs general adjoint kernels: rel-L2 dK {ek:.2e}, drho {er:.2e}", flush=True)
     assert np.abs(gk1).max() > 0
     assert ek <= 1e-5 and er <= 1e-5
+
+
+@pytest.mark.parametrize("case", ["2d_iso", "2d_aniso", "2d_exp", "2d_one_speed"])
+def test_tiled_gathers_equal_per_cell_gathers(monkeypatch, case):
+    """The transposed gathers on LDS tiles (2D) against the per-cell gathers
+    (FULLWAVE_HIP_ADJ_GATHERS=percell): the same arithmetic in the same order, so the same bits;
+    the forward kernels pinned so that both sweeps recompute the same states."""
+    inp = CASES[case]()
+    w = np.random.default_rng(67).standard_normal((inp.n_records, inp.ncoordsout)).astype(np.float32)
+    pin = {"velocity": {"variant": "percell", "width": 0, "chunk": 0},
+           "pressure": {"variant": "percell", "width": 0, "chunk": 0}}
+    out = {}
+    for mode in ("tiles", "percell"):
+        if mode == "percell":
+            monkeypatch.setenv("FULLWAVE_HIP_ADJ_GATHERS", "percell")
+        else:
+            monkeypatch.delenv("FULLWAVE_HIP_ADJ_GATHERS", raising=False)
+        with HipAdjoint(inp, checkpoint_every=23, kernel_config=pin) as adj:
+            adj.forward()
+            out[mode] = adj.backward(w)
+    assert np.abs(out["tiles"][0]).max() > 0
+    np.testing.assert_array_equal(out["tiles"][0], out["percell"][0])
+    np.testing.assert_array_equal(out["tiles"][1], out["percell"][1])
