@@ -346,11 +346,13 @@ def bench_batched_mix(args) -> dict:
 
 
 def bench_adjoint(args) -> dict:
-    """The misfit-gradient job of the served mix on its own: 1024^2 interior, 2000 steps,
-    checkpoints every 200 steps, the L2 misfit of a smoothed initial model against the traces
-    of a seeded heterogeneous true medium; the time of the forward run with checkpoints and of
-    the reverse sweep (recomputing each segment from its checkpoint), against a plain forward
-    run of the same job."""
+    """The misfit-gradient job of the served mix on its own: 1024^2 interior, 2000 steps, the L2
+    misfit of a smoothed initial model against the traces of a seeded heterogeneous true medium;
+    the time of the forward run (which writes the tape and the checkpoints) and of the reverse
+    sweep, against a plain forward run of the same job. ``--checkpoint-every 0`` (default) lets
+    the engine choose the segment length (the whole run taped when it fits in half of the free
+    device memory, so nothing is recomputed); the same job with 200-step segments (each segment
+    but the last recomputed from its checkpoint) is timed beside it."""
     from fullwave25_amd import adjoint, engine, workloads
 
     initial, true = workloads.misfit_media_2d(nx=args.nx, ny=args.ny, nt=args.steps)
@@ -365,24 +367,40 @@ def bench_adjoint(args) -> dict:
         sim.run(inp.nt)
         sim.sensors()
     t_plain = time.perf_counter() - t0
-    with adjoint.HipAdjoint(inp, checkpoint_every=args.checkpoint_every) as adj:
-        t1 = time.perf_counter()
-        tr = adj.forward()
-        t2 = time.perf_counter()
-        gk, gr = adj.backward((tr - observed).astype(np.float32))
-        t3 = time.perf_counter()
+
+    def gradient(every: int):
+        with adjoint.HipAdjoint(inp, checkpoint_every=every) as adj:
+            t1 = time.perf_counter()
+            tr = adj.forward()
+            t2 = time.perf_counter()
+            gk, gr = adj.backward((tr - observed).astype(np.float32))
+            t3 = time.perf_counter()
+            return adj.checkpoint_every, t2 - t1, t3 - t2, gk, gr, adj.sim.kernel_config()
+
+    every, t_fwd, t_rev, gk, gr, kcfg = gradient(args.checkpoint_every)
+    seg = min(200, inp.nt)
+    alt = gradient(seg) if every != seg else None
+    tape_gb = every * (inp.ndim + 1) * inp.ncells * 4 / 1e9
     cells = inp.ncells * inp.nt
-    return {
+    out = {
         "metric": "misfit gradient (dJ/dK, dJ/drho): time vs a plain forward run (1 GPU)",
-        "value": round((t3 - t1) / t_plain, 3), "unit": "forward-run equivalents", "n_gpus": 1,
+        "value": round((t_fwd + t_rev) / t_plain, 3), "unit": "forward-run equivalents", "n_gpus": 1,
         "higher_is_better": False, "dtype": "f32", "data": "synthetic",
-        "plain_forward_s": round(t_plain, 4), "forward_with_checkpoints_s": round(t2 - t1, 4),
-        "reverse_sweep_s": round(t3 - t2, 4), "gradient_jobs_per_hour": round(3600 / (t3 - t1), 1),
+        "plain_forward_s": round(t_plain, 4), "forward_with_tape_s": round(t_fwd, 4),
+        "reverse_sweep_s": round(t_rev, 4), "gradient_jobs_per_hour": round(3600 / (t_fwd + t_rev), 1),
+        "checkpoint_every": every, "tape_gb": round(tape_gb, 2), "kernel_config": kcfg,
         "forward_glups": round(cells / t_plain / 1e9, 3), "finite": bool(np.isfinite(gk).all() and np.isfinite(gr).all()),
         "config": {"workload": f"misfit_media_2d: {args.nx}x{args.ny} interior ({inp.nx}x{inp.ny} extended), "
                                f"smoothed initial model of a seeded layered medium ({inp.ndmap} sound speeds), "
-                               f"{inp.nt} steps, checkpoint every {args.checkpoint_every}", "cells": inp.ncells},
+                               f"{inp.nt} steps, segments of {every} steps", "cells": inp.ncells},
     }
+    if alt is not None:
+        out[f"segments_of_{seg}"] = {
+            "value": round((alt[1] + alt[2]) / t_plain, 3), "forward_with_tape_s": round(alt[1], 4),
+            "reverse_sweep_s": round(alt[2], 4),
+            "max_rel_diff_grad_K": float(np.abs(alt[3] - gk).max() / max(np.abs(gk).max(), 1e-30)),
+        }
+    return out
 
 
 def _served_jobs(args):
@@ -443,8 +461,8 @@ def bench_served(args) -> dict:
     print(f"# served mix: {len(jobs)} jobs built in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
 
     def run_plain(kind, inp):
-        if kind == "adjoint":
-            with adjoint.HipAdjoint(inp) as adj:
+        if kind == "adjoint":  # the server's segment length (SimulationServer.submit_adjoint)
+            with adjoint.HipAdjoint(inp, checkpoint_every=200) as adj:
                 tr = adj.forward()
                 adj.backward((tr - observed[id(inp)]).astype(np.float32))
             return
@@ -745,7 +763,8 @@ def main() -> int:
                     help="batched_2d, batched_mix: transmits on the shared medium (batched_mix: 128)")
     ap.add_argument("--shots-per-job", type=int, default=32, help="batched_mix: transmits per batched job")
     ap.add_argument("--jobs", type=int, default=20, help="served_mix, grouped_2d: number of jobs")
-    ap.add_argument("--checkpoint-every", type=int, default=200, help="adjoint_2d: checkpoint spacing")
+    ap.add_argument("--checkpoint-every", type=int, default=0,
+                    help="adjoint_2d: segment length (0: the engine's choice)")
     ap.add_argument("--group-width", type=int, default=0, help="grouped_2d: lanes per workgroup (0: timed)")
     ap.add_argument("--group-chunk", type=int, default=0, help="grouped_2d: rows per workgroup (0: timed)")
     ap.add_argument("--group-variant", choices=("auto", "stream", "tile"), default="auto",

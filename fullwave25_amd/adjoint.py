@@ -1,11 +1,12 @@
 """Gradients of sensor-trace functionals with respect to the sound-speed and density maps.
 
 The reference simulates forward only; this module has no counterpart there. The gradient comes
-from the engine's hand-written adjoint kernels (the transposed time step, run backwards with
-checkpointed recomputation; ``fw_adj_*`` in include/fullwave_hip.h), exposed as
+from the engine's hand-written adjoint kernels (the transposed time step, run backwards over a
+tape of the forward run, with checkpointed recomputation where the whole tape does not fit;
+``fw_adj_*`` in include/fullwave_hip.h), exposed as
 
-* :class:`HipAdjoint` — one forward run from rest and any number of reverse sweeps on its
-  checkpoints: ``backward(w)`` returns ``dJ/dK`` and ``dJ/drho`` for ``J = sum(w * traces)``;
+* :class:`HipAdjoint` — one forward run from rest and any number of reverse sweeps on its tape
+  and checkpoints: ``backward(w)`` returns ``dJ/dK`` and ``dJ/drho`` for ``J = sum(w * traces)``;
 * :class:`TraceFunction` — a ``torch.autograd.Function``: ``traces = TraceFunction.apply(c, rho,
   template)`` with ``c`` and ``rho`` maps on the padded grid, so any torch loss of the traces
   back-propagates into ``c`` and ``rho``;
@@ -30,10 +31,13 @@ from .solver.inputs import SimulationInputs
 
 
 class HipAdjoint:
-    """A forward run of ``inputs`` kept for reverse sweeps (checkpoint every
-    ``checkpoint_every`` steps)."""
+    """A forward run of ``inputs`` kept for reverse sweeps. The sweep reads a tape of the run
+    (``ndim + 1`` grids a step); the run is cut into segments of ``checkpoint_every`` steps whose
+    tapes are recomputed from checkpoints, except the last. ``checkpoint_every=0`` (default) tapes
+    the whole run when that fits in half of the device's free memory (no recomputation), else
+    picks the segment length that needs the least memory; ``self.checkpoint_every`` says which."""
 
-    def __init__(self, inputs: SimulationInputs, *, checkpoint_every: int = 200, device: int = 0,
+    def __init__(self, inputs: SimulationInputs, *, checkpoint_every: int = 0, device: int = 0,
                  kernel_config: dict | None = None) -> None:
         self.inp = inputs
         self.sim = engine.HipSimulation(inputs, device)
@@ -43,8 +47,11 @@ class HipAdjoint:
         self._a = ctypes.c_void_p()
         try:
             engine._check(self.lib, self.lib.fw_adj_create(self.sim._h, int(checkpoint_every), ctypes.byref(self._a)))
+            every = ctypes.c_int64()
+            engine._check(self.lib, self.lib.fw_adj_checkpoint_every(self._a, ctypes.byref(every)))
+            self.checkpoint_every = int(every.value)
         except Exception:
-            self.sim.close()
+            self.close()
             raise
 
     def forward(self) -> np.ndarray:
@@ -104,7 +111,7 @@ def chain_to_c_rho(c, rho, gk, grho):
     return gk * 2.0 * rho * c, np.asarray(grho, dtype=np.float64) + gk * c**2
 
 
-def misfit_gradient(template: SimulationInputs, c, rho, observed, *, checkpoint_every: int = 200,
+def misfit_gradient(template: SimulationInputs, c, rho, observed, *, checkpoint_every: int = 0,
                     device: int = 0) -> tuple[float, np.ndarray, np.ndarray, np.ndarray]:
     """L2 sensor misfit ``0.5 * sum((traces - observed)^2)`` of the medium ``(c, rho)`` and its
     gradient: returns ``(loss, dL/dc, dL/drho, traces)``."""
@@ -124,7 +131,7 @@ def _torch():
 
 
 class TraceFunction:
-    """``traces = TraceFunction.apply(c, rho, template, checkpoint_every=200)``: sensor traces of
+    """``traces = TraceFunction.apply(c, rho, template, checkpoint_every=0)``: sensor traces of
     the template simulation with the sound-speed and density maps ``c``, ``rho`` (torch tensors
     on the padded grid), differentiable with respect to both."""
 
@@ -158,7 +165,7 @@ class TraceFunction:
         return _Fn
 
     @classmethod
-    def apply(cls, c, rho, template: SimulationInputs, checkpoint_every: int = 200):
+    def apply(cls, c, rho, template: SimulationInputs, checkpoint_every: int = 0):
         if cls._fn is None:
             cls._fn = cls._build()
         return cls._fn.apply(c, rho, template, checkpoint_every)

@@ -55,6 +55,15 @@ struct DevSim {
     const uint16_t* rcode;
     const float* rtab_v;
     const float* rtab_p;
+
+    // adjoint tape of the step (null: none). When tape_div is set, the velocity kernel also
+    // writes, at every updated cell, tape_u[a][c] = D+_a p / kappa + psi1' + psi2' (exponential
+    // model: A D+_a p), and the pressure kernel tape_div[c] = sum_a (D-_a v_a' / kappa + phi1' +
+    // phi2') (exponential model: sum_a D-_a v_a'): what the adjoint needs of the forward state.
+    // Single-transmit handles only; the per-cell and tiled 2D kernels write it (launch_velocity
+    // / launch_pressure take those while a tape is set).
+    float* tape_u[kMaxDim];
+    float* tape_div;
 };
 
 constexpr int kCodeVals = 11;          // values per tuple: velocity side, pressure side, dcmap
@@ -170,17 +179,15 @@ struct DevBoundary {
 };
 
 // Adjoint sweep of one time step (fw_kernels_adjoint.hip): the adjoint state (lp, lv, lpsi,
-// lphi, laid out like the forward state), the forward state around the step (p before it; v,
-// psi, phi after it), per-axis scratch H and G, and the gradient accumulators.
+// lphi, laid out like the forward state), the step's tape (DevSim::tape_u / tape_div as the
+// forward kernels wrote them), per-axis scratch H and G, and the gradient accumulators.
 struct AdjView {
     float* lp;
     float* lv[kMaxDim];
     float* lpsi[kMaxDim][kNRelax];
     float* lphi[kMaxDim][kNRelax];
-    const float* fp_prev;
-    const float* fv[kMaxDim];
-    const float* fpsi[kMaxDim][kNRelax];
-    const float* fphi[kMaxDim][kNRelax];
+    const float* tu[kMaxDim];
+    const float* tdiv;
     float* H[kMaxDim];
     float* G[kMaxDim];
     float* gK;               // d/dK, summed over steps
@@ -272,11 +279,46 @@ hipError_t launch_code_table(const CodeMaps& m, int32_t ncodes, const int32_t* r
                              hipStream_t st);
 hipError_t launch_code_verify(const CodeMaps& m, int64_t n, const uint16_t* code, int32_t ncodes, const float* tab_v,
                               const float* tab_p, int32_t* status, hipStream_t st);
-hipError_t launch_adj_records(float* lp, const float* grow, const int64_t* vox, const int32_t* first,
-                              const int32_t* order, int64_t nvox, hipStream_t st);
-hipError_t launch_adj_mask(float* lp, const int64_t* src_lin, int64_t nsrc, const int64_t* zero_lin, int64_t nzero,
-                           hipStream_t st);
-hipError_t launch_adj_step(const DevSim& s, const AdjView& w, hipStream_t st);
+// The sparse part of a reverse step, in one launch: the records' weights added at the distinct
+// sensor voxels (grow: the step's row of weights, or null when the step records nothing), then
+// the source points (nsrc of them while the step injects, else 0) and the pressure-release
+// points zeroed. vmask[t] says whether voxel t is one of those (1: pressure release, 2: a
+// source point); its record is then dropped, as the zeroing would overwrite it.
+struct AdjSparse {
+    const int64_t* vox;
+    const int32_t* first;
+    const int32_t* order;
+    const int8_t* vmask;
+    int64_t nvox;
+    const int64_t* src_lin;
+    int64_t nsrc;
+    const int64_t* zero_lin;
+    int64_t nzero;
+};
+hipError_t launch_adj_sparse(float* lp, const float* grow, const AdjSparse& sp, bool injecting, hipStream_t st);
+// The start of the next (earlier) step of the sweep, done by launch_adj_rest when
+// adj_fuses_next: that step's sparse part (as k_adj_sparse; kind[c]: bit 0 pressure release,
+// bit 1 source point, bit 2 sensor voxel, per device cell) and its pressure transposed (tdiv:
+// that step's tape).
+struct AdjNext {
+    const float* tdiv;
+    const float* grow;  // the step's record weights, or null
+    int injecting;
+    const uint8_t* kind;
+    const int64_t* vox;  // sorted distinct sensor voxels
+    const int32_t* first;
+    const int32_t* order;
+    int64_t nvox;
+};
+// One step of the sweep, after launch_adj_sparse: launch_adj_pressure (the pressure transposed),
+// then launch_adj_rest (the rest of the step; with `next` when adj_fuses_next, also the start of
+// the next step, whose launch_adj_sparse and launch_adj_pressure are then skipped).
+bool adj_fuses_next(const DevSim& s);
+hipError_t launch_adj_pressure(const DevSim& s, const AdjView& w, hipStream_t st);
+hipError_t launch_adj_rest(const DevSim& s, const AdjView& w, const AdjNext* next, hipStream_t st);
+// the forward kernels that write a tape (DevSim::tape_div set): the per-cell kernels, and the
+// tiled 2D kernels (8 rows per workgroup) where tile2d_supported
+bool tape_capable(const DevSim& s, const LaunchCfg& cfg);
 hipError_t launch_adj_rho(const DevSim& s, const AdjView& w, float* grho, hipStream_t st);
 
 }  // namespace fw

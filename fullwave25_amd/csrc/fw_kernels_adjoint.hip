@@ -1,25 +1,31 @@
 // Adjoint (reverse-mode) kernels: one time step of the update transposed.
 //
 // The forward step is, per active cell c and axis a (h_a folded into the coefficients):
-//   velocity:  g_a = D+_a p,  psi' = b psi + a g_a,  v_a' = v_a - beta_a (g_a / kappa + sum psi')
-//   pressure:  h_a = D-_a v_a',  phi' = b phi + a h_a,  p' = p - dt K sum_a (h_a / kappa + sum phi')
+//   velocity:  g_a = D+_a p,  psi' = b psi + a g_a,  u_a = g_a / kappa + sum psi',
+//              v_a' = v_a - beta_a u_a
+//   pressure:  h_a = D-_a v_a',  phi' = b phi + a h_a,  div = sum_a (h_a / kappa + sum phi'),
+//              p' = p - dt K div
 //   then p'[source] = signal, p'[air] = 0, and the sensors record p'
-// (the exponential model: v_a' = A (v_a - beta_a g_a), p' = A (p - dt K sum_a h_a)), with
-// beta_a = dt / rho_bar_a. Going back through step n with the adjoint state lambda of the state
-// after the step:
-//   1. lambda_p[sensor] += dL/d record   (k_adj_records)
-//   2. lambda_p[source] = lambda_p[air] = 0   (k_adj_mask; those values were overwritten)
-//   3. pressure transposed (k_adj_pressure): t = -dt K mu, mu = lambda_p;
+// (the exponential model: u_a = A g_a, v_a' = A v_a - beta_a u_a; div = sum_a h_a,
+// p' = A (p - dt K div)), with beta_a = dt / rho_bar_a. The forward kernels write u_a and div
+// of every step to a tape (DevSim::tape_u, tape_div): that is all the sweep needs of the forward
+// state. Going back through step n with the adjoint state lambda of the state after the step:
+//   1. lambda_p[sensor] += dL/d record, then lambda_p[source] = lambda_p[air] = 0 (k_adj_sparse;
+//      those values were overwritten)
+//   2. pressure transposed (k_adj_pressure): t = -dt K mu, mu = lambda_p;
 //        lambda_phi_tot = lambda_phi + t, H_a = t / kappa + sum a lambda_phi_tot,
-//        lambda_phi = b lambda_phi_tot, dL/dK += mu * (-dt) * div (div from the stored state)
-//   4. lambda_v_a += (D-_a)^T H_a   (k_adj_dminus_t, gather form)
-//   5. velocity transposed (k_adj_velocity): nu = lambda_v_a;
+//        lambda_phi = b lambda_phi_tot, dL/dK += mu * (-dt) * div
+//   3. lambda_v_a += (D-_a)^T H_a   (gather form)
+//   4. velocity transposed (k_adj_velocity): nu = lambda_v_a;
 //        lambda_psi_tot = lambda_psi - beta nu, G_a = -beta nu / kappa + sum a lambda_psi_tot,
-//        lambda_psi = b lambda_psi_tot, dL/d rho_bar_a += nu * u_a * dt / rho_bar^2 with
-//        u_a = g_a / kappa + sum psi'
-//   6. lambda_p += sum_a (D+_a)^T G_a   (k_adj_dplus_t, gather form)
+//        lambda_psi = b lambda_psi_tot, dL/d rho_bar_a += nu * u_a * dt / rho_bar^2
+//   5. lambda_p += sum_a (D+_a)^T G_a   (gather form)
 // The coefficient sets (dcmap) are constants. The transposed stencils gather: the value at x
-// collects every active cell whose stencil reads x, with that cell's coefficients.
+// collects every active cell whose stencil reads x, with that cell's coefficients. In 2D, steps
+// 3 and 4 run as one tiled kernel (the gather yields nu of its own cell, which step 4 needs
+// alone) and step 5 as another, which also runs steps 1 and 2 of the next (earlier) step for its
+// cells when that step's tape is held (they need lambda_p of the cell alone); elsewhere each step
+// is its own per-cell kernel.
 #include <cstdlib>
 #include <cstring>
 
@@ -51,85 +57,43 @@ __device__ inline const float* wts(const DevSim& s, int64_t c, int nd, int a) {
     return s.coef + ((int64_t)s.dcmap[c] * nd + a) * kNCoef;
 }
 
-template <int ND, bool UNI>
-__device__ inline float dplus(const DevSim& s, const float* p, int64_t c, int a, const int64_t* st) {
-    const float* w = wts<UNI>(s, c, ND, a);
-    const int64_t sa = st[a];
-    float acc = 0.f;
-#pragma unroll
-    for (int m = 1; m <= 8; ++m) acc += w[m - 1] * (p[c + m * sa] - p[c - (m - 1) * sa]);
-    float tr = 0.f;
-#pragma unroll
-    for (int b = 0; b < ND; ++b) {
-        if (b == a) continue;
-        const int64_t sb = st[b];
-        tr += (p[c + sa + sb] - p[c + sb]) + (p[c + sa - sb] - p[c - sb]);
-    }
-    return acc + w[8] * tr;
-}
-
-template <int ND, bool UNI>
-__device__ inline float dminus(const DevSim& s, const float* f, int64_t c, int a, const int64_t* st) {
-    const float* w = wts<UNI>(s, c, ND, a);
-    const int64_t sa = st[a];
-    float acc = 0.f;
-#pragma unroll
-    for (int m = 1; m <= 8; ++m) acc += w[m - 1] * (f[c + (m - 1) * sa] - f[c - m * sa]);
-    float tr = 0.f;
-#pragma unroll
-    for (int b = 0; b < ND; ++b) {
-        if (b == a) continue;
-        const int64_t sb = st[b];
-        tr += (f[c + sb] - f[c - sa + sb]) + (f[c - sb] - f[c - sa - sb]);
-    }
-    return acc + w[8] * tr;
-}
-
-// records of one step: one thread per distinct sensor voxel sums its sensors' weights
-__global__ __launch_bounds__(256) void k_adj_records(float* lp, const float* grow, const int64_t* vox,
-                                                     const int32_t* first, const int32_t* order, int64_t nvox) {
+// Records, then the overwritten points, of one step (see AdjSparse). One thread per distinct
+// sensor voxel sums its sensors' weights; a voxel that the same step zeroes keeps no record, so
+// no two threads write one address.
+__global__ __launch_bounds__(256) void k_adj_sparse(float* lp, const float* grow, AdjSparse sp, int injecting) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nvox) return;
-    float acc = 0.f;
-    for (int32_t q = first[t]; q < first[t + 1]; ++q) acc += grow[order[q]];
-    lp[vox[t]] += acc;
+    if (grow && t < sp.nvox) {
+        const int m = sp.vmask[t];
+        if (!(m == 1 || (m == 2 && injecting))) {
+            float acc = 0.f;
+            for (int32_t q = sp.first[t]; q < sp.first[t + 1]; ++q) acc += grow[sp.order[q]];
+            lp[sp.vox[t]] += acc;
+        }
+    }
+    if (injecting && t < sp.nsrc && sp.src_lin[t] >= 0) lp[sp.src_lin[t]] = 0.f;
+    if (t < sp.nzero) lp[sp.zero_lin[t]] = 0.f;
 }
 
-__global__ __launch_bounds__(256) void k_adj_mask(float* lp, const int64_t* src_lin, int64_t nsrc,
-                                                  const int64_t* zero_lin, int64_t nzero) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < nsrc && src_lin[t] >= 0) lp[src_lin[t]] = 0.f;
-    if (t < nzero) lp[zero_lin[t]] = 0.f;
-}
-
-// CODED (isotropic relaxation with codes): the relaxation coefficients of the cell come from its
-// code's table row (2 bytes a cell) instead of the five per-cell maps
-template <int ND, bool EXPO, bool UNI, bool CODED>
-__global__ __launch_bounds__(256) void k_adj_pressure(DevSim s, AdjView w) {
-    int64_t i, j, k;
-    if (!adj_cell<ND>(s, i, j, k)) return;
-    const int64_t c = i * s.sx + j * s.sy + k;
-    const int64_t st[3] = {s.sx, s.sy, 1};
-    const float mu = w.lp[c];
-    float div = 0.f;
+// The pressure transposed at active cell c with mu = lambda_p[c] (after the step's sparse part)
+// and div the step's tape; leaves lambda_p[c] = mu (the exponential model: A mu). CODED
+// (isotropic relaxation with codes): the relaxation coefficients of the cell come from its code's
+// table row (2 bytes a cell) instead of the five per-cell maps.
+template <int ND, bool EXPO, bool CODED, bool STORE_MU>
+__device__ inline void adj_pressure_cell(const DevSim& s, const AdjView& w, int64_t c, float mu, float div) {
     if (EXPO) {
         const float A = s.aexp[c];
         const float t = -A * s.dt * s.K[c] * mu;
 #pragma unroll
-        for (int a = 0; a < ND; ++a) {
-            div += dminus<ND, UNI>(s, w.fv[a], c, a, st);
-            w.H[a][c] = t;
-        }
+        for (int a = 0; a < ND; ++a) w.H[a][c] = t;
         w.lp[c] = A * mu;
         w.gK[c] += mu * (-A * s.dt) * div;
         return;
     }
     const float t = -s.dt * s.K[c] * mu;
     CodedCoef kc{};
-    if (CODED) kc = cell_coef<false, UNI>(s, (uint32_t)c);
+    if (CODED) kc = cell_coef<false, true>(s, (uint32_t)c);
 #pragma unroll
     for (int a = 0; a < ND; ++a) {
-        const float h = dminus<ND, UNI>(s, w.fv[a], c, a, st);
         const float ik = CODED ? kc.ik : s.ikp[a][c];
         float lh = t * ik;
 #pragma unroll
@@ -140,10 +104,36 @@ __global__ __launch_bounds__(256) void k_adj_pressure(DevSim s, AdjView w) {
             lh += an * tot;
             w.lphi[a][nu][c] = bn * tot;
         }
-        div += h * ik + (w.fphi[a][0][c] + w.fphi[a][1][c]);
         w.H[a][c] = lh;
     }
+    if (STORE_MU) w.lp[c] = mu;
     w.gK[c] += mu * (-s.dt) * div;
+}
+
+template <int ND, bool EXPO, bool CODED>
+__global__ __launch_bounds__(256) void k_adj_pressure(DevSim s, AdjView w) {
+    int64_t i, j, k;
+    if (!adj_cell<ND>(s, i, j, k)) return;
+    const int64_t c = i * s.sx + j * s.sy + k;
+    adj_pressure_cell<ND, EXPO, CODED, false>(s, w, c, w.lp[c], w.tdiv[c]);
+}
+
+// lambda_p[c] of the next step after its sparse part (as k_adj_sparse), from lp = lambda_p[c]
+// after this step's gathers
+__device__ inline float next_mu(const AdjNext& nx, int64_t c, float lp) {
+    const unsigned kd = nx.kind[c];
+    if (!kd) return lp;
+    if ((kd & 1u) || ((kd & 2u) && nx.injecting)) return 0.f;
+    if (!(kd & 4u) || !nx.grow) return lp;
+    int64_t lo = 0, hi = nx.nvox - 1;  // c is one of the sorted voxels
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (nx.vox[mid] < c) lo = mid + 1;
+        else hi = mid;
+    }
+    float acc = 0.f;
+    for (int32_t q = nx.first[lo]; q < nx.first[lo + 1]; ++q) acc += nx.grow[nx.order[q]];
+    return lp + acc;
 }
 
 // lambda_v_a += (D-_a)^T H_a at active cells
@@ -177,44 +167,44 @@ __global__ __launch_bounds__(256) void k_adj_dminus_t(DevSim s, AdjView w) {
     }
 }
 
-template <int ND, bool EXPO, bool UNI, bool CODED>
+// The velocity transposed of axis a at active cell c, with nu = lambda_v_a[c] (already holding
+// the gathered H) and kc the cell's coded coefficients (CODED).
+template <bool EXPO, bool CODED>
+__device__ inline void adj_velocity_axis(const DevSim& s, const AdjView& w, int64_t c, int a, int64_t sa, float nu,
+                                         const CodedCoef& kc) {
+    const float rb = 0.5f * (s.rho[c] + s.rho[c + sa]);
+    const float beta = s.dt / rb;
+    const float u = w.tu[a][c];
+    float lg;
+    if (EXPO) {
+        lg = -s.aexp[c] * beta * nu;
+        w.lv[a][c] = s.aexp[c] * nu;
+    } else {
+        const float ik = CODED ? kc.ik : s.ikv[a][c];
+        lg = -beta * nu * ik;
+#pragma unroll
+        for (int q = 0; q < kNRelax; ++q) {
+            const float tot = w.lpsi[a][q][c] - beta * nu;
+            const float aq = CODED ? (q ? kc.a2 : kc.a1) : s.av[a][q][c];
+            const float bq = CODED ? (q ? kc.b2 : kc.b1) : s.bv[a][q][c];
+            lg += aq * tot;
+            w.lpsi[a][q][c] = bq * tot;
+        }
+    }
+    w.G[a][c] = lg;
+    w.gRb[a][c] += nu * u * s.dt / (rb * rb);
+}
+
+template <int ND, bool EXPO, bool CODED>
 __global__ __launch_bounds__(256) void k_adj_velocity(DevSim s, AdjView w) {
     int64_t i, j, k;
     if (!adj_cell<ND>(s, i, j, k)) return;
     const int64_t c = i * s.sx + j * s.sy + k;
     const int64_t st[3] = {s.sx, s.sy, 1};
-    const float rho_c = s.rho[c];
     CodedCoef kc{};
-    if (CODED) kc = cell_coef<true, UNI>(s, (uint32_t)c);
+    if (CODED) kc = cell_coef<true, true>(s, (uint32_t)c);
 #pragma unroll
-    for (int a = 0; a < ND; ++a) {
-        const int64_t sa = st[a];
-        const float nu = w.lv[a][c];
-        const float rb = 0.5f * (rho_c + s.rho[c + sa]);
-        const float beta = s.dt / rb;
-        const float g = dplus<ND, UNI>(s, w.fp_prev, c, a, st);
-        float u, lg;
-        if (EXPO) {
-            const float A = s.aexp[c];
-            u = A * g;
-            lg = -A * beta * nu;
-            w.lv[a][c] = A * nu;
-        } else {
-            const float ik = CODED ? kc.ik : s.ikv[a][c];
-            u = g * ik + (w.fpsi[a][0][c] + w.fpsi[a][1][c]);
-            lg = -beta * nu * ik;
-#pragma unroll
-            for (int q = 0; q < kNRelax; ++q) {
-                const float tot = w.lpsi[a][q][c] - beta * nu;
-                const float aq = CODED ? (q ? kc.a2 : kc.a1) : s.av[a][q][c];
-                const float bq = CODED ? (q ? kc.b2 : kc.b1) : s.bv[a][q][c];
-                lg += aq * tot;
-                w.lpsi[a][q][c] = bq * tot;
-            }
-        }
-        w.G[a][c] = lg;
-        w.gRb[a][c] += nu * u * s.dt / (rb * rb);
-    }
+    for (int a = 0; a < ND; ++a) adj_velocity_axis<EXPO, CODED>(s, w, c, a, st[a], w.lv[a][c], kc);
 }
 
 // lambda_p += sum_a (D+_a)^T G_a at active cells
@@ -248,47 +238,90 @@ __global__ __launch_bounds__(256) void k_adj_dplus_t(DevSim s, AdjView w) {
     w.lp[x] += acc;
 }
 
-// The two transposed gathers of a 2D step on tiles, with the gathered field (H or G), the
+// The transposed gathers of a 2D step on tiles, with the gathered field (H or G), the
 // coefficient-set index of every cell it comes from and the coefficient table staged in LDS (the
 // per-cell kernels above read each neighbour's index and weights from global memory). A
 // workgroup of 256 lanes owns a tile of ATX rows (x) by ATY columns (y); lane (r, j) updates
 // rows r, r + 4, ... of column j. Axis x's field is staged over rows [i0 - 8, i0 + ATX + 8) and
 // columns [j0 - 1, j0 + ATY + 1), axis y's over rows [i0 - 1, i0 + ATX + 1) and columns
 // [j0 - 8, j0 + ATY + 8): every offset either gather reads (along its axis -8..+8, across it
-// -1..+1). Cells outside the grid stage 0 (H and G are 0 off the updated cells anyway). The
+// -1..+1). Cells outside the grid stage 0 (H and G are 0 off the updated cells anyway). Every
+// lane issues all of its staging loads before it stores any of them, so one memory latency
+// covers the staging. The coefficient table takes the handle's sets only (dynamic LDS). The
 // arithmetic and its order are the per-cell kernels', so the results are the same bit for bit.
-constexpr int ATX = 16, ATY = 64;
+constexpr int ATX = 16, ATY = 64, kAdjLanes = 256;
 constexpr int AXR = ATX + 16, AXC = ATY + 2;  // axis-x staged region
 constexpr int AYR = ATX + 2, AYC = ATY + 16;  // axis-y staged region
-constexpr int kAdjLdsSets = 256;              // coefficient sets staged (ndmap beyond: per-cell kernels)
+constexpr int NSX = (AXR * AXC + kAdjLanes - 1) / kAdjLanes;  // staged elements per lane
+constexpr int NSY = (AYR * AYC + kAdjLanes - 1) / kAdjLanes;
+constexpr int kAdjLdsSets = 256;  // coefficient sets staged (ndmap beyond: per-cell kernels)
+constexpr int NCT = (kAdjLdsSets * 2 * kNCoef + kAdjLanes - 1) / kAdjLanes;
 
-// DMINUS: lambda_v_a += (D-_a)^T H_a (k_adj_dminus_t); else lambda_p += sum_a (D+_a)^T G_a
-template <bool UNI, bool DMINUS>
-__global__ __launch_bounds__(256) void k_adj_gather_tile2d(DevSim s, AdjView w) {
+template <bool UNI>
+__device__ inline void stage_gather(const DevSim& s, const float* fx, const float* fy, int64_t i0, int64_t j0,
+                                    float* FX, float* FY, int16_t* DX, int16_t* DY, float* CT) {
+    const int nct = (UNI ? 1 : s.ndmap) * 2 * kNCoef;
+    float vx[NSX], vy[NSY], vc[NCT];
+    int32_t dx[NSX], dy[NSY];
+#pragma unroll
+    for (int q = 0; q < NSX; ++q) {
+        const int t = threadIdx.x + q * kAdjLanes;
+        const int r = t / AXC, cc = t - r * AXC;
+        const int64_t i = i0 - 8 + r, j = j0 - 1 + cc;
+        const bool in = t < AXR * AXC && i >= 0 && i < s.nx && j >= 0 && j < s.ny;
+        vx[q] = in ? fx[i * s.sx + j] : 0.f;
+        if (!UNI) dx[q] = in ? s.dcmap[i * s.sx + j] : 0;
+    }
+#pragma unroll
+    for (int q = 0; q < NSY; ++q) {
+        const int t = threadIdx.x + q * kAdjLanes;
+        const int r = t / AYC, cc = t - r * AYC;
+        const int64_t i = i0 - 1 + r, j = j0 - 8 + cc;
+        const bool in = t < AYR * AYC && i >= 0 && i < s.nx && j >= 0 && j < s.ny;
+        vy[q] = in ? fy[i * s.sx + j] : 0.f;
+        if (!UNI) dy[q] = in ? s.dcmap[i * s.sx + j] : 0;
+    }
+#pragma unroll
+    for (int q = 0; q < NCT; ++q) {
+        const int t = threadIdx.x + q * kAdjLanes;
+        vc[q] = t < nct ? s.coef[t] : 0.f;
+    }
+#pragma unroll
+    for (int q = 0; q < NSX; ++q) {
+        const int t = threadIdx.x + q * kAdjLanes;
+        if (t < AXR * AXC) {
+            FX[t] = vx[q];
+            if (!UNI) DX[t] = (int16_t)dx[q];
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < NSY; ++q) {
+        const int t = threadIdx.x + q * kAdjLanes;
+        if (t < AYR * AYC) {
+            FY[t] = vy[q];
+            if (!UNI) DY[t] = (int16_t)dy[q];
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < NCT; ++q) {
+        const int t = threadIdx.x + q * kAdjLanes;
+        if (t < nct) CT[t] = vc[q];
+    }
+}
+
+// DMINUS: lambda_v_a += (D-_a)^T H_a, then the velocity transposed of the cell (EXPO, CODED as
+// k_adj_velocity); else lambda_p += sum_a (D+_a)^T G_a, then with NEXT the next step's sparse
+// part and pressure transposed of the cell (EXPO, CODED as k_adj_pressure; w.tdiv is then the
+// next step's tape)
+template <bool UNI, bool DMINUS, bool EXPO, bool CODED, bool NEXT>
+__global__ __launch_bounds__(kAdjLanes) void k_adj_gather_tile2d(DevSim s, AdjView w, AdjNext nx) {
     __shared__ float FX[AXR * AXC];
     __shared__ float FY[AYR * AYC];
     __shared__ int16_t DX[UNI ? 1 : AXR * AXC];
     __shared__ int16_t DY[UNI ? 1 : AYR * AYC];
-    __shared__ float CT[UNI ? 2 * kNCoef : kAdjLdsSets * 2 * kNCoef];  // [set][axis][9]
-    const float* fx = DMINUS ? w.H[0] : w.G[0];
-    const float* fy = DMINUS ? w.H[1] : w.G[1];
+    extern __shared__ float CT[];  // [set][axis][9], the handle's sets
     const int64_t i0 = kHalo + (int64_t)blockIdx.y * ATX, j0 = kHalo + (int64_t)blockIdx.x * ATY;
-    const int nsets = UNI ? 1 : s.ndmap;
-    for (int t = threadIdx.x; t < nsets * 2 * kNCoef; t += 256) CT[t] = s.coef[t];
-    for (int t = threadIdx.x; t < AXR * AXC; t += 256) {
-        const int r = t / AXC, cc = t - r * AXC;
-        const int64_t i = i0 - 8 + r, j = j0 - 1 + cc;
-        const bool in = i >= 0 && i < s.nx && j >= 0 && j < s.ny;
-        FX[t] = in ? fx[i * s.sx + j] : 0.f;
-        if (!UNI) DX[t] = in ? (int16_t)s.dcmap[i * s.sx + j] : 0;
-    }
-    for (int t = threadIdx.x; t < AYR * AYC; t += 256) {
-        const int r = t / AYC, cc = t - r * AYC;
-        const int64_t i = i0 - 1 + r, j = j0 - 8 + cc;
-        const bool in = i >= 0 && i < s.nx && j >= 0 && j < s.ny;
-        FY[t] = in ? fy[i * s.sx + j] : 0.f;
-        if (!UNI) DY[t] = in ? (int16_t)s.dcmap[i * s.sx + j] : 0;
-    }
+    stage_gather<UNI>(s, DMINUS ? w.H[0] : w.G[0], DMINUS ? w.H[1] : w.G[1], i0, j0, FX, FY, DX, DY, CT);
     __syncthreads();
     const int jl = threadIdx.x % ATY, rl = threadIdx.x / ATY;
     const int64_t j = j0 + jl;
@@ -315,7 +348,7 @@ __global__ __launch_bounds__(256) void k_adj_gather_tile2d(DevSim s, AdjView w) 
                 const int e1 = ex(il, 0, -sg), e2 = ex(il, 1, -sg);
                 acc += wx(e1, 8) * FX[e1] - wx(e2, 8) * FX[e2];
             }
-            w.lv[0][x] += acc;
+            const float nu0 = w.lv[0][x] + acc;
             acc = 0.f;
 #pragma unroll
             for (int m = 1; m <= 8; ++m) {
@@ -327,7 +360,16 @@ __global__ __launch_bounds__(256) void k_adj_gather_tile2d(DevSim s, AdjView w) 
                 const int e1 = ey(il, -sg, 0), e2 = ey(il, -sg, 1);
                 acc += wy(e1, 8) * FY[e1] - wy(e2, 8) * FY[e2];
             }
-            w.lv[1][x] += acc;
+            const float nu1 = w.lv[1][x] + acc;
+            // the velocity transposed; lambda_v keeps nu (the exponential model rescales it)
+            if (!EXPO) {
+                w.lv[0][x] = nu0;
+                w.lv[1][x] = nu1;
+            }
+            CodedCoef kc{};
+            if (CODED) kc = cell_coef<true, true>(s, (uint32_t)x);
+            adj_velocity_axis<EXPO, CODED>(s, w, x, 0, s.sx, nu0, kc);
+            adj_velocity_axis<EXPO, CODED>(s, w, x, 1, 1, nu1, kc);
         } else {
             // axis x: c1 = x - m e_x, c2 = x + (m-1) e_x; transverse c1 = x - e_x -+ e_y, c2 = x -+ e_y
             float acc = 0.f;
@@ -351,7 +393,12 @@ __global__ __launch_bounds__(256) void k_adj_gather_tile2d(DevSim s, AdjView w) 
                 const int e1 = ey(il, -sg, -1), e2 = ey(il, -sg, 0);
                 acc += wy(e1, 8) * FY[e1] - wy(e2, 8) * FY[e2];
             }
-            w.lp[x] += acc;
+            if (NEXT) {
+                const float mu = next_mu(nx, x, w.lp[x] + acc);
+                adj_pressure_cell<2, EXPO, CODED, true>(s, w, x, mu, nx.tdiv[x]);
+            } else {
+                w.lp[x] += acc;
+            }
         }
     }
 }
@@ -384,20 +431,13 @@ static dim3 adj_grid(const DevSim& s) {
     return dim3((unsigned)((s.nz + 63) / 64), (unsigned)((s.ny + 3) / 4), (unsigned)s.nx);
 }
 
-hipError_t launch_adj_records(float* lp, const float* grow, const int64_t* vox, const int32_t* first,
-                              const int32_t* order, int64_t nvox, hipStream_t st) {
-    if (nvox == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_adj_records, dim3((unsigned)((nvox + 255) / 256)), dim3(256), 0, st, lp, grow, vox, first,
-                       order, nvox);
-    return hipGetLastError();
-}
-
-hipError_t launch_adj_mask(float* lp, const int64_t* src_lin, int64_t nsrc, const int64_t* zero_lin, int64_t nzero,
-                           hipStream_t st) {
-    const int64_t n = nsrc > nzero ? nsrc : nzero;
+hipError_t launch_adj_sparse(float* lp, const float* grow, const AdjSparse& sp, bool injecting, hipStream_t st) {
+    int64_t n = sp.nzero;
+    if (grow && sp.nvox > n) n = sp.nvox;
+    if (injecting && sp.nsrc > n) n = sp.nsrc;
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_adj_mask, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, lp, src_lin, nsrc, zero_lin,
-                       nzero);
+    hipLaunchKernelGGL(k_adj_sparse, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, lp, grow, sp,
+                       injecting ? 1 : 0);
     return hipGetLastError();
 }
 
@@ -405,44 +445,76 @@ hipError_t launch_adj_mask(float* lp, const int64_t* src_lin, int64_t nsrc, cons
 // staged set indices are 16-bit)
 bool adj_tiles(const DevSim& s) { return s.ndim == 2 && s.ndmap <= kAdjLdsSets; }
 
-// FULLWAVE_HIP_ADJ_GATHERS=percell keeps the per-cell gathers (to compare the two forms)
+// FULLWAVE_HIP_ADJ_GATHERS=percell keeps the per-cell kernels (to compare the two forms)
 bool adj_percell_gathers() {
     const char* e = getenv("FULLWAVE_HIP_ADJ_GATHERS");
     return e && !strncmp(e, "percell", 7);
 }
 
-template <bool DMINUS>
-void launch_gather_tile2d(const DevSim& s, const AdjView& w, hipStream_t st) {
+bool adj_fuses_next(const DevSim& s) { return adj_tiles(s) && !adj_percell_gathers(); }
+
+template <bool DMINUS, bool EXPO, bool CODED, bool NEXT>
+void launch_gather_tile2d(const DevSim& s, const AdjView& w, const AdjNext& nx, hipStream_t st) {
     const dim3 g((unsigned)((s.ny - 2 * kHalo + ATY - 1) / ATY), (unsigned)((s.nx - 2 * kHalo + ATX - 1) / ATX));
-    if (s.ndmap == 1) hipLaunchKernelGGL((k_adj_gather_tile2d<true, DMINUS>), g, dim3(256), 0, st, s, w);
-    else hipLaunchKernelGGL((k_adj_gather_tile2d<false, DMINUS>), g, dim3(256), 0, st, s, w);
+    const bool uni = s.ndmap == 1;
+    const size_t lds = (size_t)(uni ? 1 : s.ndmap) * 2 * kNCoef * sizeof(float);
+    if (uni)
+        hipLaunchKernelGGL((k_adj_gather_tile2d<true, DMINUS, EXPO, CODED, NEXT>), g, dim3(kAdjLanes), lds, st, s, w, nx);
+    else
+        hipLaunchKernelGGL((k_adj_gather_tile2d<false, DMINUS, EXPO, CODED, NEXT>), g, dim3(kAdjLanes), lds, st, s, w, nx);
+}
+
+static bool adj_coded(const DevSim& s) { return !s.aexp && s.isotropic && s.rcode; }
+
+template <int ND>
+void launch_pressure_nd(const DevSim& s, const AdjView& w, dim3 g, dim3 b, hipStream_t st) {
+    if (s.aexp) hipLaunchKernelGGL((k_adj_pressure<ND, true, false>), g, b, 0, st, s, w);
+    else if (adj_coded(s)) hipLaunchKernelGGL((k_adj_pressure<ND, false, true>), g, b, 0, st, s, w);
+    else hipLaunchKernelGGL((k_adj_pressure<ND, false, false>), g, b, 0, st, s, w);
 }
 
 template <int ND, bool UNI>
-void launch_adj_step_nd(const DevSim& s, const AdjView& w, dim3 g, dim3 b, hipStream_t st) {
-    const bool coded = !s.aexp && s.isotropic && s.rcode;
-    const bool tiles = adj_tiles(s) && !adj_percell_gathers();
-    if (s.aexp) hipLaunchKernelGGL((k_adj_pressure<ND, true, UNI, false>), g, b, 0, st, s, w);
-    else if (coded) hipLaunchKernelGGL((k_adj_pressure<ND, false, UNI, true>), g, b, 0, st, s, w);
-    else hipLaunchKernelGGL((k_adj_pressure<ND, false, UNI, false>), g, b, 0, st, s, w);
-    if (tiles) launch_gather_tile2d<true>(s, w, st);
-    else hipLaunchKernelGGL((k_adj_dminus_t<ND, UNI>), g, b, 0, st, s, w);
-    if (s.aexp) hipLaunchKernelGGL((k_adj_velocity<ND, true, UNI, false>), g, b, 0, st, s, w);
-    else if (coded) hipLaunchKernelGGL((k_adj_velocity<ND, false, UNI, true>), g, b, 0, st, s, w);
-    else hipLaunchKernelGGL((k_adj_velocity<ND, false, UNI, false>), g, b, 0, st, s, w);
-    if (tiles) launch_gather_tile2d<false>(s, w, st);
-    else hipLaunchKernelGGL((k_adj_dplus_t<ND, UNI>), g, b, 0, st, s, w);
+void launch_rest_nd(const DevSim& s, const AdjView& w, dim3 g, dim3 b, hipStream_t st) {
+    hipLaunchKernelGGL((k_adj_dminus_t<ND, UNI>), g, b, 0, st, s, w);
+    if (s.aexp) hipLaunchKernelGGL((k_adj_velocity<ND, true, false>), g, b, 0, st, s, w);
+    else if (adj_coded(s)) hipLaunchKernelGGL((k_adj_velocity<ND, false, true>), g, b, 0, st, s, w);
+    else hipLaunchKernelGGL((k_adj_velocity<ND, false, false>), g, b, 0, st, s, w);
+    hipLaunchKernelGGL((k_adj_dplus_t<ND, UNI>), g, b, 0, st, s, w);
 }
 
-hipError_t launch_adj_step(const DevSim& s, const AdjView& w, hipStream_t st) {
+hipError_t launch_adj_pressure(const DevSim& s, const AdjView& w, hipStream_t st) {
+    const dim3 g = adj_grid(s), b(64, 4);
+    if (s.ndim == 2) launch_pressure_nd<2>(s, w, g, b, st);
+    else launch_pressure_nd<3>(s, w, g, b, st);
+    return hipGetLastError();
+}
+
+hipError_t launch_adj_rest(const DevSim& s, const AdjView& w, const AdjNext* next, hipStream_t st) {
+    if (adj_fuses_next(s)) {
+        const AdjNext none{};
+        if (s.aexp) launch_gather_tile2d<true, true, false, false>(s, w, none, st);
+        else if (adj_coded(s)) launch_gather_tile2d<true, false, true, false>(s, w, none, st);
+        else launch_gather_tile2d<true, false, false, false>(s, w, none, st);
+        if (!next) {
+            launch_gather_tile2d<false, false, false, false>(s, w, none, st);
+        } else {
+            AdjView wn = w;  // the next step's pressure transposed reads its own tape
+            wn.tdiv = next->tdiv;
+            if (s.aexp) launch_gather_tile2d<false, true, false, true>(s, wn, *next, st);
+            else if (adj_coded(s)) launch_gather_tile2d<false, false, true, true>(s, wn, *next, st);
+            else launch_gather_tile2d<false, false, false, true>(s, wn, *next, st);
+        }
+        return hipGetLastError();
+    }
+    if (next) return hipErrorInvalidValue;  // the per-cell kernels do not take the next step
     const dim3 g = adj_grid(s), b(64, 4);
     const bool u = s.ndmap == 1;
     if (s.ndim == 2) {
-        if (u) launch_adj_step_nd<2, true>(s, w, g, b, st);
-        else launch_adj_step_nd<2, false>(s, w, g, b, st);
+        if (u) launch_rest_nd<2, true>(s, w, g, b, st);
+        else launch_rest_nd<2, false>(s, w, g, b, st);
     } else {
-        if (u) launch_adj_step_nd<3, true>(s, w, g, b, st);
-        else launch_adj_step_nd<3, false>(s, w, g, b, st);
+        if (u) launch_rest_nd<3, true>(s, w, g, b, st);
+        else launch_rest_nd<3, false>(s, w, g, b, st);
     }
     return hipGetLastError();
 }

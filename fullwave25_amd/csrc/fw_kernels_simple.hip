@@ -23,7 +23,8 @@ __device__ inline bool cell_of_thread(const DevSim& s, unsigned zl, int64_t& i, 
            k < s.nz - kHalo;
 }
 
-template <int ND, bool EXPO>
+// TAPE: also write the adjoint tape of the step (DevSim::tape_u, tape_div)
+template <int ND, bool EXPO, bool TAPE>
 __global__ __launch_bounds__(256) void k_velocity_simple(DevSim batch) {
     unsigned zl;
     const DevSim s = shot_view(batch, shot_of_block(batch, zl));
@@ -53,17 +54,20 @@ __global__ __launch_bounds__(256) void k_velocity_simple(DevSim batch) {
         const float rho_bar = 0.5f * (rho_c + s.rho[c + sa]);
         if (EXPO) {
             s.v[a][c] = decay * (s.v[a][c] - s.dt / rho_bar * D);
+            if (TAPE) s.tape_u[a][c] = decay * D;
             continue;
         }
         const float q1 = s.bv[a][0][c] * s.psi[a][0][c] + s.av[a][0][c] * D;
         const float q2 = s.bv[a][1][c] * s.psi[a][1][c] + s.av[a][1][c] * D;
         s.psi[a][0][c] = q1;
         s.psi[a][1][c] = q2;
-        s.v[a][c] -= s.dt / rho_bar * (D * s.ikv[a][c] + (q1 + q2));
+        const float u = D * s.ikv[a][c] + (q1 + q2);
+        s.v[a][c] -= s.dt / rho_bar * u;
+        if (TAPE) s.tape_u[a][c] = u;
     }
 }
 
-template <int ND, bool EXPO>
+template <int ND, bool EXPO, bool TAPE>
 __global__ __launch_bounds__(256) void k_pressure_simple(DevSim batch) {
     unsigned zl;
     const DevSim s = shot_view(batch, shot_of_block(batch, zl));
@@ -103,6 +107,7 @@ __global__ __launch_bounds__(256) void k_pressure_simple(DevSim batch) {
         s.p[c] = s.aexp[c] * (s.p[c] - s.dt * s.K[c] * div);
     else
         s.p[c] -= s.dt * s.K[c] * div;
+    if (TAPE) s.tape_div[c] = div;
 }
 
 // Source injection, then pressure release, then sensor gather, for step `n`. Sources that
@@ -243,21 +248,31 @@ static dim3 simple_grid(const DevSim& s) {
     return dim3((unsigned)((s.nz + 63) / 64), (unsigned)((s.ny + 3) / 4), (unsigned)(s.nx * s.nshots));
 }
 
-hipError_t launch_velocity_simple(const DevSim& s, hipStream_t st) {
+template <bool TAPE>
+static void launch_simple(const DevSim& s, bool velocity, hipStream_t st) {
     const dim3 g = simple_grid(s), b(64, 4);
-    if (s.ndim == 2 && s.aexp) hipLaunchKernelGGL((k_velocity_simple<2, true>), g, b, 0, st, s);
-    else if (s.ndim == 2) hipLaunchKernelGGL((k_velocity_simple<2, false>), g, b, 0, st, s);
-    else if (s.aexp) hipLaunchKernelGGL((k_velocity_simple<3, true>), g, b, 0, st, s);
-    else hipLaunchKernelGGL((k_velocity_simple<3, false>), g, b, 0, st, s);
+    if (velocity) {
+        if (s.ndim == 2 && s.aexp) hipLaunchKernelGGL((k_velocity_simple<2, true, TAPE>), g, b, 0, st, s);
+        else if (s.ndim == 2) hipLaunchKernelGGL((k_velocity_simple<2, false, TAPE>), g, b, 0, st, s);
+        else if (s.aexp) hipLaunchKernelGGL((k_velocity_simple<3, true, TAPE>), g, b, 0, st, s);
+        else hipLaunchKernelGGL((k_velocity_simple<3, false, TAPE>), g, b, 0, st, s);
+    } else {
+        if (s.ndim == 2 && s.aexp) hipLaunchKernelGGL((k_pressure_simple<2, true, TAPE>), g, b, 0, st, s);
+        else if (s.ndim == 2) hipLaunchKernelGGL((k_pressure_simple<2, false, TAPE>), g, b, 0, st, s);
+        else if (s.aexp) hipLaunchKernelGGL((k_pressure_simple<3, true, TAPE>), g, b, 0, st, s);
+        else hipLaunchKernelGGL((k_pressure_simple<3, false, TAPE>), g, b, 0, st, s);
+    }
+}
+
+hipError_t launch_velocity_simple(const DevSim& s, hipStream_t st) {
+    if (s.tape_div) launch_simple<true>(s, true, st);
+    else launch_simple<false>(s, true, st);
     return hipGetLastError();
 }
 
 hipError_t launch_pressure_simple(const DevSim& s, hipStream_t st) {
-    const dim3 g = simple_grid(s), b(64, 4);
-    if (s.ndim == 2 && s.aexp) hipLaunchKernelGGL((k_pressure_simple<2, true>), g, b, 0, st, s);
-    else if (s.ndim == 2) hipLaunchKernelGGL((k_pressure_simple<2, false>), g, b, 0, st, s);
-    else if (s.aexp) hipLaunchKernelGGL((k_pressure_simple<3, true>), g, b, 0, st, s);
-    else hipLaunchKernelGGL((k_pressure_simple<3, false>), g, b, 0, st, s);
+    if (s.tape_div) launch_simple<true>(s, false, st);
+    else launch_simple<false>(s, false, st);
     return hipGetLastError();
 }
 

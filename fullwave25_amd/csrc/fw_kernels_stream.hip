@@ -454,8 +454,15 @@ bool streaming_supported(const DevSim& s) {
 bool tile2d_supported(const DevSim& s);
 hipError_t launch_tile2d(const DevSim& s, const LaunchCfg& cfg, bool velocity, hipStream_t st);
 
+bool tape_capable(const DevSim& s, const LaunchCfg& cfg) {
+    return cfg.variant == 0 || (cfg.variant == 2 && tile2d_supported(s));
+}
+
+// A step that writes the adjoint tape runs the tiled 2D kernels when `cfg` asks for them and the
+// per-cell kernels otherwise (the only families that write it).
 hipError_t launch_velocity(const DevSim& s, const LaunchCfg& cfg, hipStream_t st) {
     if (cfg.variant == 2 && tile2d_supported(s)) return launch_tile2d(s, cfg, true, st);
+    if (s.tape_div) return launch_velocity_simple(s, st);
     if ((cfg.variant == 3 || cfg.variant == 4) && s.ndim == 3 && streaming_supported(s))
         return launch_pipe3d(s, cfg, true, st);
     if (cfg.variant == 0 || !streaming_supported(s)) return launch_velocity_simple(s, st);
@@ -465,6 +472,7 @@ hipError_t launch_velocity(const DevSim& s, const LaunchCfg& cfg, hipStream_t st
 
 hipError_t launch_pressure(const DevSim& s, const LaunchCfg& cfg, hipStream_t st) {
     if (cfg.variant == 2 && tile2d_supported(s)) return launch_tile2d(s, cfg, false, st);
+    if (s.tape_div) return launch_pressure_simple(s, st);
     if ((cfg.variant == 3 || cfg.variant == 4) && s.ndim == 3 && streaming_supported(s))
         return launch_pipe3d(s, cfg, false, st);
     if (cfg.variant == 0 || !streaming_supported(s)) return launch_pressure_simple(s, st);

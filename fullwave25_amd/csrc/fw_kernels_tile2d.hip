@@ -169,8 +169,9 @@ struct VelTile {
 };
 
 // The velocity update of tile (bx, by) of grid s: columns [bx * 256, bx * 256 + 256), rows
-// [8 + by * TX, 8 + by * TX + TX).
-template <int TX, bool UNIFORM, bool ISO, bool CODED>
+// [8 + by * TX, 8 + by * TX + TX). TAPE: also the adjoint tape (DevSim::tape_u; 0 in the ghost
+// columns of a quad).
+template <int TX, bool UNIFORM, bool ISO, bool CODED, bool TAPE = false>
 __device__ inline void velocity_tile(const DevSim& s, unsigned bx, unsigned by, VelTile<TX>& L, float* lds_coef) {
     constexpr int PITCH = VelTile<TX>::PITCH;
     constexpr int NROWS = VelTile<TX>::NROWS;
@@ -201,8 +202,10 @@ __device__ inline void velocity_tile(const DevSim& s, unsigned bx, unsigned by, 
         constexpr int NA = ISO ? 1 : 2;
         quad_coef<CODED, UNIFORM, true, NA>(s, c, ik, b1, b2, a1, a2, dc);
         F4 ps1[2], ps2[2], v[2];
+        F4 tu[2];
 #pragma unroll
         for (int a = 0; a < 2; ++a) {
+            if (TAPE) tu[a] = F4{0.f, 0.f, 0.f, 0.f};
             ps1[a] = ld4nt(s.psi[a][0], c);
             ps2[a] = ld4nt(s.psi[a][1], c);
             v[a] = ld4(s.v[a], c);
@@ -227,7 +230,9 @@ __device__ inline void velocity_tile(const DevSim& s, unsigned bx, unsigned by, 
                 ps1[0][e] = q1;
                 ps2[0][e] = q2;
                 const float rho_bar = 0.5f * (rho0[e] + rho1[e]);
-                v[0][e] -= dt / rho_bar * (D * ik[0][e] + (q1 + q2));
+                const float u = D * ik[0][e] + (q1 + q2);
+                v[0][e] -= dt / rho_bar * u;
+                if (TAPE) tu[0][e] = u;
             }
             {
                 constexpr int ax = ISO ? 0 : 1;
@@ -244,7 +249,9 @@ __device__ inline void velocity_tile(const DevSim& s, unsigned bx, unsigned by, 
                 ps2[1][e] = q2;
                 const float rho_right = e < 3 ? rho0[e + 1] : rho_r;
                 const float rho_bar = 0.5f * (rho0[e] + rho_right);
-                v[1][e] -= dt / rho_bar * (D * ik[ax][e] + (q1 + q2));
+                const float u = D * ik[ax][e] + (q1 + q2);
+                v[1][e] -= dt / rho_bar * u;
+                if (TAPE) tu[1][e] = u;
             }
         }
         // cells outside the active columns keep their loaded values, so whole quads store back
@@ -253,6 +260,7 @@ __device__ inline void velocity_tile(const DevSim& s, unsigned bx, unsigned by, 
             st4nt(s.psi[a][0], c, ps1[a]);
             st4nt(s.psi[a][1], c, ps2[a]);
             st4(s.v[a], c, v[a]);
+            if (TAPE) st4(s.tape_u[a], c, tu[a]);
         }
     }
 }
@@ -269,8 +277,8 @@ struct PreTile {
     float VY[YR * YP];
 };
 
-// The pressure update of tile (bx, by) of grid s (as velocity_tile).
-template <int TX, bool UNIFORM, bool ISO, bool CODED>
+// The pressure update of tile (bx, by) of grid s (as velocity_tile; TAPE: DevSim::tape_div).
+template <int TX, bool UNIFORM, bool ISO, bool CODED, bool TAPE = false>
 __device__ inline void pressure_tile(const DevSim& s, unsigned bx, unsigned by, PreTile<TX>& L, float* lds_coef) {
     constexpr int XP = PreTile<TX>::XP;
     constexpr int XR = PreTile<TX>::XR;
@@ -304,6 +312,7 @@ __device__ inline void pressure_tile(const DevSim& s, unsigned bx, unsigned by, 
         F4 ik[2], b1[2], b2[2], a1[2], a2[2];
         quad_coef<CODED, UNIFORM, false, NA>(s, c, ik, b1, b2, a1, a2, dc);
         F4 ph1[2], ph2[2];
+        F4 td = F4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int a = 0; a < 2; ++a) {
             ph1[a] = ld4nt(s.phi[a][0], c);
@@ -348,8 +357,10 @@ __device__ inline void pressure_tile(const DevSim& s, unsigned bx, unsigned by, 
                 div += D * ik[ax][e] + (q1 + q2);
             }
             pv[e] -= dt * K[e] * div;
+            if (TAPE) td[e] = div;
         }
         st4(s.p, c, pv);
+        if (TAPE) st4(s.tape_div, c, td);
 #pragma unroll
         for (int a = 0; a < 2; ++a) {
             st4nt(s.phi[a][0], c, ph1[a]);
@@ -358,18 +369,18 @@ __device__ inline void pressure_tile(const DevSim& s, unsigned bx, unsigned by, 
     }
 }
 
-template <int TX, bool UNIFORM, bool ISO, bool CODED>
+template <int TX, bool UNIFORM, bool ISO, bool CODED, bool TAPE>
 __global__ __launch_bounds__(kLanes) void k_velocity_tile(DevSim batch) {
     __shared__ __attribute__((aligned(16))) VelTile<TX> L;
     extern __shared__ __attribute__((aligned(16))) float lds_coef[];
-    velocity_tile<TX, UNIFORM, ISO, CODED>(shot_view(batch, blockIdx.z), blockIdx.x, blockIdx.y, L, lds_coef);
+    velocity_tile<TX, UNIFORM, ISO, CODED, TAPE>(shot_view(batch, blockIdx.z), blockIdx.x, blockIdx.y, L, lds_coef);
 }
 
-template <int TX, bool UNIFORM, bool ISO, bool CODED>
+template <int TX, bool UNIFORM, bool ISO, bool CODED, bool TAPE>
 __global__ __launch_bounds__(kLanes) void k_pressure_tile(DevSim batch) {
     __shared__ __attribute__((aligned(16))) PreTile<TX> L;
     extern __shared__ __attribute__((aligned(16))) float lds_coef[];
-    pressure_tile<TX, UNIFORM, ISO, CODED>(shot_view(batch, blockIdx.z), blockIdx.x, blockIdx.y, L, lds_coef);
+    pressure_tile<TX, UNIFORM, ISO, CODED, TAPE>(shot_view(batch, blockIdx.z), blockIdx.x, blockIdx.y, L, lds_coef);
 }
 
 // grouped launches: the tiles of several grids (different sizes and media, isotropic
@@ -413,7 +424,7 @@ hipError_t launch_group_tile_any(const GroupTile* t, int n, uint32_t blocks, int
                      : launch_group_tile<TX, false>(t, n, blocks, lds, velocity, st);
 }
 
-template <int TX>
+template <int TX, bool TAPE = false>
 hipError_t launch_tile(const DevSim& s, bool velocity, hipStream_t st) {
     const bool u = s.ndmap == 1;
     const int lds = (u ? 1 : s.ndmap) * (s.coef_axes_same ? 1 : 2) * kCS * 4;
@@ -422,19 +433,19 @@ hipError_t launch_tile(const DevSim& s, bool velocity, hipStream_t st) {
     const dim3 b(kLanes);
     const bool coded = s.isotropic && s.rcode;
     if (velocity) {
-        if (u && coded) hipLaunchKernelGGL((k_velocity_tile<TX, true, true, true>), g, b, lds, st, s);
-        else if (u && s.isotropic) hipLaunchKernelGGL((k_velocity_tile<TX, true, true, false>), g, b, lds, st, s);
-        else if (u) hipLaunchKernelGGL((k_velocity_tile<TX, true, false, false>), g, b, lds, st, s);
-        else if (coded) hipLaunchKernelGGL((k_velocity_tile<TX, false, true, true>), g, b, lds, st, s);
-        else if (s.isotropic) hipLaunchKernelGGL((k_velocity_tile<TX, false, true, false>), g, b, lds, st, s);
-        else hipLaunchKernelGGL((k_velocity_tile<TX, false, false, false>), g, b, lds, st, s);
+        if (u && coded) hipLaunchKernelGGL((k_velocity_tile<TX, true, true, true, TAPE>), g, b, lds, st, s);
+        else if (u && s.isotropic) hipLaunchKernelGGL((k_velocity_tile<TX, true, true, false, TAPE>), g, b, lds, st, s);
+        else if (u) hipLaunchKernelGGL((k_velocity_tile<TX, true, false, false, TAPE>), g, b, lds, st, s);
+        else if (coded) hipLaunchKernelGGL((k_velocity_tile<TX, false, true, true, TAPE>), g, b, lds, st, s);
+        else if (s.isotropic) hipLaunchKernelGGL((k_velocity_tile<TX, false, true, false, TAPE>), g, b, lds, st, s);
+        else hipLaunchKernelGGL((k_velocity_tile<TX, false, false, false, TAPE>), g, b, lds, st, s);
     } else {
-        if (u && coded) hipLaunchKernelGGL((k_pressure_tile<TX, true, true, true>), g, b, lds, st, s);
-        else if (u && s.isotropic) hipLaunchKernelGGL((k_pressure_tile<TX, true, true, false>), g, b, lds, st, s);
-        else if (u) hipLaunchKernelGGL((k_pressure_tile<TX, true, false, false>), g, b, lds, st, s);
-        else if (coded) hipLaunchKernelGGL((k_pressure_tile<TX, false, true, true>), g, b, lds, st, s);
-        else if (s.isotropic) hipLaunchKernelGGL((k_pressure_tile<TX, false, true, false>), g, b, lds, st, s);
-        else hipLaunchKernelGGL((k_pressure_tile<TX, false, false, false>), g, b, lds, st, s);
+        if (u && coded) hipLaunchKernelGGL((k_pressure_tile<TX, true, true, true, TAPE>), g, b, lds, st, s);
+        else if (u && s.isotropic) hipLaunchKernelGGL((k_pressure_tile<TX, true, true, false, TAPE>), g, b, lds, st, s);
+        else if (u) hipLaunchKernelGGL((k_pressure_tile<TX, true, false, false, TAPE>), g, b, lds, st, s);
+        else if (coded) hipLaunchKernelGGL((k_pressure_tile<TX, false, true, true, TAPE>), g, b, lds, st, s);
+        else if (s.isotropic) hipLaunchKernelGGL((k_pressure_tile<TX, false, true, false, TAPE>), g, b, lds, st, s);
+        else hipLaunchKernelGGL((k_pressure_tile<TX, false, false, false, TAPE>), g, b, lds, st, s);
     }
     return hipGetLastError();
 }
@@ -470,6 +481,7 @@ hipError_t launch_group_tile2d(const GroupTile* t, int n, uint32_t blocks, int r
 }
 
 hipError_t launch_tile2d(const DevSim& s, const LaunchCfg& cfg, bool velocity, hipStream_t st) {
+    if (s.tape_div) return launch_tile<8, true>(s, velocity, st);  // taped steps: 8-row tiles
     switch (cfg.chunk) {
         case 8: return launch_tile<8>(s, velocity, st);
         case 32: return launch_tile<32>(s, velocity, st);

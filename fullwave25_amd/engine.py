@@ -36,7 +36,7 @@ EXPORTED_SYMBOLS = (
     "fw_sim_set_relaxation_codes", "fw_sim_relaxation_codes", "fw_sim_run_timed",
     "fw_sim_destroy", "fw_group_create", "fw_group_config", "fw_group_create_shaped", "fw_group_shape",
     "fw_group_run", "fw_group_set_graph_steps", "fw_group_destroy",
-    "fw_adj_create", "fw_adj_forward", "fw_adj_backward", "fw_adj_destroy",
+    "fw_adj_create", "fw_adj_checkpoint_every", "fw_adj_forward", "fw_adj_backward", "fw_adj_destroy",
 )
 VARIANTS = {"percell": 0, "stream": 1, "tile": 2, "pipe": 3, "pipelite": 4}
 
@@ -110,6 +110,7 @@ def load_library(path: Path | None = None) -> ctypes.CDLL:
         lib.fw_group_destroy.argtypes = [vp]
         lib.fw_group_set_graph_steps.argtypes = [vp, i64]
         lib.fw_adj_create.argtypes = [vp, i64, ctypes.POINTER(vp)]
+        lib.fw_adj_checkpoint_every.argtypes = [vp, ctypes.POINTER(i64)]
         lib.fw_adj_forward.argtypes = [vp]
         lib.fw_adj_backward.argtypes = [vp, vp, sz, vp, vp, sz]
         lib.fw_adj_destroy.argtypes = [vp]

@@ -78,14 +78,15 @@ class _Job:
 
 
 def adjoint_device_bytes(inp: SimulationInputs, checkpoint_every: int) -> int:
-    """Device memory of an adjoint job: the forward handle, its checkpoints, one segment of
-    states, the adjoint state and scratch."""
+    """Device memory of an adjoint job with segments of ``checkpoint_every`` steps (> 0): the
+    forward handle, its checkpoints, the tape of one segment (ndim + 1 grids a step), the
+    adjoint state and scratch."""
     nd = inp.ndim
     cells = inp.nx * inp.ny * (-(-inp.nz // 32) * 32) if nd == 3 else inp.ncells
     state = (1 + nd + (0 if inp.exponential else 4 * nd)) * cells
     every = max(1, min(checkpoint_every, inp.nt))
     n_ckpt = -(-inp.nt // every)
-    extra = (n_ckpt + every + 1) * state + (3 * nd + 2) * cells + inp.n_records * inp.ncoordsout
+    extra = (n_ckpt + 1) * state + every * (nd + 1) * cells + (3 * nd + 2) * cells + inp.n_records * inp.ncoordsout
     return engine.device_bytes(inp) + 4 * extra
 
 
@@ -301,8 +302,12 @@ class SimulationServer:
     def kernel_config_for(self, inp: SimulationInputs, n_shots: int = 1, *, adjoint: bool = False) -> dict | None:
         """The kernel configuration this server runs (or ran) jobs of this shape with: the group
         shape for grouped forward jobs, else the one tuned on the first handle of the shape (None
-        if no such job has run yet). Misfit-gradient jobs (``adjoint``) never join a group."""
-        if not adjoint and self._grouped(inp, n_shots):
+        if no such job has run yet). Misfit-gradient jobs (``adjoint``) never join a group and
+        keep their own configurations (their handles run kernels that write the adjoint tape)."""
+        if adjoint:
+            with self._lock:
+                return self._configs.get(_shape_key(inp, 1) + ("adjoint",))
+        if self._grouped(inp, n_shots):
             shape = _group_shape(inp)
             return {"velocity": shape, "pressure": shape}
         with self._lock:
@@ -396,7 +401,7 @@ class SimulationServer:
         from .adjoint import HipAdjoint
 
         inp = job.inputs
-        key = _shape_key(inp, 1)
+        key = _shape_key(inp, 1) + ("adjoint",)
         with self._lock:
             cfg = self._configs.get(key)
         # the adjoint handle fixes its kernels when it is created: pass a known configuration in

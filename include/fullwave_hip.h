@@ -224,12 +224,21 @@ int fw_group_destroy(fw_group* group);
 /* None of these has a counterpart in the reference, which runs forward simulations only.
  * For J = sum_{n,j} w[n][j] * rec[n][j] (rec: the traces of a full run from rest, w: e.g. the
  * residual of an L2 misfit), the sweep returns dJ/dK and dJ/drho per voxel, the coefficient
- * sets (dcmap, dmap) held fixed. The forward run keeps its state every `checkpoint_every` steps;
- * the backward sweep recomputes one segment at a time from its checkpoint. */
+ * sets (dcmap, dmap) held fixed. The sweep reads a tape of the forward run (ndim + 1 grids a
+ * step: each velocity update's corrected gradient per axis and the pressure update's corrected
+ * divergence). The run is cut into segments of `checkpoint_every` steps: the forward run keeps
+ * its state before each and tapes the last; the backward sweep recomputes the tape of every
+ * other segment from its checkpoint. checkpoint_every >= nT tapes the whole run, with no
+ * recomputation. */
 typedef struct fw_adj fw_adj;
 /* Bind to a single-transmit handle whose inputs are uploaded (n_shots 1); allocates the
- * checkpoints (ceil(nT / checkpoint_every) states), one segment of states and scratch. */
+ * checkpoints (ceil(nT / checkpoint_every) states), the tape of one segment and scratch.
+ * checkpoint_every 0: the whole run when its tape fits in half of the device's free memory,
+ * else the segment length that minimises checkpoints plus tape. The handle's velocity and
+ * pressure kernels become the tiled 2D or per-cell ones if it had chosen others. */
 int fw_adj_create(fw_sim* sim, int64_t checkpoint_every, fw_adj** out);
+/* The segment length in use (as chosen for checkpoint_every 0). */
+int fw_adj_checkpoint_every(const fw_adj* adj, int64_t* every);
 /* Reset `sim`, run its nT steps from rest keeping checkpoints; the traces are then available
  * through fw_sim_download_sensors. */
 int fw_adj_forward(fw_adj* adj);
