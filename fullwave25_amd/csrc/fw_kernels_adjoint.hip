@@ -28,6 +28,7 @@
 // is its own per-cell kernel.
 #include <cstdlib>
 #include <cstring>
+#include <type_traits>
 
 #include "fw_internal.h"
 
@@ -74,54 +75,99 @@ __global__ __launch_bounds__(256) void k_adj_sparse(float* lp, const float* grow
     if (t < sp.nzero) lp[sp.zero_lin[t]] = 0.f;
 }
 
-// The pressure transposed at active cell c with mu = lambda_p[c] (after the step's sparse part)
-// and div the step's tape; leaves lambda_p[c] = mu (the exponential model: A mu). CODED
-// (isotropic relaxation with codes): the relaxation coefficients of the cell come from its code's
-// table row (2 bytes a cell) instead of the five per-cell maps.
-template <int ND, bool EXPO, bool CODED, bool STORE_MU>
-__device__ inline void adj_pressure_cell(const DevSim& s, const AdjView& w, int64_t c, float mu, float div) {
+// The pointwise steps take their operands from a struct loaded ahead of the arithmetic: the tiled
+// kernels load a row's operands before the gathers that precede their use, so that the loads are
+// in flight during the gathers. The per-cell kernels use the same structs and the same arithmetic.
+
+// the relaxation coefficients of the cell once its code is in (CODED; escaped cells read the maps)
+template <bool VEL>
+__device__ inline CodedCoef decode_cell(const DevSim& s, int64_t c, uint32_t code) {
+    if (code != kCodeEscape) return coded_coef(VEL ? s.rtab_v : s.rtab_p, code);
+    if (VEL) return mapped_coef<true>(s.ikv[0], s.av[0][0], s.av[0][1], s.bv[0][0], s.bv[0][1], s.dcmap, (uint32_t)c);
+    return mapped_coef<true>(s.ikp[0], s.ap[0][0], s.ap[0][1], s.bp[0][0], s.bp[0][1], s.dcmap, (uint32_t)c);
+}
+
+// operands of the pressure transposed at one cell
+template <int ND, bool EXPO, bool CODED>
+struct PreOps {
+    float lp, K, div, gk, A;
+    float phi[ND][kNRelax];
+    CodedCoef kc[ND];
+    uint32_t code, kind;
+};
+
+template <int ND, bool EXPO, bool CODED>
+__device__ inline void pre_load(const DevSim& s, const AdjView& w, int64_t c, PreOps<ND, EXPO, CODED>& o) {
+    o.lp = w.lp[c];
+    o.K = s.K[c];
+    o.div = w.tdiv[c];
+    o.gk = w.gK[c];
     if (EXPO) {
-        const float A = s.aexp[c];
-        const float t = -A * s.dt * s.K[c] * mu;
-#pragma unroll
-        for (int a = 0; a < ND; ++a) w.H[a][c] = t;
-        w.lp[c] = A * mu;
-        w.gK[c] += mu * (-A * s.dt) * div;
+        o.A = s.aexp[c];
         return;
     }
-    const float t = -s.dt * s.K[c] * mu;
-    CodedCoef kc{};
-    if (CODED) kc = cell_coef<false, true>(s, (uint32_t)c);
 #pragma unroll
     for (int a = 0; a < ND; ++a) {
-        const float ik = CODED ? kc.ik : s.ikp[a][c];
-        float lh = t * ik;
+#pragma unroll
+        for (int nu = 0; nu < kNRelax; ++nu) o.phi[a][nu] = w.lphi[a][nu][c];
+        if (!CODED) o.kc[a] = CodedCoef{s.ikp[a][c], s.ap[a][0][c], s.ap[a][1][c], s.bp[a][0][c], s.bp[a][1][c], 0};
+    }
+    if (CODED) o.code = gld16(s.rcode, (uint32_t)c);
+}
+
+template <int ND, bool EXPO, bool CODED>
+__device__ inline void pre_decode(const DevSim& s, int64_t c, PreOps<ND, EXPO, CODED>& o) {
+    if (!CODED || EXPO) return;
+    const CodedCoef k = decode_cell<false>(s, c, o.code);
+#pragma unroll
+    for (int a = 0; a < ND; ++a) o.kc[a] = k;
+}
+
+// The pressure transposed at active cell c with mu = lambda_p[c] after the step's sparse part;
+// leaves lambda_p[c] = mu (STORE_MU; the exponential model always stores A mu).
+template <int ND, bool EXPO, bool CODED, bool STORE_MU>
+__device__ inline void pre_store(const DevSim& s, const AdjView& w, int64_t c, float mu, const PreOps<ND, EXPO, CODED>& o) {
+    if (EXPO) {
+        const float t = -o.A * s.dt * o.K * mu;
+#pragma unroll
+        for (int a = 0; a < ND; ++a) w.H[a][c] = t;
+        w.lp[c] = o.A * mu;
+        w.gK[c] = o.gk + mu * (-o.A * s.dt) * o.div;
+        return;
+    }
+    const float t = -s.dt * o.K * mu;
+#pragma unroll
+    for (int a = 0; a < ND; ++a) {
+        const CodedCoef& k = o.kc[a];
+        float lh = t * k.ik;
 #pragma unroll
         for (int nu = 0; nu < kNRelax; ++nu) {
-            const float tot = w.lphi[a][nu][c] + t;
-            const float an = CODED ? (nu ? kc.a2 : kc.a1) : s.ap[a][nu][c];
-            const float bn = CODED ? (nu ? kc.b2 : kc.b1) : s.bp[a][nu][c];
-            lh += an * tot;
-            w.lphi[a][nu][c] = bn * tot;
+            const float tot = o.phi[a][nu] + t;
+            lh += (nu ? k.a2 : k.a1) * tot;
+            w.lphi[a][nu][c] = (nu ? k.b2 : k.b1) * tot;
         }
         w.H[a][c] = lh;
     }
     if (STORE_MU) w.lp[c] = mu;
-    w.gK[c] += mu * (-s.dt) * div;
+    w.gK[c] = o.gk + mu * (-s.dt) * o.div;
 }
 
+// CODED (isotropic relaxation with codes): the relaxation coefficients of the cell come from its
+// code's table row (2 bytes a cell) instead of the five per-cell maps
 template <int ND, bool EXPO, bool CODED>
 __global__ __launch_bounds__(256) void k_adj_pressure(DevSim s, AdjView w) {
     int64_t i, j, k;
     if (!adj_cell<ND>(s, i, j, k)) return;
     const int64_t c = i * s.sx + j * s.sy + k;
-    adj_pressure_cell<ND, EXPO, CODED, false>(s, w, c, w.lp[c], w.tdiv[c]);
+    PreOps<ND, EXPO, CODED> o;
+    pre_load(s, w, c, o);
+    pre_decode(s, c, o);
+    pre_store<ND, EXPO, CODED, false>(s, w, c, o.lp, o);
 }
 
 // lambda_p[c] of the next step after its sparse part (as k_adj_sparse), from lp = lambda_p[c]
-// after this step's gathers
-__device__ inline float next_mu(const AdjNext& nx, int64_t c, float lp) {
-    const unsigned kd = nx.kind[c];
+// after this step's gathers and kd = the cell's kind
+__device__ inline float next_mu(const AdjNext& nx, int64_t c, unsigned kd, float lp) {
     if (!kd) return lp;
     if ((kd & 1u) || ((kd & 2u) && nx.injecting)) return 0.f;
     if (!(kd & 4u) || !nx.grow) return lp;
@@ -167,32 +213,74 @@ __global__ __launch_bounds__(256) void k_adj_dminus_t(DevSim s, AdjView w) {
     }
 }
 
-// The velocity transposed of axis a at active cell c, with nu = lambda_v_a[c] (already holding
-// the gathered H) and kc the cell's coded coefficients (CODED).
-template <bool EXPO, bool CODED>
-__device__ inline void adj_velocity_axis(const DevSim& s, const AdjView& w, int64_t c, int a, int64_t sa, float nu,
-                                         const CodedCoef& kc) {
-    const float rb = 0.5f * (s.rho[c] + s.rho[c + sa]);
-    const float beta = s.dt / rb;
-    const float u = w.tu[a][c];
-    float lg;
-    if (EXPO) {
-        lg = -s.aexp[c] * beta * nu;
-        w.lv[a][c] = s.aexp[c] * nu;
-    } else {
-        const float ik = CODED ? kc.ik : s.ikv[a][c];
-        lg = -beta * nu * ik;
+// operands of the velocity transposed at one cell (nu: lambda_v as loaded)
+template <int ND, bool EXPO, bool CODED>
+struct VelOps {
+    float nu[ND], rc, rn[ND], u[ND], grb[ND], A;
+    float psi[ND][kNRelax];
+    CodedCoef kc[ND];
+    uint32_t code;
+};
+
+template <int ND, bool EXPO, bool CODED>
+__device__ inline void vel_load(const DevSim& s, const AdjView& w, int64_t c, VelOps<ND, EXPO, CODED>& o) {
+    const int64_t st[3] = {s.sx, s.sy, 1};
+    o.rc = s.rho[c];
 #pragma unroll
-        for (int q = 0; q < kNRelax; ++q) {
-            const float tot = w.lpsi[a][q][c] - beta * nu;
-            const float aq = CODED ? (q ? kc.a2 : kc.a1) : s.av[a][q][c];
-            const float bq = CODED ? (q ? kc.b2 : kc.b1) : s.bv[a][q][c];
-            lg += aq * tot;
-            w.lpsi[a][q][c] = bq * tot;
-        }
+    for (int a = 0; a < ND; ++a) {
+        o.nu[a] = w.lv[a][c];
+        o.rn[a] = s.rho[c + st[a]];
+        o.u[a] = w.tu[a][c];
+        o.grb[a] = w.gRb[a][c];
     }
-    w.G[a][c] = lg;
-    w.gRb[a][c] += nu * u * s.dt / (rb * rb);
+    if (EXPO) {
+        o.A = s.aexp[c];
+        return;
+    }
+#pragma unroll
+    for (int a = 0; a < ND; ++a) {
+#pragma unroll
+        for (int q = 0; q < kNRelax; ++q) o.psi[a][q] = w.lpsi[a][q][c];
+        if (!CODED) o.kc[a] = CodedCoef{s.ikv[a][c], s.av[a][0][c], s.av[a][1][c], s.bv[a][0][c], s.bv[a][1][c], 0};
+    }
+    if (CODED) o.code = gld16(s.rcode, (uint32_t)c);
+}
+
+template <int ND, bool EXPO, bool CODED>
+__device__ inline void vel_decode(const DevSim& s, int64_t c, VelOps<ND, EXPO, CODED>& o) {
+    if (!CODED || EXPO) return;
+    const CodedCoef k = decode_cell<true>(s, c, o.code);
+#pragma unroll
+    for (int a = 0; a < ND; ++a) o.kc[a] = k;
+}
+
+// The velocity transposed at active cell c with o.nu[a] = nu (lambda_v_a after the gather);
+// STORE_NU: also store lambda_v = nu (the exponential model always stores A nu).
+template <int ND, bool EXPO, bool CODED, bool STORE_NU>
+__device__ inline void vel_store(const DevSim& s, const AdjView& w, int64_t c, const VelOps<ND, EXPO, CODED>& o) {
+#pragma unroll
+    for (int a = 0; a < ND; ++a) {
+        const float nu = o.nu[a];
+        const float rb = 0.5f * (o.rc + o.rn[a]);
+        const float beta = s.dt / rb;
+        float lg;
+        if (EXPO) {
+            lg = -o.A * beta * nu;
+            w.lv[a][c] = o.A * nu;
+        } else {
+            const CodedCoef& k = o.kc[a];
+            lg = -beta * nu * k.ik;
+#pragma unroll
+            for (int q = 0; q < kNRelax; ++q) {
+                const float tot = o.psi[a][q] - beta * nu;
+                lg += (q ? k.a2 : k.a1) * tot;
+                w.lpsi[a][q][c] = (q ? k.b2 : k.b1) * tot;
+            }
+            if (STORE_NU) w.lv[a][c] = nu;
+        }
+        w.G[a][c] = lg;
+        w.gRb[a][c] = o.grb[a] + nu * o.u[a] * s.dt / (rb * rb);
+    }
 }
 
 template <int ND, bool EXPO, bool CODED>
@@ -200,11 +288,10 @@ __global__ __launch_bounds__(256) void k_adj_velocity(DevSim s, AdjView w) {
     int64_t i, j, k;
     if (!adj_cell<ND>(s, i, j, k)) return;
     const int64_t c = i * s.sx + j * s.sy + k;
-    const int64_t st[3] = {s.sx, s.sy, 1};
-    CodedCoef kc{};
-    if (CODED) kc = cell_coef<true, true>(s, (uint32_t)c);
-#pragma unroll
-    for (int a = 0; a < ND; ++a) adj_velocity_axis<EXPO, CODED>(s, w, c, a, st[a], w.lv[a][c], kc);
+    VelOps<ND, EXPO, CODED> o;
+    vel_load(s, w, c, o);
+    vel_decode(s, c, o);
+    vel_store<ND, EXPO, CODED, false>(s, w, c, o);
 }
 
 // lambda_p += sum_a (D+_a)^T G_a at active cells
@@ -312,7 +399,8 @@ __device__ inline void stage_gather(const DevSim& s, const float* fx, const floa
 // DMINUS: lambda_v_a += (D-_a)^T H_a, then the velocity transposed of the cell (EXPO, CODED as
 // k_adj_velocity); else lambda_p += sum_a (D+_a)^T G_a, then with NEXT the next step's sparse
 // part and pressure transposed of the cell (EXPO, CODED as k_adj_pressure; w.tdiv is then the
-// next step's tape)
+// next step's tape). Each lane loads the operands of its first row before the staging and those
+// of row m + 1 before the gathers of row m, so their memory latency overlaps other work.
 template <bool UNI, bool DMINUS, bool EXPO, bool CODED, bool NEXT>
 __global__ __launch_bounds__(kAdjLanes) void k_adj_gather_tile2d(DevSim s, AdjView w, AdjNext nx) {
     __shared__ float FX[AXR * AXC];
@@ -320,63 +408,77 @@ __global__ __launch_bounds__(kAdjLanes) void k_adj_gather_tile2d(DevSim s, AdjVi
     __shared__ int16_t DX[UNI ? 1 : AXR * AXC];
     __shared__ int16_t DY[UNI ? 1 : AYR * AYC];
     extern __shared__ float CT[];  // [set][axis][9], the handle's sets
+    constexpr int NR = ATX / 4;    // rows per lane: rl, rl + 4, ...
     const int64_t i0 = kHalo + (int64_t)blockIdx.y * ATX, j0 = kHalo + (int64_t)blockIdx.x * ATY;
-    stage_gather<UNI>(s, DMINUS ? w.H[0] : w.G[0], DMINUS ? w.H[1] : w.G[1], i0, j0, FX, FY, DX, DY, CT);
-    __syncthreads();
     const int jl = threadIdx.x % ATY, rl = threadIdx.x / ATY;
     const int64_t j = j0 + jl;
-    if (j >= s.ny - kHalo) return;
+    // this lane's rows that hold updated cells
+    const int64_t left = s.nx - kHalo - (i0 + rl);
+    const int nrows = j >= s.ny - kHalo || left <= 0 ? 0 : (int)(left >= 4 * NR ? NR : (left + 3) / 4);
+    using Ops = std::conditional_t<DMINUS, VelOps<2, EXPO, CODED>, PreOps<2, EXPO, CODED>>;
+    Ops ops[2];
+    auto load = [&](int m, Ops& o) {
+        const int64_t x = (i0 + rl + 4 * m) * s.sx + j;
+        if constexpr (DMINUS) {
+            vel_load(s, w, x, o);
+        } else if constexpr (NEXT) {
+            pre_load(s, w, x, o);
+            o.kind = nx.kind[x];
+        } else {
+            o.lp = w.lp[x];
+        }
+    };
+    if (nrows > 0) load(0, ops[0]);
+    stage_gather<UNI>(s, DMINUS ? w.H[0] : w.G[0], DMINUS ? w.H[1] : w.G[1], i0, j0, FX, FY, DX, DY, CT);
+    __syncthreads();
     // staged element at (di, dj) from cell (il, jl) of the tile, for axis x's and axis y's field
     auto ex = [&](int il, int di, int dj) { return (il + 8 + di) * AXC + (jl + 1 + dj); };
     auto ey = [&](int il, int di, int dj) { return (il + 1 + di) * AYC + (jl + 8 + dj); };
     auto wx = [&](int e, int k) { return CT[(UNI ? 0 : (int)DX[e] * 2) * kNCoef + k]; };             // axis 0 set
     auto wy = [&](int e, int k) { return CT[(UNI ? kNCoef : ((int)DY[e] * 2 + 1) * kNCoef) + k]; };  // axis 1 set
-    for (int il = rl; il < ATX; il += 4) {
-        const int64_t i = i0 + il;
-        if (i >= s.nx - kHalo) break;
-        const int64_t x = i * s.sx + j;
-        if (DMINUS) {
+#pragma unroll
+    for (int m = 0; m < NR; ++m) {
+        if (m >= nrows) break;
+        Ops& o = ops[m & 1];
+        const int il = rl + 4 * m;
+        const int64_t x = (i0 + il) * s.sx + j;
+        if constexpr (DMINUS) vel_decode(s, x, o);
+        else if constexpr (NEXT) pre_decode(s, x, o);
+        if (m + 1 < nrows) load(m + 1, ops[(m + 1) & 1]);
+        if constexpr (DMINUS) {
             // axis x: c1 = x - (m-1) e_x, c2 = x + m e_x; transverse c1 = x -+ e_y, c2 = x + e_x -+ e_y
             float acc = 0.f;
 #pragma unroll
-            for (int m = 1; m <= 8; ++m) {
-                const int e1 = ex(il, -(m - 1), 0), e2 = ex(il, m, 0);
-                acc += wx(e1, m - 1) * FX[e1] - wx(e2, m - 1) * FX[e2];
+            for (int q = 1; q <= 8; ++q) {
+                const int e1 = ex(il, -(q - 1), 0), e2 = ex(il, q, 0);
+                acc += wx(e1, q - 1) * FX[e1] - wx(e2, q - 1) * FX[e2];
             }
 #pragma unroll
             for (int sg = -1; sg <= 1; sg += 2) {
                 const int e1 = ex(il, 0, -sg), e2 = ex(il, 1, -sg);
                 acc += wx(e1, 8) * FX[e1] - wx(e2, 8) * FX[e2];
             }
-            const float nu0 = w.lv[0][x] + acc;
+            o.nu[0] = o.nu[0] + acc;
             acc = 0.f;
 #pragma unroll
-            for (int m = 1; m <= 8; ++m) {
-                const int e1 = ey(il, 0, -(m - 1)), e2 = ey(il, 0, m);
-                acc += wy(e1, m - 1) * FY[e1] - wy(e2, m - 1) * FY[e2];
+            for (int q = 1; q <= 8; ++q) {
+                const int e1 = ey(il, 0, -(q - 1)), e2 = ey(il, 0, q);
+                acc += wy(e1, q - 1) * FY[e1] - wy(e2, q - 1) * FY[e2];
             }
 #pragma unroll
             for (int sg = -1; sg <= 1; sg += 2) {
                 const int e1 = ey(il, -sg, 0), e2 = ey(il, -sg, 1);
                 acc += wy(e1, 8) * FY[e1] - wy(e2, 8) * FY[e2];
             }
-            const float nu1 = w.lv[1][x] + acc;
-            // the velocity transposed; lambda_v keeps nu (the exponential model rescales it)
-            if (!EXPO) {
-                w.lv[0][x] = nu0;
-                w.lv[1][x] = nu1;
-            }
-            CodedCoef kc{};
-            if (CODED) kc = cell_coef<true, true>(s, (uint32_t)x);
-            adj_velocity_axis<EXPO, CODED>(s, w, x, 0, s.sx, nu0, kc);
-            adj_velocity_axis<EXPO, CODED>(s, w, x, 1, 1, nu1, kc);
+            o.nu[1] = o.nu[1] + acc;
+            vel_store<2, EXPO, CODED, true>(s, w, x, o);
         } else {
             // axis x: c1 = x - m e_x, c2 = x + (m-1) e_x; transverse c1 = x - e_x -+ e_y, c2 = x -+ e_y
             float acc = 0.f;
 #pragma unroll
-            for (int m = 1; m <= 8; ++m) {
-                const int e1 = ex(il, -m, 0), e2 = ex(il, m - 1, 0);
-                acc += wx(e1, m - 1) * FX[e1] - wx(e2, m - 1) * FX[e2];
+            for (int q = 1; q <= 8; ++q) {
+                const int e1 = ex(il, -q, 0), e2 = ex(il, q - 1, 0);
+                acc += wx(e1, q - 1) * FX[e1] - wx(e2, q - 1) * FX[e2];
             }
 #pragma unroll
             for (int sg = -1; sg <= 1; sg += 2) {
@@ -384,21 +486,17 @@ __global__ __launch_bounds__(kAdjLanes) void k_adj_gather_tile2d(DevSim s, AdjVi
                 acc += wx(e1, 8) * FX[e1] - wx(e2, 8) * FX[e2];
             }
 #pragma unroll
-            for (int m = 1; m <= 8; ++m) {
-                const int e1 = ey(il, 0, -m), e2 = ey(il, 0, m - 1);
-                acc += wy(e1, m - 1) * FY[e1] - wy(e2, m - 1) * FY[e2];
+            for (int q = 1; q <= 8; ++q) {
+                const int e1 = ey(il, 0, -q), e2 = ey(il, 0, q - 1);
+                acc += wy(e1, q - 1) * FY[e1] - wy(e2, q - 1) * FY[e2];
             }
 #pragma unroll
             for (int sg = -1; sg <= 1; sg += 2) {
                 const int e1 = ey(il, -sg, -1), e2 = ey(il, -sg, 0);
                 acc += wy(e1, 8) * FY[e1] - wy(e2, 8) * FY[e2];
             }
-            if (NEXT) {
-                const float mu = next_mu(nx, x, w.lp[x] + acc);
-                adj_pressure_cell<2, EXPO, CODED, true>(s, w, x, mu, nx.tdiv[x]);
-            } else {
-                w.lp[x] += acc;
-            }
+            if constexpr (NEXT) pre_store<2, EXPO, CODED, true>(s, w, x, next_mu(nx, x, o.kind, o.lp + acc), o);
+            else w.lp[x] = o.lp + acc;
         }
     }
 }
