@@ -443,10 +443,12 @@ def bench_served(args) -> dict:
     """Poisson arrivals of the served mix through the simulation server, against the same jobs
     run one at a time on plain engine handles."""
     from fullwave25_amd import adjoint, engine, workloads
-    from fullwave25_amd.serving import SimulationServer
+    from fullwave25_amd.serving import SimulationServer, adjoint_segment
 
     t0 = time.time()
     jobs, _ = _served_jobs(args)
+    # gradient segments as the server picks them for its default budget (90 % of the free memory)
+    share = int(0.9 * engine.device_memory(0)[0]) // 4
     n3 = 96 if args.mix == "small" else args.nz
     w3 = workloads.plane_wave_3d(n3, n3, n3, nt=1000) if any(k == "forward3d" for k, _ in jobs) else None
     jobs = [(k, w3.inputs if k == "forward3d" else inp) for k, inp in jobs]
@@ -462,7 +464,7 @@ def bench_served(args) -> dict:
 
     def run_plain(kind, inp):
         if kind == "adjoint":  # the server's segment length (SimulationServer.submit_adjoint)
-            with adjoint.HipAdjoint(inp, checkpoint_every=200) as adj:
+            with adjoint.HipAdjoint(inp, checkpoint_every=adjoint_segment(inp, share)) as adj:
                 tr = adj.forward()
                 adj.backward((tr - observed[id(inp)]).astype(np.float32))
             return
@@ -503,7 +505,8 @@ def bench_served(args) -> dict:
         "config": {"workload": (f"{len(jobs)} jobs: 2D forward 256^2-1024^2 x 500-4000 steps, 512^2 misfit "
                                 f"gradients (1000 steps), 96^3 3D forward (1000 steps)" if args.mix == "small" else
                                 f"{len(jobs)} jobs: 2D forward 512^2-4096^2 x 1000-8000 steps, 1024^2 misfit "
-                                f"gradients (2000 steps, checkpoint every 200), {args.nz}^3 3D forward (1000 steps)")
+                                f"gradients (2000 steps, whole run taped when it fits a quarter of the budget), "
+                                f"{args.nz}^3 3D forward (1000 steps)")
                                + f"; Poisson arrivals at {args.arrival_factor} x the sequential service rate",
                    "mix": args.mix, "slots": args.slots, "jobs": len(jobs)},
     }

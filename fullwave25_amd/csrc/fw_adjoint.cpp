@@ -36,6 +36,7 @@ struct fw_adj {
     int32_t* order = nullptr;
     int8_t* vmask = nullptr;  // per distinct voxel: 1 pressure release, 2 source point, 0 neither
     uint8_t* kind = nullptr;  // per device cell (2D): bit 0 pressure release, 1 source point, 2 sensor voxel
+    uint8_t* dc8 = nullptr;   // per device cell: coefficient-set index (grids of the tiled reverse kernels)
     int64_t nvox = 0;
     bool forward_done = false;
 };
@@ -212,6 +213,10 @@ int fw_adj_create(fw_sim* f, int64_t checkpoint_every, fw_adj** out) {
              hipMemcpyAsync(a->vmask, vmask.data(), vmask.size(), hipMemcpyHostToDevice, f->stream) == hipSuccess;
     if (!kind.empty())
         ok = ok && hipMemcpyAsync(a->kind, kind.data(), kind.size(), hipMemcpyHostToDevice, f->stream) == hipSuccess;
+    if (ok && fw::adj_tiles(f->ds)) {
+        if (!dalloc((void**)&a->dc8, (size_t)n)) return cleanup(fw::set_error(FW_ERR_NOMEM, "adjoint set-index map"));
+        ok = fw::launch_adj_set_index(f->ds.dcmap, a->dc8, n, f->stream) == hipSuccess;
+    }
     ok = ok && hipMemcpyAsync(a->first, first.data(), first.size() * 4, hipMemcpyHostToDevice, f->stream) == hipSuccess;
     ok = ok && hipStreamSynchronize(f->stream) == hipSuccess;
     if (!ok) return cleanup(fw::set_error(FW_ERR_DEVICE, "upload of the sensor tables failed"));
@@ -279,6 +284,7 @@ int fw_adj_backward(fw_adj* a, const float* grad_rec, size_t grad_bytes, float* 
         w.gRb[ax] = a->work + (2 * nd + ax) * n;
     }
     w.gK = a->work + 3 * nd * n;
+    w.dc8 = a->dc8;
     float* grho = a->work + (3 * nd + 1) * n;
     fw::AdjSparse sp{};
     sp.vox = a->vox;
@@ -344,7 +350,7 @@ int fw_adj_destroy(fw_adj* a) {
     Guard g(a->f->device);
     if (a->f->stream) (void)hipStreamSynchronize(a->f->stream);
     for (void* p : {(void*)a->ckpt, (void*)a->tape, (void*)a->lam, (void*)a->work, (void*)a->grec, (void*)a->vox,
-                    (void*)a->first, (void*)a->order, (void*)a->vmask, (void*)a->kind})
+                    (void*)a->first, (void*)a->order, (void*)a->vmask, (void*)a->kind, (void*)a->dc8})
         if (p) (void)hipFree(p);
     delete a;
     return FW_OK;

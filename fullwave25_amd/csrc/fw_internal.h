@@ -188,6 +188,7 @@ struct AdjView {
     float* lphi[kMaxDim][kNRelax];
     const float* tu[kMaxDim];
     const float* tdiv;
+    const uint8_t* dc8;  // coefficient-set index per device cell (the tiled 2D kernels; ndmap <= 256)
     float* H[kMaxDim];
     float* G[kMaxDim];
     float* gK;               // d/dK, summed over steps
@@ -313,7 +314,10 @@ struct AdjNext {
 // One step of the sweep, after launch_adj_sparse: launch_adj_pressure (the pressure transposed),
 // then launch_adj_rest (the rest of the step; with `next` when adj_fuses_next, also the start of
 // the next step, whose launch_adj_sparse and launch_adj_pressure are then skipped).
+bool adj_tiles(const DevSim& s);  // the grid takes the tiled 2D reverse kernels
 bool adj_fuses_next(const DevSim& s);
+// the set-index map the tiled kernels read (AdjView::dc8), from dcmap; only where adj_fuses_next
+hipError_t launch_adj_set_index(const int32_t* dcmap, uint8_t* dc8, int64_t n, hipStream_t st);
 hipError_t launch_adj_pressure(const DevSim& s, const AdjView& w, hipStream_t st);
 hipError_t launch_adj_rest(const DevSim& s, const AdjView& w, const AdjNext* next, hipStream_t st);
 // the forward kernels that write a tape (DevSim::tape_div set): the per-cell kernels, and the

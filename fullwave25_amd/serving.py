@@ -31,6 +31,7 @@ Latency of a job is measured from ``submit`` to its result being available.
 
 from __future__ import annotations
 
+import math
 import threading
 import time
 from collections import deque
@@ -88,6 +89,17 @@ def adjoint_device_bytes(inp: SimulationInputs, checkpoint_every: int) -> int:
     n_ckpt = -(-inp.nt // every)
     extra = (n_ckpt + 1) * state + every * (nd + 1) * cells + (3 * nd + 2) * cells + inp.n_records * inp.ncoordsout
     return engine.device_bytes(inp) + 4 * extra
+
+
+def adjoint_segment(inp: SimulationInputs, share_bytes: int) -> int:
+    """Segment length of an adjoint job allowed ``share_bytes`` of device memory: the whole run
+    (taped in its forward run, nothing recomputed) when that fits, else the length that needs
+    the least memory (ceil(nT / C) checkpoints of the state against C steps of tape)."""
+    if adjoint_device_bytes(inp, inp.nt) <= share_bytes:
+        return int(inp.nt)
+    nd = inp.ndim
+    state = 1 + nd + (0 if inp.exponential else 4 * nd)
+    return int(min(inp.nt, max(1, math.ceil(math.sqrt(inp.nt * state / (nd + 1))))))
 
 
 @dataclass
@@ -191,10 +203,14 @@ class SimulationServer:
         return fut
 
     def submit_adjoint(self, inputs: SimulationInputs, observed: np.ndarray, *,
-                       checkpoint_every: int = 200) -> Future:
-        """Queue a misfit-gradient job: the result is ``{"loss", "grad_K", "grad_rho",
-        "traces"}`` for ``0.5 * ||traces - observed||^2`` (gradients with respect to the K and
-        rho maps of ``inputs``; see :mod:`fullwave25_amd.adjoint`)."""
+                       checkpoint_every: int = 0) -> Future:
+        """Queue a misfit-gradient job: the result is ``{"loss", "grad_K", "grad_rho", "traces",
+        "checkpoint_every"}`` for ``0.5 * ||traces - observed||^2`` (gradients with respect to the K and
+        rho maps of ``inputs``; see :mod:`fullwave25_amd.adjoint`). ``checkpoint_every=0``: the
+        whole run taped when that needs at most a quarter of the memory budget, else the segment
+        length that needs the least memory (:func:`adjoint_segment`)."""
+        if checkpoint_every <= 0:
+            checkpoint_every = adjoint_segment(inputs, self.budget // 4)
         need = adjoint_device_bytes(inputs, checkpoint_every)
         if need > self.budget:
             raise ValueError(f"job needs {need} bytes of device memory, the budget is {self.budget}")
@@ -412,7 +428,8 @@ class SimulationServer:
             traces = adj.forward()
             resid = traces.astype(np.float64) - job.observed
             gk, gr = adj.backward(resid.astype(np.float32))
-        return {"loss": 0.5 * float(np.sum(resid**2)), "grad_K": gk, "grad_rho": gr, "traces": traces}
+        return {"loss": 0.5 * float(np.sum(resid**2)), "grad_K": gk, "grad_rho": gr, "traces": traces,
+                "checkpoint_every": job.checkpoint_every}
 
     def _work(self, job: _Job) -> None:
         rec = job.record

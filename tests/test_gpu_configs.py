@@ -133,7 +133,7 @@ def _served():
             ("forward", fwd_small)]
     print("served mix: " + ", ".join(f"{k} {i.shape} x {i.nt}" for k, i in jobs), flush=True)
     with SimulationServer(max_slots=4) as srv:
-        futs = [srv.submit_adjoint(i, observed, checkpoint_every=200) if k == "adjoint" else srv.submit(i)
+        futs = [srv.submit_adjoint(i, observed) if k == "adjoint" else srv.submit(i)
                 for k, i in jobs]
         results = [f.result(timeout=600) for f in futs]
         configs = [srv.kernel_config_for(i, adjoint=k == "adjoint") for k, i in jobs]
@@ -148,7 +148,8 @@ def test_served_mix_equals_alone():
     jobs, results, configs, observed = _served()
     for (kind, inp), got, cfg in zip(jobs, results, configs):
         if kind == "adjoint":
-            with HipAdjoint(inp, checkpoint_every=200, kernel_config=cfg) as adj:
+            assert got["checkpoint_every"] == inp.nt  # the whole run's tape fits the server's budget
+            with HipAdjoint(inp, checkpoint_every=got["checkpoint_every"], kernel_config=cfg) as adj:
                 traces = adj.forward()
                 resid = traces.astype(np.float64) - observed
                 gk, gr = adj.backward(resid.astype(np.float32))
@@ -168,7 +169,7 @@ def test_served_mix_equals_alone():
 
 
 def test_served_gradient_against_fp64_adjoint():
-    """The 1024^2 misfit gradient (240 steps: a full 200-step segment and a partial one) against
+    """The 1024^2 misfit gradient (240 steps, the whole run taped as the server chose) against
     the FP64 host adjoint fed the same residual."""
     jobs, results, _, observed = _served()
     inp, got = jobs[2][1], results[2]

@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 from fullwave25_amd import engine
-from fullwave25_amd.serving import SimulationServer
+from fullwave25_amd.serving import SimulationServer, adjoint_device_bytes, adjoint_segment
 from tests.helpers import raw_inputs
 
 
@@ -105,3 +105,23 @@ def test_group_membership_and_shapes():
         assert srv.kernel_config_for(many)["velocity"]["variant"] == "stream"
     with SimulationServer(hbm_budget=10**9, batching=False) as srv:
         assert not srv._grouped(aligned, 1) and srv.kernel_config_for(aligned) is None
+
+
+def test_adjoint_segment_policy():
+    """A gradient job tapes its whole run when that fits its share of the budget; otherwise its
+    segment length minimises checkpoints (ceil(nT / C) states) plus tape (C steps of ndim + 1
+    grids), checked here against every C by brute force."""
+    import math
+
+    inp = raw_inputs((64, 48), nt=400, icc=[[20, 20]], icmat=np.ones((1, 400)), outc=[[21, 21]])
+    full = adjoint_device_bytes(inp, inp.nt)
+    assert adjoint_segment(inp, full) == inp.nt
+    c = adjoint_segment(inp, full - 1)
+    assert 1 <= c < inp.nt
+    state = 1 + 2 + 4 * 2
+
+    def floats(e):
+        return math.ceil(inp.nt / e) * state + e * 3
+
+    best = min(floats(e) for e in range(1, inp.nt + 1))
+    assert floats(c) <= 1.05 * best
