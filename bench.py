@@ -349,7 +349,8 @@ def bench_adjoint(args) -> dict:
     """The misfit-gradient job of the served mix on its own: 1024^2 interior, 2000 steps, the L2
     misfit of a smoothed initial model against the traces of a seeded heterogeneous true medium;
     the time of the forward run (which writes the tape and the checkpoints) and of the reverse
-    sweep, against a plain forward run of the same job. ``--checkpoint-every 0`` (default) lets
+    sweep, against a plain forward run of the same job, each from a handle that is set up (the
+    handles' creation, uploads and kernel tuning are outside all three times). ``--checkpoint-every 0`` (default) lets
     the engine choose the segment length (the whole run taped when it fits in half of the free
     device memory, so nothing is recomputed); the same job with 200-step segments (each segment
     but the last recomputed from its checkpoint) is timed beside it."""
@@ -362,11 +363,16 @@ def bench_adjoint(args) -> dict:
         observed = sim.sensors().astype(np.float64)
     with engine.HipSimulation(inp) as sim:
         sim.run(10)
+    # the plain run timed as the gradient's legs are: from a handle whose set-up (uploads, codes,
+    # kernel tuning) is done, to the downloaded traces; with the set-up beside it
     t0 = time.perf_counter()
     with engine.HipSimulation(inp) as sim:
+        sim.kernel_config()
+        t1 = time.perf_counter()
         sim.run(inp.nt)
         sim.sensors()
-    t_plain = time.perf_counter() - t0
+        t_plain = time.perf_counter() - t1
+    t_plain_setup = time.perf_counter() - t0
 
     def gradient(every: int):
         with adjoint.HipAdjoint(inp, checkpoint_every=every) as adj:
@@ -386,7 +392,8 @@ def bench_adjoint(args) -> dict:
         "metric": "misfit gradient (dJ/dK, dJ/drho): time vs a plain forward run (1 GPU)",
         "value": round((t_fwd + t_rev) / t_plain, 3), "unit": "forward-run equivalents", "n_gpus": 1,
         "higher_is_better": False, "dtype": "f32", "data": "synthetic",
-        "plain_forward_s": round(t_plain, 4), "forward_with_tape_s": round(t_fwd, 4),
+        "plain_forward_s": round(t_plain, 4), "plain_forward_with_setup_s": round(t_plain_setup, 4),
+        "forward_with_tape_s": round(t_fwd, 4),
         "reverse_sweep_s": round(t_rev, 4), "gradient_jobs_per_hour": round(3600 / (t_fwd + t_rev), 1),
         "checkpoint_every": every, "tape_gb": round(tape_gb, 2), "kernel_config": kcfg,
         "forward_glups": round(cells / t_plain / 1e9, 3), "finite": bool(np.isfinite(gk).all() and np.isfinite(gr).all()),
