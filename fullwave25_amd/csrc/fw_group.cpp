@@ -1,8 +1,10 @@
 // Grouped time steps: several engine handles (different grids and media) advanced together.
-// Every step is one velocity launch, one pressure launch and one source/air/sensor launch over
-// the tiles of all members (marching kernels in fw_kernels_stream.hip or tiled kernels in
-// fw_kernels_tile2d.hip; fw_kernels_simple.hip), so many small or uneven jobs fill the GPU as
-// one job does, without one launch per job and kernel.
+// Every step is one velocity launch and one pressure launch over the tiles of all members
+// (marching kernels in fw_kernels_stream.hip or tiled kernels in fw_kernels_tile2d.hip); each
+// workgroup of the pressure launch also injects the sources, zeroes the pressure-release points
+// and records the sensors of its own tile once the tile is updated (tile_boundary in
+// fw_internal.h), so many small or uneven jobs fill the GPU as one job does, with two launches
+// a step instead of three per job.
 #include <hip/hip_runtime.h>
 
 #include <set>
@@ -21,9 +23,9 @@ struct fw_group {
     std::vector<fw::GroupBoundary> bounds_h;
     fw::GroupTile* tiles = nullptr;
     fw::GroupBoundary* bounds = nullptr;
+    int32_t* points = nullptr;  // every member's points grouped by tile (GroupBoundary lists)
     int ntiles = 0;
     uint32_t blocks = 0;
-    int64_t max_points = 0;
     int variant = 1;  // 1: marching kernels (width lanes, chunk rows); 2: tiled (256 x chunk)
     int width = 256, chunk = 16, max_ndmap = 1;
     int coef_sets = 1;      // LDS coefficient sets of the tiled kernels (largest member)
@@ -34,6 +36,32 @@ struct fw_group {
     int64_t* base_dev = nullptr;
     hipGraphExec_t graph = nullptr;
 };
+
+namespace fw {
+
+void points_by_tile(int64_t nx, int64_t ny, unsigned gx, int tile_w, int rows, const std::vector<int64_t>& lin,
+                    std::vector<int32_t>* off, std::vector<int32_t>* idx) {
+    const unsigned gy = (unsigned)((nx - 2 * kHalo + rows - 1) / rows);
+    const size_t ntiles = (size_t)gx * gy;
+    std::vector<int64_t> owner(lin.size(), -1);
+    off->assign(ntiles + 1, 0);
+    for (size_t q = 0; q < lin.size(); ++q) {
+        if (lin[q] < 0) continue;
+        int64_t i = lin[q] / ny;
+        const int64_t j = lin[q] % ny;
+        i = i < kHalo ? kHalo : (i > nx - kHalo - 1 ? nx - kHalo - 1 : i);
+        const int64_t t = ((i - kHalo) / rows) * (int64_t)gx + j / tile_w;
+        owner[q] = t;
+        ++(*off)[(size_t)t + 1];
+    }
+    for (size_t t = 0; t < ntiles; ++t) (*off)[t + 1] += (*off)[t];
+    idx->assign((size_t)(*off)[ntiles], 0);
+    std::vector<int32_t> fill(off->begin(), off->end() - 1);
+    for (size_t q = 0; q < lin.size(); ++q)
+        if (owner[q] >= 0) (*idx)[(size_t)fill[(size_t)owner[q]]++] = (int32_t)q;
+}
+
+}  // namespace fw
 
 namespace {
 
@@ -79,6 +107,7 @@ void release(fw_group* g) {
     if (g->stream) {
         free_table(g, g->tiles);
         free_table(g, g->bounds);
+        free_table(g, g->points);
         free_table(g, g->base_dev);
         (void)hipStreamSynchronize(g->stream);
         (void)hipStreamDestroy(g->stream);
@@ -86,9 +115,16 @@ void release(fw_group* g) {
     delete g;
 }
 
+// Offsets of one member's lists in the group's point table.
+struct PointSlots {
+    size_t src_off, src_idx, zero_off, zero_idx, out_off, out_idx;
+};
+
 int build(fw_group* g) {
     std::vector<fw::GroupTile> tiles;
     std::vector<fw::GroupBoundary> bounds;
+    std::vector<int32_t> pts;
+    std::vector<PointSlots> slots;  // per bounds entry
     uint64_t blocks = 0;
     for (fw_sim* s : g->sims) {
         unsigned gx, gy;
@@ -106,6 +142,22 @@ int build(fw_group* g) {
             if (chunk < 2) chunk = 2;
             gy = fw::group_tiles_y(s->ds.nx, (int)chunk);
         }
+        // the member's points by owning tile, shared by its shots
+        fw::BoundaryLists bl;
+        int rc = fw::resolve_boundary(s->d, s->ds.sx, s->ds.sy, s->icc, s->outc, s->zero, &bl);
+        if (rc != FW_OK) return rc;
+        const int tile_w = g->variant == 2 ? 256 : g->width;
+        PointSlots ps{};
+        std::vector<int32_t> off, idx;
+        const std::vector<int64_t>* lists[3] = {&bl.src_lin, &bl.zero_lin, &bl.out_lin};
+        size_t* at[3][2] = {{&ps.src_off, &ps.src_idx}, {&ps.zero_off, &ps.zero_idx}, {&ps.out_off, &ps.out_idx}};
+        for (int k = 0; k < 3; ++k) {
+            fw::points_by_tile(s->ds.nx, s->ds.ny, gx, tile_w, (int)chunk, *lists[k], &off, &idx);
+            *at[k][0] = pts.size();
+            pts.insert(pts.end(), off.begin(), off.end());
+            *at[k][1] = pts.size();
+            pts.insert(pts.end(), idx.begin(), idx.end());
+        }
         for (int b = 0; b < s->nshots; ++b) {
             fw::GroupTile t{};
             t.s = fw::shot_view(s->ds, b);
@@ -122,10 +174,7 @@ int build(fw_group* g) {
             e.modt = s->d.modt;
             e.step0 = s->step;
             bounds.push_back(e);
-            int64_t pts = s->db.ncoords;
-            if (s->db.nzero > pts) pts = s->db.nzero;
-            if (s->db.nout > pts) pts = s->db.nout;
-            if (pts > g->max_points) g->max_points = pts;
+            slots.push_back(ps);
         }
         if (s->ds.ndmap > g->max_ndmap) g->max_ndmap = s->ds.ndmap;
         const int sets = s->ds.ndmap * (s->ds.coef_axes_same ? 1 : 2);
@@ -135,10 +184,23 @@ int build(fw_group* g) {
     }
     if (blocks >= (1ull << 32)) return fw::set_error(FW_ERR_INVALID, "the group has too many tiles for one launch");
     if (bounds.size() > fw::kMaxGridZ) return fw::set_error(FW_ERR_INVALID, "a group holds at most 65535 grids");
+    if (pts.size() >= ((size_t)1 << 31)) return fw::set_error(FW_ERR_INVALID, "the group has too many points");
     g->blocks = (uint32_t)blocks;
     g->ntiles = (int)tiles.size();
     GR_HIP(alloc_table(g, (void**)&g->tiles, tiles.size() * sizeof(fw::GroupTile)));
     GR_HIP(alloc_table(g, (void**)&g->bounds, bounds.size() * sizeof(fw::GroupBoundary)));
+    GR_HIP(alloc_table(g, (void**)&g->points, pts.size() * sizeof(int32_t)));
+    for (size_t k = 0; k < bounds.size(); ++k) {
+        const PointSlots& ps = slots[k];
+        fw::GroupBoundary& e = bounds[k];
+        e.src_off = g->points + ps.src_off;
+        e.src_idx = g->points + ps.src_idx;
+        e.zero_off = g->points + ps.zero_off;
+        e.zero_idx = g->points + ps.zero_idx;
+        e.out_off = g->points + ps.out_off;
+        e.out_idx = g->points + ps.out_idx;
+    }
+    GR_HIP(hipMemcpyAsync(g->points, pts.data(), pts.size() * sizeof(int32_t), hipMemcpyHostToDevice, g->stream));
     GR_HIP(hipMemcpyAsync(g->tiles, tiles.data(), tiles.size() * sizeof(fw::GroupTile), hipMemcpyHostToDevice,
                           g->stream));
     GR_HIP(hipMemcpyAsync(g->bounds, bounds.data(), bounds.size() * sizeof(fw::GroupBoundary), hipMemcpyHostToDevice,
@@ -151,15 +213,22 @@ int build(fw_group* g) {
 void free_tables(fw_group* g) {
     free_table(g, g->tiles);
     free_table(g, g->bounds);
+    free_table(g, g->points);
     g->tiles = nullptr;
     g->bounds = nullptr;
+    g->points = nullptr;
 }
 
-hipError_t launch_stencil(const fw_group* g, bool velocity) {
+// The velocity launch, or the pressure launch with the boundary work of iteration `it` of a run
+// (base: the device counter a graph replay adds; boundary == false: the stencil alone, as when
+// shapes are timed).
+hipError_t launch_stencil(const fw_group* g, bool velocity, bool boundary = false, const int64_t* base = nullptr,
+                          int64_t it = 0) {
+    const fw::GroupStep gs{boundary ? g->bounds : nullptr, base, it};
     if (g->variant == 2)
         return fw::launch_group_tile2d(g->tiles, g->ntiles, g->blocks, g->chunk, g->coef_sets, g->all_coded, velocity,
-                                       g->stream);
-    return fw::launch_group_2d(g->tiles, g->ntiles, g->blocks, g->width, g->max_ndmap, velocity, g->stream);
+                                       gs, g->stream);
+    return fw::launch_group_2d(g->tiles, g->ntiles, g->blocks, g->width, g->max_ndmap, velocity, gs, g->stream);
 }
 
 // Milliseconds of `reps` grouped velocity + pressure launches with the current tables.
@@ -238,8 +307,8 @@ int choose_shape(fw_group* g, bool tune_variant, bool tune_width, bool tune_chun
     return build(g);
 }
 
-// Capture graph_steps group steps (velocity, pressure and boundary launches, the boundary pass
-// reading the device counter) and one counter advance into a graph, once per group.
+// Capture graph_steps group steps (velocity and pressure launches, the pressure launch's
+// boundary work reading the device counter) and one counter advance into a graph, once per group.
 int capture_graph(fw_group* g) {
     if (!g->base_dev) GR_HIP(alloc_table(g, (void**)&g->base_dev, sizeof(int64_t)));
     hipGraph_t graph = nullptr;
@@ -247,8 +316,7 @@ int capture_graph(fw_group* g) {
     hipError_t e = hipSuccess;
     for (int64_t j = 0; j < g->graph_steps && e == hipSuccess; ++j) {
         e = launch_stencil(g, true);
-        if (e == hipSuccess) e = launch_stencil(g, false);
-        if (e == hipSuccess) e = fw::launch_group_boundary(g->bounds, g->ntiles, g->max_points, g->base_dev, j, g->stream);
+        if (e == hipSuccess) e = launch_stencil(g, false, true, g->base_dev, j);
     }
     if (e == hipSuccess) e = fw::launch_advance(g->base_dev, g->graph_steps, g->stream);
     const hipError_t e2 = hipStreamEndCapture(g->stream, &graph);
@@ -390,8 +458,7 @@ int fw_group_run(fw_group* g, int64_t n_steps) {
     }
     for (int64_t it = done; it < n_steps; ++it) {
         GR_HIP(launch_stencil(g, true));
-        GR_HIP(launch_stencil(g, false));
-        GR_HIP(fw::launch_group_boundary(g->bounds, g->ntiles, g->max_points, nullptr, it, g->stream));
+        GR_HIP(launch_stencil(g, false, true, nullptr, it));
     }
     GR_HIP(hipStreamSynchronize(g->stream));
     for (fw_sim* s : g->sims) {

@@ -216,12 +216,65 @@ __device__ inline int group_entry(const GroupTile* __restrict__ t, int n) {
     return lo;
 }
 
-// One grid (or shot) of a grouped source/air/sensor pass.
+// The source, pressure-release and sensor points of one grid (or shot) of a group, for the
+// group's pressure launch: each workgroup, once its tile is updated, injects the sources, zeroes
+// the pressure-release points and records the sensors whose voxels its tile owns. Points are
+// grouped by owning tile: those of tile t are idx[off[t] .. off[t + 1]), as indices into the
+// grid's point lists (DevBoundary). A voxel in the ghost border belongs to the tile that holds
+// its row and column clamped to the updated region.
 struct GroupBoundary {
     float* p;             // this shot's pressure field
     DevBoundary b;        // icmat_t and rec already offset to this shot
     int64_t modt;
     int64_t step0;        // the grid's step when the current fw_group_run call began
+    const int32_t* src_off;   // [tiles + 1]
+    const int32_t* src_idx;   // source rows (into src_lin and the icmat rows)
+    const int32_t* zero_off;  // [tiles + 1]
+    const int32_t* zero_idx;  // into zero_lin
+    const int32_t* out_off;   // [tiles + 1]
+    const int32_t* out_idx;   // sensor columns (into out_lin, out_src, out_air and the records)
+};
+
+// Step n's sources, pressure release and sensor records at the points that tile `tile` of grid
+// `e` owns, by the lanes of one workgroup after it has updated the tile's pressure (the caller
+// synchronizes the workgroup first). The semantics are the plain boundary pass's (k_boundary):
+// a source dropped for a pressure-release point or a later repeat has lin == -1, and a sensor
+// on a source or release point records the value written there, so no lane reads what another
+// lane of this pass writes.
+__device__ inline void tile_boundary(const GroupBoundary& e, unsigned tile, int64_t n) {
+    const DevBoundary& b = e.b;
+    float* p = e.p;
+    const bool inject = n < b.ntic;
+    if (inject) {
+        for (int32_t k = e.src_off[tile] + (int32_t)threadIdx.x; k < e.src_off[tile + 1]; k += (int32_t)blockDim.x) {
+            const int32_t t = e.src_idx[k];
+            const int64_t lin = b.src_lin[t];
+            if (lin >= 0) p[lin] = b.icmat_t[n * b.ncoords + t];
+        }
+    }
+    for (int32_t k = e.zero_off[tile] + (int32_t)threadIdx.x; k < e.zero_off[tile + 1]; k += (int32_t)blockDim.x)
+        p[b.zero_lin[e.zero_idx[k]]] = 0.f;
+    if (e.modt <= 0 || n % e.modt != 0) return;
+    const int64_t rec_row = n / e.modt;
+    for (int32_t k = e.out_off[tile] + (int32_t)threadIdx.x; k < e.out_off[tile + 1]; k += (int32_t)blockDim.x) {
+        const int32_t t = e.out_idx[k];
+        float val;
+        if (b.out_air[t]) {
+            val = 0.f;
+        } else {
+            const int32_t src = b.out_src[t];
+            val = (inject && src >= 0) ? b.icmat_t[n * b.ncoords + src] : p[b.out_lin[t]];
+        }
+        b.rec[rec_row * b.nout + t] = val;
+    }
+}
+
+// The boundary work of a group's pressure launch at iteration `it` of a run (null `g`: none,
+// as when the kernel shapes are timed); the step of entry m is g[m].step0 + it (+ *base).
+struct GroupStep {
+    const GroupBoundary* g;
+    const int64_t* base;
+    int64_t it;
 };
 
 enum KernelKind { kVelocity = 0, kPressure = 1, kBoundary = 2, kNumKinds = 3 };
@@ -255,18 +308,16 @@ hipError_t launch_advance(int64_t* counter, int64_t k, hipStream_t st);
 // grouped 2D x-marching kernels (isotropic relaxation); `width` 64/128/256 lanes, `chunk` rows
 unsigned group_tiles_x(int64_t ny, int width);
 unsigned group_tiles_y(int64_t nx, int chunk);
+// the pressure launches (velocity == false) also do each member's boundary work of the step (gs)
 hipError_t launch_group_2d(const GroupTile* t, int n, uint32_t blocks, int width, int max_ndmap, bool velocity,
-                           hipStream_t st);
+                           const GroupStep& gs, hipStream_t st);
 // grouped tiled 2D kernels (isotropic relaxation, ny % 4 == 0): 256 columns x `rows` rows
 // (8, 16 or 32) per workgroup; coef_sets: the most coefficient sets any member stages in LDS
 // (ndmap, times 2 unless its axes share their sets); all_coded: every member has codes (the
 // kernel then holds the coded path only, fewer registers)
 unsigned group_tile_rows(int64_t nx, int rows);
 hipError_t launch_group_tile2d(const GroupTile* t, int n, uint32_t blocks, int rows, int coef_sets, bool all_coded,
-                               bool velocity, hipStream_t st);
-// step of entry e: e.step0 + it (+ *base when base is not null)
-hipError_t launch_group_boundary(const GroupBoundary* g, int n, int64_t max_points, const int64_t* base, int64_t it,
-                                 hipStream_t st);
+                               bool velocity, const GroupStep& gs, hipStream_t st);
 hipError_t launch_reciprocal(float* data, int64_t n, hipStream_t st);
 // [nshots][ncoords][ntic] -> [nshots][ntic][ncoords]
 hipError_t launch_transpose_icmat(const float* in, float* out, int64_t nshots, int64_t ncoords, int64_t ntic,

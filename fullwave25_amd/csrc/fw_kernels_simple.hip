@@ -163,35 +163,6 @@ __global__ __launch_bounds__(256) void k_boundary_dev(DevSim s, DevBoundary b, c
     }
 }
 
-// The same pass for every grid of a group, at iteration `it` of a run: grid y of the launch is
-// member y (or one shot of it), at step step0 + it; its pointers are already offset to it.
-// `base` (may be null): a device counter added to `it` (graph replay advances it between replays)
-__global__ __launch_bounds__(256) void k_boundary_group(const GroupBoundary* __restrict__ g, const int64_t* base,
-                                                        int64_t it) {
-    const GroupBoundary& e = g[blockIdx.y];
-    const DevBoundary& b = e.b;
-    const int64_t n = e.step0 + it + (base ? *base : 0);
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    float* p = e.p;
-    const int64_t rec_row = (e.modt > 0 && n % e.modt == 0) ? n / e.modt : -1;
-    const bool inject = n < b.ntic;
-    if (inject && t < b.ncoords) {
-        const int64_t lin = b.src_lin[t];
-        if (lin >= 0) p[lin] = b.icmat_t[n * b.ncoords + t];
-    }
-    if (t < b.nzero) p[b.zero_lin[t]] = 0.f;
-    if (rec_row >= 0 && t < b.nout) {
-        float val;
-        if (b.out_air[t]) {
-            val = 0.f;
-        } else {
-            const int32_t src = b.out_src[t];
-            val = (inject && src >= 0) ? b.icmat_t[n * b.ncoords + src] : p[b.out_lin[t]];
-        }
-        b.rec[rec_row * b.nout + t] = val;
-    }
-}
-
 // Advance (or set, with set >= 0) the device step counter; one lane writes it.
 __global__ void k_tick(int64_t* step, int64_t set) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *step = set >= 0 ? set : *step + 1;
@@ -297,14 +268,6 @@ hipError_t launch_boundary_dev(const DevSim& s, const DevBoundary& b, const int6
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(k_boundary_dev, dim3((unsigned)((n + 255) / 256), (unsigned)s.nshots), dim3(256), 0, st, s, b,
                        step, modt);
-    return hipGetLastError();
-}
-
-hipError_t launch_group_boundary(const GroupBoundary* g, int n, int64_t max_points, const int64_t* base, int64_t it,
-                                 hipStream_t st) {
-    if (n <= 0 || max_points <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_boundary_group, dim3((unsigned)((max_points + 255) / 256), (unsigned)n), dim3(256), 0, st, g,
-                       base, it);
     return hipGetLastError();
 }
 

@@ -369,9 +369,10 @@ __global__ __launch_bounds__(TY) void k_velocity_2d_group(const GroupTile* __res
     else velocity_2d_tile<TY, false, true, false>(s, bx, by, chunk, lds_coef);
 }
 
+// the pressure update of the tile, then its share of the step's boundary work (gs.g non-null)
 template <int TY>
 __global__ __launch_bounds__(TY) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_pressure_2d_group(
-    const GroupTile* __restrict__ t, int n) {
+    const GroupTile* __restrict__ t, int n, GroupStep gs) {
     extern __shared__ float lds_coef[];
     const int m = group_entry(t, n);
     const DevSim s = t[m].s;
@@ -380,6 +381,10 @@ __global__ __launch_bounds__(TY) __attribute__((amdgpu_waves_per_eu(5, 8))) void
     const unsigned bx = local % t[m].gx, by = local / t[m].gx;
     if (s.rcode) pressure_2d_tile<TY, false, true, true>(s, bx, by, chunk, lds_coef);
     else pressure_2d_tile<TY, false, true, false>(s, bx, by, chunk, lds_coef);
+    if (!gs.g) return;
+    __syncthreads();
+    const GroupBoundary& e = gs.g[m];
+    tile_boundary(e, local, e.step0 + gs.it + (gs.base ? *gs.base : 0));
 }
 
 template <int TY>
@@ -418,9 +423,10 @@ hipError_t launch_2d_any(const DevSim& s, const LaunchCfg& cfg, bool velocity, h
 }
 
 template <int TY>
-hipError_t launch_group(const GroupTile* t, int n, uint32_t blocks, int lds, bool velocity, hipStream_t st) {
+hipError_t launch_group(const GroupTile* t, int n, uint32_t blocks, int lds, bool velocity, const GroupStep& gs,
+                        hipStream_t st) {
     if (velocity) hipLaunchKernelGGL((k_velocity_2d_group<TY>), dim3(blocks), dim3(TY), lds, st, t, n);
-    else hipLaunchKernelGGL((k_pressure_2d_group<TY>), dim3(blocks), dim3(TY), lds, st, t, n);
+    else hipLaunchKernelGGL((k_pressure_2d_group<TY>), dim3(blocks), dim3(TY), lds, st, t, n, gs);
     return hipGetLastError();
 }
 
@@ -430,13 +436,13 @@ unsigned group_tiles_x(int64_t ny, int width) { return (unsigned)((ny + width - 
 unsigned group_tiles_y(int64_t nx, int chunk) { return (unsigned)((nx - 2 * kHalo + chunk - 1) / chunk); }
 
 hipError_t launch_group_2d(const GroupTile* t, int n, uint32_t blocks, int width, int max_ndmap, bool velocity,
-                           hipStream_t st) {
+                           const GroupStep& gs, hipStream_t st) {
     if (n <= 0 || blocks == 0) return hipSuccess;
     const int lds = max_ndmap * 2 * kNCoef * 4;
     switch (width) {
-        case 64: return launch_group<64>(t, n, blocks, lds, velocity, st);
-        case 128: return launch_group<128>(t, n, blocks, lds, velocity, st);
-        default: return launch_group<256>(t, n, blocks, lds, velocity, st);
+        case 64: return launch_group<64>(t, n, blocks, lds, velocity, gs, st);
+        case 128: return launch_group<128>(t, n, blocks, lds, velocity, gs, st);
+        default: return launch_group<256>(t, n, blocks, lds, velocity, gs, st);
     }
 }
 

@@ -398,8 +398,9 @@ __global__ __launch_bounds__(kLanes) void k_velocity_tile_group(const GroupTile*
     else velocity_tile<TX, false, true, false>(s, bx, by, L, lds_coef);
 }
 
+// the pressure update of the tile, then its share of the step's boundary work (gs.g non-null)
 template <int TX, bool ALL_CODED>
-__global__ __launch_bounds__(kLanes) void k_pressure_tile_group(const GroupTile* __restrict__ t, int n) {
+__global__ __launch_bounds__(kLanes) void k_pressure_tile_group(const GroupTile* __restrict__ t, int n, GroupStep gs) {
     __shared__ __attribute__((aligned(16))) PreTile<TX> L;
     extern __shared__ __attribute__((aligned(16))) float lds_coef[];
     const int m = group_entry(t, n);
@@ -408,20 +409,25 @@ __global__ __launch_bounds__(kLanes) void k_pressure_tile_group(const GroupTile*
     const unsigned bx = local % t[m].gx, by = local / t[m].gx;
     if (ALL_CODED || s.rcode) pressure_tile<TX, false, true, true>(s, bx, by, L, lds_coef);
     else pressure_tile<TX, false, true, false>(s, bx, by, L, lds_coef);
+    if (!gs.g) return;
+    __syncthreads();
+    const GroupBoundary& e = gs.g[m];
+    tile_boundary(e, local, e.step0 + gs.it + (gs.base ? *gs.base : 0));
 }
 
 template <int TX, bool ALL_CODED>
-hipError_t launch_group_tile(const GroupTile* t, int n, uint32_t blocks, int lds, bool velocity, hipStream_t st) {
+hipError_t launch_group_tile(const GroupTile* t, int n, uint32_t blocks, int lds, bool velocity, const GroupStep& gs,
+                             hipStream_t st) {
     if (velocity) hipLaunchKernelGGL((k_velocity_tile_group<TX, ALL_CODED>), dim3(blocks), dim3(kLanes), lds, st, t, n);
-    else hipLaunchKernelGGL((k_pressure_tile_group<TX, ALL_CODED>), dim3(blocks), dim3(kLanes), lds, st, t, n);
+    else hipLaunchKernelGGL((k_pressure_tile_group<TX, ALL_CODED>), dim3(blocks), dim3(kLanes), lds, st, t, n, gs);
     return hipGetLastError();
 }
 
 template <int TX>
 hipError_t launch_group_tile_any(const GroupTile* t, int n, uint32_t blocks, int lds, bool all_coded, bool velocity,
-                                 hipStream_t st) {
-    return all_coded ? launch_group_tile<TX, true>(t, n, blocks, lds, velocity, st)
-                     : launch_group_tile<TX, false>(t, n, blocks, lds, velocity, st);
+                                 const GroupStep& gs, hipStream_t st) {
+    return all_coded ? launch_group_tile<TX, true>(t, n, blocks, lds, velocity, gs, st)
+                     : launch_group_tile<TX, false>(t, n, blocks, lds, velocity, gs, st);
 }
 
 template <int TX, bool TAPE = false>
@@ -470,13 +476,13 @@ bool tile2d_supported(const DevSim& s) {
 unsigned group_tile_rows(int64_t nx, int rows) { return (unsigned)((nx - 2 * kHalo + rows - 1) / rows); }
 
 hipError_t launch_group_tile2d(const GroupTile* t, int n, uint32_t blocks, int rows, int coef_sets, bool all_coded,
-                               bool velocity, hipStream_t st) {
+                               bool velocity, const GroupStep& gs, hipStream_t st) {
     if (n <= 0 || blocks == 0) return hipSuccess;
     const int lds = coef_sets * kCS * 4;  // the largest member's staged sets (stage_coef)
     switch (rows) {
-        case 8: return launch_group_tile_any<8>(t, n, blocks, lds, all_coded, velocity, st);
-        case 32: return launch_group_tile_any<32>(t, n, blocks, lds, all_coded, velocity, st);
-        default: return launch_group_tile_any<16>(t, n, blocks, lds, all_coded, velocity, st);
+        case 8: return launch_group_tile_any<8>(t, n, blocks, lds, all_coded, velocity, gs, st);
+        case 32: return launch_group_tile_any<32>(t, n, blocks, lds, all_coded, velocity, gs, st);
+        default: return launch_group_tile_any<16>(t, n, blocks, lds, all_coded, velocity, gs, st);
     }
 }
 

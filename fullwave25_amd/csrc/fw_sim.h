@@ -85,6 +85,12 @@ struct BoundaryLists {
 // row; a sensor on a source or air point reads the written value.
 int resolve_boundary(const fw_desc& d, int64_t sx, int64_t sy, const std::vector<int32_t>& icc,
                      const std::vector<int32_t>& outc, const std::vector<int32_t>& zero, BoundaryLists* out);
+// The points of a 2D grid (linear indices `lin`, entries < 0 left out) by owning tile of a
+// grouped launch (GroupBoundary): tiles of `tile_w` columns by `rows` rows from row kHalo, gx
+// across; tile t holds (*idx)[(*off)[t] .. (*off)[t + 1]), indices into `lin` in list order. A
+// point in the ghost border belongs to the tile of its row clamped to the updated rows.
+void points_by_tile(int64_t nx, int64_t ny, unsigned gx, int tile_w, int rows, const std::vector<int64_t>& lin,
+                    std::vector<int32_t>* off, std::vector<int32_t>* idx);
 // Coefficient table [ndmap][ndim][9] (w_1..w_8, e) from dmap [9][2][ndmap] with each axis's spacing
 // folded in; *axes_same = 1 when every axis has the same set for each sound speed.
 std::vector<float> coefficient_table(const fw_desc& d, const std::vector<float>& dmap, int* axes_same);

@@ -359,8 +359,9 @@ class HipSimulation:
 
 class HipGroup:
     """Several 2D isotropic simulations on one device (any grid sizes, media and shot counts)
-    advanced together: each time step is one velocity, one pressure and one source/sensor
-    launch over all of them (``fw_group_*``). Members keep their own records
+    advanced together: each time step is one velocity and one pressure launch over all of them
+    (``fw_group_*``), the pressure launch also injecting each member's sources, zeroing its
+    pressure-release points and recording its sensors. Members keep their own records
     (``member.sensors()``), bit-identical to ``member.run`` with the group's kernel shape
     (``kernel_config()``): ``variant="stream"``, the x-marching kernels ``stream:<width>:<chunk>``;
     ``variant="tile"``, the tiled kernels ``tile:<chunk>`` (members with ny % 4 == 0);

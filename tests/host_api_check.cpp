@@ -2,7 +2,9 @@
 // without a GPU (built and run by tests/test_sanitize.py). Covers what the C ABI computes on the
 // host before any device call: descriptor validation, the directory contract's scalar files and
 // the allocation-free run's size check, coordinate-to-index resolution of sources, air points and
-// sensors (repeats, overlaps, the grid's corners, out-of-grid points), and the coefficient table.
+// sensors (repeats, overlaps, the grid's corners, out-of-grid points), the coefficient table, and
+// the grouping of those points by the tile of a grouped launch that owns them.
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -199,6 +201,45 @@ static void coefficients(int ndim) {
     CHECK(fw::coefficient_table(d, dmap, &same).empty());
 }
 
+// The points of a grouped launch by owning tile: every point in exactly one tile, the one whose
+// columns hold its column and whose rows hold its row (a ghost-border row taken to the nearest
+// updated row), each tile's points in list order, skipped entries (< 0) in none.
+static void point_tiles(unsigned seed) {
+    std::mt19937 rng(seed);
+    const int64_t nx = 17 + (int64_t)(rng() % 300), ny = 17 + (int64_t)(rng() % 700);
+    const int widths[] = {64, 128, 256};
+    const int tile_w = widths[rng() % 3];
+    const int rows = 2 + (int)(rng() % 31);
+    const unsigned gx = (unsigned)((ny + tile_w - 1) / tile_w);
+    const int64_t gy = (nx - 16 + rows - 1) / rows;
+    std::vector<int64_t> lin(rng() % 400);
+    for (auto& l : lin) l = (rng() % 7 == 0) ? -1 : (int64_t)(rng() % (uint64_t)(nx * ny));
+    std::vector<int32_t> off, idx;
+    fw::points_by_tile(nx, ny, gx, tile_w, rows, lin, &off, &idx);
+    CHECK(off.size() == (size_t)(gx * gy + 1) && off[0] == 0);
+    size_t valid = 0;
+    for (int64_t l : lin) valid += l >= 0;
+    CHECK((size_t)off.back() == valid && idx.size() == valid);
+    std::vector<int> seen(lin.size(), 0);
+    for (size_t t = 0; t + 1 < off.size(); ++t) {
+        CHECK(off[t] <= off[t + 1]);
+        const int64_t bx = (int64_t)(t % gx), by = (int64_t)(t / gx);
+        for (int32_t k = off[t]; k < off[t + 1]; ++k) {
+            const int32_t q = idx[(size_t)k];
+            CHECK(q >= 0 && (size_t)q < lin.size() && lin[(size_t)q] >= 0);
+            if (q < 0 || (size_t)q >= lin.size() || lin[(size_t)q] < 0) continue;
+            ++seen[(size_t)q];
+            if (k > off[t]) CHECK(idx[(size_t)k - 1] < q);
+            int64_t i = lin[(size_t)q] / ny;
+            const int64_t j = lin[(size_t)q] % ny;
+            i = std::max<int64_t>(8, std::min<int64_t>(i, nx - 9));
+            CHECK(j >= bx * tile_w && j < (bx + 1) * tile_w);
+            CHECK(i >= 8 + by * rows && i < 8 + (by + 1) * rows);
+        }
+    }
+    for (size_t q = 0; q < lin.size(); ++q) CHECK(seen[q] == (lin[q] >= 0 ? 1 : 0));
+}
+
 int main(int argc, char** argv) {
     const char* tmp = argc > 1 ? argv[1] : "/tmp";
     descriptors();
@@ -209,6 +250,7 @@ int main(int argc, char** argv) {
     }
     coefficients(2);
     coefficients(3);
+    for (unsigned seed = 1; seed <= 50; ++seed) point_tiles(seed);
     std::printf("host api check: %s (%d failures)\n", failures ? "FAILED" : "ok", failures);
     return failures ? 1 : 0;
 }

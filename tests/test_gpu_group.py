@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 from fullwave25_amd import engine, workloads
-from tests.helpers import rel_l2
+from tests.helpers import raw_inputs, rel_l2
 
 pytestmark = pytest.mark.gpu
 SHAPE = "stream:256:16"
@@ -236,4 +236,56 @@ def test_group_graph_replay_matches_plain_launches():
         assert np.abs(plain[0].sensors()).max() > 0
     finally:
         for s in plain + graph:
+            s.close()
+
+
+def _edge_point_media():
+    """Two media whose source, pressure-release and sensor points sit where the group's pressure
+    launch hands them to different workgroups: in the ghost border, on both sides of tile
+    boundaries (rows 8 + 8k, columns 64k/128k/256k), repeated, and on each other; the sources
+    stop before the run ends and every third step is recorded."""
+    rng = np.random.default_rng(71)
+    out = []
+    for shape, nt in (((75, 300), 60), ((44, 132), 60)):
+        nx, ny = shape
+        c = rng.uniform(1450.0, 1600.0, shape)
+        rho = rng.uniform(900.0, 1100.0, shape)
+        icc = [[3, 5], [8, 255], [8, 256], [15, 63], [16, 64], [nx - 1, ny - 1], [30, 30], [30, 30], [20, 40],
+               [20, 9], [nx - 9, 127], [24, 128]]
+        zero = [[20, 40], [1, 1], [nx - 12, ny - 3], [23, 23], [24, 128]]
+        outc = [[3, 5], [30, 30], [20, 40], [1, 1], [8, 256], [15, 63], [16, 64], [nx - 1, ny - 1], [40, 100],
+                [40, 100], [nx - 12, ny - 3], [0, 0], [24, 128], [nx - 9, 127], [31, 255], [32, 256]]
+        icc = [[min(i, nx - 1), min(j, ny - 1)] for i, j in icc]
+        zero = [[min(i, nx - 1), min(j, ny - 1)] for i, j in zero]
+        outc = [[min(i, nx - 1), min(j, ny - 1)] for i, j in outc]
+        sig = rng.standard_normal((len(icc), 40)).astype(np.float32)
+        out.append(raw_inputs(shape, c=c, rho=rho, nt=nt, icc=icc, icmat=sig, outc=outc, icczero=zero, modt=3))
+    return out
+
+
+@pytest.mark.parametrize("variant,width,chunk", [("stream", 256, 16), ("stream", 64, 8), ("tile", 256, 8),
+                                                 ("tile", 256, 32)])
+def test_group_boundary_points_at_tile_edges(monkeypatch, variant, width, chunk):
+    """The pressure launch's per-tile boundary work: every member equals its own run (source,
+    release and sensor passes of the plain engine) bit for bit, and the first meets the FP64
+    oracle."""
+    import oracle
+
+    media = _edge_point_media()
+    shape = f"tile:{chunk}" if variant == "tile" else f"stream:{width}:{chunk}"
+    sims = [engine.HipSimulation(m) for m in media]
+    try:
+        with engine.HipGroup(sims, variant=variant, width=width, chunk=chunk) as grp:
+            grp.run(25)
+            grp.run(35)
+        for i, (m, s) in enumerate(zip(media, sims)):
+            want = _alone(m, m.nt, monkeypatch, shape=shape)
+            print(f"{shape} member {i}: shape {m.shape}, max |rec| {np.abs(want).max():.3e}", flush=True)
+            assert np.abs(want).max() > 0
+            np.testing.assert_array_equal(s.sensors(), want)
+        ref = oracle.OracleSimulation(media[0], 64)
+        ref.run(media[0].nt)
+        assert rel_l2(sims[0].sensors().astype(np.float64), ref.sensors()) <= 1e-4
+    finally:
+        for s in sims:
             s.close()

@@ -5,7 +5,8 @@
 * the engine library's host side (tests/host_api_check.cpp, built with every library source and
   the sanitizers on the host part only): descriptor validation, the directory contract's scalar
   files and the allocation-free run's size check, the resolution of source, air and sensor
-  coordinates to device indices, and the coefficient table.
+  coordinates to device indices, the coefficient table, and the grouping of those points by the
+  tile of a grouped launch that owns them.
 
 Any finding aborts the program, so the test checks the exit status and the final line.
 """
