@@ -693,6 +693,13 @@ def bench_single(args) -> dict:
     dominant = max(("velocity", "pressure"), key=lambda k: kms[k])
     avg_s = kms[dominant] / 1e3 / args.steps
     achieved = bpc[dominant] * upd / avg_s / 1e9
+    # the dominant kernel's per-cell streams without its stencils, on arrays of the grid's size:
+    # velocity reads p, rho, the code, 2 ndim memory variables and ndim velocities and writes the
+    # last two; pressure reads ndim velocities, K, the code, 2 ndim memory variables and p and
+    # writes p and the memory variables
+    nd = inp.ndim
+    streams = (3 + 3 * nd, 3 * nd) if dominant == "velocity" else (3 + 3 * nd, 1 + 2 * nd)
+    stream_gbs = engine.stream_bandwidth(inp.ncells, *streams, repeats=5)
     dims = f"{args.nx}x{args.ny}" + (f"x{args.nz}" if inp.ndim == 3 else "")
     ext = "x".join(str(s) for s in inp.shape)
     return {
@@ -738,6 +745,8 @@ def bench_single(args) -> dict:
             "kernel_ms_per_step": {k: round(v / args.steps, 5) for k, v in kms.items()},
             "copy_kernel_gbs": round(copy_gbs, 1),
             "frac_of_copy": round(achieved / copy_gbs, 4),
+            "stream_probe": {"reads": streams[0], "writes": streams[1], "cells": inp.ncells, "gbs": round(stream_gbs, 1)},
+            "frac_of_stream_probe": round(achieved / stream_gbs, 4),
             "kernel_config": kcfg,
             "kernels": engine.kernel_override_string(kcfg),
             "kernel_source": source,
@@ -769,6 +778,8 @@ def main() -> int:
     ap.add_argument("--sweep", type=int, default=0,
                     help="time every kernel family/shape over this many steps and exit")
     ap.add_argument("--sweep-shapes", default="", help="comma-separated kernel shapes for --sweep")
+    ap.add_argument("--stream-probe", default="",
+                    help="R,W,CELLS: only run the streaming probe (R read and W write arrays of CELLS floats)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run the plain copy kernel over this many bytes (counter calibration)")
     ap.add_argument("--kernels", default="",
@@ -820,6 +831,13 @@ def main() -> int:
 
         gbs = engine.copy_bandwidth(args.copy_probe, repeats=5)
         print(json.dumps({"copy_bytes": args.copy_probe, "copy_gbs": round(gbs, 1)}), flush=True)
+        return 0
+    if args.stream_probe:
+        from fullwave25_amd import engine
+
+        r, w, cells = (int(x) for x in args.stream_probe.split(","))
+        gbs = engine.stream_bandwidth(cells, r, w, repeats=5)
+        print(json.dumps({"reads": r, "writes": w, "cells": cells, "stream_gbs": round(gbs, 1)}), flush=True)
         return 0
     if args.config in ("batched_2d", "batched_mix", "served_mix", "grouped_2d", "adjoint_2d"):
         if args.config in ("batched_2d", "adjoint_2d") and "--steps" not in sys.argv:

@@ -1011,6 +1011,45 @@ int fw_device_copy_bandwidth(int device, size_t bytes, int repeats, double* gbs)
     return rc;
 }
 
+int fw_device_stream_bandwidth(int device, int64_t cells, int32_t nread, int32_t nwrite, int32_t repeats,
+                               double* gbs) {
+    if (!gbs || repeats < 1 || cells < 1 || nread < 0 || nwrite < 1 || nread > fw::kMaxStreams ||
+        nwrite > fw::kMaxStreams)
+        return fail(FW_ERR_INVALID, "need gbs, repeats >= 1, cells >= 1, 0..%d read and 1..%d write streams",
+                    fw::kMaxStreams, fw::kMaxStreams);
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev)
+        return fail(FW_ERR_DEVICE, "device %d not available", device);
+    DeviceGuard g(device);
+    const size_t slot = ((size_t)cells * 4 + 255) / 256 * 256;
+    char* base = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = FW_OK;
+    if (hipMalloc((void**)&base, slot * (size_t)(nread + nwrite)) != hipSuccess) {
+        rc = fail(FW_ERR_NOMEM, "hipMalloc of %d arrays of %lld floats failed", (int)(nread + nwrite), (long long)cells);
+    } else if (hipMemset(base, 0, slot * (size_t)(nread + nwrite)) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+               hipEventCreate(&e1) != hipSuccess) {
+        rc = fail(FW_ERR_DEVICE, "setup failed");
+    } else {
+        fw::StreamSet a{};
+        for (int k = 0; k < nread; ++k) a.r[k] = (const float*)(base + slot * (size_t)k);
+        for (int k = 0; k < nwrite; ++k) a.w[k] = (float*)(base + slot * (size_t)(nread + k));
+        hipError_t e = fw::launch_streams(a, nread, nwrite, cells, nullptr);  // warm-up
+        (void)hipEventRecord(e0, nullptr);
+        for (int r = 0; r < repeats && e == hipSuccess; ++r) e = fw::launch_streams(a, nread, nwrite, cells, nullptr);
+        (void)hipEventRecord(e1, nullptr);
+        float ms = 0.f;
+        if (e != hipSuccess || hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+            rc = fail(FW_ERR_DEVICE, "stream kernel failed");
+        else
+            *gbs = 4.0 * (double)cells * (nread + nwrite) * repeats / (ms * 1e-3) / 1e9;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (base) (void)hipFree(base);
+    return rc;
+}
+
 int fw_device_memory(int device, size_t* free_bytes, size_t* total_bytes) {
     if (!free_bytes || !total_bytes) return fail(FW_ERR_INVALID, "null argument");
     int ndev = 0;

@@ -128,13 +128,24 @@ __device__ __forceinline__ CodedCoef mapped_coef(const float* ik, const float* a
     return CodedCoef{gld32(ik, c), gld32(a1, c), gld32(a2, c), gld32(b1, c), gld32(b2, c),
                      UNIFORM ? 0 : gld32(dcmap, c)};
 }
-// the coefficients of cell c for the velocity (VEL) or pressure update of a coded grid
+// the per-cell map coefficients of cell c (a cell without a code), velocity (VEL) or pressure side
 template <bool VEL, bool UNIFORM>
-__device__ __forceinline__ CodedCoef cell_coef(const DevSim& s, uint32_t c) {
-    const uint32_t code = gld16(s.rcode, c);
+__device__ __forceinline__ CodedCoef escape_coef(const DevSim& s, uint32_t c) {
+    if (VEL) return mapped_coef<UNIFORM>(s.ikv[0], s.av[0][0], s.av[0][1], s.bv[0][0], s.bv[0][1], s.dcmap, c);
+    return mapped_coef<UNIFORM>(s.ikp[0], s.ap[0][0], s.ap[0][1], s.bp[0][0], s.bp[0][1], s.dcmap, c);
+}
+// the coefficients of cell c, whose code is `code`, for the velocity (VEL) or pressure update of
+// a coded grid
+template <bool VEL, bool UNIFORM>
+__device__ __forceinline__ CodedCoef coef_of_code(const DevSim& s, uint32_t c, uint32_t code) {
     if (code != kCodeEscape) return coded_coef(VEL ? s.rtab_v : s.rtab_p, code);
     if (VEL) return mapped_coef<UNIFORM>(s.ikv[0], s.av[0][0], s.av[0][1], s.bv[0][0], s.bv[0][1], s.dcmap, c);
     return mapped_coef<UNIFORM>(s.ikp[0], s.ap[0][0], s.ap[0][1], s.bp[0][0], s.bp[0][1], s.dcmap, c);
+}
+// the same, reading the code first (the row gather then waits for the code)
+template <bool VEL, bool UNIFORM>
+__device__ __forceinline__ CodedCoef cell_coef(const DevSim& s, uint32_t c) {
+    return coef_of_code<VEL, UNIFORM>(s, c, gld16(s.rcode, c));
 }
 
 // The fields of one shot of the batch.
@@ -323,6 +334,13 @@ hipError_t launch_reciprocal(float* data, int64_t n, hipStream_t st);
 hipError_t launch_transpose_icmat(const float* in, float* out, int64_t nshots, int64_t ncoords, int64_t ntic,
                                   hipStream_t st);
 hipError_t launch_copy(const void* src, void* dst, int64_t n16, hipStream_t st);
+// streaming probe (fw_device_stream_bandwidth): up to kMaxStreams read and write arrays
+constexpr int kMaxStreams = 16;
+struct StreamSet {
+    const float* r[kMaxStreams];
+    float* w[kMaxStreams];
+};
+hipError_t launch_streams(const StreamSet& a, int nr, int nw, int64_t n, hipStream_t st);
 hipError_t launch_code_insert(const CodeMaps& m, int64_t n, unsigned long long* keys, uint32_t nslots, uint32_t* count,
                               int32_t* rep, int32_t* status, hipStream_t st);
 hipError_t launch_code_assign(const CodeMaps& m, int64_t n, const unsigned long long* keys, uint32_t nslots,

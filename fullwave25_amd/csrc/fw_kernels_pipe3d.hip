@@ -12,6 +12,11 @@
 //     at iteration i, before the barrier, and consumed one iteration later.
 // One barrier per plane. A 4-slot ring is enough: at iteration i the lanes write slot i + 1
 // (mod 4) while lanes still in iteration i - 1 read slots i - 2 .. i.
+// With coded relaxation coefficients, a cell's code is loaded one iteration before the iteration
+// that gathers its table row: memory operations complete in issue order for the wait counters,
+// so a row gather issued right after its code load waits for every load issued before the code,
+// and each plane took two memory round trips (measured: pressure kernel 9.99 -> 9.51 ms, velocity
+// 10.95 -> 10.80 ms on the 904 x 904 x 776 grid, with the row read as two vector loads).
 #include "fw_internal.h"
 
 namespace fw {
@@ -128,6 +133,12 @@ struct HaloStager {
     }
 };
 
+// the code of cell c (coded grids, active cells), else 0
+template <bool CODED>
+__device__ inline uint32_t load_code(const DevSim& s, uint32_t c, bool active) {
+    return CODED && active ? gld16(s.rcode, c) : 0u;
+}
+
 template <bool UNIFORM>
 __device__ inline const float* coefp(const float* coefs, int32_t dc, int axis) {
     if (UNIFORM) return coefs + axis * kNCoef;
@@ -152,11 +163,30 @@ struct VelOps {
     float a[NA][2], b[NA][2];
     float ik[NA];
     int32_t dc;
+    bool esc;  // coded grid, cell without a code: the coefficients come from the maps (fix_escape)
 };
 
-// CODED (isotropic only): the relaxation coefficients come from the code table (rcode, rtab_v/p)
+// The coefficients of a cell without a code (o.esc) from the per-cell maps (coded grids).
+template <bool VEL, bool UNIFORM, class Ops>
+__device__ inline void fix_escape(Ops& o, const DevSim& s, uint32_t c) {
+    if (!o.esc) return;
+    const CodedCoef k = escape_coef<VEL, UNIFORM>(s, c);
+    o.a[0][0] = k.a1;
+    o.a[0][1] = k.a2;
+    o.b[0][0] = k.b1;
+    o.b[0][1] = k.b2;
+    o.ik[0] = k.ik;
+    o.dc = UNIFORM ? 0 : k.dc;
+}
+
+// CODED (isotropic only): the relaxation coefficients come from row `code` of the code table
+// (rtab_v), `code` being the cell's code (rcode), loaded beforehand. The row is read as one 16-byte
+// and one 8-byte load by every lane (a cell without a code reads row 0 and is fixed up after the
+// barrier by fix_escape): with the rare map reads in a branch of their own, the compiler does not
+// turn the gather into six lane-selected loads.
 template <bool UNIFORM, bool ISO, bool CODED>
-__device__ inline void load_vel(VelOps<ISO>& o, const DevSim& s, uint32_t c, bool in_grid, bool active) {
+__device__ inline void load_vel(VelOps<ISO>& o, const DevSim& s, uint32_t c, bool in_grid, bool active,
+                                uint32_t code) {
     const uint32_t sx = (uint32_t)s.sx;
     o.qnew = in_grid ? gld(s.p, c + 8 * sx) : 0.f;
     o.rho_x = in_grid ? gld(s.rho, c + sx) : 0.f;
@@ -170,7 +200,8 @@ __device__ inline void load_vel(VelOps<ISO>& o, const DevSim& s, uint32_t c, boo
         o.psi[a][1] = gldnt(s.psi[a][1], c);
     }
     if (CODED) {
-        const CodedCoef k = cell_coef<true, UNIFORM>(s, c);
+        o.esc = code == kCodeEscape;
+        const CodedCoef k = coded_coef(s.rtab_v, o.esc ? 0u : code);
         o.a[0][0] = k.a1;
         o.a[0][1] = k.a2;
         o.b[0][0] = k.b1;
@@ -221,22 +252,30 @@ __global__ __launch_bounds__(PZ3 * TY) void k_velocity_pipe3d(DevSim batch, int 
     st.load(s.p, i0 + 1, s);
     float rho_i = in_grid ? gld(s.rho, (uint32_t)i0 * sx + col) : 0.f;
     VelOps<ISO> cur, nxt;
-    if (AHEAD) load_vel<UNIFORM, ISO, CODED>(cur, s, (uint32_t)i0 * sx + col, in_grid, active);
+    // ca: the code of the plane whose operands this iteration loads (i, or i + 1 with AHEAD)
+    constexpr int kAhead = AHEAD ? 1 : 0;
+    if (AHEAD)
+        load_vel<UNIFORM, ISO, CODED>(cur, s, (uint32_t)i0 * sx + col, in_grid, active,
+                                      load_code<CODED>(s, (uint32_t)i0 * sx + col, active));
+    uint32_t ca = i0 + kAhead < i1 ? load_code<CODED>(s, (uint32_t)(i0 + kAhead) * sx + col, active) : 0u;
 
     for (int64_t i = i0; i < i1; ++i) {
         st.store(P, i + 1, q[9], ty, tz);
         const bool more = i + 1 < i1;
-        if (!AHEAD) load_vel<UNIFORM, ISO, CODED>(cur, s, (uint32_t)i * sx + col, in_grid, active);
+        const uint32_t cb = i + kAhead + 1 < i1 ? load_code<CODED>(s, (uint32_t)(i + kAhead + 1) * sx + col, active) : 0u;
+        if (!AHEAD) load_vel<UNIFORM, ISO, CODED>(cur, s, (uint32_t)i * sx + col, in_grid, active, ca);
         if (more) {
             st.load(s.p, i + 2, s);
-            if (AHEAD) load_vel<UNIFORM, ISO, CODED>(nxt, s, (uint32_t)(i + 1) * sx + col, in_grid, active);
+            if (AHEAD) load_vel<UNIFORM, ISO, CODED>(nxt, s, (uint32_t)(i + 1) * sx + col, in_grid, active, ca);
         }
+        ca = cb;
 #pragma unroll
         for (int m = 0; m < 15; ++m) q[m] = q[m + 1];
         q[15] = cur.qnew;
         __syncthreads();
         if (active) {
             const uint32_t c = (uint32_t)i * sx + col;
+            if (CODED) fix_escape<true, UNIFORM>(cur, s, c);
             const float* pm = P.slot(i - 1);
             const float* p0 = P.slot(i);
             const float* pp = P.slot(i + 1);
@@ -299,10 +338,13 @@ struct PreOps {
     float ik[NA];
     float K, p;
     int32_t dc;
+    bool esc;  // as VelOps::esc
 };
 
+
 template <bool UNIFORM, bool ISO, bool CODED>
-__device__ inline void load_pre(PreOps<ISO>& o, const DevSim& s, uint32_t c, bool in_grid, bool active) {
+__device__ inline void load_pre(PreOps<ISO>& o, const DevSim& s, uint32_t c, bool in_grid, bool active,
+                                uint32_t code) {
     o.qnew = in_grid ? gld((const float*)s.v[0], c + 7 * (uint32_t)s.sx) : 0.f;
     if (!active) return;
 #pragma unroll
@@ -311,7 +353,8 @@ __device__ inline void load_pre(PreOps<ISO>& o, const DevSim& s, uint32_t c, boo
         o.phi[a][1] = gldnt(s.phi[a][1], c);
     }
     if (CODED) {
-        const CodedCoef k = cell_coef<false, UNIFORM>(s, c);
+        o.esc = code == kCodeEscape;
+        const CodedCoef k = coded_coef(s.rtab_p, o.esc ? 0u : code);
         o.a[0][0] = k.a1;
         o.a[0][1] = k.a2;
         o.b[0][0] = k.b1;
@@ -377,26 +420,34 @@ __global__ __launch_bounds__(PZ3 * TY) void k_pressure_pipe3d(DevSim batch, int 
     sy_.load(s.v[1], i0 + 1, s);
     sz_.load(s.v[2], i0 + 1, s);
     PreOps<ISO> cur, nxt;
-    if (AHEAD) load_pre<UNIFORM, ISO, CODED>(cur, s, (uint32_t)i0 * sx + col, in_grid, active);
+    // ca: the code of the plane whose operands this iteration loads (i, or i + 1 with AHEAD)
+    constexpr int kAhead = AHEAD ? 1 : 0;
+    if (AHEAD)
+        load_pre<UNIFORM, ISO, CODED>(cur, s, (uint32_t)i0 * sx + col, in_grid, active,
+                                      load_code<CODED>(s, (uint32_t)i0 * sx + col, active));
+    uint32_t ca = i0 + kAhead < i1 ? load_code<CODED>(s, (uint32_t)(i0 + kAhead) * sx + col, active) : 0u;
 
     for (int64_t i = i0; i < i1; ++i) {
         sx_.store(VX, i, q[9], ty, tz);
         sy_.store(VY, i + 1);
         sz_.store(VZ, i + 1);
         const bool more = i + 1 < i1;
-        if (!AHEAD) load_pre<UNIFORM, ISO, CODED>(cur, s, (uint32_t)i * sx + col, in_grid, active);
+        const uint32_t cb = i + kAhead + 1 < i1 ? load_code<CODED>(s, (uint32_t)(i + kAhead + 1) * sx + col, active) : 0u;
+        if (!AHEAD) load_pre<UNIFORM, ISO, CODED>(cur, s, (uint32_t)i * sx + col, in_grid, active, ca);
         if (more) {
             sx_.load(s.v[0], i + 1, s);
             sy_.load(s.v[1], i + 2, s);
             sz_.load(s.v[2], i + 2, s);
-            if (AHEAD) load_pre<UNIFORM, ISO, CODED>(nxt, s, (uint32_t)(i + 1) * sx + col, in_grid, active);
+            if (AHEAD) load_pre<UNIFORM, ISO, CODED>(nxt, s, (uint32_t)(i + 1) * sx + col, in_grid, active, ca);
         }
+        ca = cb;
 #pragma unroll
         for (int m = 0; m < 15; ++m) q[m] = q[m + 1];
         q[15] = cur.qnew;
         __syncthreads();
         if (active) {
             const uint32_t c = (uint32_t)i * sx + col;
+            if (CODED) fix_escape<false, UNIFORM>(cur, s, c);
             const float* xm = VX.slot(i - 1);
             const float* x0 = VX.slot(i);
             const float* ym = VY.slot(i - 1);

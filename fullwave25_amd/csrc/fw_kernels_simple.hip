@@ -180,6 +180,23 @@ __global__ __launch_bounds__(256) void k_copy(const float4* __restrict__ src, fl
     if (t < n) dst[t] = src[t];
 }
 
+// `nr` read streams and `nw` write streams of one float per lane at the same element index: the
+// access pattern of the stencil kernels' per-cell operands without their stencils (the velocity
+// kernel of a 3D grid reads 12 such streams and writes 9).
+__global__ __launch_bounds__(256) void k_streams(StreamSet a, int nr, int nw, int64_t n) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    float v[kMaxStreams];
+#pragma unroll
+    for (int r = 0; r < kMaxStreams; ++r) v[r] = r < nr ? a.r[r][t] : 0.f;
+    float sum = 0.f;
+#pragma unroll
+    for (int r = 0; r < kMaxStreams; ++r) sum += v[r];
+#pragma unroll
+    for (int w = 0; w < kMaxStreams; ++w)
+        if (w < nw) a.w[w][t] = sum * (float)(w + 1);
+}
+
 __global__ __launch_bounds__(256) void k_reciprocal(float* data, int64_t n) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t < n) data[t] = data[t] != 0.f ? 1.0f / data[t] : 0.f;  // zero: row padding
@@ -268,6 +285,14 @@ hipError_t launch_boundary_dev(const DevSim& s, const DevBoundary& b, const int6
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(k_boundary_dev, dim3((unsigned)((n + 255) / 256), (unsigned)s.nshots), dim3(256), 0, st, s, b,
                        step, modt);
+    return hipGetLastError();
+}
+
+hipError_t launch_streams(const StreamSet& a, int nr, int nw, int64_t n, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    const int64_t blocks = (n + 255) / 256;
+    if (blocks > 0x7fffffff) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_streams, dim3((unsigned)blocks), dim3(256), 0, st, a, nr, nw, n);
     return hipGetLastError();
 }
 

@@ -29,7 +29,7 @@ KERNEL_KINDS = ("velocity", "pressure", "boundary")
 # every symbol include/fullwave_hip.h declares
 EXPORTED_SYMBOLS = (
     "fw_abi_version", "fw_version_string", "fw_device_count", "fw_last_error", "fw_run_dir", "fw_run_dir_noalloc",
-    "fw_device_copy_bandwidth", "fw_device_memory",
+    "fw_device_copy_bandwidth", "fw_device_stream_bandwidth", "fw_device_memory",
     "fw_sim_create", "fw_sim_upload", "fw_sim_run", "fw_sim_reset", "fw_sim_step_count",
     "fw_sim_record_count", "fw_sim_download_sensors", "fw_sim_download_field",
     "fw_sim_set_field", "fw_sim_kernel_config", "fw_sim_set_kernel_config", "fw_sim_set_graph_steps",
@@ -85,6 +85,7 @@ def load_library(path: Path | None = None) -> ctypes.CDLL:
         lib.fw_run_dir.argtypes = [ctypes.c_char_p, ctypes.c_int]
         lib.fw_run_dir_noalloc.argtypes = [ctypes.c_char_p, ctypes.c_int]
         lib.fw_device_copy_bandwidth.argtypes = [ctypes.c_int, sz, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+        lib.fw_device_stream_bandwidth.argtypes = [ctypes.c_int, i64, i32, i32, i32, ctypes.POINTER(ctypes.c_double)]
         lib.fw_device_memory.argtypes = [ctypes.c_int, ctypes.POINTER(sz), ctypes.POINTER(sz)]
         lib.fw_sim_create.argtypes = [ctypes.POINTER(FwDesc), ctypes.c_int, ctypes.POINTER(vp)]
         lib.fw_sim_upload.argtypes = [vp, ctypes.c_char_p, vp, sz]
@@ -138,6 +139,16 @@ def copy_bandwidth(nbytes: int = 1 << 30, repeats: int = 10, device: int = 0) ->
     lib = load_library()
     g = ctypes.c_double(0)
     _check(lib, lib.fw_device_copy_bandwidth(int(device), int(nbytes), int(repeats), ctypes.byref(g)))
+    return g.value
+
+
+def stream_bandwidth(cells: int, nread: int, nwrite: int, repeats: int = 5, device: int = 0) -> float:
+    """GB/s (read + write) of a kernel that reads ``nread`` and writes ``nwrite`` arrays of
+    ``cells`` floats, one float per lane (the stencil kernels' per-cell operand pattern)."""
+    lib = load_library()
+    g = ctypes.c_double(0)
+    _check(lib, lib.fw_device_stream_bandwidth(int(device), int(cells), int(nread), int(nwrite), int(repeats),
+                                               ctypes.byref(g)))
     return g.value
 
 

@@ -82,6 +82,13 @@ const char* fw_last_error(void);
  * `repeats` times and returns the achieved read + write rate in GB/s. Used to calibrate
  * hardware-counter byte counts against a kernel of known traffic. */
 int fw_device_copy_bandwidth(int device, size_t bytes, int repeats, double* gbs);
+/* Streaming self-check; no counterpart in the reference. Reads `nread` (0..16) and writes
+ * `nwrite` (1..16) device arrays of `cells` floats, one float per lane at the same index in
+ * every array (the per-cell operand pattern of the stencil kernels, without their stencils),
+ * `repeats` times, and returns the achieved read + write rate in GB/s: the rate a kernel with
+ * that many per-cell streams can reach on this device. */
+int fw_device_stream_bandwidth(int device, int64_t cells, int32_t nread, int32_t nwrite, int32_t repeats,
+                               double* gbs);
 /* Free and total device memory in bytes (hipMemGetInfo); no counterpart in the reference.
  * Used to admit jobs by memory budget. */
 int fw_device_memory(int device, size_t* free_bytes, size_t* total_bytes);
