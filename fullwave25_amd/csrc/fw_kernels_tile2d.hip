@@ -56,28 +56,44 @@ __device__ inline void st4nt(float* base, uint32_t i, const F4& v) {
 }
 
 // the relaxation coefficients and coefficient-set indices of a quad: from the code table
-// (CODED; a cell without a code reads the per-cell maps) or from the per-cell maps
+// (CODED: `cw`, the quad's four codes, loaded beforehand) or from the per-cell maps. Each coded
+// cell reads its table row as one 16-byte and one 8-byte load (a cell without a code reads row
+// 0 first); the rare cells without a code then take their maps in a branch of their own, so
+// that the compiler does not turn the row reads into lane-selected 4-byte loads.
 template <bool CODED, bool UNIFORM, bool VEL, int NA>
-__device__ inline void quad_coef(const DevSim& s, uint32_t c, F4* ik, F4* b1, F4* b2, F4* a1, F4* a2, I4& dc) {
+__device__ inline void quad_coef(const DevSim& s, uint32_t c, uint2 cw, F4* ik, F4* b1, F4* b2, F4* a1, F4* a2,
+                                 I4& dc) {
     const float* const* ik_m = VEL ? s.ikv : s.ikp;
     const float* const (*a_m)[kNRelax] = VEL ? s.av : s.ap;
     const float* const (*b_m)[kNRelax] = VEL ? s.bv : s.bp;
     if (CODED) {
-        const uint2 cw = ldc4(s.rcode, c);
         const uint32_t cs[4] = {cw.x & 0xffffu, cw.x >> 16, cw.y & 0xffffu, cw.y >> 16};
         int d[4];
+        bool esc = false;
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-            const CodedCoef k = cs[e] != kCodeEscape
-                                    ? coded_coef(VEL ? s.rtab_v : s.rtab_p, cs[e])
-                                    : mapped_coef<UNIFORM>(ik_m[0], a_m[0][0], a_m[0][1], b_m[0][0], b_m[0][1],
-                                                           s.dcmap, c + e);
+            const CodedCoef k = coded_coef(VEL ? s.rtab_v : s.rtab_p, cs[e] != kCodeEscape ? cs[e] : 0u);
             ik[0][e] = k.ik;
             a1[0][e] = k.a1;
             a2[0][e] = k.a2;
             b1[0][e] = k.b1;
             b2[0][e] = k.b2;
             d[e] = UNIFORM ? 0 : k.dc;
+            esc = esc || cs[e] == kCodeEscape;
+        }
+        if (esc) {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                if (cs[e] != kCodeEscape) continue;
+                const CodedCoef k = mapped_coef<UNIFORM>(ik_m[0], a_m[0][0], a_m[0][1], b_m[0][0], b_m[0][1],
+                                                         s.dcmap, c + e);
+                ik[0][e] = k.ik;
+                a1[0][e] = k.a1;
+                a2[0][e] = k.a2;
+                b1[0][e] = k.b1;
+                b2[0][e] = k.b2;
+                d[e] = UNIFORM ? 0 : k.dc;
+            }
         }
         dc = I4{d[0], d[1], d[2], d[3]};
         return;
@@ -91,6 +107,16 @@ __device__ inline void quad_coef(const DevSim& s, uint32_t c, F4* ik, F4* b1, F4
         a1[a] = ld4(a_m[a][0], c);
         a2[a] = ld4(a_m[a][1], c);
     }
+}
+
+// The codes of the quad at row i, column jq of a coded grid, for rows the tile updates (i <
+// nx - 8; ny % 4 == 0, so the quad lies in its row); else 0. A tile loads the codes of a lane's
+// next row one row ahead: its row gathers then do not wait behind the row's other loads (the
+// memory wait counter completes loads in issue order).
+template <bool CODED>
+__device__ inline uint2 quad_codes(const DevSim& s, int64_t i, int64_t jq) {
+    if (!CODED || i >= s.nx - kHalo) return uint2{0u, 0u};
+    return ldc4(s.rcode, (uint32_t)(i * s.ny + jq));
 }
 
 // LDS tile layout: each staged row of `pitch` floats is split into four planes of pitch / 4
@@ -179,13 +205,14 @@ __device__ inline void velocity_tile(const DevSim& s, unsigned bx, unsigned by, 
     const int64_t nx = s.nx, ny = s.ny;
     const int64_t j0 = (int64_t)bx * kTW;
     const int64_t i0 = kHalo + (int64_t)by * TX;
+    const int qd = threadIdx.x % kQuads;
+    const int ty = threadIdx.x / kQuads;
+    const int64_t jq = j0 + 4 * qd;  // first column of this lane's quad
+    uint2 cw_next = jq < ny ? quad_codes<CODED>(s, i0 + ty, jq) : uint2{0u, 0u};
     const int axes = stage_coef<UNIFORM>(lds_coef, s);
     stage_tile(P, s.p, i0 - 7, NROWS, j0, 8, 8, PITCH, nx, ny);
     __syncthreads();
 
-    const int qd = threadIdx.x % kQuads;
-    const int ty = threadIdx.x / kQuads;
-    const int64_t jq = j0 + 4 * qd;  // first column of this lane's quad
     if (jq >= ny) return;
     const float dt = s.dt;
 #pragma unroll 1
@@ -194,13 +221,15 @@ __device__ inline void velocity_tile(const DevSim& s, unsigned bx, unsigned by, 
         const int64_t i = i0 + lr;
         if (i >= nx - kHalo) break;
         const uint32_t c = (uint32_t)(i * ny + jq);
+        const uint2 cw = cw_next;
+        if (m + 1 < TX / 4) cw_next = quad_codes<CODED>(s, i + 4, jq);
         const F4 rho0 = ld4(s.rho, c);
         const F4 rho1 = ld4(s.rho, c + (uint32_t)ny);
         const float rho_r = (jq + 4 < ny) ? ld1(s.rho, c + 4) : 0.f;
         I4 dc;
         F4 ik[2], b1[2], b2[2], a1[2], a2[2];
         constexpr int NA = ISO ? 1 : 2;
-        quad_coef<CODED, UNIFORM, true, NA>(s, c, ik, b1, b2, a1, a2, dc);
+        quad_coef<CODED, UNIFORM, true, NA>(s, c, cw, ik, b1, b2, a1, a2, dc);
         F4 ps1[2], ps2[2], v[2];
         F4 tu[2];
 #pragma unroll
@@ -289,14 +318,15 @@ __device__ inline void pressure_tile(const DevSim& s, unsigned bx, unsigned by, 
     const int64_t nx = s.nx, ny = s.ny;
     const int64_t j0 = (int64_t)bx * kTW;
     const int64_t i0 = kHalo + (int64_t)by * TX;
+    const int qd = threadIdx.x % kQuads;
+    const int ty = threadIdx.x / kQuads;
+    const int64_t jq = j0 + 4 * qd;
+    uint2 cw_next = jq < ny ? quad_codes<CODED>(s, i0 + ty, jq) : uint2{0u, 0u};
     const int axes = stage_coef<UNIFORM>(lds_coef, s);
     stage_tile(VX, s.v[0], i0 - 8, XR, j0, 4, 4, XP, nx, ny);
     stage_tile(VY, s.v[1], i0 - 1, YR, j0, 8, 8, YP, nx, ny);
     __syncthreads();
 
-    const int qd = threadIdx.x % kQuads;
-    const int ty = threadIdx.x / kQuads;
-    const int64_t jq = j0 + 4 * qd;
     if (jq >= ny) return;
     const float dt = s.dt;
 #pragma unroll 1
@@ -305,12 +335,14 @@ __device__ inline void pressure_tile(const DevSim& s, unsigned bx, unsigned by, 
         const int64_t i = i0 + lr;
         if (i >= nx - kHalo) break;
         const uint32_t c = (uint32_t)(i * ny + jq);
+        const uint2 cw = cw_next;
+        if (m + 1 < TX / 4) cw_next = quad_codes<CODED>(s, i + 4, jq);
         F4 pv = ld4(s.p, c);
         const F4 K = ld4(s.K, c);
         I4 dc;
         constexpr int NA = ISO ? 1 : 2;
         F4 ik[2], b1[2], b2[2], a1[2], a2[2];
-        quad_coef<CODED, UNIFORM, false, NA>(s, c, ik, b1, b2, a1, a2, dc);
+        quad_coef<CODED, UNIFORM, false, NA>(s, c, cw, ik, b1, b2, a1, a2, dc);
         F4 ph1[2], ph2[2];
         F4 td = F4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
