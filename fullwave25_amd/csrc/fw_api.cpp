@@ -700,6 +700,57 @@ int recode(fw_sim* s) {
     return encode_relaxation(s);
 }
 
+// The boundary entries of a handle whose pressure kernel does the step's boundary work: one per
+// shot, with the handle's current pointers (step0 0: the launch passes the absolute step).
+std::vector<fw::GroupBoundary> fused_entries(const fw_sim* s, const fw::GroupBoundary& lists) {
+    std::vector<fw::GroupBoundary> out;
+    for (int b = 0; b < s->nshots; ++b) {
+        fw::GroupBoundary e = lists;
+        e.p = s->ds.p + (int64_t)b * s->ds.shot_stride;
+        e.b = s->db;
+        e.b.icmat_t = s->db.icmat_t + (int64_t)b * s->db.icmat_stride;
+        e.b.rec = s->db.rec + (int64_t)b * s->db.rec_stride;
+        e.modt = s->d.modt;
+        e.step0 = 0;
+        out.push_back(e);
+    }
+    return out;
+}
+
+// Group the handle's points by the tiles of its pressure kernel when that kernel can do the
+// step's boundary work (2D marching or tiled kernels; not for a fixed-region handle, whose
+// footprint does not count these lists).
+int setup_fused_boundary(fw_sim* s, const fw::BoundaryLists& bl) {
+    s->fused_boundary = false;
+    if (s->fixed_base || !fw::pressure_fuses_boundary(s->ds, s->cfg_p)) return FW_OK;
+    const char* env = getenv("FULLWAVE_HIP_FUSED_BOUNDARY");  // "0": a launch of its own (tests)
+    if (env && env[0] == '0') return FW_OK;
+    const fw::LaunchCfg& c = s->cfg_p;
+    const int tile_w = c.variant == 2 ? 256 : (c.width == 64 || c.width == 256 ? c.width : 128);
+    const int rows = c.variant == 2 ? c.chunk : (c.chunk < 2 ? 2 : c.chunk);
+    const unsigned gx = (unsigned)((s->ds.ny + tile_w - 1) / tile_w);
+    std::vector<int32_t> pts;
+    const fw::PointTable pt = fw::point_table(s->ds.nx, s->ds.ny, gx, tile_w, rows, bl, &pts);
+    void* p;
+    int rc;
+    if ((rc = alloc(s, "_fpoints", pts.size() * 4, &p)) != FW_OK) return rc;
+    FW_HIP(copy_sync(s, p, pts.data(), pts.size() * 4, hipMemcpyHostToDevice));
+    const int32_t* base = (const int32_t*)p;
+    fw::GroupBoundary lists_e{};
+    lists_e.ntiles = pt.ntiles;
+    lists_e.off = base + pt.off;
+    lists_e.src = reinterpret_cast<const int4*>(base + pt.src);
+    lists_e.zero = reinterpret_cast<const int64_t*>(base + pt.zero);
+    lists_e.out = reinterpret_cast<const int4*>(base + pt.out);
+    s->fbound_h = fused_entries(s, lists_e);
+    if ((rc = alloc(s, "_fbound", s->fbound_h.size() * sizeof(fw::GroupBoundary), &p)) != FW_OK) return rc;
+    s->fbound = (fw::GroupBoundary*)p;
+    FW_HIP(copy_sync(s, s->fbound, s->fbound_h.data(), s->fbound_h.size() * sizeof(fw::GroupBoundary),
+                     hipMemcpyHostToDevice));
+    s->fused_boundary = true;
+    return FW_OK;
+}
+
 // Resolve coordinate lists into device index arrays and build the coefficient table.
 int finalize(fw_sim* s) {
     const fw_desc& d = s->d;
@@ -783,6 +834,7 @@ int finalize(fw_sim* s) {
     s->ds.isotropic = d.isotropic;
     if ((rc = encode_relaxation(s)) != FW_OK) return rc;
     if ((rc = choose_kernels(s)) != FW_OK) return rc;
+    if ((rc = setup_fused_boundary(s, bl)) != FW_OK) return rc;
     s->ready = true;
     return FW_OK;
 }
@@ -805,10 +857,17 @@ int launch_step(fw_sim* s, bool timed, double* ms) {
     int rc;
     rc = timed_launch(fw::kVelocity, [&] { return fw::launch_velocity(s->ds, s->cfg_v, s->stream); });
     if (rc != FW_OK) return rc;
-    rc = timed_launch(fw::kPressure, [&] { return fw::launch_pressure(s->ds, s->cfg_p, s->stream); });
-    if (rc != FW_OK) return rc;
-    rc = timed_launch(fw::kBoundary, [&] { return fw::launch_boundary(s->ds, s->db, s->step, s->d.modt, s->stream); });
-    if (rc != FW_OK) return rc;
+    if (s->fused_boundary && !s->ds.tape_div) {  // the pressure launch does the boundary work
+        const fw::GroupStep gs{s->fbound, nullptr, s->step};
+        rc = timed_launch(fw::kPressure, [&] { return fw::launch_pressure(s->ds, s->cfg_p, s->stream, gs); });
+        if (rc != FW_OK) return rc;
+    } else {
+        rc = timed_launch(fw::kPressure, [&] { return fw::launch_pressure(s->ds, s->cfg_p, s->stream); });
+        if (rc != FW_OK) return rc;
+        rc = timed_launch(fw::kBoundary,
+                          [&] { return fw::launch_boundary(s->ds, s->db, s->step, s->d.modt, s->stream); });
+        if (rc != FW_OK) return rc;
+    }
     s->step += 1;
     s->state_zero = false;
     return FW_OK;
@@ -823,8 +882,12 @@ int capture_graph(fw_sim* s) {
     hipError_t e = hipSuccess;
     for (int64_t it = 0; it < s->graph_steps && e == hipSuccess; ++it) {
         e = fw::launch_velocity(s->ds, s->cfg_v, s->stream);
-        if (e == hipSuccess) e = fw::launch_pressure(s->ds, s->cfg_p, s->stream);
-        if (e == hipSuccess) e = fw::launch_boundary_dev(s->ds, s->db, s->step_dev, s->d.modt, s->stream);
+        if (e == hipSuccess && s->fused_boundary) {
+            e = fw::launch_pressure(s->ds, s->cfg_p, s->stream, fw::GroupStep{s->fbound, s->step_dev, 0});
+        } else {
+            if (e == hipSuccess) e = fw::launch_pressure(s->ds, s->cfg_p, s->stream);
+            if (e == hipSuccess) e = fw::launch_boundary_dev(s->ds, s->db, s->step_dev, s->d.modt, s->stream);
+        }
         if (e == hipSuccess) e = fw::launch_tick(s->step_dev, -1, s->stream);
     }
     hipError_t e2 = hipStreamEndCapture(s->stream, &graph);
@@ -868,6 +931,15 @@ int run_impl(fw_sim* s, int64_t n_steps, bool timed, double* ms) {
     if (s->step + n_steps > s->d.nt)
         return fail(FW_ERR_INVALID, "running %lld steps from step %lld exceeds nt = %lld", (long long)n_steps,
                     (long long)s->step, (long long)s->d.nt);
+    if (s->fused_boundary) {  // the entries follow the handle's buffers (a source matrix re-uploaded)
+        std::vector<fw::GroupBoundary> now = fused_entries(s, s->fbound_h[0]);
+        if (memcmp(now.data(), s->fbound_h.data(), now.size() * sizeof(fw::GroupBoundary)) != 0) {
+            s->fbound_h = now;
+            FW_HIP(hipMemcpyAsync(s->fbound, s->fbound_h.data(), now.size() * sizeof(fw::GroupBoundary),
+                                  hipMemcpyHostToDevice, s->stream));
+            FW_HIP(hipStreamSynchronize(s->stream));
+        }
+    }
     int64_t left = n_steps;
     if (!timed && s->graph_steps > 0 && left >= s->graph_steps) {
         int rc = run_graph(s, &left);

@@ -61,6 +61,42 @@ void points_by_tile(int64_t nx, int64_t ny, unsigned gx, int tile_w, int rows, c
         if (owner[q] >= 0) (*idx)[(size_t)fill[(size_t)owner[q]]++] = (int32_t)q;
 }
 
+PointTable point_table(int64_t nx, int64_t ny, unsigned gx, int tile_w, int rows, const BoundaryLists& bl,
+                       std::vector<int32_t>* tab) {
+    auto align4 = [&] {
+        while (tab->size() % 4) tab->push_back(0);
+    };
+    auto push_lin = [&](int64_t lin) {
+        tab->push_back((int32_t)(uint32_t)((uint64_t)lin & 0xffffffffu));
+        tab->push_back((int32_t)(uint32_t)((uint64_t)lin >> 32));
+    };
+    std::vector<int32_t> off[3], idx[3];
+    const std::vector<int64_t>* lists[3] = {&bl.src_lin, &bl.zero_lin, &bl.out_lin};
+    for (int k = 0; k < 3; ++k) points_by_tile(nx, ny, gx, tile_w, rows, *lists[k], &off[k], &idx[k]);
+    PointTable pt{};
+    pt.ntiles = (int32_t)(off[0].size() - 1);
+    align4();
+    pt.off = tab->size();
+    for (int k = 0; k < 3; ++k) tab->insert(tab->end(), off[k].begin(), off[k].end());
+    align4();
+    pt.src = tab->size();
+    for (int32_t q : idx[0]) {  // {source row, 0, voxel}
+        tab->push_back(q);
+        tab->push_back(0);
+        push_lin(bl.src_lin[(size_t)q]);
+    }
+    pt.zero = tab->size();
+    for (int32_t q : idx[1]) push_lin(bl.zero_lin[(size_t)q]);
+    align4();
+    pt.out = tab->size();
+    for (int32_t q : idx[2]) {  // {sensor column, source row at the voxel, voxel or -1 on a release point}
+        tab->push_back(q);
+        tab->push_back(bl.out_src[(size_t)q]);
+        push_lin(bl.out_air[(size_t)q] ? -1 : bl.out_lin[(size_t)q]);
+    }
+    return pt;
+}
+
 }  // namespace fw
 
 namespace {
@@ -115,16 +151,11 @@ void release(fw_group* g) {
     delete g;
 }
 
-// Offsets of one member's lists in the group's point table.
-struct PointSlots {
-    size_t src_off, src_idx, zero_off, zero_idx, out_off, out_idx;
-};
-
 int build(fw_group* g) {
     std::vector<fw::GroupTile> tiles;
     std::vector<fw::GroupBoundary> bounds;
     std::vector<int32_t> pts;
-    std::vector<PointSlots> slots;  // per bounds entry
+    std::vector<fw::PointTable> slots;  // per bounds entry
     uint64_t blocks = 0;
     for (fw_sim* s : g->sims) {
         unsigned gx, gy;
@@ -147,17 +178,7 @@ int build(fw_group* g) {
         int rc = fw::resolve_boundary(s->d, s->ds.sx, s->ds.sy, s->icc, s->outc, s->zero, &bl);
         if (rc != FW_OK) return rc;
         const int tile_w = g->variant == 2 ? 256 : g->width;
-        PointSlots ps{};
-        std::vector<int32_t> off, idx;
-        const std::vector<int64_t>* lists[3] = {&bl.src_lin, &bl.zero_lin, &bl.out_lin};
-        size_t* at[3][2] = {{&ps.src_off, &ps.src_idx}, {&ps.zero_off, &ps.zero_idx}, {&ps.out_off, &ps.out_idx}};
-        for (int k = 0; k < 3; ++k) {
-            fw::points_by_tile(s->ds.nx, s->ds.ny, gx, tile_w, (int)chunk, *lists[k], &off, &idx);
-            *at[k][0] = pts.size();
-            pts.insert(pts.end(), off.begin(), off.end());
-            *at[k][1] = pts.size();
-            pts.insert(pts.end(), idx.begin(), idx.end());
-        }
+        const fw::PointTable ps = fw::point_table(s->ds.nx, s->ds.ny, gx, tile_w, (int)chunk, bl, &pts);
         for (int b = 0; b < s->nshots; ++b) {
             fw::GroupTile t{};
             t.s = fw::shot_view(s->ds, b);
@@ -191,14 +212,13 @@ int build(fw_group* g) {
     GR_HIP(alloc_table(g, (void**)&g->bounds, bounds.size() * sizeof(fw::GroupBoundary)));
     GR_HIP(alloc_table(g, (void**)&g->points, pts.size() * sizeof(int32_t)));
     for (size_t k = 0; k < bounds.size(); ++k) {
-        const PointSlots& ps = slots[k];
+        const fw::PointTable& ps = slots[k];
         fw::GroupBoundary& e = bounds[k];
-        e.src_off = g->points + ps.src_off;
-        e.src_idx = g->points + ps.src_idx;
-        e.zero_off = g->points + ps.zero_off;
-        e.zero_idx = g->points + ps.zero_idx;
-        e.out_off = g->points + ps.out_off;
-        e.out_idx = g->points + ps.out_idx;
+        e.ntiles = ps.ntiles;
+        e.off = g->points + ps.off;
+        e.src = reinterpret_cast<const int4*>(g->points + ps.src);
+        e.zero = reinterpret_cast<const int64_t*>(g->points + ps.zero);
+        e.out = reinterpret_cast<const int4*>(g->points + ps.out);
     }
     GR_HIP(hipMemcpyAsync(g->points, pts.data(), pts.size() * sizeof(int32_t), hipMemcpyHostToDevice, g->stream));
     GR_HIP(hipMemcpyAsync(g->tiles, tiles.data(), tiles.size() * sizeof(fw::GroupTile), hipMemcpyHostToDevice,

@@ -227,56 +227,80 @@ __device__ inline int group_entry(const GroupTile* __restrict__ t, int n) {
     return lo;
 }
 
-// The source, pressure-release and sensor points of one grid (or shot) of a group, for the
-// group's pressure launch: each workgroup, once its tile is updated, injects the sources, zeroes
-// the pressure-release points and records the sensors whose voxels its tile owns. Points are
-// grouped by owning tile: those of tile t are idx[off[t] .. off[t + 1]), as indices into the
-// grid's point lists (DevBoundary). A voxel in the ghost border belongs to the tile that holds
-// its row and column clamped to the updated region.
+// The source, pressure-release and sensor points of one grid (or shot), for a pressure launch
+// that does the step's boundary work (grouped launches, 2D single grids): each workgroup, once
+// its tile is updated, injects the sources, zeroes the pressure-release points and records the
+// sensors whose voxels its tile owns. Points are grouped by owning tile, tile t's being
+// src[off[t] .. off[t + 1]), zero[off[T + 1 + t] .. off[T + 2 + t]) and
+// out[off[2 T + 2 + t] .. off[2 T + 3 + t]) for T tiles; each entry carries what the pass needs
+// (no index to look up), so the work after the tile is one load and one store a point. A voxel in
+// the ghost border belongs to the tile that holds its row and column clamped to the updated region.
 struct GroupBoundary {
     float* p;             // this shot's pressure field
     DevBoundary b;        // icmat_t and rec already offset to this shot
     int64_t modt;
-    int64_t step0;        // the grid's step when the current fw_group_run call began
-    const int32_t* src_off;   // [tiles + 1]
-    const int32_t* src_idx;   // source rows (into src_lin and the icmat rows)
-    const int32_t* zero_off;  // [tiles + 1]
-    const int32_t* zero_idx;  // into zero_lin
-    const int32_t* out_off;   // [tiles + 1]
-    const int32_t* out_idx;   // sensor columns (into out_lin, out_src, out_air and the records)
+    int64_t step0;        // the grid's step when the current run call began
+    int32_t ntiles;       // T
+    const int32_t* off;   // [3][T + 1]
+    const int4* src;      // {source row, 0, voxel (int64, low and high word)}: sources not dropped
+    const int64_t* zero;  // pressure-release voxels
+    const int4* out;      // {sensor column, source row at the voxel or -1, voxel or -1 on a release point}
 };
 
-// Step n's sources, pressure release and sensor records at the points that tile `tile` of grid
-// `e` owns, by the lanes of one workgroup after it has updated the tile's pressure (the caller
-// synchronizes the workgroup first). The semantics are the plain boundary pass's (k_boundary):
-// a source dropped for a pressure-release point or a later repeat has lin == -1, and a sensor
-// on a source or release point records the value written there, so no lane reads what another
-// lane of this pass writes.
-__device__ inline void tile_boundary(const GroupBoundary& e, unsigned tile, int64_t n) {
-    const DevBoundary& b = e.b;
-    float* p = e.p;
-    const bool inject = n < b.ntic;
-    if (inject) {
-        for (int32_t k = e.src_off[tile] + (int32_t)threadIdx.x; k < e.src_off[tile + 1]; k += (int32_t)blockDim.x) {
-            const int32_t t = e.src_idx[k];
-            const int64_t lin = b.src_lin[t];
-            if (lin >= 0) p[lin] = b.icmat_t[n * b.ncoords + t];
+// What a workgroup needs for the boundary work of its tile at step n, read when it starts (so
+// that the work after the tile waits for no table lookups): the tile's point ranges and the
+// grid's pointers and sizes.
+struct TileWork {
+    float* p;
+    const float* icmat;  // row n of the source signals
+    float* rec;          // record row of step n (null: step n records nothing)
+    const int4* src;
+    const int64_t* zero;
+    const int4* out;
+    int32_t s0, s1, z0, z1, o0, o1;
+    bool inject;
+};
+__device__ inline TileWork tile_work(const GroupBoundary& e, unsigned tile, int64_t n) {
+    const int32_t t1 = e.ntiles + 1;
+    TileWork w;
+    w.p = e.p;
+    w.inject = n < e.b.ntic;
+    w.icmat = e.b.icmat_t + n * e.b.ncoords;
+    w.rec = (e.modt > 0 && n % e.modt == 0) ? e.b.rec + (n / e.modt) * e.b.nout : nullptr;
+    w.src = e.src;
+    w.zero = e.zero;
+    w.out = e.out;
+    w.s0 = e.off[tile];
+    w.s1 = e.off[tile + 1];
+    w.z0 = e.off[t1 + tile];
+    w.z1 = e.off[t1 + tile + 1];
+    w.o0 = e.off[2 * t1 + tile];
+    w.o1 = e.off[2 * t1 + tile + 1];
+    return w;
+}
+
+// Step n's sources, pressure release and sensor records at the points of one tile (`w`), by the
+// lanes of one workgroup after it has updated the tile's pressure (the caller synchronizes the
+// workgroup first). The semantics are the plain boundary pass's (k_boundary): a source dropped
+// for a pressure-release point or a later repeat is not listed, and a sensor on a source or
+// release point records the value written there, so no lane reads what another lane of this pass
+// writes.
+__device__ inline void tile_boundary(const TileWork& w) {
+    const int32_t lane = (int32_t)threadIdx.x, step = (int32_t)blockDim.x;
+    if (w.inject) {
+        for (int32_t k = w.s0 + lane; k < w.s1; k += step) {
+            const int4 q = w.src[k];
+            w.p[(int64_t)(((uint64_t)(uint32_t)q.w << 32) | (uint32_t)q.z)] = w.icmat[q.x];
         }
     }
-    for (int32_t k = e.zero_off[tile] + (int32_t)threadIdx.x; k < e.zero_off[tile + 1]; k += (int32_t)blockDim.x)
-        p[b.zero_lin[e.zero_idx[k]]] = 0.f;
-    if (e.modt <= 0 || n % e.modt != 0) return;
-    const int64_t rec_row = n / e.modt;
-    for (int32_t k = e.out_off[tile] + (int32_t)threadIdx.x; k < e.out_off[tile + 1]; k += (int32_t)blockDim.x) {
-        const int32_t t = e.out_idx[k];
-        float val;
-        if (b.out_air[t]) {
-            val = 0.f;
-        } else {
-            const int32_t src = b.out_src[t];
-            val = (inject && src >= 0) ? b.icmat_t[n * b.ncoords + src] : p[b.out_lin[t]];
-        }
-        b.rec[rec_row * b.nout + t] = val;
+    for (int32_t k = w.z0 + lane; k < w.z1; k += step) w.p[w.zero[k]] = 0.f;
+    if (!w.rec) return;
+    for (int32_t k = w.o0 + lane; k < w.o1; k += step) {
+        const int4 q = w.out[k];
+        const int64_t lin = (int64_t)(((uint64_t)(uint32_t)q.w << 32) | (uint32_t)q.z);
+        float val = 0.f;  // a pressure-release point
+        if (lin >= 0) val = (w.inject && q.y >= 0) ? w.icmat[q.y] : w.p[lin];
+        w.rec[q.x] = val;
     }
 }
 
@@ -309,7 +333,11 @@ bool streaming_supported(const DevSim& s);
 bool tile2d_supported(const DevSim& s);
 hipError_t launch_velocity(const DevSim& s, const LaunchCfg& cfg, hipStream_t st);
 hipError_t launch_pipe3d(const DevSim& s, const LaunchCfg& cfg, bool velocity, hipStream_t st);
-hipError_t launch_pressure(const DevSim& s, const LaunchCfg& cfg, hipStream_t st);
+// gs.g non-null (2D marching and tiled kernels, untaped steps: pressure_fuses_boundary): the
+// pressure launch also does the step's sources, pressure release and sensors (tile_boundary;
+// gs.g holds one entry per shot, whose points are grouped by the tiles of `cfg`)
+hipError_t launch_pressure(const DevSim& s, const LaunchCfg& cfg, hipStream_t st, const GroupStep& gs = GroupStep{});
+bool pressure_fuses_boundary(const DevSim& s, const LaunchCfg& cfg);
 hipError_t launch_boundary(const DevSim& s, const DevBoundary& b, int64_t step, int64_t modt,
                            hipStream_t st);
 hipError_t launch_boundary_dev(const DevSim& s, const DevBoundary& b, const int64_t* step, int64_t modt,

@@ -347,10 +347,22 @@ __device__ __forceinline__ void pressure_2d_tile(const DevSim& s, unsigned bx, u
 
 // at most 96 VGPRs (five waves per SIMD instead of four): measured 3-15 % faster across the
 // 128/256-lane shapes, for a few bytes of spill per lane
+// gs.g non-null: each workgroup then also does the step's source, pressure-release and sensor
+// work of its tile (tile blockIdx.y * gridDim.x + blockIdx.x of shot blockIdx.z), as the group
+// kernels do
 template <int TY, bool UNIFORM, bool ISO, bool CODED>
-__global__ __launch_bounds__(TY) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_pressure_2d(DevSim batch, int chunk) {
+__global__ __launch_bounds__(TY) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_pressure_2d(DevSim batch, int chunk,
+                                                                                               GroupStep gs) {
     extern __shared__ float lds_coef[];
+    TileWork w{};
+    if (gs.g) {
+        const GroupBoundary& e = gs.g[blockIdx.z];
+        w = tile_work(e, blockIdx.y * gridDim.x + blockIdx.x, e.step0 + gs.it + (gs.base ? *gs.base : 0));
+    }
     pressure_2d_tile<TY, UNIFORM, ISO, CODED>(shot_view(batch, blockIdx.z), blockIdx.x, blockIdx.y, chunk, lds_coef);
+    if (!gs.g) return;
+    __syncthreads();
+    tile_boundary(w);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -379,16 +391,17 @@ __global__ __launch_bounds__(TY) __attribute__((amdgpu_waves_per_eu(5, 8))) void
     const int chunk = t[m].chunk;
     const unsigned local = blockIdx.x - t[m].start;
     const unsigned bx = local % t[m].gx, by = local / t[m].gx;
+    TileWork w{};
+    if (gs.g) w = tile_work(gs.g[m], local, gs.g[m].step0 + gs.it + (gs.base ? *gs.base : 0));
     if (s.rcode) pressure_2d_tile<TY, false, true, true>(s, bx, by, chunk, lds_coef);
     else pressure_2d_tile<TY, false, true, false>(s, bx, by, chunk, lds_coef);
     if (!gs.g) return;
     __syncthreads();
-    const GroupBoundary& e = gs.g[m];
-    tile_boundary(e, local, e.step0 + gs.it + (gs.base ? *gs.base : 0));
+    tile_boundary(w);
 }
 
 template <int TY>
-hipError_t launch_2d(const DevSim& s, int chunk, bool velocity, hipStream_t st) {
+hipError_t launch_2d(const DevSim& s, int chunk, bool velocity, const GroupStep& gs, hipStream_t st) {
     const bool u = s.ndmap == 1;
     const int lds = (u ? 1 : s.ndmap) * 2 * kNCoef * 4;
     const int64_t rows = s.nx - 2 * kHalo;
@@ -403,22 +416,22 @@ hipError_t launch_2d(const DevSim& s, int chunk, bool velocity, hipStream_t st) 
         else if (s.isotropic) hipLaunchKernelGGL((k_velocity_2d<TY, false, true, false>), g, b, lds, st, s, chunk);
         else hipLaunchKernelGGL((k_velocity_2d<TY, false, false, false>), g, b, lds, st, s, chunk);
     } else {
-        if (u && coded) hipLaunchKernelGGL((k_pressure_2d<TY, true, true, true>), g, b, lds, st, s, chunk);
-        else if (u && s.isotropic) hipLaunchKernelGGL((k_pressure_2d<TY, true, true, false>), g, b, lds, st, s, chunk);
-        else if (u) hipLaunchKernelGGL((k_pressure_2d<TY, true, false, false>), g, b, lds, st, s, chunk);
-        else if (coded) hipLaunchKernelGGL((k_pressure_2d<TY, false, true, true>), g, b, lds, st, s, chunk);
-        else if (s.isotropic) hipLaunchKernelGGL((k_pressure_2d<TY, false, true, false>), g, b, lds, st, s, chunk);
-        else hipLaunchKernelGGL((k_pressure_2d<TY, false, false, false>), g, b, lds, st, s, chunk);
+        if (u && coded) hipLaunchKernelGGL((k_pressure_2d<TY, true, true, true>), g, b, lds, st, s, chunk, gs);
+        else if (u && s.isotropic) hipLaunchKernelGGL((k_pressure_2d<TY, true, true, false>), g, b, lds, st, s, chunk, gs);
+        else if (u) hipLaunchKernelGGL((k_pressure_2d<TY, true, false, false>), g, b, lds, st, s, chunk, gs);
+        else if (coded) hipLaunchKernelGGL((k_pressure_2d<TY, false, true, true>), g, b, lds, st, s, chunk, gs);
+        else if (s.isotropic) hipLaunchKernelGGL((k_pressure_2d<TY, false, true, false>), g, b, lds, st, s, chunk, gs);
+        else hipLaunchKernelGGL((k_pressure_2d<TY, false, false, false>), g, b, lds, st, s, chunk, gs);
     }
     return hipGetLastError();
 }
 
-hipError_t launch_2d_any(const DevSim& s, const LaunchCfg& cfg, bool velocity, hipStream_t st) {
+hipError_t launch_2d_any(const DevSim& s, const LaunchCfg& cfg, bool velocity, const GroupStep& gs, hipStream_t st) {
     const int chunk = cfg.chunk < 2 ? 2 : cfg.chunk;
     switch (cfg.width) {
-        case 64: return launch_2d<64>(s, chunk, velocity, st);
-        case 256: return launch_2d<256>(s, chunk, velocity, st);
-        default: return launch_2d<128>(s, chunk, velocity, st);
+        case 64: return launch_2d<64>(s, chunk, velocity, gs, st);
+        case 256: return launch_2d<256>(s, chunk, velocity, gs, st);
+        default: return launch_2d<128>(s, chunk, velocity, gs, st);
     }
 }
 
@@ -458,7 +471,7 @@ bool streaming_supported(const DevSim& s) {
 }
 
 bool tile2d_supported(const DevSim& s);
-hipError_t launch_tile2d(const DevSim& s, const LaunchCfg& cfg, bool velocity, hipStream_t st);
+hipError_t launch_tile2d(const DevSim& s, const LaunchCfg& cfg, bool velocity, const GroupStep& gs, hipStream_t st);
 
 bool tape_capable(const DevSim& s, const LaunchCfg& cfg) {
     return cfg.variant == 0 || (cfg.variant == 2 && tile2d_supported(s));
@@ -467,23 +480,30 @@ bool tape_capable(const DevSim& s, const LaunchCfg& cfg) {
 // A step that writes the adjoint tape runs the tiled 2D kernels when `cfg` asks for them and the
 // per-cell kernels otherwise (the only families that write it).
 hipError_t launch_velocity(const DevSim& s, const LaunchCfg& cfg, hipStream_t st) {
-    if (cfg.variant == 2 && tile2d_supported(s)) return launch_tile2d(s, cfg, true, st);
+    if (cfg.variant == 2 && tile2d_supported(s)) return launch_tile2d(s, cfg, true, GroupStep{}, st);
     if (s.tape_div) return launch_velocity_simple(s, st);
     if ((cfg.variant == 3 || cfg.variant == 4) && s.ndim == 3 && streaming_supported(s))
         return launch_pipe3d(s, cfg, true, st);
     if (cfg.variant == 0 || !streaming_supported(s)) return launch_velocity_simple(s, st);
     if (s.ndim == 3) return launch_velocity_3d(s, cfg, st);
-    return launch_2d_any(s, cfg, true, st);
+    return launch_2d_any(s, cfg, true, GroupStep{}, st);
 }
 
-hipError_t launch_pressure(const DevSim& s, const LaunchCfg& cfg, hipStream_t st) {
-    if (cfg.variant == 2 && tile2d_supported(s)) return launch_tile2d(s, cfg, false, st);
+bool pressure_fuses_boundary(const DevSim& s, const LaunchCfg& cfg) {
+    if (s.tape_div || s.ndim != 2) return false;
+    if (cfg.variant == 2) return tile2d_supported(s);
+    return cfg.variant == 1 && streaming_supported(s);
+}
+
+hipError_t launch_pressure(const DevSim& s, const LaunchCfg& cfg, hipStream_t st, const GroupStep& gs) {
+    if (gs.g && !pressure_fuses_boundary(s, cfg)) return hipErrorInvalidValue;
+    if (cfg.variant == 2 && tile2d_supported(s)) return launch_tile2d(s, cfg, false, gs, st);
     if (s.tape_div) return launch_pressure_simple(s, st);
     if ((cfg.variant == 3 || cfg.variant == 4) && s.ndim == 3 && streaming_supported(s))
         return launch_pipe3d(s, cfg, false, st);
     if (cfg.variant == 0 || !streaming_supported(s)) return launch_pressure_simple(s, st);
     if (s.ndim == 3) return launch_pressure_3d(s, cfg, st);
-    return launch_2d_any(s, cfg, false, st);
+    return launch_2d_any(s, cfg, false, gs, st);
 }
 
 }  // namespace fw

@@ -408,11 +408,22 @@ __global__ __launch_bounds__(kLanes) void k_velocity_tile(DevSim batch) {
     velocity_tile<TX, UNIFORM, ISO, CODED, TAPE>(shot_view(batch, blockIdx.z), blockIdx.x, blockIdx.y, L, lds_coef);
 }
 
+// gs.g non-null: each workgroup then also does the step's source, pressure-release and sensor
+// work of its tile (tile blockIdx.y * gridDim.x + blockIdx.x of shot blockIdx.z), as the group
+// kernels do
 template <int TX, bool UNIFORM, bool ISO, bool CODED, bool TAPE>
-__global__ __launch_bounds__(kLanes) void k_pressure_tile(DevSim batch) {
+__global__ __launch_bounds__(kLanes) void k_pressure_tile(DevSim batch, GroupStep gs) {
     __shared__ __attribute__((aligned(16))) PreTile<TX> L;
     extern __shared__ __attribute__((aligned(16))) float lds_coef[];
+    TileWork w{};
+    if (gs.g) {
+        const GroupBoundary& e = gs.g[blockIdx.z];
+        w = tile_work(e, blockIdx.y * gridDim.x + blockIdx.x, e.step0 + gs.it + (gs.base ? *gs.base : 0));
+    }
     pressure_tile<TX, UNIFORM, ISO, CODED, TAPE>(shot_view(batch, blockIdx.z), blockIdx.x, blockIdx.y, L, lds_coef);
+    if (!gs.g) return;
+    __syncthreads();
+    tile_boundary(w);
 }
 
 // grouped launches: the tiles of several grids (different sizes and media, isotropic
@@ -439,12 +450,13 @@ __global__ __launch_bounds__(kLanes) void k_pressure_tile_group(const GroupTile*
     const DevSim s = t[m].s;
     const unsigned local = blockIdx.x - t[m].start;
     const unsigned bx = local % t[m].gx, by = local / t[m].gx;
+    TileWork w{};
+    if (gs.g) w = tile_work(gs.g[m], local, gs.g[m].step0 + gs.it + (gs.base ? *gs.base : 0));
     if (ALL_CODED || s.rcode) pressure_tile<TX, false, true, true>(s, bx, by, L, lds_coef);
     else pressure_tile<TX, false, true, false>(s, bx, by, L, lds_coef);
     if (!gs.g) return;
     __syncthreads();
-    const GroupBoundary& e = gs.g[m];
-    tile_boundary(e, local, e.step0 + gs.it + (gs.base ? *gs.base : 0));
+    tile_boundary(w);
 }
 
 template <int TX, bool ALL_CODED>
@@ -463,7 +475,7 @@ hipError_t launch_group_tile_any(const GroupTile* t, int n, uint32_t blocks, int
 }
 
 template <int TX, bool TAPE = false>
-hipError_t launch_tile(const DevSim& s, bool velocity, hipStream_t st) {
+hipError_t launch_tile(const DevSim& s, bool velocity, const GroupStep& gs, hipStream_t st) {
     const bool u = s.ndmap == 1;
     const int lds = (u ? 1 : s.ndmap) * (s.coef_axes_same ? 1 : 2) * kCS * 4;
     const int64_t rows = s.nx - 2 * kHalo;
@@ -478,12 +490,12 @@ hipError_t launch_tile(const DevSim& s, bool velocity, hipStream_t st) {
         else if (s.isotropic) hipLaunchKernelGGL((k_velocity_tile<TX, false, true, false, TAPE>), g, b, lds, st, s);
         else hipLaunchKernelGGL((k_velocity_tile<TX, false, false, false, TAPE>), g, b, lds, st, s);
     } else {
-        if (u && coded) hipLaunchKernelGGL((k_pressure_tile<TX, true, true, true, TAPE>), g, b, lds, st, s);
-        else if (u && s.isotropic) hipLaunchKernelGGL((k_pressure_tile<TX, true, true, false, TAPE>), g, b, lds, st, s);
-        else if (u) hipLaunchKernelGGL((k_pressure_tile<TX, true, false, false, TAPE>), g, b, lds, st, s);
-        else if (coded) hipLaunchKernelGGL((k_pressure_tile<TX, false, true, true, TAPE>), g, b, lds, st, s);
-        else if (s.isotropic) hipLaunchKernelGGL((k_pressure_tile<TX, false, true, false, TAPE>), g, b, lds, st, s);
-        else hipLaunchKernelGGL((k_pressure_tile<TX, false, false, false, TAPE>), g, b, lds, st, s);
+        if (u && coded) hipLaunchKernelGGL((k_pressure_tile<TX, true, true, true, TAPE>), g, b, lds, st, s, gs);
+        else if (u && s.isotropic) hipLaunchKernelGGL((k_pressure_tile<TX, true, true, false, TAPE>), g, b, lds, st, s, gs);
+        else if (u) hipLaunchKernelGGL((k_pressure_tile<TX, true, false, false, TAPE>), g, b, lds, st, s, gs);
+        else if (coded) hipLaunchKernelGGL((k_pressure_tile<TX, false, true, true, TAPE>), g, b, lds, st, s, gs);
+        else if (s.isotropic) hipLaunchKernelGGL((k_pressure_tile<TX, false, true, false, TAPE>), g, b, lds, st, s, gs);
+        else hipLaunchKernelGGL((k_pressure_tile<TX, false, false, false, TAPE>), g, b, lds, st, s, gs);
     }
     return hipGetLastError();
 }
@@ -518,12 +530,12 @@ hipError_t launch_group_tile2d(const GroupTile* t, int n, uint32_t blocks, int r
     }
 }
 
-hipError_t launch_tile2d(const DevSim& s, const LaunchCfg& cfg, bool velocity, hipStream_t st) {
-    if (s.tape_div) return launch_tile<8, true>(s, velocity, st);  // taped steps: 8-row tiles
+hipError_t launch_tile2d(const DevSim& s, const LaunchCfg& cfg, bool velocity, const GroupStep& gs, hipStream_t st) {
+    if (s.tape_div) return launch_tile<8, true>(s, velocity, GroupStep{}, st);  // taped steps: 8-row tiles
     switch (cfg.chunk) {
-        case 8: return launch_tile<8>(s, velocity, st);
-        case 32: return launch_tile<32>(s, velocity, st);
-        default: return launch_tile<16>(s, velocity, st);
+        case 8: return launch_tile<8>(s, velocity, gs, st);
+        case 32: return launch_tile<32>(s, velocity, gs, st);
+        default: return launch_tile<16>(s, velocity, gs, st);
     }
 }
 

@@ -38,6 +38,11 @@ struct fw_sim {
     int64_t step = 0;
     int64_t rec_cap = 0;
     fw::LaunchCfg cfg_v, cfg_p;  // velocity and pressure kernels, chosen independently
+    // 2D marching or tiled pressure kernels: the step's sources, pressure release and sensors
+    // run inside the pressure launch (one entry per shot, points grouped by the kernel's tiles)
+    bool fused_boundary = false;
+    fw::GroupBoundary* fbound = nullptr;
+    std::vector<fw::GroupBoundary> fbound_h;  // what fbound holds
     bool pinned[2] = {false, false};  // set by fw_sim_set_kernel_config: not autotuned
     fw::LaunchCfg pin[2];
     bool fields_set = false;  // a field was set before the first run: do not tune on it
@@ -91,6 +96,15 @@ int resolve_boundary(const fw_desc& d, int64_t sx, int64_t sy, const std::vector
 // point in the ghost border belongs to the tile of its row clamped to the updated rows.
 void points_by_tile(int64_t nx, int64_t ny, unsigned gx, int tile_w, int rows, const std::vector<int64_t>& lin,
                     std::vector<int32_t>* off, std::vector<int32_t>* idx);
+// The point table of one grid for a pressure launch that does the step's boundary work
+// (GroupBoundary), appended to `tab` (int32 words; `tab` starts 16-byte aligned on the device):
+// the word offsets of its parts and its tile count.
+struct PointTable {
+    size_t off, src, zero, out;
+    int32_t ntiles;
+};
+PointTable point_table(int64_t nx, int64_t ny, unsigned gx, int tile_w, int rows, const BoundaryLists& bl,
+                       std::vector<int32_t>* tab);
 // Coefficient table [ndmap][ndim][9] (w_1..w_8, e) from dmap [9][2][ndmap] with each axis's spacing
 // folded in; *axes_same = 1 when every axis has the same set for each sound speed.
 std::vector<float> coefficient_table(const fw_desc& d, const std::vector<float>& dmap, int* axes_same);

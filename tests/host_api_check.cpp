@@ -3,7 +3,8 @@
 // host before any device call: descriptor validation, the directory contract's scalar files and
 // the allocation-free run's size check, coordinate-to-index resolution of sources, air points and
 // sensors (repeats, overlaps, the grid's corners, out-of-grid points), the coefficient table, and
-// the grouping of those points by the tile of a grouped launch that owns them.
+// the grouping of those points by the tile of a grouped launch that owns them, and the point table
+// the pressure launches read.
 #include <algorithm>
 #include <cstdio>
 #include <cstdlib>
@@ -238,6 +239,48 @@ static void point_tiles(unsigned seed) {
         }
     }
     for (size_t q = 0; q < lin.size(); ++q) CHECK(seen[q] == (lin[q] >= 0 ? 1 : 0));
+
+    // the point table built from such lists: every listed point once, in its owning tile's range,
+    // with the fields the boundary pass reads (the voxel, the source row, -1 on a release point)
+    fw::BoundaryLists bl;
+    bl.src_lin = lin;
+    for (size_t q = 0; q < lin.size(); ++q)
+        if (lin[q] >= 0 && rng() % 3 == 0) bl.zero_lin.push_back(lin[q]);
+    for (size_t q = 0; q < lin.size(); ++q) {
+        bl.out_lin.push_back(lin[q] >= 0 ? lin[q] : (int64_t)(rng() % (uint64_t)(nx * ny)));
+        bl.out_src.push_back(rng() % 2 ? (int32_t)q : -1);
+        bl.out_air.push_back((int8_t)(rng() % 5 == 0));
+    }
+    std::vector<int32_t> tab(3, 7);  // the table is appended to
+    const fw::PointTable pt = fw::point_table(nx, ny, gx, tile_w, rows, bl, &tab);
+    CHECK(pt.ntiles == (int32_t)(gx * gy) && pt.off % 4 == 0 && pt.src % 4 == 0 && pt.zero % 2 == 0 && pt.out % 4 == 0);
+    auto lin_at = [&](size_t w) { return (int64_t)(((uint64_t)(uint32_t)tab[w + 1] << 32) | (uint32_t)tab[w]); };
+    const size_t t1 = (size_t)pt.ntiles + 1;
+    std::vector<int> src_seen(lin.size(), 0), out_seen(lin.size(), 0);
+    size_t zeros = 0;
+    for (size_t t = 0; t < (size_t)pt.ntiles; ++t) {
+        for (int32_t k = tab[pt.off + t]; k < tab[pt.off + t + 1]; ++k) {
+            const size_t w = pt.src + 4 * (size_t)k;
+            const int32_t q = tab[w];
+            CHECK(q >= 0 && (size_t)q < lin.size() && tab[w + 1] == 0 && lin_at(w + 2) == lin[(size_t)q]);
+            if (q >= 0 && (size_t)q < lin.size()) ++src_seen[(size_t)q];
+        }
+        for (int32_t k = tab[pt.off + t1 + t]; k < tab[pt.off + t1 + t + 1]; ++k) {
+            CHECK(std::find(bl.zero_lin.begin(), bl.zero_lin.end(), lin_at(pt.zero + 2 * (size_t)k)) != bl.zero_lin.end());
+            ++zeros;
+        }
+        for (int32_t k = tab[pt.off + 2 * t1 + t]; k < tab[pt.off + 2 * t1 + t + 1]; ++k) {
+            const size_t w = pt.out + 4 * (size_t)k;
+            const int32_t q = tab[w];
+            CHECK(q >= 0 && (size_t)q < lin.size());
+            if (q < 0 || (size_t)q >= lin.size()) continue;
+            ++out_seen[(size_t)q];
+            CHECK(tab[w + 1] == bl.out_src[(size_t)q]);
+            CHECK(lin_at(w + 2) == (bl.out_air[(size_t)q] ? -1 : bl.out_lin[(size_t)q]));
+        }
+    }
+    CHECK(zeros == bl.zero_lin.size());
+    for (size_t q = 0; q < lin.size(); ++q) CHECK(src_seen[q] == (lin[q] >= 0 ? 1 : 0) && out_seen[q] == 1);
 }
 
 int main(int argc, char** argv) {

@@ -449,3 +449,29 @@ def test_empty_lists_and_degenerate_counts():
             assert rel_l2(got, o.sensors()) <= TOL, name
         assert rel_l2(p_got, o.field("p")) <= TOL, name
         print(f"{name}: records {got.shape}, field rel-L2 {rel_l2(p_got, o.field('p')):.2e}", flush=True)
+
+
+@pytest.mark.parametrize("shape", ["tile:8", "tile:32", "stream:256:16", "stream:64:8"])
+def test_fused_boundary_matches_boundary_launch(monkeypatch, shape):
+    """The 2D pressure kernels do the step's sources, pressure release and sensors for the points
+    their tiles own; the records equal those of the same kernels followed by the boundary pass
+    launched on its own, bit for bit: points in the ghost border, on tile edges, repeated and on
+    each other, three shots, a recording interval of 3, sources that stop before the run ends,
+    and steps replayed from a graph."""
+    from tests.test_gpu_group import _edge_point_media
+
+    inp = _edge_point_media()[0]
+    rng = np.random.default_rng(72)
+    shots = (inp.icmat[None] * (1 + rng.random((3, 1, 1)))).astype(np.float32)
+    monkeypatch.setenv("FULLWAVE_HIP_KERNELS", shape)
+    out = {}
+    for fused in ("1", "0"):
+        monkeypatch.setenv("FULLWAVE_HIP_FUSED_BOUNDARY", fused)
+        with engine.HipSimulation(inp, shots=shots) as sim:
+            sim.run(25)
+            sim.set_graph_steps(10)
+            sim.run(inp.nt - 25)
+            out[fused] = sim.sensors().copy()
+    print(f"{shape}: records {out['1'].shape}, max |rec| {np.abs(out['1']).max():.3e}", flush=True)
+    assert np.abs(out["1"]).max() > 0
+    np.testing.assert_array_equal(out["1"], out["0"])
