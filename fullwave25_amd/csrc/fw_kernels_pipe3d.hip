@@ -139,10 +139,14 @@ __device__ inline uint32_t load_code(const DevSim& s, uint32_t c, bool active) {
     return CODED && active ? gld16(s.rcode, c) : 0u;
 }
 
-template <bool UNIFORM>
+// The weights of axis `axis` for coefficient set dc. SAME (equal spacings: every axis has the
+// same weights, DevSim::coef_axes_same): axis 0's, so a cell reads its 9 weights once instead of
+// three times (27 loads a cell on grids whose sets are read from the global table).
+template <bool UNIFORM, bool SAME>
 __device__ inline const float* coefp(const float* coefs, int32_t dc, int axis) {
-    if (UNIFORM) return coefs + axis * kNCoef;
-    return coefs + ((int64_t)dc * 3 + axis) * kNCoef;
+    const int a = SAME ? 0 : axis;
+    if (UNIFORM) return coefs + a * kNCoef;
+    return coefs + ((int64_t)dc * 3 + a) * kNCoef;
 }
 
 __device__ inline const float* coef_table(const DevSim& s, float* lds_coef, bool uniform) {
@@ -223,7 +227,7 @@ __device__ inline void load_vel(VelOps<ISO>& o, const DevSim& s, uint32_t c, boo
 
 // AHEAD: per-cell operands loaded one iteration ahead (more registers); otherwise at the top
 // of their own iteration, before the barrier
-template <int TY, bool UNIFORM, bool ISO, bool AHEAD, bool CODED>
+template <int TY, bool UNIFORM, bool ISO, bool AHEAD, bool CODED, bool SAME = false>
 __global__ __launch_bounds__(PZ3 * TY) void k_velocity_pipe3d(DevSim batch, int chunk) {
     using TP = PTile<TY, kHalo, kHalo>;
     unsigned zl;
@@ -282,7 +286,7 @@ __global__ __launch_bounds__(PZ3 * TY) void k_velocity_pipe3d(DevSim batch, int 
             auto A = [&](const float* pl, int dy, int dz) { return pl[TP::at(ty, tz, dy, dz)]; };
             float D[3];
             {
-                const float* w = coefp<UNIFORM>(coefs, cur.dc, 0);
+                const float* w = coefp<UNIFORM, SAME>(coefs, cur.dc, 0);
                 float acc = 0.f;
 #pragma unroll
                 for (int m = 1; m <= 8; ++m) acc += w[m - 1] * (q[7 + m] - q[8 - m]);
@@ -292,7 +296,7 @@ __global__ __launch_bounds__(PZ3 * TY) void k_velocity_pipe3d(DevSim batch, int 
                 D[0] = acc + w[8] * tr;
             }
             {
-                const float* w = coefp<UNIFORM>(coefs, cur.dc, 1);
+                const float* w = coefp<UNIFORM, SAME>(coefs, cur.dc, 1);
                 float acc = 0.f;
 #pragma unroll
                 for (int m = 1; m <= 8; ++m) acc += w[m - 1] * (A(p0, m, 0) - A(p0, -(m - 1), 0));
@@ -302,7 +306,7 @@ __global__ __launch_bounds__(PZ3 * TY) void k_velocity_pipe3d(DevSim batch, int 
                 D[1] = acc + w[8] * tr;
             }
             {
-                const float* w = coefp<UNIFORM>(coefs, cur.dc, 2);
+                const float* w = coefp<UNIFORM, SAME>(coefs, cur.dc, 2);
                 float acc = 0.f;
 #pragma unroll
                 for (int m = 1; m <= 8; ++m) acc += w[m - 1] * (A(p0, 0, m) - A(p0, 0, -(m - 1)));
@@ -376,7 +380,7 @@ __device__ inline void load_pre(PreOps<ISO>& o, const DevSim& s, uint32_t c, boo
     o.p = gld((const float*)s.p, c);
 }
 
-template <int TY, bool UNIFORM, bool ISO, bool AHEAD, bool CODED>
+template <int TY, bool UNIFORM, bool ISO, bool AHEAD, bool CODED, bool SAME = false>
 __global__ __launch_bounds__(PZ3 * TY) void k_pressure_pipe3d(DevSim batch, int chunk) {
     using TX_ = PTile<TY, 1, 1>;
     using TY_ = PTile<TY, kHalo, 1>;
@@ -461,7 +465,7 @@ __global__ __launch_bounds__(PZ3 * TY) void k_pressure_pipe3d(DevSim batch, int 
             auto Z = [&](const float* pl, int dy, int dz) { return pl[TZ_::at(ty, tz, dy, dz)]; };
             float D[3];
             {
-                const float* w = coefp<UNIFORM>(coefs, cur.dc, 0);
+                const float* w = coefp<UNIFORM, SAME>(coefs, cur.dc, 0);
                 float acc = 0.f;
 #pragma unroll
                 for (int m = 1; m <= 8; ++m) acc += w[m - 1] * (q[7 + m] - q[8 - m]);
@@ -471,7 +475,7 @@ __global__ __launch_bounds__(PZ3 * TY) void k_pressure_pipe3d(DevSim batch, int 
                 D[0] = acc + w[8] * tr;
             }
             {
-                const float* w = coefp<UNIFORM>(coefs, cur.dc, 1);
+                const float* w = coefp<UNIFORM, SAME>(coefs, cur.dc, 1);
                 float acc = 0.f;
 #pragma unroll
                 for (int m = 1; m <= 8; ++m) acc += w[m - 1] * (Y(y0, m - 1, 0) - Y(y0, -m, 0));
@@ -481,7 +485,7 @@ __global__ __launch_bounds__(PZ3 * TY) void k_pressure_pipe3d(DevSim batch, int 
                 D[1] = acc + w[8] * tr;
             }
             {
-                const float* w = coefp<UNIFORM>(coefs, cur.dc, 2);
+                const float* w = coefp<UNIFORM, SAME>(coefs, cur.dc, 2);
                 float acc = 0.f;
 #pragma unroll
                 for (int m = 1; m <= 8; ++m) acc += w[m - 1] * (Z(z0, 0, m - 1) - Z(z0, 0, -m));
@@ -515,15 +519,24 @@ void launch_pipe(bool velocity, const DevSim& s, int chunk, hipStream_t st) {
                  (unsigned)((s.nx - 2 * kHalo + chunk - 1) / chunk * s.nshots));
     const dim3 b(PZ3 * TY);
     const bool coded = s.isotropic && s.rcode;
+    const bool same = s.coef_axes_same != 0;
     if (velocity) {
-        if (u && coded) hipLaunchKernelGGL((k_velocity_pipe3d<TY, true, true, AHEAD, true>), g, b, lds, st, s, chunk);
+        if (u && coded && same)
+            hipLaunchKernelGGL((k_velocity_pipe3d<TY, true, true, AHEAD, true, true>), g, b, lds, st, s, chunk);
+        else if (coded && same)
+            hipLaunchKernelGGL((k_velocity_pipe3d<TY, false, true, AHEAD, true, true>), g, b, lds, st, s, chunk);
+        else if (u && coded) hipLaunchKernelGGL((k_velocity_pipe3d<TY, true, true, AHEAD, true>), g, b, lds, st, s, chunk);
         else if (u && s.isotropic) hipLaunchKernelGGL((k_velocity_pipe3d<TY, true, true, AHEAD, false>), g, b, lds, st, s, chunk);
         else if (u) hipLaunchKernelGGL((k_velocity_pipe3d<TY, true, false, AHEAD, false>), g, b, lds, st, s, chunk);
         else if (coded) hipLaunchKernelGGL((k_velocity_pipe3d<TY, false, true, AHEAD, true>), g, b, lds, st, s, chunk);
         else if (s.isotropic) hipLaunchKernelGGL((k_velocity_pipe3d<TY, false, true, AHEAD, false>), g, b, lds, st, s, chunk);
         else hipLaunchKernelGGL((k_velocity_pipe3d<TY, false, false, AHEAD, false>), g, b, lds, st, s, chunk);
     } else {
-        if (u && coded) hipLaunchKernelGGL((k_pressure_pipe3d<TY, true, true, AHEAD, true>), g, b, lds, st, s, chunk);
+        if (u && coded && same)
+            hipLaunchKernelGGL((k_pressure_pipe3d<TY, true, true, AHEAD, true, true>), g, b, lds, st, s, chunk);
+        else if (coded && same)
+            hipLaunchKernelGGL((k_pressure_pipe3d<TY, false, true, AHEAD, true, true>), g, b, lds, st, s, chunk);
+        else if (u && coded) hipLaunchKernelGGL((k_pressure_pipe3d<TY, true, true, AHEAD, true>), g, b, lds, st, s, chunk);
         else if (u && s.isotropic) hipLaunchKernelGGL((k_pressure_pipe3d<TY, true, true, AHEAD, false>), g, b, lds, st, s, chunk);
         else if (u) hipLaunchKernelGGL((k_pressure_pipe3d<TY, true, false, AHEAD, false>), g, b, lds, st, s, chunk);
         else if (coded) hipLaunchKernelGGL((k_pressure_pipe3d<TY, false, true, AHEAD, true>), g, b, lds, st, s, chunk);
