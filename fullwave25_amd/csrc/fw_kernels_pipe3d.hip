@@ -26,6 +26,9 @@ namespace {
 constexpr int PZ3 = 64;  // lanes along z
 constexpr int kRingP = 4;
 constexpr int kLdsSetsP = 128;
+// grids whose axes share their weights (SAME kernels): one set per sound speed, staged in LDS
+// up to this many sound speeds (9 KB)
+constexpr int kLdsSetsSame = 256;
 
 // global memory at 32-bit element indices (gld32/gst32 in fw_internal.h)
 template <class T>
@@ -133,6 +136,21 @@ struct HaloStager {
     }
 };
 
+// SAME kernels (every axis has the same weights): each workgroup stages one set per sound speed
+// in LDS, [set][9], and a cell reads its weights there (LDS reads instead of three vector-memory
+// loads a cell from the global table; neighbouring cells mostly share a set, so the reads are
+// mostly broadcasts).
+template <bool UNIFORM>
+__device__ inline void stage_same_sets(const DevSim& s, float* lds) {
+    const int nsets = UNIFORM ? 1 : s.ndmap;
+    for (int t = threadIdx.x; t < nsets * kNCoef; t += blockDim.x)
+        lds[t] = s.coef[(t / kNCoef) * 3 * kNCoef + t % kNCoef];
+}
+__device__ inline void same_weights(float (&w)[kNCoef], const float* lds, int32_t dc) {
+#pragma unroll
+    for (int m = 0; m < kNCoef; ++m) w[m] = lds[dc * kNCoef + m];
+}
+
 // the code of cell c (coded grids, active cells), else 0
 template <bool CODED>
 __device__ inline uint32_t load_code(const DevSim& s, uint32_t c, bool active) {
@@ -161,7 +179,9 @@ template <bool ISO>
 struct VelOps {
     static constexpr int NA = ISO ? 1 : 3;
     float qnew;            // p(i + 8): the queue's next element
-    float rho_x, rho_y, rho_z;  // rho at c + e_x, c + e_y, c + e_z
+    float rho_x;           // rho at c + e_x: the next plane's rho_i
+    float rho_y, rho_z;    // rho at c + e_y, c + e_z (from the LDS row, after the barrier)
+    float ryn, rzn;        // the next plane's rho beyond the tile (RhoEdge), edge lanes only
     float psi[3][2];
     float v[3];
     float a[NA][2], b[NA][2];
@@ -195,8 +215,6 @@ __device__ inline void load_vel(VelOps<ISO>& o, const DevSim& s, uint32_t c, boo
     o.qnew = in_grid ? gld(s.p, c + 8 * sx) : 0.f;
     o.rho_x = in_grid ? gld(s.rho, c + sx) : 0.f;
     if (!active) return;
-    o.rho_y = gld(s.rho, c + (uint32_t)s.sy);
-    o.rho_z = gld(s.rho, c + 1);
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         o.v[a] = gld((const float*)s.v[a], c);
@@ -225,6 +243,22 @@ __device__ inline void load_vel(VelOps<ISO>& o, const DevSim& s, uint32_t c, boo
     }
 }
 
+// The rho neighbours of the velocity update (c + e_y, c + e_z) come from a plane of rho in LDS
+// that every lane fills with its own rho_i; the row below the tile and the column right of it
+// are loaded by a few lanes only: the lanes of the tile's last row (the row below, one load in
+// one wavefront) and lanes 0 .. TY - 1 of the first row (one cell of the right column each).
+// Two loads a workgroup and plane instead of two a cell. In-plane offsets:
+struct RhoEdge {
+    uint32_t row, colz;       // (j + 1, k) for row lanes; (j0 + tz, k0 + 64) for column lanes
+    bool row_lane, col_lane;
+};
+// the next plane's edge values of a lane (plane base `base`)
+template <class Ops>
+__device__ inline void load_rho_edge(Ops& o, const DevSim& s, uint32_t base, const RhoEdge& e) {
+    o.ryn = e.row_lane ? gld(s.rho, base + e.row) : 0.f;
+    o.rzn = e.col_lane ? gld(s.rho, base + e.colz) : 0.f;
+}
+
 // AHEAD: per-cell operands loaded one iteration ahead (more registers); otherwise at the top
 // of their own iteration, before the barrier
 template <int TY, bool UNIFORM, bool ISO, bool AHEAD, bool CODED, bool SAME = false>
@@ -234,6 +268,7 @@ __global__ __launch_bounds__(PZ3 * TY) void k_velocity_pipe3d(DevSim batch, int 
     const DevSim s = shot_view(batch, shot_of_block(batch, zl));
     extern __shared__ float lds_coef[];
     __shared__ TP P;
+    __shared__ float RH[2][TY + 1][PZ3 + 1];  // rho of plane i (slot i % 2), with the row below and column right
     const int tz = threadIdx.x % PZ3, ty = threadIdx.x / PZ3;
     const int64_t k0 = (int64_t)blockIdx.x * PZ3, j0 = (int64_t)blockIdx.y * TY;
     const int64_t k = k0 + tz, j = j0 + ty;
@@ -242,7 +277,8 @@ __global__ __launch_bounds__(PZ3 * TY) void k_velocity_pipe3d(DevSim batch, int 
     const bool in_grid = j < s.ny && k < s.nz;
     const bool active = j >= kHalo && j < s.ny - kHalo && k >= kHalo && k < s.nz - kHalo;
     const uint32_t sx = (uint32_t)s.sx, col = (uint32_t)(j * s.sy + k);
-    const float* coefs = coef_table(s, lds_coef, UNIFORM);
+    const float* coefs = SAME ? nullptr : coef_table(s, lds_coef, UNIFORM);
+    if (SAME) stage_same_sets<UNIFORM>(s, lds_coef);
 
     HaloStager<TY, kHalo, kHalo> st;
     st.init(s, j0, k0);
@@ -255,22 +291,43 @@ __global__ __launch_bounds__(PZ3 * TY) void k_velocity_pipe3d(DevSim batch, int 
     st.store(P, i0, q[8], ty, tz);
     st.load(s.p, i0 + 1, s);
     float rho_i = in_grid ? gld(s.rho, (uint32_t)i0 * sx + col) : 0.f;
+    RhoEdge edge;
+    edge.row_lane = ty == TY - 1 && active;
+    edge.row = col + (uint32_t)s.sy;
+    edge.col_lane = ty == 0 && tz < TY && j0 + tz < s.ny && k0 + PZ3 < s.nz;
+    edge.colz = (uint32_t)((j0 + tz) * s.sy + k0 + PZ3);
     VelOps<ISO> cur, nxt;
+    load_rho_edge(cur, s, (uint32_t)i0 * sx, edge);  // plane i0's edge values
+    float ryn_i = cur.ryn, rzn_i = cur.rzn;
     // ca: the code of the plane whose operands this iteration loads (i, or i + 1 with AHEAD)
     constexpr int kAhead = AHEAD ? 1 : 0;
-    if (AHEAD)
+    if (AHEAD) {
         load_vel<UNIFORM, ISO, CODED>(cur, s, (uint32_t)i0 * sx + col, in_grid, active,
                                       load_code<CODED>(s, (uint32_t)i0 * sx + col, active));
+        load_rho_edge(cur, s, (uint32_t)(i0 + 1) * sx, edge);
+    }
     uint32_t ca = i0 + kAhead < i1 ? load_code<CODED>(s, (uint32_t)(i0 + kAhead) * sx + col, active) : 0u;
 
     for (int64_t i = i0; i < i1; ++i) {
         st.store(P, i + 1, q[9], ty, tz);
+        {  // rho of plane i: own cell, the row below the tile, the column right of it
+            float(*rh)[PZ3 + 1] = RH[i & 1];
+            rh[ty][tz] = rho_i;
+            if (edge.row_lane) rh[TY][tz] = ryn_i;
+            if (edge.col_lane) rh[tz][PZ3] = rzn_i;
+        }
         const bool more = i + 1 < i1;
         const uint32_t cb = i + kAhead + 1 < i1 ? load_code<CODED>(s, (uint32_t)(i + kAhead + 1) * sx + col, active) : 0u;
-        if (!AHEAD) load_vel<UNIFORM, ISO, CODED>(cur, s, (uint32_t)i * sx + col, in_grid, active, ca);
+        if (!AHEAD) {
+            load_vel<UNIFORM, ISO, CODED>(cur, s, (uint32_t)i * sx + col, in_grid, active, ca);
+            load_rho_edge(cur, s, (uint32_t)(i + 1) * sx, edge);  // i + 1 <= nx - 8
+        }
         if (more) {
             st.load(s.p, i + 2, s);
-            if (AHEAD) load_vel<UNIFORM, ISO, CODED>(nxt, s, (uint32_t)(i + 1) * sx + col, in_grid, active, ca);
+            if (AHEAD) {
+                load_vel<UNIFORM, ISO, CODED>(nxt, s, (uint32_t)(i + 1) * sx + col, in_grid, active, ca);
+                load_rho_edge(nxt, s, (uint32_t)(i + 2) * sx, edge);  // i + 2 <= nx - 8
+            }
         }
         ca = cb;
 #pragma unroll
@@ -280,13 +337,17 @@ __global__ __launch_bounds__(PZ3 * TY) void k_velocity_pipe3d(DevSim batch, int 
         if (active) {
             const uint32_t c = (uint32_t)i * sx + col;
             if (CODED) fix_escape<true, UNIFORM>(cur, s, c);
+            float ws[kNCoef];
+            if (SAME) same_weights(ws, lds_coef, UNIFORM ? 0 : cur.dc);
+            cur.rho_y = RH[i & 1][ty + 1][tz];
+            cur.rho_z = RH[i & 1][ty][tz + 1];
             const float* pm = P.slot(i - 1);
             const float* p0 = P.slot(i);
             const float* pp = P.slot(i + 1);
             auto A = [&](const float* pl, int dy, int dz) { return pl[TP::at(ty, tz, dy, dz)]; };
             float D[3];
             {
-                const float* w = coefp<UNIFORM, SAME>(coefs, cur.dc, 0);
+                const float* w = SAME ? ws : coefp<UNIFORM, SAME>(coefs, cur.dc, 0);
                 float acc = 0.f;
 #pragma unroll
                 for (int m = 1; m <= 8; ++m) acc += w[m - 1] * (q[7 + m] - q[8 - m]);
@@ -296,7 +357,7 @@ __global__ __launch_bounds__(PZ3 * TY) void k_velocity_pipe3d(DevSim batch, int 
                 D[0] = acc + w[8] * tr;
             }
             {
-                const float* w = coefp<UNIFORM, SAME>(coefs, cur.dc, 1);
+                const float* w = SAME ? ws : coefp<UNIFORM, SAME>(coefs, cur.dc, 1);
                 float acc = 0.f;
 #pragma unroll
                 for (int m = 1; m <= 8; ++m) acc += w[m - 1] * (A(p0, m, 0) - A(p0, -(m - 1), 0));
@@ -306,7 +367,7 @@ __global__ __launch_bounds__(PZ3 * TY) void k_velocity_pipe3d(DevSim batch, int 
                 D[1] = acc + w[8] * tr;
             }
             {
-                const float* w = coefp<UNIFORM, SAME>(coefs, cur.dc, 2);
+                const float* w = SAME ? ws : coefp<UNIFORM, SAME>(coefs, cur.dc, 2);
                 float acc = 0.f;
 #pragma unroll
                 for (int m = 1; m <= 8; ++m) acc += w[m - 1] * (A(p0, 0, m) - A(p0, 0, -(m - 1)));
@@ -328,6 +389,8 @@ __global__ __launch_bounds__(PZ3 * TY) void k_velocity_pipe3d(DevSim batch, int 
             }
         }
         rho_i = cur.rho_x;
+        ryn_i = cur.ryn;
+        rzn_i = cur.rzn;
         if (AHEAD) cur = nxt;
     }
 }
@@ -399,7 +462,8 @@ __global__ __launch_bounds__(PZ3 * TY) void k_pressure_pipe3d(DevSim batch, int 
     const bool in_grid = j < s.ny && k < s.nz;
     const bool active = j >= kHalo && j < s.ny - kHalo && k >= kHalo && k < s.nz - kHalo;
     const uint32_t sx = (uint32_t)s.sx, col = (uint32_t)(j * s.sy + k);
-    const float* coefs = coef_table(s, lds_coef, UNIFORM);
+    const float* coefs = SAME ? nullptr : coef_table(s, lds_coef, UNIFORM);
+    if (SAME) stage_same_sets<UNIFORM>(s, lds_coef);
 
     HaloStager<TY, 1, 1> sx_;
     Stager<TY, kHalo, 1> sy_;
@@ -452,6 +516,8 @@ __global__ __launch_bounds__(PZ3 * TY) void k_pressure_pipe3d(DevSim batch, int 
         if (active) {
             const uint32_t c = (uint32_t)i * sx + col;
             if (CODED) fix_escape<false, UNIFORM>(cur, s, c);
+            float ws[kNCoef];
+            if (SAME) same_weights(ws, lds_coef, UNIFORM ? 0 : cur.dc);
             const float* xm = VX.slot(i - 1);
             const float* x0 = VX.slot(i);
             const float* ym = VY.slot(i - 1);
@@ -465,7 +531,7 @@ __global__ __launch_bounds__(PZ3 * TY) void k_pressure_pipe3d(DevSim batch, int 
             auto Z = [&](const float* pl, int dy, int dz) { return pl[TZ_::at(ty, tz, dy, dz)]; };
             float D[3];
             {
-                const float* w = coefp<UNIFORM, SAME>(coefs, cur.dc, 0);
+                const float* w = SAME ? ws : coefp<UNIFORM, SAME>(coefs, cur.dc, 0);
                 float acc = 0.f;
 #pragma unroll
                 for (int m = 1; m <= 8; ++m) acc += w[m - 1] * (q[7 + m] - q[8 - m]);
@@ -475,7 +541,7 @@ __global__ __launch_bounds__(PZ3 * TY) void k_pressure_pipe3d(DevSim batch, int 
                 D[0] = acc + w[8] * tr;
             }
             {
-                const float* w = coefp<UNIFORM, SAME>(coefs, cur.dc, 1);
+                const float* w = SAME ? ws : coefp<UNIFORM, SAME>(coefs, cur.dc, 1);
                 float acc = 0.f;
 #pragma unroll
                 for (int m = 1; m <= 8; ++m) acc += w[m - 1] * (Y(y0, m - 1, 0) - Y(y0, -m, 0));
@@ -485,7 +551,7 @@ __global__ __launch_bounds__(PZ3 * TY) void k_pressure_pipe3d(DevSim batch, int 
                 D[1] = acc + w[8] * tr;
             }
             {
-                const float* w = coefp<UNIFORM, SAME>(coefs, cur.dc, 2);
+                const float* w = SAME ? ws : coefp<UNIFORM, SAME>(coefs, cur.dc, 2);
                 float acc = 0.f;
 #pragma unroll
                 for (int m = 1; m <= 8; ++m) acc += w[m - 1] * (Z(z0, 0, m - 1) - Z(z0, 0, -m));
@@ -514,12 +580,12 @@ template <int TY, bool AHEAD>
 void launch_pipe(bool velocity, const DevSim& s, int chunk, hipStream_t st) {
     const bool u = s.ndmap == 1;
     const int nsets = u ? 1 : s.ndmap;
-    const int lds = nsets > kLdsSetsP ? 0 : nsets * 3 * kNCoef * 4;
+    const bool same = s.coef_axes_same != 0 && nsets <= kLdsSetsSame;
+    const int lds = same ? nsets * kNCoef * 4 : (nsets > kLdsSetsP ? 0 : nsets * 3 * kNCoef * 4);
     const dim3 g((unsigned)((s.nz + PZ3 - 1) / PZ3), (unsigned)((s.ny + TY - 1) / TY),
                  (unsigned)((s.nx - 2 * kHalo + chunk - 1) / chunk * s.nshots));
     const dim3 b(PZ3 * TY);
     const bool coded = s.isotropic && s.rcode;
-    const bool same = s.coef_axes_same != 0;
     if (velocity) {
         if (u && coded && same)
             hipLaunchKernelGGL((k_velocity_pipe3d<TY, true, true, AHEAD, true, true>), g, b, lds, st, s, chunk);
