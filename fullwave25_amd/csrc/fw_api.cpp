@@ -621,6 +621,7 @@ int encode_relaxation(fw_sim* s) {
             rep = (int32_t*)rep_b.p;
             FW_HIP(hipMemsetAsync(keys, 0, nslots * 8, s->stream));
             FW_HIP(hipMemsetAsync(count, 0, nslots * 4, s->stream));
+            FW_HIP(hipMemsetAsync(rep, 0x7f, nslots * 4, s->stream));  // above every cell index
             FW_HIP(hipMemsetAsync(status, 0, 16, s->stream));
             FW_HIP(fw::launch_code_insert(m, s->ncell, keys, (uint32_t)nslots, count, rep, status, s->stream));
             FW_HIP(copy_sync(s, st, status, 16, hipMemcpyDeviceToHost));
@@ -647,6 +648,10 @@ int encode_relaxation(fw_sim* s) {
         int64_t covered = 0;
         for (uint32_t q : occupied) covered += hc[q];
         if (2 * covered < s->ncell) return FW_OK;
+        // codes in the order of each tuple's first cell: cells close together that hold
+        // different tuples gather rows close together (3D pressure kernel on the 904 x 904 x 776
+        // inclusion grid: 8.68 -> 8.61 ms a step; 8.34 ms if every cell read the same row)
+        std::sort(occupied.begin(), occupied.end(), [&](uint32_t x, uint32_t y) { return hr[x] < hr[y]; });
         std::vector<uint16_t> slot_code(nslots, (uint16_t)fw::kCodeEscape);
         std::vector<int32_t> rep_of_code(occupied.size());
         for (size_t i = 0; i < occupied.size(); ++i) {

@@ -12,11 +12,13 @@
 //
 // Encoding runs once per medium, on the device:
 //   1. insert: every cell hashes its tuple (the raw bits of the eleven words) into an
-//      open-addressing table of 64-bit keys (linear probing, bounded probe count) and counts
-//      it; the lane whose compare-and-swap claims an empty slot records its cell as the slot's
-//      representative (for grids up to 2^24 cells a table of 2^16 slots first, 2^22 if that one
-//      overflows);
-//   2. the host numbers the most frequent occupied slots 0 .. n-1 (n <= 65535);
+//      open-addressing table of 64-bit keys (linear probing, bounded probe count), counts it
+//      and keeps the lowest cell holding it as the slot's representative (an atomic minimum;
+//      for grids up to 2^24 cells a table of 2^16 slots first, 2^22 if that one overflows);
+//   2. the host picks the most frequent occupied slots (n <= 65535) and numbers them in the
+//      order of their representative cells: tuples that first occur close together get
+//      neighbouring codes, so the cells of a row that hold different tuples (the ramp of the
+//      absorbing layer along the fastest axis) read rows that share cache lines of the tables;
 //   3. assign: every cell finds its key again and stores the slot's code (or the escape);
 //   4. table: each code copies its representative cell's eleven words into the tables;
 //   5. verify: every coded cell compares its eleven words, bit for bit, with its code's row.
@@ -69,10 +71,7 @@ __global__ __launch_bounds__(256) void k_code_insert(CodeMaps m, int64_t n, unsi
             unsigned long long k = __hip_atomic_load(&keys[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (k == 0) {
                 k = atomicCAS(&keys[slot], 0ull, h);
-                if (k == 0) {
-                    rep[slot] = (int32_t)c;
-                    k = h;
-                }
+                if (k == 0) k = h;
             }
             if (k == h) {
                 placed = true;
@@ -80,8 +79,12 @@ __global__ __launch_bounds__(256) void k_code_insert(CodeMaps m, int64_t n, unsi
             }
             slot = (slot + 1) & mask;
         }
-        if (placed) atomicAdd(&count[slot], same ? (uint32_t)__popcll(group) : 1u);
-        else status[0] = 1;  // every writer stores the same value
+        if (placed) {
+            atomicAdd(&count[slot], same ? (uint32_t)__popcll(group) : 1u);
+            atomicMin(&rep[slot], (int32_t)c);  // the leader is the group's lowest cell
+        } else {
+            status[0] = 1;  // every writer stores the same value
+        }
     }
 }
 
