@@ -510,8 +510,8 @@ int choose_kernels_unpinned(fw_sim* s) {
     fw::LaunchCfg pc;
     pc.variant = 0;
     cands.push_back(pc);
-    for (int ch : {8, 16, 32, 64}) {  // 64 planes per workgroup: 3D only
-        if (s->ndim == 2 && ch == 64) continue;
+    for (int ch : {8, 16, 32, 48, 64, 96, 128}) {  // beyond 32 planes per workgroup: 3D only
+        if (s->ndim == 2 && ch > 32) continue;
         if (s->ndim == 2) {
             for (int w : {128, 256}) {
                 fw::LaunchCfg c;

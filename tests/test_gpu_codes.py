@@ -40,8 +40,9 @@ def test_codes_in_use_only_for_isotropic_relaxation():
             assert sim.relaxation_codes() == 0
 
 
-def test_codes_count_matches_distinct_tuples():
-    inp = workloads.random_medium_3d(30, 26, 41, 10, seed=44).inputs
+def _tuple_counts(inp):
+    """How many cells hold each distinct 11-word tuple the encoder sees (the ten isotropic
+    relaxation maps, with 1/kappa as the device holds it, and dcmap), host cells only."""
     names = ["kappau", "apmlu1", "apmlu2", "bpmlu1", "bpmlu2", "kappax", "apmlx1", "apmlx2", "bpmlx1", "bpmlx2"]
     cols = []
     for k in names:
@@ -50,8 +51,14 @@ def test_codes_count_matches_distinct_tuples():
             m = np.where(m != 0, np.float32(1) / np.where(m != 0, m, np.float32(1)), np.float32(0)).astype(np.float32)
         cols.append(m.view(np.uint32))
     cols.append(np.ascontiguousarray(inp.dcmap, np.int32).reshape(-1).view(np.uint32))
-    st = np.stack(cols, 1)
-    want = np.unique(st, axis=0).shape[0]
+    st = np.ascontiguousarray(np.stack(cols, 1))
+    rows = st.view(np.dtype((np.void, st.shape[1] * 4))).reshape(-1)
+    return np.unique(rows, return_counts=True)[1]
+
+
+def test_codes_count_matches_distinct_tuples():
+    inp = workloads.random_medium_3d(30, 26, 41, 10, seed=44).inputs
+    want = len(_tuple_counts(inp))
     with engine.HipSimulation(inp) as sim:
         got = sim.relaxation_codes()
     # 3D rows are padded on the device to 32-float multiples; padding cells add the all-zero tuple
@@ -114,6 +121,25 @@ def _oracle_traces(inp):
     o = oracle.OracleSimulation(inp, 64)
     o.run(inp.nt)
     return o.sensors()
+
+
+def test_code_coverage_matches_exact_tuple_counts():
+    """The encoder counts a wavefront whose lanes hold one tuple in registers across its
+    iterations and the others with atomics; on a 6.4 M-cell grid (more cells than the encoder
+    has lanes) with more distinct tuples than codes, the cells its 65535 codes cover are exactly
+    those of the 65535 most frequent tuples counted on the host."""
+    base = workloads.random_medium_2d(2600, 2400, 10, seed=50).inputs
+    region = np.zeros(base.shape, bool)
+    region[:40] = True
+    inp = _perturbed(base, 50, region)
+    cnt = _tuple_counts(inp)
+    assert len(cnt) > 65535
+    want = np.sort(cnt)[::-1][:65535].sum() / cnt.sum()
+    with engine.HipSimulation(inp) as sim:
+        n, frac = sim.relaxation_coding()
+    print(f"{len(cnt)} tuples over {cnt.sum()} cells: coverage {frac:.9f}, host counts {want:.9f}", flush=True)
+    assert n == 65535
+    assert abs(frac - want) < 1e-12
 
 
 @pytest.mark.parametrize("shape", ["stream:256:8", "tile:8", "tile:16/stream:64:4"])
