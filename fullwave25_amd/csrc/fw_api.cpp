@@ -327,25 +327,6 @@ void dev_free(fw_sim* s, void* p) {
     }
 }
 
-// A process's first large device allocations run the stencil kernels slower than later ones,
-// and memory the pool keeps from them stays slow. Before the first large handle of a process on
-// a device, an untouched allocation of the handle's size is made and released at once (hipFree
-// waits for the device: once per device and process). Measured on the first handle of a
-// process, 904 x 904 x 776 grid: pressure kernel 9.69 -> 8.48 ms a step, velocity 10.43 ->
-// 10.31 ms; the handle's set-up time does not change (1.4 s).
-constexpr size_t kPrimeBytes = 4ull << 30;
-
-void prime_device_memory(fw_sim* s, size_t bytes) {
-    if (s->fixed_base || bytes < kPrimeBytes) return;
-    static std::mutex mu;
-    static std::set<int> primed;
-    std::lock_guard<std::mutex> lock(mu);
-    if (!primed.insert(s->device).second) return;
-    void* p = nullptr;
-    if (hipMalloc(&p, bytes) == hipSuccess) (void)hipFree(p);
-    (void)hipGetLastError();  // no memory for it: nothing to gain, nothing lost
-}
-
 int alloc(fw_sim* s, const std::string& name, size_t bytes, void** out) {
     auto it = s->dev.find(name);
     if (it != s->dev.end()) {
@@ -1219,7 +1200,6 @@ int create_impl(const fw_desc* desc, int device, void* fixed, size_t fixed_bytes
     const bool relax = d.attenuation_model == 0;  // the exponential model has no memory variables
     const int64_t field = s->ncell * nshots;  // one wave field of every shot
     s->nstate = (int64_t)(1 + nd + (relax ? 4 * nd : 0)) * field;
-    prime_device_memory(s, (size_t)s->nstate * 4 + (size_t)s->ncell * 4 * required_maps(d).size());
     if (dev_malloc(s, (void**)&s->state, (size_t)s->nstate * 4) != hipSuccess)
         return cleanup(fail(FW_ERR_NOMEM, "hipMalloc of %lld state floats failed", (long long)s->nstate));
     if (fill_zero_sync(s, s->state, (size_t)s->nstate * 4) != hipSuccess)
